@@ -1,0 +1,1477 @@
+// ncx_oracle.hpp -- CPU restatement of ncollide3d 0.33's collision hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This code is the parity checker for the HIP
+// library in ncollide_amd/.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load it; the product never links it.
+//
+// Every function restates one reference function operation by operation
+// (citations are file:line under the reference crate's src/).  It is
+// templated on the scalar type so the same code gives the f32 result the
+// GPU must match and the f64 "ncollide3d<f64>" result used for error reports.
+// Compile with -ffp-contract=off: the reference (Rust) never contracts a*b+c.
+//
+// Arithmetic conventions fixed here (nalgebra 0.30 is not vendored, so these
+// are the restatement's choices; both the oracle and the GPU follow them):
+//   dot(a,b)      = (a.x*b.x + a.y*b.y) + a.z*b.z
+//   cross(a,b)    = (a.y*b.z - a.z*b.y, a.z*b.x - a.x*b.z, a.x*b.y - a.y*b.x)
+//   norm(v)       = sqrt(dot(v,v));  normalize(v) = v / norm(v) (component division)
+//   Unit::try_new(v, eps): n = norm(v); None if n <= eps else v / n
+//   q * v         = t = 2*(q.xyz x v); (t*q.w + q.xyz x t) + v
+//   q^-1 * v      = same formula with (-q.xyz, q.w)
+//   iso * p       = (q * p) + t ;  iso^-1 * p = q^-1 * (p - t)
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+#include <algorithm>
+#include <functional>
+
+namespace ncxo {
+
+template <class R> struct Eps;
+template <> struct Eps<float> { static float eps() { return 1.1920929e-07f; } static float maxv() { return 3.40282347e+38f; } };
+template <> struct Eps<double> { static double eps() { return 2.2204460492503131e-16; } static double maxv() { return 1.7976931348623157e+308; } };
+
+// ---------------------------------------------------------------- vectors
+template <class R> struct V3 {
+  R x, y, z;
+  V3() : x(0), y(0), z(0) {}
+  V3(R a, R b, R c) : x(a), y(b), z(c) {}
+  R& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+  R operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return V3<R>(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return V3<R>(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class R> inline V3<R> operator-(V3<R> a) { return V3<R>(-a.x, -a.y, -a.z); }
+template <class R> inline V3<R> operator*(V3<R> a, R s) { return V3<R>(a.x * s, a.y * s, a.z * s); }
+template <class R> inline V3<R> operator/(V3<R> a, R s) { return V3<R>(a.x / s, a.y / s, a.z / s); }
+template <class R> inline R dot(V3<R> a, V3<R> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
+  return V3<R>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <class R> inline R norm_sq(V3<R> a) { return dot(a, a); }
+template <class R> inline R norm(V3<R> a) { return std::sqrt(norm_sq(a)); }
+template <class R> inline V3<R> normalize(V3<R> a) { return a / norm(a); }
+// nalgebra Unit::try_new_and_get: None when norm <= min_norm.
+template <class R> inline bool try_normalize(V3<R> v, R min_norm, V3<R>* out, R* n_out = nullptr) {
+  R n = norm(v);
+  if (n <= min_norm) return false;
+  *out = v / n;
+  if (n_out) *n_out = n;
+  return true;
+}
+template <class R> inline V3<R> vmin(V3<R> a, V3<R> b) { return V3<R>(std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z)); }
+template <class R> inline V3<R> vmax(V3<R> a, V3<R> b) { return V3<R>(std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z)); }
+
+template <class R> struct Quat { R i, j, k, w; };
+template <class R> inline V3<R> qrot(const Quat<R>& q, V3<R> v) {
+  V3<R> qv(q.i, q.j, q.k);
+  V3<R> t = cross(qv, v) * R(2);
+  V3<R> c = cross(qv, t);
+  return (t * q.w + c) + v;
+}
+template <class R> inline V3<R> qrot_inv(const Quat<R>& q, V3<R> v) {
+  Quat<R> c{-q.i, -q.j, -q.k, q.w};
+  return qrot(c, v);
+}
+template <class R> struct Iso {
+  V3<R> t;
+  Quat<R> q;
+  static Iso identity() { Iso m; m.t = V3<R>(0, 0, 0); m.q = Quat<R>{0, 0, 0, 1}; return m; }
+};
+template <class R> inline V3<R> xform_pt(const Iso<R>& m, V3<R> p) { return qrot(m.q, p) + m.t; }
+template <class R> inline V3<R> xform_vec(const Iso<R>& m, V3<R> v) { return qrot(m.q, v); }
+template <class R> inline V3<R> inv_xform_pt(const Iso<R>& m, V3<R> p) { return qrot_inv(m.q, p - m.t); }
+template <class R> inline V3<R> inv_xform_vec(const Iso<R>& m, V3<R> v) { return qrot_inv(m.q, v); }
+// UnitQuaternion::to_rotation_matrix (row-major), used by the cuboid AABB.
+template <class R> inline void quat_to_mat(const Quat<R>& q, R m[3][3]) {
+  R i = q.i, j = q.j, k = q.k, w = q.w;
+  R ww = w * w, ii = i * i, jj = j * j, kk = k * k;
+  R ij = i * j * R(2), wk = w * k * R(2), wj = w * j * R(2);
+  R ik = i * k * R(2), jk = j * k * R(2), wi = w * i * R(2);
+  m[0][0] = ww + ii - jj - kk; m[0][1] = ij - wk; m[0][2] = wj + ik;
+  m[1][0] = wk + ij; m[1][1] = ww - ii + jj - kk; m[1][2] = jk - wi;
+  m[2][0] = ik - wj; m[2][1] = wi + jk; m[2][2] = ww - ii - jj + kk;
+}
+
+template <class R> inline R gjk_eps_tol() { return Eps<R>::eps() * R(10); }  // gjk.rs:26-29
+
+// ---------------------------------------------------------------- shapes
+enum ShapeType : uint32_t { BALL = 0, CUBOID = 1, CAPSULE = 2, HULL = 3, ORIGIN = 4 };
+
+template <class R> struct Shape {
+  uint32_t type;
+  R a, b, c;              // ball: r; cuboid: hx,hy,hz; capsule: half_height, radius
+  const V3<R>* pts;       // hull vertices (local frame)
+  uint32_t npts;
+};
+
+// utils/point_cloud_support_point.rs:6-24 (strict '>' keeps the first maximum)
+template <class R> inline uint32_t point_cloud_support_id(V3<R> dir, const V3<R>* pts, uint32_t n) {
+  uint32_t best = 0;
+  R best_dot = dot(pts[0], dir);
+  for (uint32_t i = 1; i < n; ++i) {
+    R d = dot(pts[i], dir);
+    if (d > best_dot) { best_dot = d; best = i; }
+  }
+  return best;
+}
+
+// SupportMap::support_point (shape/support_map.rs:26-29) with the overrides of
+// Ball (ball.rs:31-38), Cuboid (cuboid.rs:135-146), Capsule (capsule.rs:72-84),
+// ConvexHull (convex.rs:418-423) and ConstantOrigin (special_support_maps.rs:9-13).
+template <class R> inline V3<R> support_point(const Shape<R>& g, const Iso<R>& m, V3<R> dir) {
+  switch (g.type) {
+    case BALL: {
+      V3<R> u = normalize(dir);
+      return m.t + u * g.a;
+    }
+    case CUBOID: {
+      V3<R> ld = inv_xform_vec(m, dir);
+      V3<R> p(std::copysign(g.a, ld.x), std::copysign(g.b, ld.y), std::copysign(g.c, ld.z));
+      return xform_pt(m, p);
+    }
+    case CAPSULE: {
+      V3<R> ld = inv_xform_vec(m, dir);
+      V3<R> u = normalize(ld);
+      V3<R> res(R(0), std::copysign(g.a, u.y), R(0));
+      V3<R> p = res + u * g.b;
+      return xform_pt(m, p);
+    }
+    case HULL: {
+      V3<R> ld = inv_xform_vec(m, dir);
+      uint32_t id = point_cloud_support_id(ld, g.pts, g.npts);
+      return xform_pt(m, g.pts[id]);
+    }
+    default:
+      return V3<R>(0, 0, 0);
+  }
+}
+
+// ---------------------------------------------------------------- CSO point (query/algorithms/cso_point.rs:13-105)
+template <class R> struct CSOPoint {
+  V3<R> point, orig1, orig2;
+};
+template <class R> inline CSOPoint<R> cso_from_shapes(const Iso<R>& m1, const Shape<R>& g1, const Iso<R>& m2,
+                                                      const Shape<R>& g2, V3<R> dir) {
+  CSOPoint<R> c;
+  c.orig1 = support_point(g1, m1, dir);
+  c.orig2 = support_point(g2, m2, -dir);
+  c.point = c.orig1 - c.orig2;
+  return c;
+}
+
+// ---------------------------------------------------------------- Voronoi projections
+// Location codes shared by the three projections.
+enum LocKind { ON_VERTEX = 0, ON_EDGE = 1, ON_FACE = 2, ON_SOLID = 3 };
+template <class R> struct Loc {
+  int kind;
+  int id;
+  R bc[3];
+};
+
+// query/point/point_segment.rs:52-91 (identity isometry, point = origin)
+template <class R> inline V3<R> project_segment(V3<R> a, V3<R> b, V3<R> p, Loc<R>* loc) {
+  Iso<R> m = Iso<R>::identity();
+  V3<R> ls_pt = inv_xform_pt(m, p);
+  V3<R> ab = b - a;
+  V3<R> ap = ls_pt - a;
+  R ab_ap = dot(ab, ap);
+  R sqnab = norm_sq(ab);
+  if (ab_ap <= R(0)) {
+    loc->kind = ON_VERTEX; loc->id = 0;
+    return xform_pt(m, a);
+  } else if (ab_ap >= sqnab) {
+    loc->kind = ON_VERTEX; loc->id = 1;
+    return xform_pt(m, b);
+  } else {
+    R u = ab_ap / sqnab;
+    loc->kind = ON_EDGE; loc->id = 0;
+    loc->bc[0] = R(1) - u; loc->bc[1] = u;
+    V3<R> proj = a + ab * u;
+    return xform_pt(m, proj);
+  }
+}
+
+// query/point/point_triangle.rs:61-309 (dim3, fixed-point feature off, solid as given)
+template <class R> inline V3<R> project_triangle(V3<R> a, V3<R> b, V3<R> c, V3<R> pt, bool solid, Loc<R>* loc) {
+  Iso<R> m = Iso<R>::identity();
+  V3<R> p = inv_xform_pt(m, pt);
+  V3<R> ab = b - a, ac = c - a, ap = p - a;
+  R ab_ap = dot(ab, ap), ac_ap = dot(ac, ap);
+  if (ab_ap <= R(0) && ac_ap <= R(0)) { loc->kind = ON_VERTEX; loc->id = 0; return xform_pt(m, a); }
+  V3<R> bp = p - b;
+  R ab_bp = dot(ab, bp), ac_bp = dot(ac, bp);
+  if (ab_bp >= R(0) && ac_bp <= ab_bp) { loc->kind = ON_VERTEX; loc->id = 1; return xform_pt(m, b); }
+  V3<R> cp = p - c;
+  R ab_cp = dot(ab, cp), ac_cp = dot(ac, cp);
+  if (ac_cp >= R(0) && ab_cp <= ac_cp) { loc->kind = ON_VERTEX; loc->id = 2; return xform_pt(m, c); }
+  V3<R> bc = c - b;
+  // stable_check_edges_voronoi (dim3): point_triangle.rs:158-191
+  V3<R> n = cross(ab, ac);
+  R vc = dot(n, cross(ab, ap));
+  if (vc < R(0) && ab_ap >= R(0) && ab_bp <= R(0)) {
+    R v = ab_ap / norm_sq(ab);
+    loc->kind = ON_EDGE; loc->id = 0; loc->bc[0] = R(1) - v; loc->bc[1] = v;
+    return xform_pt(m, a + ab * v);
+  }
+  R vb = -dot(n, cross(ac, cp));
+  if (vb < R(0) && ac_ap >= R(0) && ac_cp <= R(0)) {
+    R w = ac_ap / norm_sq(ac);
+    loc->kind = ON_EDGE; loc->id = 2; loc->bc[0] = R(1) - w; loc->bc[1] = w;
+    return xform_pt(m, a + ac * w);
+  }
+  R va = dot(n, cross(bc, bp));
+  if (va < R(0) && ac_bp - ab_bp >= R(0) && ab_cp - ac_cp >= R(0)) {
+    R w = dot(bc, bp) / norm_sq(bc);
+    loc->kind = ON_EDGE; loc->id = 1; loc->bc[0] = R(1) - w; loc->bc[1] = w;
+    return xform_pt(m, b + bc * w);
+  }
+  int clockwise = dot(n, ap) >= R(0) ? 0 : 1;
+  if (va + vb + vc != R(0)) {
+    R denom = R(1) / (va + vb + vc);
+    R v = vb * denom;
+    R w = vc * denom;
+    loc->kind = ON_FACE; loc->id = clockwise;
+    loc->bc[0] = R(1) - v - w; loc->bc[1] = v; loc->bc[2] = w;
+    V3<R> res = a + ab * v + ac * w;
+    return xform_pt(m, res);
+  }
+  if (solid) { loc->kind = ON_SOLID; loc->id = 0; return pt; }
+  // Non-solid fallback (point_triangle.rs:261-307); unused by GJK/EPA (solid=true).
+  R v = ab_ap / (ab_ap - ab_bp);
+  R w = ac_ap / (ac_ap - ac_cp);
+  R u = (ac_bp - ab_bp) / (ac_bp - ab_bp + ab_cp - ac_cp);
+  R d_ab = norm_sq(ap) - (norm_sq(ab) * v * v);
+  R d_ac = norm_sq(ap) - (norm_sq(ac) * u * u);
+  R d_bc = norm_sq(bp) - (norm_sq(bc) * w * w);
+  if (d_ab < d_ac) {
+    if (d_ab < d_bc) { loc->kind = ON_EDGE; loc->id = 0; loc->bc[0] = R(1) - v; loc->bc[1] = v; return xform_pt(m, a + ab * v); }
+    loc->kind = ON_EDGE; loc->id = 1; loc->bc[0] = R(1) - u; loc->bc[1] = u; return xform_pt(m, b + bc * u);
+  }
+  if (d_ac < d_bc) { loc->kind = ON_EDGE; loc->id = 2; loc->bc[0] = R(1) - w; loc->bc[1] = w; return xform_pt(m, a + ac * w); }
+  loc->kind = ON_EDGE; loc->id = 1; loc->bc[0] = R(1) - u; loc->bc[1] = u; return xform_pt(m, b + bc * u);
+}
+
+// query/point/point_tetrahedron.rs:35-353 (solid = true)
+template <class R> inline V3<R> project_tetrahedron(V3<R> A, V3<R> B, V3<R> C, V3<R> D, V3<R> pt, Loc<R>* loc) {
+  Iso<R> m = Iso<R>::identity();
+  V3<R> p = inv_xform_pt(m, pt);
+  V3<R> ab = B - A, ac = C - A, ad = D - A, ap = p - A;
+  R ap_ab = dot(ap, ab), ap_ac = dot(ap, ac), ap_ad = dot(ap, ad);
+  if (ap_ab <= R(0) && ap_ac <= R(0) && ap_ad <= R(0)) { loc->kind = ON_VERTEX; loc->id = 0; return xform_pt(m, A); }
+  V3<R> bc = C - B, bd = D - B, bp = p - B;
+  R bp_bc = dot(bp, bc), bp_bd = dot(bp, bd), bp_ab = dot(bp, ab);
+  if (bp_bc <= R(0) && bp_bd <= R(0) && bp_ab >= R(0)) { loc->kind = ON_VERTEX; loc->id = 1; return xform_pt(m, B); }
+  V3<R> cd = D - C, cp = p - C;
+  R cp_ac = dot(cp, ac), cp_bc = dot(cp, bc), cp_cd = dot(cp, cd);
+  if (cp_cd <= R(0) && cp_bc >= R(0) && cp_ac >= R(0)) { loc->kind = ON_VERTEX; loc->id = 2; return xform_pt(m, C); }
+  V3<R> dp = p - D;
+  R dp_cd = dot(dp, cd), dp_bd = dot(dp, bd), dp_ad = dot(dp, ad);
+  if (dp_ad >= R(0) && dp_bd >= R(0) && dp_cd >= R(0)) { loc->kind = ON_VERTEX; loc->id = 3; return xform_pt(m, D); }
+
+  // check_edge (point_tetrahedron.rs:106-152)
+  auto check_edge = [&](int i, V3<R> a, V3<R> nabc, V3<R> nabd, V3<R> ap_, V3<R> ab_, R ap_ab_, R bp_ab_,
+                        R* dabc, R* dabd, V3<R>* out) -> bool {
+    R ab_ab = ap_ab_ - bp_ab_;
+    V3<R> ap_x_ab = cross(ap_, ab_);
+    *dabc = dot(ap_x_ab, nabc);
+    *dabd = dot(ap_x_ab, nabd);
+    if (ab_ab != R(0) && *dabc >= R(0) && *dabd >= R(0) && ap_ab_ >= R(0) && ap_ab_ <= ab_ab) {
+      R u = ap_ab_ / ab_ab;
+      loc->kind = ON_EDGE; loc->id = i; loc->bc[0] = R(1) - u; loc->bc[1] = u;
+      *out = xform_pt(m, a + ab_ * u);
+      return true;
+    }
+    return false;
+  };
+  V3<R> out;
+  V3<R> nabc = cross(ab, ac), nabd = cross(ab, ad);
+  R dabc, dabd, dacd, dacb, dadb, dadc, dbca, dbcd, dbdc, dbda, dcda, dcdb;
+  if (check_edge(0, A, nabc, nabd, ap, ab, ap_ab, bp_ab, &dabc, &dabd, &out)) return out;
+  V3<R> nacd = cross(ac, ad);
+  if (check_edge(1, A, nacd, -nabc, ap, ac, ap_ac, cp_ac, &dacd, &dacb, &out)) return out;
+  if (check_edge(2, A, -nabd, -nacd, ap, ad, ap_ad, dp_ad, &dadb, &dadc, &out)) return out;
+  V3<R> nbcd = cross(bc, bd);
+  if (check_edge(3, B, nabc, nbcd, bp, bc, bp_bc, cp_bc, &dbca, &dbcd, &out)) return out;
+  if (check_edge(4, B, -nbcd, nabd, bp, bd, bp_bd, dp_bd, &dbdc, &dbda, &out)) return out;
+  if (check_edge(5, C, nacd, nbcd, cp, cd, cp_cd, dp_cd, &dcda, &dcdb, &out)) return out;
+
+  // check_face (point_tetrahedron.rs:251-306)
+  auto check_face = [&](int i, V3<R> a, V3<R> b, V3<R> c, V3<R> ap_, V3<R> bp_, V3<R> cp_, V3<R> ab_, V3<R> ac_,
+                        V3<R> ad_, R d1, R d2, R d3, V3<R>* o) -> bool {
+    if (d1 < R(0) && d2 < R(0) && d3 < R(0)) {
+      V3<R> n = cross(ab_, ac_);
+      if (dot(n, ad_) * dot(n, ap_) < R(0)) {
+        V3<R> normal;
+        if (!try_normalize(n, Eps<R>::eps(), &normal)) return false;
+        R vc = dot(normal, cross(ap_, bp_));
+        R va = dot(normal, cross(bp_, cp_));
+        R vb = dot(normal, cross(cp_, ap_));
+        R denom = va + vb + vc;
+        R inv_denom = R(1) / denom;
+        R bcs[3] = {va * inv_denom, vb * inv_denom, vc * inv_denom};
+        V3<R> res = a * bcs[0] + b * bcs[1] + c * bcs[2];
+        loc->kind = ON_FACE; loc->id = i; loc->bc[0] = bcs[0]; loc->bc[1] = bcs[1]; loc->bc[2] = bcs[2];
+        *o = xform_pt(m, res);
+        return true;
+      }
+    }
+    return false;
+  };
+  if (check_face(0, A, B, C, ap, bp, cp, ab, ac, ad, dabc, dbca, dacb, &out)) return out;
+  if (check_face(1, A, B, D, ap, bp, dp, ab, ad, ac, dadb, dabd, dbda, &out)) return out;
+  if (check_face(2, A, C, D, ap, cp, dp, ac, ad, ab, dacd, dcda, dadc, &out)) return out;
+  if (check_face(3, B, C, D, bp, cp, dp, bc, bd, -ab, dbcd, dcdb, dbdc, &out)) return out;
+  loc->kind = ON_SOLID; loc->id = 0;
+  return xform_pt(m, p);
+}
+
+// ---------------------------------------------------------------- VoronoiSimplex (voronoi_simplex3.rs:12-352)
+template <class R> struct VoronoiSimplex {
+  int prev_vertices[4] = {0, 1, 2, 3};
+  R prev_proj[3] = {0, 0, 0};
+  int prev_dim = 0;
+  CSOPoint<R> vertices[4];
+  R proj[3] = {0, 0, 0};
+  int dim = 0;
+
+  void swap(int i1, int i2) {
+    std::swap(vertices[i1], vertices[i2]);
+    std::swap(prev_vertices[i1], prev_vertices[i2]);
+  }
+  void reset(const CSOPoint<R>& pt) { dim = 0; prev_dim = 0; vertices[0] = pt; }
+  bool add_point(const CSOPoint<R>& pt) {
+    prev_dim = dim;
+    prev_proj[0] = proj[0]; prev_proj[1] = proj[1]; prev_proj[2] = proj[2];
+    prev_vertices[0] = 0; prev_vertices[1] = 1; prev_vertices[2] = 2; prev_vertices[3] = 3;
+    R tol = gjk_eps_tol<R>();
+    if (dim == 0) {
+      if (norm_sq(vertices[0].point - pt.point) < tol) return false;
+    } else if (dim == 1) {
+      V3<R> ab = vertices[1].point - vertices[0].point;
+      V3<R> ac = pt.point - vertices[0].point;
+      if (norm_sq(cross(ab, ac)) < tol) return false;
+    } else {
+      V3<R> ab = vertices[1].point - vertices[0].point;
+      V3<R> ac = vertices[2].point - vertices[0].point;
+      V3<R> ap = pt.point - vertices[0].point;
+      V3<R> n = normalize(cross(ab, ac));
+      if (std::fabs(dot(n, ap)) < tol) return false;
+    }
+    dim += 1;
+    vertices[dim] = pt;
+    return true;
+  }
+  V3<R> project_origin_and_reduce() {
+    V3<R> o(0, 0, 0);
+    if (dim == 0) {
+      proj[0] = R(1);
+      return vertices[0].point;
+    } else if (dim == 1) {
+      Loc<R> loc;
+      V3<R> pr = project_segment(vertices[0].point, vertices[1].point, o, &loc);
+      if (loc.kind == ON_VERTEX && loc.id == 0) { proj[0] = R(1); dim = 0; }
+      else if (loc.kind == ON_VERTEX) { swap(0, 1); proj[0] = R(1); dim = 0; }
+      else { proj[0] = loc.bc[0]; proj[1] = loc.bc[1]; }
+      return pr;
+    } else if (dim == 2) {
+      Loc<R> loc;
+      V3<R> pr = project_triangle(vertices[0].point, vertices[1].point, vertices[2].point, o, true, &loc);
+      if (loc.kind == ON_VERTEX) { swap(0, loc.id); proj[0] = R(1); dim = 0; }
+      else if (loc.kind == ON_EDGE) {
+        if (loc.id == 0) { proj[0] = loc.bc[0]; proj[1] = loc.bc[1]; }
+        else if (loc.id == 1) { swap(0, 2); proj[0] = loc.bc[1]; proj[1] = loc.bc[0]; }
+        else { swap(1, 2); proj[0] = loc.bc[0]; proj[1] = loc.bc[1]; }
+        dim = 1;
+      } else if (loc.kind == ON_FACE) {
+        proj[0] = loc.bc[0]; proj[1] = loc.bc[1]; proj[2] = loc.bc[2];
+      }
+      return pr;
+    } else {
+      Loc<R> loc;
+      V3<R> pr = project_tetrahedron(vertices[0].point, vertices[1].point, vertices[2].point, vertices[3].point, o, &loc);
+      if (loc.kind == ON_VERTEX) { swap(0, loc.id); proj[0] = R(1); dim = 0; }
+      else if (loc.kind == ON_EDGE) {
+        switch (loc.id) {
+          case 0: break;
+          case 1: swap(1, 2); break;
+          case 2: swap(1, 3); break;
+          case 3: swap(0, 2); break;
+          case 4: swap(0, 3); break;
+          case 5: swap(0, 2); swap(1, 3); break;
+        }
+        if (loc.id == 3 || loc.id == 4) { proj[0] = loc.bc[1]; proj[1] = loc.bc[0]; }
+        else { proj[0] = loc.bc[0]; proj[1] = loc.bc[1]; }
+        dim = 1;
+      } else if (loc.kind == ON_FACE) {
+        switch (loc.id) {
+          case 0: proj[0] = loc.bc[0]; proj[1] = loc.bc[1]; proj[2] = loc.bc[2]; break;
+          case 1: vertices[2] = vertices[3]; proj[0] = loc.bc[0]; proj[1] = loc.bc[1]; proj[2] = loc.bc[2]; break;
+          case 2: vertices[1] = vertices[3]; proj[0] = loc.bc[0]; proj[1] = loc.bc[2]; proj[2] = loc.bc[1]; break;
+          case 3: vertices[0] = vertices[3]; proj[0] = loc.bc[2]; proj[1] = loc.bc[0]; proj[2] = loc.bc[1]; break;
+        }
+        dim = 2;
+      }
+      return pr;
+    }
+  }
+  const CSOPoint<R>& prev_point(int i) const { return vertices[prev_vertices[i]]; }
+};
+
+// gjk.rs:367-388
+template <class R> inline void gjk_result(const VoronoiSimplex<R>& s, bool prev, V3<R>* p1, V3<R>* p2) {
+  V3<R> r1(0, 0, 0), r2(0, 0, 0);
+  if (prev) {
+    for (int i = 0; i < s.prev_dim + 1; ++i) {
+      R coord = s.prev_proj[i];
+      const CSOPoint<R>& pt = s.prev_point(i);
+      r1 = r1 + pt.orig1 * coord;
+      r2 = r2 + pt.orig2 * coord;
+    }
+  } else {
+    for (int i = 0; i < s.dim + 1; ++i) {
+      R coord = s.proj[i];
+      const CSOPoint<R>& pt = s.vertices[i];
+      r1 = r1 + pt.orig1 * coord;
+      r2 = r2 + pt.orig2 * coord;
+    }
+  }
+  *p1 = r1; *p2 = r2;
+}
+
+// ---------------------------------------------------------------- GJK (gjk.rs:76-177)
+enum GJKStatus { GJK_INTERSECTION = 0, GJK_CLOSEST = 1, GJK_PROXIMITY = 2, GJK_NO_INTERSECTION = 3, GJK_NAN = 4 };
+template <class R> struct GJKResult {
+  int status;
+  V3<R> p1, p2, dir;
+  int iters;  // support-point evaluations inside the loop (the reference's niter plus the final one)
+};
+
+template <class R> inline GJKResult<R> gjk_closest_points(const Iso<R>& m1, const Shape<R>& g1, const Iso<R>& m2,
+                                                          const Shape<R>& g2, R max_dist, bool exact_dist,
+                                                          VoronoiSimplex<R>& simplex, int max_iter = 10000) {
+  GJKResult<R> res;
+  res.iters = 0;
+  R eps_tol = gjk_eps_tol<R>();
+  R eps_rel = std::sqrt(eps_tol);
+  V3<R> proj = simplex.project_origin_and_reduce();
+  V3<R> old_dir;
+  {
+    V3<R> pd;
+    if (try_normalize(proj, R(0), &pd)) old_dir = -pd;
+    else { res.status = GJK_INTERSECTION; return res; }
+  }
+  R max_bound = Eps<R>::maxv();
+  V3<R> dir;
+  int niter = 0;
+  for (;;) {
+    R old_max_bound = max_bound;
+    {
+      V3<R> nd; R dist;
+      if (try_normalize(-proj, eps_tol, &nd, &dist)) { dir = nd; max_bound = dist; }
+      else { res.status = GJK_INTERSECTION; return res; }
+    }
+    if (max_bound >= old_max_bound) {
+      if (exact_dist) { gjk_result(simplex, true, &res.p1, &res.p2); res.dir = old_dir; res.status = GJK_CLOSEST; }
+      else { res.dir = old_dir; res.status = GJK_PROXIMITY; }
+      return res;
+    }
+    CSOPoint<R> cso = cso_from_shapes(m1, g1, m2, g2, dir);
+    res.iters += 1;
+    R min_bound = -dot(dir, cso.point);
+    if (!(min_bound == min_bound)) { res.status = GJK_NAN; return res; }
+    if (min_bound > max_dist) { res.dir = dir; res.status = GJK_NO_INTERSECTION; return res; }
+    else if (!exact_dist && min_bound > R(0) && max_bound <= max_dist) { res.dir = old_dir; res.status = GJK_PROXIMITY; return res; }
+    else if (max_bound - min_bound <= eps_rel * max_bound) {
+      if (exact_dist) { gjk_result(simplex, false, &res.p1, &res.p2); res.dir = dir; res.status = GJK_CLOSEST; }
+      else { res.dir = dir; res.status = GJK_PROXIMITY; }
+      return res;
+    }
+    if (!simplex.add_point(cso)) {
+      if (exact_dist) { gjk_result(simplex, false, &res.p1, &res.p2); res.dir = dir; res.status = GJK_CLOSEST; }
+      else { res.dir = dir; res.status = GJK_PROXIMITY; }
+      return res;
+    }
+    old_dir = dir;
+    proj = simplex.project_origin_and_reduce();
+    if (simplex.dim == 3) {
+      if (min_bound >= eps_tol) {
+        if (exact_dist) { gjk_result(simplex, true, &res.p1, &res.p2); res.dir = old_dir; res.status = GJK_CLOSEST; }
+        else { res.dir = old_dir; res.status = GJK_PROXIMITY; }
+        return res;
+      } else { res.status = GJK_INTERSECTION; return res; }
+    }
+    niter += 1;
+    if (niter == max_iter) { res.dir = V3<R>(1, 0, 0); res.status = GJK_NO_INTERSECTION; return res; }
+  }
+}
+
+// ---------------------------------------------------------------- Rust std BinaryHeap (max-heap) restatement
+// push = sift_up(0, len); pop = swap-with-last, sift_down_to_bottom(0) then sift_up.
+// `le(a, b)` is PartialOrd::le of the element type.
+template <class T, class LE> struct RustHeap {
+  std::vector<T> data;
+  LE le;
+  void sift_up(size_t start, size_t pos) {
+    T elt = data[pos];
+    while (pos > start) {
+      size_t parent = (pos - 1) / 2;
+      if (le(elt, data[parent])) break;
+      data[pos] = data[parent];
+      pos = parent;
+    }
+    data[pos] = elt;
+  }
+  void push(const T& v) { data.push_back(v); sift_up(0, data.size() - 1); }
+  bool empty() const { return data.empty(); }
+  const T& peek() const { return data[0]; }
+  bool pop(T* out) {
+    if (data.empty()) return false;
+    T item = data.back();
+    data.pop_back();
+    if (!data.empty()) {
+      std::swap(item, data[0]);
+      // sift_down_to_bottom(0)
+      size_t end = data.size();
+      size_t pos = 0;
+      T elt = data[0];
+      size_t child = 1;
+      while (end >= 2 && child <= end - 2) {
+        child += le(data[child], data[child + 1]) ? 1 : 0;
+        data[pos] = data[child];
+        pos = child;
+        child = 2 * pos + 1;
+      }
+      if (child == end - 1) { data[pos] = data[child]; pos = child; }
+      data[pos] = elt;
+      sift_up(0, pos);
+    }
+    *out = item;
+    return true;
+  }
+};
+
+// ---------------------------------------------------------------- EPA (epa3.rs:171-454)
+template <class R> struct EPAFaceId { uint32_t id; R neg_dist; };
+template <class R> struct EPAFaceLE { bool operator()(const EPAFaceId<R>& a, const EPAFaceId<R>& b) const { return a.neg_dist <= b.neg_dist; } };
+
+template <class R> struct EPAFace {
+  uint32_t pts[3], adj[3];
+  V3<R> normal;
+  R bcoords[3];
+  bool deleted;
+};
+
+// utils/triangle.rs:85-102 (relative_eq!(|p1p2 x p1p3|^2, 0, epsilon = (100 eps)^2))
+template <class R> inline bool is_affinely_dependent_triangle(V3<R> p1, V3<R> p2, V3<R> p3) {
+  V3<R> p1p2 = p2 - p1, p1p3 = p3 - p1;
+  R e = Eps<R>::eps() * R(100);
+  R x = norm_sq(cross(p1p2, p1p3));
+  R epsilon = e * e;
+  if (x == R(0)) return true;
+  if (std::isinf(x)) return false;
+  R diff = std::fabs(x - R(0));
+  if (diff <= epsilon) return true;
+  R largest = std::fabs(x);
+  return diff <= largest * Eps<R>::eps();
+}
+
+template <class R> struct EPA {
+  std::vector<CSOPoint<R>> vertices;
+  std::vector<EPAFace<R>> faces;
+  struct Sil { uint32_t face_id, opp_pt_id; };
+  std::vector<Sil> silhouette;
+  RustHeap<EPAFaceId<R>, EPAFaceLE<R>> heap;
+  int iters = 0;
+
+  // Face::new (epa3.rs:96-115) + new_with_proj (:63-94)
+  bool make_face(const uint32_t pts[3], const uint32_t adj[3], EPAFace<R>* f) {
+    Loc<R> loc;
+    project_triangle(vertices[pts[0]].point, vertices[pts[1]].point, vertices[pts[2]].point, V3<R>(0, 0, 0), true, &loc);
+    for (int i = 0; i < 3; ++i) { f->pts[i] = pts[i]; f->adj[i] = adj[i]; }
+    f->deleted = false;
+    // utils/ccw_face_normal.rs:21-27
+    V3<R> ab = vertices[pts[1]].point - vertices[pts[0]].point;
+    V3<R> ac = vertices[pts[2]].point - vertices[pts[0]].point;
+    V3<R> n;
+    if (try_normalize(cross(ab, ac), Eps<R>::eps(), &n)) f->normal = n;
+    else f->normal = V3<R>(0, 0, 0);
+    if (loc.kind == ON_FACE) {
+      f->bcoords[0] = loc.bc[0]; f->bcoords[1] = loc.bc[1]; f->bcoords[2] = loc.bc[2];
+      return true;
+    }
+    f->bcoords[0] = f->bcoords[1] = f->bcoords[2] = R(0);
+    return false;
+  }
+  void face_closest_points(const EPAFace<R>& f, V3<R>* p1, V3<R>* p2) {
+    *p1 = vertices[f.pts[0]].orig1 * f.bcoords[0] + vertices[f.pts[1]].orig1 * f.bcoords[1] + vertices[f.pts[2]].orig1 * f.bcoords[2];
+    *p2 = vertices[f.pts[0]].orig2 * f.bcoords[0] + vertices[f.pts[1]].orig2 * f.bcoords[1] + vertices[f.pts[2]].orig2 * f.bcoords[2];
+  }
+  static uint32_t next_ccw_pt_id(const EPAFace<R>& f, uint32_t id) {
+    if (f.pts[0] == id) return 1;
+    if (f.pts[1] == id) return 2;
+    return 0;
+  }
+  bool can_be_seen_by(const EPAFace<R>& f, uint32_t point, uint32_t opp_pt_id) {
+    V3<R> p0 = vertices[f.pts[opp_pt_id]].point;
+    V3<R> p1 = vertices[f.pts[(opp_pt_id + 1) % 3]].point;
+    V3<R> p2 = vertices[f.pts[(opp_pt_id + 2) % 3]].point;
+    V3<R> pt = vertices[point].point;
+    return dot(pt - p0, f.normal) >= -gjk_eps_tol<R>() || is_affinely_dependent_triangle(p1, p2, pt);
+  }
+  void compute_silhouette(uint32_t point, uint32_t id, uint32_t opp_pt_id) {
+    if (!faces[id].deleted) {
+      if (!can_be_seen_by(faces[id], point, opp_pt_id)) {
+        silhouette.push_back(Sil{id, opp_pt_id});
+      } else {
+        faces[id].deleted = true;
+        uint32_t adj_pt_id1 = (opp_pt_id + 2) % 3;
+        uint32_t adj_pt_id2 = opp_pt_id;
+        uint32_t adj1 = faces[id].adj[adj_pt_id1];
+        uint32_t adj2 = faces[id].adj[adj_pt_id2];
+        uint32_t o1 = next_ccw_pt_id(faces[adj1], faces[id].pts[adj_pt_id1]);
+        uint32_t o2 = next_ccw_pt_id(faces[adj2], faces[id].pts[adj_pt_id2]);
+        compute_silhouette(point, adj1, o1);
+        compute_silhouette(point, adj2, o2);
+      }
+    }
+  }
+
+  // Returns false for the reference's `None`.
+  bool closest_points(const Iso<R>& m1, const Shape<R>& g1, const Iso<R>& m2, const Shape<R>& g2,
+                      const VoronoiSimplex<R>& simplex, V3<R>* p1, V3<R>* p2, V3<R>* n, int max_iter = 10000) {
+    R eps_tol = Eps<R>::eps() * R(100);
+    vertices.clear(); faces.clear(); heap.data.clear(); silhouette.clear();
+    iters = 0;
+    for (int i = 0; i < simplex.dim + 1; ++i) vertices.push_back(simplex.vertices[i]);
+    if (simplex.dim == 0) {
+      *p1 = V3<R>(0, 0, 0); *p2 = V3<R>(0, 0, 0); *n = V3<R>(0, 1, 0);
+      return true;
+    } else if (simplex.dim == 3) {
+      V3<R> dp1 = vertices[1].point - vertices[0].point;
+      V3<R> dp2 = vertices[2].point - vertices[0].point;
+      V3<R> dp3 = vertices[3].point - vertices[0].point;
+      if (dot(cross(dp1, dp2), dp3) > R(0)) std::swap(vertices[1], vertices[2]);
+      const uint32_t P[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {0, 3, 1}};
+      const uint32_t A[4][3] = {{3, 1, 2}, {3, 2, 0}, {0, 1, 3}, {2, 1, 0}};
+      bool inside[4];
+      for (int f = 0; f < 4; ++f) { EPAFace<R> face; inside[f] = make_face(P[f], A[f], &face); faces.push_back(face); }
+      for (int f = 0; f < 4; ++f) {
+        if (inside[f]) {
+          R dist = dot(faces[f].normal, vertices[f].point);
+          if (-dist > gjk_eps_tol<R>()) return false;  // FaceId::new(..)? -> None
+          heap.push(EPAFaceId<R>{(uint32_t)f, -dist});
+        }
+      }
+    } else {
+      if (simplex.dim == 1) {
+        V3<R> dpt = vertices[1].point - vertices[0].point;
+        // Vector::orthonormal_subspace_basis(&[dpt], f) (nalgebra, 3D, one vector):
+        // a = (v.z, 0, -v.x) if |v.x| > |v.y| else (0, -v.z, v.y); a /= |a|;
+        // the first callback receives a x v (epa3.rs:299-305 stops after it).
+        // dpt is not normalised by the caller, so the direction is not unit.
+        V3<R> a;
+        if (std::fabs(dpt.x) > std::fabs(dpt.y)) a = V3<R>(dpt.z, R(0), -dpt.x);
+        else a = V3<R>(R(0), -dpt.z, dpt.y);
+        a = normalize(a);
+        V3<R> b = cross(a, dpt);
+        vertices.push_back(cso_from_shapes(m1, g1, m2, g2, b));
+      }
+      const uint32_t P1[3] = {0, 1, 2}, P2[3] = {0, 2, 1};
+      const uint32_t A1[3] = {1, 1, 1}, A2[3] = {0, 0, 0};
+      EPAFace<R> f1, f2;
+      make_face(P1, A1, &f1); make_face(P2, A2, &f2);
+      faces.push_back(f1); faces.push_back(f2);
+      heap.push(EPAFaceId<R>{0, R(0)});
+      heap.push(EPAFaceId<R>{1, R(0)});
+    }
+    int niter = 0;
+    R max_dist = Eps<R>::maxv();
+    if (heap.empty()) return false;  // the reference's heap.peek().unwrap() would panic here
+    EPAFaceId<R> best_face_id = heap.peek();
+    EPAFaceId<R> face_id;
+    while (heap.pop(&face_id)) {
+      EPAFace<R> face = faces[face_id.id];
+      if (face.deleted) continue;
+      CSOPoint<R> cso = cso_from_shapes(m1, g1, m2, g2, face.normal);
+      iters += 1;
+      uint32_t support_point_id = (uint32_t)vertices.size();
+      vertices.push_back(cso);
+      R candidate_max_dist = dot(cso.point, face.normal);
+      if (candidate_max_dist < max_dist) { best_face_id = face_id; max_dist = candidate_max_dist; }
+      R curr_dist = -face_id.neg_dist;
+      if (max_dist - curr_dist < eps_tol) {
+        const EPAFace<R>& best = faces[best_face_id.id];
+        face_closest_points(best, p1, p2);
+        *n = best.normal;
+        return true;
+      }
+      faces[face_id.id].deleted = true;
+      uint32_t o1 = next_ccw_pt_id(faces[face.adj[0]], face.pts[0]);
+      uint32_t o2 = next_ccw_pt_id(faces[face.adj[1]], face.pts[1]);
+      uint32_t o3 = next_ccw_pt_id(faces[face.adj[2]], face.pts[2]);
+      compute_silhouette(support_point_id, face.adj[0], o1);
+      compute_silhouette(support_point_id, face.adj[1], o2);
+      compute_silhouette(support_point_id, face.adj[2], o3);
+      uint32_t first_new_face_id = (uint32_t)faces.size();
+      if (silhouette.empty()) return false;
+      for (size_t s = 0; s < silhouette.size(); ++s) {
+        Sil edge = silhouette[s];
+        if (!faces[edge.face_id].deleted) {
+          uint32_t new_face_id = (uint32_t)faces.size();
+          uint32_t pt_id1 = faces[edge.face_id].pts[(edge.opp_pt_id + 2) % 3];
+          uint32_t pt_id2 = faces[edge.face_id].pts[(edge.opp_pt_id + 1) % 3];
+          uint32_t pts[3] = {pt_id1, pt_id2, support_point_id};
+          uint32_t adj[3] = {edge.face_id, new_face_id + 1, new_face_id - 1};
+          EPAFace<R> nf;
+          bool inside = make_face(pts, adj, &nf);
+          faces[edge.face_id].adj[(edge.opp_pt_id + 1) % 3] = new_face_id;
+          faces.push_back(nf);
+          if (inside) {
+            V3<R> pt = vertices[faces[new_face_id].pts[0]].point;
+            R dist = dot(faces[new_face_id].normal, pt);
+            if (dist < curr_dist) {
+              face_closest_points(face, p1, p2);
+              *n = face.normal;
+              return true;
+            }
+            if (-dist > gjk_eps_tol<R>()) return false;
+            heap.push(EPAFaceId<R>{new_face_id, -dist});
+          }
+        }
+      }
+      if (first_new_face_id == faces.size()) return false;
+      faces[first_new_face_id].adj[2] = (uint32_t)faces.size() - 1;
+      faces.back().adj[1] = first_new_face_id;
+      silhouette.clear();
+      niter += 1;
+      if (niter > max_iter) return false;
+    }
+    const EPAFace<R>& best = faces[best_face_id.id];
+    face_closest_points(best, p1, p2);
+    *n = best.normal;
+    return true;
+  }
+};
+
+// ---------------------------------------------------------------- contact / distance dispatch
+template <class R> struct Contact {
+  V3<R> world1, world2, normal;
+  R depth;
+};
+
+// contact_support_map_support_map_with_params (contact_support_map_support_map.rs:38-88).
+// init_dir == nullptr is the reference's None.  Returns false for `NoIntersection`.
+template <class R> inline bool contact_support_map_support_map(const Iso<R>& m1, const Shape<R>& g1, const Iso<R>& m2,
+                                                               const Shape<R>& g2, R prediction, const V3<R>* init_dir,
+                                                               Contact<R>* out, int* gjk_iters = nullptr,
+                                                               int* epa_iters = nullptr, int gjk_max_iter = 10000,
+                                                               int* status_out = nullptr) {
+  V3<R> dir;
+  if (init_dir) dir = *init_dir;
+  else if (!try_normalize(m2.t - m1.t, Eps<R>::eps(), &dir)) dir = V3<R>(1, 0, 0);
+  VoronoiSimplex<R> simplex;
+  simplex.reset(cso_from_shapes(m1, g1, m2, g2, dir));
+  GJKResult<R> r = gjk_closest_points(m1, g1, m2, g2, prediction, true, simplex, gjk_max_iter);
+  if (gjk_iters) *gjk_iters = r.iters;
+  if (epa_iters) *epa_iters = 0;
+  if (status_out) *status_out = r.status;
+  V3<R> w1, w2, n;
+  if (r.status == GJK_CLOSEST) { w1 = r.p1; w2 = r.p2; n = r.dir; }
+  else if (r.status == GJK_INTERSECTION) {
+    EPA<R> epa;
+    bool ok = epa.closest_points(m1, g1, m2, g2, simplex, &w1, &w2, &n);
+    if (epa_iters) *epa_iters = epa.iters;
+    if (!ok) return false;
+  } else {
+    return false;
+  }
+  // Contact::new_wo_depth (contact.rs:43-46)
+  out->world1 = w1; out->world2 = w2; out->normal = n;
+  out->depth = -dot(n, w2 - w1);
+  return true;
+}
+
+// contact_ball_ball (contact_ball_ball.rs:8-38)
+template <class R> inline bool contact_ball_ball(V3<R> c1, R r1, V3<R> c2, R r2, R prediction, Contact<R>* out) {
+  V3<R> delta = c2 - c1;
+  R d2 = norm_sq(delta);
+  R sum_radius = r1 + r2;
+  R sre = sum_radius + prediction;
+  if (d2 < sre * sre) {
+    V3<R> normal = (d2 != R(0)) ? normalize(delta) : V3<R>(1, 0, 0);
+    out->world1 = c1 + normal * r1;
+    out->world2 = c2 + normal * (-r2);
+    out->normal = normal;
+    out->depth = sum_radius - std::sqrt(d2);
+    return true;
+  }
+  return false;
+}
+
+// point_projection_on_support_map (point_support_map.rs:15-53), solid = false.
+template <class R> inline V3<R> project_point_support_map(const Iso<R>& m_in, const Shape<R>& g, V3<R> point, bool* inside) {
+  Iso<R> id = Iso<R>::identity();
+  Iso<R> m = m_in;
+  m.t = (-point) + m_in.t;
+  V3<R> dir;
+  if (!try_normalize(-m.t, Eps<R>::eps(), &dir)) dir = V3<R>(1, 0, 0);
+  Shape<R> origin{ORIGIN, 0, 0, 0, nullptr, 0};
+  VoronoiSimplex<R> simplex;
+  simplex.reset(cso_from_shapes(m, g, id, origin, dir));
+  GJKResult<R> r = gjk_closest_points(m, g, id, origin, Eps<R>::maxv(), true, simplex);
+  if (r.status == GJK_CLOSEST) { *inside = false; return r.p1 + point; }
+  // GJKResult::Intersection -> EPA
+  EPA<R> epa;
+  V3<R> p1, p2, n;
+  if (r.status == GJK_INTERSECTION && epa.closest_points(m, g, id, origin, simplex, &p1, &p2, &n)) {
+    *inside = true;
+    return p1 + point;
+  }
+  *inside = true;
+  return point;
+}
+
+// AABB::local_point_projection + project_point_with_feature (point_aabb.rs:9-135), cuboid h, solid=false.
+// feature: kind 0 face(id), 1 edge(id), 2 vertex(id), 3 unknown.
+template <class R> inline V3<R> project_point_cuboid(const Iso<R>& m, V3<R> h, V3<R> pt, bool* inside_out, int* fkind, int* fid) {
+  V3<R> mins = -h, maxs = h;  // Point::origin() + (-h)
+  mins = V3<R>(R(0) + mins.x, R(0) + mins.y, R(0) + mins.z);
+  maxs = V3<R>(R(0) + maxs.x, R(0) + maxs.y, R(0) + maxs.z);
+  V3<R> ls_pt = inv_xform_pt(m, pt);
+  V3<R> mins_pt = mins - ls_pt;
+  V3<R> pt_maxs = ls_pt - maxs;
+  V3<R> zero(0, 0, 0);
+  V3<R> shift = vmax(mins_pt, zero) - vmax(pt_maxs, zero);
+  bool inside = (shift.x == R(0) && shift.y == R(0) && shift.z == R(0));
+  V3<R> proj_ls;
+  if (!inside) {
+    proj_ls = ls_pt + shift;
+  } else {
+    R best = -Eps<R>::maxv();
+    bool is_mins = false;
+    int best_id = 0;
+    for (int i = 0; i < 3; ++i) {
+      R mins_pt_i = mins_pt[i], pt_maxs_i = pt_maxs[i];
+      if (mins_pt_i < pt_maxs_i) {
+        if (pt_maxs[i] > best) { best_id = i; is_mins = false; best = pt_maxs_i; }
+      } else if (mins_pt_i > best) { best_id = i; is_mins = true; best = mins_pt_i; }
+    }
+    shift = zero;
+    if (is_mins) shift[best_id] = best; else shift[best_id] = -best;
+    proj_ls = ls_pt + shift;
+  }
+  *inside_out = inside;
+  V3<R> proj = xform_pt(m, proj_ls);
+  // feature (point_aabb.rs:79-134)
+  int nzero = 0, last_zero = 0, last_not_zero = 0;
+  for (int i = 0; i < 3; ++i) { if (shift[i] == R(0)) { nzero++; last_zero = i; } else last_not_zero = i; }
+  V3<R> center = (mins + maxs) * R(0.5);
+  if (nzero == 3) {
+    *fkind = 3; *fid = 0;
+    for (int i = 0; i < 3; ++i) {
+      if (proj_ls[i] > maxs[i] - Eps<R>::eps()) { *fkind = 0; *fid = i; break; }
+      if (proj_ls[i] <= mins[i] + Eps<R>::eps()) { *fkind = 0; *fid = i + 3; break; }
+    }
+  } else if (nzero == 2) {
+    *fkind = 0;
+    *fid = (proj_ls[last_not_zero] < center[last_not_zero]) ? last_not_zero + 3 : last_not_zero;
+  } else {
+    int id = 0;
+    for (int i = 0; i < 3; ++i) if (proj_ls[i] < center[i]) id |= 1 << i;
+    if (nzero == 0) { *fkind = 2; *fid = id; } else { *fkind = 1; *fid = (id << 2) | last_zero; }
+  }
+  return proj;
+}
+
+// Cuboid::feature_normal (cuboid.rs:511-561), local frame (the reference does not rotate it).
+template <class R> inline V3<R> cuboid_feature_normal(int fkind, int fid) {
+  V3<R> d(0, 0, 0);
+  if (fkind == 0) {
+    if (fid < 3) d[fid] = R(1); else d[fid - 3] = R(-1);
+    return d;
+  } else if (fkind == 1) {
+    int edge = fid & 3, face1 = (edge + 1) % 3, face2 = (edge + 2) % 3, signs = fid >> 2;
+    d[face1] = (signs & (1 << face1)) ? R(-1) : R(1);
+    d[face2] = (signs & (1 << face2)) ? R(-1) : R(1);
+    return normalize(d);
+  } else {
+    for (int i = 0; i < 3; ++i) d[i] = (fid & (1 << i)) ? R(-1) : R(1);
+    return normalize(d);
+  }
+}
+
+// contact_ball_convex_polyhedron (contact_ball_convex_polyhedron.rs:11-58) for Cuboid and ConvexHull.
+// For ConvexHull, the reference's zero-distance branch needs the hull's face/edge adjacency
+// (convex.rs:388-415); this restatement returns None there (ball centre within eps of the surface).
+template <class R> inline bool contact_ball_convex(V3<R> c1, R r, const Iso<R>& m2, const Shape<R>& g2, R prediction, Contact<R>* out) {
+  bool inside;
+  V3<R> world2;
+  int fkind = 3, fid = 0;
+  if (g2.type == CUBOID) world2 = project_point_cuboid(m2, V3<R>(g2.a, g2.b, g2.c), c1, &inside, &fkind, &fid);
+  else world2 = project_point_support_map(m2, g2, c1, &inside);
+  V3<R> dpt = world2 - c1;
+  V3<R> dir; R dist;
+  R depth; V3<R> normal;
+  if (try_normalize(dpt, Eps<R>::eps(), &dir, &dist)) {
+    if (inside) { depth = dist + r; normal = -dir; }
+    else { depth = -dist + r; normal = dir; }
+  } else {
+    if (g2.type != CUBOID || fkind == 3) return false;
+    depth = r;
+    normal = -cuboid_feature_normal<R>(fkind, fid);
+  }
+  if (depth >= -prediction) {
+    out->world1 = c1 + normal * r;
+    out->world2 = world2;
+    out->normal = normal;
+    out->depth = depth;
+    return true;
+  }
+  return false;
+}
+
+// query::contact (contact_shape_shape.rs:10-48) for {Ball, Cuboid, Capsule, ConvexHull}.
+template <class R> inline bool contact(const Iso<R>& m1, const Shape<R>& g1, const Iso<R>& m2, const Shape<R>& g2,
+                                       R prediction, Contact<R>* out, int* gjk_iters = nullptr, int* epa_iters = nullptr) {
+  if (gjk_iters) *gjk_iters = 0;
+  if (epa_iters) *epa_iters = 0;
+  bool b1 = g1.type == BALL, b2 = g2.type == BALL;
+  bool poly1 = g1.type == CUBOID || g1.type == HULL, poly2 = g2.type == CUBOID || g2.type == HULL;
+  if (b1 && b2) return contact_ball_ball(m1.t, g1.a, m2.t, g2.a, prediction, out);
+  if (b1 && poly2) return contact_ball_convex(m1.t, g1.a, m2, g2, prediction, out);
+  if (poly1 && b2) {
+    bool ok = contact_ball_convex(m2.t, g2.a, m1, g1, prediction, out);
+    if (ok) { std::swap(out->world1, out->world2); out->normal = -out->normal; }  // Contact::flip
+    return ok;
+  }
+  return contact_support_map_support_map(m1, g1, m2, g2, prediction, (const V3<R>*)nullptr, out, gjk_iters, epa_iters);
+}
+
+// query::distance (distance.rs:9-33)
+template <class R> inline R distance(const Iso<R>& m1, const Shape<R>& g1, const Iso<R>& m2, const Shape<R>& g2,
+                                     const V3<R>* init_dir = nullptr, int* gjk_iters = nullptr) {
+  if (gjk_iters) *gjk_iters = 0;
+  if (g1.type == BALL && g2.type == BALL) {
+    // distance_ball_ball.rs:7-24
+    V3<R> delta = m2.t - m1.t;
+    R d2 = norm_sq(delta);
+    R s = g1.a + g2.a;
+    if (d2 <= s * s) return R(0);
+    return std::sqrt(d2) - s;
+  }
+  // distance_support_map_support_map_with_params (distance_support_map_support_map.rs:25-53)
+  V3<R> d0 = init_dir ? *init_dir : (m1.t - m2.t);
+  V3<R> dir;
+  VoronoiSimplex<R> simplex;
+  if (try_normalize(d0, Eps<R>::eps(), &dir)) simplex.reset(cso_from_shapes(m1, g1, m2, g2, dir));
+  else simplex.reset(cso_from_shapes(m1, g1, m2, g2, V3<R>(1, 0, 0)));
+  GJKResult<R> r = gjk_closest_points(m1, g1, m2, g2, Eps<R>::maxv(), true, simplex);
+  if (gjk_iters) *gjk_iters = r.iters;
+  if (r.status == GJK_CLOSEST) return norm(r.p1 - r.p2);
+  return R(0);
+}
+
+// ---------------------------------------------------------------- AABBs
+template <class R> struct AABB { V3<R> mins, maxs; };
+
+template <class R> inline bool aabb_intersects(const AABB<R>& a, const AABB<R>& b) {  // aabb.rs:156-158
+  return a.mins.x <= b.maxs.x && a.mins.y <= b.maxs.y && a.mins.z <= b.maxs.z &&
+         a.maxs.x >= b.mins.x && a.maxs.y >= b.mins.y && a.maxs.z >= b.mins.z;
+}
+template <class R> inline bool aabb_contains(const AABB<R>& a, const AABB<R>& b) {  // aabb.rs:161-163
+  return a.mins.x <= b.mins.x && a.mins.y <= b.mins.y && a.mins.z <= b.mins.z &&
+         a.maxs.x >= b.maxs.x && a.maxs.y >= b.maxs.y && a.maxs.z >= b.maxs.z;
+}
+template <class R> inline AABB<R> aabb_merged(const AABB<R>& a, const AABB<R>& b) { return AABB<R>{vmin(a.mins, b.mins), vmax(a.maxs, b.maxs)}; }
+template <class R> inline AABB<R> aabb_loosened(const AABB<R>& a, R m) {
+  return AABB<R>{a.mins + V3<R>(-m, -m, -m), a.maxs + V3<R>(m, m, m)};
+}
+template <class R> inline V3<R> aabb_center(const AABB<R>& a) { return (a.mins + a.maxs) * R(0.5); }
+
+// Shape::aabb for the four shapes (aabb_ball.rs:8-13, aabb_cuboid.rs:7-14, aabb_support_map.rs:35-45
+// -> aabb_utils.rs:9-31, aabb_convex.rs:7-11 -> aabb_utils.rs:59-79).
+template <class R> inline AABB<R> shape_aabb(const Shape<R>& g, const Iso<R>& m) {
+  AABB<R> bb;
+  switch (g.type) {
+    case BALL:
+      bb.mins = m.t + V3<R>(-g.a, -g.a, -g.a);
+      bb.maxs = m.t + V3<R>(g.a, g.a, g.a);
+      return bb;
+    case CUBOID: {
+      R M[3][3];
+      quat_to_mat(m.q, M);
+      V3<R> h(g.a, g.b, g.c), ws;
+      for (int r = 0; r < 3; ++r) ws[r] = (std::fabs(M[r][0]) * h.x + std::fabs(M[r][1]) * h.y) + std::fabs(M[r][2]) * h.z;
+      bb.mins = m.t - ws;
+      bb.maxs = m.t + ws;
+      return bb;
+    }
+    case CAPSULE: {
+      V3<R> mn, mx;
+      for (int d = 0; d < 3; ++d) {
+        V3<R> basis(0, 0, 0);
+        basis[d] = R(1);
+        mx[d] = support_point(g, m, basis)[d];
+        basis[d] = R(-1);
+        mn[d] = support_point(g, m, basis)[d];
+      }
+      bb.mins = mn; bb.maxs = mx;
+      return bb;
+    }
+    case HULL: {
+      V3<R> w0 = xform_pt(m, g.pts[0]);
+      V3<R> mn = w0, mx = w0;
+      for (uint32_t i = 1; i < g.npts; ++i) { V3<R> w = xform_pt(m, g.pts[i]); mn = vmin(mn, w); mx = vmax(mx, w); }
+      bb.mins = mn; bb.maxs = mx;
+      return bb;
+    }
+  }
+  bb.mins = bb.maxs = m.t;
+  return bb;
+}
+
+// ---------------------------------------------------------------- ray casting
+template <class R> struct Ray { V3<R> origin, dir; };
+
+// AABB::toi_with_ray (ray_aabb.rs:13-50), identity isometry.
+template <class R> inline bool ray_aabb_toi(const AABB<R>& bb, const Ray<R>& ray, R max_toi, bool solid, R* toi) {
+  R tmin = R(0), tmax = max_toi;
+  for (int i = 0; i < 3; ++i) {
+    if (ray.dir[i] == R(0)) {
+      if (ray.origin[i] < bb.mins[i] || ray.origin[i] > bb.maxs[i]) return false;
+    } else {
+      R denom = R(1) / ray.dir[i];
+      R tn = (bb.mins[i] - ray.origin[i]) * denom;
+      R tf = (bb.maxs[i] - ray.origin[i]) * denom;
+      if (tn > tf) std::swap(tn, tf);
+      tmin = std::fmax(tmin, tn);
+      tmax = std::fmin(tmax, tf);
+      if (tmin > tmax) return false;
+    }
+  }
+  *toi = (tmin == R(0) && !solid) ? tmax : tmin;
+  return true;
+}
+
+// ray_intersection_with_triangle (ray_triangle.rs:32-114); fid 0 = front, 1 = back.
+template <class R> inline bool ray_triangle(V3<R> a, V3<R> b, V3<R> c, const Ray<R>& ray, R* toi_out, V3<R>* n_out, int* fid_out) {
+  V3<R> ab = b - a, ac = c - a;
+  V3<R> n = cross(ab, ac);
+  R d = dot(n, ray.dir);
+  if (d == R(0)) return false;
+  V3<R> ap = ray.origin - a;
+  R t = dot(ap, n);
+  if ((t < R(0) && d < R(0)) || (t > R(0) && d > R(0))) return false;
+  int fid = d < R(0) ? 0 : 1;
+  d = std::fabs(d);
+  V3<R> e = -cross(ray.dir, ap);
+  R v, w, toi;
+  V3<R> normal;
+  if (t < R(0)) {
+    v = -dot(ac, e);
+    if (v < R(0) || v > d) return false;
+    w = dot(ab, e);
+    if (w < R(0) || v + w > d) return false;
+    R invd = R(1) / d;
+    toi = -t * invd;
+    normal = -normalize(n);
+  } else {
+    v = dot(ac, e);
+    if (v < R(0) || v > d) return false;
+    w = -dot(ab, e);
+    if (w < R(0) || v + w > d) return false;
+    R invd = R(1) / d;
+    toi = t * invd;
+    normal = normalize(n);
+  }
+  *toi_out = toi; *n_out = normal; *fid_out = fid;
+  return true;
+}
+
+// ---------------------------------------------------------------- BVT (partitioning/bvt.rs)
+template <class R> struct BVT {
+  struct Node { bool leaf; uint32_t idx; };
+  struct Internal { AABB<R> bv; Node left, right; };
+  struct Leaf { AABB<R> bv; uint32_t data; };
+  std::vector<Internal> internals;
+  std::vector<Leaf> leaves;
+  Node root{true, 0};
+
+  // utils/median.rs:5-17
+  static R median(std::vector<R>& v) {
+    std::sort(v.begin(), v.end());
+    size_t n = v.size();
+    if (n % 2 == 0) return (v[n / 2 - 1] + v[n / 2]) / R(2);
+    return v[n / 2];
+  }
+  // median_partitioning_with_centers (bvt.rs:290-350) + _from_partitioning (:352-404)
+  Node build(size_t depth, std::vector<std::pair<uint32_t, AABB<R>>>&& elts) {
+    if (elts.size() == 1) {
+      leaves.push_back(Leaf{elts[0].second, elts[0].first});
+      return Node{true, (uint32_t)leaves.size() - 1};
+    }
+    int axis = (int)(depth % 3);
+    std::vector<R> med;
+    med.reserve(elts.size());
+    for (auto& l : elts) med.push_back(aabb_center(l.second)[axis]);
+    R m = median(med);
+    std::vector<std::pair<uint32_t, AABB<R>>> left, right;
+    AABB<R> bb = elts[0].second;
+    bool insert_left = false;
+    for (auto& l : elts) {
+      bb = aabb_merged(bb, l.second);
+      R pos = aabb_center(l.second)[axis];
+      if (pos < m || (pos == m && insert_left)) { left.push_back(l); insert_left = false; }
+      else { right.push_back(l); insert_left = true; }
+    }
+    if (left.empty()) { left.push_back(right.back()); right.pop_back(); }
+    else if (right.empty()) { right.push_back(left.back()); left.pop_back(); }
+    std::vector<std::pair<uint32_t, AABB<R>>>().swap(elts);
+    Node l = build(depth + 1, std::move(left));
+    Node r = build(depth + 1, std::move(right));
+    internals.push_back(Internal{bb, l, r});
+    return Node{false, (uint32_t)internals.size() - 1};
+  }
+  void build_balanced(std::vector<std::pair<uint32_t, AABB<R>>>&& elts) {
+    internals.clear(); leaves.clear();
+    if (elts.empty()) return;
+    root = build(0, std::move(elts));
+  }
+  const AABB<R>& bv(Node n) const { return n.leaf ? leaves[n.idx].bv : internals[n.idx].bv; }
+};
+
+// TriMesh: points + indices, BVT over local triangle AABBs (trimesh.rs:100-197).
+template <class R> struct TriMesh {
+  std::vector<V3<R>> points;
+  std::vector<uint32_t> idx;  // 3 per face
+  BVT<R> bvt;
+  void build() {
+    size_t nf = idx.size() / 3;
+    std::vector<std::pair<uint32_t, AABB<R>>> leaves;
+    leaves.reserve(nf);
+    for (size_t i = 0; i < nf; ++i) {
+      V3<R> a = points[idx[3 * i]], b = points[idx[3 * i + 1]], c = points[idx[3 * i + 2]];
+      AABB<R> bb;  // aabb_triangle.rs:27-41
+      for (int d = 0; d < 3; ++d) {
+        bb.mins[d] = std::fmin(std::fmin(a[d], b[d]), c[d]);
+        bb.maxs[d] = std::fmax(std::fmax(a[d], b[d]), c[d]);
+      }
+      leaves.push_back({(uint32_t)i, bb});
+    }
+    bvt.build_balanced(std::move(leaves));
+  }
+  struct Hit { R toi; V3<R> normal; uint32_t feature; };
+  struct Weighted { typename BVT<R>::Node node; R cost; };
+  struct WLE { bool operator()(const Weighted& a, const Weighted& b) const { return a.cost <= b.cost; } };
+
+  // TriMesh::toi_and_normal_with_ray (ray_trimesh.rs:23-50) -> BVH::best_first_search (bvh.rs:101-160)
+  // with TriMeshRayToiAndNormalVisitor (ray_trimesh.rs:150-191). Mesh at `m`.
+  bool cast(const Iso<R>& m, const Ray<R>& wray, R max_toi, Hit* hit, uint32_t* nodes_visited = nullptr) const {
+    if (bvt.leaves.empty()) return false;
+    Ray<R> ray{inv_xform_pt(m, wray.origin), inv_xform_vec(m, wray.dir)};
+    uint32_t nf = (uint32_t)(idx.size() / 3);
+    uint32_t visits = 0;
+    auto visit = [&](R best, typename BVT<R>::Node n, R* cost, bool* has_res, Hit* res) -> bool {
+      visits++;
+      R toi;
+      if (!ray_aabb_toi(bvt.bv(n), ray, max_toi, true, &toi)) return false;  // Stop
+      *cost = toi;
+      *has_res = false;
+      if (n.leaf && toi < best) {
+        uint32_t f = bvt.leaves[n.idx].data;
+        V3<R> a = points[idx[3 * f]], b = points[idx[3 * f + 1]], c = points[idx[3 * f + 2]];
+        R t; V3<R> nn; int fid;
+        if (ray_triangle(a, b, c, ray, &t, &nn, &fid) && t <= max_toi) {
+          *cost = t;
+          *has_res = true;
+          res->toi = t;
+          res->normal = nn;
+          res->feature = fid == 1 ? f + nf : f;
+        }
+      }
+      return true;
+    };
+    RustHeap<Weighted, WLE> queue;
+    R best_cost = Eps<R>::maxv();
+    bool found = false;
+    Hit best_res{};
+    R cost; bool has; Hit r;
+    if (!visit(best_cost, bvt.root, &cost, &has, &r)) { if (nodes_visited) *nodes_visited = visits; return false; }
+    if (has) { best_cost = cost; best_res = r; found = true; }
+    queue.push(Weighted{bvt.root, -cost});
+    Weighted e;
+    while (queue.pop(&e)) {
+      if (-e.cost >= best_cost) break;
+      if (e.node.leaf) continue;
+      const auto& in = bvt.internals[e.node.idx];
+      typename BVT<R>::Node ch[2] = {in.left, in.right};
+      for (int i = 0; i < 2; ++i) {
+        if (visit(best_cost, ch[i], &cost, &has, &r)) {
+          if (cost < best_cost) {
+            if (has) { best_cost = cost; best_res = r; found = true; }
+            queue.push(Weighted{ch[i], -cost});
+          }
+        }
+      }
+    }
+    if (nodes_visited) *nodes_visited = visits;
+    if (!found) return false;
+    best_res.normal = xform_vec(m, best_res.normal);
+    *hit = best_res;
+    return true;
+  }
+};
+
+// ---------------------------------------------------------------- slab allocator (slab crate: LIFO free list)
+template <class T> struct Slab {
+  struct Entry { bool occupied; T val; size_t next; };
+  std::vector<Entry> entries;
+  size_t next = 0, len = 0;
+  size_t insert(const T& v) {
+    size_t key = next;
+    if (key == entries.size()) { entries.push_back(Entry{true, v, 0}); next = key + 1; }
+    else { next = entries[key].next; entries[key].occupied = true; entries[key].val = v; }
+    len++;
+    return key;
+  }
+  T remove(size_t key) {
+    T v = entries[key].val;
+    entries[key].occupied = false;
+    entries[key].next = next;
+    next = key;
+    len--;
+    return v;
+  }
+  void clear() { entries.clear(); next = 0; len = 0; }
+  bool contains(size_t k) const { return k < entries.size() && entries[k].occupied; }
+  T& operator[](size_t k) { return entries[k].val; }
+  const T& operator[](size_t k) const { return entries[k].val; }
+};
+
+// ---------------------------------------------------------------- DBVT (partitioning/dbvt.rs)
+template <class R> struct DBVT {
+  enum PKind { ROOT, LEFT_OF, RIGHT_OF };
+  struct PId { PKind kind; size_t p; };
+  struct NId { bool leaf; size_t id; };
+  struct Leaf { AABB<R> bv; V3<R> center; uint32_t data; PId parent; };
+  struct Internal { AABB<R> bv; V3<R> center; NId left, right; PId parent; };
+  NId root{true, 0};
+  Slab<Leaf> leaves;
+  Slab<Internal> internals;
+
+  V3<R> center_of(NId n) const { return n.leaf ? leaves[n.id].center : internals[n.id].center; }
+  // dbvt.rs:158-255
+  size_t insert(AABB<R> bv, uint32_t data) {
+    Leaf leaf{bv, aabb_center(bv), data, PId{ROOT, 0}};
+    if (leaves.len == 0) {
+      size_t nid = leaves.insert(leaf);
+      leaves[nid].parent = PId{ROOT, 0};
+      root = NId{true, nid};
+      return nid;
+    }
+    if (!root.leaf) {
+      NId curr = root;
+      for (;;) {
+        if (!curr.leaf) {
+          Internal& node = internals[curr.id];
+          node.bv = aabb_merged(node.bv, leaf.bv);
+          NId l = node.left, r = node.right;
+          V3<R> d1v = center_of(l) - leaf.center;
+          V3<R> d2v = center_of(r) - leaf.center;
+          R d1 = norm_sq(d1v), d2 = norm_sq(d2v);
+          curr = d1 < d2 ? l : r;
+        } else {
+          size_t id = curr.id;
+          AABB<R> pbv = aabb_merged(leaves[id].bv, leaf.bv);
+          PId gp = leaves[id].parent;
+          size_t nid = leaves.insert(leaf);
+          Internal par{pbv, aabb_center(pbv), curr, NId{true, nid}, gp};
+          size_t pid = internals.insert(par);
+          leaves[id].parent = PId{LEFT_OF, pid};
+          leaves[nid].parent = PId{RIGHT_OF, pid};
+          if (gp.kind == LEFT_OF) internals[gp.p].left = NId{false, pid};
+          else internals[gp.p].right = NId{false, pid};
+          return nid;
+        }
+      }
+    } else {
+      size_t id = root.id;
+      size_t nid = leaves.insert(leaf);
+      AABB<R> rbv = aabb_merged(leaves[id].bv, leaves[nid].bv);
+      Internal rt{rbv, aabb_center(rbv), NId{true, id}, NId{true, nid}, PId{ROOT, 0}};
+      size_t rid = internals.insert(rt);
+      leaves[id].parent = PId{LEFT_OF, rid};
+      leaves[nid].parent = PId{RIGHT_OF, rid};
+      root = NId{false, rid};
+      return nid;
+    }
+  }
+  // dbvt.rs:260-328
+  Leaf remove(size_t leaf_id) {
+    Leaf leaf = leaves.remove(leaf_id);
+    if (leaf.parent.kind != ROOT) {
+      size_t p = leaf.parent.p;
+      NId other = leaf.parent.kind == RIGHT_OF ? internals[p].left : internals[p].right;
+      PId pp = internals[p].parent;
+      if (pp.kind == ROOT) {
+        if (other.leaf) leaves[other.id].parent = PId{ROOT, 0}; else internals[other.id].parent = PId{ROOT, 0};
+        root = other;
+      } else {
+        if (other.leaf) leaves[other.id].parent = PId{pp.kind, pp.p}; else internals[other.id].parent = PId{pp.kind, pp.p};
+        if (pp.kind == RIGHT_OF) internals[pp.p].right = other; else internals[pp.p].left = other;
+      }
+      internals.remove(p);
+    } else {
+      leaves.clear();
+      internals.clear();
+    }
+    return leaf;
+  }
+  // BVH::visit (bvh.rs:24-45) + BoundingVolumeInterferencesCollector (bounding_volume_interferences_collector.rs:41-51)
+  void collect(const AABB<R>& q, std::vector<uint32_t>& out) const {
+    if (leaves.len == 0) return;
+    std::vector<NId> stack;
+    stack.push_back(root);
+    while (!stack.empty()) {
+      NId n = stack.back();
+      stack.pop_back();
+      const AABB<R>& bv = n.leaf ? leaves[n.id].bv : internals[n.id].bv;
+      if (aabb_intersects(bv, q)) {
+        if (n.leaf) out.push_back(leaves[n.id].data);
+        else { stack.push_back(internals[n.id].left); stack.push_back(internals[n.id].right); }
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------- DBVTBroadPhase (pipeline/broad_phase/dbvt_broad_phase.rs)
+template <class R> struct DBVTBroadPhase {
+  enum Status { DETACHED_NONE, DETACHED_SOME, ON_DYNAMIC, ON_STATIC, DELETED };
+  struct Proxy { uint32_t data; Status status; size_t leaf; size_t energy; bool updated; };
+  static const size_t DEACTIVATION_THRESHOLD = 100;
+  Slab<Proxy> proxies;
+  DBVT<R> tree, stree;
+  // pair set keyed by (min, max) handle; value = up_to_date flag
+  std::vector<std::pair<uint64_t, bool>> pairs;  // kept unordered like a HashMap; order is not part of the contract
+  R margin;
+  struct PendingLeaf { AABB<R> bv; uint32_t handle; bool valid; };
+  std::vector<PendingLeaf> leaves_to_update;
+  std::vector<std::pair<uint32_t, AABB<R>>> proxies_to_update;  // VecDeque (push_back order)
+  std::function<bool(uint32_t, uint32_t)> allowed;
+  std::vector<uint64_t> started, stopped;
+
+  explicit DBVTBroadPhase(R m) : margin(m) {}
+  static uint64_t key(uint32_t a, uint32_t b) { return a < b ? ((uint64_t)a << 32) | b : ((uint64_t)b << 32) | a; }
+  size_t find_pair(uint64_t k) const {
+    for (size_t i = 0; i < pairs.size(); ++i) if (pairs[i].first == k) return i;
+    return (size_t)-1;
+  }
+
+  uint32_t create_proxy(const AABB<R>& bv) {  // :275-280 (stores the box as given)
+    Proxy p{0, DETACHED_NONE, 0, 0, true};
+    uint32_t h = (uint32_t)proxies.insert(p);
+    proxies[h].data = h;
+    proxies_to_update.push_back({h, bv});
+    return h;
+  }
+  void deferred_set_bounding_volume(uint32_t h, const AABB<R>& bv) {  // :325-347
+    Proxy& p = proxies[h];
+    bool needs;
+    if (p.status == ON_STATIC) needs = !aabb_contains(stree.leaves[p.leaf].bv, bv);
+    else if (p.status == ON_DYNAMIC) needs = !aabb_contains(tree.leaves[p.leaf].bv, bv);
+    else needs = true;
+    if (needs) proxies_to_update.push_back({h, aabb_loosened(bv, margin)});
+  }
+  const AABB<R>* stored_bv(uint32_t h) const {
+    const Proxy& p = proxies[h];
+    if (p.status == ON_STATIC) return &stree.leaves[p.leaf].bv;
+    if (p.status == ON_DYNAMIC) return &tree.leaves[p.leaf].bv;
+    return nullptr;
+  }
+  void remove(const std::vector<uint32_t>& hs) {  // :282-323
+    for (uint32_t h : hs) {
+      Proxy& p = proxies[h];
+      if (p.status == ON_STATIC) stree.remove(p.leaf);
+      else if (p.status == ON_DYNAMIC) tree.remove(p.leaf);
+      p.status = DELETED;
+    }
+    std::vector<std::pair<uint64_t, bool>> kept;
+    for (auto& pr : pairs) {
+      uint32_t a = (uint32_t)(pr.first >> 32), b = (uint32_t)pr.first;
+      if (proxies[a].status == DELETED || proxies[b].status == DELETED) stopped.push_back(pr.first);
+      else kept.push_back(pr);
+    }
+    pairs.swap(kept);
+    for (uint32_t h : hs) proxies.remove(h);
+  }
+  void update() {  // :174-259
+    started.clear();
+    for (auto& pu : proxies_to_update) {
+      uint32_t h = pu.first;
+      if (!proxies.contains(h)) continue;
+      Proxy& p = proxies[h];
+      bool set_status = true;
+      if (p.status == ON_STATIC) { stree.remove(p.leaf); leaves_to_update.push_back({pu.second, h, true}); }
+      else if (p.status == ON_DYNAMIC) { tree.remove(p.leaf); leaves_to_update.push_back({pu.second, h, true}); }
+      else if (p.status == DETACHED_NONE) { leaves_to_update.push_back({pu.second, h, true}); }
+      else if (p.status == DETACHED_SOME) { leaves_to_update[p.leaf] = {pu.second, h, true}; set_status = false; }
+      p.updated = true;
+      if (set_status) { p.status = DETACHED_SOME; p.leaf = leaves_to_update.size() - 1; }
+    }
+    proxies_to_update.clear();
+    bool some = !leaves_to_update.empty();
+    std::vector<uint32_t> collector;
+    for (auto& lf : leaves_to_update) {
+      collector.clear();
+      tree.collect(lf.bv, collector);
+      stree.collect(lf.bv, collector);
+      for (uint32_t h2 : collector) {
+        if (allowed && !allowed(lf.handle, h2)) continue;
+        uint64_t k = key(lf.handle, h2);
+        size_t at = find_pair(k);
+        if (at != (size_t)-1) pairs[at].second = true;
+        else { started.push_back(k); pairs.push_back({k, true}); }
+      }
+      size_t leaf = tree.insert(lf.bv, lf.handle);
+      proxies[lf.handle].status = ON_DYNAMIC;
+      proxies[lf.handle].leaf = leaf;
+      proxies[lf.handle].energy = DEACTIVATION_THRESHOLD;
+    }
+    leaves_to_update.clear();
+    if (some) {  // purge_some_contact_pairs (:101-147), purge_all = false
+      std::vector<std::pair<uint64_t, bool>> kept;
+      for (auto& pr : pairs) {
+        bool retain = true;
+        if (!pr.second) {
+          uint32_t a = (uint32_t)(pr.first >> 32), b = (uint32_t)pr.first;
+          if (proxies[a].updated || proxies[b].updated) {
+            if (!allowed || allowed(a, b)) {
+              if (!aabb_intersects(*stored_bv(a), *stored_bv(b))) { stopped.push_back(pr.first); retain = false; }
+            } else { stopped.push_back(pr.first); retain = false; }
+          }
+        }
+        pr.second = false;
+        if (retain) kept.push_back(pr);
+      }
+      pairs.swap(kept);
+    }
+    // update_activation_states (:149-165)
+    for (size_t h = 0; h < proxies.entries.size(); ++h) {
+      if (!proxies.entries[h].occupied) continue;
+      Proxy& p = proxies.entries[h].val;
+      if (p.status == ON_DYNAMIC) {
+        if (p.energy == 1) {
+          typename DBVT<R>::Leaf old = tree.remove(p.leaf);
+          size_t nl = stree.insert(old.bv, old.data);
+          p.status = ON_STATIC; p.leaf = nl;
+        } else p.energy -= 1;
+      }
+    }
+  }
+};
+
+}  // namespace ncxo

@@ -1,0 +1,377 @@
+// oracle_capi.cpp -- extern "C" surface of the CPU oracle (ncx_oracle.hpp).
+//
+// TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py as the checker; never by the product library.
+//
+// Layouts (all host memory, caller-owned):
+//   shapes : types[n] (0 ball, 1 cuboid, 2 capsule, 3 hull), params[3n],
+//            hull_off[n], hull_cnt[n] index into hull_pts[3 * total] (xyz AoS)
+//   poses  : 7 per shape, (tx, ty, tz, qi, qj, qk, qw)
+//   pairs  : ia[np], ib[np] shape indices
+#include <thread>
+#include <vector>
+#include <algorithm>
+#include "ncx_oracle.hpp"
+
+using namespace ncxo;
+
+namespace {
+
+template <class R> struct ShapeSet {
+  std::vector<Shape<R>> shapes;
+  std::vector<V3<R>> pts;
+  void load(int64_t n, const uint32_t* types, const R* params, const uint32_t* hoff, const uint32_t* hcnt,
+            const R* hpts, int64_t npts_total) {
+    pts.resize((size_t)npts_total);
+    for (int64_t i = 0; i < npts_total; ++i) pts[i] = V3<R>(hpts[3 * i], hpts[3 * i + 1], hpts[3 * i + 2]);
+    shapes.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+      Shape<R>& s = shapes[i];
+      s.type = types[i];
+      s.a = params[3 * i]; s.b = params[3 * i + 1]; s.c = params[3 * i + 2];
+      s.pts = nullptr; s.npts = 0;
+      if (s.type == HULL) { s.pts = pts.data() + hoff[i]; s.npts = hcnt[i]; }
+    }
+  }
+};
+
+template <class R> inline Iso<R> pose_at(const R* poses, int64_t i) {
+  Iso<R> m;
+  const R* p = poses + 7 * i;
+  m.t = V3<R>(p[0], p[1], p[2]);
+  m.q = Quat<R>{p[3], p[4], p[5], p[6]};
+  return m;
+}
+
+template <class F> void parallel_for(int64_t n, int nthreads, F f) {
+  if (nthreads <= 1 || n < 2) { for (int64_t i = 0; i < n; ++i) f(i); return; }
+  std::vector<std::thread> th;
+  int64_t chunk = (n + nthreads - 1) / nthreads;
+  for (int t = 0; t < nthreads; ++t) {
+    int64_t b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([=]() { for (int64_t i = b; i < e; ++i) f(i); });
+  }
+  for (auto& t : th) t.join();
+}
+
+template <class R>
+int contact_batch(int64_t ns, const uint32_t* types, const R* params, const uint32_t* hoff, const uint32_t* hcnt,
+                  const R* hpts, int64_t npts_total, const R* poses, const uint32_t* ia, const uint32_t* ib, int64_t np,
+                  R prediction, uint8_t* has, R* w1, R* w2, R* nrm, R* depth, int32_t* gjk_it, int32_t* epa_it,
+                  int nthreads) {
+  ShapeSet<R> ss;
+  ss.load(ns, types, params, hoff, hcnt, hpts, npts_total);
+  parallel_for(np, nthreads, [&](int64_t k) {
+    uint32_t a = ia[k], b = ib[k];
+    Contact<R> c;
+    int gi = 0, ei = 0;
+    bool ok = contact(pose_at(poses, a), ss.shapes[a], pose_at(poses, b), ss.shapes[b], prediction, &c, &gi, &ei);
+    has[k] = ok ? 1 : 0;
+    if (gjk_it) gjk_it[k] = gi;
+    if (epa_it) epa_it[k] = ei;
+    if (ok) {
+      for (int d = 0; d < 3; ++d) { w1[3 * k + d] = c.world1[d]; w2[3 * k + d] = c.world2[d]; nrm[3 * k + d] = c.normal[d]; }
+      depth[k] = c.depth;
+    } else {
+      for (int d = 0; d < 3; ++d) { w1[3 * k + d] = 0; w2[3 * k + d] = 0; nrm[3 * k + d] = 0; }
+      depth[k] = 0;
+    }
+  });
+  return 0;
+}
+
+template <class R>
+int contact_sm_batch(int64_t ns, const uint32_t* types, const R* params, const uint32_t* hoff, const uint32_t* hcnt,
+                     const R* hpts, int64_t npts_total, const R* poses, const uint32_t* ia, const uint32_t* ib, int64_t np,
+                     R prediction, const R* init_dirs, int32_t gjk_max_iter, uint8_t* has, R* w1, R* w2, R* nrm,
+                     R* depth, int32_t* gjk_it, int32_t* epa_it, int32_t* status, int nthreads) {
+  ShapeSet<R> ss;
+  ss.load(ns, types, params, hoff, hcnt, hpts, npts_total);
+  parallel_for(np, nthreads, [&](int64_t k) {
+    uint32_t a = ia[k], b = ib[k];
+    Contact<R> c;
+    int gi = 0, ei = 0, st = 0;
+    V3<R> d0;
+    const V3<R>* dptr = nullptr;
+    if (init_dirs) { d0 = V3<R>(init_dirs[3 * k], init_dirs[3 * k + 1], init_dirs[3 * k + 2]); dptr = &d0; }
+    bool ok = contact_support_map_support_map(pose_at(poses, a), ss.shapes[a], pose_at(poses, b), ss.shapes[b],
+                                              prediction, dptr, &c, &gi, &ei, gjk_max_iter, &st);
+    has[k] = ok ? 1 : 0;
+    gjk_it[k] = gi; epa_it[k] = ei; status[k] = st;
+    for (int d = 0; d < 3; ++d) {
+      w1[3 * k + d] = ok ? c.world1[d] : R(0);
+      w2[3 * k + d] = ok ? c.world2[d] : R(0);
+      nrm[3 * k + d] = ok ? c.normal[d] : R(0);
+    }
+    depth[k] = ok ? c.depth : R(0);
+  });
+  return 0;
+}
+
+template <class R>
+int distance_batch(int64_t ns, const uint32_t* types, const R* params, const uint32_t* hoff, const uint32_t* hcnt,
+                   const R* hpts, int64_t npts_total, const R* poses, const uint32_t* ia, const uint32_t* ib, int64_t np,
+                   R* dist, int32_t* gjk_it, int nthreads) {
+  ShapeSet<R> ss;
+  ss.load(ns, types, params, hoff, hcnt, hpts, npts_total);
+  parallel_for(np, nthreads, [&](int64_t k) {
+    uint32_t a = ia[k], b = ib[k];
+    int gi = 0;
+    dist[k] = distance(pose_at(poses, a), ss.shapes[a], pose_at(poses, b), ss.shapes[b], (const V3<R>*)nullptr, &gi);
+    if (gjk_it) gjk_it[k] = gi;
+  });
+  return 0;
+}
+
+template <class R>
+int shape_aabbs(int64_t ns, const uint32_t* types, const R* params, const uint32_t* hoff, const uint32_t* hcnt,
+                const R* hpts, int64_t npts_total, const R* poses, R* out6) {
+  ShapeSet<R> ss;
+  ss.load(ns, types, params, hoff, hcnt, hpts, npts_total);
+  for (int64_t i = 0; i < ns; ++i) {
+    AABB<R> bb = shape_aabb(ss.shapes[i], pose_at(poses, i));
+    out6[6 * i + 0] = bb.mins.x; out6[6 * i + 1] = bb.mins.y; out6[6 * i + 2] = bb.mins.z;
+    out6[6 * i + 3] = bb.maxs.x; out6[6 * i + 4] = bb.maxs.y; out6[6 * i + 5] = bb.maxs.z;
+  }
+  return 0;
+}
+
+template <class R> struct MeshHandle { TriMesh<R> mesh; };
+
+template <class R> void* trimesh_create(const R* pts, int64_t npts, const uint32_t* idx, int64_t ntri) {
+  auto* h = new MeshHandle<R>();
+  h->mesh.points.resize((size_t)npts);
+  for (int64_t i = 0; i < npts; ++i) h->mesh.points[i] = V3<R>(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+  h->mesh.idx.assign(idx, idx + 3 * ntri);
+  h->mesh.build();
+  return h;
+}
+
+template <class R>
+int trimesh_cast(void* hp, const R* pose7, const R* orig, const R* dir, int64_t n, R max_toi, R* toi, R* nrm,
+                 uint32_t* feature, uint32_t* visits, int nthreads) {
+  auto* h = (MeshHandle<R>*)hp;
+  Iso<R> m = pose_at(pose7, 0);
+  parallel_for(n, nthreads, [&](int64_t k) {
+    Ray<R> ray{V3<R>(orig[3 * k], orig[3 * k + 1], orig[3 * k + 2]), V3<R>(dir[3 * k], dir[3 * k + 1], dir[3 * k + 2])};
+    typename TriMesh<R>::Hit hit;
+    uint32_t v = 0;
+    bool ok = h->mesh.cast(m, ray, max_toi, &hit, &v);
+    if (visits) visits[k] = v;
+    if (ok) {
+      toi[k] = hit.toi; feature[k] = hit.feature;
+      for (int d = 0; d < 3; ++d) nrm[3 * k + d] = hit.normal[d];
+    } else {
+      toi[k] = -1; feature[k] = 0xffffffffu;
+      for (int d = 0; d < 3; ++d) nrm[3 * k + d] = 0;
+    }
+  });
+  return 0;
+}
+
+// Brute force over every triangle: (toi, face id) lexicographic minimum.
+template <class R>
+int trimesh_brute(const R* pts, const uint32_t* idx, int64_t ntri, const R* pose7, const R* orig, const R* dir,
+                  int64_t n, R max_toi, R* toi, R* nrm, uint32_t* feature, uint32_t* nhits, int nthreads) {
+  Iso<R> m = pose_at(pose7, 0);
+  parallel_for(n, nthreads, [&](int64_t k) {
+    Ray<R> wr{V3<R>(orig[3 * k], orig[3 * k + 1], orig[3 * k + 2]), V3<R>(dir[3 * k], dir[3 * k + 1], dir[3 * k + 2])};
+    Ray<R> ray{inv_xform_pt(m, wr.origin), inv_xform_vec(m, wr.dir)};
+    bool found = false;
+    R bt = 0; V3<R> bn; uint32_t bf = 0; uint32_t cnt = 0;
+    for (int64_t f = 0; f < ntri; ++f) {
+      const R* a = pts + 3 * idx[3 * f];
+      const R* b = pts + 3 * idx[3 * f + 1];
+      const R* c = pts + 3 * idx[3 * f + 2];
+      R t; V3<R> nn; int fid;
+      if (ray_triangle(V3<R>(a[0], a[1], a[2]), V3<R>(b[0], b[1], b[2]), V3<R>(c[0], c[1], c[2]), ray, &t, &nn, &fid) && t <= max_toi) {
+        cnt++;
+        if (!found || t < bt) { found = true; bt = t; bn = nn; bf = fid == 1 ? (uint32_t)(f + ntri) : (uint32_t)f; }
+      }
+    }
+    if (nhits) nhits[k] = cnt;
+    if (found) {
+      bn = xform_vec(m, bn);
+      toi[k] = bt; feature[k] = bf;
+      for (int d = 0; d < 3; ++d) nrm[3 * k + d] = bn[d];
+    } else {
+      toi[k] = -1; feature[k] = 0xffffffffu;
+      for (int d = 0; d < 3; ++d) nrm[3 * k + d] = 0;
+    }
+  });
+  return 0;
+}
+
+template <class R> inline AABB<R> aabb_from6(const R* b) {
+  return AABB<R>{V3<R>(b[0], b[1], b[2]), V3<R>(b[3], b[4], b[5])};
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_contact_f32(int64_t ns, const uint32_t* types, const float* params, const uint32_t* hoff, const uint32_t* hcnt,
+                    const float* hpts, int64_t npts, const float* poses, const uint32_t* ia, const uint32_t* ib,
+                    int64_t np, float prediction, uint8_t* has, float* w1, float* w2, float* n, float* depth,
+                    int32_t* gi, int32_t* ei, int nthreads) {
+  return contact_batch<float>(ns, types, params, hoff, hcnt, hpts, npts, poses, ia, ib, np, prediction, has, w1, w2, n,
+                              depth, gi, ei, nthreads);
+}
+int orc_contact_f64(int64_t ns, const uint32_t* types, const double* params, const uint32_t* hoff,
+                    const uint32_t* hcnt, const double* hpts, int64_t npts, const double* poses, const uint32_t* ia,
+                    const uint32_t* ib, int64_t np, double prediction, uint8_t* has, double* w1, double* w2,
+                    double* n, double* depth, int32_t* gi, int32_t* ei, int nthreads) {
+  return contact_batch<double>(ns, types, params, hoff, hcnt, hpts, npts, poses, ia, ib, np, prediction, has, w1, w2,
+                               n, depth, gi, ei, nthreads);
+}
+int orc_contact_sm_f32(int64_t ns, const uint32_t* types, const float* params, const uint32_t* hoff,
+                       const uint32_t* hcnt, const float* hpts, int64_t npts, const float* poses, const uint32_t* ia,
+                       const uint32_t* ib, int64_t np, float prediction, const float* init_dirs, int32_t max_iter,
+                       uint8_t* has, float* w1, float* w2, float* n, float* depth, int32_t* gi, int32_t* ei,
+                       int32_t* st, int nthreads) {
+  return contact_sm_batch<float>(ns, types, params, hoff, hcnt, hpts, npts, poses, ia, ib, np, prediction, init_dirs,
+                                 max_iter, has, w1, w2, n, depth, gi, ei, st, nthreads);
+}
+int orc_contact_sm_f64(int64_t ns, const uint32_t* types, const double* params, const uint32_t* hoff,
+                       const uint32_t* hcnt, const double* hpts, int64_t npts, const double* poses, const uint32_t* ia,
+                       const uint32_t* ib, int64_t np, double prediction, const double* init_dirs, int32_t max_iter,
+                       uint8_t* has, double* w1, double* w2, double* n, double* depth, int32_t* gi, int32_t* ei,
+                       int32_t* st, int nthreads) {
+  return contact_sm_batch<double>(ns, types, params, hoff, hcnt, hpts, npts, poses, ia, ib, np, prediction, init_dirs,
+                                  max_iter, has, w1, w2, n, depth, gi, ei, st, nthreads);
+}
+int orc_distance_f32(int64_t ns, const uint32_t* types, const float* params, const uint32_t* hoff,
+                     const uint32_t* hcnt, const float* hpts, int64_t npts, const float* poses, const uint32_t* ia,
+                     const uint32_t* ib, int64_t np, float* dist, int32_t* gi, int nthreads) {
+  return distance_batch<float>(ns, types, params, hoff, hcnt, hpts, npts, poses, ia, ib, np, dist, gi, nthreads);
+}
+int orc_distance_f64(int64_t ns, const uint32_t* types, const double* params, const uint32_t* hoff,
+                     const uint32_t* hcnt, const double* hpts, int64_t npts, const double* poses, const uint32_t* ia,
+                     const uint32_t* ib, int64_t np, double* dist, int32_t* gi, int nthreads) {
+  return distance_batch<double>(ns, types, params, hoff, hcnt, hpts, npts, poses, ia, ib, np, dist, gi, nthreads);
+}
+int orc_shape_aabb_f32(int64_t ns, const uint32_t* types, const float* params, const uint32_t* hoff,
+                       const uint32_t* hcnt, const float* hpts, int64_t npts, const float* poses, float* out6) {
+  return shape_aabbs<float>(ns, types, params, hoff, hcnt, hpts, npts, poses, out6);
+}
+int orc_shape_aabb_f64(int64_t ns, const uint32_t* types, const double* params, const uint32_t* hoff,
+                       const uint32_t* hcnt, const double* hpts, int64_t npts, const double* poses, double* out6) {
+  return shape_aabbs<double>(ns, types, params, hoff, hcnt, hpts, npts, poses, out6);
+}
+
+// ---- TriMesh
+void* orc_trimesh_create_f32(const float* pts, int64_t npts, const uint32_t* idx, int64_t ntri) {
+  return trimesh_create<float>(pts, npts, idx, ntri);
+}
+void orc_trimesh_free_f32(void* h) { delete (MeshHandle<float>*)h; }
+int orc_trimesh_cast_f32(void* h, const float* pose7, const float* orig, const float* dir, int64_t n, float max_toi,
+                         float* toi, float* nrm, uint32_t* feature, uint32_t* visits, int nthreads) {
+  return trimesh_cast<float>(h, pose7, orig, dir, n, max_toi, toi, nrm, feature, visits, nthreads);
+}
+void* orc_trimesh_create_f64(const double* pts, int64_t npts, const uint32_t* idx, int64_t ntri) {
+  return trimesh_create<double>(pts, npts, idx, ntri);
+}
+void orc_trimesh_free_f64(void* h) { delete (MeshHandle<double>*)h; }
+int orc_trimesh_cast_f64(void* h, const double* pose7, const double* orig, const double* dir, int64_t n,
+                         double max_toi, double* toi, double* nrm, uint32_t* feature, uint32_t* visits, int nthreads) {
+  return trimesh_cast<double>(h, pose7, orig, dir, n, max_toi, toi, nrm, feature, visits, nthreads);
+}
+int orc_trimesh_brute_f32(const float* pts, const uint32_t* idx, int64_t ntri, const float* pose7, const float* orig,
+                          const float* dir, int64_t n, float max_toi, float* toi, float* nrm, uint32_t* feature,
+                          uint32_t* nhits, int nthreads) {
+  return trimesh_brute<float>(pts, idx, ntri, pose7, orig, dir, n, max_toi, toi, nrm, feature, nhits, nthreads);
+}
+int orc_trimesh_brute_f64(const double* pts, const uint32_t* idx, int64_t ntri, const double* pose7,
+                          const double* orig, const double* dir, int64_t n, double max_toi, double* toi, double* nrm,
+                          uint32_t* feature, uint32_t* nhits, int nthreads) {
+  return trimesh_brute<double>(pts, idx, ntri, pose7, orig, dir, n, max_toi, toi, nrm, feature, nhits, nthreads);
+}
+// Single ray vs triangle (for closed-form tests).
+int orc_ray_triangle_f64(const double* abc, const double* orig, const double* dir, double* toi, double* n, int* fid) {
+  Ray<double> r{V3<double>(orig[0], orig[1], orig[2]), V3<double>(dir[0], dir[1], dir[2])};
+  V3<double> nn;
+  bool ok = ray_triangle(V3<double>(abc[0], abc[1], abc[2]), V3<double>(abc[3], abc[4], abc[5]),
+                         V3<double>(abc[6], abc[7], abc[8]), r, toi, &nn, fid);
+  for (int d = 0; d < 3; ++d) n[d] = nn[d];
+  return ok ? 1 : 0;
+}
+int orc_ray_aabb_f64(const double* bb6, const double* orig, const double* dir, double max_toi, int solid, double* toi) {
+  Ray<double> r{V3<double>(orig[0], orig[1], orig[2]), V3<double>(dir[0], dir[1], dir[2])};
+  return ray_aabb_toi(aabb_from6(bb6), r, max_toi, solid != 0, toi) ? 1 : 0;
+}
+
+// ---- Voronoi projections of the origin (solid), for brute-force pinning.
+int orc_project_f64(int kind, const double* pts, double* proj, int* loc_kind, int* loc_id, double* bc) {
+  Loc<double> loc{0, 0, {0, 0, 0}};
+  V3<double> P[4];
+  for (int i = 0; i < 4; ++i) P[i] = V3<double>(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+  V3<double> o(0, 0, 0), r;
+  if (kind == 1) r = project_segment(P[0], P[1], o, &loc);
+  else if (kind == 2) r = project_triangle(P[0], P[1], P[2], o, true, &loc);
+  else r = project_tetrahedron(P[0], P[1], P[2], P[3], o, &loc);
+  for (int d = 0; d < 3; ++d) proj[d] = r[d];
+  *loc_kind = loc.kind; *loc_id = loc.id;
+  for (int d = 0; d < 3; ++d) bc[d] = loc.bc[d];
+  return 0;
+}
+
+// ---- DBVTBroadPhase
+void* orc_bp_create(double margin) { return new DBVTBroadPhase<float>((float)margin); }
+void orc_bp_free(void* h) { delete (DBVTBroadPhase<float>*)h; }
+uint32_t orc_bp_create_proxy(void* h, const float* bb6) { return ((DBVTBroadPhase<float>*)h)->create_proxy(aabb_from6(bb6)); }
+void orc_bp_set_bv(void* h, uint32_t handle, const float* bb6) {
+  ((DBVTBroadPhase<float>*)h)->deferred_set_bounding_volume(handle, aabb_from6(bb6));
+}
+void orc_bp_remove(void* h, const uint32_t* hs, int64_t n) {
+  std::vector<uint32_t> v(hs, hs + n);
+  ((DBVTBroadPhase<float>*)h)->remove(v);
+}
+void orc_bp_update(void* h) { ((DBVTBroadPhase<float>*)h)->update(); }
+int64_t orc_bp_num_pairs(void* h) { return (int64_t)((DBVTBroadPhase<float>*)h)->pairs.size(); }
+// Sorted (a << 32 | b) keys with a < b.
+int64_t orc_bp_pairs(void* h, uint64_t* out, int64_t cap) {
+  auto* bp = (DBVTBroadPhase<float>*)h;
+  std::vector<uint64_t> k;
+  for (auto& p : bp->pairs) k.push_back(p.first);
+  std::sort(k.begin(), k.end());
+  int64_t n = std::min<int64_t>(cap, (int64_t)k.size());
+  for (int64_t i = 0; i < n; ++i) out[i] = k[i];
+  return (int64_t)k.size();
+}
+int64_t orc_bp_events(void* h, int which, uint64_t* out, int64_t cap) {
+  auto* bp = (DBVTBroadPhase<float>*)h;
+  std::vector<uint64_t> k = which == 0 ? bp->started : bp->stopped;
+  std::sort(k.begin(), k.end());
+  int64_t n = std::min<int64_t>(cap, (int64_t)k.size());
+  for (int64_t i = 0; i < n; ++i) out[i] = k[i];
+  return (int64_t)k.size();
+}
+void orc_bp_clear_events(void* h) {
+  auto* bp = (DBVTBroadPhase<float>*)h;
+  bp->started.clear(); bp->stopped.clear();
+}
+int orc_bp_stored_bv(void* h, uint32_t handle, float* out6) {
+  auto* bp = (DBVTBroadPhase<float>*)h;
+  const AABB<float>* b = bp->stored_bv(handle);
+  if (!b) return 0;
+  out6[0] = b->mins.x; out6[1] = b->mins.y; out6[2] = b->mins.z;
+  out6[3] = b->maxs.x; out6[4] = b->maxs.y; out6[5] = b->maxs.z;
+  return 1;
+}
+// O(N^2) brute-force pair set over AoS boxes (6 floats each): sorted keys i<j.
+int64_t orc_pairs_brute(const float* bb6, int64_t n, uint64_t* out, int64_t cap) {
+  int64_t cnt = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    AABB<float> a = aabb_from6(bb6 + 6 * i);
+    for (int64_t j = i + 1; j < n; ++j) {
+      if (aabb_intersects(a, aabb_from6(bb6 + 6 * j))) {
+        if (cnt < cap) out[cnt] = ((uint64_t)i << 32) | (uint64_t)j;
+        cnt++;
+      }
+    }
+  }
+  return cnt;
+}
+
+}  // extern "C"

@@ -1,0 +1,284 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, where it is the checker.  The product package
+(ncollide_amd) never imports it.
+
+Shape sets use the same layout as the product: types (0 ball, 1 cuboid,
+2 capsule, 3 hull), params (3 per shape), hull_off/hull_cnt into a hull point
+pool (xyz per point).  Poses are (tx, ty, tz, qi, qj, qk, qw).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+P = C.c_void_p
+I64 = C.c_int64
+
+
+def _declare(L):
+    def sig(name, res, *args):
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = list(args)
+
+    for t, R in (("f32", C.c_float), ("f64", C.c_double)):
+        sig(f"orc_contact_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P, P, I64, R, P, P, P, P, P, P, P, C.c_int)
+        sig(f"orc_contact_sm_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P, P, I64, R, P, C.c_int32,
+            P, P, P, P, P, P, P, P, C.c_int)
+        sig(f"orc_distance_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P, P, I64, P, P, C.c_int)
+        sig(f"orc_shape_aabb_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P)
+        sig(f"orc_trimesh_create_{t}", P, P, I64, P, I64)
+        sig(f"orc_trimesh_free_{t}", None, P)
+        sig(f"orc_trimesh_cast_{t}", C.c_int, P, P, P, P, I64, R, P, P, P, P, C.c_int)
+        sig(f"orc_trimesh_brute_{t}", C.c_int, P, P, I64, P, P, P, I64, R, P, P, P, P, C.c_int)
+    sig("orc_ray_triangle_f64", C.c_int, P, P, P, P, P, P)
+    sig("orc_ray_aabb_f64", C.c_int, P, P, P, C.c_double, C.c_int, P)
+    sig("orc_project_f64", C.c_int, C.c_int, P, P, P, P, P)
+    sig("orc_bp_create", P, C.c_double)
+    sig("orc_bp_free", None, P)
+    sig("orc_bp_create_proxy", C.c_uint32, P, P)
+    sig("orc_bp_set_bv", None, P, C.c_uint32, P)
+    sig("orc_bp_remove", None, P, P, I64)
+    sig("orc_bp_update", None, P)
+    sig("orc_bp_num_pairs", I64, P)
+    sig("orc_bp_pairs", I64, P, P, I64)
+    sig("orc_bp_events", I64, P, C.c_int, P, I64)
+    sig("orc_bp_clear_events", None, P)
+    sig("orc_bp_stored_bv", C.c_int, P, C.c_uint32, P)
+    sig("orc_pairs_brute", I64, P, I64, P, I64)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _dt(dtype):
+    return ("f32", np.float32) if np.dtype(dtype) == np.float32 else ("f64", np.float64)
+
+
+class ShapeSet:
+    """Host arrays describing shapes, in the oracle's scalar type."""
+
+    def __init__(self, types, params, hull_off, hull_cnt, hull_pts, dtype=np.float32):
+        t, R = _dt(dtype)
+        self.tag = t
+        self.R = R
+        self.types = np.ascontiguousarray(types, dtype=np.uint32)
+        self.params = np.ascontiguousarray(params, dtype=R).reshape(-1, 3)
+        self.hull_off = np.ascontiguousarray(hull_off, dtype=np.uint32)
+        self.hull_cnt = np.ascontiguousarray(hull_cnt, dtype=np.uint32)
+        hp = np.asarray(hull_pts, dtype=R).reshape(-1, 3)
+        if hp.shape[0] == 0:
+            hp = np.zeros((1, 3), dtype=R)
+        self.hull_pts = np.ascontiguousarray(hp)
+        self.n = len(self.types)
+
+    def args(self):
+        return (self.n, _p(self.types), _p(self.params), _p(self.hull_off), _p(self.hull_cnt),
+                _p(self.hull_pts), self.hull_pts.shape[0])
+
+
+def contact(ss, poses, ia, ib, prediction, nthreads=1):
+    L = lib()
+    R = ss.R
+    poses = np.ascontiguousarray(poses, dtype=R).reshape(-1, 7)
+    ia = np.ascontiguousarray(ia, dtype=np.uint32)
+    ib = np.ascontiguousarray(ib, dtype=np.uint32)
+    n = len(ia)
+    has = np.zeros(n, np.uint8)
+    w1 = np.zeros((n, 3), R); w2 = np.zeros((n, 3), R); nr = np.zeros((n, 3), R)
+    depth = np.zeros(n, R)
+    gi = np.zeros(n, np.int32); ei = np.zeros(n, np.int32)
+    getattr(L, f"orc_contact_{ss.tag}")(*ss.args(), _p(poses), _p(ia), _p(ib), n, prediction, _p(has), _p(w1),
+                                         _p(w2), _p(nr), _p(depth), _p(gi), _p(ei), nthreads)
+    return dict(has=has.astype(bool), world1=w1, world2=w2, normal=nr, depth=depth, gjk_iters=gi, epa_iters=ei)
+
+
+def contact_support_map(ss, poses, ia, ib, prediction, init_dirs=None, max_iter=10000, nthreads=1):
+    L = lib()
+    R = ss.R
+    poses = np.ascontiguousarray(poses, dtype=R).reshape(-1, 7)
+    ia = np.ascontiguousarray(ia, dtype=np.uint32)
+    ib = np.ascontiguousarray(ib, dtype=np.uint32)
+    n = len(ia)
+    dirs = None if init_dirs is None else np.ascontiguousarray(init_dirs, dtype=R).reshape(-1, 3)
+    has = np.zeros(n, np.uint8)
+    w1 = np.zeros((n, 3), R); w2 = np.zeros((n, 3), R); nr = np.zeros((n, 3), R)
+    depth = np.zeros(n, R)
+    gi = np.zeros(n, np.int32); ei = np.zeros(n, np.int32); st = np.zeros(n, np.int32)
+    getattr(L, f"orc_contact_sm_{ss.tag}")(*ss.args(), _p(poses), _p(ia), _p(ib), n, prediction, _p(dirs),
+                                            max_iter, _p(has), _p(w1), _p(w2), _p(nr), _p(depth), _p(gi), _p(ei),
+                                            _p(st), nthreads)
+    return dict(has=has.astype(bool), world1=w1, world2=w2, normal=nr, depth=depth, gjk_iters=gi, epa_iters=ei,
+                status=st)
+
+
+def distance(ss, poses, ia, ib, nthreads=1):
+    L = lib()
+    R = ss.R
+    poses = np.ascontiguousarray(poses, dtype=R).reshape(-1, 7)
+    ia = np.ascontiguousarray(ia, dtype=np.uint32)
+    ib = np.ascontiguousarray(ib, dtype=np.uint32)
+    n = len(ia)
+    d = np.zeros(n, R)
+    gi = np.zeros(n, np.int32)
+    getattr(L, f"orc_distance_{ss.tag}")(*ss.args(), _p(poses), _p(ia), _p(ib), n, _p(d), _p(gi), nthreads)
+    return d, gi
+
+
+def shape_aabbs(ss, poses):
+    L = lib()
+    poses = np.ascontiguousarray(poses, dtype=ss.R).reshape(-1, 7)
+    out = np.zeros((ss.n, 6), ss.R)
+    getattr(L, f"orc_shape_aabb_{ss.tag}")(*ss.args(), _p(poses), _p(out))
+    return out
+
+
+class TriMesh:
+    """The reference TriMesh: median-split BVT + best-first ray cast."""
+
+    def __init__(self, points, indices, dtype=np.float32):
+        self.tag, self.R = _dt(dtype)
+        self.points = np.ascontiguousarray(points, dtype=self.R).reshape(-1, 3)
+        self.indices = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
+        self.h = getattr(lib(), f"orc_trimesh_create_{self.tag}")(_p(self.points), self.points.shape[0],
+                                                                    _p(self.indices), self.indices.shape[0])
+
+    def __del__(self):
+        try:
+            getattr(lib(), f"orc_trimesh_free_{self.tag}")(self.h)
+        except Exception:
+            pass
+
+    def cast(self, origins, dirs, max_toi, pose=None, nthreads=1):
+        R = self.R
+        pose = np.array([0, 0, 0, 0, 0, 0, 1], R) if pose is None else np.ascontiguousarray(pose, dtype=R)
+        o = np.ascontiguousarray(origins, dtype=R).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=R).reshape(-1, 3)
+        n = o.shape[0]
+        toi = np.zeros(n, R); nr = np.zeros((n, 3), R)
+        feat = np.zeros(n, np.uint32); visits = np.zeros(n, np.uint32)
+        getattr(lib(), f"orc_trimesh_cast_{self.tag}")(self.h, _p(pose), _p(o), _p(d), n, max_toi, _p(toi),
+                                                       _p(nr), _p(feat), _p(visits), nthreads)
+        return toi, nr, feat, visits
+
+    def brute(self, origins, dirs, max_toi, pose=None, nthreads=1):
+        R = self.R
+        pose = np.array([0, 0, 0, 0, 0, 0, 1], R) if pose is None else np.ascontiguousarray(pose, dtype=R)
+        o = np.ascontiguousarray(origins, dtype=R).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=R).reshape(-1, 3)
+        n = o.shape[0]
+        toi = np.zeros(n, R); nr = np.zeros((n, 3), R)
+        feat = np.zeros(n, np.uint32); nh = np.zeros(n, np.uint32)
+        getattr(lib(), f"orc_trimesh_brute_{self.tag}")(_p(self.points), _p(self.indices), self.indices.shape[0],
+                                                        _p(pose), _p(o), _p(d), n, max_toi, _p(toi), _p(nr),
+                                                        _p(feat), _p(nh), nthreads)
+        return toi, nr, feat, nh
+
+
+def ray_triangle(abc, origin, direction):
+    abc = np.ascontiguousarray(abc, np.float64).reshape(9)
+    o = np.ascontiguousarray(origin, np.float64); d = np.ascontiguousarray(direction, np.float64)
+    toi = np.zeros(1); n = np.zeros(3); fid = np.zeros(1, np.int32)
+    ok = lib().orc_ray_triangle_f64(_p(abc), _p(o), _p(d), _p(toi), _p(n), _p(fid))
+    return (bool(ok), toi[0], n, int(fid[0]))
+
+
+def ray_aabb(bb6, origin, direction, max_toi, solid=True):
+    bb = np.ascontiguousarray(bb6, np.float64)
+    o = np.ascontiguousarray(origin, np.float64); d = np.ascontiguousarray(direction, np.float64)
+    toi = np.zeros(1)
+    ok = lib().orc_ray_aabb_f64(_p(bb), _p(o), _p(d), max_toi, 1 if solid else 0, _p(toi))
+    return bool(ok), toi[0]
+
+
+def project_origin(points):
+    """Voronoi-region projection of the origin on a segment/triangle/tetrahedron (f64)."""
+    pts = np.zeros((4, 3))
+    pts[: len(points)] = points
+    pts = np.ascontiguousarray(pts)
+    proj = np.zeros(3); lk = np.zeros(1, np.int32); li = np.zeros(1, np.int32); bc = np.zeros(3)
+    lib().orc_project_f64(len(points) - 1, _p(pts), _p(proj), _p(lk), _p(li), _p(bc))
+    return proj, int(lk[0]), int(li[0]), bc
+
+
+class BroadPhase:
+    """The reference DBVTBroadPhase restated (f32 AABBs)."""
+
+    def __init__(self, margin):
+        self.h = lib().orc_bp_create(margin)
+
+    def __del__(self):
+        try:
+            lib().orc_bp_free(self.h)
+        except Exception:
+            pass
+
+    def create_proxy(self, bb6):
+        b = np.ascontiguousarray(bb6, np.float32)
+        return int(lib().orc_bp_create_proxy(self.h, _p(b)))
+
+    def deferred_set_bounding_volume(self, handle, bb6):
+        b = np.ascontiguousarray(bb6, np.float32)
+        lib().orc_bp_set_bv(self.h, handle, _p(b))
+
+    def remove(self, handles):
+        hs = np.ascontiguousarray(handles, np.uint32)
+        lib().orc_bp_remove(self.h, _p(hs), len(hs))
+
+    def update(self):
+        lib().orc_bp_update(self.h)
+
+    def num_interferences(self):
+        return int(lib().orc_bp_num_pairs(self.h))
+
+    def pairs(self):
+        n = self.num_interferences()
+        out = np.zeros(max(n, 1), np.uint64)
+        lib().orc_bp_pairs(self.h, _p(out), n)
+        return out[:n]
+
+    def events(self):
+        res = []
+        for which in (0, 1):
+            n = lib().orc_bp_events(self.h, which, None, 0)
+            out = np.zeros(max(n, 1), np.uint64)
+            lib().orc_bp_events(self.h, which, _p(out), n)
+            res.append(out[:n])
+        lib().orc_bp_clear_events(self.h)
+        return res[0], res[1]
+
+    def stored_bv(self, handle):
+        out = np.zeros(6, np.float32)
+        ok = lib().orc_bp_stored_bv(self.h, handle, _p(out))
+        return out if ok else None
+
+
+def pairs_brute(bb6):
+    bb = np.ascontiguousarray(bb6, np.float32).reshape(-1, 6)
+    n = lib().orc_pairs_brute(_p(bb), bb.shape[0], None, 0)
+    out = np.zeros(max(n, 1), np.uint64)
+    lib().orc_pairs_brute(_p(bb), bb.shape[0], _p(out), n)
+    return out[:n]
