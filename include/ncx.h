@@ -1,0 +1,177 @@
+/* ncx.h -- C ABI of the MI355X collision hot path (libncx.so).
+ *
+ * Batch-first replacement for ncollide3d 0.33's broad phase, convex contact /
+ * distance queries and TriMesh ray casting.  Every function returns 0 on
+ * success or a negative NCX_E_* code; ncx_last_error() gives a message.
+ * Buffers named *_dev are device pointers (hipMalloc'd or torch CUDA tensors)
+ * resident on the context's device; all work is enqueued on the context's
+ * stream and is complete when the call returns unless noted.  Handles are
+ * opaque; calls on one handle are not thread-safe (the reference takes
+ * `&mut self`), distinct handles are independent.
+ *
+ * Layouts shared by several entry points:
+ *   pose      : ncx_pose, 32 bytes: translation then unit quaternion (i, j, k, w),
+ *               the fields of nalgebra's Isometry3<f32> (reference src/lib.rs:94-136).
+ *   AABB SoA  : six float arrays minx, miny, minz, maxx, maxy, maxz (reference
+ *               AABB{mins, maxs}, src/bounding_volume/aabb.rs:37-40).
+ *   shape     : ncx_shape, 32 bytes; hull vertices live in a pool of three float
+ *               arrays (x, y, z), local frame, indexed by hull_offset/hull_count.
+ *   pair keys : (a << 32) | b with a < b, the reference's SortedPair
+ *               (src/utils/sorted_pair.rs:7-26).
+ */
+#ifndef NCX_H
+#define NCX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NCX_OK 0
+#define NCX_E_INVALID (-1)    /* bad argument (null handle, size mismatch, bad shape type) */
+#define NCX_E_HIP (-2)        /* HIP runtime error */
+#define NCX_E_CAPACITY (-3)   /* caller-provided output capacity too small; *n holds the size needed */
+#define NCX_E_NODEVICE (-4)   /* no GPU visible */
+
+enum ncx_shape_type { NCX_BALL = 0, NCX_CUBOID = 1, NCX_CAPSULE = 2, NCX_CONVEX_HULL = 3 };
+
+typedef struct ncx_pose {
+  float tx, ty, tz, pad;
+  float qi, qj, qk, qw;
+} ncx_pose;
+
+/* Ball: p0 = radius (src/shape/ball.rs); Cuboid: p0..p2 = half extents
+ * (src/shape/cuboid.rs); Capsule: p0 = half height along local y, p1 = radius
+ * (src/shape/capsule.rs); ConvexHull: hull_offset/hull_count into the pool
+ * (src/shape/convex.rs). */
+typedef struct ncx_shape {
+  uint32_t type;
+  uint32_t hull_offset;
+  uint32_t hull_count;
+  float p0, p1, p2;
+  uint32_t pad[2];
+} ncx_shape;
+
+typedef struct ncx_ctx ncx_ctx;
+typedef struct ncx_bp ncx_bp;
+typedef struct ncx_shapes ncx_shapes;
+typedef struct ncx_mesh ncx_mesh;
+
+const char* ncx_last_error(void);
+
+/* Context: device + stream.  `stream` is a hipStream_t (NULL = the default stream).
+ * No reference counterpart: the reference is a single-threaded CPU library. */
+int ncx_init(ncx_ctx** ctx, int device, void* stream);
+int ncx_free(ncx_ctx* ctx);
+int ncx_set_stream(ncx_ctx* ctx, void* stream);
+int ncx_synchronize(ncx_ctx* ctx);
+
+/* ---------------------------------------------------------------- broad phase
+ * Replaces DBVTBroadPhase (src/pipeline/broad_phase/dbvt_broad_phase.rs:57-460)
+ * behind the BroadPhase trait (src/pipeline/broad_phase/broad_phase.rs:41-86).
+ * The stored box of each proxy follows the reference exactly: create_proxy
+ * stores the box as given (:275-280); deferred_set_bounding_volume replaces it
+ * by box.loosened(margin) only when the new box is not contained in the stored
+ * one (:325-347).  After ncx_bp_update the pair set is
+ * { (a,b) : a<b, stored_a intersects stored_b } (aabb.rs:156-158), the set the
+ * reference's HashMap holds after update (:174-259); it is recomputed on the GPU
+ * by a Morton-sorted LBVH instead of the incremental DBVT. */
+int ncx_bp_create(ncx_ctx* ctx, float margin, ncx_bp** bp);                    /* DBVTBroadPhase::new (:81) */
+int ncx_bp_destroy(ncx_bp* bp);
+/* create_proxy for n boxes (host SoA); handles are slab keys (LIFO reuse, as the
+ * slab crate behind `proxies: Slab<...>` at :58). */
+int ncx_bp_create_proxies(ncx_bp* bp, const float* const aabb_soa_host[6], size_t n, uint32_t* handles_out);
+/* deferred_set_bounding_volume for n proxies, boxes on the device (SoA). */
+int ncx_bp_set_aabbs_dev(ncx_bp* bp, const uint32_t* handles_dev, const float* const aabb_soa_dev[6], size_t n);
+/* Same with host arrays. */
+int ncx_bp_set_aabbs(ncx_bp* bp, const uint32_t* handles, const float* const aabb_soa_host[6], size_t n);
+/* remove (:282-323). */
+int ncx_bp_remove(ncx_bp* bp, const uint32_t* handles, size_t n);
+/* update (:174-259): rebuild the pair set on the GPU. */
+int ncx_bp_update(ncx_bp* bp);
+/* num_interferences (:97-99). */
+int ncx_bp_num_pairs(ncx_bp* bp, size_t* n);
+/* Current pairs as sorted keys (host copy). Returns NCX_E_CAPACITY with *n set if cap is too small. */
+int ncx_bp_pairs(ncx_bp* bp, uint64_t* keys_host, size_t cap, size_t* n);
+/* Device pointers to the current pair list (a[i] < b[i]), valid until the next update. */
+int ncx_bp_pairs_dev(ncx_bp* bp, const uint32_t** a_dev, const uint32_t** b_dev, size_t* n);
+/* interference_started / interference_stopped of the last update (broad_phase.rs:28-38), sorted keys. */
+int ncx_bp_events(ncx_bp* bp, uint64_t* started, size_t cap_started, size_t* n_started, uint64_t* stopped,
+                  size_t cap_stopped, size_t* n_stopped);
+/* The stored box of a proxy (BroadPhase::proxy, :262-273), 6 floats. Returns NCX_E_INVALID if absent. */
+int ncx_bp_proxy_aabb(ncx_bp* bp, uint32_t handle, float out[6]);
+
+/* ---------------------------------------------------------------- shapes
+ * Uploads a shape set (host arrays) to the device.  Hull pools are local-frame
+ * vertices (ConvexHull::points, src/shape/convex.rs:350-353). */
+int ncx_shapes_create(ncx_ctx* ctx, const ncx_shape* shapes, size_t n, const float* hull_x, const float* hull_y,
+                      const float* hull_z, size_t n_hull_points, ncx_shapes** out);
+int ncx_shapes_destroy(ncx_shapes* s);
+int ncx_shapes_count(ncx_shapes* s, size_t* n);
+
+/* World AABB of every shape at the given poses (device arrays), optionally loosened by
+ * `margin` (Shape::aabb: src/bounding_volume/aabb_ball.rs:8-13, aabb_cuboid.rs:7-14,
+ * aabb_support_map.rs:35-45, aabb_convex.rs:7-11; AABB::loosened aabb.rs:190-199). */
+int ncx_shapes_aabbs_dev(ncx_shapes* s, const ncx_pose* poses_dev, float margin, float* const aabb_soa_dev[6]);
+
+/* ---------------------------------------------------------------- contact / distance
+ * Batched query::contact (src/query/contact/contact_shape_shape.rs:10-48) over
+ * pairs (ia[k], ib[k]) of one shape set at `poses_dev` (indexed by shape).
+ * Dispatch, closed forms, GJK (src/query/algorithms/gjk.rs:76-177) and EPA
+ * (src/query/algorithms/epa3.rs:219-430) follow the reference operation by
+ * operation in f32.  Outputs (device): has_contact[k] = 1 for Some(Contact);
+ * world1/world2/normal as 3 floats per pair; depth.  gjk_iters may be NULL.
+ * init_dir_dev (nullable, 3 floats per pair): the `init_dir` argument of
+ * contact_support_map_support_map_with_params
+ * (src/query/contact/contact_support_map_support_map.rs:38-61) for pairs that
+ * reach the support-map path; NULL is the reference's None. */
+int ncx_contact_batch(ncx_shapes* s, const ncx_pose* poses_dev, const uint32_t* ia_dev, const uint32_t* ib_dev,
+                      size_t npairs, float prediction, const float* init_dir_dev, uint8_t* has_contact_dev,
+                      float* world1_dev, float* world2_dev, float* normal_dev, float* depth_dev,
+                      int32_t* gjk_iters_dev);
+/* Batched query::distance (src/query/distance/distance.rs:9-33). */
+int ncx_distance_batch(ncx_shapes* s, const ncx_pose* poses_dev, const uint32_t* ia_dev, const uint32_t* ib_dev,
+                       size_t npairs, float* dist_dev, int32_t* gjk_iters_dev);
+
+/* ---------------------------------------------------------------- TriMesh ray casting
+ * TriMesh::new (src/shape/trimesh.rs:100-197) with a GPU-built LBVH in place of
+ * the median-split BVT; host arrays. */
+int ncx_trimesh_create(ncx_ctx* ctx, const float* points_xyz, size_t npoints, const uint32_t* indices,
+                       size_t ntriangles, ncx_mesh** mesh);
+int ncx_trimesh_destroy(ncx_mesh* mesh);
+/* RayCast::toi_and_normal_with_ray for TriMesh (src/query/ray/ray_trimesh.rs:23-50) on
+ * n rays (device SoA origins ox/oy/oz and directions dx/dy/dz), mesh at `pose`.
+ * Hit: toi, world normal (3 floats per ray), feature = face id, or face id +
+ * ntriangles for a back-face hit (ray_trimesh.rs:41-47).  Miss: feature =
+ * UINT32_MAX, toi = -1.  Among hits with equal toi the smallest feature wins. */
+int ncx_trimesh_cast_rays(ncx_mesh* mesh, const ncx_pose* pose, const float* const ray_soa_dev[6], size_t n,
+                          float max_toi, float* toi_dev, float* normal_dev, uint32_t* feature_dev);
+/* Traversal statistics of the same cast (nodes visited, triangles tested per ray), for roofline accounting. */
+int ncx_trimesh_cast_stats(ncx_mesh* mesh, const ncx_pose* pose, const float* const ray_soa_dev[6], size_t n,
+                           float max_toi, uint32_t* nodes_visited_dev, uint32_t* tris_tested_dev);
+
+/* ---------------------------------------------------------------- input preparation (host)
+ * ConvexHull::try_from_points (src/shape/convex.rs:88-97 -> transformation::convex_hull3,
+ * src/transformation/convex_hull3.rs:28-149): writes the indices of the input points that are
+ * vertices of their convex hull, ascending; *n_out = 0 for a degenerate (flat) cloud. */
+int ncx_convex_hull(const float* points_xyz, size_t npoints, uint32_t* out_ids, size_t* n_out);
+/* Many clouds on host threads: cloud i is points[offsets[i] .. offsets[i] + counts[i]); its hull
+ * vertex ids (local to the cloud) go to out_ids[offsets[i] ..], their number to out_counts[i]. */
+int ncx_convex_hull_batch(const float* points_xyz, const uint32_t* offsets, const uint32_t* counts, size_t nclouds,
+                          uint32_t* out_ids, uint32_t* out_counts, int nthreads);
+
+/* Warm-started contact batch: like ncx_contact_batch with init_dir, but GJK from init_dir stops
+ * after max_iter iterations and such a pair is recomputed with init_dir = None (plain GJK).
+ * No reference counterpart beyond the init_dir hook of
+ * src/query/contact/contact_support_map_support_map.rs:38-61. */
+int ncx_contact_batch_capped(ncx_shapes* s, const ncx_pose* poses_dev, const uint32_t* ia_dev,
+                             const uint32_t* ib_dev, size_t npairs, float prediction, const float* init_dir_dev,
+                             int32_t max_iter, uint8_t* has_contact_dev, float* world1_dev, float* world2_dev,
+                             float* normal_dev, float* depth_dev, int32_t* gjk_iters_dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NCX_H */

@@ -1,0 +1,117 @@
+"""DBVTBroadPhase (src/pipeline/broad_phase/dbvt_broad_phase.rs) on the GPU.
+
+Same method names and meaning as the reference's BroadPhase trait
+(src/pipeline/broad_phase/broad_phase.rs:41-86); boxes are 6 floats
+(minx, miny, minz, maxx, maxy, maxz).  Batch variants take (n, 6) arrays or
+six device arrays.  Pair keys are (a << 32) | b with a < b.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .context import default_context
+
+
+class DBVTBroadPhase:
+    def __init__(self, margin, ctx=None):
+        self.ctx = ctx or default_context()
+        if margin < 0:
+            raise ValueError("The loosening margin must be positive.")
+        h = C.c_void_p()
+        _lib.check(self.ctx.lib.ncx_bp_create(self.ctx.h, float(margin), C.byref(h)))
+        self.h = h
+        self.margin = float(margin)
+
+    # -- proxies
+    def create_proxy(self, aabb):
+        return int(self.create_proxies(np.asarray(aabb, np.float32).reshape(1, 6))[0])
+
+    def create_proxies(self, aabbs):
+        a = np.ascontiguousarray(np.asarray(aabbs, np.float32).reshape(-1, 6).T)
+        n = a.shape[1]
+        out = np.zeros(n, np.uint32)
+        cols = [np.ascontiguousarray(a[d]) for d in range(6)]
+        _lib.check(self.ctx.lib.ncx_bp_create_proxies(self.h, _lib.ptr6(cols), n, _lib.ptr(out)))
+        return out
+
+    def deferred_set_bounding_volume(self, handle, aabb):
+        self.deferred_set_bounding_volumes(np.array([handle], np.uint32), np.asarray(aabb, np.float32).reshape(1, 6))
+
+    def deferred_set_bounding_volumes(self, handles, aabbs):
+        """Host arrays: handles (n,), boxes (n, 6).  Handles within one call must be distinct."""
+        hs = np.ascontiguousarray(handles, np.uint32)
+        a = np.ascontiguousarray(np.asarray(aabbs, np.float32).reshape(-1, 6).T)
+        cols = [np.ascontiguousarray(a[d]) for d in range(6)]
+        _lib.check(self.ctx.lib.ncx_bp_set_aabbs(self.h, _lib.ptr(hs), _lib.ptr6(cols), len(hs)))
+
+    def deferred_set_bounding_volumes_dev(self, handles_dev, aabb_soa_dev):
+        """Device variant: handles (n,) int32/uint32 tensor, aabb_soa_dev = six (n,) float32 tensors."""
+        _lib.check(self.ctx.lib.ncx_bp_set_aabbs_dev(self.h, _lib.ptr(handles_dev), _lib.ptr6(aabb_soa_dev),
+                                                     handles_dev.numel()))
+
+    def remove(self, handles):
+        hs = np.ascontiguousarray(handles, np.uint32)
+        _lib.check(self.ctx.lib.ncx_bp_remove(self.h, _lib.ptr(hs), len(hs)))
+
+    # -- update and results
+    def update(self):
+        _lib.check(self.ctx.lib.ncx_bp_update(self.h))
+
+    def num_interferences(self):
+        n = C.c_size_t()
+        _lib.check(self.ctx.lib.ncx_bp_num_pairs(self.h, C.byref(n)))
+        return n.value
+
+    def pairs(self):
+        """Sorted pair keys (numpy uint64)."""
+        n = self.num_interferences()
+        out = np.zeros(max(n, 1), np.uint64)
+        got = C.c_size_t()
+        _lib.check(self.ctx.lib.ncx_bp_pairs(self.h, _lib.ptr(out), len(out), C.byref(got)))
+        return out[:got.value]
+
+    def pairs_dev(self):
+        """(a, b) device pointers + count of the current (unsorted) pair list, as int32 tensors (copies)."""
+        a = C.c_void_p()
+        b = C.c_void_p()
+        n = C.c_size_t()
+        _lib.check(self.ctx.lib.ncx_bp_pairs_dev(self.h, C.byref(a), C.byref(b), C.byref(n)))
+        return a.value, b.value, n.value
+
+    def events(self):
+        """(started, stopped) sorted pair keys of the last update."""
+        ns = C.c_size_t()
+        nt = C.c_size_t()
+        rc = self.ctx.lib.ncx_bp_events(self.h, None, 0, C.byref(ns), None, 0, C.byref(nt))
+        if rc not in (0, -3):
+            _lib.check(rc)
+        s = np.zeros(max(ns.value, 1), np.uint64)
+        t = np.zeros(max(nt.value, 1), np.uint64)
+        _lib.check(self.ctx.lib.ncx_bp_events(self.h, _lib.ptr(s), len(s), C.byref(ns), _lib.ptr(t), len(t),
+                                              C.byref(nt)))
+        return s[:ns.value], t[:nt.value]
+
+    def proxy(self, handle):
+        out = np.zeros(6, np.float32)
+        _lib.check(self.ctx.lib.ncx_bp_proxy_aabb(self.h, int(handle), _lib.ptr(out)))
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.ctx.lib.ncx_bp_destroy(self.h)
+        except Exception:
+            pass
+
+
+def pair_keys_to_arrays(keys):
+    keys = np.asarray(keys, np.uint64)
+    return (keys >> np.uint64(32)).astype(np.uint32), (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
+def pair_keys_from_tensors(a, b):
+    a = a.to(torch.int64).cpu().numpy().astype(np.uint64)
+    b = b.to(torch.int64).cpu().numpy().astype(np.uint64)
+    return np.sort((a << np.uint64(32)) | b)

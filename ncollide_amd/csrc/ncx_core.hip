@@ -1,0 +1,468 @@
+// ncx_core.hip -- context, errors, device-wide scan, LSD radix sort and the
+// Morton-sorted LBVH (Karras 2012 hierarchy, atomic bottom-up refit) shared by
+// the broad phase and TriMesh ray casting.
+#include "ncx_internal.h"
+#include <cstdio>
+#include <mutex>
+
+static thread_local std::string g_last_error;
+void ncx_set_error(const std::string& msg) { g_last_error = msg; }
+extern "C" const char* ncx_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int ncx_init(ncx_ctx** out, int device, void* stream) {
+  NCX_REQUIRE(out != nullptr, "ncx_init: null output");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    ncx_set_error("ncx_init: no HIP device visible");
+    return NCX_E_NODEVICE;
+  }
+  NCX_REQUIRE(device >= 0 && device < count, "ncx_init: device index out of range");
+  NCX_HIP_CHECK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  NCX_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+  ncx_ctx* c = new ncx_ctx;
+  c->device = device;
+  c->stream = (hipStream_t)stream;
+  c->num_cus = prop.multiProcessorCount;
+  *out = c;
+  return NCX_OK;
+}
+
+extern "C" int ncx_free(ncx_ctx* ctx) {
+  delete ctx;
+  return NCX_OK;
+}
+
+extern "C" int ncx_set_stream(ncx_ctx* ctx, void* stream) {
+  NCX_REQUIRE(ctx, "ncx_set_stream: null context");
+  ctx->stream = (hipStream_t)stream;
+  return NCX_OK;
+}
+
+extern "C" int ncx_synchronize(ncx_ctx* ctx) {
+  NCX_REQUIRE(ctx, "ncx_synchronize: null context");
+  NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return NCX_OK;
+}
+
+// ------------------------------------------------------------------ wave helpers
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t o = __shfl_up(v, off, 64);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+// ------------------------------------------------------------------ exclusive scan
+#define SCAN_BLOCK 256
+#define SCAN_ITEMS 8
+#define SCAN_TILE (SCAN_BLOCK * SCAN_ITEMS)
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                            size_t n, uint32_t* __restrict__ block_sums) {
+  __shared__ uint32_t wsum[SCAN_BLOCK / 64];
+  const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t tot = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    size_t e = base + i;
+    v[i] = e < n ? in[e] : 0u;
+    tot += v[i];
+  }
+  uint32_t incl = wave_incl_scan(tot);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  uint32_t wpre = 0, bsum = 0;
+#pragma unroll
+  for (int w = 0; w < SCAN_BLOCK / 64; ++w) {
+    uint32_t s = wsum[w];
+    if (w < wid) wpre += s;
+    bsum += s;
+  }
+  uint32_t run = wpre + incl - tot;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    size_t e = base + i;
+    if (e < n) out[e] = run;
+    run += v[i];
+  }
+  if (threadIdx.x == 0 && block_sums) block_sums[blockIdx.x] = bsum;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_add(uint32_t* __restrict__ out, size_t n,
+                                                          const uint32_t* __restrict__ block_offs) {
+  const uint32_t add = block_offs[blockIdx.x];
+  const size_t base = (size_t)blockIdx.x * SCAN_TILE;
+  for (int i = threadIdx.x; i < SCAN_TILE; i += SCAN_BLOCK) {
+    size_t e = base + i;
+    if (e < n) out[e] += add;
+  }
+}
+
+__global__ void k_total(const uint32_t* in_last, const uint32_t* out_last, uint32_t* total) {
+  *total = *in_last + *out_last;
+}
+
+static int scan_rec(ncx_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp, size_t tmp_words) {
+  unsigned nb = ceil_div(n, SCAN_TILE);
+  if (nb <= 1) {
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_BLOCK), 0, ctx->stream, in, out, n, (uint32_t*)nullptr);
+    NCX_HIP_CHECK(hipGetLastError());
+    return NCX_OK;
+  }
+  NCX_REQUIRE(tmp_words >= (size_t)nb * 2, "scan: scratch too small");
+  uint32_t* sums = tmp;
+  uint32_t* scanned = tmp + nb;
+  uint32_t* rest = tmp + 2 * (size_t)nb;
+  hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(SCAN_BLOCK), 0, ctx->stream, in, out, n, sums);
+  NCX_HIP_CHECK(hipGetLastError());
+  int rc = scan_rec(ctx, sums, scanned, nb, rest, tmp_words - 2 * (size_t)nb);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(SCAN_BLOCK), 0, ctx->stream, out, n, scanned);
+  NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
+}
+
+int scan_exclusive_u32(ncx_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n, DevBuf& tmp, uint32_t* total_dev) {
+  if (n == 0) return NCX_OK;
+  NCX_REQUIRE(in != out, "scan_exclusive_u32: in-place scan not supported");
+  size_t words = 0, m = n;
+  while (m > SCAN_TILE) { m = ceil_div(m, SCAN_TILE); words += 2 * m; }
+  words += 2 * SCAN_TILE + 64;
+  int rc = tmp.reserve(words * 4);
+  if (rc) return rc;
+  rc = scan_rec(ctx, in, out, n, tmp.as<uint32_t>(), words);
+  if (rc) return rc;
+  if (total_dev) {
+    hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, ctx->stream, in + n - 1, out + n - 1, total_dev);
+    NCX_HIP_CHECK(hipGetLastError());
+  }
+  return NCX_OK;
+}
+
+// ------------------------------------------------------------------ LSD radix sort (8-bit digits)
+#define RS_BLOCK 256
+#define RS_ITEMS 8
+#define RS_TILE (RS_BLOCK * RS_ITEMS)
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint32_t* __restrict__ keys, size_t n, int shift,
+                                                       uint32_t nbits, uint32_t* __restrict__ hist, unsigned nblocks) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t mask = (1u << nbits) - 1u;
+  const size_t base = (size_t)blockIdx.x * RS_TILE;
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    size_t e = base + (size_t)r * RS_BLOCK + threadIdx.x;
+    if (e < n) atomicAdd(&h[(keys[e] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint32_t* __restrict__ keys_in,
+                                                          const uint32_t* __restrict__ vals_in,
+                                                          uint32_t* __restrict__ keys_out,
+                                                          uint32_t* __restrict__ vals_out, size_t n, int shift,
+                                                          uint32_t nbits, const uint32_t* __restrict__ offs,
+                                                          unsigned nblocks) {
+  __shared__ uint32_t base_d[256];
+  __shared__ uint32_t run_d[256];
+  __shared__ uint32_t cnt[RS_BLOCK / 64][256];
+  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  base_d[tid] = offs[(size_t)tid * nblocks + blockIdx.x];
+  run_d[tid] = 0;
+  const uint32_t mask = (1u << nbits) - 1u;
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  const size_t tbase = (size_t)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ITEMS; ++r) {
+#pragma unroll
+    for (int w = 0; w < RS_BLOCK / 64; ++w) cnt[w][tid] = 0;
+    __syncthreads();
+    size_t e = tbase + (size_t)r * RS_BLOCK + tid;
+    bool valid = e < n;
+    uint32_t key = valid ? keys_in[e] : 0u;
+    uint32_t val = (valid && vals_in) ? vals_in[e] : 0u;
+    uint32_t d = (key >> shift) & mask;
+    uint64_t peers = __ballot(valid);
+    for (uint32_t b = 0; b < nbits; ++b) {
+      bool bit = (d >> b) & 1u;
+      uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    uint32_t rank = __popcll(peers & lt);
+    if (valid && rank == 0) cnt[wid][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pre = run_d[d];
+      for (int w = 0; w < wid; ++w) pre += cnt[w][d];
+      uint32_t dst = base_d[d] + pre + rank;
+      keys_out[dst] = key;
+      if (vals_out) vals_out[dst] = val;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < RS_BLOCK / 64; ++w) add += cnt[w][tid];
+    run_d[tid] += add;
+    __syncthreads();
+  }
+}
+
+int radix_sort_u32(ncx_ctx* ctx, uint32_t* keys, uint32_t* vals, size_t n, int begin_bit, int end_bit,
+                   DevBuf& tmp_keys, DevBuf& tmp_vals, DevBuf& hist, DevBuf& scan_tmp) {
+  if (n <= 1 || end_bit <= begin_bit) return NCX_OK;
+  NCX_REQUIRE(n < 0xFFFFFFFFull, "radix_sort_u32: too many keys");
+  unsigned nb = ceil_div(n, RS_TILE);
+  int rc;
+  if ((rc = tmp_keys.reserve(n * 4))) return rc;
+  if (vals && (rc = tmp_vals.reserve(n * 4))) return rc;
+  if ((rc = hist.reserve((size_t)256 * nb * 4 * 2))) return rc;
+  uint32_t* h = hist.as<uint32_t>();
+  uint32_t* hs = h + (size_t)256 * nb;
+  uint32_t *ki = keys, *vi = vals, *ko = tmp_keys.as<uint32_t>(), *vo = vals ? tmp_vals.as<uint32_t>() : nullptr;
+  for (int shift = begin_bit; shift < end_bit; shift += 8) {
+    uint32_t nbits = (uint32_t)((end_bit - shift) < 8 ? (end_bit - shift) : 8);
+    hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_BLOCK), 0, ctx->stream, ki, n, shift, nbits, h, nb);
+    NCX_HIP_CHECK(hipGetLastError());
+    if ((rc = scan_exclusive_u32(ctx, h, hs, (size_t)256 * nb, scan_tmp, nullptr))) return rc;
+    hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_BLOCK), 0, ctx->stream, ki, vi, ko, vo, n, shift, nbits, hs, nb);
+    NCX_HIP_CHECK(hipGetLastError());
+    std::swap(ki, ko);
+    std::swap(vi, vo);
+  }
+  if (ki != keys) {
+    NCX_HIP_CHECK(hipMemcpyAsync(keys, ki, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    if (vals) NCX_HIP_CHECK(hipMemcpyAsync(vals, vi, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return NCX_OK;
+}
+
+int radix_sort_pairs64(ncx_ctx* ctx, uint32_t* hi, uint32_t* lo, size_t n, int hi_bits, int lo_bits, DevBuf& t1,
+                       DevBuf& t2, DevBuf& t3, DevBuf& hist, DevBuf& scan_tmp) {
+  (void)t3;
+  int rc = radix_sort_u32(ctx, lo, hi, n, 0, lo_bits, t1, t2, hist, scan_tmp);
+  if (rc) return rc;
+  return radix_sort_u32(ctx, hi, lo, n, 0, hi_bits, t1, t2, hist, scan_tmp);
+}
+
+// ------------------------------------------------------------------ LBVH
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+__global__ void k_bounds_init(uint32_t* b) {
+  int t = threadIdx.x;
+  if (t < 3) b[t] = 0xFFFFFFFFu;
+  else if (t < 6) b[t] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_centroid_bounds(const float* __restrict__ minx, const float* __restrict__ miny,
+                                                          const float* __restrict__ minz, const float* __restrict__ maxx,
+                                                          const float* __restrict__ maxy, const float* __restrict__ maxz,
+                                                          size_t n, uint32_t* __restrict__ b) {
+  float lo[3] = {NCX_FLT_MAX, NCX_FLT_MAX, NCX_FLT_MAX}, hi[3] = {-NCX_FLT_MAX, -NCX_FLT_MAX, -NCX_FLT_MAX};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    float c[3] = {(minx[i] + maxx[i]) * 0.5f, (miny[i] + maxy[i]) * 0.5f, (minz[i] + maxz[i]) * 0.5f};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { lo[d] = fminf(lo[d], c[d]); hi[d] = fmaxf(hi[d], c[d]); }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    for (int off = 32; off > 0; off >>= 1) {
+      lo[d] = fminf(lo[d], __shfl_xor(lo[d], off, 64));
+      hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], off, 64));
+    }
+  }
+  if (__lane_id() == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { atomicMin(&b[d], f2ord(lo[d])); atomicMax(&b[3 + d], f2ord(hi[d])); }
+  }
+}
+
+__device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float* __restrict__ minx, const float* __restrict__ miny,
+                                                 const float* __restrict__ minz, const float* __restrict__ maxx,
+                                                 const float* __restrict__ maxy, const float* __restrict__ maxz,
+                                                 size_t n, const uint32_t* __restrict__ b, uint32_t* __restrict__ keys,
+                                                 uint32_t* __restrict__ vals) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float c[3] = {(minx[i] + maxx[i]) * 0.5f, (miny[i] + maxy[i]) * 0.5f, (minz[i] + maxz[i]) * 0.5f};
+  uint32_t q[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float lo = ord2f(b[d]), hi = ord2f(b[3 + d]);
+    float ext = hi - lo;
+    float u = ext > 0.f ? (c[d] - lo) / ext : 0.f;
+    float s = fminf(fmaxf(u * 1024.f, 0.f), 1023.f);
+    q[d] = (uint32_t)s;
+  }
+  keys[i] = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+  vals[i] = (uint32_t)i;
+}
+
+__device__ __forceinline__ int lbvh_delta(const uint32_t* __restrict__ keys, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  uint32_t a = keys[i], b = keys[j];
+  if (a != b) return __clz(a ^ b);
+  return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+}
+
+// Karras 2012, "Maximizing parallelism in the construction of BVHs, octrees and k-d trees".
+__global__ __launch_bounds__(256) void k_hierarchy(const uint32_t* __restrict__ keys, int n, LNode* __restrict__ nodes,
+                                                    uint32_t* __restrict__ parent) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  int d = (lbvh_delta(keys, n, i, i + 1) - lbvh_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  int dmin = lbvh_delta(keys, n, i, i - d);
+  int lmax = 2;
+  while (lbvh_delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (lbvh_delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  int j = i + l * d;
+  int dnode = lbvh_delta(keys, n, i, j);
+  int s = 0, t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (lbvh_delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  int gamma = i + s * d + (d < 0 ? d : 0);
+  int first = i < j ? i : j, last = i < j ? j : i;
+  uint32_t cl, cr;
+  if (first == gamma) { cl = (uint32_t)gamma | NCX_LEAF_BIT; parent[(n - 1) + gamma] = (uint32_t)i; }
+  else { cl = (uint32_t)gamma; parent[gamma] = (uint32_t)i; }
+  if (last == gamma + 1) { cr = (uint32_t)(gamma + 1) | NCX_LEAF_BIT; parent[(n - 1) + gamma + 1] = (uint32_t)i | NCX_LEAF_BIT; }
+  else { cr = (uint32_t)(gamma + 1); parent[gamma + 1] = (uint32_t)i | NCX_LEAF_BIT; }
+  // NCX_LEAF_BIT on a parent entry marks "right child of".
+  nodes[i].child_l = cl;
+  nodes[i].child_r = cr;
+  nodes[i].last_l = (uint32_t)gamma;
+  nodes[i].last_r = (uint32_t)last;
+}
+
+__global__ void k_zero_u32(uint32_t* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+int fill_u32(ncx_ctx* ctx, uint32_t* p, size_t n, uint32_t v) {
+  if (n == 0) return NCX_OK;
+  hipLaunchKernelGGL(k_fill_u32, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, p, n, v);
+  NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
+}
+
+__device__ __forceinline__ void store_slot(LNode* nd, bool right, const float b[6]) {
+  float* f = (float*)nd;
+  int o = right ? 6 : 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) __hip_atomic_store(&f[o + k], b[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(256) void k_refit(const float* __restrict__ minx, const float* __restrict__ miny,
+                                                const float* __restrict__ minz, const float* __restrict__ maxx,
+                                                const float* __restrict__ maxy, const float* __restrict__ maxz,
+                                                const uint32_t* __restrict__ order, int n, LNode* nodes,
+                                                const uint32_t* __restrict__ parent, uint32_t* flags, float expand) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  uint32_t p = order[k];
+  float b[6] = {minx[p], miny[p], minz[p], maxx[p], maxy[p], maxz[p]};
+  if (expand > 0.f) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      b[d] -= expand * (1.f + fabsf(b[d]));
+      b[3 + d] += expand * (1.f + fabsf(b[3 + d]));
+    }
+  }
+  uint32_t pe = parent[(n - 1) + k];
+  for (;;) {
+    uint32_t par = pe & ~NCX_LEAF_BIT;
+    bool right = (pe & NCX_LEAF_BIT) != 0;
+    store_slot(&nodes[par], right, b);
+    uint32_t old = __hip_atomic_fetch_add(&flags[par], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;
+    // Second arrival: both child slots are final; merge them.
+    const float* f = (const float*)&nodes[par];
+    float s[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) s[q] = __hip_atomic_load(&f[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      b[d] = fminf(s[d], s[6 + d]);
+      b[3 + d] = fmaxf(s[3 + d], s[9 + d]);
+    }
+    if (par == 0) return;
+    pe = parent[par];
+  }
+}
+
+int lbvh_build(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], size_t n, float expand) {
+  int rc;
+  NCX_REQUIRE(n < (size_t)0x7FFFFFFF, "lbvh_build: too many primitives");
+  bvh.n = n;
+  if (n == 0) return NCX_OK;
+  if ((rc = bvh.order.reserve(n * 4))) return rc;
+  if ((rc = bvh.keys.reserve(n * 4))) return rc;
+  if ((rc = bvh.bounds.reserve(64))) return rc;
+  if ((rc = bvh.nodes.reserve((n > 1 ? n - 1 : 1) * sizeof(LNode)))) return rc;
+  if ((rc = bvh.parent.reserve((2 * n) * 4))) return rc;
+  if ((rc = bvh.flags.reserve(n * 4))) return rc;
+  uint32_t* b = bvh.bounds.as<uint32_t>();
+  hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, ctx->stream, b);
+  unsigned gb = ceil_div(n, 256);
+  if (gb > 1024) gb = 1024;
+  hipLaunchKernelGGL(k_centroid_bounds, dim3(gb), dim3(256), 0, ctx->stream, box[0], box[1], box[2], box[3], box[4],
+                     box[5], n, b);
+  hipLaunchKernelGGL(k_morton, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, box[0], box[1], box[2], box[3],
+                     box[4], box[5], n, b, bvh.keys.as<uint32_t>(), bvh.order.as<uint32_t>());
+  NCX_HIP_CHECK(hipGetLastError());
+  if ((rc = radix_sort_u32(ctx, bvh.keys.as<uint32_t>(), bvh.order.as<uint32_t>(), n, 0, 30, bvh.tmp_keys,
+                           bvh.tmp_vals, bvh.hist, bvh.scan_tmp)))
+    return rc;
+  if (n == 1) return NCX_OK;
+  hipLaunchKernelGGL(k_hierarchy, dim3(ceil_div(n - 1, 256)), dim3(256), 0, ctx->stream, bvh.keys.as<uint32_t>(),
+                     (int)n, bvh.nodes.as<LNode>(), bvh.parent.as<uint32_t>());
+  hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, bvh.flags.as<uint32_t>(), n);
+  hipLaunchKernelGGL(k_refit, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, box[0], box[1], box[2], box[3],
+                     box[4], box[5], bvh.order.as<uint32_t>(), (int)n, bvh.nodes.as<LNode>(),
+                     bvh.parent.as<uint32_t>(), bvh.flags.as<uint32_t>(), expand);
+  NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
+}
+
+// Refit only (same leaf order and topology), used when boxes move but the proxy set is unchanged.
+int lbvh_refit(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], float expand) {
+  size_t n = bvh.n;
+  if (n <= 1) return NCX_OK;
+  hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, bvh.flags.as<uint32_t>(), n);
+  hipLaunchKernelGGL(k_refit, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, box[0], box[1], box[2], box[3],
+                     box[4], box[5], bvh.order.as<uint32_t>(), (int)n, bvh.nodes.as<LNode>(),
+                     bvh.parent.as<uint32_t>(), bvh.flags.as<uint32_t>(), expand);
+  NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
+}

@@ -1,0 +1,178 @@
+// ncx_internal.h -- shared host/device definitions of libncx (gfx950 only).
+//
+// Device arithmetic follows ncollide3d's operation order with no FMA
+// contraction (the library is built with -ffp-contract=off):
+//   dot(a,b)   = (a.x*b.x + a.y*b.y) + a.z*b.z
+//   cross(a,b) = (a.y*b.z - a.z*b.y, a.z*b.x - a.x*b.z, a.x*b.y - a.y*b.x)
+//   normalize  = v / sqrt(dot(v,v))   (IEEE division and square root)
+//   q * v      = t = 2 (q.xyz x v); (t q.w + q.xyz x t) + v
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+#include "../../include/ncx.h"
+
+#define NCX_FLT_EPS 1.1920929e-07f
+#define NCX_FLT_MAX 3.40282347e+38f
+
+// ------------------------------------------------------------------ errors
+void ncx_set_error(const std::string& msg);
+#define NCX_HIP_CHECK(expr)                                                                \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess) {                                                                \
+      ncx_set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                    \
+      return NCX_E_HIP;                                                                    \
+    }                                                                                      \
+  } while (0)
+#define NCX_REQUIRE(cond, msg)            \
+  do {                                    \
+    if (!(cond)) {                        \
+      ncx_set_error(msg);                 \
+      return NCX_E_INVALID;               \
+    }                                     \
+  } while (0)
+
+struct ncx_ctx {
+  int device;
+  hipStream_t stream;
+  int num_cus;
+};
+
+// Device buffer owned by the library (grows, never shrinks).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int reserve(size_t nbytes) {
+    if (nbytes <= bytes) return NCX_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t nb = nbytes < 256 ? 256 : nbytes;
+    hipError_t e = hipMalloc(&p, nb);
+    if (e != hipSuccess) {
+      ncx_set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+      return NCX_E_HIP;
+    }
+    bytes = nb;
+    return NCX_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+// ------------------------------------------------------------------ device math
+struct f3 {
+  float x, y, z;
+};
+__host__ __device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__host__ __device__ __forceinline__ f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ __forceinline__ f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ __forceinline__ f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+__host__ __device__ __forceinline__ f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ __forceinline__ f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+__host__ __device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__host__ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__host__ __device__ __forceinline__ float norm_sq(f3 a) { return dot(a, a); }
+__device__ __forceinline__ float norm(f3 a) { return __builtin_sqrtf(norm_sq(a)); }
+__device__ __forceinline__ f3 normalize(f3 a) { return a / norm(a); }
+// nalgebra Unit::try_new_and_get: fails when norm <= min_norm.
+__device__ __forceinline__ bool try_normalize(f3 v, float min_norm, f3* out, float* n_out) {
+  float n = norm(v);
+  if (n <= min_norm) return false;
+  *out = v / n;
+  *n_out = n;
+  return true;
+}
+__device__ __forceinline__ float comp(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+__device__ __forceinline__ void set_comp(f3& v, int i, float s) {
+  if (i == 0) v.x = s; else if (i == 1) v.y = s; else v.z = s;
+}
+
+struct quat {
+  float i, j, k, w;
+};
+__device__ __forceinline__ f3 qrot(quat q, f3 v) {
+  f3 qv = f3{q.i, q.j, q.k};
+  f3 t = cross(qv, v) * 2.0f;
+  f3 c = cross(qv, t);
+  return (t * q.w + c) + v;
+}
+__device__ __forceinline__ f3 qrot_inv(quat q, f3 v) { return qrot(quat{-q.i, -q.j, -q.k, q.w}, v); }
+
+struct iso {
+  f3 t;
+  quat q;
+};
+__device__ __forceinline__ f3 xform_pt(const iso& m, f3 p) { return qrot(m.q, p) + m.t; }
+__device__ __forceinline__ f3 xform_vec(const iso& m, f3 v) { return qrot(m.q, v); }
+__device__ __forceinline__ f3 inv_xform_pt(const iso& m, f3 p) { return qrot_inv(m.q, p - m.t); }
+__device__ __forceinline__ f3 inv_xform_vec(const iso& m, f3 v) { return qrot_inv(m.q, v); }
+__device__ __forceinline__ iso load_pose(const ncx_pose* poses, uint32_t i) {
+  const float4* p = (const float4*)(poses + i);
+  float4 a = p[0], b = p[1];
+  iso m;
+  m.t = f3{a.x, a.y, a.z};
+  m.q = quat{b.x, b.y, b.z, b.w};
+  return m;
+}
+__device__ __forceinline__ iso iso_identity() {
+  iso m;
+  m.t = f3{0.f, 0.f, 0.f};
+  m.q = quat{0.f, 0.f, 0.f, 1.f};
+  return m;
+}
+
+// ------------------------------------------------------------------ LBVH (shared by broad phase and ray casting)
+// Internal node, 64 bytes: the boxes of both children inline so one node visit is
+// four 16-byte loads.  child_l/child_r: internal index, or (leaf position | LEAF_BIT).
+// last_l/last_r: the largest sorted leaf position under each child.
+#define NCX_LEAF_BIT 0x80000000u
+// In LBVH::parent entries the same bit means "right child of".
+#define NCX_RIGHT_BIT 0x80000000u
+struct alignas(16) LNode {
+  float lminx, lminy, lminz, lmaxx;
+  float lmaxy, lmaxz, rminx, rminy;
+  float rminz, rmaxx, rmaxy, rmaxz;
+  uint32_t child_l, child_r, last_l, last_r;
+};
+
+struct LBVH {
+  size_t n = 0;          // number of leaves
+  DevBuf nodes;          // LNode[n-1]
+  DevBuf parent;         // uint32 parent of internal nodes [n-1] then leaves [n]; side bit 31
+  DevBuf order;          // uint32 sorted position -> primitive index
+  DevBuf keys;           // Morton codes (sorted)
+  DevBuf tmp_keys, tmp_vals, hist, flags, bounds, scan_tmp;
+  void release() {
+    nodes.release(); parent.release(); order.release(); keys.release();
+    tmp_keys.release(); tmp_vals.release(); hist.release(); flags.release(); bounds.release(); scan_tmp.release();
+  }
+};
+
+// Builds the hierarchy over n boxes given as device SoA.  `expand` > 0 grows every
+// stored box outward by expand * (1 + |coordinate|) (used for conservative ray culling).
+int lbvh_build(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], size_t n, float expand);
+// Recomputes every node box for new primitive boxes, keeping order and topology.
+int lbvh_refit(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], float expand);
+
+// Fills n uint32 with v on the context stream (a kernel, so it is safe inside graph capture).
+int fill_u32(ncx_ctx* ctx, uint32_t* p, size_t n, uint32_t v);
+// Device-wide exclusive scan of n uint32 (in != out); returns total through *total_dev (device) if non-null.
+int scan_exclusive_u32(ncx_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n, DevBuf& tmp, uint32_t* total_dev);
+// Stable LSD radix sort of (key, val) uint32 pairs on bits [begin_bit, end_bit).
+int radix_sort_u32(ncx_ctx* ctx, uint32_t* keys, uint32_t* vals, size_t n, int begin_bit, int end_bit,
+                   DevBuf& tmp_keys, DevBuf& tmp_vals, DevBuf& hist, DevBuf& scan_tmp);
+// Stable sort of 64-bit keys (given as hi/lo uint32 arrays) with an optional uint32 payload.
+int radix_sort_pairs64(ncx_ctx* ctx, uint32_t* hi, uint32_t* lo, size_t n, int hi_bits, int lo_bits, DevBuf& t1,
+                       DevBuf& t2, DevBuf& t3, DevBuf& hist, DevBuf& scan_tmp);
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }

@@ -1,0 +1,1338 @@
+// ncx_narrow.hip -- batched convex contact / distance (GJK + EPA) on the GPU.
+//
+// One group of G lanes (G = 4..64, a power of two inside a 64-wide wavefront)
+// handles one pair at a time; the grid loops over pairs.  Every lane of a group
+// runs the same GJK / EPA control flow on the same values (results are
+// identical across the group), and the lanes split the vertex scans of
+// ConvexHull support functions -- the dominant cost -- with an argmax reduction
+// that reproduces point_cloud_support_point_id's first-maximum rule
+// (src/utils/point_cloud_support_point.rs:6-24).  EPA's growing polytope lives
+// in a per-group arena in device memory; a pair whose polytope outgrows the
+// arena is re-run by the host in a second launch with a larger arena.
+//
+// All arithmetic restates the reference in f32 with its operation order:
+//   GJK      src/query/algorithms/gjk.rs:76-177
+//   simplex  src/query/algorithms/voronoi_simplex3.rs:49-280
+//   EPA      src/query/algorithms/epa3.rs:219-454
+//   dispatch src/query/contact/contact_shape_shape.rs:10-48,
+//            src/query/distance/distance.rs:9-33
+#include "ncx_internal.h"
+
+#define SHAPE_ORIGIN 4u
+#define GJK_S_INTERSECTION 0
+#define GJK_S_CLOSEST 1
+#define GJK_S_NOINTER 3
+#define GJK_S_NAN 4
+#define GJK_S_CAP 5  // iteration cap reached (NoIntersection(+x) in the reference)
+
+struct ncx_shapes {
+  ncx_ctx* ctx;
+  size_t n = 0, npts = 0;
+  DevBuf shapes;      // ncx_shape[n]
+  DevBuf hx, hy, hz;  // hull pool
+  DevBuf arena;       // EPA arenas
+  DevBuf ovf;         // overflow counter + list
+};
+
+struct ShapeD {
+  uint32_t type, off, cnt;
+  float a, b, c;
+};
+
+__device__ __forceinline__ ShapeD load_shape(const ncx_shape* __restrict__ s, uint32_t i) {
+  const uint4* p = (const uint4*)(s + i);
+  uint4 u = p[0];
+  uint4 v = p[1];
+  ShapeD g;
+  g.type = u.x;
+  g.off = u.y;
+  g.cnt = u.z;
+  g.a = __uint_as_float(u.w);
+  g.b = __uint_as_float(v.x);
+  g.c = __uint_as_float(v.y);
+  return g;
+}
+
+struct Pool {
+  const float* __restrict__ x;
+  const float* __restrict__ y;
+  const float* __restrict__ z;
+};
+
+// First index of the maximum of dot(p_i, dir) over the hull, split over G lanes.
+template <int G>
+__device__ __forceinline__ uint32_t hull_support_id(const Pool& P, uint32_t off, uint32_t cnt, f3 dir, int gl) {
+  float best = 0.f;
+  uint32_t bi = 0xFFFFFFFFu;
+  for (uint32_t v = gl; v < cnt; v += G) {
+    f3 p = f3{P.x[off + v], P.y[off + v], P.z[off + v]};
+    float d = dot(p, dir);
+    if (bi == 0xFFFFFFFFu || d > best) {
+      best = d;
+      bi = v;
+    }
+  }
+#pragma unroll
+  for (int o = G / 2; o >= 1; o >>= 1) {
+    float ob = __shfl_xor(best, o, 64);
+    uint32_t oi = __shfl_xor(bi, o, 64);
+    bool take = oi != 0xFFFFFFFFu && (bi == 0xFFFFFFFFu || ob > best || (ob == best && oi < bi));
+    if (take) {
+      best = ob;
+      bi = oi;
+    }
+  }
+  return bi == 0xFFFFFFFFu ? 0u : bi;
+}
+
+// SupportMap::support_point (src/shape/support_map.rs:26-29) and the overrides of
+// Ball (ball.rs:31-38), Cuboid (cuboid.rs:135-146), Capsule (capsule.rs:72-84),
+// ConvexHull (convex.rs:418-423), ConstantOrigin (special_support_maps.rs:9-13).
+template <int G>
+__device__ __forceinline__ f3 support(const ShapeD& g, const iso& m, f3 dir, const Pool& P, int gl) {
+  switch (g.type) {
+    case NCX_BALL: {
+      f3 u = normalize(dir);
+      return m.t + u * g.a;
+    }
+    case NCX_CUBOID: {
+      f3 ld = inv_xform_vec(m, dir);
+      f3 p = f3{copysignf(g.a, ld.x), copysignf(g.b, ld.y), copysignf(g.c, ld.z)};
+      return xform_pt(m, p);
+    }
+    case NCX_CAPSULE: {
+      f3 ld = inv_xform_vec(m, dir);
+      f3 u = normalize(ld);
+      f3 res = f3{0.f, copysignf(g.a, u.y), 0.f};
+      return xform_pt(m, res + u * g.b);
+    }
+    case NCX_CONVEX_HULL: {
+      f3 ld = inv_xform_vec(m, dir);
+      uint32_t id = hull_support_id<G>(P, g.off, g.cnt, ld, gl);
+      f3 p = f3{P.x[g.off + id], P.y[g.off + id], P.z[g.off + id]};
+      return xform_pt(m, p);
+    }
+    default:
+      return f3{0.f, 0.f, 0.f};
+  }
+}
+
+struct CSO {
+  f3 p, o1, o2;
+};
+
+struct Ctx2 {
+  iso m1, m2;
+  ShapeD g1, g2;
+};
+
+template <int G>
+__device__ __forceinline__ CSO cso_from(const Ctx2& c, f3 dir, const Pool& P, int gl) {
+  CSO r;
+  r.o1 = support<G>(c.g1, c.m1, dir, P, gl);
+  r.o2 = support<G>(c.g2, c.m2, -dir, P, gl);
+  r.p = r.o1 - r.o2;
+  return r;
+}
+
+// ------------------------------------------------------------------ Voronoi projections of the origin
+#define LOC_VERTEX 0
+#define LOC_EDGE 1
+#define LOC_FACE 2
+#define LOC_SOLID 3
+struct Loc {
+  int kind, id;
+  float bc0, bc1, bc2;
+};
+
+// point_segment.rs:52-91 (identity isometry, point = origin)
+__device__ __forceinline__ f3 proj_segment(f3 a, f3 b, Loc& loc) {
+  iso m = iso_identity();
+  f3 ls = inv_xform_pt(m, f3{0.f, 0.f, 0.f});
+  f3 ab = b - a, ap = ls - a;
+  float ab_ap = dot(ab, ap), sqnab = norm_sq(ab);
+  if (ab_ap <= 0.f) { loc.kind = LOC_VERTEX; loc.id = 0; return xform_pt(m, a); }
+  if (ab_ap >= sqnab) { loc.kind = LOC_VERTEX; loc.id = 1; return xform_pt(m, b); }
+  float u = ab_ap / sqnab;
+  loc.kind = LOC_EDGE; loc.id = 0; loc.bc0 = 1.f - u; loc.bc1 = u;
+  return xform_pt(m, a + ab * u);
+}
+
+// point_triangle.rs:61-309, dim3, solid = true
+__device__ __forceinline__ f3 proj_triangle(f3 a, f3 b, f3 c, Loc& loc) {
+  iso m = iso_identity();
+  f3 o = f3{0.f, 0.f, 0.f};
+  f3 p = inv_xform_pt(m, o);
+  f3 ab = b - a, ac = c - a, ap = p - a;
+  float ab_ap = dot(ab, ap), ac_ap = dot(ac, ap);
+  if (ab_ap <= 0.f && ac_ap <= 0.f) { loc.kind = LOC_VERTEX; loc.id = 0; return xform_pt(m, a); }
+  f3 bp = p - b;
+  float ab_bp = dot(ab, bp), ac_bp = dot(ac, bp);
+  if (ab_bp >= 0.f && ac_bp <= ab_bp) { loc.kind = LOC_VERTEX; loc.id = 1; return xform_pt(m, b); }
+  f3 cp = p - c;
+  float ab_cp = dot(ab, cp), ac_cp = dot(ac, cp);
+  if (ac_cp >= 0.f && ab_cp <= ac_cp) { loc.kind = LOC_VERTEX; loc.id = 2; return xform_pt(m, c); }
+  f3 bc = c - b;
+  f3 n = cross(ab, ac);
+  float vc = dot(n, cross(ab, ap));
+  if (vc < 0.f && ab_ap >= 0.f && ab_bp <= 0.f) {
+    float v = ab_ap / norm_sq(ab);
+    loc.kind = LOC_EDGE; loc.id = 0; loc.bc0 = 1.f - v; loc.bc1 = v;
+    return xform_pt(m, a + ab * v);
+  }
+  float vb = -dot(n, cross(ac, cp));
+  if (vb < 0.f && ac_ap >= 0.f && ac_cp <= 0.f) {
+    float w = ac_ap / norm_sq(ac);
+    loc.kind = LOC_EDGE; loc.id = 2; loc.bc0 = 1.f - w; loc.bc1 = w;
+    return xform_pt(m, a + ac * w);
+  }
+  float va = dot(n, cross(bc, bp));
+  if (va < 0.f && ac_bp - ab_bp >= 0.f && ab_cp - ac_cp >= 0.f) {
+    float w = dot(bc, bp) / norm_sq(bc);
+    loc.kind = LOC_EDGE; loc.id = 1; loc.bc0 = 1.f - w; loc.bc1 = w;
+    return xform_pt(m, b + bc * w);
+  }
+  int clockwise = dot(n, ap) >= 0.f ? 0 : 1;
+  if (va + vb + vc != 0.f) {
+    float denom = 1.f / (va + vb + vc);
+    float v = vb * denom, w = vc * denom;
+    loc.kind = LOC_FACE; loc.id = clockwise; loc.bc0 = 1.f - v - w; loc.bc1 = v; loc.bc2 = w;
+    return xform_pt(m, a + ab * v + ac * w);
+  }
+  loc.kind = LOC_SOLID; loc.id = 0;
+  return o;
+}
+
+// check_edge of point_tetrahedron.rs:106-152
+__device__ __forceinline__ bool tet_edge(int i, const iso& m, f3 a, f3 nabc, f3 nabd, f3 ap, f3 ab, float ap_ab,
+                                         float bp_ab, float& dabc, float& dabd, Loc& loc, f3& out) {
+  float ab_ab = ap_ab - bp_ab;
+  f3 ap_x_ab = cross(ap, ab);
+  dabc = dot(ap_x_ab, nabc);
+  dabd = dot(ap_x_ab, nabd);
+  if (ab_ab != 0.f && dabc >= 0.f && dabd >= 0.f && ap_ab >= 0.f && ap_ab <= ab_ab) {
+    float u = ap_ab / ab_ab;
+    loc.kind = LOC_EDGE; loc.id = i; loc.bc0 = 1.f - u; loc.bc1 = u;
+    out = xform_pt(m, a + ab * u);
+    return true;
+  }
+  return false;
+}
+
+// check_face of point_tetrahedron.rs:251-306
+__device__ __forceinline__ bool tet_face(int i, const iso& m, f3 a, f3 b, f3 c, f3 ap, f3 bp, f3 cp, f3 ab, f3 ac,
+                                         f3 ad, float d1, float d2, float d3, Loc& loc, f3& out) {
+  if (d1 < 0.f && d2 < 0.f && d3 < 0.f) {
+    f3 n = cross(ab, ac);
+    if (dot(n, ad) * dot(n, ap) < 0.f) {
+      f3 nn;
+      float nl;
+      if (!try_normalize(n, NCX_FLT_EPS, &nn, &nl)) return false;
+      float vc = dot(nn, cross(ap, bp));
+      float va = dot(nn, cross(bp, cp));
+      float vb = dot(nn, cross(cp, ap));
+      float denom = va + vb + vc;
+      float inv_denom = 1.f / denom;
+      float b0 = va * inv_denom, b1 = vb * inv_denom, b2 = vc * inv_denom;
+      loc.kind = LOC_FACE; loc.id = i; loc.bc0 = b0; loc.bc1 = b1; loc.bc2 = b2;
+      out = xform_pt(m, a * b0 + b * b1 + c * b2);
+      return true;
+    }
+  }
+  return false;
+}
+
+// point_tetrahedron.rs:35-353, solid = true
+__device__ f3 proj_tetrahedron(f3 A, f3 B, f3 C, f3 D, Loc& loc) {
+  iso m = iso_identity();
+  f3 p = inv_xform_pt(m, f3{0.f, 0.f, 0.f});
+  f3 ab = B - A, ac = C - A, ad = D - A, ap = p - A;
+  float ap_ab = dot(ap, ab), ap_ac = dot(ap, ac), ap_ad = dot(ap, ad);
+  if (ap_ab <= 0.f && ap_ac <= 0.f && ap_ad <= 0.f) { loc.kind = LOC_VERTEX; loc.id = 0; return xform_pt(m, A); }
+  f3 bc = C - B, bd = D - B, bp = p - B;
+  float bp_bc = dot(bp, bc), bp_bd = dot(bp, bd), bp_ab = dot(bp, ab);
+  if (bp_bc <= 0.f && bp_bd <= 0.f && bp_ab >= 0.f) { loc.kind = LOC_VERTEX; loc.id = 1; return xform_pt(m, B); }
+  f3 cd = D - C, cp = p - C;
+  float cp_ac = dot(cp, ac), cp_bc = dot(cp, bc), cp_cd = dot(cp, cd);
+  if (cp_cd <= 0.f && cp_bc >= 0.f && cp_ac >= 0.f) { loc.kind = LOC_VERTEX; loc.id = 2; return xform_pt(m, C); }
+  f3 dp = p - D;
+  float dp_cd = dot(dp, cd), dp_bd = dot(dp, bd), dp_ad = dot(dp, ad);
+  if (dp_ad >= 0.f && dp_bd >= 0.f && dp_cd >= 0.f) { loc.kind = LOC_VERTEX; loc.id = 3; return xform_pt(m, D); }
+  f3 out;
+  float dabc, dabd, dacd, dacb, dadb, dadc, dbca, dbcd, dbdc, dbda, dcda, dcdb;
+  f3 nabc = cross(ab, ac), nabd = cross(ab, ad);
+  if (tet_edge(0, m, A, nabc, nabd, ap, ab, ap_ab, bp_ab, dabc, dabd, loc, out)) return out;
+  f3 nacd = cross(ac, ad);
+  if (tet_edge(1, m, A, nacd, -nabc, ap, ac, ap_ac, cp_ac, dacd, dacb, loc, out)) return out;
+  if (tet_edge(2, m, A, -nabd, -nacd, ap, ad, ap_ad, dp_ad, dadb, dadc, loc, out)) return out;
+  f3 nbcd = cross(bc, bd);
+  if (tet_edge(3, m, B, nabc, nbcd, bp, bc, bp_bc, cp_bc, dbca, dbcd, loc, out)) return out;
+  if (tet_edge(4, m, B, -nbcd, nabd, bp, bd, bp_bd, dp_bd, dbdc, dbda, loc, out)) return out;
+  if (tet_edge(5, m, C, nacd, nbcd, cp, cd, cp_cd, dp_cd, dcda, dcdb, loc, out)) return out;
+  if (tet_face(0, m, A, B, C, ap, bp, cp, ab, ac, ad, dabc, dbca, dacb, loc, out)) return out;
+  if (tet_face(1, m, A, B, D, ap, bp, dp, ab, ad, ac, dadb, dabd, dbda, loc, out)) return out;
+  if (tet_face(2, m, A, C, D, ap, cp, dp, ac, ad, ab, dacd, dcda, dadc, loc, out)) return out;
+  if (tet_face(3, m, B, C, D, bp, cp, dp, bc, bd, -ab, dbcd, dcdb, dbdc, loc, out)) return out;
+  loc.kind = LOC_SOLID; loc.id = 0;
+  return xform_pt(m, p);
+}
+
+// ------------------------------------------------------------------ VoronoiSimplex (voronoi_simplex3.rs:12-352)
+struct Simplex {
+  CSO v[4];
+  int pv[4];
+  float proj[3], pproj[3];
+  int dim, pdim;
+};
+
+__device__ __forceinline__ void sx_swap(Simplex& s, int i, int j) {
+  CSO t = s.v[i]; s.v[i] = s.v[j]; s.v[j] = t;
+  int u = s.pv[i]; s.pv[i] = s.pv[j]; s.pv[j] = u;
+}
+
+__device__ __forceinline__ void sx_init(Simplex& s) {
+  for (int i = 0; i < 4; ++i) s.pv[i] = i;
+  s.proj[0] = s.proj[1] = s.proj[2] = 0.f;
+  s.pproj[0] = s.pproj[1] = s.pproj[2] = 0.f;
+  s.dim = 0;
+  s.pdim = 0;
+}
+
+__device__ __forceinline__ void sx_reset(Simplex& s, const CSO& p) {
+  s.dim = 0;
+  s.pdim = 0;
+  s.v[0] = p;
+}
+
+__device__ __forceinline__ bool sx_add(Simplex& s, const CSO& pt) {
+  const float tol = NCX_FLT_EPS * 10.f;
+  s.pdim = s.dim;
+  s.pproj[0] = s.proj[0]; s.pproj[1] = s.proj[1]; s.pproj[2] = s.proj[2];
+  s.pv[0] = 0; s.pv[1] = 1; s.pv[2] = 2; s.pv[3] = 3;
+  if (s.dim == 0) {
+    if (norm_sq(s.v[0].p - pt.p) < tol) return false;
+  } else if (s.dim == 1) {
+    f3 ab = s.v[1].p - s.v[0].p, ac = pt.p - s.v[0].p;
+    if (norm_sq(cross(ab, ac)) < tol) return false;
+  } else {
+    f3 ab = s.v[1].p - s.v[0].p, ac = s.v[2].p - s.v[0].p, ap = pt.p - s.v[0].p;
+    f3 n = normalize(cross(ab, ac));
+    if (fabsf(dot(n, ap)) < tol) return false;
+  }
+  s.dim += 1;
+  s.v[s.dim] = pt;
+  return true;
+}
+
+__device__ f3 sx_project_reduce(Simplex& s) {
+  if (s.dim == 0) {
+    s.proj[0] = 1.f;
+    return s.v[0].p;
+  } else if (s.dim == 1) {
+    Loc loc;
+    f3 pr = proj_segment(s.v[0].p, s.v[1].p, loc);
+    if (loc.kind == LOC_VERTEX && loc.id == 0) { s.proj[0] = 1.f; s.dim = 0; }
+    else if (loc.kind == LOC_VERTEX) { sx_swap(s, 0, 1); s.proj[0] = 1.f; s.dim = 0; }
+    else { s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; }
+    return pr;
+  } else if (s.dim == 2) {
+    Loc loc;
+    f3 pr = proj_triangle(s.v[0].p, s.v[1].p, s.v[2].p, loc);
+    if (loc.kind == LOC_VERTEX) { sx_swap(s, 0, loc.id); s.proj[0] = 1.f; s.dim = 0; }
+    else if (loc.kind == LOC_EDGE) {
+      if (loc.id == 0) { s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; }
+      else if (loc.id == 1) { sx_swap(s, 0, 2); s.proj[0] = loc.bc1; s.proj[1] = loc.bc0; }
+      else { sx_swap(s, 1, 2); s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; }
+      s.dim = 1;
+    } else if (loc.kind == LOC_FACE) {
+      s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; s.proj[2] = loc.bc2;
+    }
+    return pr;
+  } else {
+    Loc loc;
+    f3 pr = proj_tetrahedron(s.v[0].p, s.v[1].p, s.v[2].p, s.v[3].p, loc);
+    if (loc.kind == LOC_VERTEX) { sx_swap(s, 0, loc.id); s.proj[0] = 1.f; s.dim = 0; }
+    else if (loc.kind == LOC_EDGE) {
+      switch (loc.id) {
+        case 1: sx_swap(s, 1, 2); break;
+        case 2: sx_swap(s, 1, 3); break;
+        case 3: sx_swap(s, 0, 2); break;
+        case 4: sx_swap(s, 0, 3); break;
+        case 5: sx_swap(s, 0, 2); sx_swap(s, 1, 3); break;
+        default: break;
+      }
+      if (loc.id == 3 || loc.id == 4) { s.proj[0] = loc.bc1; s.proj[1] = loc.bc0; }
+      else { s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; }
+      s.dim = 1;
+    } else if (loc.kind == LOC_FACE) {
+      switch (loc.id) {
+        case 0: s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; s.proj[2] = loc.bc2; break;
+        case 1: s.v[2] = s.v[3]; s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; s.proj[2] = loc.bc2; break;
+        case 2: s.v[1] = s.v[3]; s.proj[0] = loc.bc0; s.proj[1] = loc.bc2; s.proj[2] = loc.bc1; break;
+        default: s.v[0] = s.v[3]; s.proj[0] = loc.bc2; s.proj[1] = loc.bc0; s.proj[2] = loc.bc1; break;
+      }
+      s.dim = 2;
+    }
+    return pr;
+  }
+}
+
+// gjk.rs:367-388
+__device__ __forceinline__ void gjk_result(const Simplex& s, bool prev, f3& p1, f3& p2) {
+  f3 r1 = f3{0.f, 0.f, 0.f}, r2 = f3{0.f, 0.f, 0.f};
+  if (prev) {
+    for (int i = 0; i < s.pdim + 1; ++i) {
+      float c = s.pproj[i];
+      const CSO& pt = s.v[s.pv[i]];
+      r1 = r1 + pt.o1 * c;
+      r2 = r2 + pt.o2 * c;
+    }
+  } else {
+    for (int i = 0; i < s.dim + 1; ++i) {
+      float c = s.proj[i];
+      const CSO& pt = s.v[i];
+      r1 = r1 + pt.o1 * c;
+      r2 = r2 + pt.o2 * c;
+    }
+  }
+  p1 = r1;
+  p2 = r2;
+}
+
+struct GJKOut {
+  int status;
+  f3 p1, p2, dir;
+  int iters;
+};
+
+// gjk::closest_points (gjk.rs:76-177), exact_dist = true.
+template <int G>
+__device__ GJKOut gjk(const Ctx2& c, float max_dist, Simplex& s, const Pool& P, int gl, int max_iter) {
+  GJKOut r;
+  r.iters = 0;
+  const float eps_tol = NCX_FLT_EPS * 10.f;
+  const float eps_rel = __builtin_sqrtf(eps_tol);
+  f3 proj = sx_project_reduce(s);
+  f3 old_dir;
+  {
+    f3 pd;
+    float nl;
+    if (try_normalize(proj, 0.f, &pd, &nl)) old_dir = -pd;
+    else { r.status = GJK_S_INTERSECTION; return r; }
+  }
+  float max_bound = NCX_FLT_MAX;
+  f3 dir;
+  int niter = 0;
+  for (;;) {
+    float old_max_bound = max_bound;
+    {
+      f3 nd;
+      float dist;
+      if (try_normalize(-proj, eps_tol, &nd, &dist)) { dir = nd; max_bound = dist; }
+      else { r.status = GJK_S_INTERSECTION; return r; }
+    }
+    if (max_bound >= old_max_bound) {
+      gjk_result(s, true, r.p1, r.p2);
+      r.dir = old_dir;
+      r.status = GJK_S_CLOSEST;
+      return r;
+    }
+    CSO cso = cso_from<G>(c, dir, P, gl);
+    r.iters += 1;
+    float min_bound = -dot(dir, cso.p);
+    if (!(min_bound == min_bound)) { r.status = GJK_S_NAN; return r; }
+    if (min_bound > max_dist) { r.dir = dir; r.status = GJK_S_NOINTER; return r; }
+    else if (max_bound - min_bound <= eps_rel * max_bound) {
+      gjk_result(s, false, r.p1, r.p2);
+      r.dir = dir;
+      r.status = GJK_S_CLOSEST;
+      return r;
+    }
+    if (!sx_add(s, cso)) {
+      gjk_result(s, false, r.p1, r.p2);
+      r.dir = dir;
+      r.status = GJK_S_CLOSEST;
+      return r;
+    }
+    old_dir = dir;
+    proj = sx_project_reduce(s);
+    if (s.dim == 3) {
+      if (min_bound >= eps_tol) {
+        gjk_result(s, true, r.p1, r.p2);
+        r.dir = old_dir;
+        r.status = GJK_S_CLOSEST;
+        return r;
+      }
+      r.status = GJK_S_INTERSECTION;
+      return r;
+    }
+    niter += 1;
+    if (niter == max_iter) { r.dir = f3{1.f, 0.f, 0.f}; r.status = GJK_S_CAP; return r; }
+  }
+}
+
+// ------------------------------------------------------------------ EPA arena
+struct FaceD {
+  uint32_t pts[3];
+  uint32_t adj[3];
+  float n[3];
+  float bc[3];
+  uint32_t deleted;
+};
+struct HeapE {
+  float neg_dist;
+  uint32_t id;
+};
+struct SilE {
+  uint32_t face, opp;
+};
+
+struct Arena {
+  CSO* verts;
+  FaceD* faces;
+  HeapE* heap;
+  SilE* sil;
+  SilE* stack;
+  uint32_t vcap, fcap;
+};
+
+__host__ __device__ inline size_t arena_bytes(uint32_t vcap, uint32_t fcap) {
+  return (size_t)vcap * sizeof(CSO) + (size_t)fcap * (sizeof(FaceD) + sizeof(HeapE) + 3 * sizeof(SilE));
+}
+
+__device__ __forceinline__ Arena arena_at(uint8_t* base, size_t slot, uint32_t vcap, uint32_t fcap) {
+  uint8_t* p = base + slot * arena_bytes(vcap, fcap);
+  Arena a;
+  a.verts = (CSO*)p;
+  p += (size_t)vcap * sizeof(CSO);
+  a.faces = (FaceD*)p;
+  p += (size_t)fcap * sizeof(FaceD);
+  a.heap = (HeapE*)p;
+  p += (size_t)fcap * sizeof(HeapE);
+  a.sil = (SilE*)p;
+  p += (size_t)fcap * sizeof(SilE);
+  a.stack = (SilE*)p;
+  a.vcap = vcap;
+  a.fcap = fcap;
+  return a;
+}
+
+// Rust std BinaryHeap (max-heap on neg_dist): push = sift_up; pop = swap with the
+// last element, sift_down_to_bottom, sift_up.  Element order is PartialOrd::le.
+__device__ __forceinline__ void heap_sift_up(HeapE* h, uint32_t pos) {
+  HeapE e = h[pos];
+  while (pos > 0) {
+    uint32_t parent = (pos - 1) / 2;
+    if (e.neg_dist <= h[parent].neg_dist) break;
+    h[pos] = h[parent];
+    pos = parent;
+  }
+  h[pos] = e;
+}
+__device__ __forceinline__ bool heap_pop(HeapE* h, uint32_t& len, HeapE& out) {
+  if (len == 0) return false;
+  len -= 1;
+  HeapE item = h[len];
+  if (len > 0) {
+    HeapE top = h[0];
+    h[0] = item;
+    item = top;
+    uint32_t end = len, pos = 0, child = 1;
+    HeapE e = h[0];
+    while (end >= 2 && child <= end - 2) {
+      child += (h[child].neg_dist <= h[child + 1].neg_dist) ? 1u : 0u;
+      h[pos] = h[child];
+      pos = child;
+      child = 2 * pos + 1;
+    }
+    if (child == end - 1) { h[pos] = h[child]; pos = child; }
+    h[pos] = e;
+    heap_sift_up(h, pos);
+  }
+  out = item;
+  return true;
+}
+
+#define EPA_OK 1
+#define EPA_NONE 0
+#define EPA_OVERFLOW 2
+
+// Face::new (epa3.rs:96-115) + new_with_proj (:63-94) + ccw_face_normal (utils/ccw_face_normal.rs:21-27).
+__device__ __forceinline__ bool make_face(const Arena& A, uint32_t p0, uint32_t p1, uint32_t p2, uint32_t a0,
+                                          uint32_t a1, uint32_t a2, FaceD& f) {
+  f3 va = A.verts[p0].p, vb = A.verts[p1].p, vc = A.verts[p2].p;
+  Loc loc;
+  proj_triangle(va, vb, vc, loc);
+  f.pts[0] = p0; f.pts[1] = p1; f.pts[2] = p2;
+  f.adj[0] = a0; f.adj[1] = a1; f.adj[2] = a2;
+  f.deleted = 0;
+  f3 nn;
+  float nl;
+  if (try_normalize(cross(vb - va, vc - va), NCX_FLT_EPS, &nn, &nl)) { f.n[0] = nn.x; f.n[1] = nn.y; f.n[2] = nn.z; }
+  else { f.n[0] = 0.f; f.n[1] = 0.f; f.n[2] = 0.f; }
+  if (loc.kind == LOC_FACE) {
+    f.bc[0] = loc.bc0; f.bc[1] = loc.bc1; f.bc[2] = loc.bc2;
+    return true;
+  }
+  f.bc[0] = f.bc[1] = f.bc[2] = 0.f;
+  return false;
+}
+
+__device__ __forceinline__ f3 fnormal(const FaceD& f) { return f3{f.n[0], f.n[1], f.n[2]}; }
+
+__device__ __forceinline__ void face_points(const Arena& A, const FaceD& f, f3& p1, f3& p2) {
+  const CSO& a = A.verts[f.pts[0]];
+  const CSO& b = A.verts[f.pts[1]];
+  const CSO& c = A.verts[f.pts[2]];
+  p1 = a.o1 * f.bc[0] + b.o1 * f.bc[1] + c.o1 * f.bc[2];
+  p2 = a.o2 * f.bc[0] + b.o2 * f.bc[1] + c.o2 * f.bc[2];
+}
+
+__device__ __forceinline__ uint32_t next_ccw(const FaceD& f, uint32_t id) {
+  if (f.pts[0] == id) return 1;
+  if (f.pts[1] == id) return 2;
+  return 0;
+}
+
+// utils/triangle.rs:85-102
+__device__ __forceinline__ bool affinely_dependent(f3 p1, f3 p2, f3 p3) {
+  f3 p1p2 = p2 - p1, p1p3 = p3 - p1;
+  float e = NCX_FLT_EPS * 100.f;
+  float x = norm_sq(cross(p1p2, p1p3));
+  float epsilon = e * e;
+  if (x == 0.f) return true;
+  if (isinf(x)) return false;
+  float diff = fabsf(x - 0.f);
+  if (diff <= epsilon) return true;
+  return diff <= fabsf(x) * NCX_FLT_EPS;
+}
+
+// Face::can_be_seen_by (epa3.rs:143-156)
+__device__ __forceinline__ bool can_be_seen(const Arena& A, const FaceD& f, uint32_t point, uint32_t opp) {
+  f3 p0 = A.verts[f.pts[opp]].p;
+  f3 p1 = A.verts[f.pts[(opp + 1) % 3]].p;
+  f3 p2 = A.verts[f.pts[(opp + 2) % 3]].p;
+  f3 pt = A.verts[point].p;
+  return dot(pt - p0, fnormal(f)) >= -(NCX_FLT_EPS * 10.f) || affinely_dependent(p1, p2, pt);
+}
+
+// EPA::closest_points (epa3.rs:219-430).  compute_silhouette's recursion
+// (:432-454) becomes an explicit stack visiting faces in the same order.
+template <int G>
+__device__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const Arena& A, f3& p1, f3& p2, f3& n,
+                   int& iters) {
+  const float eps_tol = NCX_FLT_EPS * 100.f;
+  const float gjk_tol = NCX_FLT_EPS * 10.f;
+  iters = 0;
+  uint32_t nv = 0, nf = 0, hl = 0;
+  for (int i = 0; i < s.dim + 1; ++i) A.verts[nv++] = s.v[i];
+  if (s.dim == 0) {
+    p1 = f3{0.f, 0.f, 0.f};
+    p2 = f3{0.f, 0.f, 0.f};
+    n = f3{0.f, 1.f, 0.f};
+    return EPA_OK;
+  } else if (s.dim == 3) {
+    f3 dp1 = A.verts[1].p - A.verts[0].p;
+    f3 dp2 = A.verts[2].p - A.verts[0].p;
+    f3 dp3 = A.verts[3].p - A.verts[0].p;
+    if (dot(cross(dp1, dp2), dp3) > 0.f) {
+      CSO t = A.verts[1]; A.verts[1] = A.verts[2]; A.verts[2] = t;
+    }
+    const uint32_t PT[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {0, 3, 1}};
+    const uint32_t AD[4][3] = {{3, 1, 2}, {3, 2, 0}, {0, 1, 3}, {2, 1, 0}};
+    bool inside[4];
+    for (int f = 0; f < 4; ++f) {
+      FaceD fd;
+      inside[f] = make_face(A, PT[f][0], PT[f][1], PT[f][2], AD[f][0], AD[f][1], AD[f][2], fd);
+      A.faces[nf++] = fd;
+    }
+    for (int f = 0; f < 4; ++f) {
+      if (inside[f]) {
+        float dist = dot(fnormal(A.faces[f]), A.verts[f].p);
+        if (-dist > gjk_tol) return EPA_NONE;
+        A.heap[hl] = HeapE{-dist, (uint32_t)f};
+        heap_sift_up(A.heap, hl);
+        hl++;
+      }
+    }
+  } else {
+    if (s.dim == 1) {
+      f3 dpt = A.verts[1].p - A.verts[0].p;
+      // nalgebra orthonormal_subspace_basis for one vector; epa3.rs:299-305 keeps a x dpt.
+      f3 a = fabsf(dpt.x) > fabsf(dpt.y) ? f3{dpt.z, 0.f, -dpt.x} : f3{0.f, -dpt.z, dpt.y};
+      a = normalize(a);
+      A.verts[nv++] = cso_from<G>(c, cross(a, dpt), P, gl);
+    }
+    FaceD f1, f2;
+    make_face(A, 0, 1, 2, 1, 1, 1, f1);
+    make_face(A, 0, 2, 1, 0, 0, 0, f2);
+    A.faces[nf++] = f1;
+    A.faces[nf++] = f2;
+    A.heap[hl] = HeapE{0.f, 0u}; heap_sift_up(A.heap, hl); hl++;
+    A.heap[hl] = HeapE{0.f, 1u}; heap_sift_up(A.heap, hl); hl++;
+  }
+  if (hl == 0) return EPA_NONE;
+  int niter = 0;
+  float max_dist = NCX_FLT_MAX;
+  HeapE best = A.heap[0];
+  HeapE fid;
+  while (heap_pop(A.heap, hl, fid)) {
+    FaceD face = A.faces[fid.id];
+    if (face.deleted) continue;
+    if (nv >= A.vcap) return EPA_OVERFLOW;
+    CSO cso = cso_from<G>(c, fnormal(face), P, gl);
+    iters += 1;
+    uint32_t sp_id = nv;
+    A.verts[nv++] = cso;
+    float cand = dot(cso.p, fnormal(face));
+    if (cand < max_dist) { best = fid; max_dist = cand; }
+    float curr_dist = -fid.neg_dist;
+    if (max_dist - curr_dist < eps_tol) {
+      FaceD bf = A.faces[best.id];
+      face_points(A, bf, p1, p2);
+      n = fnormal(bf);
+      return EPA_OK;
+    }
+    A.faces[fid.id].deleted = 1;
+    // compute_silhouette for the three neighbours, depth first.
+    uint32_t ns = 0, sp = 0;
+    for (int k = 2; k >= 0; --k) {
+      uint32_t adj = face.adj[k];
+      A.stack[sp++] = SilE{adj, next_ccw(A.faces[adj], face.pts[k])};
+    }
+    while (sp > 0) {
+      SilE e = A.stack[--sp];
+      FaceD& fa = A.faces[e.face];
+      if (fa.deleted) continue;
+      if (!can_be_seen(A, fa, sp_id, e.opp)) {
+        if (ns >= A.fcap) return EPA_OVERFLOW;
+        A.sil[ns++] = e;
+      } else {
+        fa.deleted = 1;
+        uint32_t i1 = (e.opp + 2) % 3, i2 = e.opp;
+        uint32_t adj1 = fa.adj[i1], adj2 = fa.adj[i2];
+        uint32_t o1 = next_ccw(A.faces[adj1], fa.pts[i1]);
+        uint32_t o2 = next_ccw(A.faces[adj2], fa.pts[i2]);
+        if (sp + 2 > 2 * A.fcap) return EPA_OVERFLOW;
+        A.stack[sp++] = SilE{adj2, o2};
+        A.stack[sp++] = SilE{adj1, o1};
+      }
+    }
+    uint32_t first_new = nf;
+    if (ns == 0) return EPA_NONE;
+    for (uint32_t q = 0; q < ns; ++q) {
+      SilE edge = A.sil[q];
+      if (!A.faces[edge.face].deleted) {
+        if (nf >= A.fcap || hl >= A.fcap) return EPA_OVERFLOW;
+        uint32_t new_id = nf;
+        uint32_t pt1 = A.faces[edge.face].pts[(edge.opp + 2) % 3];
+        uint32_t pt2 = A.faces[edge.face].pts[(edge.opp + 1) % 3];
+        FaceD nfc;
+        bool inside = make_face(A, pt1, pt2, sp_id, edge.face, new_id + 1, new_id - 1, nfc);
+        A.faces[edge.face].adj[(edge.opp + 1) % 3] = new_id;
+        A.faces[nf++] = nfc;
+        if (inside) {
+          float dist = dot(fnormal(nfc), A.verts[nfc.pts[0]].p);
+          if (dist < curr_dist) {
+            face_points(A, face, p1, p2);
+            n = fnormal(face);
+            return EPA_OK;
+          }
+          if (-dist > gjk_tol) return EPA_NONE;
+          A.heap[hl] = HeapE{-dist, new_id};
+          heap_sift_up(A.heap, hl);
+          hl++;
+        }
+      }
+    }
+    if (first_new == nf) return EPA_NONE;
+    A.faces[first_new].adj[2] = nf - 1;
+    A.faces[nf - 1].adj[1] = first_new;
+    niter += 1;
+    if (niter > 10000) return EPA_NONE;
+  }
+  FaceD bf = A.faces[best.id];
+  face_points(A, bf, p1, p2);
+  n = fnormal(bf);
+  return EPA_OK;
+}
+
+// ------------------------------------------------------------------ per-pair queries
+struct PairOut {
+  int has;  // 1 = Some(contact); 0 = None; 2 = arena overflow (re-run)
+  f3 w1, w2, n;
+  float depth;
+  int gjk_iters;
+};
+
+// contact_support_map_support_map_with_params (contact_support_map_support_map.rs:38-88)
+template <int G>
+__device__ PairOut contact_sm(const Ctx2& c, float prediction, const f3* init_dir, const Pool& P, int gl,
+                              const Arena& A, int max_iter) {
+  PairOut o;
+  o.has = 0;
+  o.gjk_iters = 0;
+  f3 dir;
+  if (init_dir) dir = *init_dir;
+  else {
+    float nl;
+    if (!try_normalize(c.m2.t - c.m1.t, NCX_FLT_EPS, &dir, &nl)) dir = f3{1.f, 0.f, 0.f};
+  }
+  Simplex s;
+  sx_init(s);
+  sx_reset(s, cso_from<G>(c, dir, P, gl));
+  GJKOut r = gjk<G>(c, prediction, s, P, gl, max_iter);
+  o.gjk_iters = r.iters;
+  if (r.status == GJK_S_CAP && max_iter < 10000) {
+    o.has = 3;  // iteration cap of a warm-started run: the caller re-runs plain GJK
+    return o;
+  }
+  f3 w1, w2, n;
+  if (r.status == GJK_S_CLOSEST) { w1 = r.p1; w2 = r.p2; n = r.dir; }
+  else if (r.status == GJK_S_INTERSECTION) {
+    int it;
+    int e = epa<G>(c, s, P, gl, A, w1, w2, n, it);
+    if (e == EPA_OVERFLOW) { o.has = 2; return o; }
+    if (e != EPA_OK) return o;
+  } else {
+    return o;
+  }
+  o.has = 1;
+  o.w1 = w1; o.w2 = w2; o.n = n;
+  o.depth = -dot(n, w2 - w1);  // Contact::new_wo_depth (contact.rs:43-46)
+  return o;
+}
+
+// AABB::local_point_projection + project_point_with_feature for a cuboid (point_aabb.rs:9-135), solid = false.
+__device__ f3 project_cuboid(const iso& m, f3 h, f3 pt, bool& inside_out, int& fkind, int& fid) {
+  f3 mins = f3{0.f + (-h.x), 0.f + (-h.y), 0.f + (-h.z)};
+  f3 maxs = f3{0.f + h.x, 0.f + h.y, 0.f + h.z};
+  f3 ls = inv_xform_pt(m, pt);
+  f3 mins_pt = mins - ls, pt_maxs = ls - maxs;
+  f3 shift = f3{fmaxf(mins_pt.x, 0.f), fmaxf(mins_pt.y, 0.f), fmaxf(mins_pt.z, 0.f)} -
+             f3{fmaxf(pt_maxs.x, 0.f), fmaxf(pt_maxs.y, 0.f), fmaxf(pt_maxs.z, 0.f)};
+  bool inside = shift.x == 0.f && shift.y == 0.f && shift.z == 0.f;
+  if (inside) {
+    float best = -NCX_FLT_MAX;
+    bool is_mins = false;
+    int best_id = 0;
+    for (int i = 0; i < 3; ++i) {
+      float a = comp(mins_pt, i), b = comp(pt_maxs, i);
+      if (a < b) { if (b > best) { best_id = i; is_mins = false; best = b; } }
+      else if (a > best) { best_id = i; is_mins = true; best = a; }
+    }
+    shift = f3{0.f, 0.f, 0.f};
+    set_comp(shift, best_id, is_mins ? best : -best);
+  }
+  f3 pl = ls + shift;
+  inside_out = inside;
+  int nzero = 0, last_zero = 0, last_nz = 0;
+  for (int i = 0; i < 3; ++i) { if (comp(shift, i) == 0.f) { nzero++; last_zero = i; } else last_nz = i; }
+  f3 center = (mins + maxs) * 0.5f;
+  if (nzero == 3) {
+    fkind = 3; fid = 0;
+    for (int i = 0; i < 3; ++i) {
+      if (comp(pl, i) > comp(maxs, i) - NCX_FLT_EPS) { fkind = 0; fid = i; break; }
+      if (comp(pl, i) <= comp(mins, i) + NCX_FLT_EPS) { fkind = 0; fid = i + 3; break; }
+    }
+  } else if (nzero == 2) {
+    fkind = 0;
+    fid = comp(pl, last_nz) < comp(center, last_nz) ? last_nz + 3 : last_nz;
+  } else {
+    int id = 0;
+    for (int i = 0; i < 3; ++i) if (comp(pl, i) < comp(center, i)) id |= 1 << i;
+    if (nzero == 0) { fkind = 2; fid = id; } else { fkind = 1; fid = (id << 2) | last_zero; }
+  }
+  return xform_pt(m, pl);
+}
+
+// Cuboid::feature_normal (cuboid.rs:511-561), local frame as in the reference.
+__device__ f3 cuboid_feature_normal(int fkind, int fid) {
+  f3 d = f3{0.f, 0.f, 0.f};
+  if (fkind == 0) {
+    if (fid < 3) set_comp(d, fid, 1.f); else set_comp(d, fid - 3, -1.f);
+    return d;
+  } else if (fkind == 1) {
+    int edge = fid & 3, f1 = (edge + 1) % 3, f2 = (edge + 2) % 3, signs = fid >> 2;
+    set_comp(d, f1, (signs & (1 << f1)) ? -1.f : 1.f);
+    set_comp(d, f2, (signs & (1 << f2)) ? -1.f : 1.f);
+    return normalize(d);
+  }
+  for (int i = 0; i < 3; ++i) set_comp(d, i, (fid & (1 << i)) ? -1.f : 1.f);
+  return normalize(d);
+}
+
+// contact_ball_convex_polyhedron (contact_ball_convex_polyhedron.rs:11-58), Cuboid or ConvexHull.
+template <int G>
+__device__ PairOut contact_ball_convex(f3 c1, float r, const iso& m2, const ShapeD& g2, float prediction,
+                                       const Pool& P, int gl, const Arena& A) {
+  PairOut o;
+  o.has = 0;
+  o.gjk_iters = 0;
+  bool inside;
+  int fkind = 3, fid = 0;
+  f3 world2;
+  if (g2.type == NCX_CUBOID) {
+    world2 = project_cuboid(m2, f3{g2.a, g2.b, g2.c}, c1, inside, fkind, fid);
+  } else {
+    // point_projection_on_support_map (point_support_map.rs:15-53), solid = false
+    Ctx2 c;
+    c.m1 = m2;
+    c.m1.t = (-c1) + m2.t;
+    c.g1 = g2;
+    c.m2 = iso_identity();
+    c.g2.type = SHAPE_ORIGIN;
+    f3 dir;
+    float nl;
+    if (!try_normalize(-c.m1.t, NCX_FLT_EPS, &dir, &nl)) dir = f3{1.f, 0.f, 0.f};
+    Simplex s;
+    sx_init(s);
+    sx_reset(s, cso_from<G>(c, dir, P, gl));
+    GJKOut r2 = gjk<G>(c, NCX_FLT_MAX, s, P, gl, 10000);
+    o.gjk_iters = r2.iters;
+    if (r2.status == GJK_S_CLOSEST) {
+      inside = false;
+      world2 = r2.p1 + c1;
+    } else {
+      f3 p1, p2, nn;
+      int it;
+      int e = r2.status == GJK_S_INTERSECTION ? epa<G>(c, s, P, gl, A, p1, p2, nn, it) : EPA_NONE;
+      if (e == EPA_OVERFLOW) { o.has = 2; return o; }
+      inside = true;
+      world2 = e == EPA_OK ? p1 + c1 : c1;
+    }
+  }
+  f3 dpt = world2 - c1;
+  f3 dir;
+  float dist, depth;
+  f3 normal;
+  if (try_normalize(dpt, NCX_FLT_EPS, &dir, &dist)) {
+    if (inside) { depth = dist + r; normal = -dir; }
+    else { depth = -dist + r; normal = dir; }
+  } else {
+    if (g2.type != NCX_CUBOID || fkind == 3) return o;
+    depth = r;
+    normal = -cuboid_feature_normal(fkind, fid);
+  }
+  if (depth >= -prediction) {
+    o.has = 1;
+    o.w1 = c1 + normal * r;
+    o.w2 = world2;
+    o.n = normal;
+    o.depth = depth;
+  }
+  return o;
+}
+
+// query::contact dispatch (contact_shape_shape.rs:10-48).
+template <int G>
+__device__ PairOut contact_pair(const Ctx2& c, float prediction, const f3* init_dir, const Pool& P, int gl,
+                                const Arena& A, int max_iter) {
+  bool b1 = c.g1.type == NCX_BALL, b2 = c.g2.type == NCX_BALL;
+  bool poly1 = c.g1.type == NCX_CUBOID || c.g1.type == NCX_CONVEX_HULL;
+  bool poly2 = c.g2.type == NCX_CUBOID || c.g2.type == NCX_CONVEX_HULL;
+  if (b1 && b2) {
+    // contact_ball_ball (contact_ball_ball.rs:8-38)
+    PairOut o;
+    o.has = 0;
+    o.gjk_iters = 0;
+    float r1 = c.g1.a, r2 = c.g2.a;
+    f3 delta = c.m2.t - c.m1.t;
+    float d2 = norm_sq(delta);
+    float sr = r1 + r2, sre = sr + prediction;
+    if (d2 < sre * sre) {
+      f3 nrm = d2 != 0.f ? normalize(delta) : f3{1.f, 0.f, 0.f};
+      o.has = 1;
+      o.w1 = c.m1.t + nrm * r1;
+      o.w2 = c.m2.t + nrm * (-r2);
+      o.n = nrm;
+      o.depth = sr - __builtin_sqrtf(d2);
+    }
+    return o;
+  }
+  if (b1 && poly2) return contact_ball_convex<G>(c.m1.t, c.g1.a, c.m2, c.g2, prediction, P, gl, A);
+  if (poly1 && b2) {
+    PairOut o = contact_ball_convex<G>(c.m2.t, c.g2.a, c.m1, c.g1, prediction, P, gl, A);
+    if (o.has == 1) {  // Contact::flip (contact.rs:52-55)
+      f3 t = o.w1; o.w1 = o.w2; o.w2 = t;
+      o.n = -o.n;
+    }
+    return o;
+  }
+  return contact_sm<G>(c, prediction, init_dir, P, gl, A, max_iter);
+}
+
+struct KArgs {
+  const ncx_shape* shapes;
+  Pool P;
+  const ncx_pose* poses;
+  const uint32_t* ia;
+  const uint32_t* ib;
+  const uint32_t* list;  // optional indirection (re-runs)
+  size_t npairs;
+  float prediction;
+  const float* init_dir;
+  int max_iter;
+  uint8_t* has;
+  float* w1;
+  float* w2;
+  float* nrm;
+  float* depth;
+  int32_t* iters;
+  float* dist;
+  uint8_t* arena;
+  uint32_t vcap, fcap;
+  uint32_t* ovf;  // [0] = count, [1..] = list of pair indices
+  uint32_t ovf_cap;
+  int distance_mode;
+  uint32_t nshapes;
+  uint32_t* bad_index;  // set to 1 when a pair names a shape >= nshapes
+};
+
+template <int G>
+__global__ __launch_bounds__(256) void k_pairs(KArgs a) {
+  const int gl = threadIdx.x & (G - 1);
+  const size_t group = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const size_t ngroups = ((size_t)gridDim.x * blockDim.x) / G;
+  Arena A = arena_at(a.arena, group, a.vcap, a.fcap);
+  for (size_t q = group; q < a.npairs; q += ngroups) {
+    size_t k = a.list ? a.list[q] : q;
+    uint32_t i1 = a.ia[k], i2 = a.ib[k];
+    if (i1 >= a.nshapes || i2 >= a.nshapes) {
+      if (gl == 0) atomicOr(a.bad_index, 1u);
+      continue;
+    }
+    Ctx2 c;
+    c.g1 = load_shape(a.shapes, i1);
+    c.g2 = load_shape(a.shapes, i2);
+    c.m1 = load_pose(a.poses, i1);
+    c.m2 = load_pose(a.poses, i2);
+    if (a.distance_mode) {
+      float d = 0.f;
+      int it = 0;
+      bool ovf = false;
+      if (c.g1.type == NCX_BALL && c.g2.type == NCX_BALL) {
+        // distance_ball_ball (distance_ball_ball.rs:7-24)
+        f3 delta = c.m2.t - c.m1.t;
+        float d2 = norm_sq(delta);
+        float sr = c.g1.a + c.g2.a;
+        d = d2 <= sr * sr ? 0.f : __builtin_sqrtf(d2) - sr;
+      } else {
+        // distance_support_map_support_map_with_params (distance_support_map_support_map.rs:25-53)
+        f3 dir;
+        float nl;
+        if (!try_normalize(c.m1.t - c.m2.t, NCX_FLT_EPS, &dir, &nl)) dir = f3{1.f, 0.f, 0.f};
+        Simplex s;
+        sx_init(s);
+        sx_reset(s, cso_from<G>(c, dir, a.P, gl));
+        GJKOut r = gjk<G>(c, NCX_FLT_MAX, s, a.P, gl, 10000);
+        it = r.iters;
+        d = r.status == GJK_S_CLOSEST ? norm(r.p1 - r.p2) : 0.f;
+      }
+      (void)ovf;
+      if (gl == 0) {
+        a.dist[k] = d;
+        if (a.iters) a.iters[k] = it;
+      }
+      continue;
+    }
+    f3 idir;
+    const f3* ip = nullptr;
+    if (a.init_dir) {
+      idir = f3{a.init_dir[3 * k], a.init_dir[3 * k + 1], a.init_dir[3 * k + 2]};
+      ip = &idir;
+    }
+    PairOut o = contact_pair<G>(c, a.prediction, ip, a.P, gl, A, a.max_iter);
+    int extra_iters = 0;
+    if (o.has == 3) {
+      // Warm start hit its iteration cap: plain GJK (init_dir = None) judges the pair.
+      extra_iters = o.gjk_iters;
+      o = contact_pair<G>(c, a.prediction, nullptr, a.P, gl, A, 10000);
+    }
+    if (o.has == 2) {
+      if (gl == 0) {
+        uint32_t slot = atomicAdd(&a.ovf[0], 1u);
+        if (slot < a.ovf_cap) a.ovf[1 + slot] = (uint32_t)k;
+      }
+      continue;
+    }
+    if (gl == 0) {
+      a.has[k] = (uint8_t)o.has;
+      if (o.has == 1) {
+        a.w1[3 * k] = o.w1.x; a.w1[3 * k + 1] = o.w1.y; a.w1[3 * k + 2] = o.w1.z;
+        a.w2[3 * k] = o.w2.x; a.w2[3 * k + 1] = o.w2.y; a.w2[3 * k + 2] = o.w2.z;
+        a.nrm[3 * k] = o.n.x; a.nrm[3 * k + 1] = o.n.y; a.nrm[3 * k + 2] = o.n.z;
+        a.depth[k] = o.depth;
+      } else {
+        a.w1[3 * k] = 0.f; a.w1[3 * k + 1] = 0.f; a.w1[3 * k + 2] = 0.f;
+        a.w2[3 * k] = 0.f; a.w2[3 * k + 1] = 0.f; a.w2[3 * k + 2] = 0.f;
+        a.nrm[3 * k] = 0.f; a.nrm[3 * k + 1] = 0.f; a.nrm[3 * k + 2] = 0.f;
+        a.depth[k] = 0.f;
+      }
+      if (a.iters) a.iters[k] = o.gjk_iters + extra_iters;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ host side
+extern "C" int ncx_shapes_create(ncx_ctx* ctx, const ncx_shape* shapes, size_t n, const float* hx, const float* hy,
+                                 const float* hz, size_t npts, ncx_shapes** out) {
+  NCX_REQUIRE(ctx && out && (shapes || n == 0), "ncx_shapes_create: null argument");
+  for (size_t i = 0; i < n; ++i) {
+    const ncx_shape& s = shapes[i];
+    NCX_REQUIRE(s.type <= NCX_CONVEX_HULL, "ncx_shapes_create: unknown shape type");
+    if (s.type == NCX_CONVEX_HULL)
+      NCX_REQUIRE(s.hull_count > 0 && (size_t)s.hull_offset + s.hull_count <= npts,
+                  "ncx_shapes_create: hull range outside the point pool");
+  }
+  ncx_shapes* S = new ncx_shapes;
+  S->ctx = ctx;
+  S->n = n;
+  S->npts = npts;
+  int rc;
+  if ((rc = S->shapes.reserve(n * sizeof(ncx_shape) + 64)) || (rc = S->hx.reserve(npts * 4 + 4)) ||
+      (rc = S->hy.reserve(npts * 4 + 4)) || (rc = S->hz.reserve(npts * 4 + 4)) || (rc = S->ovf.reserve(4 * 4096))) {
+    delete S;
+    return rc;
+  }
+  if (n) NCX_HIP_CHECK(hipMemcpyAsync(S->shapes.p, shapes, n * sizeof(ncx_shape), hipMemcpyHostToDevice, ctx->stream));
+  if (npts) {
+    NCX_HIP_CHECK(hipMemcpyAsync(S->hx.p, hx, npts * 4, hipMemcpyHostToDevice, ctx->stream));
+    NCX_HIP_CHECK(hipMemcpyAsync(S->hy.p, hy, npts * 4, hipMemcpyHostToDevice, ctx->stream));
+    NCX_HIP_CHECK(hipMemcpyAsync(S->hz.p, hz, npts * 4, hipMemcpyHostToDevice, ctx->stream));
+  }
+  NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  *out = S;
+  return NCX_OK;
+}
+
+extern "C" int ncx_shapes_destroy(ncx_shapes* s) {
+  if (!s) return NCX_OK;
+  s->shapes.release(); s->hx.release(); s->hy.release(); s->hz.release(); s->arena.release(); s->ovf.release();
+  delete s;
+  return NCX_OK;
+}
+
+extern "C" int ncx_shapes_count(ncx_shapes* s, size_t* n) {
+  NCX_REQUIRE(s && n, "ncx_shapes_count: null argument");
+  *n = s->n;
+  return NCX_OK;
+}
+
+// Group width per launch: the lanes that share one pair's support scans.
+static int pick_group(ncx_shapes* s) {
+  (void)s;
+  return 16;
+}
+
+template <int G>
+static int launch_pairs(ncx_shapes* S, KArgs a, size_t npairs, unsigned blocks) {
+  a.npairs = npairs;
+  hipLaunchKernelGGL(k_pairs<G>, dim3(blocks), dim3(256), 0, S->ctx->stream, a);
+  NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
+}
+
+static int run_pairs(ncx_shapes* S, KArgs a, size_t npairs) {
+  if (npairs == 0) return NCX_OK;
+  ncx_ctx* ctx = S->ctx;
+  int G = pick_group(S);
+  // Persistent-style grid: enough groups to fill the chip, each with its own arena.
+  unsigned blocks = (unsigned)ctx->num_cus * 8;
+  size_t groups_needed = (npairs + 0) * G / 256 + 1;
+  if (blocks > groups_needed) blocks = (unsigned)groups_needed;
+  if (blocks == 0) blocks = 1;
+  uint32_t vcap = 160, fcap = 320;
+  size_t ngroups = (size_t)blocks * 256 / G;
+  int rc;
+  if ((rc = S->arena.reserve(ngroups * arena_bytes(vcap, fcap)))) return rc;
+  const uint32_t ovf_cap = 4090;
+  if ((rc = fill_u32(ctx, S->ovf.as<uint32_t>(), 2, 0u))) return rc;
+  if ((rc = fill_u32(ctx, S->ovf.as<uint32_t>() + 4095, 1, 0u))) return rc;
+  a.arena = S->arena.as<uint8_t>();
+  a.nshapes = (uint32_t)S->n;
+  a.bad_index = S->ovf.as<uint32_t>() + 4095;
+  a.vcap = vcap;
+  a.fcap = fcap;
+  a.ovf = S->ovf.as<uint32_t>();
+  a.ovf_cap = ovf_cap;
+  a.list = nullptr;
+  switch (G) {
+    case 4: rc = launch_pairs<4>(S, a, npairs, blocks); break;
+    case 8: rc = launch_pairs<8>(S, a, npairs, blocks); break;
+    case 32: rc = launch_pairs<32>(S, a, npairs, blocks); break;
+    case 64: rc = launch_pairs<64>(S, a, npairs, blocks); break;
+    default: rc = launch_pairs<16>(S, a, npairs, blocks); break;
+  }
+  if (rc) return rc;
+  // Re-run pairs whose EPA polytope outgrew the arena, with larger arenas.
+  uint32_t novf = 0, bad = 0;
+  NCX_HIP_CHECK(hipMemcpyAsync(&novf, S->ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  NCX_HIP_CHECK(hipMemcpyAsync(&bad, S->ovf.as<uint32_t>() + 4095, 4, hipMemcpyDeviceToHost, ctx->stream));
+  NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  NCX_REQUIRE(bad == 0, "contact batch: a pair index is not a shape of the set");
+  while (novf > 0) {
+    NCX_REQUIRE(novf <= ovf_cap, "contact batch: too many EPA arena overflows");
+    std::vector<uint32_t> list(novf);
+    NCX_HIP_CHECK(hipMemcpy(list.data(), S->ovf.as<uint32_t>() + 1, novf * 4, hipMemcpyDeviceToHost));
+    vcap *= 8;
+    fcap *= 8;
+    NCX_REQUIRE(vcap <= 20480, "contact batch: EPA polytope exceeds 20480 vertices");
+    DevBuf lst, big;
+    if ((rc = lst.reserve(novf * 4)) || (rc = big.reserve((size_t)novf * arena_bytes(vcap, fcap)))) return rc;
+    NCX_HIP_CHECK(hipMemcpyAsync(lst.p, list.data(), novf * 4, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = fill_u32(ctx, S->ovf.as<uint32_t>(), 1, 0u))) return rc;
+    KArgs b = a;
+    b.list = lst.as<uint32_t>();
+    b.arena = big.as<uint8_t>();
+    b.vcap = vcap;
+    b.fcap = fcap;
+    unsigned rb = ceil_div((size_t)novf * 64, 256);
+    rc = launch_pairs<64>(S, b, novf, rb);
+    if (rc) return rc;
+    NCX_HIP_CHECK(hipMemcpyAsync(&novf, S->ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    lst.release();
+    big.release();
+  }
+  return NCX_OK;
+}
+
+extern "C" int ncx_contact_batch(ncx_shapes* s, const ncx_pose* poses, const uint32_t* ia, const uint32_t* ib,
+                                 size_t npairs, float prediction, const float* init_dir, uint8_t* has, float* w1,
+                                 float* w2, float* nrm, float* depth, int32_t* iters) {
+  NCX_REQUIRE(s && poses && ia && ib && has && w1 && w2 && nrm && depth, "ncx_contact_batch: null argument");
+  KArgs a = {};
+  a.shapes = s->shapes.as<ncx_shape>();
+  a.P = Pool{s->hx.as<float>(), s->hy.as<float>(), s->hz.as<float>()};
+  a.poses = poses;
+  a.ia = ia;
+  a.ib = ib;
+  a.prediction = prediction;
+  a.init_dir = init_dir;
+  a.max_iter = 10000;
+  a.has = has;
+  a.w1 = w1;
+  a.w2 = w2;
+  a.nrm = nrm;
+  a.depth = depth;
+  a.iters = iters;
+  a.distance_mode = 0;
+  return run_pairs(s, a, npairs);
+}
+
+// Warm-started variant used by the learned initial direction: GJK from init_dir is capped at
+// max_iter iterations, after which plain GJK (init_dir = None) recomputes the pair.
+extern "C" int ncx_contact_batch_capped(ncx_shapes* s, const ncx_pose* poses, const uint32_t* ia, const uint32_t* ib,
+                                        size_t npairs, float prediction, const float* init_dir, int32_t max_iter,
+                                        uint8_t* has, float* w1, float* w2, float* nrm, float* depth,
+                                        int32_t* iters) {
+  NCX_REQUIRE(s && poses && ia && ib && has && w1 && w2 && nrm && depth, "ncx_contact_batch_capped: null argument");
+  NCX_REQUIRE(max_iter >= 1 && max_iter <= 10000, "ncx_contact_batch_capped: max_iter out of range");
+  KArgs a = {};
+  a.shapes = s->shapes.as<ncx_shape>();
+  a.P = Pool{s->hx.as<float>(), s->hy.as<float>(), s->hz.as<float>()};
+  a.poses = poses;
+  a.ia = ia;
+  a.ib = ib;
+  a.prediction = prediction;
+  a.init_dir = init_dir;
+  a.max_iter = init_dir ? max_iter : 10000;
+  a.has = has;
+  a.w1 = w1;
+  a.w2 = w2;
+  a.nrm = nrm;
+  a.depth = depth;
+  a.iters = iters;
+  a.distance_mode = 0;
+  return run_pairs(s, a, npairs);
+}
+
+extern "C" int ncx_distance_batch(ncx_shapes* s, const ncx_pose* poses, const uint32_t* ia, const uint32_t* ib,
+                                  size_t npairs, float* dist, int32_t* iters) {
+  NCX_REQUIRE(s && poses && ia && ib && dist, "ncx_distance_batch: null argument");
+  KArgs a = {};
+  a.shapes = s->shapes.as<ncx_shape>();
+  a.P = Pool{s->hx.as<float>(), s->hy.as<float>(), s->hz.as<float>()};
+  a.poses = poses;
+  a.ia = ia;
+  a.ib = ib;
+  a.max_iter = 10000;
+  a.dist = dist;
+  a.iters = iters;
+  a.distance_mode = 1;
+  return run_pairs(s, a, npairs);
+}
+
+// ------------------------------------------------------------------ shape AABBs
+__global__ __launch_bounds__(256) void k_shape_aabbs(const ncx_shape* __restrict__ shapes, Pool P,
+                                                      const ncx_pose* __restrict__ poses, size_t n, float margin,
+                                                      float* b0, float* b1, float* b2, float* b3, float* b4,
+                                                      float* b5) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ShapeD g = load_shape(shapes, (uint32_t)i);
+  iso m = load_pose(poses, (uint32_t)i);
+  f3 mn, mx;
+  switch (g.type) {
+    case NCX_BALL:  // ball_aabb (aabb_ball.rs:8-13)
+      mn = m.t + f3{-g.a, -g.a, -g.a};
+      mx = m.t + f3{g.a, g.a, g.a};
+      break;
+    case NCX_CUBOID: {  // center +- |R| h (aabb_cuboid.rs:7-14, isometry_ops.rs:43-45)
+      float qi = m.q.i, qj = m.q.j, qk = m.q.k, qw = m.q.w;
+      float ww = qw * qw, ii = qi * qi, jj = qj * qj, kk = qk * qk;
+      float ij = qi * qj * 2.f, wk = qw * qk * 2.f, wj = qw * qj * 2.f;
+      float ik = qi * qk * 2.f, jk = qj * qk * 2.f, wi = qw * qi * 2.f;
+      float M[3][3] = {{ww + ii - jj - kk, ij - wk, wj + ik},
+                       {wk + ij, ww - ii + jj - kk, jk - wi},
+                       {ik - wj, wi + jk, ww - ii - jj + kk}};
+      f3 ws;
+      ws.x = (fabsf(M[0][0]) * g.a + fabsf(M[0][1]) * g.b) + fabsf(M[0][2]) * g.c;
+      ws.y = (fabsf(M[1][0]) * g.a + fabsf(M[1][1]) * g.b) + fabsf(M[1][2]) * g.c;
+      ws.z = (fabsf(M[2][0]) * g.a + fabsf(M[2][1]) * g.b) + fabsf(M[2][2]) * g.c;
+      mn = m.t - ws;
+      mx = m.t + ws;
+      break;
+    }
+    case NCX_CAPSULE: {  // support_map_aabb (aabb_utils.rs:9-31)
+      for (int d = 0; d < 3; ++d) {
+        f3 e = f3{0.f, 0.f, 0.f};
+        set_comp(e, d, 1.f);
+        set_comp(mx, d, comp(support<1>(g, m, e, P, 0), d));
+        set_comp(e, d, -1.f);
+        set_comp(mn, d, comp(support<1>(g, m, e, P, 0), d));
+      }
+      break;
+    }
+    default: {  // point_cloud_aabb (aabb_utils.rs:59-79)
+      f3 w = xform_pt(m, f3{P.x[g.off], P.y[g.off], P.z[g.off]});
+      mn = w;
+      mx = w;
+      for (uint32_t v = 1; v < g.cnt; ++v) {
+        f3 p = xform_pt(m, f3{P.x[g.off + v], P.y[g.off + v], P.z[g.off + v]});
+        mn = f3{fminf(mn.x, p.x), fminf(mn.y, p.y), fminf(mn.z, p.z)};
+        mx = f3{fmaxf(mx.x, p.x), fmaxf(mx.y, p.y), fmaxf(mx.z, p.z)};
+      }
+      break;
+    }
+  }
+  if (margin != 0.f) {  // AABB::loosened (aabb.rs:190-199)
+    mn = mn + f3{-margin, -margin, -margin};
+    mx = mx + f3{margin, margin, margin};
+  }
+  b0[i] = mn.x; b1[i] = mn.y; b2[i] = mn.z;
+  b3[i] = mx.x; b4[i] = mx.y; b5[i] = mx.z;
+}
+
+extern "C" int ncx_shapes_aabbs_dev(ncx_shapes* s, const ncx_pose* poses, float margin, float* const out[6]) {
+  NCX_REQUIRE(s && poses && out, "ncx_shapes_aabbs_dev: null argument");
+  NCX_REQUIRE(margin >= 0.f, "ncx_shapes_aabbs_dev: the loosening margin must be positive");
+  for (int d = 0; d < 6; ++d) NCX_REQUIRE(out[d], "ncx_shapes_aabbs_dev: null output");
+  if (s->n == 0) return NCX_OK;
+  hipLaunchKernelGGL(k_shape_aabbs, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->ctx->stream,
+                     s->shapes.as<ncx_shape>(), Pool{s->hx.as<float>(), s->hy.as<float>(), s->hz.as<float>()}, poses,
+                     s->n, margin, out[0], out[1], out[2], out[3], out[4], out[5]);
+  NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
+}

@@ -1,0 +1,273 @@
+"""GPU parity: the HIP kernels of libncx.so (called through the C ABI) against the
+oracle on the same seeded inputs.
+
+Bars: pair sets, ray-hit features and AABBs bit-exact; contact/distance within
+1e-5 relative + 1e-6 absolute, normals within 1e-4 rad (the tolerance of the
+north-star contract), and the fraction of bit-identical results is reported.
+"""
+import numpy as np
+import pytest
+import torch
+
+import ncollide_amd as ncx
+from ncollide_amd import workloads
+from ncollide_amd.math import to_oracle_poses
+
+pytestmark = pytest.mark.gpu
+
+REL, ABS, ANG = 1e-5, 1e-6, 1e-4
+
+
+def oracle_shapes(orc, records, pool, dtype=np.float32):
+    params = np.stack([records["p0"], records["p1"], records["p2"]], 1)
+    return orc.ShapeSet(records["type"], params, records["hull_offset"], records["hull_count"], pool, dtype=dtype)
+
+
+def assert_contacts_match(gpu, ref, label):
+    has_g = gpu["has_contact"].cpu().numpy()
+    has_r = ref["has"]
+    mism = np.nonzero(has_g != has_r)[0]
+    assert len(mism) == 0, f"{label}: has_contact differs at {mism[:10]}"
+    idx = np.nonzero(has_r)[0]
+    dg = gpu["depth"].cpu().numpy()[idx].astype(np.float64)
+    dr = ref["depth"][idx].astype(np.float64)
+    err = np.abs(dg - dr)
+    ok = err <= REL * np.abs(dr) + ABS
+    assert ok.all(), f"{label}: depth off at {idx[~ok][:10]}: gpu {dg[~ok][:5]} ref {dr[~ok][:5]}"
+    ng = gpu["normal"].cpu().numpy()[idx].astype(np.float64)
+    nr = ref["normal"][idx].astype(np.float64)
+    cosang = np.clip(np.sum(ng * nr, 1) / (np.linalg.norm(ng, axis=1) * np.linalg.norm(nr, axis=1) + 1e-300), -1, 1)
+    ang = np.arccos(cosang)
+    assert np.all(ang <= ANG), f"{label}: normal off by {ang.max()} rad"
+    exact = np.mean(dg == dr) if len(idx) else 1.0
+    print(f"{label}: {len(idx)} contacts, bit-exact depth fraction {exact:.4f}, max |d depth| {err.max() if len(err) else 0:.3g}")
+    return exact
+
+
+# ------------------------------------------------------------------ ray casting
+def test_raycast_heightfield_matches_oracle(ctx, orc):
+    pts, tri = workloads.heightfield_mesh(seed=2)
+    o, d = workloads.heightfield_rays(seed=2, n=20000)
+    mesh = ncx.TriMesh(pts, tri, ctx=ctx)
+    toi, nrm, feat = mesh.toi_and_normal_with_ray(o, d, 100.0)
+    toi, nrm, feat = toi.cpu().numpy(), nrm.cpu().numpy(), feat.cpu().numpy().view(np.uint32)
+    om = orc.TriMesh(pts, tri)
+    sel = np.arange(0, 20000, 4)
+    bt, bn, bf, nh = om.brute(o[sel], d[sel], 100.0, nthreads=8)
+    assert np.array_equal(feat[sel], bf), "feature ids differ from the brute-force oracle"
+    assert np.array_equal(toi[sel], bt)
+    assert np.array_equal(nrm[sel], bn)
+    rt, rn, rf, _ = om.cast(o, d, 100.0, nthreads=8)  # the reference's best-first search over its BVT
+    diff = np.nonzero(feat != rf)[0]
+    for k in diff:  # only exact toi ties may resolve to another triangle
+        assert toi[k] == rt[k]
+    assert np.array_equal(toi, rt)
+    print(f"rays: {len(o)}, hits {np.mean(feat != 0xFFFFFFFF):.3f}, tie-resolved differences {len(diff)}")
+
+
+def test_raycast_edge_cases(ctx, orc):
+    # single triangle, rays along axes (zero direction components), back faces, max_toi cut
+    pts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    tri = np.array([[0, 1, 2]], np.uint32)
+    o = np.array([[0.25, 0.25, 1], [0.25, 0.25, -1], [0.25, 0.25, 1], [2, 2, 1], [0.0, 0.0, 1], [0.25, 0.25, 5]],
+                 np.float32)
+    d = np.array([[0, 0, -1], [0, 0, 1], [0, 0, 1], [0, 0, -1], [0, 0, -1], [0, 0, -1]], np.float32)
+    mesh = ncx.TriMesh(pts, tri, ctx=ctx)
+    toi, nrm, feat = mesh.toi_and_normal_with_ray(o, d, 2.0)
+    om = orc.TriMesh(pts, tri)
+    rt, rn, rf, _ = om.cast(o, d, 2.0)
+    assert np.array_equal(feat.cpu().numpy().view(np.uint32), rf)
+    assert np.array_equal(toi.cpu().numpy(), rt)
+    f = feat.cpu().numpy().view(np.uint32)
+    assert f[0] == 0 and f[1] == 1 and f[2] == 0xFFFFFFFF and f[3] == 0xFFFFFFFF and f[5] == 0xFFFFFFFF
+
+
+def test_raycast_posed_mesh(ctx, orc):
+    pts, tri = workloads.heightfield_mesh(seed=7, cells=64)
+    rng = np.random.default_rng(5)
+    q = workloads.uniform_quats(rng, 1)[0]
+    t = np.array([1.0, -2.0, 0.5], np.float32)
+    pose8 = np.zeros(8, np.float32)
+    pose8[:3] = t
+    pose8[4:] = q
+    o = rng.uniform(-5, 5, (4000, 3)).astype(np.float32)
+    d = workloads.unit_vectors(rng, 4000).astype(np.float32)
+    mesh = ncx.TriMesh(pts, tri, ctx=ctx)
+    toi, nrm, feat = mesh.toi_and_normal_with_ray(o, d, 50.0, pose=pose8)
+    om = orc.TriMesh(pts, tri)
+    bt, bn, bf, _ = om.brute(o, d, 50.0, pose=np.concatenate([t, q]), nthreads=8)
+    assert np.array_equal(feat.cpu().numpy().view(np.uint32), bf)
+    assert np.array_equal(toi.cpu().numpy(), bt)
+    assert np.array_equal(nrm.cpu().numpy(), bn)
+
+
+# ------------------------------------------------------------------ shape AABBs and broad phase
+def test_shape_aabbs_bit_exact(ctx, orc):
+    w = workloads.full_pipeline(seed=5, n=4096)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    bb = ss.aabbs(poses, margin=0.0).cpu().numpy().T
+    ref = orc.shape_aabbs(oracle_shapes(orc, w["records"], w["pool"]), to_oracle_poses(w["poses"]))
+    assert np.array_equal(bb, ref)
+
+
+def test_broad_phase_plain_pipeline(ctx, orc):
+    w = workloads.plain_pipeline(seed=1)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    bb = ss.aabbs(poses, margin=w["margin"]).cpu().numpy().T
+    bp = ncx.DBVTBroadPhase(w["margin"], ctx=ctx)
+    hs = bp.create_proxies(bb)
+    assert np.array_equal(hs, np.arange(1024))
+    bp.update()
+    ref = orc.BroadPhase(w["margin"])
+    for b in bb:
+        ref.create_proxy(b)
+    ref.update()
+    assert np.array_equal(bp.pairs(), ref.pairs())
+    assert np.array_equal(bp.pairs(), orc.pairs_brute(bb))
+    s, t = bp.events()
+    rs, rt = ref.events()
+    assert np.array_equal(s, rs) and len(t) == 0
+
+
+def test_broad_phase_dbvt_example(ctx):
+    """examples/dbvt_broad_phase3d.rs:21-61."""
+    bp = ncx.DBVTBroadPhase(0.2, ctx=ctx)
+    hs = bp.create_proxies(np.array([[x - .5, y - .5, -.5, x + .5, y + .5, .5]
+                                     for x, y in [(0, 0), (0, .5), (.5, 0), (.5, .5)]], np.float32))
+    bp.update()
+    assert bp.num_interferences() == 6
+    bp.remove(hs[:2])
+    bp.update()
+    assert bp.num_interferences() == 1
+
+
+def test_broad_phase_moving_matches_reference(ctx, orc):
+    """Several steps of deferred_set_bounding_volume + update, with removals and re-created proxies,
+    against the DBVTBroadPhase restatement (pair sets and started/stopped events)."""
+    rng = np.random.default_rng(12)
+    n = 3000
+    c = rng.uniform(0, 30, (n, 3)).astype(np.float32)
+    r = rng.uniform(0.2, 0.8, (n, 1)).astype(np.float32)
+    bb = np.concatenate([c - r, c + r], 1)
+    bp = ncx.DBVTBroadPhase(0.05, ctx=ctx)
+    ref = orc.BroadPhase(0.05)
+    hs = bp.create_proxies(bb)
+    rh = [ref.create_proxy(b) for b in bb]
+    assert np.array_equal(hs, rh)
+    for step in range(6):
+        bp.update()
+        ref.update()
+        assert np.array_equal(bp.pairs(), ref.pairs()), f"step {step}"
+        s, t = bp.events()
+        rs, rt = ref.events()
+        assert np.array_equal(s, rs) and np.array_equal(t, rt), f"events step {step}"
+        c = c + rng.normal(0, 0.1, c.shape).astype(np.float32)
+        bb = np.concatenate([c - r, c + r], 1)
+        alive = np.array(sorted(set(range(n))), np.uint32)
+        bp.deferred_set_bounding_volumes(alive, bb)
+        for h in alive:
+            ref.deferred_set_bounding_volume(int(h), bb[h])
+
+
+def test_broad_phase_clustered(ctx, orc):
+    w = workloads.full_pipeline(seed=5, n=20000)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    bb = ss.aabbs(torch.as_tensor(w["poses"], device=ctx.device), margin=0.02).cpu().numpy().T
+    bp = ncx.DBVTBroadPhase(0.02, ctx=ctx)
+    bp.create_proxies(bb)
+    bp.update()
+    assert np.array_equal(bp.pairs(), orc.pairs_brute(bb))
+
+
+# ------------------------------------------------------------------ contact / distance
+def test_contact_plain_pipeline(ctx, orc):
+    w = workloads.plain_pipeline(seed=1)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    bb = ss.aabbs(poses, margin=w["margin"]).cpu().numpy().T
+    bp = ncx.DBVTBroadPhase(w["margin"], ctx=ctx)
+    bp.create_proxies(bb)
+    bp.update()
+    keys = bp.pairs()
+    ia = (keys >> np.uint64(32)).astype(np.uint32)
+    ib = (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    g = ncx.query.contact(ss, poses, ia, ib, w["prediction"])
+    r = orc.contact(oracle_shapes(orc, w["records"], w["pool"]), to_oracle_poses(w["poses"]), ia, ib,
+                    w["prediction"])
+    assert_contacts_match(g, r, "plain pipeline")
+
+
+def test_contact_hull_pairs(ctx, orc):
+    w = workloads.hull_pairs(seed=3, npairs=2048)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    g = ncx.query.contact(ss, poses, w["ia"], w["ib"], 0.0)
+    r = orc.contact(oracle_shapes(orc, w["records"], w["pool"]), to_oracle_poses(w["poses"]), w["ia"], w["ib"], 0.0,
+                    nthreads=8)
+    assert_contacts_match(g, r, "hull pairs")
+    assert np.array_equal(g["gjk_iters"].cpu().numpy(), r["gjk_iters"])
+
+
+def test_contact_mixed_shapes(ctx, orc):
+    w = workloads.full_pipeline(seed=5, n=3000, box=30.0, nclusters=4)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    bb = ss.aabbs(poses, margin=0.02).cpu().numpy().T
+    keys = orc.pairs_brute(bb)
+    ia = (keys >> np.uint64(32)).astype(np.uint32)
+    ib = (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    assert len(ia) > 500
+    for pred in (0.0, 0.3):
+        g = ncx.query.contact(ss, poses, ia, ib, pred)
+        r = orc.contact(oracle_shapes(orc, w["records"], w["pool"]), to_oracle_poses(w["poses"]), ia, ib, pred,
+                        nthreads=8)
+        assert_contacts_match(g, r, f"mixed shapes prediction {pred}")
+
+
+def test_distance_mixed_shapes(ctx, orc):
+    w = workloads.full_pipeline(seed=6, n=2000, box=20.0, nclusters=2)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    rng = np.random.default_rng(1)
+    ia = rng.integers(0, 2000, 4000).astype(np.uint32)
+    ib = rng.integers(0, 2000, 4000).astype(np.uint32)
+    keep = ia != ib
+    ia, ib = ia[keep], ib[keep]
+    d, it = ncx.query.distance(ss, poses, ia, ib)
+    rd, rit = orc.distance(oracle_shapes(orc, w["records"], w["pool"]), to_oracle_poses(w["poses"]), ia, ib,
+                           nthreads=8)
+    d = d.cpu().numpy().astype(np.float64)
+    err = np.abs(d - rd)
+    assert np.all(err <= REL * np.abs(rd) + ABS), f"max err {err.max()}"
+    print(f"distance: bit-exact fraction {np.mean(d == rd):.4f}")
+
+
+def test_contact_known_answers_gpu(ctx):
+    """epa3.rs:7-22 and contact_query3d.rs in f32 on the GPU."""
+    shapes = [ncx.Cuboid([2, 1, 1]), ncx.Cuboid([2, 1, 1]), ncx.Ball(1.0), ncx.Cuboid([1, 1, 1])]
+    ss = ncx.ShapeSet(shapes, ctx=ctx)
+    poses = np.zeros((4, 8), np.float32)
+    poses[:, 7] = 1
+    poses[0, :3] = [3.5, 0, 0]
+    g = ncx.query.contact(ss, poses, [0], [1], 10.0)
+    assert g["depth"].item() == 0.5 and np.array_equal(g["normal"].cpu().numpy()[0], [-1, 0, 0])
+    poses[0, :3] = [0, 0.2, 0]
+    g = ncx.query.contact(ss, poses, [0], [1], 10.0)
+    assert abs(g["depth"].item() - 1.8) < 1e-6 and np.array_equal(g["normal"].cpu().numpy()[0], [0, -1, 0])
+    for k in (1, 2, 3):
+        poses[2, :3] = [k, k, k]
+        g = ncx.query.contact(ss, poses, [2], [3], 1.0)
+        if k == 3:
+            assert not g["has_contact"].item()
+        else:
+            assert abs(g["depth"].item() - (1 - np.sqrt(3) * (k - 1))) < 1e-6
+
+
+def test_contact_bad_index_raises(ctx):
+    ss = ncx.ShapeSet([ncx.Ball(1.0), ncx.Ball(1.0)], ctx=ctx)
+    poses = np.zeros((2, 8), np.float32)
+    poses[:, 7] = 1
+    with pytest.raises(ncx.NcxError):
+        ncx.query.contact(ss, poses, [0], [5], 0.0)
