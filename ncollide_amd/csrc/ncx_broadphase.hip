@@ -129,6 +129,7 @@ __global__ void k_bp_stage(const uint32_t* __restrict__ handles, const float* __
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t h = handles[i];
+  if (h >= cap) return;  // not a proxy slot: ignored (the host entry point rejects it up front)
   float b[6] = {b0[i], b1[i], b2[i], b3[i], b4[i], b5[i]};
   if (mode == 1) {
 #pragma unroll
