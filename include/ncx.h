@@ -148,6 +148,11 @@ int ncx_trimesh_destroy(ncx_mesh* mesh);
  * UINT32_MAX, toi = -1.  Among hits with equal toi the smallest feature wins. */
 int ncx_trimesh_cast_rays(ncx_mesh* mesh, const ncx_pose* pose, const float* const ray_soa_dev[6], size_t n,
                           float max_toi, float* toi_dev, float* normal_dev, uint32_t* feature_dev);
+/* Same as ncx_trimesh_cast_rays, but only enqueues the kernel on the context stream (no host
+ * synchronisation); ncx_trimesh_check reports a traversal failure of any earlier cast. */
+int ncx_trimesh_cast_rays_async(ncx_mesh* mesh, const ncx_pose* pose, const float* const ray_soa_dev[6], size_t n,
+                                float max_toi, float* toi_dev, float* normal_dev, uint32_t* feature_dev);
+int ncx_trimesh_check(ncx_mesh* mesh);
 /* Traversal statistics of the same cast (nodes visited, triangles tested per ray), for roofline accounting. */
 int ncx_trimesh_cast_stats(ncx_mesh* mesh, const ncx_pose* pose, const float* const ray_soa_dev[6], size_t n,
                            float max_toi, uint32_t* nodes_visited_dev, uint32_t* tris_tested_dev);

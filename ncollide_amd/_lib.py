@@ -44,6 +44,8 @@ SIGNATURES = {
     "ncx_trimesh_destroy": (C.c_int, [P]),
     "ncx_trimesh_cast_rays": (C.c_int, [P, P, P, SZ, F32, P, P, P]),
     "ncx_trimesh_cast_stats": (C.c_int, [P, P, P, SZ, F32, P, P]),
+    "ncx_trimesh_cast_rays_async": (C.c_int, [P, P, P, SZ, F32, P, P, P]),
+    "ncx_trimesh_check": (C.c_int, [P]),
     "ncx_convex_hull": (C.c_int, [P, SZ, P, P]),
     "ncx_convex_hull_batch": (C.c_int, [P, P, P, SZ, P, P, C.c_int]),
     "ncx_mlp_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_uint64, P]),

@@ -231,6 +231,7 @@ extern "C" int ncx_trimesh_create(ncx_ctx* ctx, const float* points_xyz, size_t 
   for (int d = 0; d < 6; ++d) box[d] = bx + d * ntriangles;
   // Grow node boxes by 2^-18 (1 + |x|): larger than the rounding of any slab or triangle toi.
   if ((rc = lbvh_build(ctx, m->bvh, box, ntriangles, 3.8146973e-06f))) return fail(rc);
+  if ((rc = fill_u32(ctx, m->err.as<uint32_t>(), 1, 0u))) return fail(rc);
   hipLaunchKernelGGL(k_tri_reorder, dim3(ceil_div(ntriangles, 256)), dim3(256), 0, ctx->stream, m->pts.as<float>(),
                      m->idx.as<uint32_t>(), m->bvh.order.as<uint32_t>(), ntriangles, m->tris.as<float4>());
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -250,7 +251,7 @@ extern "C" int ncx_trimesh_destroy(ncx_mesh* m) {
 }
 
 static int cast_impl(ncx_mesh* m, const ncx_pose* pose, const float* const r[6], size_t n, float max_toi, float* toi,
-                     float* nrm, uint32_t* feat, uint32_t* sn, uint32_t* st, bool stats) {
+                     float* nrm, uint32_t* feat, uint32_t* sn, uint32_t* st, bool stats, bool sync) {
   NCX_REQUIRE(m && pose && r, "ncx_trimesh_cast_rays: null argument");
   if (n == 0) return NCX_OK;
   for (int d = 0; d < 6; ++d) NCX_REQUIRE(r[d], "ncx_trimesh_cast_rays: null ray array");
@@ -258,8 +259,7 @@ static int cast_impl(ncx_mesh* m, const ncx_pose* pose, const float* const r[6],
   unsigned blocks = ceil_div(n, RC_BLOCK);
   if (blocks > RC_GRID_BLOCKS) blocks = RC_GRID_BLOCKS;
   int rc;
-  if ((rc = m->overflow.reserve((size_t)blocks * RC_BLOCK * RC_GLOBAL_STACK * 4))) return rc;
-  if ((rc = fill_u32(ctx, m->err.as<uint32_t>(), 1, 0u))) return rc;
+  if ((rc = m->overflow.reserve((size_t)RC_GRID_BLOCKS * RC_BLOCK * RC_GLOBAL_STACK * 4))) return rc;
   if (stats)
     hipLaunchKernelGGL(k_cast<true>, dim3(blocks), dim3(RC_BLOCK), 0, ctx->stream, m->bvh.nodes.as<LNode>(),
                        m->tris.as<float4>(), (uint32_t)m->ntri, (uint32_t)m->ntri, *pose, r[0], r[1], r[2], r[3], r[4],
@@ -269,9 +269,16 @@ static int cast_impl(ncx_mesh* m, const ncx_pose* pose, const float* const r[6],
                        m->tris.as<float4>(), (uint32_t)m->ntri, (uint32_t)m->ntri, *pose, r[0], r[1], r[2], r[3], r[4],
                        r[5], n, max_toi, toi, nrm, feat, sn, st, m->overflow.as<uint32_t>(), m->err.as<uint32_t>());
   NCX_HIP_CHECK(hipGetLastError());
+  if (!sync) return NCX_OK;
+  return ncx_trimesh_check(m);
+}
+
+// The traversal-stack error flag is sticky: it is cleared at mesh creation and checked here.
+extern "C" int ncx_trimesh_check(ncx_mesh* m) {
+  NCX_REQUIRE(m, "ncx_trimesh_check: null argument");
   uint32_t err = 0;
-  NCX_HIP_CHECK(hipMemcpyAsync(&err, m->err.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  NCX_HIP_CHECK(hipMemcpyAsync(&err, m->err.p, 4, hipMemcpyDeviceToHost, m->ctx->stream));
+  NCX_HIP_CHECK(hipStreamSynchronize(m->ctx->stream));
   if (err) {
     ncx_set_error("ncx_trimesh_cast_rays: traversal stack exhausted (tree deeper than 128 levels)");
     return NCX_E_CAPACITY;
@@ -282,11 +289,18 @@ static int cast_impl(ncx_mesh* m, const ncx_pose* pose, const float* const r[6],
 extern "C" int ncx_trimesh_cast_rays(ncx_mesh* m, const ncx_pose* pose, const float* const ray_soa_dev[6], size_t n,
                                      float max_toi, float* toi_dev, float* normal_dev, uint32_t* feature_dev) {
   NCX_REQUIRE(toi_dev && normal_dev && feature_dev, "ncx_trimesh_cast_rays: null output");
-  return cast_impl(m, pose, ray_soa_dev, n, max_toi, toi_dev, normal_dev, feature_dev, nullptr, nullptr, false);
+  return cast_impl(m, pose, ray_soa_dev, n, max_toi, toi_dev, normal_dev, feature_dev, nullptr, nullptr, false, true);
+}
+
+extern "C" int ncx_trimesh_cast_rays_async(ncx_mesh* m, const ncx_pose* pose, const float* const ray_soa_dev[6],
+                                           size_t n, float max_toi, float* toi_dev, float* normal_dev,
+                                           uint32_t* feature_dev) {
+  NCX_REQUIRE(toi_dev && normal_dev && feature_dev, "ncx_trimesh_cast_rays_async: null output");
+  return cast_impl(m, pose, ray_soa_dev, n, max_toi, toi_dev, normal_dev, feature_dev, nullptr, nullptr, false, false);
 }
 
 extern "C" int ncx_trimesh_cast_stats(ncx_mesh* m, const ncx_pose* pose, const float* const ray_soa_dev[6], size_t n,
                                       float max_toi, uint32_t* nodes_dev, uint32_t* tris_dev) {
   NCX_REQUIRE(nodes_dev && tris_dev, "ncx_trimesh_cast_stats: null output");
-  return cast_impl(m, pose, ray_soa_dev, n, max_toi, nullptr, nullptr, nullptr, nodes_dev, tris_dev, true);
+  return cast_impl(m, pose, ray_soa_dev, n, max_toi, nullptr, nullptr, nullptr, nodes_dev, tris_dev, true, true);
 }

@@ -244,10 +244,13 @@ def test_distance_mixed_shapes(ctx, orc):
     print(f"distance: bit-exact fraction {np.mean(d == rd):.4f}")
 
 
-def test_contact_known_answers_gpu(ctx):
-    """epa3.rs:7-22 and contact_query3d.rs in f32 on the GPU."""
+def test_contact_known_answers_gpu(ctx, orc):
+    """epa3.rs:7-22 and contact_query3d.rs in f32 on the GPU.  The reference test runs in f64
+    (the oracle reproduces 0.5 / 1.8 exactly there, see test_oracle.py); in f32 the (0, 0.2, 0)
+    case takes EPA's numerical-issue return (epa3.rs:397-402), so the GPU is held to the f32 oracle."""
     shapes = [ncx.Cuboid([2, 1, 1]), ncx.Cuboid([2, 1, 1]), ncx.Ball(1.0), ncx.Cuboid([1, 1, 1])]
     ss = ncx.ShapeSet(shapes, ctx=ctx)
+    oss = oracle_shapes(orc, ss.records, ss.pool)
     poses = np.zeros((4, 8), np.float32)
     poses[:, 7] = 1
     poses[0, :3] = [3.5, 0, 0]
@@ -255,7 +258,9 @@ def test_contact_known_answers_gpu(ctx):
     assert g["depth"].item() == 0.5 and np.array_equal(g["normal"].cpu().numpy()[0], [-1, 0, 0])
     poses[0, :3] = [0, 0.2, 0]
     g = ncx.query.contact(ss, poses, [0], [1], 10.0)
-    assert abs(g["depth"].item() - 1.8) < 1e-6 and np.array_equal(g["normal"].cpu().numpy()[0], [0, -1, 0])
+    r = orc.contact(oss, to_oracle_poses(poses), [0], [1], 10.0)
+    assert g["depth"].item() == r["depth"][0]
+    assert np.array_equal(g["normal"].cpu().numpy()[0], r["normal"][0])
     for k in (1, 2, 3):
         poses[2, :3] = [k, k, k]
         g = ncx.query.contact(ss, poses, [2], [3], 1.0)
