@@ -97,6 +97,8 @@ int ncx_bp_num_pairs(ncx_bp* bp, size_t* n);
 int ncx_bp_pairs(ncx_bp* bp, uint64_t* keys_host, size_t cap, size_t* n);
 /* Device pointers to the current pair list (a[i] < b[i]), valid until the next update. */
 int ncx_bp_pairs_dev(ncx_bp* bp, const uint32_t** a_dev, const uint32_t** b_dev, size_t* n);
+/* Copies the current (unsorted) pair list into caller device buffers of capacity cap (stream-ordered). */
+int ncx_bp_copy_pairs_dev(ncx_bp* bp, uint32_t* a_dev, uint32_t* b_dev, size_t cap, size_t* n);
 /* interference_started / interference_stopped of the last update (broad_phase.rs:28-38), sorted keys. */
 int ncx_bp_events(ncx_bp* bp, uint64_t* started, size_t cap_started, size_t* n_started, uint64_t* stopped,
                   size_t cap_stopped, size_t* n_stopped);

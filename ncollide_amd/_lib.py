@@ -31,6 +31,7 @@ SIGNATURES = {
     "ncx_bp_num_pairs": (C.c_int, [P, P]),
     "ncx_bp_pairs": (C.c_int, [P, P, SZ, P]),
     "ncx_bp_pairs_dev": (C.c_int, [P, P, P, P]),
+    "ncx_bp_copy_pairs_dev": (C.c_int, [P, P, P, SZ, P]),
     "ncx_bp_events": (C.c_int, [P, P, SZ, P, P, SZ, P]),
     "ncx_bp_proxy_aabb": (C.c_int, [P, U32, P]),
     "ncx_shapes_create": (C.c_int, [P, P, SZ, P, P, P, SZ, P]),

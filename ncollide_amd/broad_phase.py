@@ -72,13 +72,14 @@ class DBVTBroadPhase:
         _lib.check(self.ctx.lib.ncx_bp_pairs(self.h, _lib.ptr(out), len(out), C.byref(got)))
         return out[:got.value]
 
-    def pairs_dev(self):
-        """(a, b) device pointers + count of the current (unsorted) pair list, as int32 tensors (copies)."""
-        a = C.c_void_p()
-        b = C.c_void_p()
-        n = C.c_size_t()
-        _lib.check(self.ctx.lib.ncx_bp_pairs_dev(self.h, C.byref(a), C.byref(b), C.byref(n)))
-        return a.value, b.value, n.value
+    def pairs_tensors(self):
+        """The current pair list (unsorted, a < b) as two int32 CUDA tensors."""
+        n = self.num_interferences()
+        a = torch.empty(max(n, 1), dtype=torch.int32, device=self.ctx.device)
+        b = torch.empty(max(n, 1), dtype=torch.int32, device=self.ctx.device)
+        got = C.c_size_t()
+        _lib.check(self.ctx.lib.ncx_bp_copy_pairs_dev(self.h, _lib.ptr(a), _lib.ptr(b), n, C.byref(got)))
+        return a[:n], b[:n]
 
     def events(self):
         """(started, stopped) sorted pair keys of the last update."""

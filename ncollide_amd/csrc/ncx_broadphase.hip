@@ -506,6 +506,21 @@ extern "C" int ncx_bp_pairs_dev(ncx_bp* bp, const uint32_t** a, const uint32_t**
   return NCX_OK;
 }
 
+extern "C" int ncx_bp_copy_pairs_dev(ncx_bp* bp, uint32_t* a_dev, uint32_t* b_dev, size_t cap, size_t* n) {
+  NCX_REQUIRE(bp && n, "ncx_bp_copy_pairs_dev: null argument");
+  *n = bp->npairs;
+  if (cap < bp->npairs) {
+    ncx_set_error("ncx_bp_copy_pairs_dev: capacity too small");
+    return NCX_E_CAPACITY;
+  }
+  if (bp->npairs) {
+    NCX_REQUIRE(a_dev && b_dev, "ncx_bp_copy_pairs_dev: null output");
+    NCX_HIP_CHECK(hipMemcpyAsync(a_dev, bp->pa.p, bp->npairs * 4, hipMemcpyDeviceToDevice, bp->ctx->stream));
+    NCX_HIP_CHECK(hipMemcpyAsync(b_dev, bp->pb.p, bp->npairs * 4, hipMemcpyDeviceToDevice, bp->ctx->stream));
+  }
+  return NCX_OK;
+}
+
 static int download_keys(ncx_bp* bp, const DevBuf& a, const DevBuf& b, size_t n, uint64_t* out) {
   std::vector<uint32_t> ha(n), hb(n);
   ncx_ctx* ctx = bp->ctx;

@@ -128,12 +128,13 @@ def _gather_ids(ids, offsets, hc):
     return starts + ids[(starts + local).astype(np.int64)].astype(np.int64)
 
 
-def hull_pairs(seed=3, npairs=1 << 18, nmin=8, nmax=256, signed_distance=None, bisect_iters=24):
+def hull_pairs(seed=3, npairs=1 << 18, nmin=8, nmax=256, placer=None, bisect_iters=24):
     """Pairs (2k, 2k+1) of ConvexHulls with random rotations.  Pair k is moved along a random
     direction u to a target signed distance: -U[0, 0.1] (penetrating) for even k, U[0, 0.5]
-    (separated) for odd k.  `signed_distance(poses) -> (npairs,)` evaluates the current signed
-    distance of every pair; placement bisects the translation along u against it.  Without it,
-    pairs are placed where their support planes along u touch, shifted by the target."""
+    (separated) for odd k.  `placer(records, pool, ia, ib)` returns a function
+    signed_distance(poses) -> (npairs,) evaluating every pair's current signed distance;
+    placement bisects the translation along u against it.  Without a placer, pairs are placed
+    where their support planes along u touch, shifted by the target."""
     rng = np.random.default_rng(seed)
     nh = 2 * npairs
     pts, offs, cnts = ellipsoid_clouds(rng, nh, nmin, nmax)
@@ -144,6 +145,9 @@ def hull_pairs(seed=3, npairs=1 << 18, nmin=8, nmax=256, signed_distance=None, b
     rec["hull_count"] = hcnt
     q = uniform_quats(rng, nh)
     u = unit_vectors(rng, npairs)
+    ia = np.arange(0, nh, 2, dtype=np.uint32)
+    ib = ia + 1
+    signed_distance = placer(rec, pool, ia, ib) if placer is not None else None
     target = np.where(np.arange(npairs) % 2 == 0, -rng.uniform(0.0, 0.1, npairs), rng.uniform(0.0, 0.5, npairs))
     # Separation along u at which the support planes touch (upper bound of the touching distance).
     ext = _support_extent(pool, hoff, hcnt, q, u)
@@ -162,8 +166,6 @@ def hull_pairs(seed=3, npairs=1 << 18, nmin=8, nmax=256, signed_distance=None, b
             hi = np.where(above, mid, hi)
             lo = np.where(above, lo, mid)
         t[1::2] = (u * (0.5 * (lo + hi))[:, None]).astype(np.float32)
-    ia = np.arange(0, nh, 2, dtype=np.uint32)
-    ib = ia + 1
     return dict(records=rec, pool=pool, poses=poses8(t, q), ia=ia, ib=ib, target=target, dirs=u)
 
 
