@@ -278,22 +278,40 @@ __device__ f3 proj_tetrahedron(f3 A, f3 B, f3 C, f3 D, Loc& loc) {
 }
 
 // ------------------------------------------------------------------ VoronoiSimplex (voronoi_simplex3.rs:12-352)
+// The four vertices are separate members and the "previous vertex" permutation is
+// packed 2 bits per slot, so no runtime index reaches an array: everything stays in
+// registers.
 struct Simplex {
-  CSO v[4];
-  int pv[4];
-  float proj[3], pproj[3];
+  CSO v0, v1, v2, v3;
+  uint32_t pv;  // prev_vertices, slot i in bits [2i, 2i+2)
+  float proj0, proj1, proj2, pproj0, pproj1, pproj2;
   int dim, pdim;
 };
 
+__device__ __forceinline__ CSO sx_get(const Simplex& s, int i) {
+  return i == 0 ? s.v0 : (i == 1 ? s.v1 : (i == 2 ? s.v2 : s.v3));
+}
+__device__ __forceinline__ void sx_set(Simplex& s, int i, const CSO& c) {
+  if (i == 0) s.v0 = c;
+  else if (i == 1) s.v1 = c;
+  else if (i == 2) s.v2 = c;
+  else s.v3 = c;
+}
 __device__ __forceinline__ void sx_swap(Simplex& s, int i, int j) {
-  CSO t = s.v[i]; s.v[i] = s.v[j]; s.v[j] = t;
-  int u = s.pv[i]; s.pv[i] = s.pv[j]; s.pv[j] = u;
+  CSO a = sx_get(s, i), b = sx_get(s, j);
+  sx_set(s, i, b);
+  sx_set(s, j, a);
+  uint32_t pi = (s.pv >> (2 * i)) & 3u, pj = (s.pv >> (2 * j)) & 3u;
+  s.pv &= ~((3u << (2 * i)) | (3u << (2 * j)));
+  s.pv |= (pj << (2 * i)) | (pi << (2 * j));
 }
 
+#define PV_IDENTITY 0xE4u  // slots (0, 1, 2, 3)
+
 __device__ __forceinline__ void sx_init(Simplex& s) {
-  for (int i = 0; i < 4; ++i) s.pv[i] = i;
-  s.proj[0] = s.proj[1] = s.proj[2] = 0.f;
-  s.pproj[0] = s.pproj[1] = s.pproj[2] = 0.f;
+  s.pv = PV_IDENTITY;
+  s.proj0 = s.proj1 = s.proj2 = 0.f;
+  s.pproj0 = s.pproj1 = s.pproj2 = 0.f;
   s.dim = 0;
   s.pdim = 0;
 }
@@ -301,57 +319,59 @@ __device__ __forceinline__ void sx_init(Simplex& s) {
 __device__ __forceinline__ void sx_reset(Simplex& s, const CSO& p) {
   s.dim = 0;
   s.pdim = 0;
-  s.v[0] = p;
+  s.v0 = p;
 }
 
 __device__ __forceinline__ bool sx_add(Simplex& s, const CSO& pt) {
   const float tol = NCX_FLT_EPS * 10.f;
   s.pdim = s.dim;
-  s.pproj[0] = s.proj[0]; s.pproj[1] = s.proj[1]; s.pproj[2] = s.proj[2];
-  s.pv[0] = 0; s.pv[1] = 1; s.pv[2] = 2; s.pv[3] = 3;
+  s.pproj0 = s.proj0; s.pproj1 = s.proj1; s.pproj2 = s.proj2;
+  s.pv = PV_IDENTITY;
   if (s.dim == 0) {
-    if (norm_sq(s.v[0].p - pt.p) < tol) return false;
+    if (norm_sq(s.v0.p - pt.p) < tol) return false;
   } else if (s.dim == 1) {
-    f3 ab = s.v[1].p - s.v[0].p, ac = pt.p - s.v[0].p;
+    f3 ab = s.v1.p - s.v0.p, ac = pt.p - s.v0.p;
     if (norm_sq(cross(ab, ac)) < tol) return false;
   } else {
-    f3 ab = s.v[1].p - s.v[0].p, ac = s.v[2].p - s.v[0].p, ap = pt.p - s.v[0].p;
+    f3 ab = s.v1.p - s.v0.p, ac = s.v2.p - s.v0.p, ap = pt.p - s.v0.p;
     f3 n = normalize(cross(ab, ac));
     if (fabsf(dot(n, ap)) < tol) return false;
   }
   s.dim += 1;
-  s.v[s.dim] = pt;
+  if (s.dim == 1) s.v1 = pt;
+  else if (s.dim == 2) s.v2 = pt;
+  else s.v3 = pt;
   return true;
 }
 
 __device__ f3 sx_project_reduce(Simplex& s) {
   if (s.dim == 0) {
-    s.proj[0] = 1.f;
-    return s.v[0].p;
+    s.proj0 = 1.f;
+    return s.v0.p;
   } else if (s.dim == 1) {
     Loc loc;
-    f3 pr = proj_segment(s.v[0].p, s.v[1].p, loc);
-    if (loc.kind == LOC_VERTEX && loc.id == 0) { s.proj[0] = 1.f; s.dim = 0; }
-    else if (loc.kind == LOC_VERTEX) { sx_swap(s, 0, 1); s.proj[0] = 1.f; s.dim = 0; }
-    else { s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; }
+    f3 pr = proj_segment(s.v0.p, s.v1.p, loc);
+    if (loc.kind == LOC_VERTEX && loc.id == 0) { s.proj0 = 1.f; s.dim = 0; }
+    else if (loc.kind == LOC_VERTEX) { sx_swap(s, 0, 1); s.proj0 = 1.f; s.dim = 0; }
+    else { s.proj0 = loc.bc0; s.proj1 = loc.bc1; }
     return pr;
   } else if (s.dim == 2) {
     Loc loc;
-    f3 pr = proj_triangle(s.v[0].p, s.v[1].p, s.v[2].p, loc);
-    if (loc.kind == LOC_VERTEX) { sx_swap(s, 0, loc.id); s.proj[0] = 1.f; s.dim = 0; }
+    f3 pr = proj_triangle(s.v0.p, s.v1.p, s.v2.p, loc);
+    if (loc.kind == LOC_VERTEX) { sx_swap(s, 0, loc.id); s.proj0 = 1.f; s.dim = 0; }
     else if (loc.kind == LOC_EDGE) {
-      if (loc.id == 0) { s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; }
-      else if (loc.id == 1) { sx_swap(s, 0, 2); s.proj[0] = loc.bc1; s.proj[1] = loc.bc0; }
-      else { sx_swap(s, 1, 2); s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; }
+      if (loc.id == 0) { s.proj0 = loc.bc0; s.proj1 = loc.bc1; }
+      else if (loc.id == 1) { sx_swap(s, 0, 2); s.proj0 = loc.bc1; s.proj1 = loc.bc0; }
+      else { sx_swap(s, 1, 2); s.proj0 = loc.bc0; s.proj1 = loc.bc1; }
       s.dim = 1;
     } else if (loc.kind == LOC_FACE) {
-      s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; s.proj[2] = loc.bc2;
+      s.proj0 = loc.bc0; s.proj1 = loc.bc1; s.proj2 = loc.bc2;
     }
     return pr;
   } else {
     Loc loc;
-    f3 pr = proj_tetrahedron(s.v[0].p, s.v[1].p, s.v[2].p, s.v[3].p, loc);
-    if (loc.kind == LOC_VERTEX) { sx_swap(s, 0, loc.id); s.proj[0] = 1.f; s.dim = 0; }
+    f3 pr = proj_tetrahedron(s.v0.p, s.v1.p, s.v2.p, s.v3.p, loc);
+    if (loc.kind == LOC_VERTEX) { sx_swap(s, 0, loc.id); s.proj0 = 1.f; s.dim = 0; }
     else if (loc.kind == LOC_EDGE) {
       switch (loc.id) {
         case 1: sx_swap(s, 1, 2); break;
@@ -361,15 +381,15 @@ __device__ f3 sx_project_reduce(Simplex& s) {
         case 5: sx_swap(s, 0, 2); sx_swap(s, 1, 3); break;
         default: break;
       }
-      if (loc.id == 3 || loc.id == 4) { s.proj[0] = loc.bc1; s.proj[1] = loc.bc0; }
-      else { s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; }
+      if (loc.id == 3 || loc.id == 4) { s.proj0 = loc.bc1; s.proj1 = loc.bc0; }
+      else { s.proj0 = loc.bc0; s.proj1 = loc.bc1; }
       s.dim = 1;
     } else if (loc.kind == LOC_FACE) {
       switch (loc.id) {
-        case 0: s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; s.proj[2] = loc.bc2; break;
-        case 1: s.v[2] = s.v[3]; s.proj[0] = loc.bc0; s.proj[1] = loc.bc1; s.proj[2] = loc.bc2; break;
-        case 2: s.v[1] = s.v[3]; s.proj[0] = loc.bc0; s.proj[1] = loc.bc2; s.proj[2] = loc.bc1; break;
-        default: s.v[0] = s.v[3]; s.proj[0] = loc.bc2; s.proj[1] = loc.bc0; s.proj[2] = loc.bc1; break;
+        case 0: s.proj0 = loc.bc0; s.proj1 = loc.bc1; s.proj2 = loc.bc2; break;
+        case 1: s.v2 = s.v3; s.proj0 = loc.bc0; s.proj1 = loc.bc1; s.proj2 = loc.bc2; break;
+        case 2: s.v1 = s.v3; s.proj0 = loc.bc0; s.proj1 = loc.bc2; s.proj2 = loc.bc1; break;
+        default: s.v0 = s.v3; s.proj0 = loc.bc2; s.proj1 = loc.bc0; s.proj2 = loc.bc1; break;
       }
       s.dim = 2;
     }
@@ -380,17 +400,13 @@ __device__ f3 sx_project_reduce(Simplex& s) {
 // gjk.rs:367-388
 __device__ __forceinline__ void gjk_result(const Simplex& s, bool prev, f3& p1, f3& p2) {
   f3 r1 = f3{0.f, 0.f, 0.f}, r2 = f3{0.f, 0.f, 0.f};
-  if (prev) {
-    for (int i = 0; i < s.pdim + 1; ++i) {
-      float c = s.pproj[i];
-      const CSO& pt = s.v[s.pv[i]];
-      r1 = r1 + pt.o1 * c;
-      r2 = r2 + pt.o2 * c;
-    }
-  } else {
-    for (int i = 0; i < s.dim + 1; ++i) {
-      float c = s.proj[i];
-      const CSO& pt = s.v[i];
+  const int n = prev ? s.pdim : s.dim;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i <= n) {
+      float c = prev ? (i == 0 ? s.pproj0 : (i == 1 ? s.pproj1 : s.pproj2))
+                     : (i == 0 ? s.proj0 : (i == 1 ? s.proj1 : s.proj2));
+      CSO pt = sx_get(s, prev ? (int)((s.pv >> (2 * i)) & 3u) : i);
       r1 = r1 + pt.o1 * c;
       r2 = r2 + pt.o2 * c;
     }
@@ -625,7 +641,10 @@ __device__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const
   const float gjk_tol = NCX_FLT_EPS * 10.f;
   iters = 0;
   uint32_t nv = 0, nf = 0, hl = 0;
-  for (int i = 0; i < s.dim + 1; ++i) A.verts[nv++] = s.v[i];
+  A.verts[nv++] = s.v0;
+  if (s.dim >= 1) A.verts[nv++] = s.v1;
+  if (s.dim >= 2) A.verts[nv++] = s.v2;
+  if (s.dim >= 3) A.verts[nv++] = s.v3;
   if (s.dim == 0) {
     p1 = f3{0.f, 0.f, 0.f};
     p2 = f3{0.f, 0.f, 0.f};
