@@ -128,7 +128,7 @@ int ncx_shapes_aabbs_dev(ncx_shapes* s, const ncx_pose* poses_dev, float margin,
  * init_dir_dev (nullable, 3 floats per pair): the `init_dir` argument of
  * contact_support_map_support_map_with_params
  * (src/query/contact/contact_support_map_support_map.rs:38-61) for pairs that
- * reach the support-map path; NULL is the reference's None. */
+ * reach the support-map path; NULL is the reference's None, and so is an all-zero entry. */
 int ncx_contact_batch(ncx_shapes* s, const ncx_pose* poses_dev, const uint32_t* ia_dev, const uint32_t* ib_dev,
                       size_t npairs, float prediction, const float* init_dir_dev, uint8_t* has_contact_dev,
                       float* world1_dev, float* world2_dev, float* normal_dev, float* depth_dev,
@@ -177,6 +177,36 @@ int ncx_contact_batch_capped(ncx_shapes* s, const ncx_pose* poses_dev, const uin
                              const uint32_t* ib_dev, size_t npairs, float prediction, const float* init_dir_dev,
                              int32_t max_iter, uint8_t* has_contact_dev, float* world1_dev, float* world2_dev,
                              float* normal_dev, float* depth_dev, int32_t* gjk_iters_dev);
+
+/* ---------------------------------------------------------------- learned initial direction
+ * No reference counterpart: a 44-64-64-3 MLP (FP16 operands, FP32 accumulation) that predicts
+ * the init_dir argument of contact_support_map_support_map_with_params
+ * (src/query/contact/contact_support_map_support_map.rs:38-61).  Features of pair (a, b):
+ * R_a^T R_b (9), R_a^T (t_b - t_a) (3), 16 local support values of a and of b
+ * (ncx_shape_descriptors), padded to 48 FP16 per pair.  Parameters: a flat FP32 vector of
+ * 8336 entries (W1 64x48, b1 64, W2 64x64, b2 64, W3 16x64, b3 16; padding stays zero). */
+typedef struct ncx_mlp ncx_mlp;
+int ncx_mlp_create(ncx_ctx* ctx, int in, int hidden, int out, uint64_t seed, ncx_mlp** mlp);
+int ncx_mlp_destroy(ncx_mlp* mlp);
+int ncx_mlp_num_params(ncx_mlp* mlp, size_t* n);
+int ncx_mlp_get_params(ncx_mlp* mlp, float* params_host);
+int ncx_mlp_set_params(ncx_mlp* mlp, const float* params_host);
+/* 16 support values per shape along fixed Fibonacci directions of its local frame (16 floats per shape). */
+int ncx_shape_descriptors(ncx_shapes* s, float* desc_dev);
+/* Features (48 FP16 per pair) of pairs (ia[k], ib[k]). */
+int ncx_features(ncx_shapes* s, const ncx_pose* poses_dev, const float* desc_dev, const uint32_t* ia_dev,
+                 const uint32_t* ib_dev, size_t npairs, void* feat_f16_dev);
+/* Raw network output (3 floats per row, shape-a frame). */
+int ncx_mlp_forward(ncx_mlp* mlp, const void* feat_f16_dev, size_t n, float* out_dev);
+/* Mean squared error (1/n) sum |f(x) - y|^2: its gradient (8336 floats) and value, on the device. */
+int ncx_mlp_grad(ncx_mlp* mlp, const void* feat_f16_dev, const float* target_dev, size_t n, float* grad_dev,
+                 float* loss_dev);
+/* One Adam step (lr; beta 0.9 / 0.999) on the batch; *loss_host (nullable) gets the batch loss. */
+int ncx_mlp_train_step(ncx_mlp* mlp, const void* feat_f16_dev, const float* target_dev, size_t n, float lr,
+                       float* loss_host);
+/* Features + forward + rotation to world + normalisation, fused: unit init_dir per pair. */
+int ncx_mlp_predict_dirs(ncx_mlp* mlp, const ncx_pose* poses_dev, const float* desc_dev, const uint32_t* ia_dev,
+                         const uint32_t* ib_dev, size_t npairs, float* dirs_dev);
 
 #ifdef __cplusplus
 }

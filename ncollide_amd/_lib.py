@@ -57,7 +57,8 @@ SIGNATURES = {
     "ncx_mlp_forward": (C.c_int, [P, P, SZ, P]),
     "ncx_mlp_train_step": (C.c_int, [P, P, P, SZ, F32, P]),
     "ncx_mlp_grad": (C.c_int, [P, P, P, SZ, P, P]),
-    "ncx_features": (C.c_int, [P, P, P, P, SZ, P, P]),
+    "ncx_mlp_predict_dirs": (C.c_int, [P, P, P, P, P, SZ, P]),
+    "ncx_features": (C.c_int, [P, P, P, P, P, SZ, P]),
     "ncx_shape_descriptors": (C.c_int, [P, P]),
 }
 

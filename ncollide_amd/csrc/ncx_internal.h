@@ -67,6 +67,16 @@ struct DevBuf {
   template <class T> T* as() const { return (T*)p; }
 };
 
+// Device-resident shape set (ncx_shapes_create).
+struct ncx_shapes {
+  ncx_ctx* ctx;
+  size_t n = 0, npts = 0;
+  DevBuf shapes;      // ncx_shape[n]
+  DevBuf hx, hy, hz;  // hull pool
+  DevBuf arena;       // EPA arenas
+  DevBuf ovf;         // overflow counter + list
+};
+
 // ------------------------------------------------------------------ device math
 struct f3 {
   float x, y, z;

@@ -25,15 +25,6 @@
 #define GJK_S_NAN 4
 #define GJK_S_CAP 5  // iteration cap reached (NoIntersection(+x) in the reference)
 
-struct ncx_shapes {
-  ncx_ctx* ctx;
-  size_t n = 0, npts = 0;
-  DevBuf shapes;      // ncx_shape[n]
-  DevBuf hx, hy, hz;  // hull pool
-  DevBuf arena;       // EPA arenas
-  DevBuf ovf;         // overflow counter + list
-};
-
 struct ShapeD {
   uint32_t type, off, cnt;
   float a, b, c;
@@ -1059,7 +1050,8 @@ __global__ __launch_bounds__(256) void k_pairs(KArgs a) {
     const f3* ip = nullptr;
     if (a.init_dir) {
       idir = f3{a.init_dir[3 * k], a.init_dir[3 * k + 1], a.init_dir[3 * k + 2]};
-      ip = &idir;
+      // An all-zero entry means None for this pair (the reference's default direction).
+      if (idir.x != 0.f || idir.y != 0.f || idir.z != 0.f) ip = &idir;
     }
     PairOut o = contact_pair<G>(c, a.prediction, ip, a.P, gl, A, a.max_iter);
     int extra_iters = 0;

@@ -443,7 +443,10 @@ template <class R> inline void gjk_result(const VoronoiSimplex<R>& s, bool prev,
 }
 
 // ---------------------------------------------------------------- GJK (gjk.rs:76-177)
-enum GJKStatus { GJK_INTERSECTION = 0, GJK_CLOSEST = 1, GJK_PROXIMITY = 2, GJK_NO_INTERSECTION = 3, GJK_NAN = 4 };
+// GJK_CAP: the loop stopped at a caller-set iteration cap below the reference's 10000
+// (the reference's own cap returns NoIntersection(+x), gjk.rs:172-175).
+enum GJKStatus { GJK_INTERSECTION = 0, GJK_CLOSEST = 1, GJK_PROXIMITY = 2, GJK_NO_INTERSECTION = 3, GJK_NAN = 4,
+                 GJK_CAP = 5 };
 template <class R> struct GJKResult {
   int status;
   V3<R> p1, p2, dir;
@@ -505,7 +508,11 @@ template <class R> inline GJKResult<R> gjk_closest_points(const Iso<R>& m1, cons
       } else { res.status = GJK_INTERSECTION; return res; }
     }
     niter += 1;
-    if (niter == max_iter) { res.dir = V3<R>(1, 0, 0); res.status = GJK_NO_INTERSECTION; return res; }
+    if (niter == max_iter) {
+      res.dir = V3<R>(1, 0, 0);
+      res.status = max_iter < 10000 ? GJK_CAP : GJK_NO_INTERSECTION;
+      return res;
+    }
   }
 }
 

@@ -94,7 +94,10 @@ int contact_sm_batch(int64_t ns, const uint32_t* types, const R* params, const u
     int gi = 0, ei = 0, st = 0;
     V3<R> d0;
     const V3<R>* dptr = nullptr;
-    if (init_dirs) { d0 = V3<R>(init_dirs[3 * k], init_dirs[3 * k + 1], init_dirs[3 * k + 2]); dptr = &d0; }
+    if (init_dirs) {
+      d0 = V3<R>(init_dirs[3 * k], init_dirs[3 * k + 1], init_dirs[3 * k + 2]);
+      if (d0.x != R(0) || d0.y != R(0) || d0.z != R(0)) dptr = &d0;  // all-zero entry = None
+    }
     bool ok = contact_support_map_support_map(pose_at(poses, a), ss.shapes[a], pose_at(poses, b), ss.shapes[b],
                                               prediction, dptr, &c, &gi, &ei, gjk_max_iter, &st);
     has[k] = ok ? 1 : 0;

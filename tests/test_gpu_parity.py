@@ -276,3 +276,25 @@ def test_contact_bad_index_raises(ctx):
     poses[:, 7] = 1
     with pytest.raises(ncx.NcxError):
         ncx.query.contact(ss, poses, [0], [5], 0.0)
+
+
+def test_warm_started_contacts_match_oracle(ctx, orc):
+    """Warm start with given init directions and a 32-iteration cap (plain GJK recomputes capped
+    pairs) against the oracle run with the same directions and the same fallback rule."""
+    w = workloads.hull_pairs(seed=13, npairs=1500)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    rng = np.random.default_rng(3)
+    dirs = workloads.unit_vectors(rng, len(w["ia"])).astype(np.float32)
+    dirs[::3] = w["dirs"][::3]  # a third of the pairs get a good direction
+    g = ncx.query.contact(ss, poses, w["ia"], w["ib"], 0.0, init_dir=torch.as_tensor(dirs, device=ctx.device),
+                          max_iter=32)
+    oss = oracle_shapes(orc, w["records"], w["pool"])
+    op = to_oracle_poses(w["poses"])
+    r = orc.contact_support_map(oss, op, w["ia"], w["ib"], 0.0, init_dirs=dirs, max_iter=32, nthreads=8)
+    capped = r["status"] == 5
+    if capped.any():
+        rp = orc.contact_support_map(oss, op, w["ia"][capped], w["ib"][capped], 0.0, nthreads=8)
+        for key in ("has", "world1", "world2", "normal", "depth"):
+            r[key][capped] = rp[key]
+    assert_contacts_match(g, r, "warm-started hull pairs")
