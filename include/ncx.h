@@ -208,6 +208,30 @@ int ncx_mlp_train_step(ncx_mlp* mlp, const void* feat_f16_dev, const float* targ
 int ncx_mlp_predict_dirs(ncx_mlp* mlp, const ncx_pose* poses_dev, const float* desc_dev, const uint32_t* ia_dev,
                          const uint32_t* ib_dev, size_t npairs, float* dirs_dev);
 
+/* ---------------------------------------------------------------- one collision step of a scene
+ * The hot path of CollisionWorld::update (src/pipeline/world.rs:104-119) for a shape set whose
+ * shapes are all broad-phase proxies: world AABBs loosened by `prediction`, the DBVTBroadPhase
+ * stored-box rule with `margin` (dbvt_broad_phase.rs:325-347; the first step stores boxes as
+ * given, :275-280), LBVH refit (rebuilt every k steps), every intersecting stored-box pair, and
+ * query::contact at `prediction` on each pair.  No host synchronisation inside a step: refit steps
+ * replay a captured hipGraph when use_graph != 0.  Outputs are capacity-`max_pairs` device arrays. */
+typedef struct ncx_world ncx_world;
+int ncx_world_create(ncx_ctx* ctx, ncx_shapes* shapes, float margin, float prediction, size_t max_pairs,
+                     ncx_world** world);
+int ncx_world_destroy(ncx_world* world);
+/* Learned init_dir (ncx_mlp) for polytope-polytope pairs, GJK capped at 32 iterations with plain
+ * GJK on the cap; NULL turns it off. */
+int ncx_world_set_warmstart(ncx_world* world, ncx_mlp* mlp);
+int ncx_world_set_rebuild_every(ncx_world* world, int k);
+int ncx_world_step(ncx_world* world, const ncx_pose* poses_dev, int use_graph);
+/* Waits for the last step: *npairs = pair count; NCX_E_CAPACITY if it exceeded max_pairs.
+ * *n_epa_overflow = pairs whose EPA needs a larger arena than the step kernel has (left with
+ * has_contact = 0; recompute them with ncx_contact_batch). */
+int ncx_world_sync(ncx_world* world, size_t* npairs, size_t* n_epa_overflow);
+int ncx_world_outputs(ncx_world* world, uint32_t** ia, uint32_t** ib, uint8_t** has_contact, float** world1,
+                      float** world2, float** normal, float** depth, int32_t** gjk_iters);
+int ncx_world_stored_aabbs(ncx_world* world, const float** soa_dev);
+
 #ifdef __cplusplus
 }
 #endif

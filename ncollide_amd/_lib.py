@@ -60,6 +60,14 @@ SIGNATURES = {
     "ncx_mlp_predict_dirs": (C.c_int, [P, P, P, P, P, SZ, P]),
     "ncx_features": (C.c_int, [P, P, P, P, P, SZ, P]),
     "ncx_shape_descriptors": (C.c_int, [P, P]),
+    "ncx_world_create": (C.c_int, [P, P, F32, F32, SZ, P]),
+    "ncx_world_destroy": (C.c_int, [P]),
+    "ncx_world_set_warmstart": (C.c_int, [P, P]),
+    "ncx_world_set_rebuild_every": (C.c_int, [P, C.c_int]),
+    "ncx_world_step": (C.c_int, [P, P, C.c_int]),
+    "ncx_world_sync": (C.c_int, [P, P, P]),
+    "ncx_world_outputs": (C.c_int, [P, P, P, P, P, P, P, P, P]),
+    "ncx_world_stored_aabbs": (C.c_int, [P, P]),
 }
 
 

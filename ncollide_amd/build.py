@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libncx.so")
 SOURCES = ["ncx_core.hip", "ncx_raycast.hip", "ncx_broadphase.hip", "ncx_narrow.hip", "ncx_mlp.hip",
-           "ncx_hull.cpp"]
+           "ncx_world.hip", "ncx_hull.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

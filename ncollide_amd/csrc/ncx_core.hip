@@ -4,6 +4,7 @@
 #include "ncx_internal.h"
 #include <cstdio>
 #include <mutex>
+#include <algorithm>
 
 static thread_local std::string g_last_error;
 void ncx_set_error(const std::string& msg) { g_last_error = msg; }
@@ -284,9 +285,19 @@ __global__ __launch_bounds__(256) void k_centroid_bounds(const float* __restrict
       hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], off, 64));
     }
   }
+  __shared__ float red[6][4];
+  const int w = threadIdx.x >> 6;
   if (__lane_id() == 0) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) { atomicMin(&b[d], f2ord(lo[d])); atomicMax(&b[3 + d], f2ord(hi[d])); }
+    for (int d = 0; d < 3; ++d) { red[d][w] = lo[d]; red[3 + d][w] = hi[d]; }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int d = threadIdx.x;
+    float v = red[d][0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) v = d < 3 ? fminf(v, red[d][q]) : fmaxf(v, red[d][q]);
+    if (d < 3) atomicMin(&b[d], f2ord(v));
+    else atomicMax(&b[d], f2ord(v));
   }
 }
 
@@ -328,7 +339,7 @@ __device__ __forceinline__ int lbvh_delta(const uint32_t* __restrict__ keys, int
 
 // Karras 2012, "Maximizing parallelism in the construction of BVHs, octrees and k-d trees".
 __global__ __launch_bounds__(256) void k_hierarchy(const uint32_t* __restrict__ keys, int n, LNode* __restrict__ nodes,
-                                                    uint32_t* __restrict__ parent) {
+                                                    uint32_t* __restrict__ parent, uint32_t* __restrict__ first_out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   int d = (lbvh_delta(keys, n, i, i + 1) - lbvh_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -357,6 +368,7 @@ __global__ __launch_bounds__(256) void k_hierarchy(const uint32_t* __restrict__ 
   nodes[i].child_r = cr;
   nodes[i].last_l = (uint32_t)gamma;
   nodes[i].last_r = (uint32_t)last;
+  first_out[i] = (uint32_t)first;
 }
 
 __global__ void k_zero_u32(uint32_t* p, size_t n) {
@@ -376,22 +388,17 @@ int fill_u32(ncx_ctx* ctx, uint32_t* p, size_t n, uint32_t v) {
   return NCX_OK;
 }
 
-__device__ __forceinline__ void store_slot(LNode* nd, bool right, const float b[6]) {
-  float* f = (float*)nd;
-  int o = right ? 6 : 0;
-#pragma unroll
-  for (int k = 0; k < 6; ++k) __hip_atomic_store(&f[o + k], b[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// ------------------------------------------------------------------ refit
+// Node boxes live in the parent's child slots, so refitting writes every node's box into its
+// parent.  Stage 1 (one workgroup per REFIT_B consecutive leaves) refits every node whose leaf
+// range lies inside the workgroup's leaves, with flags and boxes in LDS (the second child to
+// arrive merges).  Stage 2 (one workgroup) refits the few nodes whose range spans workgroups,
+// deepest level first, with a barrier between levels.  No device-scope synchronisation is needed.
+#define REFIT_B 256
 
-__global__ __launch_bounds__(256) void k_refit(const float* __restrict__ minx, const float* __restrict__ miny,
-                                                const float* __restrict__ minz, const float* __restrict__ maxx,
-                                                const float* __restrict__ maxy, const float* __restrict__ maxz,
-                                                const uint32_t* __restrict__ order, int n, LNode* nodes,
-                                                const uint32_t* __restrict__ parent, uint32_t* flags, float expand) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  uint32_t p = order[k];
-  float b[6] = {minx[p], miny[p], minz[p], maxx[p], maxy[p], maxz[p]};
+__device__ __forceinline__ void leaf_box(const float* const* box, uint32_t p, float expand, float b[6]) {
+#pragma unroll
+  for (int d = 0; d < 6; ++d) b[d] = box[d][p];
   if (expand > 0.f) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -399,26 +406,165 @@ __global__ __launch_bounds__(256) void k_refit(const float* __restrict__ minx, c
       b[3 + d] += expand * (1.f + fabsf(b[3 + d]));
     }
   }
+}
+
+__device__ __forceinline__ void put_slot(LNode* nd, bool right, const float b[6]) {
+  float* f = (float*)nd + (right ? 6 : 0);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) f[k] = b[k];
+}
+
+struct BoxPtrs {
+  const float* p[6];
+};
+
+__global__ __launch_bounds__(REFIT_B) void k_refit_local(BoxPtrs bx, const uint32_t* __restrict__ order, int n,
+                                                          LNode* __restrict__ nodes,
+                                                          const uint32_t* __restrict__ parent,
+                                                          const uint32_t* __restrict__ first, float expand) {
+  __shared__ uint32_t lflag[REFIT_B];
+  __shared__ float lbox[REFIT_B][12];
+  const int tid = threadIdx.x;
+  lflag[tid] = 0u;
+  __syncthreads();
+  const int b0 = blockIdx.x * REFIT_B;
+  const int k = b0 + tid;
+  if (k >= n) return;
+  const int b1 = min(b0 + REFIT_B, n) - 1;
+  float b[6];
+  leaf_box(bx.p, order[k], expand, b);
   uint32_t pe = parent[(n - 1) + k];
   for (;;) {
-    uint32_t par = pe & ~NCX_LEAF_BIT;
-    bool right = (pe & NCX_LEAF_BIT) != 0;
-    store_slot(&nodes[par], right, b);
-    uint32_t old = __hip_atomic_fetch_add(&flags[par], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;
-    // Second arrival: both child slots are final; merge them.
-    const float* f = (const float*)&nodes[par];
-    float s[12];
+    uint32_t par = pe & ~NCX_RIGHT_BIT;
+    bool right = (pe & NCX_RIGHT_BIT) != 0;
+    put_slot(&nodes[par], right, b);
+    int fl = (int)first[par], la = (int)nodes[par].last_r;
+    if (fl < b0 || la > b1) return;  // spans workgroups: stage 2
+    int slot = (int)par - b0;
 #pragma unroll
-    for (int q = 0; q < 12; ++q) s[q] = __hip_atomic_load(&f[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int q = 0; q < 6; ++q) lbox[slot][(right ? 6 : 0) + q] = b[q];
+    uint32_t old = __hip_atomic_fetch_add(&lflag[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old == 0u) return;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      b[d] = fminf(s[d], s[6 + d]);
-      b[3 + d] = fmaxf(s[3 + d], s[9 + d]);
+      b[d] = fminf(lbox[slot][d], lbox[slot][6 + d]);
+      b[3 + d] = fmaxf(lbox[slot][3 + d], lbox[slot][9 + d]);
     }
     if (par == 0) return;
     pe = parent[par];
   }
+}
+
+__global__ __launch_bounds__(1024) void k_refit_upper(LNode* __restrict__ nodes, const uint32_t* __restrict__ parent,
+                                                       const uint32_t* __restrict__ upper,
+                                                       const uint32_t* __restrict__ level_off, int nlevels) {
+  for (int lv = 0; lv < nlevels; ++lv) {
+    uint32_t a = level_off[lv], e = level_off[lv + 1];
+    for (uint32_t i = a + threadIdx.x; i < e; i += blockDim.x) {
+      uint32_t p = upper[i];
+      const float* f = (const float*)&nodes[p];
+      float b[6];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        b[d] = fminf(f[d], f[6 + d]);
+        b[3 + d] = fmaxf(f[3 + d], f[9 + d]);
+      }
+      if (p != 0) {
+        uint32_t pe = parent[p];
+        put_slot(&nodes[pe & ~NCX_RIGHT_BIT], (pe & NCX_RIGHT_BIT) != 0, b);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Depth of every internal node and whether its leaf range spans refit workgroups.
+__global__ __launch_bounds__(256) void k_upper_mark(const LNode* __restrict__ nodes, const uint32_t* __restrict__ parent,
+                                                     const uint32_t* __restrict__ first, int n,
+                                                     uint32_t* __restrict__ is_upper, uint32_t* __restrict__ depth) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  uint32_t d = 0, p = (uint32_t)i;
+  while (p != 0 && d < (uint32_t)n) {
+    p = parent[p] & ~NCX_RIGHT_BIT;
+    d++;
+  }
+  depth[i] = d;
+  is_upper[i] = (first[i] / REFIT_B != nodes[i].last_r / REFIT_B) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_upper_compact(const uint32_t* __restrict__ is_upper,
+                                                        const uint32_t* __restrict__ pos,
+                                                        const uint32_t* __restrict__ depth, int m,
+                                                        uint32_t* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || !is_upper[i]) return;
+  out[2 * pos[i]] = (uint32_t)i;
+  out[2 * pos[i] + 1] = depth[i];
+}
+
+static int refit_prepare(ncx_ctx* ctx, LBVH& bvh) {
+  size_t n = bvh.n;
+  size_t m = n - 1;
+  int rc;
+  DevBuf isu, pos, dep, total, lst;
+  if ((rc = isu.reserve(m * 4)) || (rc = pos.reserve(m * 4)) || (rc = dep.reserve(m * 4)) ||
+      (rc = total.reserve(16)))
+    return rc;
+  hipLaunchKernelGGL(k_upper_mark, dim3(ceil_div(m, 256)), dim3(256), 0, ctx->stream, bvh.nodes.as<LNode>(),
+                     bvh.parent.as<uint32_t>(), bvh.first.as<uint32_t>(), (int)n, isu.as<uint32_t>(),
+                     dep.as<uint32_t>());
+  NCX_HIP_CHECK(hipGetLastError());
+  if ((rc = scan_exclusive_u32(ctx, isu.as<uint32_t>(), pos.as<uint32_t>(), m, bvh.scan_tmp, total.as<uint32_t>())))
+    return rc;
+  uint32_t nu = 0;
+  NCX_HIP_CHECK(hipMemcpyAsync(&nu, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+  NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  std::vector<uint32_t> pairs(2 * (size_t)nu);
+  if (nu) {
+    if ((rc = lst.reserve((size_t)nu * 8))) return rc;
+    hipLaunchKernelGGL(k_upper_compact, dim3(ceil_div(m, 256)), dim3(256), 0, ctx->stream, isu.as<uint32_t>(),
+                       pos.as<uint32_t>(), dep.as<uint32_t>(), (int)m, lst.as<uint32_t>());
+    NCX_HIP_CHECK(hipGetLastError());
+    NCX_HIP_CHECK(hipMemcpyAsync(pairs.data(), lst.p, (size_t)nu * 8, hipMemcpyDeviceToHost, ctx->stream));
+    NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+  // deepest level first
+  uint32_t maxd = 0;
+  for (uint32_t i = 0; i < nu; ++i) maxd = std::max(maxd, pairs[2 * i + 1]);
+  std::vector<std::vector<uint32_t>> byd(nu ? maxd + 1 : 0);
+  for (uint32_t i = 0; i < nu; ++i) byd[pairs[2 * i + 1]].push_back(pairs[2 * i]);
+  std::vector<uint32_t> order, off(1, 0);
+  for (int d = (int)byd.size() - 1; d >= 0; --d) {
+    if (byd[d].empty()) continue;
+    order.insert(order.end(), byd[d].begin(), byd[d].end());
+    off.push_back((uint32_t)order.size());
+  }
+  bvh.n_levels = (int)off.size() - 1;
+  if ((rc = bvh.upper.reserve((order.size() + 1) * 4)) || (rc = bvh.upper_off.reserve(off.size() * 4))) return rc;
+  if (!order.empty())
+    NCX_HIP_CHECK(hipMemcpyAsync(bvh.upper.p, order.data(), order.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  NCX_HIP_CHECK(hipMemcpyAsync(bvh.upper_off.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  isu.release(); pos.release(); dep.release(); total.release(); lst.release();
+  return NCX_OK;
+}
+
+static int refit_launch(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], float expand) {
+  size_t n = bvh.n;
+  BoxPtrs bp;
+  for (int d = 0; d < 6; ++d) bp.p[d] = box[d];
+  hipLaunchKernelGGL(k_refit_local, dim3(ceil_div(n, REFIT_B)), dim3(REFIT_B), 0, ctx->stream, bp,
+                     bvh.order.as<uint32_t>(), (int)n, bvh.nodes.as<LNode>(), bvh.parent.as<uint32_t>(),
+                     bvh.first.as<uint32_t>(), expand);
+  NCX_HIP_CHECK(hipGetLastError());
+  if (bvh.n_levels > 0) {
+    hipLaunchKernelGGL(k_refit_upper, dim3(1), dim3(1024), 0, ctx->stream, bvh.nodes.as<LNode>(),
+                       bvh.parent.as<uint32_t>(), bvh.upper.as<uint32_t>(), bvh.upper_off.as<uint32_t>(),
+                       bvh.n_levels);
+    NCX_HIP_CHECK(hipGetLastError());
+  }
+  return NCX_OK;
 }
 
 int lbvh_build(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], size_t n, float expand) {
@@ -435,7 +581,7 @@ int lbvh_build(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], size_t n, flo
   uint32_t* b = bvh.bounds.as<uint32_t>();
   hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, ctx->stream, b);
   unsigned gb = ceil_div(n, 256);
-  if (gb > 1024) gb = 1024;
+  if (gb > 256) gb = 256;
   hipLaunchKernelGGL(k_centroid_bounds, dim3(gb), dim3(256), 0, ctx->stream, box[0], box[1], box[2], box[3], box[4],
                      box[5], n, b);
   hipLaunchKernelGGL(k_morton, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, box[0], box[1], box[2], box[3],
@@ -445,24 +591,16 @@ int lbvh_build(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], size_t n, flo
                            bvh.tmp_vals, bvh.hist, bvh.scan_tmp)))
     return rc;
   if (n == 1) return NCX_OK;
+  if ((rc = bvh.first.reserve((n - 1) * 4))) return rc;
   hipLaunchKernelGGL(k_hierarchy, dim3(ceil_div(n - 1, 256)), dim3(256), 0, ctx->stream, bvh.keys.as<uint32_t>(),
-                     (int)n, bvh.nodes.as<LNode>(), bvh.parent.as<uint32_t>());
-  hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, bvh.flags.as<uint32_t>(), n);
-  hipLaunchKernelGGL(k_refit, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, box[0], box[1], box[2], box[3],
-                     box[4], box[5], bvh.order.as<uint32_t>(), (int)n, bvh.nodes.as<LNode>(),
-                     bvh.parent.as<uint32_t>(), bvh.flags.as<uint32_t>(), expand);
+                     (int)n, bvh.nodes.as<LNode>(), bvh.parent.as<uint32_t>(), bvh.first.as<uint32_t>());
   NCX_HIP_CHECK(hipGetLastError());
-  return NCX_OK;
+  if ((rc = refit_prepare(ctx, bvh))) return rc;
+  return refit_launch(ctx, bvh, box, expand);
 }
 
 // Refit only (same leaf order and topology), used when boxes move but the proxy set is unchanged.
 int lbvh_refit(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], float expand) {
-  size_t n = bvh.n;
-  if (n <= 1) return NCX_OK;
-  hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, bvh.flags.as<uint32_t>(), n);
-  hipLaunchKernelGGL(k_refit, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, box[0], box[1], box[2], box[3],
-                     box[4], box[5], bvh.order.as<uint32_t>(), (int)n, bvh.nodes.as<LNode>(),
-                     bvh.parent.as<uint32_t>(), bvh.flags.as<uint32_t>(), expand);
-  NCX_HIP_CHECK(hipGetLastError());
-  return NCX_OK;
+  if (bvh.n <= 1) return NCX_OK;
+  return refit_launch(ctx, bvh, box, expand);
 }

@@ -162,7 +162,11 @@ struct LBVH {
   DevBuf order;          // uint32 sorted position -> primitive index
   DevBuf keys;           // Morton codes (sorted)
   DevBuf tmp_keys, tmp_vals, hist, flags, bounds, scan_tmp;
+  DevBuf first;          // uint32 first sorted leaf position under each internal node
+  DevBuf upper, upper_off;  // refit stage-2 node list (deepest level first) and level offsets
+  int n_levels = 0;
   void release() {
+    first.release(); upper.release(); upper_off.release();
     nodes.release(); parent.release(); order.release(); keys.release();
     tmp_keys.release(); tmp_vals.release(); hist.release(); flags.release(); bounds.release(); scan_tmp.release();
   }
@@ -186,3 +190,9 @@ int radix_sort_pairs64(ncx_ctx* ctx, uint32_t* hi, uint32_t* lo, size_t n, int h
                        DevBuf& t2, DevBuf& t3, DevBuf& hist, DevBuf& scan_tmp);
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Contact kernel over pairs counted on the device (ncx_narrow.hip); no host synchronisation.
+int contacts_launch_device_count(ncx_shapes* S, ncx_ctx* ctx, const ncx_pose* poses, const uint32_t* ia,
+                                 const uint32_t* ib, const uint32_t* n_dev, size_t cap, float prediction,
+                                 const float* init_dir, int max_iter, uint8_t* has, float* w1, float* w2, float* nrm,
+                                 float* depth, int32_t* iters);

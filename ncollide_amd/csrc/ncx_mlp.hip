@@ -438,17 +438,22 @@ __global__ void k_to_half(const float* __restrict__ P, _Float16* __restrict__ H)
 }
 
 // Fused inference: features -> MLP -> world direction, one wave per 16 pairs.
+// `shapes` (nullable): pairs that are not polytope-polytope (cuboid / convex hull on both sides)
+// get an all-zero direction, i.e. the reference's default.  `n_dev` (nullable): device-side count
+// capping n.
 __global__ __launch_bounds__(256) void k_predict_dirs(const _Float16* __restrict__ Wh, const float* __restrict__ P,
                                                        const ncx_pose* __restrict__ poses,
                                                        const float* __restrict__ desc,
                                                        const uint32_t* __restrict__ ia,
-                                                       const uint32_t* __restrict__ ib, size_t n,
-                                                       float* __restrict__ dirs) {
+                                                       const uint32_t* __restrict__ ib, size_t ncap,
+                                                       float* __restrict__ dirs, const ncx_shape* __restrict__ shapes,
+                                                       const uint32_t* __restrict__ n_dev) {
   __shared__ _Float16 xs[4][16 * MLP_INP];
   const int l = __lane_id();
   const int w = threadIdx.x / 64;
   const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const size_t nw = ((size_t)gridDim.x * blockDim.x) / 64;
+  const size_t n = n_dev ? min(ncap, (size_t)*n_dev) : ncap;
   for (size_t row0 = wave * 16; row0 < n; row0 += nw * 16) {
     // 4 lanes per pair build the 48 features (12 each)
     size_t row = row0 + (l >> 2);
@@ -467,10 +472,16 @@ __global__ __launch_bounds__(256) void k_predict_dirs(const _Float16* __restrict
     __builtin_amdgcn_wave_barrier();
     size_t r2 = row0 + (l & 15);
     if ((l >> 4) == 0 && r2 < n) {
+      bool poly = true;
+      if (shapes) {
+        uint32_t t1 = shapes[ia[r2]].type, t2 = shapes[ib[r2]].type;
+        poly = (t1 == NCX_CUBOID || t1 == NCX_CONVEX_HULL) && (t2 == NCX_CUBOID || t2 == NCX_CONVEX_HULL);
+      }
       iso m1 = load_pose(poses, ia[r2]);
       f3 d = xform_vec(m1, f3{z3[0], z3[1], z3[2]});
       float nl = sqrtf(norm_sq(d));
       if (!(nl > 1e-6f)) d = f3{1.f, 0.f, 0.f}, nl = 1.f;
+      if (!poly) d = f3{0.f, 0.f, 0.f}, nl = 1.f;
       dirs[3 * r2] = d.x / nl;
       dirs[3 * r2 + 1] = d.y / nl;
       dirs[3 * r2 + 2] = d.z / nl;
@@ -636,7 +647,26 @@ extern "C" int ncx_mlp_predict_dirs(ncx_mlp* m, const ncx_pose* poses_dev, const
   unsigned blocks = ceil_div(ceil_div(n, 16), 4);
   if (blocks > (unsigned)m->ctx->num_cus * 16) blocks = (unsigned)m->ctx->num_cus * 16;
   hipLaunchKernelGGL(k_predict_dirs, dim3(blocks), dim3(256), 0, m->ctx->stream, m->half.as<_Float16>(),
-                     m->master.as<float>(), poses_dev, desc_dev, ia_dev, ib_dev, n, dirs_dev);
+                     m->master.as<float>(), poses_dev, desc_dev, ia_dev, ib_dev, n, dirs_dev,
+                     (const ncx_shape*)nullptr, (const uint32_t*)nullptr);
+  NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
+}
+
+int mlp_predict_device_count(ncx_mlp* m, ncx_ctx* ctx, const ncx_shape* shapes, const ncx_pose* poses,
+                             const float* desc, const uint32_t* ia, const uint32_t* ib, const uint32_t* n_dev,
+                             size_t cap, float* dirs) {
+  unsigned blocks = (unsigned)ctx->num_cus * 8;
+  hipLaunchKernelGGL(k_predict_dirs, dim3(blocks), dim3(256), 0, ctx->stream, m->half.as<_Float16>(),
+                     m->master.as<float>(), poses, desc, ia, ib, cap, dirs, shapes, n_dev);
+  NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
+}
+
+int shape_descriptors_on(ncx_shapes* s, ncx_ctx* ctx, float* desc) {
+  if (s->n == 0) return NCX_OK;
+  hipLaunchKernelGGL(k_descriptors, dim3(ceil_div(s->n * MLP_DESC, 256)), dim3(256), 0, ctx->stream,
+                     s->shapes.as<ncx_shape>(), s->hx.as<float>(), s->hy.as<float>(), s->hz.as<float>(), s->n, desc);
   NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
 }

@@ -997,6 +997,7 @@ struct KArgs {
   int distance_mode;
   uint32_t nshapes;
   uint32_t* bad_index;  // set to 1 when a pair names a shape >= nshapes
+  const uint32_t* n_dev;  // optional device-side pair count (capped by npairs)
 };
 
 template <int G>
@@ -1005,7 +1006,9 @@ __global__ __launch_bounds__(256) void k_pairs(KArgs a) {
   const size_t group = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const size_t ngroups = ((size_t)gridDim.x * blockDim.x) / G;
   Arena A = arena_at(a.arena, group, a.vcap, a.fcap);
-  for (size_t q = group; q < a.npairs; q += ngroups) {
+  size_t npairs = a.npairs;
+  if (a.n_dev) npairs = min(npairs, (size_t)*a.n_dev);
+  for (size_t q = group; q < npairs; q += ngroups) {
     size_t k = a.list ? a.list[q] : q;
     uint32_t i1 = a.ia[k], i2 = a.ib[k];
     if (i1 >= a.nshapes || i2 >= a.nshapes) {
@@ -1140,6 +1143,46 @@ template <int G>
 static int launch_pairs(ncx_shapes* S, KArgs a, size_t npairs, unsigned blocks) {
   a.npairs = npairs;
   hipLaunchKernelGGL(k_pairs<G>, dim3(blocks), dim3(256), 0, S->ctx->stream, a);
+  NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
+}
+
+// Graph-safe launch over a device-counted pair list: no host synchronisation.  EPA arena
+// overflows are appended to S->ovf (count at [0], pair indices from [1]); the caller checks
+// them after the work completes (contacts_finish).
+int contacts_launch_device_count(ncx_shapes* S, ncx_ctx* ctx, const ncx_pose* poses, const uint32_t* ia,
+                                 const uint32_t* ib, const uint32_t* n_dev, size_t cap, float prediction,
+                                 const float* init_dir, int max_iter, uint8_t* has, float* w1, float* w2, float* nrm,
+                                 float* depth, int32_t* iters) {
+  KArgs a = {};
+  a.shapes = S->shapes.as<ncx_shape>();
+  a.P = Pool{S->hx.as<float>(), S->hy.as<float>(), S->hz.as<float>()};
+  a.poses = poses;
+  a.ia = ia;
+  a.ib = ib;
+  a.prediction = prediction;
+  a.init_dir = init_dir;
+  a.max_iter = max_iter;
+  a.has = has; a.w1 = w1; a.w2 = w2; a.nrm = nrm; a.depth = depth; a.iters = iters;
+  a.distance_mode = 0;
+  a.n_dev = n_dev;
+  const int G = 16;
+  unsigned blocks = (unsigned)ctx->num_cus * 8;
+  uint32_t vcap = 160, fcap = 320;
+  size_t ngroups = (size_t)blocks * 256 / G;
+  int rc;
+  if ((rc = S->arena.reserve(ngroups * arena_bytes(vcap, fcap)))) return rc;
+  if ((rc = fill_u32(ctx, S->ovf.as<uint32_t>(), 2, 0u))) return rc;
+  if ((rc = fill_u32(ctx, S->ovf.as<uint32_t>() + 4095, 1, 0u))) return rc;
+  a.arena = S->arena.as<uint8_t>();
+  a.vcap = vcap;
+  a.fcap = fcap;
+  a.ovf = S->ovf.as<uint32_t>();
+  a.ovf_cap = 4090;
+  a.nshapes = (uint32_t)S->n;
+  a.bad_index = S->ovf.as<uint32_t>() + 4095;
+  a.npairs = cap;
+  hipLaunchKernelGGL(k_pairs<16>, dim3(blocks), dim3(256), 0, ctx->stream, a);
   NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
 }
@@ -1334,6 +1377,15 @@ __global__ __launch_bounds__(256) void k_shape_aabbs(const ncx_shape* __restrict
   }
   b0[i] = mn.x; b1[i] = mn.y; b2[i] = mn.z;
   b3[i] = mx.x; b4[i] = mx.y; b5[i] = mx.z;
+}
+
+int shapes_aabbs_on(ncx_shapes* s, ncx_ctx* ctx, const ncx_pose* poses, float margin, float* const out[6]) {
+  if (s->n == 0) return NCX_OK;
+  hipLaunchKernelGGL(k_shape_aabbs, dim3(ceil_div(s->n, 256)), dim3(256), 0, ctx->stream,
+                     s->shapes.as<ncx_shape>(), Pool{s->hx.as<float>(), s->hy.as<float>(), s->hz.as<float>()}, poses,
+                     s->n, margin, out[0], out[1], out[2], out[3], out[4], out[5]);
+  NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
 }
 
 extern "C" int ncx_shapes_aabbs_dev(ncx_shapes* s, const ncx_pose* poses, float margin, float* const out[6]) {

@@ -1,0 +1,395 @@
+// ncx_world.hip -- one collision step of a whole scene on the GPU, capturable in a hipGraph.
+//
+// Stands in for one CollisionWorld::update (src/pipeline/world.rs:104-119) restricted to the hot
+// path: every shape is a broad-phase proxy; per step
+//   1. world AABBs loosened by the query limit (glue/setup.rs:27-28; collision_object.rs:100-113),
+//   2. the DBVTBroadPhase stored-box rule: a proxy's box is replaced by box.loosened(margin) only
+//      when the new box is not contained in it (dbvt_broad_phase.rs:325-347); the first step
+//      stores the boxes as given (create_proxy, :275-280),
+//   3. LBVH refit over the stored boxes (rebuilt every `rebuild_every` steps),
+//   4. all intersecting stored-box pairs, counted, prefix-summed and written on the device,
+//   5. query::contact on every pair (contact_shape_shape.rs:10-48), optionally with the learned
+//      init_dir on polytope pairs.
+// Steps 1-5 contain no host synchronisation, so a refit step is captured once in a hipGraph and
+// replayed; rebuild steps run eagerly and invalidate the graph.
+#include "ncx_internal.h"
+
+#define WQ_BLOCK 128
+#define WQ_STACK 48
+
+struct ncx_mlp;
+int mlp_predict_device_count(ncx_mlp* m, ncx_ctx* ctx, const ncx_shape* shapes, const ncx_pose* poses,
+                             const float* desc, const uint32_t* ia, const uint32_t* ib, const uint32_t* n_dev,
+                             size_t cap, float* dirs);
+int shapes_aabbs_on(ncx_shapes* s, ncx_ctx* ctx, const ncx_pose* poses, float margin, float* const out[6]);
+int shape_descriptors_on(ncx_shapes* s, ncx_ctx* ctx, float* desc);
+
+struct ncx_world {
+  ncx_ctx* ctx;   // caller context (its stream orders the step with the caller's work)
+  ncx_ctx wctx;   // the world's own stream (capturable)
+  ncx_shapes* shapes;
+  size_t n, cap;
+  float margin, prediction;
+  int rebuild_every = 8;
+  int since_build = 0;
+  bool first = true;
+  DevBuf box_new, stored;
+  LBVH bvh;
+  uint32_t depth = 0;
+  DevBuf gstack;
+  DevBuf cnt, offs, counters;  // counters: [0] pair total, [1] pair overflow
+  DevBuf pa, pb;
+  DevBuf has, w1, w2, nrm, depth_out, iters;
+  ncx_mlp* mlp = nullptr;
+  DevBuf desc, dirs;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const ncx_pose* graph_poses = nullptr;
+  bool graph_valid = false;
+  int last_was_graph = 0;
+};
+
+__global__ void k_world_stage(const float* __restrict__ nb, float* __restrict__ st, size_t n, float margin,
+                              int first) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float b[6];
+#pragma unroll
+  for (int d = 0; d < 6; ++d) b[d] = nb[d * n + i];
+  if (first) {
+#pragma unroll
+    for (int d = 0; d < 6; ++d) st[d * n + i] = b[d];
+    return;
+  }
+  // AABB::contains (aabb.rs:161-163)
+  bool contained = st[0 * n + i] <= b[0] && st[1 * n + i] <= b[1] && st[2 * n + i] <= b[2] &&
+                   st[3 * n + i] >= b[3] && st[4 * n + i] >= b[4] && st[5 * n + i] >= b[5];
+  if (!contained) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      st[d * n + i] = b[d] + (-margin);
+      st[(3 + d) * n + i] = b[3 + d] + margin;
+    }
+  }
+}
+
+__device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float mny, float mnz, float mxx, float mxy,
+                                             float mxz) {
+  return mnx <= q[3] && mny <= q[4] && mnz <= q[5] && mxx >= q[0] && mxy >= q[1] && mxz >= q[2];
+}
+
+// Self-query of sorted leaf k against leaves at later positions.  WRITE: emit into offs[k]...
+template <bool WRITE, bool DEEP>
+__global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restrict__ nodes,
+                                                           const uint32_t* __restrict__ order,
+                                                           const float* __restrict__ boxes, uint32_t n,
+                                                           uint32_t* __restrict__ cnt, const uint32_t* __restrict__ offs,
+                                                           size_t cap, uint32_t* __restrict__ pa,
+                                                           uint32_t* __restrict__ pb, uint32_t* __restrict__ counters,
+                                                           uint32_t* __restrict__ gstack, uint32_t depth) {
+  __shared__ uint32_t lstack[DEEP ? 1 : WQ_STACK * WQ_BLOCK];
+  const int tid = threadIdx.x;
+  uint32_t k = blockIdx.x * WQ_BLOCK + tid;
+  if (k >= n) return;
+  uint32_t* gs = gstack + (size_t)k * depth;
+  uint32_t prim = order[k];
+  float q[6];
+#pragma unroll
+  for (int d = 0; d < 6; ++d) q[d] = boxes[(size_t)d * n + prim];
+  uint32_t c = 0;
+  size_t w = WRITE ? offs[k] : 0;
+  auto emit = [&](uint32_t leafpos) {
+    if (WRITE) {
+      if (w < cap) {
+        uint32_t h2 = order[leafpos];
+        pa[w] = prim < h2 ? prim : h2;
+        pb[w] = prim < h2 ? h2 : prim;
+      } else {
+        counters[1] = 1u;
+      }
+      w++;
+    } else {
+      c++;
+    }
+  };
+  int sp = 0;
+  uint32_t node = 0;
+  for (;;) {
+    const float4* nq = (const float4*)(nodes + node);
+    float4 q0 = nq[0], q1 = nq[1], q2 = nq[2], q3 = nq[3];
+    uint32_t cl = __float_as_uint(q3.x), cr = __float_as_uint(q3.y);
+    uint32_t ll = __float_as_uint(q3.z), lr = __float_as_uint(q3.w);
+    bool hl = ll > k && wbox_overlap(q, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
+    bool hr = lr > k && wbox_overlap(q, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
+    if (hl && (cl & NCX_LEAF_BIT)) { emit(cl & ~NCX_LEAF_BIT); hl = false; }
+    if (hr && (cr & NCX_LEAF_BIT)) { emit(cr & ~NCX_LEAF_BIT); hr = false; }
+    if (hl && hr) {
+      if (DEEP) gs[sp] = cr;
+      else lstack[sp * WQ_BLOCK + tid] = cr;
+      sp++;
+      node = cl;
+    } else if (hl) node = cl;
+    else if (hr) node = cr;
+    else {
+      if (sp == 0) break;
+      sp--;
+      node = DEEP ? gs[sp] : lstack[sp * WQ_BLOCK + tid];
+    }
+  }
+  if (!WRITE) cnt[k] = c;
+}
+
+__global__ void k_world_tree_depth(const uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ out) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  uint32_t d = 1, p = parent[(n - 1) + k] & ~NCX_RIGHT_BIT;
+  while (p != 0 && d <= n) {
+    p = parent[p] & ~NCX_RIGHT_BIT;
+    d++;
+  }
+  atomicMax(out, d);
+}
+
+static int world_build(ncx_world* w) {
+  ncx_ctx* c = &w->wctx;
+  const float* box[6];
+  for (int d = 0; d < 6; ++d) box[d] = w->stored.as<float>() + d * w->n;
+  int rc = lbvh_build(c, w->bvh, box, w->n, 0.f);
+  if (rc) return rc;
+  if (w->n > 1) {
+    if ((rc = fill_u32(c, w->counters.as<uint32_t>() + 2, 1, 0u))) return rc;
+    hipLaunchKernelGGL(k_world_tree_depth, dim3(ceil_div(w->n, 256)), dim3(256), 0, c->stream,
+                       w->bvh.parent.as<uint32_t>(), (uint32_t)w->n, w->counters.as<uint32_t>() + 2);
+    uint32_t depth = 0;
+    NCX_HIP_CHECK(hipMemcpyAsync(&depth, w->counters.as<uint32_t>() + 2, 4, hipMemcpyDeviceToHost, c->stream));
+    NCX_HIP_CHECK(hipStreamSynchronize(c->stream));
+    w->depth = depth;
+    if (depth >= WQ_STACK && (rc = w->gstack.reserve(w->n * (size_t)(depth + 1) * 4))) return rc;
+  }
+  w->since_build = 0;
+  return NCX_OK;
+}
+
+// Enqueues steps 1, 2 and 4, 5 (and refit when `refit`) on the world stream.
+static int world_enqueue(ncx_world* w, const ncx_pose* poses, bool refit) {
+  ncx_ctx* c = &w->wctx;
+  size_t n = w->n;
+  int rc;
+  float* nb[6];
+  for (int d = 0; d < 6; ++d) nb[d] = w->box_new.as<float>() + d * n;
+  if ((rc = shapes_aabbs_on(w->shapes, c, poses, w->prediction, nb))) return rc;
+  hipLaunchKernelGGL(k_world_stage, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, w->box_new.as<float>(),
+                     w->stored.as<float>(), n, w->margin, w->first ? 1 : 0);
+  NCX_HIP_CHECK(hipGetLastError());
+  if (refit) {
+    const float* box[6];
+    for (int d = 0; d < 6; ++d) box[d] = w->stored.as<float>() + d * n;
+    if ((rc = lbvh_refit(c, w->bvh, box, 0.f))) return rc;
+  }
+  return NCX_OK;
+}
+
+static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses) {
+  ncx_ctx* c = &w->wctx;
+  size_t n = w->n;
+  int rc;
+  uint32_t* ctr = w->counters.as<uint32_t>();
+  if ((rc = fill_u32(c, ctr, 2, 0u))) return rc;
+  if (n >= 2) {
+    bool deep = w->depth >= WQ_STACK;
+    unsigned nb = ceil_div(n, WQ_BLOCK);
+    const LNode* nodes = w->bvh.nodes.as<LNode>();
+    const uint32_t* order = w->bvh.order.as<uint32_t>();
+    const float* st = w->stored.as<float>();
+    uint32_t* gs = deep ? w->gstack.as<uint32_t>() : nullptr;
+    uint32_t dd = w->depth + 1;
+    if (deep)
+      hipLaunchKernelGGL((k_world_query<false, true>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
+                         (uint32_t)n, w->cnt.as<uint32_t>(), (const uint32_t*)nullptr, w->cap, (uint32_t*)nullptr,
+                         (uint32_t*)nullptr, ctr, gs, dd);
+    else
+      hipLaunchKernelGGL((k_world_query<false, false>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
+                         (uint32_t)n, w->cnt.as<uint32_t>(), (const uint32_t*)nullptr, w->cap, (uint32_t*)nullptr,
+                         (uint32_t*)nullptr, ctr, gs, dd);
+    NCX_HIP_CHECK(hipGetLastError());
+    if ((rc = scan_exclusive_u32(c, w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), n, w->bvh.scan_tmp, ctr)))
+      return rc;
+    if (deep)
+      hipLaunchKernelGGL((k_world_query<true, true>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
+                         (uint32_t)n, (uint32_t*)nullptr, w->offs.as<uint32_t>(), w->cap, w->pa.as<uint32_t>(),
+                         w->pb.as<uint32_t>(), ctr, gs, dd);
+    else
+      hipLaunchKernelGGL((k_world_query<true, false>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
+                         (uint32_t)n, (uint32_t*)nullptr, w->offs.as<uint32_t>(), w->cap, w->pa.as<uint32_t>(),
+                         w->pb.as<uint32_t>(), ctr, gs, dd);
+    NCX_HIP_CHECK(hipGetLastError());
+  }
+  const float* idir = nullptr;
+  if (w->mlp) {
+    if ((rc = mlp_predict_device_count(w->mlp, c, w->shapes->shapes.as<ncx_shape>(), poses, w->desc.as<float>(),
+                                       w->pa.as<uint32_t>(), w->pb.as<uint32_t>(), ctr, w->cap, w->dirs.as<float>())))
+      return rc;
+    idir = w->dirs.as<float>();
+  }
+  // Pairs whose warm start hits the 32-iteration cap are recomputed by plain GJK inside the kernel.
+  return contacts_launch_device_count(w->shapes, c, poses, w->pa.as<uint32_t>(), w->pb.as<uint32_t>(), ctr, w->cap,
+                                      w->prediction, idir, w->mlp ? 32 : 10000, w->has.as<uint8_t>(),
+                                      w->w1.as<float>(), w->w2.as<float>(), w->nrm.as<float>(),
+                                      w->depth_out.as<float>(), w->iters.as<int32_t>());
+}
+
+extern "C" int ncx_world_create(ncx_ctx* ctx, ncx_shapes* shapes, float margin, float prediction, size_t max_pairs,
+                                ncx_world** out) {
+  NCX_REQUIRE(ctx && shapes && out, "ncx_world_create: null argument");
+  NCX_REQUIRE(margin >= 0.f, "ncx_world_create: the loosening margin must be positive");
+  NCX_REQUIRE(max_pairs > 0 && max_pairs < 0xFFFFFFFFull, "ncx_world_create: max_pairs out of range");
+  ncx_world* w = new ncx_world;
+  w->ctx = ctx;
+  w->wctx = *ctx;
+  w->shapes = shapes;
+  w->n = shapes->n;
+  w->cap = max_pairs;
+  w->margin = margin;
+  w->prediction = prediction;
+  size_t n = w->n, cap = max_pairs;
+  int rc;
+  if (hipStreamCreateWithFlags(&w->wctx.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&w->ev_in, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&w->ev_out, hipEventDisableTiming) != hipSuccess) {
+    delete w;
+    ncx_set_error("ncx_world_create: stream/event creation failed");
+    return NCX_E_HIP;
+  }
+  if ((rc = w->box_new.reserve(n * 24 + 64)) || (rc = w->stored.reserve(n * 24 + 64)) ||
+      (rc = w->cnt.reserve(n * 4 + 64)) || (rc = w->offs.reserve(n * 4 + 64)) || (rc = w->counters.reserve(64)) ||
+      (rc = w->pa.reserve(cap * 4)) || (rc = w->pb.reserve(cap * 4)) || (rc = w->has.reserve(cap)) ||
+      (rc = w->w1.reserve(cap * 12)) || (rc = w->w2.reserve(cap * 12)) || (rc = w->nrm.reserve(cap * 12)) ||
+      (rc = w->depth_out.reserve(cap * 4)) || (rc = w->iters.reserve(cap * 4))) {
+    delete w;
+    return rc;
+  }
+  *out = w;
+  return NCX_OK;
+}
+
+extern "C" int ncx_world_set_warmstart(ncx_world* w, ncx_mlp* mlp) {
+  NCX_REQUIRE(w, "ncx_world_set_warmstart: null world");
+  w->mlp = mlp;
+  w->graph_valid = false;
+  if (mlp) {
+    int rc;
+    if ((rc = w->desc.reserve(w->n * 16 * 4 + 64)) || (rc = w->dirs.reserve(w->cap * 12))) return rc;
+    if ((rc = shape_descriptors_on(w->shapes, &w->wctx, w->desc.as<float>()))) return rc;
+    NCX_HIP_CHECK(hipStreamSynchronize(w->wctx.stream));
+  }
+  return NCX_OK;
+}
+
+extern "C" int ncx_world_set_rebuild_every(ncx_world* w, int k) {
+  NCX_REQUIRE(w && k >= 1, "ncx_world_set_rebuild_every: bad argument");
+  w->rebuild_every = k;
+  return NCX_OK;
+}
+
+extern "C" int ncx_world_destroy(ncx_world* w) {
+  if (!w) return NCX_OK;
+  if (w->exec) (void)hipGraphExecDestroy(w->exec);
+  if (w->graph) (void)hipGraphDestroy(w->graph);
+  if (w->wctx.stream) {
+    (void)hipStreamSynchronize(w->wctx.stream);
+    (void)hipStreamDestroy(w->wctx.stream);
+  }
+  if (w->ev_in) (void)hipEventDestroy(w->ev_in);
+  if (w->ev_out) (void)hipEventDestroy(w->ev_out);
+  DevBuf* bufs[] = {&w->box_new, &w->stored, &w->gstack, &w->cnt, &w->offs, &w->counters, &w->pa, &w->pb,
+                    &w->has, &w->w1, &w->w2, &w->nrm, &w->depth_out, &w->iters, &w->desc, &w->dirs};
+  for (DevBuf* b : bufs) b->release();
+  w->bvh.release();
+  delete w;
+  return NCX_OK;
+}
+
+// One step.  use_graph != 0: refit steps replay a captured hipGraph (captured on first use, and
+// again after a rebuild or when `poses_dev` changes).  The step is ordered after prior work on the
+// caller's stream and later caller work waits for it; ncx_world_sync waits for completion.
+extern "C" int ncx_world_step(ncx_world* w, const ncx_pose* poses_dev, int use_graph) {
+  NCX_REQUIRE(w && poses_dev, "ncx_world_step: null argument");
+  hipStream_t s = w->wctx.stream;
+  NCX_HIP_CHECK(hipEventRecord(w->ev_in, w->ctx->stream));
+  NCX_HIP_CHECK(hipStreamWaitEvent(s, w->ev_in, 0));
+  int rc;
+  bool rebuild = w->first || w->since_build >= w->rebuild_every;
+  w->last_was_graph = 0;
+  if (rebuild) {
+    if ((rc = world_enqueue(w, poses_dev, false))) return rc;
+    if ((rc = world_build(w))) return rc;
+    if ((rc = world_pairs_and_contacts(w, poses_dev))) return rc;
+    w->graph_valid = false;
+  } else if (use_graph) {
+    if (!w->graph_valid || w->graph_poses != poses_dev) {
+      if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
+      if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
+      NCX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int r1 = world_enqueue(w, poses_dev, true);
+      int r2 = r1 ? r1 : world_pairs_and_contacts(w, poses_dev);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (r2) { if (g) (void)hipGraphDestroy(g); return r2; }
+      NCX_HIP_CHECK(e);
+      w->graph = g;
+      NCX_HIP_CHECK(hipGraphInstantiate(&w->exec, g, nullptr, nullptr, 0));
+      w->graph_valid = true;
+      w->graph_poses = poses_dev;
+    }
+    NCX_HIP_CHECK(hipGraphLaunch(w->exec, s));
+    w->last_was_graph = 1;
+  } else {
+    if ((rc = world_enqueue(w, poses_dev, true))) return rc;
+    if ((rc = world_pairs_and_contacts(w, poses_dev))) return rc;
+  }
+  w->first = false;
+  w->since_build++;
+  NCX_HIP_CHECK(hipEventRecord(w->ev_out, s));
+  NCX_HIP_CHECK(hipStreamWaitEvent(w->ctx->stream, w->ev_out, 0));
+  return NCX_OK;
+}
+
+// Waits for the last step; reports the pair count and fails on a pair-capacity overflow or a
+// contact that needs a larger EPA arena than the in-graph kernel has (re-run with
+// ncx_contact_batch on those pairs).
+extern "C" int ncx_world_sync(ncx_world* w, size_t* npairs, size_t* n_epa_overflow) {
+  NCX_REQUIRE(w && npairs, "ncx_world_sync: null argument");
+  uint32_t ctr[2] = {0, 0}, ovf = 0;
+  NCX_HIP_CHECK(hipMemcpyAsync(ctr, w->counters.p, 8, hipMemcpyDeviceToHost, w->wctx.stream));
+  NCX_HIP_CHECK(hipMemcpyAsync(&ovf, w->shapes->ovf.p, 4, hipMemcpyDeviceToHost, w->wctx.stream));
+  NCX_HIP_CHECK(hipStreamSynchronize(w->wctx.stream));
+  *npairs = ctr[0];
+  if (n_epa_overflow) *n_epa_overflow = ovf;
+  if (ctr[1] || ctr[0] > w->cap) {
+    ncx_set_error("ncx_world_sync: more pairs than max_pairs");
+    return NCX_E_CAPACITY;
+  }
+  return NCX_OK;
+}
+
+extern "C" int ncx_world_outputs(ncx_world* w, uint32_t** ia, uint32_t** ib, uint8_t** has, float** world1,
+                                 float** world2, float** normal, float** depth, int32_t** gjk_iters) {
+  NCX_REQUIRE(w, "ncx_world_outputs: null argument");
+  if (ia) *ia = w->pa.as<uint32_t>();
+  if (ib) *ib = w->pb.as<uint32_t>();
+  if (has) *has = w->has.as<uint8_t>();
+  if (world1) *world1 = w->w1.as<float>();
+  if (world2) *world2 = w->w2.as<float>();
+  if (normal) *normal = w->nrm.as<float>();
+  if (depth) *depth = w->depth_out.as<float>();
+  if (gjk_iters) *gjk_iters = w->iters.as<int32_t>();
+  return NCX_OK;
+}
+
+// The stored (fat) box of every proxy after the last step, device SoA (6 x n floats).
+extern "C" int ncx_world_stored_aabbs(ncx_world* w, const float** soa) {
+  NCX_REQUIRE(w && soa, "ncx_world_stored_aabbs: null argument");
+  *soa = w->stored.as<float>();
+  return NCX_OK;
+}

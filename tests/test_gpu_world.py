@@ -1,0 +1,84 @@
+"""The scene step (ncx_world) against the broad-phase and contact oracles over moving steps,
+and hipGraph replay against eager execution."""
+import numpy as np
+import pytest
+import torch
+
+import ncollide_amd as ncx
+from ncollide_amd import workloads
+from ncollide_amd.math import to_oracle_poses
+from ncollide_amd.world import World
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_shapes(orc, records, pool):
+    params = np.stack([records["p0"], records["p1"], records["p2"]], 1)
+    return orc.ShapeSet(records["type"], params, records["hull_offset"], records["hull_count"], pool,
+                        dtype=np.float32)
+
+
+def keys_of(r):
+    a = r["ia"].cpu().numpy().astype(np.uint64)
+    b = r["ib"].cpu().numpy().astype(np.uint64)
+    return (a << np.uint64(32)) | b
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_world_steps_match_reference(ctx, orc, use_graph):
+    w = workloads.full_pipeline(seed=21, n=6000, box=40.0, nclusters=3)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    world = World(ss, margin=0.05, prediction=0.0, max_pairs=1 << 20, rebuild_every=3)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    ref_bp = orc.BroadPhase(0.05)
+    oss = oracle_shapes(orc, w["records"], w["pool"])
+    p_np = w["poses"].copy()
+    vel = w["velocities"] * 0.2
+    for step in range(7):
+        world.step(poses, use_graph=use_graph)
+        r = world.results()
+        # reference broad phase fed the same boxes through create_proxy / deferred_set_bounding_volume
+        bb = orc.shape_aabbs(oss, to_oracle_poses(p_np))
+        if step == 0:
+            for b in bb:
+                ref_bp.create_proxy(b)
+        else:
+            for h in range(len(bb)):
+                ref_bp.deferred_set_bounding_volume(h, bb[h])
+        ref_bp.update()
+        stored = world.stored_aabbs().cpu().numpy().T
+        for h in range(0, len(bb), 97):
+            assert np.array_equal(stored[h], ref_bp.stored_bv(h)), (step, h)
+        order = np.argsort(keys_of(r))
+        assert np.array_equal(keys_of(r)[order], ref_bp.pairs()), f"pairs differ at step {step}"
+        ia = r["ia"].cpu().numpy()[order]
+        ib = r["ib"].cpu().numpy()[order]
+        ref = orc.contact(oss, to_oracle_poses(p_np), ia, ib, 0.0, nthreads=8)
+        assert r["epa_overflow"] == 0
+        assert np.array_equal(r["has_contact"].cpu().numpy()[order], ref["has"]), f"has differs at step {step}"
+        idx = np.nonzero(ref["has"])[0]
+        dg = r["depth"].cpu().numpy()[order][idx].astype(np.float64)
+        dr = ref["depth"][idx].astype(np.float64)
+        assert np.all(np.abs(dg - dr) <= 1e-5 * np.abs(dr) + 1e-6), f"depth differs at step {step}"
+        print(f"step {step}: {len(order)} pairs, {len(idx)} contacts, bit-exact {np.mean(dg == dr):.4f}")
+        p_np[:, 0:3] += vel
+        poses.copy_(torch.as_tensor(p_np, device=ctx.device))
+
+
+def test_world_graph_replay_equals_eager(ctx):
+    w = workloads.full_pipeline(seed=22, n=20000, box=60.0, nclusters=4)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    pa = torch.as_tensor(w["poses"], device=ctx.device)
+    pb = pa.clone()
+    wa = World(ss, margin=0.05, max_pairs=1 << 21, rebuild_every=4)
+    wb = World(ss, margin=0.05, max_pairs=1 << 21, rebuild_every=4)
+    vel = torch.as_tensor(w["velocities"] * 0.3, device=ctx.device)
+    for step in range(9):
+        wa.step(pa, use_graph=False)
+        ra = wa.results()
+        wb.step(pb, use_graph=True)
+        rb = wb.results()
+        for k in ("ia", "ib", "has_contact", "depth", "normal"):
+            assert torch.equal(ra[k], rb[k]), (step, k)
+        pa[:, 0:3] += vel
+        pb[:, 0:3] += vel
