@@ -4,8 +4,10 @@ Headline line: TriMesh ray casting, 1,048,576 rays against a 257x257 heightfield
 TriMesh (131,072 triangles), max_toi 100, first-hit toi + normal + triangle id.
 One step = one cast of the whole ray batch; rays and mesh are resident in HBM.
 
-Also reported in the same JSON line ("secondary"): batched convex contacts on
-262,144 ConvexHull pairs and the 1,024-shape plain pipeline step.
+Also reported in the same JSON line ("secondary"): batched contact + distance on
+262,144 ConvexHull pairs (with its CPU baseline), the learned warm start against
+plain GJK over 64 coherent steps, the 1,048,576-shape scene step, and the
+1,024-shape plain pipeline step.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -30,8 +32,16 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--quick", action="store_true", help="skip the secondary workloads")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="copy 256 MiB device to device once before the timed casts (the reference dispatch "
+                        "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     p.add_argument("--rays", type=int, default=1 << 20)
     p.add_argument("--hull-pairs", type=int, default=1 << 18)
+    p.add_argument("--train-samples", type=int, default=2_000_000)
+    p.add_argument("--c4-steps", type=int, default=64)
+    p.add_argument("--scene-shapes", type=int, default=1 << 20)
+    p.add_argument("--scene-steps", type=int, default=32)
+    p.add_argument("--scene-max-pairs", type=int, default=40_000_000)
     return p.parse_args()
 
 
@@ -129,8 +139,25 @@ def gpu_placer(ncx, torch, ctx):
     return placer, state
 
 
+def _timed(torch, fn, reps):
+    """Median wall time of fn() (each call completes on the GPU before the next starts)."""
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts))
+
+
 def bench_hulls(args, ncx, torch, ctx):
-    """262,144 ConvexHull pairs (N~U{8..256} points per hull), contact at prediction 0 per pair."""
+    """262,144 ConvexHull pairs (N~U{8..256} points per hull, half penetrating by U[0,0.1], half
+    apart by U[0,0.5]): query::contact at prediction 0 and query::distance per pair, one batched
+    call each.  Roofline: compulsory HBM bytes per pair and call (two poses and two shape records,
+    32 B each, the pair's indices, both hulls' vertices once, the outputs)."""
     from ncollide_amd import workloads
     t = time.perf_counter()
     placer, state = gpu_placer(ncx, torch, ctx)
@@ -138,18 +165,109 @@ def bench_hulls(args, ncx, torch, ctx):
     gen_s = time.perf_counter() - t
     ss, ia, ib = state["ss"], state["ia"], state["ib"]
     poses = torch.as_tensor(w["poses"], device=ctx.device)
-    for _ in range(2):
-        g = ncx.query.contact(ss, poses, ia, ib, 0.0)
-    torch.cuda.synchronize()
-    steps = max(3, args.steps // 4)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        g = ncx.query.contact(ss, poses, ia, ib, 0.0)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
-    return dict(pairs=len(w["ia"]), ms_per_step=dt * 1e3, pairs_per_s=len(w["ia"]) / dt,
+    reps = max(3, args.steps // 4)
+    tc = _timed(torch, lambda: ncx.query.contact(ss, poses, ia, ib, 0.0), reps)
+    td = _timed(torch, lambda: ncx.query.distance(ss, poses, ia, ib), reps)
+    g = ncx.query.contact(ss, poses, ia, ib, 0.0)
+    d, dit = ncx.query.distance(ss, poses, ia, ib)
+    n = len(w["ia"])
+    nverts = w["records"]["hull_count"].astype(np.float64)
+    verts_per_pair = float(nverts[w["ia"]].mean() + nverts[w["ib"]].mean())
+    per_pair = 2 * (64 + 64 + 8) + 2 * 12 * verts_per_pair + (1 + 36 + 4 + 4) + (4 + 4)
+    achieved = per_pair * n / (tc + td)
+    cit = g["gjk_iters"].float().mean().item()
+    dits = dit.float().mean().item()
+    return dict(pairs=n, contact_ms=tc * 1e3, distance_ms=td * 1e3, pairs_per_s=n / (tc + td),
+                contact_pairs_per_s=n / tc, distance_pairs_per_s=n / td,
                 contact_fraction=g["has_contact"].float().mean().item(),
-                gjk_iters_mean=g["gjk_iters"].float().mean().item(), gen_s=gen_s, w=w)
+                contact_gjk_iters=cit, distance_gjk_iters=dits, hull_vertices_per_pair=verts_per_pair,
+                roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK, "bytes_per_pair": per_pair},
+                gen_s=gen_s, w=w)
+
+
+def cpu_baseline_hulls(w, nsample=16384):
+    """The oracle's query::contact + query::distance (the reference algorithms) on the first
+    pairs of the same workload, all host threads."""
+    from oracle import pyoracle as orc
+    from ncollide_amd.math import to_oracle_poses
+    rec, pool = w["records"], w["pool"]
+    params = np.stack([rec["p0"], rec["p1"], rec["p2"]], 1)
+    oss = orc.ShapeSet(rec["type"], params, rec["hull_offset"], rec["hull_count"], pool, dtype=np.float32)
+    op = to_oracle_poses(w["poses"])
+    ia, ib = w["ia"][:nsample], w["ib"][:nsample]
+    threads = min(os.cpu_count() or 1, 64)
+    t = time.perf_counter()
+    c = orc.contact(oss, op, ia, ib, 0.0, nthreads=threads)
+    orc.distance(oss, op, ia, ib, nthreads=threads)
+    dt = time.perf_counter() - t
+    n1 = min(2048, len(ia))
+    t = time.perf_counter()
+    orc.contact(oss, op, ia[:n1], ib[:n1], 0.0, nthreads=1)
+    orc.distance(oss, op, ia[:n1], ib[:n1], nthreads=1)
+    st = n1 / (time.perf_counter() - t)
+    return dict(value=len(ia) / dt, unit="pairs/s (contact + distance)", cores=threads, kind="port",
+                sample=f"first {len(ia)} pairs of the same workload, query::contact + query::distance "
+                       f"(GJK/EPA) per pair, {threads} threads",
+                single_thread_pairs_per_s=st, epa_iters_per_pair=float(c["epa_iters"].mean()))
+
+
+def bench_learned(args, ncx, torch, ctx):
+    """Learned warm start on coherent hull pairs: train the 44-64-64-3 MLP on plain-GJK results
+    (the library's own FP16 MFMA kernels), then per step contact + distance on every pair with
+    init_dir = None (plain) or the MLP's direction (GJK capped at 32, plain GJK on the cap)."""
+    from ncollide_amd import learned
+    t = time.perf_counter()
+    feat, y = learned.training_set(ctx, seed=41, nsamples=args.train_samples)
+    gen_s = time.perf_counter() - t
+    mlp, hist, train_s = learned.train(ctx, feat, y, epochs=8)
+    # MLP forward throughput (MFMA): one call over the training features
+    nf = feat.shape[0]
+    fwd = _timed(torch, lambda: mlp.forward(feat), 5)
+    flops = 14208.0 * nf
+    res = learned.evaluate_coherent(ctx, mlp, seed=4, npairs=args.hull_pairs, steps=args.c4_steps,
+                                    variants=("plain", "learned"))
+    p, l = res["plain"], res["learned"]
+    return dict(pairs=res["pairs"], steps=res["steps"], train_samples=int(nf), train_set_s=gen_s, train_s=train_s,
+                loss_first=hist[0], loss_last=hist[-1],
+                plain_pairs_per_s=p["pairs_per_s"], learned_pairs_per_s=l["pairs_per_s"],
+                speedup=l["pairs_per_s"] / p["pairs_per_s"],
+                plain_iters={"contact": p["contact_gjk_iters_per_pair"], "distance": p["distance_gjk_iters_per_pair"]},
+                learned_iters={"contact": l["contact_gjk_iters_per_pair"],
+                               "distance": l["distance_gjk_iters_per_pair"]},
+                agreement_with_plain={"contact": l["contact_agreement_with_plain"],
+                                      "distance": l["distance_agreement_with_plain"]},
+                mlp_forward={"samples": int(nf), "ms": fwd * 1e3, "tflops": flops / fwd / 1e12,
+                             "frac_of_fp16_dense_peak": flops / fwd / 2.5e15})
+
+
+def bench_scene(args, ncx, torch, ctx):
+    """Scene step on 1,048,576 mixed shapes in 64 Gaussian clusters: shape AABBs, LBVH refit
+    (rebuild every 8 steps), pair search, contacts; each step replayed as a hipGraph."""
+    from ncollide_amd import workloads
+    from ncollide_amd.world import World
+    t = time.perf_counter()
+    w = workloads.full_pipeline(seed=5, n=args.scene_shapes)
+    gen_s = time.perf_counter() - t
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    vel = torch.as_tensor(w["velocities"], device=ctx.device)
+    world = World(ss, margin=w["margin"], prediction=0.0, max_pairs=args.scene_max_pairs, rebuild_every=8)
+    times, pairs, large = [], [], 0
+    for s in range(args.scene_steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        world.step(poses, use_graph=True)
+        npairs, ovf = world.sync()
+        times.append(time.perf_counter() - t0)
+        pairs.append(npairs)
+        large += ovf
+        poses[:, 0:3] += vel
+    torch.cuda.synchronize()
+    ms = float(np.median(times[1:] if len(times) > 1 else times)) * 1e3
+    return dict(shapes=args.scene_shapes, steps=args.scene_steps, ms_per_step=ms, ms_first_step=times[0] * 1e3,
+                pairs_per_step=float(np.mean(pairs)), pairs_per_s=float(np.mean(pairs)) / (ms / 1e3),
+                epa_pairs_past_first_arena=int(large), gen_s=gen_s)
 
 
 def bench_plain_pipeline(ncx, torch, ctx, steps=20):
@@ -188,6 +306,12 @@ def main():
     import torch
     import ncollide_amd as ncx
     ctx = ncx.default_context()
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=ctx.device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     log("ray casting ...")
     r = bench_rays(args, ncx, torch, ctx)
     value = r["n"] / (r["wall_ms"] / 1e3)
@@ -220,12 +344,20 @@ def main():
         result["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         result["cpu_baseline"]["single_thread_rays_per_s"] = cb["single_thread"]
     if not args.quick:
+        sec = {}
         log("hull pairs ...")
         h = bench_hulls(args, ncx, torch, ctx)
-        result["secondary"] = {"hull_pairs": {k: h[k] for k in ("pairs", "ms_per_step", "pairs_per_s",
-                                                                 "contact_fraction", "gjk_iters_mean")}}
+        sec["hull_pairs"] = {k: v for k, v in h.items() if k not in ("w", "gen_s")}
+        if not args.no_cpu:
+            cb = cpu_baseline_hulls(h["w"])
+            sec["hull_pairs"]["cpu_baseline"] = cb
+        log("learned warm start ...")
+        sec["learned_warm_start"] = bench_learned(args, ncx, torch, ctx)
+        log("scene step ...")
+        sec["scene_step"] = bench_scene(args, ncx, torch, ctx)
         log("plain pipeline ...")
-        result["secondary"]["plain_pipeline"] = bench_plain_pipeline(ncx, torch, ctx)
+        sec["plain_pipeline"] = bench_plain_pipeline(ncx, torch, ctx)
+        result["secondary"] = sec
     print(json.dumps(result))
 
 

@@ -177,6 +177,13 @@ int ncx_contact_batch_capped(ncx_shapes* s, const ncx_pose* poses_dev, const uin
                              const uint32_t* ib_dev, size_t npairs, float prediction, const float* init_dir_dev,
                              int32_t max_iter, uint8_t* has_contact_dev, float* world1_dev, float* world2_dev,
                              float* normal_dev, float* depth_dev, int32_t* gjk_iters_dev);
+/* Warm-started distance batch: the init_dir argument of
+ * distance_support_map_support_map_with_params (src/query/distance/distance_support_map_support_map.rs:25-53)
+ * per pair (NULL or an all-zero entry = None), with GJK from init_dir stopped after max_iter
+ * iterations and such a pair recomputed from the default direction (plain GJK). */
+int ncx_distance_batch_capped(ncx_shapes* s, const ncx_pose* poses_dev, const uint32_t* ia_dev,
+                              const uint32_t* ib_dev, size_t npairs, const float* init_dir_dev, int32_t max_iter,
+                              float* dist_dev, int32_t* gjk_iters_dev);
 
 /* ---------------------------------------------------------------- learned initial direction
  * No reference counterpart: a 44-64-64-3 MLP (FP16 operands, FP32 accumulation) that predicts
@@ -225,8 +232,9 @@ int ncx_world_set_warmstart(ncx_world* world, ncx_mlp* mlp);
 int ncx_world_set_rebuild_every(ncx_world* world, int k);
 int ncx_world_step(ncx_world* world, const ncx_pose* poses_dev, int use_graph);
 /* Waits for the last step: *npairs = pair count; NCX_E_CAPACITY if it exceeded max_pairs.
- * *n_epa_overflow = pairs whose EPA needs a larger arena than the step kernel has (left with
- * has_contact = 0; recompute them with ncx_contact_batch). */
+ * Every contact is complete on return: pairs whose EPA polytope outgrew the step's two arena
+ * tiers are recomputed here with larger arenas, so the step's poses must stay unchanged until
+ * this returns.  *n_epa_overflow = pairs of the step that needed more than the first tier. */
 int ncx_world_sync(ncx_world* world, size_t* npairs, size_t* n_epa_overflow);
 int ncx_world_outputs(ncx_world* world, uint32_t** ia, uint32_t** ib, uint8_t** has_contact, float** world1,
                       float** world2, float** normal, float** depth, int32_t** gjk_iters);

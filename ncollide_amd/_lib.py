@@ -41,6 +41,7 @@ SIGNATURES = {
     "ncx_contact_batch": (C.c_int, [P, P, P, P, SZ, F32, P, P, P, P, P, P, P]),
     "ncx_contact_batch_capped": (C.c_int, [P, P, P, P, SZ, F32, P, I32, P, P, P, P, P, P]),
     "ncx_distance_batch": (C.c_int, [P, P, P, P, SZ, P, P]),
+    "ncx_distance_batch_capped": (C.c_int, [P, P, P, P, SZ, P, C.c_int32, P, P]),
     "ncx_trimesh_create": (C.c_int, [P, P, SZ, P, SZ, P]),
     "ncx_trimesh_destroy": (C.c_int, [P]),
     "ncx_trimesh_cast_rays": (C.c_int, [P, P, P, SZ, F32, P, P, P]),

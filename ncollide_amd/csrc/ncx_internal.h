@@ -8,6 +8,7 @@
 //   q * v      = t = 2 (q.xyz x v); (t q.w + q.xyz x t) + v
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 #include <cstddef>
 #include <string>
@@ -67,14 +68,32 @@ struct DevBuf {
   template <class T> T* as() const { return (T*)p; }
 };
 
+// Layout of ContactTiers::ovf (uint32 words): a header, then the pair indices whose EPA outgrew
+// the first-tier arena (cap entries), then those that outgrew the second tier as well (cap
+// entries).
+constexpr uint32_t NCX_OVF_T1_COUNT = 0, NCX_OVF_BAD = 1, NCX_OVF_T2_COUNT = 2, NCX_OVF_T1_WORK = 3,
+                   NCX_OVF_T2_WORK = 4, NCX_OVF_HEADER = 8;
+
+struct KArgs;  // contact kernel arguments (ncx_narrow.hip)
+
+// Overflow lists (NCX_OVF_* layout) and the arguments of the last tiered contact launch, one per
+// launch site (a shape set's batch calls, each scene), so launches on different streams never
+// share them.
+struct ContactTiers {
+  DevBuf ovf;
+  uint32_t cap = 0;  // entries per overflow list
+  KArgs* last = nullptr;
+  uint32_t* t1_list() const { return ovf.as<uint32_t>() + NCX_OVF_HEADER; }
+  uint32_t* t2_list() const { return ovf.as<uint32_t>() + NCX_OVF_HEADER + cap; }
+};
+
 // Device-resident shape set (ncx_shapes_create).
 struct ncx_shapes {
   ncx_ctx* ctx;
   size_t n = 0, npts = 0;
   DevBuf shapes;      // ncx_shape[n]
   DevBuf hx, hy, hz;  // hull pool
-  DevBuf arena;       // EPA arenas
-  DevBuf ovf;         // overflow counter + list
+  ContactTiers tiers; // for ncx_contact_batch / ncx_distance_batch
 };
 
 // ------------------------------------------------------------------ device math
@@ -191,8 +210,14 @@ int radix_sort_pairs64(ncx_ctx* ctx, uint32_t* hi, uint32_t* lo, size_t n, int h
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
-// Contact kernel over pairs counted on the device (ncx_narrow.hip); no host synchronisation.
-int contacts_launch_device_count(ncx_shapes* S, ncx_ctx* ctx, const ncx_pose* poses, const uint32_t* ia,
-                                 const uint32_t* ib, const uint32_t* n_dev, size_t cap, float prediction,
-                                 const float* init_dir, int max_iter, uint8_t* has, float* w1, float* w2, float* nrm,
-                                 float* depth, int32_t* iters);
+// Tiered contact launch over pairs counted on the device (ncx_narrow.hip); no host
+// synchronisation, so it can be captured in a graph.
+int contact_tiers_reserve(ContactTiers& t, size_t cap);
+void contact_tiers_release(ContactTiers& t);
+int contacts_launch_device_count(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const ncx_pose* poses,
+                                 const uint32_t* ia, const uint32_t* ib, const uint32_t* n_dev, size_t cap,
+                                 float prediction, const float* init_dir, int max_iter, uint8_t* has, float* w1,
+                                 float* w2, float* nrm, float* depth, int32_t* iters);
+// Host side after that launch completed: checks, re-runs what both LDS tiers could not hold,
+// and reports the number of pairs that needed more than the first-tier arena.
+int contacts_resolve_overflow(ContactTiers& T, ncx_ctx* ctx, size_t* n_large);

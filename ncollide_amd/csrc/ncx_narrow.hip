@@ -16,6 +16,7 @@
 //   EPA      src/query/algorithms/epa3.rs:219-454
 //   dispatch src/query/contact/contact_shape_shape.rs:10-48,
 //            src/query/distance/distance.rs:9-33
+#include <algorithm>
 #include "ncx_internal.h"
 
 #define SHAPE_ORIGIN 4u
@@ -50,30 +51,53 @@ struct Pool {
   const float* __restrict__ z;
 };
 
-// First index of the maximum of dot(p_i, dir) over the hull, split over G lanes.
+// point_cloud_support_point_id (src/utils/point_cloud_support_point.rs:6-24), split over G lanes.
+// The reference keeps index 0 when dot(p_0, dir) is NaN and otherwise returns the first index of
+// the maximum over the non-NaN dots (NaN never compares greater).  Each lane scans vertices
+// gl, gl + G, ... in increasing order keeping its first maximum over non-NaN dots; the lane
+// reduction keeps the larger value and, on ties, the lower index.  U vertices' loads are issued
+// together so their latencies overlap.
 template <int G>
 __device__ __forceinline__ uint32_t hull_support_id(const Pool& P, uint32_t off, uint32_t cnt, f3 dir, int gl) {
+  constexpr int U = 4;
+  constexpr uint32_t NONE = 0xFFFFFFFFu;
   float best = 0.f;
-  uint32_t bi = 0xFFFFFFFFu;
-  for (uint32_t v = gl; v < cnt; v += G) {
-    f3 p = f3{P.x[off + v], P.y[off + v], P.z[off + v]};
-    float d = dot(p, dir);
-    if (bi == 0xFFFFFFFFu || d > best) {
-      best = d;
-      bi = v;
+  uint32_t bi = NONE;
+  int nan0 = 0;
+  for (uint32_t v0 = gl; v0 < cnt; v0 += U * G) {
+    float px[U], py[U], pz[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      uint32_t v = v0 + u * G;
+      uint32_t vv = off + (v < cnt ? v : v0);  // always a vertex of this hull
+      px[u] = P.x[vv];
+      py[u] = P.y[vv];
+      pz[u] = P.z[vv];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      uint32_t v = v0 + u * G;
+      float d = dot(f3{px[u], py[u], pz[u]}, dir);
+      if (v < cnt) {
+        if (v == 0) nan0 = d != d;
+        if (d > best || (bi == NONE && d == d)) {
+          best = d;
+          bi = v;
+        }
+      }
     }
   }
 #pragma unroll
   for (int o = G / 2; o >= 1; o >>= 1) {
     float ob = __shfl_xor(best, o, 64);
     uint32_t oi = __shfl_xor(bi, o, 64);
-    bool take = oi != 0xFFFFFFFFu && (bi == 0xFFFFFFFFu || ob > best || (ob == best && oi < bi));
-    if (take) {
+    if (oi != NONE && (bi == NONE || ob > best || (ob == best && oi < bi))) {
       best = ob;
       bi = oi;
     }
   }
-  return bi == 0xFFFFFFFFu ? 0u : bi;
+  nan0 = __shfl(nan0, 0, G);  // lane 0 of the group scanned vertex 0
+  return (bi == NONE || nan0) ? 0u : bi;
 }
 
 // SupportMap::support_point (src/shape/support_map.rs:26-29) and the overrides of
@@ -234,7 +258,7 @@ __device__ __forceinline__ bool tet_face(int i, const iso& m, f3 a, f3 b, f3 c, 
 }
 
 // point_tetrahedron.rs:35-353, solid = true
-__device__ f3 proj_tetrahedron(f3 A, f3 B, f3 C, f3 D, Loc& loc) {
+__device__ __forceinline__ f3 proj_tetrahedron(f3 A, f3 B, f3 C, f3 D, Loc& loc) {
   iso m = iso_identity();
   f3 p = inv_xform_pt(m, f3{0.f, 0.f, 0.f});
   f3 ab = B - A, ac = C - A, ad = D - A, ap = p - A;
@@ -279,14 +303,19 @@ struct Simplex {
   int dim, pdim;
 };
 
+// Value selects (never a select between addresses), so the simplex stays in registers.
+__device__ __forceinline__ f3 sel3(bool c, f3 a, f3 b) { return f3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
+__device__ __forceinline__ CSO csel(bool c, const CSO& a, const CSO& b) {
+  return CSO{sel3(c, a.p, b.p), sel3(c, a.o1, b.o1), sel3(c, a.o2, b.o2)};
+}
 __device__ __forceinline__ CSO sx_get(const Simplex& s, int i) {
-  return i == 0 ? s.v0 : (i == 1 ? s.v1 : (i == 2 ? s.v2 : s.v3));
+  return csel(i == 0, s.v0, csel(i == 1, s.v1, csel(i == 2, s.v2, s.v3)));
 }
 __device__ __forceinline__ void sx_set(Simplex& s, int i, const CSO& c) {
-  if (i == 0) s.v0 = c;
-  else if (i == 1) s.v1 = c;
-  else if (i == 2) s.v2 = c;
-  else s.v3 = c;
+  s.v0 = csel(i == 0, c, s.v0);
+  s.v1 = csel(i == 1, c, s.v1);
+  s.v2 = csel(i == 2, c, s.v2);
+  s.v3 = csel(i == 3, c, s.v3);
 }
 __device__ __forceinline__ void sx_swap(Simplex& s, int i, int j) {
   CSO a = sx_get(s, i), b = sx_get(s, j);
@@ -329,13 +358,11 @@ __device__ __forceinline__ bool sx_add(Simplex& s, const CSO& pt) {
     if (fabsf(dot(n, ap)) < tol) return false;
   }
   s.dim += 1;
-  if (s.dim == 1) s.v1 = pt;
-  else if (s.dim == 2) s.v2 = pt;
-  else s.v3 = pt;
+  sx_set(s, s.dim, pt);
   return true;
 }
 
-__device__ f3 sx_project_reduce(Simplex& s) {
+__device__ __forceinline__ f3 sx_project_reduce(Simplex& s) {
   if (s.dim == 0) {
     s.proj0 = 1.f;
     return s.v0.p;
@@ -376,11 +403,15 @@ __device__ f3 sx_project_reduce(Simplex& s) {
       else { s.proj0 = loc.bc0; s.proj1 = loc.bc1; }
       s.dim = 1;
     } else if (loc.kind == LOC_FACE) {
-      switch (loc.id) {
+      const int fid = loc.id;
+      s.v2 = csel(fid == 1, s.v3, s.v2);
+      s.v1 = csel(fid == 2, s.v3, s.v1);
+      s.v0 = csel(fid >= 3, s.v3, s.v0);
+      switch (fid) {
         case 0: s.proj0 = loc.bc0; s.proj1 = loc.bc1; s.proj2 = loc.bc2; break;
-        case 1: s.v2 = s.v3; s.proj0 = loc.bc0; s.proj1 = loc.bc1; s.proj2 = loc.bc2; break;
-        case 2: s.v1 = s.v3; s.proj0 = loc.bc0; s.proj1 = loc.bc2; s.proj2 = loc.bc1; break;
-        default: s.v0 = s.v3; s.proj0 = loc.bc2; s.proj1 = loc.bc0; s.proj2 = loc.bc1; break;
+        case 1: s.proj0 = loc.bc0; s.proj1 = loc.bc1; s.proj2 = loc.bc2; break;
+        case 2: s.proj0 = loc.bc0; s.proj1 = loc.bc2; s.proj2 = loc.bc1; break;
+        default: s.proj0 = loc.bc2; s.proj1 = loc.bc0; s.proj2 = loc.bc1; break;
       }
       s.dim = 2;
     }
@@ -414,7 +445,7 @@ struct GJKOut {
 
 // gjk::closest_points (gjk.rs:76-177), exact_dist = true.
 template <int G>
-__device__ GJKOut gjk(const Ctx2& c, float max_dist, Simplex& s, const Pool& P, int gl, int max_iter) {
+__device__ __forceinline__ GJKOut gjk(const Ctx2& c, float max_dist, Simplex& s, const Pool& P, int gl, int max_iter) {
   GJKOut r;
   r.iters = 0;
   const float eps_tol = NCX_FLT_EPS * 10.f;
@@ -479,9 +510,15 @@ __device__ GJKOut gjk(const Ctx2& c, float max_dist, Simplex& s, const Pool& P, 
 }
 
 // ------------------------------------------------------------------ EPA arena
+// A polytope vertex keeps the two support points; its CSO point is recomputed as o1 - o2,
+// which is how it was formed (CSOPoint::new), so the bits are the same.  Face and vertex ids are
+// 16-bit (arena capacities stay below 65536).
+struct VertA {
+  float o1x, o1y, o1z, o2x, o2y, o2z;
+};
 struct FaceD {
-  uint32_t pts[3];
-  uint32_t adj[3];
+  uint16_t pts[3];
+  uint16_t adj[3];
   float n[3];
   float bc[3];
   uint32_t deleted;
@@ -491,37 +528,65 @@ struct HeapE {
   uint32_t id;
 };
 struct SilE {
-  uint32_t face, opp;
+  uint16_t face, opp;
 };
 
 struct Arena {
-  CSO* verts;
+  VertA* verts;
   FaceD* faces;
   HeapE* heap;
   SilE* sil;
   SilE* stack;
-  uint32_t vcap, fcap;
+  uint32_t vcap, fcap, scap, stcap;  // vertices, faces (and heap entries), silhouette, search stack
 };
 
-__host__ __device__ inline size_t arena_bytes(uint32_t vcap, uint32_t fcap) {
-  return (size_t)vcap * sizeof(CSO) + (size_t)fcap * (sizeof(FaceD) + sizeof(HeapE) + 3 * sizeof(SilE));
+__host__ __device__ constexpr size_t arena_bytes(uint32_t vcap, uint32_t fcap, uint32_t scap, uint32_t stcap) {
+  return ((size_t)vcap * sizeof(VertA) + (size_t)fcap * (sizeof(FaceD) + sizeof(HeapE)) +
+          (size_t)(scap + stcap) * sizeof(SilE) + 15) & ~(size_t)15;
+}
+// Device-memory tiers: silhouette and search stack sized by the face capacity.
+__host__ __device__ constexpr size_t arena_bytes(uint32_t vcap, uint32_t fcap) {
+  return arena_bytes(vcap, fcap, fcap, 2 * fcap);
 }
 
-__device__ __forceinline__ Arena arena_at(uint8_t* base, size_t slot, uint32_t vcap, uint32_t fcap) {
-  uint8_t* p = base + slot * arena_bytes(vcap, fcap);
+__device__ __forceinline__ Arena arena_at(uint8_t* base, size_t slot, uint32_t vcap, uint32_t fcap, uint32_t scap,
+                                          uint32_t stcap) {
+  uint8_t* p = base + slot * arena_bytes(vcap, fcap, scap, stcap);
   Arena a;
-  a.verts = (CSO*)p;
-  p += (size_t)vcap * sizeof(CSO);
+  a.verts = (VertA*)p;
+  p += (size_t)vcap * sizeof(VertA);
   a.faces = (FaceD*)p;
   p += (size_t)fcap * sizeof(FaceD);
   a.heap = (HeapE*)p;
   p += (size_t)fcap * sizeof(HeapE);
   a.sil = (SilE*)p;
-  p += (size_t)fcap * sizeof(SilE);
+  p += (size_t)scap * sizeof(SilE);
   a.stack = (SilE*)p;
   a.vcap = vcap;
   a.fcap = fcap;
+  a.scap = scap;
+  a.stcap = stcap;
   return a;
+}
+
+// Field-wise vertex copies between registers and the arena (no aggregate copies through
+// private memory).
+__device__ __forceinline__ CSO vget(const Arena& A, uint32_t i) {
+  const float* f = (const float*)(A.verts + i);
+  CSO c;
+  c.o1 = f3{f[0], f[1], f[2]};
+  c.o2 = f3{f[3], f[4], f[5]};
+  c.p = c.o1 - c.o2;
+  return c;
+}
+__device__ __forceinline__ f3 vpt(const Arena& A, uint32_t i) {
+  const float* f = (const float*)(A.verts + i);
+  return f3{f[0], f[1], f[2]} - f3{f[3], f[4], f[5]};
+}
+__device__ __forceinline__ void vset(const Arena& A, uint32_t i, const CSO& c) {
+  float* f = (float*)(A.verts + i);
+  f[0] = c.o1.x; f[1] = c.o1.y; f[2] = c.o1.z;
+  f[3] = c.o2.x; f[4] = c.o2.y; f[5] = c.o2.z;
 }
 
 // Rust std BinaryHeap (max-heap on neg_dist): push = sift_up; pop = swap with the
@@ -567,11 +632,11 @@ __device__ __forceinline__ bool heap_pop(HeapE* h, uint32_t& len, HeapE& out) {
 // Face::new (epa3.rs:96-115) + new_with_proj (:63-94) + ccw_face_normal (utils/ccw_face_normal.rs:21-27).
 __device__ __forceinline__ bool make_face(const Arena& A, uint32_t p0, uint32_t p1, uint32_t p2, uint32_t a0,
                                           uint32_t a1, uint32_t a2, FaceD& f) {
-  f3 va = A.verts[p0].p, vb = A.verts[p1].p, vc = A.verts[p2].p;
+  f3 va = vpt(A, p0), vb = vpt(A, p1), vc = vpt(A, p2);
   Loc loc;
   proj_triangle(va, vb, vc, loc);
-  f.pts[0] = p0; f.pts[1] = p1; f.pts[2] = p2;
-  f.adj[0] = a0; f.adj[1] = a1; f.adj[2] = a2;
+  f.pts[0] = (uint16_t)p0; f.pts[1] = (uint16_t)p1; f.pts[2] = (uint16_t)p2;
+  f.adj[0] = (uint16_t)a0; f.adj[1] = (uint16_t)a1; f.adj[2] = (uint16_t)a2;
   f.deleted = 0;
   f3 nn;
   float nl;
@@ -588,9 +653,9 @@ __device__ __forceinline__ bool make_face(const Arena& A, uint32_t p0, uint32_t 
 __device__ __forceinline__ f3 fnormal(const FaceD& f) { return f3{f.n[0], f.n[1], f.n[2]}; }
 
 __device__ __forceinline__ void face_points(const Arena& A, const FaceD& f, f3& p1, f3& p2) {
-  const CSO& a = A.verts[f.pts[0]];
-  const CSO& b = A.verts[f.pts[1]];
-  const CSO& c = A.verts[f.pts[2]];
+  const CSO a = vget(A, f.pts[0]);
+  const CSO b = vget(A, f.pts[1]);
+  const CSO c = vget(A, f.pts[2]);
   p1 = a.o1 * f.bc[0] + b.o1 * f.bc[1] + c.o1 * f.bc[2];
   p2 = a.o2 * f.bc[0] + b.o2 * f.bc[1] + c.o2 * f.bc[2];
 }
@@ -616,37 +681,39 @@ __device__ __forceinline__ bool affinely_dependent(f3 p1, f3 p2, f3 p3) {
 
 // Face::can_be_seen_by (epa3.rs:143-156)
 __device__ __forceinline__ bool can_be_seen(const Arena& A, const FaceD& f, uint32_t point, uint32_t opp) {
-  f3 p0 = A.verts[f.pts[opp]].p;
-  f3 p1 = A.verts[f.pts[(opp + 1) % 3]].p;
-  f3 p2 = A.verts[f.pts[(opp + 2) % 3]].p;
-  f3 pt = A.verts[point].p;
+  f3 p0 = vpt(A, f.pts[opp]);
+  f3 p1 = vpt(A, f.pts[(opp + 1) % 3]);
+  f3 p2 = vpt(A, f.pts[(opp + 2) % 3]);
+  f3 pt = vpt(A, point);
   return dot(pt - p0, fnormal(f)) >= -(NCX_FLT_EPS * 10.f) || affinely_dependent(p1, p2, pt);
 }
 
 // EPA::closest_points (epa3.rs:219-430).  compute_silhouette's recursion
 // (:432-454) becomes an explicit stack visiting faces in the same order.
 template <int G>
-__device__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const Arena& A, f3& p1, f3& p2, f3& n,
+__device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const Arena& A, f3& p1, f3& p2, f3& n,
                    int& iters) {
   const float eps_tol = NCX_FLT_EPS * 100.f;
   const float gjk_tol = NCX_FLT_EPS * 10.f;
   iters = 0;
   uint32_t nv = 0, nf = 0, hl = 0;
-  A.verts[nv++] = s.v0;
-  if (s.dim >= 1) A.verts[nv++] = s.v1;
-  if (s.dim >= 2) A.verts[nv++] = s.v2;
-  if (s.dim >= 3) A.verts[nv++] = s.v3;
+  vset(A, nv++, s.v0);
+  if (s.dim >= 1) vset(A, nv++, s.v1);
+  if (s.dim >= 2) vset(A, nv++, s.v2);
+  if (s.dim >= 3) vset(A, nv++, s.v3);
   if (s.dim == 0) {
     p1 = f3{0.f, 0.f, 0.f};
     p2 = f3{0.f, 0.f, 0.f};
     n = f3{0.f, 1.f, 0.f};
     return EPA_OK;
   } else if (s.dim == 3) {
-    f3 dp1 = A.verts[1].p - A.verts[0].p;
-    f3 dp2 = A.verts[2].p - A.verts[0].p;
-    f3 dp3 = A.verts[3].p - A.verts[0].p;
+    f3 dp1 = vpt(A, 1) - vpt(A, 0);
+    f3 dp2 = vpt(A, 2) - vpt(A, 0);
+    f3 dp3 = vpt(A, 3) - vpt(A, 0);
     if (dot(cross(dp1, dp2), dp3) > 0.f) {
-      CSO t = A.verts[1]; A.verts[1] = A.verts[2]; A.verts[2] = t;
+      CSO t1 = vget(A, 1), t2 = vget(A, 2);
+      vset(A, 1, t2);
+      vset(A, 2, t1);
     }
     const uint32_t PT[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {0, 3, 1}};
     const uint32_t AD[4][3] = {{3, 1, 2}, {3, 2, 0}, {0, 1, 3}, {2, 1, 0}};
@@ -658,7 +725,7 @@ __device__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const
     }
     for (int f = 0; f < 4; ++f) {
       if (inside[f]) {
-        float dist = dot(fnormal(A.faces[f]), A.verts[f].p);
+        float dist = dot(fnormal(A.faces[f]), vpt(A, f));
         if (-dist > gjk_tol) return EPA_NONE;
         A.heap[hl] = HeapE{-dist, (uint32_t)f};
         heap_sift_up(A.heap, hl);
@@ -667,11 +734,11 @@ __device__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const
     }
   } else {
     if (s.dim == 1) {
-      f3 dpt = A.verts[1].p - A.verts[0].p;
+      f3 dpt = vpt(A, 1) - vpt(A, 0);
       // nalgebra orthonormal_subspace_basis for one vector; epa3.rs:299-305 keeps a x dpt.
       f3 a = fabsf(dpt.x) > fabsf(dpt.y) ? f3{dpt.z, 0.f, -dpt.x} : f3{0.f, -dpt.z, dpt.y};
       a = normalize(a);
-      A.verts[nv++] = cso_from<G>(c, cross(a, dpt), P, gl);
+      vset(A, nv++, cso_from<G>(c, cross(a, dpt), P, gl));
     }
     FaceD f1, f2;
     make_face(A, 0, 1, 2, 1, 1, 1, f1);
@@ -693,7 +760,7 @@ __device__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const
     CSO cso = cso_from<G>(c, fnormal(face), P, gl);
     iters += 1;
     uint32_t sp_id = nv;
-    A.verts[nv++] = cso;
+    vset(A, nv++, cso);
     float cand = dot(cso.p, fnormal(face));
     if (cand < max_dist) { best = fid; max_dist = cand; }
     float curr_dist = -fid.neg_dist;
@@ -708,14 +775,14 @@ __device__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const
     uint32_t ns = 0, sp = 0;
     for (int k = 2; k >= 0; --k) {
       uint32_t adj = face.adj[k];
-      A.stack[sp++] = SilE{adj, next_ccw(A.faces[adj], face.pts[k])};
+      A.stack[sp++] = SilE{(uint16_t)adj, (uint16_t)next_ccw(A.faces[adj], face.pts[k])};
     }
     while (sp > 0) {
       SilE e = A.stack[--sp];
       FaceD& fa = A.faces[e.face];
       if (fa.deleted) continue;
       if (!can_be_seen(A, fa, sp_id, e.opp)) {
-        if (ns >= A.fcap) return EPA_OVERFLOW;
+        if (ns >= A.scap) return EPA_OVERFLOW;
         A.sil[ns++] = e;
       } else {
         fa.deleted = 1;
@@ -723,9 +790,9 @@ __device__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const
         uint32_t adj1 = fa.adj[i1], adj2 = fa.adj[i2];
         uint32_t o1 = next_ccw(A.faces[adj1], fa.pts[i1]);
         uint32_t o2 = next_ccw(A.faces[adj2], fa.pts[i2]);
-        if (sp + 2 > 2 * A.fcap) return EPA_OVERFLOW;
-        A.stack[sp++] = SilE{adj2, o2};
-        A.stack[sp++] = SilE{adj1, o1};
+        if (sp + 2 > A.stcap) return EPA_OVERFLOW;
+        A.stack[sp++] = SilE{(uint16_t)adj2, (uint16_t)o2};
+        A.stack[sp++] = SilE{(uint16_t)adj1, (uint16_t)o1};
       }
     }
     uint32_t first_new = nf;
@@ -739,10 +806,10 @@ __device__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const
         uint32_t pt2 = A.faces[edge.face].pts[(edge.opp + 1) % 3];
         FaceD nfc;
         bool inside = make_face(A, pt1, pt2, sp_id, edge.face, new_id + 1, new_id - 1, nfc);
-        A.faces[edge.face].adj[(edge.opp + 1) % 3] = new_id;
+        A.faces[edge.face].adj[(edge.opp + 1) % 3] = (uint16_t)new_id;
         A.faces[nf++] = nfc;
         if (inside) {
-          float dist = dot(fnormal(nfc), A.verts[nfc.pts[0]].p);
+          float dist = dot(fnormal(nfc), vpt(A, nfc.pts[0]));
           if (dist < curr_dist) {
             face_points(A, face, p1, p2);
             n = fnormal(face);
@@ -756,8 +823,8 @@ __device__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const
       }
     }
     if (first_new == nf) return EPA_NONE;
-    A.faces[first_new].adj[2] = nf - 1;
-    A.faces[nf - 1].adj[1] = first_new;
+    A.faces[first_new].adj[2] = (uint16_t)(nf - 1);
+    A.faces[nf - 1].adj[1] = (uint16_t)first_new;
     niter += 1;
     if (niter > 10000) return EPA_NONE;
   }
@@ -777,13 +844,13 @@ struct PairOut {
 
 // contact_support_map_support_map_with_params (contact_support_map_support_map.rs:38-88)
 template <int G>
-__device__ PairOut contact_sm(const Ctx2& c, float prediction, const f3* init_dir, const Pool& P, int gl,
-                              const Arena& A, int max_iter) {
+__device__ __forceinline__ PairOut contact_sm(const Ctx2& c, float prediction, bool has_init, f3 init_dir, const Pool& P,
+                              int gl, const Arena& A, int max_iter) {
   PairOut o;
   o.has = 0;
   o.gjk_iters = 0;
   f3 dir;
-  if (init_dir) dir = *init_dir;
+  if (has_init) dir = init_dir;
   else {
     float nl;
     if (!try_normalize(c.m2.t - c.m1.t, NCX_FLT_EPS, &dir, &nl)) dir = f3{1.f, 0.f, 0.f};
@@ -814,7 +881,7 @@ __device__ PairOut contact_sm(const Ctx2& c, float prediction, const f3* init_di
 }
 
 // AABB::local_point_projection + project_point_with_feature for a cuboid (point_aabb.rs:9-135), solid = false.
-__device__ f3 project_cuboid(const iso& m, f3 h, f3 pt, bool& inside_out, int& fkind, int& fid) {
+__device__ __forceinline__ f3 project_cuboid(const iso& m, f3 h, f3 pt, bool& inside_out, int& fkind, int& fid) {
   f3 mins = f3{0.f + (-h.x), 0.f + (-h.y), 0.f + (-h.z)};
   f3 maxs = f3{0.f + h.x, 0.f + h.y, 0.f + h.z};
   f3 ls = inv_xform_pt(m, pt);
@@ -857,7 +924,7 @@ __device__ f3 project_cuboid(const iso& m, f3 h, f3 pt, bool& inside_out, int& f
 }
 
 // Cuboid::feature_normal (cuboid.rs:511-561), local frame as in the reference.
-__device__ f3 cuboid_feature_normal(int fkind, int fid) {
+__device__ __forceinline__ f3 cuboid_feature_normal(int fkind, int fid) {
   f3 d = f3{0.f, 0.f, 0.f};
   if (fkind == 0) {
     if (fid < 3) set_comp(d, fid, 1.f); else set_comp(d, fid - 3, -1.f);
@@ -874,7 +941,7 @@ __device__ f3 cuboid_feature_normal(int fkind, int fid) {
 
 // contact_ball_convex_polyhedron (contact_ball_convex_polyhedron.rs:11-58), Cuboid or ConvexHull.
 template <int G>
-__device__ PairOut contact_ball_convex(f3 c1, float r, const iso& m2, const ShapeD& g2, float prediction,
+__device__ __forceinline__ PairOut contact_ball_convex(f3 c1, float r, const iso& m2, const ShapeD& g2, float prediction,
                                        const Pool& P, int gl, const Arena& A) {
   PairOut o;
   o.has = 0;
@@ -936,8 +1003,8 @@ __device__ PairOut contact_ball_convex(f3 c1, float r, const iso& m2, const Shap
 
 // query::contact dispatch (contact_shape_shape.rs:10-48).
 template <int G>
-__device__ PairOut contact_pair(const Ctx2& c, float prediction, const f3* init_dir, const Pool& P, int gl,
-                                const Arena& A, int max_iter) {
+__device__ __forceinline__ PairOut contact_pair(const Ctx2& c, float prediction, bool has_init, f3 init_dir, const Pool& P,
+                                int gl, const Arena& A, int max_iter) {
   bool b1 = c.g1.type == NCX_BALL, b2 = c.g2.type == NCX_BALL;
   bool poly1 = c.g1.type == NCX_CUBOID || c.g1.type == NCX_CONVEX_HULL;
   bool poly2 = c.g2.type == NCX_CUBOID || c.g2.type == NCX_CONVEX_HULL;
@@ -969,7 +1036,7 @@ __device__ PairOut contact_pair(const Ctx2& c, float prediction, const f3* init_
     }
     return o;
   }
-  return contact_sm<G>(c, prediction, init_dir, P, gl, A, max_iter);
+  return contact_sm<G>(c, prediction, has_init, init_dir, P, gl, A, max_iter);
 }
 
 struct KArgs {
@@ -992,23 +1059,46 @@ struct KArgs {
   float* dist;
   uint8_t* arena;
   uint32_t vcap, fcap;
-  uint32_t* ovf;  // [0] = count, [1..] = list of pair indices
+  uint32_t* ovf_count;  // pairs whose EPA outgrew the arena
+  uint32_t* ovf_list;   // their pair indices (first ovf_cap of them)
   uint32_t ovf_cap;
   int distance_mode;
   uint32_t nshapes;
   uint32_t* bad_index;  // set to 1 when a pair names a shape >= nshapes
   const uint32_t* n_dev;  // optional device-side pair count (capped by npairs)
+  uint32_t* work;         // optional work counter (zero at launch): pairs are taken from it
 };
 
-template <int G>
-__global__ __launch_bounds__(256) void k_pairs(KArgs a) {
+// Contact/distance kernel, BS threads per block in groups of G lanes (one pair per group).
+// VC > 0: every group's EPA arena (VC vertices, FC faces, SC silhouette edges, STC search-stack
+// entries) lives in LDS; a pair whose polytope outgrows it is listed in ovf for a launch with
+// larger arenas.  VC == 0: the arena is the group's slice of a.arena (a.vcap, a.fcap) in device
+// memory.
+template <int G, int BS, uint32_t VC, uint32_t FC, uint32_t SC, uint32_t STC>
+__global__ __launch_bounds__(BS) void k_pairs(KArgs a) {
   const int gl = threadIdx.x & (G - 1);
   const size_t group = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const size_t ngroups = ((size_t)gridDim.x * blockDim.x) / G;
-  Arena A = arena_at(a.arena, group, a.vcap, a.fcap);
+  constexpr bool LDS = VC > 0;
+  constexpr size_t LDS_BYTES = LDS ? (BS / G) * arena_bytes(VC, FC, SC, STC) : 16;
+  static_assert(LDS_BYTES <= 65536, "LDS arena exceeds 64 KiB per block");
+  static_assert(!LDS || (VC < 65536 && FC < 65536 && SC >= 3 && STC >= 3), "arena capacities");
+  __shared__ __align__(16) uint8_t lds_arena[LDS_BYTES];
+  Arena A;
+  if constexpr (LDS) A = arena_at(lds_arena, threadIdx.x / G, VC, FC, SC, STC);
+  else A = arena_at(a.arena, group, a.vcap, a.fcap, a.fcap, 2 * a.fcap);
   size_t npairs = a.npairs;
   if (a.n_dev) npairs = min(npairs, (size_t)*a.n_dev);
-  for (size_t q = group; q < npairs; q += ngroups) {
+  for (size_t it = 0;; ++it) {
+    size_t q;
+    if (a.work) {  // next pair from the work counter (lane 0 of the group takes it)
+      uint32_t t = 0;
+      if (gl == 0) t = atomicAdd(a.work, 1u);
+      q = (uint32_t)__shfl((int)t, 0, G);
+    } else {
+      q = group + it * ngroups;
+    }
+    if (q >= npairs) break;
     size_t k = a.list ? a.list[q] : q;
     uint32_t i1 = a.ia[k], i2 = a.ib[k];
     if (i1 >= a.nshapes || i2 >= a.nshapes) {
@@ -1032,15 +1122,34 @@ __global__ __launch_bounds__(256) void k_pairs(KArgs a) {
         d = d2 <= sr * sr ? 0.f : __builtin_sqrtf(d2) - sr;
       } else {
         // distance_support_map_support_map_with_params (distance_support_map_support_map.rs:25-53)
-        f3 dir;
-        float nl;
-        if (!try_normalize(c.m1.t - c.m2.t, NCX_FLT_EPS, &dir, &nl)) dir = f3{1.f, 0.f, 0.f};
-        Simplex s;
-        sx_init(s);
-        sx_reset(s, cso_from<G>(c, dir, a.P, gl));
-        GJKOut r = gjk<G>(c, NCX_FLT_MAX, s, a.P, gl, 10000);
-        it = r.iters;
-        d = r.status == GJK_S_CLOSEST ? norm(r.p1 - r.p2) : 0.f;
+        // init_dir (an all-zero entry means None) with GJK capped at max_iter; on the cap,
+        // plain GJK from the default direction judges the pair.
+        f3 d0 = c.m1.t - c.m2.t;
+        bool warm = false;
+        if (a.init_dir) {
+          f3 idir = f3{a.init_dir[3 * k], a.init_dir[3 * k + 1], a.init_dir[3 * k + 2]};
+          if (idir.x != 0.f || idir.y != 0.f || idir.z != 0.f) {
+            d0 = idir;
+            warm = true;
+          }
+        }
+        for (int pass = 0; pass < 2; ++pass) {
+          f3 dir;
+          float nl;
+          if (!try_normalize(d0, NCX_FLT_EPS, &dir, &nl)) dir = f3{1.f, 0.f, 0.f};
+          Simplex s;
+          sx_init(s);
+          sx_reset(s, cso_from<G>(c, dir, a.P, gl));
+          GJKOut r = gjk<G>(c, NCX_FLT_MAX, s, a.P, gl, warm ? a.max_iter : 10000);
+          it += r.iters;
+          if (warm && r.status == GJK_S_CAP && a.max_iter < 10000) {
+            warm = false;
+            d0 = c.m1.t - c.m2.t;
+            continue;
+          }
+          d = r.status == GJK_S_CLOSEST ? norm(r.p1 - r.p2) : 0.f;
+          break;
+        }
       }
       (void)ovf;
       if (gl == 0) {
@@ -1049,24 +1158,24 @@ __global__ __launch_bounds__(256) void k_pairs(KArgs a) {
       }
       continue;
     }
-    f3 idir;
-    const f3* ip = nullptr;
+    f3 idir = f3{0.f, 0.f, 0.f};
+    bool has_init = false;
     if (a.init_dir) {
       idir = f3{a.init_dir[3 * k], a.init_dir[3 * k + 1], a.init_dir[3 * k + 2]};
       // An all-zero entry means None for this pair (the reference's default direction).
-      if (idir.x != 0.f || idir.y != 0.f || idir.z != 0.f) ip = &idir;
+      has_init = idir.x != 0.f || idir.y != 0.f || idir.z != 0.f;
     }
-    PairOut o = contact_pair<G>(c, a.prediction, ip, a.P, gl, A, a.max_iter);
+    PairOut o = contact_pair<G>(c, a.prediction, has_init, idir, a.P, gl, A, has_init ? a.max_iter : 10000);
     int extra_iters = 0;
     if (o.has == 3) {
       // Warm start hit its iteration cap: plain GJK (init_dir = None) judges the pair.
       extra_iters = o.gjk_iters;
-      o = contact_pair<G>(c, a.prediction, nullptr, a.P, gl, A, 10000);
+      o = contact_pair<G>(c, a.prediction, false, idir, a.P, gl, A, 10000);
     }
     if (o.has == 2) {
       if (gl == 0) {
-        uint32_t slot = atomicAdd(&a.ovf[0], 1u);
-        if (slot < a.ovf_cap) a.ovf[1 + slot] = (uint32_t)k;
+        uint32_t slot = atomicAdd(a.ovf_count, 1u);
+        if (slot < a.ovf_cap) a.ovf_list[slot] = (uint32_t)k;
       }
       continue;
     }
@@ -1105,7 +1214,7 @@ extern "C" int ncx_shapes_create(ncx_ctx* ctx, const ncx_shape* shapes, size_t n
   S->npts = npts;
   int rc;
   if ((rc = S->shapes.reserve(n * sizeof(ncx_shape) + 64)) || (rc = S->hx.reserve(npts * 4 + 4)) ||
-      (rc = S->hy.reserve(npts * 4 + 4)) || (rc = S->hz.reserve(npts * 4 + 4)) || (rc = S->ovf.reserve(4 * 4096))) {
+      (rc = S->hy.reserve(npts * 4 + 4)) || (rc = S->hz.reserve(npts * 4 + 4)) || (rc = contact_tiers_reserve(S->tiers, 4096))) {
     delete S;
     return rc;
   }
@@ -1122,7 +1231,7 @@ extern "C" int ncx_shapes_create(ncx_ctx* ctx, const ncx_shape* shapes, size_t n
 
 extern "C" int ncx_shapes_destroy(ncx_shapes* s) {
   if (!s) return NCX_OK;
-  s->shapes.release(); s->hx.release(); s->hy.release(); s->hz.release(); s->arena.release(); s->ovf.release();
+  s->shapes.release(); s->hx.release(); s->hy.release(); s->hz.release(); contact_tiers_release(s->tiers);
   delete s;
   return NCX_OK;
 }
@@ -1133,27 +1242,151 @@ extern "C" int ncx_shapes_count(ncx_shapes* s, size_t* n) {
   return NCX_OK;
 }
 
-// Group width per launch: the lanes that share one pair's support scans.
-static int pick_group(ncx_shapes* s) {
-  (void)s;
-  return 16;
+// Arena tiers.  T1: 16 lanes per pair, 8 pairs per 128-thread block, LDS arenas of 36 vertices
+// and 72 faces (4.5 KiB per pair; 4 blocks fit a CU).  T2: 16 lanes per pair, 4 pairs per
+// 64-thread block, LDS arenas of 128 vertices and 256 faces (15.5 KiB per pair).  Beyond that,
+// device-memory arenas from 1024 vertices upward, 64 lanes per pair, host-driven.
+#define T1_G 16
+#define T1_BS 128
+#define T1_VC 36u
+#define T1_FC 72u
+#define T1_SC 32u
+#define T1_STC 32u
+#define T2_G 16
+#define T2_BS 64
+#define T2_VC 128u
+#define T2_FC 256u
+#define T2_SC 64u
+#define T2_STC 64u
+
+// Overflow lists of at least cap entries each (at most 2^24: a call with more overflowing pairs
+// fails with NCX_E_CAPACITY).
+int contact_tiers_reserve(ContactTiers& t, size_t cap) {
+  cap = std::min<size_t>(std::max<size_t>(cap, 4096), (size_t)1 << 24);
+  if (cap <= t.cap) return NCX_OK;
+  t.ovf.release();
+  t.cap = 0;
+  int rc = t.ovf.reserve(4 * (NCX_OVF_HEADER + 2 * cap));
+  if (rc) return rc;
+  t.cap = (uint32_t)cap;
+  return NCX_OK;
 }
 
-template <int G>
-static int launch_pairs(ncx_shapes* S, KArgs a, size_t npairs, unsigned blocks) {
+void contact_tiers_release(ContactTiers& t) {
+  t.ovf.release();
+  delete t.last;
+  t.last = nullptr;
+}
+
+// Device-memory tiers (VC = 0): static grid-stride over the listed pairs.
+static int launch_global_tier(hipStream_t stream, KArgs a, size_t npairs) {
   a.npairs = npairs;
-  hipLaunchKernelGGL(k_pairs<G>, dim3(blocks), dim3(256), 0, S->ctx->stream, a);
+  a.work = nullptr;
+  hipLaunchKernelGGL((k_pairs<64, 128, 0u, 0u, 0u, 0u>), dim3(ceil_div(npairs * 64, 128)), dim3(128), 0, stream, a);
   NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
 }
 
-// Graph-safe launch over a device-counted pair list: no host synchronisation.  EPA arena
-// overflows are appended to S->ovf (count at [0], pair indices from [1]); the caller checks
-// them after the work completes (contacts_finish).
-int contacts_launch_device_count(ncx_shapes* S, ncx_ctx* ctx, const ncx_pose* poses, const uint32_t* ia,
-                                 const uint32_t* ib, const uint32_t* n_dev, size_t cap, float prediction,
-                                 const float* init_dir, int max_iter, uint8_t* has, float* w1, float* w2, float* nrm,
-                                 float* depth, int32_t* iters) {
+// Re-runs the n pairs listed at list_src (device) with device-memory arenas 8x larger per round
+// (starting from 8 x the last tier's vcap/fcap) until none overflows; host-driven (synchronises
+// the stream).  Chunks keep each round's arenas <= 2 GiB.
+static int rerun_overflows(ContactTiers& T, ncx_ctx* ctx, KArgs a, uint32_t n, const uint32_t* list_src,
+                           uint32_t vcap, uint32_t fcap) {
+  if (n == 0) return NCX_OK;
+  uint32_t* ovf = T.ovf.as<uint32_t>();
+  DevBuf lst;
+  int rc;
+  if ((rc = lst.reserve((size_t)n * 4))) return rc;
+  NCX_HIP_CHECK(hipMemcpyAsync(lst.p, list_src, (size_t)n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  while (n > 0) {
+    vcap *= 8;
+    fcap *= 8;
+    if (fcap > 65535) fcap = 65535;
+    if (vcap > 32767) vcap = 32767;
+    const size_t per = arena_bytes(vcap, fcap);
+    const size_t chunk = std::max<size_t>(1, ((size_t)2 << 30) / per);
+    DevBuf big;
+    if ((rc = big.reserve(std::min<size_t>(n, chunk) * per))) return rc;
+    if ((rc = fill_u32(ctx, ovf + NCX_OVF_T2_COUNT, 1, 0u))) return rc;
+    for (size_t off = 0; off < n; off += chunk) {
+      size_t m = std::min<size_t>(chunk, n - off);
+      KArgs b = a;
+      b.list = lst.as<uint32_t>() + off;
+      b.n_dev = nullptr;
+      b.arena = big.as<uint8_t>();
+      b.vcap = vcap;
+      b.fcap = fcap;
+      b.ovf_count = ovf + NCX_OVF_T2_COUNT;
+      b.ovf_list = T.t2_list();
+      b.ovf_cap = T.cap;
+      if ((rc = launch_global_tier(ctx->stream, b, m))) return rc;
+    }
+    uint32_t left = 0;
+    NCX_HIP_CHECK(hipMemcpyAsync(&left, ovf + NCX_OVF_T2_COUNT, 4, hipMemcpyDeviceToHost, ctx->stream));
+    NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    big.release();
+    if (left) NCX_REQUIRE(fcap < 65535, "contact batch: EPA polytope exceeds 65535 faces");
+    n = left;
+    if (n) {
+      NCX_REQUIRE(n <= T.cap, "contact batch: too many EPA arena overflows");
+      NCX_HIP_CHECK(hipMemcpyAsync(lst.p, T.t2_list(), (size_t)n * 4, hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+    }
+  }
+  return NCX_OK;
+}
+
+// The two LDS tiers, with no host synchronisation (graph-safe): T1 runs every pair, listing those
+// whose EPA polytope outgrows its arena; T2 reruns that device-counted list, listing what
+// outgrows its arena in turn for contacts_resolve_overflow.  Both take pairs from a work
+// counter, so uneven pair costs balance across the chip.  npairs caps the pair count; n_dev
+// (optional) holds the actual count on the device.
+static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, size_t npairs,
+                        const uint32_t* n_dev) {
+  uint32_t* ovf = T.ovf.as<uint32_t>();
+  NCX_REQUIRE(ovf, "contact launch: overflow lists not allocated");
+  int rc;
+  if ((rc = fill_u32(ctx, ovf, NCX_OVF_HEADER, 0u))) return rc;
+  a.nshapes = (uint32_t)S->n;
+  a.bad_index = ovf + NCX_OVF_BAD;
+  a.arena = nullptr;
+  a.vcap = T1_VC;
+  a.fcap = T1_FC;
+  a.ovf_count = ovf + NCX_OVF_T1_COUNT;
+  a.ovf_list = T.t1_list();
+  a.ovf_cap = T.cap;
+  a.list = nullptr;
+  a.n_dev = n_dev;
+  a.npairs = npairs;
+  a.work = ovf + NCX_OVF_T1_WORK;
+  unsigned blocks = (unsigned)ctx->num_cus * 8;  // 4 blocks are resident per CU
+  if (!n_dev) blocks = std::max(1u, std::min<unsigned>(blocks, ceil_div(npairs * T1_G, T1_BS)));
+  hipLaunchKernelGGL((k_pairs<T1_G, T1_BS, T1_VC, T1_FC, T1_SC, T1_STC>), dim3(blocks), dim3(T1_BS), 0, ctx->stream,
+                     a);
+  NCX_HIP_CHECK(hipGetLastError());
+  KArgs b = a;
+  b.list = T.t1_list();
+  b.n_dev = ovf + NCX_OVF_T1_COUNT;
+  b.npairs = T.cap;
+  b.vcap = T2_VC;
+  b.fcap = T2_FC;
+  b.ovf_count = ovf + NCX_OVF_T2_COUNT;
+  b.ovf_list = T.t2_list();
+  b.work = ovf + NCX_OVF_T2_WORK;
+  hipLaunchKernelGGL((k_pairs<T2_G, T2_BS, T2_VC, T2_FC, T2_SC, T2_STC>), dim3(ctx->num_cus * 2), dim3(T2_BS), 0,
+                     ctx->stream, b);
+  NCX_HIP_CHECK(hipGetLastError());
+  if (!T.last) T.last = new KArgs;
+  *T.last = b;
+  return NCX_OK;
+}
+
+// Graph-safe tiered contact launch over a device-counted pair list (the scene step); the caller
+// runs contacts_resolve_overflow with the same tiers after the work completes.
+int contacts_launch_device_count(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const ncx_pose* poses,
+                                 const uint32_t* ia, const uint32_t* ib, const uint32_t* n_dev, size_t cap,
+                                 float prediction, const float* init_dir, int max_iter, uint8_t* has, float* w1,
+                                 float* w2, float* nrm, float* depth, int32_t* iters) {
   KArgs a = {};
   a.shapes = S->shapes.as<ncx_shape>();
   a.P = Pool{S->hx.as<float>(), S->hy.as<float>(), S->hz.as<float>()};
@@ -1165,91 +1398,33 @@ int contacts_launch_device_count(ncx_shapes* S, ncx_ctx* ctx, const ncx_pose* po
   a.max_iter = max_iter;
   a.has = has; a.w1 = w1; a.w2 = w2; a.nrm = nrm; a.depth = depth; a.iters = iters;
   a.distance_mode = 0;
-  a.n_dev = n_dev;
-  const int G = 16;
-  unsigned blocks = (unsigned)ctx->num_cus * 8;
-  uint32_t vcap = 160, fcap = 320;
-  size_t ngroups = (size_t)blocks * 256 / G;
-  int rc;
-  if ((rc = S->arena.reserve(ngroups * arena_bytes(vcap, fcap)))) return rc;
-  if ((rc = fill_u32(ctx, S->ovf.as<uint32_t>(), 2, 0u))) return rc;
-  if ((rc = fill_u32(ctx, S->ovf.as<uint32_t>() + 4095, 1, 0u))) return rc;
-  a.arena = S->arena.as<uint8_t>();
-  a.vcap = vcap;
-  a.fcap = fcap;
-  a.ovf = S->ovf.as<uint32_t>();
-  a.ovf_cap = 4090;
-  a.nshapes = (uint32_t)S->n;
-  a.bad_index = S->ovf.as<uint32_t>() + 4095;
-  a.npairs = cap;
-  hipLaunchKernelGGL(k_pairs<16>, dim3(blocks), dim3(256), 0, ctx->stream, a);
-  NCX_HIP_CHECK(hipGetLastError());
-  return NCX_OK;
+  return launch_tiers(S, T, ctx, a, cap, n_dev);
+}
+
+// After the last launch_tiers on T has completed on ctx's stream: fails on a bad pair index or on
+// more T1 overflows than the list holds, re-runs what T2 could not hold with device-memory
+// arenas, and reports how many pairs needed more than the T1 arena.
+int contacts_resolve_overflow(ContactTiers& T, ncx_ctx* ctx, size_t* n_large) {
+  NCX_REQUIRE(T.last, "contact resolve: no tiered launch to resolve");
+  uint32_t hdr[NCX_OVF_HEADER];
+  NCX_HIP_CHECK(hipMemcpyAsync(hdr, T.ovf.p, sizeof(hdr), hipMemcpyDeviceToHost, ctx->stream));
+  NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  NCX_REQUIRE(hdr[NCX_OVF_BAD] == 0, "contact batch: a pair index is not a shape of the set");
+  if (hdr[NCX_OVF_T1_COUNT] > T.cap || hdr[NCX_OVF_T2_COUNT] > T.cap) {
+    ncx_set_error("contact batch: more EPA arena overflows in one call than the overflow list holds");
+    return NCX_E_CAPACITY;
+  }
+  if (n_large) *n_large = hdr[NCX_OVF_T1_COUNT];
+  return rerun_overflows(T, ctx, *T.last, hdr[NCX_OVF_T2_COUNT], T.t2_list(), T2_VC,
+                         T2_FC);
 }
 
 static int run_pairs(ncx_shapes* S, KArgs a, size_t npairs) {
   if (npairs == 0) return NCX_OK;
-  ncx_ctx* ctx = S->ctx;
-  int G = pick_group(S);
-  // Persistent-style grid: enough groups to fill the chip, each with its own arena.
-  unsigned blocks = (unsigned)ctx->num_cus * 8;
-  size_t groups_needed = (npairs + 0) * G / 256 + 1;
-  if (blocks > groups_needed) blocks = (unsigned)groups_needed;
-  if (blocks == 0) blocks = 1;
-  uint32_t vcap = 160, fcap = 320;
-  size_t ngroups = (size_t)blocks * 256 / G;
   int rc;
-  if ((rc = S->arena.reserve(ngroups * arena_bytes(vcap, fcap)))) return rc;
-  const uint32_t ovf_cap = 4090;
-  if ((rc = fill_u32(ctx, S->ovf.as<uint32_t>(), 2, 0u))) return rc;
-  if ((rc = fill_u32(ctx, S->ovf.as<uint32_t>() + 4095, 1, 0u))) return rc;
-  a.arena = S->arena.as<uint8_t>();
-  a.nshapes = (uint32_t)S->n;
-  a.bad_index = S->ovf.as<uint32_t>() + 4095;
-  a.vcap = vcap;
-  a.fcap = fcap;
-  a.ovf = S->ovf.as<uint32_t>();
-  a.ovf_cap = ovf_cap;
-  a.list = nullptr;
-  switch (G) {
-    case 4: rc = launch_pairs<4>(S, a, npairs, blocks); break;
-    case 8: rc = launch_pairs<8>(S, a, npairs, blocks); break;
-    case 32: rc = launch_pairs<32>(S, a, npairs, blocks); break;
-    case 64: rc = launch_pairs<64>(S, a, npairs, blocks); break;
-    default: rc = launch_pairs<16>(S, a, npairs, blocks); break;
-  }
-  if (rc) return rc;
-  // Re-run pairs whose EPA polytope outgrew the arena, with larger arenas.
-  uint32_t novf = 0, bad = 0;
-  NCX_HIP_CHECK(hipMemcpyAsync(&novf, S->ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  NCX_HIP_CHECK(hipMemcpyAsync(&bad, S->ovf.as<uint32_t>() + 4095, 4, hipMemcpyDeviceToHost, ctx->stream));
-  NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  NCX_REQUIRE(bad == 0, "contact batch: a pair index is not a shape of the set");
-  while (novf > 0) {
-    NCX_REQUIRE(novf <= ovf_cap, "contact batch: too many EPA arena overflows");
-    std::vector<uint32_t> list(novf);
-    NCX_HIP_CHECK(hipMemcpy(list.data(), S->ovf.as<uint32_t>() + 1, novf * 4, hipMemcpyDeviceToHost));
-    vcap *= 8;
-    fcap *= 8;
-    NCX_REQUIRE(vcap <= 20480, "contact batch: EPA polytope exceeds 20480 vertices");
-    DevBuf lst, big;
-    if ((rc = lst.reserve(novf * 4)) || (rc = big.reserve((size_t)novf * arena_bytes(vcap, fcap)))) return rc;
-    NCX_HIP_CHECK(hipMemcpyAsync(lst.p, list.data(), novf * 4, hipMemcpyHostToDevice, ctx->stream));
-    if ((rc = fill_u32(ctx, S->ovf.as<uint32_t>(), 1, 0u))) return rc;
-    KArgs b = a;
-    b.list = lst.as<uint32_t>();
-    b.arena = big.as<uint8_t>();
-    b.vcap = vcap;
-    b.fcap = fcap;
-    unsigned rb = ceil_div((size_t)novf * 64, 256);
-    rc = launch_pairs<64>(S, b, novf, rb);
-    if (rc) return rc;
-    NCX_HIP_CHECK(hipMemcpyAsync(&novf, S->ovf.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-    NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    lst.release();
-    big.release();
-  }
-  return NCX_OK;
+  if ((rc = contact_tiers_reserve(S->tiers, npairs))) return rc;
+  if ((rc = launch_tiers(S, S->tiers, S->ctx, a, npairs, nullptr))) return rc;
+  return contacts_resolve_overflow(S->tiers, S->ctx, nullptr);
 }
 
 extern "C" int ncx_contact_batch(ncx_shapes* s, const ncx_pose* poses, const uint32_t* ia, const uint32_t* ib,
@@ -1312,6 +1487,27 @@ extern "C" int ncx_distance_batch(ncx_shapes* s, const ncx_pose* poses, const ui
   a.ia = ia;
   a.ib = ib;
   a.max_iter = 10000;
+  a.dist = dist;
+  a.iters = iters;
+  a.distance_mode = 1;
+  return run_pairs(s, a, npairs);
+}
+
+// distance_support_map_support_map_with_params with init_dir: GJK from init_dir capped at
+// max_iter iterations, after which plain GJK (init_dir = None) recomputes the pair.
+extern "C" int ncx_distance_batch_capped(ncx_shapes* s, const ncx_pose* poses, const uint32_t* ia,
+                                         const uint32_t* ib, size_t npairs, const float* init_dir, int32_t max_iter,
+                                         float* dist, int32_t* iters) {
+  NCX_REQUIRE(s && poses && ia && ib && dist, "ncx_distance_batch_capped: null argument");
+  NCX_REQUIRE(max_iter >= 1 && max_iter <= 10000, "ncx_distance_batch_capped: max_iter out of range");
+  KArgs a = {};
+  a.shapes = s->shapes.as<ncx_shape>();
+  a.P = Pool{s->hx.as<float>(), s->hy.as<float>(), s->hz.as<float>()};
+  a.poses = poses;
+  a.ia = ia;
+  a.ib = ib;
+  a.init_dir = init_dir;
+  a.max_iter = init_dir ? max_iter : 10000;
   a.dist = dist;
   a.iters = iters;
   a.distance_mode = 1;

@@ -40,6 +40,7 @@ struct ncx_world {
   DevBuf cnt, offs, counters;  // counters: [0] pair total, [1] pair overflow
   DevBuf pa, pb;
   DevBuf has, w1, w2, nrm, depth_out, iters;
+  ContactTiers tiers;  // this scene's contact overflow lists
   ncx_mlp* mlp = nullptr;
   DevBuf desc, dirs;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -233,7 +234,7 @@ static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses) {
     idir = w->dirs.as<float>();
   }
   // Pairs whose warm start hits the 32-iteration cap are recomputed by plain GJK inside the kernel.
-  return contacts_launch_device_count(w->shapes, c, poses, w->pa.as<uint32_t>(), w->pb.as<uint32_t>(), ctr, w->cap,
+  return contacts_launch_device_count(w->shapes, w->tiers, c, poses, w->pa.as<uint32_t>(), w->pb.as<uint32_t>(), ctr, w->cap,
                                       w->prediction, idir, w->mlp ? 32 : 10000, w->has.as<uint8_t>(),
                                       w->w1.as<float>(), w->w2.as<float>(), w->nrm.as<float>(),
                                       w->depth_out.as<float>(), w->iters.as<int32_t>());
@@ -265,7 +266,8 @@ extern "C" int ncx_world_create(ncx_ctx* ctx, ncx_shapes* shapes, float margin, 
       (rc = w->cnt.reserve(n * 4 + 64)) || (rc = w->offs.reserve(n * 4 + 64)) || (rc = w->counters.reserve(64)) ||
       (rc = w->pa.reserve(cap * 4)) || (rc = w->pb.reserve(cap * 4)) || (rc = w->has.reserve(cap)) ||
       (rc = w->w1.reserve(cap * 12)) || (rc = w->w2.reserve(cap * 12)) || (rc = w->nrm.reserve(cap * 12)) ||
-      (rc = w->depth_out.reserve(cap * 4)) || (rc = w->iters.reserve(cap * 4))) {
+      (rc = w->depth_out.reserve(cap * 4)) || (rc = w->iters.reserve(cap * 4)) || (rc = contact_tiers_reserve(w->tiers, std::min<size_t>(cap, (size_t)1 << 22)))) {
+    contact_tiers_release(w->tiers);
     delete w;
     return rc;
   }
@@ -306,6 +308,7 @@ extern "C" int ncx_world_destroy(ncx_world* w) {
                     &w->has, &w->w1, &w->w2, &w->nrm, &w->depth_out, &w->iters, &w->desc, &w->dirs};
   for (DevBuf* b : bufs) b->release();
   w->bvh.release();
+  contact_tiers_release(w->tiers);
   delete w;
   return NCX_OK;
 }
@@ -355,21 +358,24 @@ extern "C" int ncx_world_step(ncx_world* w, const ncx_pose* poses_dev, int use_g
   return NCX_OK;
 }
 
-// Waits for the last step; reports the pair count and fails on a pair-capacity overflow or a
-// contact that needs a larger EPA arena than the in-graph kernel has (re-run with
-// ncx_contact_batch on those pairs).
+// Waits for the last step and reports its pair count.  Fails on a pair-capacity overflow.
+// Contacts whose EPA polytope outgrew both in-step arena tiers are recomputed here with larger
+// arenas (the step's poses must be unchanged until this returns); *n_epa_overflow reports how
+// many pairs of the step needed more than the first-tier arena.
 extern "C" int ncx_world_sync(ncx_world* w, size_t* npairs, size_t* n_epa_overflow) {
   NCX_REQUIRE(w && npairs, "ncx_world_sync: null argument");
-  uint32_t ctr[2] = {0, 0}, ovf = 0;
+  uint32_t ctr[2] = {0, 0};
   NCX_HIP_CHECK(hipMemcpyAsync(ctr, w->counters.p, 8, hipMemcpyDeviceToHost, w->wctx.stream));
-  NCX_HIP_CHECK(hipMemcpyAsync(&ovf, w->shapes->ovf.p, 4, hipMemcpyDeviceToHost, w->wctx.stream));
   NCX_HIP_CHECK(hipStreamSynchronize(w->wctx.stream));
   *npairs = ctr[0];
-  if (n_epa_overflow) *n_epa_overflow = ovf;
   if (ctr[1] || ctr[0] > w->cap) {
     ncx_set_error("ncx_world_sync: more pairs than max_pairs");
     return NCX_E_CAPACITY;
   }
+  size_t large = 0;
+  int rc = contacts_resolve_overflow(w->tiers, &w->wctx, &large);
+  if (rc) return rc;
+  if (n_epa_overflow) *n_epa_overflow = large;
   return NCX_OK;
 }
 

@@ -3,13 +3,13 @@
 Workload (coherent_hulls): the hull-pair distribution of the batched GJK/EPA workload
 (262,144 pairs, N~U{8..256} points per hull), moved for 64 steps with per-shape linear
 velocity in a ball of radius 0.05/step and angular velocity in a ball of 0.05 rad/step.
-Variants per step:
-  plain     init_dir = None (the reference's default direction, t2 - t1)
+Each step runs contact (prediction 0) and distance on every pair.  Variants per step:
+  plain     init_dir = None (the reference's default directions)
   learned   init_dir = MLP prediction, GJK capped at 32 iterations, plain GJK on the cap
-  temporal  init_dir = the previous step's contact normal (the reference's own warm start,
+  temporal  init_dir = the previous step's final GJK/EPA direction (the reference's own warm start,
             last_gjk_dir of ConvexPolyhedronConvexPolyhedronManifoldGenerator,
             src/pipeline/narrow_phase/contact_generator/convex_polyhedron_convex_polyhedron_manifold_generator.rs:96-145)
-  oracle-dir init_dir = this step's plain result normal (the bound a perfect predictor reaches)
+  oracle-dir init_dir = this step's final GJK/EPA direction (what a perfect predictor reaches)
 """
 import time
 
@@ -103,7 +103,18 @@ def _agree(a, b):
     return 1.0 - bad / ha.numel()
 
 
-def evaluate_coherent(ctx, mlp, seed=4, npairs=1 << 18, steps=64, placer_iters=20, log=None):
+def _contact_dirs(ss, poses, ia, ib):
+    """Per-pair final GJK/EPA direction (the contact normal at a large prediction, so separated
+    pairs get their closest-points direction too); zero rows where there is none."""
+    g = query.contact(ss, poses, ia, ib, 10.0)
+    return torch.where(g["has_contact"][:, None], g["normal"], torch.zeros_like(g["normal"])).contiguous()
+
+
+def evaluate_coherent(ctx, mlp, seed=4, npairs=1 << 18, steps=64, placer_iters=20, log=None, cap=32,
+                      variants=("plain", "learned", "temporal", "oracle_dir")):
+    """Per step and variant: contact (prediction 0) and distance on every pair, both from the
+    variant's init_dir (GJK capped at `cap`, plain GJK on the cap).  Reports GJK iterations per
+    pair for each query, pairs/s for the two queries together, and agreement with plain."""
     placer, st = _placer(ctx)
     w = workloads.hull_pairs(seed=seed, npairs=npairs, placer=placer, bisect_iters=placer_iters)
     ss, ia, ib = st["ss"], st["ia"], st["ib"]
@@ -112,8 +123,10 @@ def evaluate_coherent(ctx, mlp, seed=4, npairs=1 << 18, steps=64, placer_iters=2
     v, om = workloads.coherent_motion(rng, 2 * npairs)
     poses_np = w["poses"]
     desc = descriptors(ss)
-    res = {k: dict(iters=0.0, time=0.0, agree=0.0) for k in ("plain", "learned", "temporal", "oracle_dir")}
-    prev_normal = None
+    keys = tuple(variants)
+    assert keys[0] == "plain"
+    res = {k: dict(citers=0.0, diters=0.0, time=0.0, agree=0.0, dagree=0.0) for k in keys}
+    prev_dirs = None
     n = len(w["ia"])
 
     def timed(fn):
@@ -123,37 +136,57 @@ def evaluate_coherent(ctx, mlp, seed=4, npairs=1 << 18, steps=64, placer_iters=2
         torch.cuda.synchronize()
         return out, time.perf_counter() - t0
 
+    def run(poses, dirs_fn):
+        def go():
+            dirs = dirs_fn()
+            if dirs is None:
+                c = query.contact(ss, poses, ia, ib, 0.0)
+                d = query.distance(ss, poses, ia, ib)
+            else:
+                c = query.contact(ss, poses, ia, ib, 0.0, init_dir=dirs, max_iter=cap)
+                d = query.distance(ss, poses, ia, ib, init_dir=dirs, max_iter=cap)
+            return c, d
+        return timed(go)
+
     for s in range(steps):
         poses = torch.as_tensor(poses_np, device=dev)
-        plain, tp = timed(lambda: query.contact(ss, poses, ia, ib, 0.0))
-
-        def learned_fn():
-            dirs = mlp.predict_dirs(ss, poses, desc, ia, ib)
-            return query.contact(ss, poses, ia, ib, 0.0, init_dir=dirs, max_iter=32)
-        learned, tl = timed(learned_fn)
-        if prev_normal is None:
-            temporal, tt = plain, tp
-        else:
-            temporal, tt = timed(lambda: query.contact(ss, poses, ia, ib, 0.0, init_dir=prev_normal))
-        od = torch.where(plain["has_contact"][:, None], plain["normal"],
-                         torch.zeros_like(plain["normal"])).contiguous()
-        odir, to = timed(lambda: query.contact(ss, poses, ia, ib, 0.0, init_dir=od))
-        for key, r, t in (("plain", plain, tp), ("learned", learned, tl), ("temporal", temporal, tt),
-                          ("oracle_dir", odir, to)):
-            res[key]["iters"] += r["gjk_iters"].float().mean().item() / steps
-            res[key]["time"] += t
-            res[key]["agree"] += _agree(r, plain) / steps
-        prev_normal = torch.where(plain["has_contact"][:, None], plain["normal"],
-                                  torch.zeros_like(plain["normal"])).contiguous()
+        need_dirs = "temporal" in keys or "oracle_dir" in keys
+        this_dirs = _contact_dirs(ss, poses, ia, ib) if need_dirs else None
+        out = {}
+        out["plain"] = run(poses, lambda: None)
+        if "learned" in keys:
+            out["learned"] = run(poses, lambda: mlp.predict_dirs(ss, poses, desc, ia, ib))
+        if "temporal" in keys:
+            out["temporal"] = out["plain"] if prev_dirs is None else run(poses, lambda: prev_dirs)
+        if "oracle_dir" in keys:
+            out["oracle_dir"] = run(poses, lambda: this_dirs)
+        (pc, (pd, _)), _ = out["plain"]
+        for key in keys:
+            (c, (d, dit)), t = out[key]
+            r = res[key]
+            r["citers"] += c["gjk_iters"].float().mean().item() / steps
+            r["diters"] += dit.float().mean().item() / steps
+            r["time"] += t
+            r["agree"] += _agree(c, pc) / steps
+            r["dagree"] += ((d - pd).abs() <= 1e-5 * pd.abs() + 1e-6).float().mean().item() / steps
+        prev_dirs = this_dirs
         poses_np = workloads.step_poses(poses_np, v, om)
-        if log and s % 16 == 0:
-            log(f"step {s}: plain {plain['gjk_iters'].float().mean().item():.2f} it, "
-                f"learned {learned['gjk_iters'].float().mean().item():.2f} it")
-    for key in res:
-        res[key]["pairs_per_s"] = n * steps / res[key]["time"]
-        res[key]["gjk_iters_per_pair"] = res[key].pop("iters")
-        res[key]["agreement_with_plain"] = res[key].pop("agree")
-        res[key]["seconds"] = res[key].pop("time")
+        if log and s % 16 == 0 and "learned" in keys:
+            (c0, (d0, i0)), _ = out["plain"]
+            (c1, (d1, i1)), _ = out["learned"]
+            log(f"step {s}: contact GJK plain {c0['gjk_iters'].float().mean().item():.2f} / learned "
+                f"{c1['gjk_iters'].float().mean().item():.2f} it; distance GJK plain {i0.float().mean().item():.2f}"
+                f" / learned {i1.float().mean().item():.2f} it")
+    for key in keys:
+        r = res[key]
+        r["pairs_per_s"] = n * steps / r["time"]
+        r["contact_gjk_iters_per_pair"] = r.pop("citers")
+        r["distance_gjk_iters_per_pair"] = r.pop("diters")
+        r["contact_agreement_with_plain"] = r.pop("agree")
+        r["distance_agreement_with_plain"] = r.pop("dagree")
+        r["seconds"] = r.pop("time")
+    if "learned" in keys:
+        res["speedup_learned_vs_plain"] = res["learned"]["pairs_per_s"] / res["plain"]["pairs_per_s"]
     res["pairs"] = n
     res["steps"] = steps
     return res

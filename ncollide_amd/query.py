@@ -52,16 +52,31 @@ def contact(shapes, poses, ia, ib, prediction, init_dir=None, max_iter=None):
     return dict(has_contact=has.bool(), world1=w1, world2=w2, normal=nr, depth=depth, gjk_iters=iters)
 
 
-def distance(shapes, poses, ia, ib):
-    """Minimum distance per pair (0 when touching or penetrating) and GJK iterations."""
+def distance(shapes, poses, ia, ib, init_dir=None, max_iter=None):
+    """Minimum distance per pair (0 when touching or penetrating) and GJK iterations.
+
+    init_dir (n,3), optional: distance_support_map_support_map_with_params's `init_dir`
+    (None, or an all-zero row, = the reference's None).  max_iter, optional: GJK iteration
+    cap for the init_dir run; a capped pair is recomputed by plain GJK.
+    """
     ctx = shapes.ctx
     dev = ctx.device
     poses = torch.as_tensor(poses, device=dev, dtype=torch.float32).contiguous()
     ia = _u32(ia, dev)
     ib = _u32(ib, dev)
     n = ia.numel()
+    if ib.numel() != n:
+        raise ValueError("ia and ib differ in length")
     d = torch.empty(n, dtype=torch.float32, device=dev)
     iters = torch.empty(n, dtype=torch.int32, device=dev)
-    _lib.check(ctx.lib.ncx_distance_batch(shapes.h, _lib.ptr(poses), _lib.ptr(ia), _lib.ptr(ib), n, _lib.ptr(d),
-                                          _lib.ptr(iters)))
+    if init_dir is None and max_iter is None:
+        _lib.check(ctx.lib.ncx_distance_batch(shapes.h, _lib.ptr(poses), _lib.ptr(ia), _lib.ptr(ib), n,
+                                              _lib.ptr(d), _lib.ptr(iters)))
+    else:
+        idir = None
+        if init_dir is not None:
+            idir = torch.as_tensor(init_dir, device=dev, dtype=torch.float32).contiguous()
+        _lib.check(ctx.lib.ncx_distance_batch_capped(shapes.h, _lib.ptr(poses), _lib.ptr(ia), _lib.ptr(ib), n,
+                                                     _lib.ptr(idir), int(10000 if max_iter is None else max_iter),
+                                                     _lib.ptr(d), _lib.ptr(iters)))
     return d, iters

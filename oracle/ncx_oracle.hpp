@@ -958,9 +958,11 @@ template <class R> inline bool contact(const Iso<R>& m1, const Shape<R>& g1, con
   return contact_support_map_support_map(m1, g1, m2, g2, prediction, (const V3<R>*)nullptr, out, gjk_iters, epa_iters);
 }
 
-// query::distance (distance.rs:9-33)
+// query::distance (distance.rs:9-33).  init_dir / max_iter: the init_dir argument of
+// distance_support_map_support_map_with_params with GJK capped at max_iter (< 10000) and plain
+// GJK from the default direction on the cap; gjk_iters counts both runs.
 template <class R> inline R distance(const Iso<R>& m1, const Shape<R>& g1, const Iso<R>& m2, const Shape<R>& g2,
-                                     const V3<R>* init_dir = nullptr, int* gjk_iters = nullptr) {
+                                     const V3<R>* init_dir = nullptr, int* gjk_iters = nullptr, int max_iter = 10000) {
   if (gjk_iters) *gjk_iters = 0;
   if (g1.type == BALL && g2.type == BALL) {
     // distance_ball_ball.rs:7-24
@@ -971,14 +973,21 @@ template <class R> inline R distance(const Iso<R>& m1, const Shape<R>& g1, const
     return std::sqrt(d2) - s;
   }
   // distance_support_map_support_map_with_params (distance_support_map_support_map.rs:25-53)
-  V3<R> d0 = init_dir ? *init_dir : (m1.t - m2.t);
-  V3<R> dir;
-  VoronoiSimplex<R> simplex;
-  if (try_normalize(d0, Eps<R>::eps(), &dir)) simplex.reset(cso_from_shapes(m1, g1, m2, g2, dir));
-  else simplex.reset(cso_from_shapes(m1, g1, m2, g2, V3<R>(1, 0, 0)));
-  GJKResult<R> r = gjk_closest_points(m1, g1, m2, g2, Eps<R>::maxv(), true, simplex);
-  if (gjk_iters) *gjk_iters = r.iters;
-  if (r.status == GJK_CLOSEST) return norm(r.p1 - r.p2);
+  int total = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    bool warm = init_dir && pass == 0;
+    V3<R> d0 = warm ? *init_dir : (m1.t - m2.t);
+    V3<R> dir;
+    VoronoiSimplex<R> simplex;
+    if (try_normalize(d0, Eps<R>::eps(), &dir)) simplex.reset(cso_from_shapes(m1, g1, m2, g2, dir));
+    else simplex.reset(cso_from_shapes(m1, g1, m2, g2, V3<R>(1, 0, 0)));
+    GJKResult<R> r = gjk_closest_points(m1, g1, m2, g2, Eps<R>::maxv(), true, simplex, warm ? max_iter : 10000);
+    total += r.iters;
+    if (gjk_iters) *gjk_iters = total;
+    if (r.status == GJK_CAP) continue;
+    if (r.status == GJK_CLOSEST) return norm(r.p1 - r.p2);
+    return R(0);
+  }
   return R(0);
 }
 

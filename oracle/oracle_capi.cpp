@@ -99,7 +99,7 @@ int contact_sm_batch(int64_t ns, const uint32_t* types, const R* params, const u
       if (d0.x != R(0) || d0.y != R(0) || d0.z != R(0)) dptr = &d0;  // all-zero entry = None
     }
     bool ok = contact_support_map_support_map(pose_at(poses, a), ss.shapes[a], pose_at(poses, b), ss.shapes[b],
-                                              prediction, dptr, &c, &gi, &ei, gjk_max_iter, &st);
+                                              prediction, dptr, &c, &gi, &ei, dptr ? gjk_max_iter : 10000, &st);
     has[k] = ok ? 1 : 0;
     gjk_it[k] = gi; epa_it[k] = ei; status[k] = st;
     for (int d = 0; d < 3; ++d) {
@@ -115,13 +115,19 @@ int contact_sm_batch(int64_t ns, const uint32_t* types, const R* params, const u
 template <class R>
 int distance_batch(int64_t ns, const uint32_t* types, const R* params, const uint32_t* hoff, const uint32_t* hcnt,
                    const R* hpts, int64_t npts_total, const R* poses, const uint32_t* ia, const uint32_t* ib, int64_t np,
-                   R* dist, int32_t* gjk_it, int nthreads) {
+                   const R* init_dirs, int32_t max_iter, R* dist, int32_t* gjk_it, int nthreads) {
   ShapeSet<R> ss;
   ss.load(ns, types, params, hoff, hcnt, hpts, npts_total);
   parallel_for(np, nthreads, [&](int64_t k) {
     uint32_t a = ia[k], b = ib[k];
     int gi = 0;
-    dist[k] = distance(pose_at(poses, a), ss.shapes[a], pose_at(poses, b), ss.shapes[b], (const V3<R>*)nullptr, &gi);
+    V3<R> idir;
+    const V3<R>* ip = nullptr;
+    if (init_dirs) {  // an all-zero row means None
+      idir = V3<R>(init_dirs[3 * k], init_dirs[3 * k + 1], init_dirs[3 * k + 2]);
+      if (idir.x != 0 || idir.y != 0 || idir.z != 0) ip = &idir;
+    }
+    dist[k] = distance(pose_at(poses, a), ss.shapes[a], pose_at(poses, b), ss.shapes[b], ip, &gi, max_iter);
     if (gjk_it) gjk_it[k] = gi;
   });
   return 0;
@@ -246,13 +252,17 @@ int orc_contact_sm_f64(int64_t ns, const uint32_t* types, const double* params, 
 }
 int orc_distance_f32(int64_t ns, const uint32_t* types, const float* params, const uint32_t* hoff,
                      const uint32_t* hcnt, const float* hpts, int64_t npts, const float* poses, const uint32_t* ia,
-                     const uint32_t* ib, int64_t np, float* dist, int32_t* gi, int nthreads) {
-  return distance_batch<float>(ns, types, params, hoff, hcnt, hpts, npts, poses, ia, ib, np, dist, gi, nthreads);
+                     const uint32_t* ib, int64_t np, const float* init_dirs, int32_t max_iter, float* dist, int32_t* gi,
+                     int nthreads) {
+  return distance_batch<float>(ns, types, params, hoff, hcnt, hpts, npts, poses, ia, ib, np, init_dirs, max_iter, dist,
+                               gi, nthreads);
 }
 int orc_distance_f64(int64_t ns, const uint32_t* types, const double* params, const uint32_t* hoff,
                      const uint32_t* hcnt, const double* hpts, int64_t npts, const double* poses, const uint32_t* ia,
-                     const uint32_t* ib, int64_t np, double* dist, int32_t* gi, int nthreads) {
-  return distance_batch<double>(ns, types, params, hoff, hcnt, hpts, npts, poses, ia, ib, np, dist, gi, nthreads);
+                     const uint32_t* ib, int64_t np, const double* init_dirs, int32_t max_iter, double* dist,
+                     int32_t* gi, int nthreads) {
+  return distance_batch<double>(ns, types, params, hoff, hcnt, hpts, npts, poses, ia, ib, np, init_dirs, max_iter,
+                                dist, gi, nthreads);
 }
 int orc_shape_aabb_f32(int64_t ns, const uint32_t* types, const float* params, const uint32_t* hoff,
                        const uint32_t* hcnt, const float* hpts, int64_t npts, const float* poses, float* out6) {

@@ -47,7 +47,7 @@ def _declare(L):
         sig(f"orc_contact_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P, P, I64, R, P, P, P, P, P, P, P, C.c_int)
         sig(f"orc_contact_sm_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P, P, I64, R, P, C.c_int32,
             P, P, P, P, P, P, P, P, C.c_int)
-        sig(f"orc_distance_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P, P, I64, P, P, C.c_int)
+        sig(f"orc_distance_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P, P, I64, P, C.c_int32, P, P, C.c_int)
         sig(f"orc_shape_aabb_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P)
         sig(f"orc_trimesh_create_{t}", P, P, I64, P, I64)
         sig(f"orc_trimesh_free_{t}", None, P)
@@ -135,7 +135,9 @@ def contact_support_map(ss, poses, ia, ib, prediction, init_dirs=None, max_iter=
                 status=st)
 
 
-def distance(ss, poses, ia, ib, nthreads=1):
+def distance(ss, poses, ia, ib, nthreads=1, init_dirs=None, max_iter=10000):
+    """query::distance per pair; init_dirs (n,3) optional (all-zero row = None), GJK from
+    init_dir capped at max_iter with plain GJK on the cap."""
     L = lib()
     R = ss.R
     poses = np.ascontiguousarray(poses, dtype=R).reshape(-1, 7)
@@ -144,7 +146,9 @@ def distance(ss, poses, ia, ib, nthreads=1):
     n = len(ia)
     d = np.zeros(n, R)
     gi = np.zeros(n, np.int32)
-    getattr(L, f"orc_distance_{ss.tag}")(*ss.args(), _p(poses), _p(ia), _p(ib), n, _p(d), _p(gi), nthreads)
+    dirs = None if init_dirs is None else np.ascontiguousarray(init_dirs, dtype=R).reshape(n, 3)
+    getattr(L, f"orc_distance_{ss.tag}")(*ss.args(), _p(poses), _p(ia), _p(ib), n, _p(dirs), int(max_iter), _p(d),
+                                         _p(gi), nthreads)
     return d, gi
 
 

@@ -298,3 +298,25 @@ def test_warm_started_contacts_match_oracle(ctx, orc):
         for key in ("has", "world1", "world2", "normal", "depth"):
             r[key][capped] = rp[key]
     assert_contacts_match(g, r, "warm-started hull pairs")
+
+
+def test_warm_started_distances_match_oracle(ctx, orc):
+    """Distance with init_dir and a 32-iteration cap (plain GJK recomputes capped pairs) against
+    the oracle's distance run with the same directions and the same rule; also checks that the
+    iteration counts (both runs summed) agree exactly."""
+    w = workloads.hull_pairs(seed=14, npairs=1500)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    rng = np.random.default_rng(4)
+    dirs = workloads.unit_vectors(rng, len(w["ia"])).astype(np.float32)
+    dirs[::3] = w["dirs"][::3]
+    dirs[1::7] = 0.0  # all-zero rows are None
+    d, it = ncx.query.distance(ss, poses, w["ia"], w["ib"], init_dir=torch.as_tensor(dirs, device=ctx.device),
+                               max_iter=32)
+    rd, rit = orc.distance(oracle_shapes(orc, w["records"], w["pool"]), to_oracle_poses(w["poses"]), w["ia"],
+                           w["ib"], nthreads=8, init_dirs=dirs, max_iter=32)
+    d = d.cpu().numpy()
+    assert np.array_equal(it.cpu().numpy(), rit)
+    err = np.abs(d.astype(np.float64) - rd)
+    assert np.all(err <= REL * np.abs(rd) + ABS), f"max err {err.max()}"
+    print(f"warm-started distance: bit-exact fraction {np.mean(d == rd):.4f}")

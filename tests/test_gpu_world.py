@@ -34,6 +34,7 @@ def test_world_steps_match_reference(ctx, orc, use_graph):
     oss = oracle_shapes(orc, w["records"], w["pool"])
     p_np = w["poses"].copy()
     vel = w["velocities"] * 0.2
+    n_large = 0
     for step in range(7):
         world.step(poses, use_graph=use_graph)
         r = world.results()
@@ -54,13 +55,14 @@ def test_world_steps_match_reference(ctx, orc, use_graph):
         ia = r["ia"].cpu().numpy()[order]
         ib = r["ib"].cpu().numpy()[order]
         ref = orc.contact(oss, to_oracle_poses(p_np), ia, ib, 0.0, nthreads=8)
-        assert r["epa_overflow"] == 0
+        n_large += r["epa_overflow"]
         assert np.array_equal(r["has_contact"].cpu().numpy()[order], ref["has"]), f"has differs at step {step}"
         idx = np.nonzero(ref["has"])[0]
         dg = r["depth"].cpu().numpy()[order][idx].astype(np.float64)
         dr = ref["depth"][idx].astype(np.float64)
         assert np.all(np.abs(dg - dr) <= 1e-5 * np.abs(dr) + 1e-6), f"depth differs at step {step}"
-        print(f"step {step}: {len(order)} pairs, {len(idx)} contacts, bit-exact {np.mean(dg == dr):.4f}")
+        print(f"step {step}: {len(order)} pairs, {len(idx)} contacts, bit-exact {np.mean(dg == dr):.4f}, "
+              f"{r['epa_overflow']} pairs past the first EPA arena")
         p_np[:, 0:3] += vel
         poses.copy_(torch.as_tensor(p_np, device=ctx.device))
 

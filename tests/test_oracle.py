@@ -318,6 +318,41 @@ def test_gjk_distance_exact_for_polytopes(orc):
             assert abs(d[0] - ex) < 1e-6 * max(1, ex)
 
 
+def test_warm_started_distance_rules(orc):
+    """init_dir = the exact separating direction keeps the exact distance and needs fewer GJK
+    iterations on average (not on every pair: the first support point along the exact normal is
+    a vertex of the closest face, and GJK still has to assemble that face); a cap of one
+    iteration from a poor direction falls back to plain GJK, so the distance equals the plain one
+    bit for bit and the count is 1 + the plain count."""
+    rng = np.random.default_rng(17)
+    plain_its, warm_its = [], []
+    for _ in range(25):
+        A = rng.normal(size=(40, 3)) * rng.uniform(0.3, 1.0, 3)
+        B = rng.normal(size=(40, 3)) * rng.uniform(0.3, 1.0, 3)
+        u = workloads.unit_vectors(rng, 1)[0]
+        tb = u * rng.uniform(3.0, 5.0)
+        ss = shapeset(orc, [("hull", A), ("hull", B)])
+        poses = np.stack([pose7((0, 0, 0)), pose7(tb)])
+        d0, it0 = orc.distance(ss, poses, [0], [1])
+        if d0[0] <= 0:  # penetrating: _exact_distance measures boundaries only
+            continue
+        ex = _exact_distance(A, B + tb)
+        # exact direction from shape 1 to shape 2 (the contact normal at a large prediction)
+        r = orc.contact(ss, poses, [0], [1], 100.0)
+        n = r["normal"][0]
+        d1, it1 = orc.distance(ss, poses, [0], [1], init_dirs=n[None], max_iter=32)
+        assert abs(d1[0] - ex) < 1e-6 * max(1, ex)
+        plain_its.append(it0[0])
+        warm_its.append(it1[0])
+        d2, it2 = orc.distance(ss, poses, [0], [1], init_dirs=-n[None], max_iter=1)
+        assert d2[0] == d0[0]
+        assert it2[0] == 1 + it0[0]
+        # an all-zero direction is None
+        d3, it3 = orc.distance(ss, poses, [0], [1], init_dirs=np.zeros((1, 3)), max_iter=1)
+        assert d3[0] == d0[0] and it3[0] == it0[0]
+    assert len(plain_its) > 10 and np.mean(warm_its) < np.mean(plain_its)
+
+
 def test_epa_depth_exact_for_polytopes(orc):
     rng = np.random.default_rng(8)
     checked = 0
