@@ -31,6 +31,8 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--quick", action="store_true", help="skip the secondary workloads")
+    p.add_argument("--only", choices=["hulls", "learned", "scene", "plain"], default=None,
+                   help="run one secondary workload alone (after the ray-cast headline); for profiling")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--copy-calibration", action="store_true",
                    help="copy 256 MiB device to device once before the timed casts (the reference dispatch "
@@ -333,7 +335,7 @@ def main():
                    "rays": r["n"], "triangles": 131072, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": r["achieved"] / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": r["achieved"] / HBM_PEAK, "traffic": _traffic("raycast"),
-                     "kernel": "k_cast<false>", "kernel_ms": r["kernel_ms"],
+                     "kernel": "k_cast_pt", "kernel_ms": r["kernel_ms"],
                      "alg_bytes_per_launch": r["alg_bytes"], "nodes_per_ray": r["nodes_per_ray"],
                      "tris_per_ray": r["tris_per_ray"]},
         "hit_fraction": r["hits"],
@@ -345,18 +347,22 @@ def main():
         result["cpu_baseline"]["single_thread_rays_per_s"] = cb["single_thread"]
     if not args.quick:
         sec = {}
-        log("hull pairs ...")
-        h = bench_hulls(args, ncx, torch, ctx)
-        sec["hull_pairs"] = {k: v for k, v in h.items() if k not in ("w", "gen_s")}
-        if not args.no_cpu:
-            cb = cpu_baseline_hulls(h["w"])
-            sec["hull_pairs"]["cpu_baseline"] = cb
-        log("learned warm start ...")
-        sec["learned_warm_start"] = bench_learned(args, ncx, torch, ctx)
-        log("scene step ...")
-        sec["scene_step"] = bench_scene(args, ncx, torch, ctx)
-        log("plain pipeline ...")
-        sec["plain_pipeline"] = bench_plain_pipeline(ncx, torch, ctx)
+        want = (lambda k: args.only is None or args.only == k)
+        if want("hulls"):
+            log("hull pairs ...")
+            h = bench_hulls(args, ncx, torch, ctx)
+            sec["hull_pairs"] = {k: v for k, v in h.items() if k not in ("w", "gen_s")}
+            if not args.no_cpu:
+                sec["hull_pairs"]["cpu_baseline"] = cpu_baseline_hulls(h["w"])
+        if want("learned"):
+            log("learned warm start ...")
+            sec["learned_warm_start"] = bench_learned(args, ncx, torch, ctx)
+        if want("scene"):
+            log("scene step ...")
+            sec["scene_step"] = bench_scene(args, ncx, torch, ctx)
+        if want("plain"):
+            log("plain pipeline ...")
+            sec["plain_pipeline"] = bench_plain_pipeline(ncx, torch, ctx)
         result["secondary"] = sec
     print(json.dumps(result))
 

@@ -317,11 +317,16 @@ __global__ __launch_bounds__(256) void k_morton(const float* __restrict__ minx, 
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float c[3] = {(minx[i] + maxx[i]) * 0.5f, (miny[i] + maxy[i]) * 0.5f, (minz[i] + maxz[i]) * 0.5f};
+  // One scale for all three axes (the largest centroid extent): a flat set of primitives (a
+  // heightfield) then splits along its long axes first instead of cutting it into thin layers
+  // that each span the whole set.
+  float ext = 0.f;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) ext = fmaxf(ext, ord2f(b[3 + d]) - ord2f(b[d]));
   uint32_t q[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    float lo = ord2f(b[d]), hi = ord2f(b[3 + d]);
-    float ext = hi - lo;
+    float lo = ord2f(b[d]);
     float u = ext > 0.f ? (c[d] - lo) / ext : 0.f;
     float s = fminf(fmaxf(u * 1024.f, 0.f), 1023.f);
     q[d] = (uint32_t)s;

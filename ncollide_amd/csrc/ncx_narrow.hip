@@ -519,9 +519,7 @@ struct VertA {
 struct FaceD {
   uint16_t pts[3];
   uint16_t adj[3];
-  float n[3];
-  float bc[3];
-  uint32_t deleted;
+  uint16_t deleted, pad;
 };
 struct HeapE {
   float neg_dist;
@@ -629,35 +627,48 @@ __device__ __forceinline__ bool heap_pop(HeapE* h, uint32_t& len, HeapE& out) {
 #define EPA_NONE 0
 #define EPA_OVERFLOW 2
 
-// Face::new (epa3.rs:96-115) + new_with_proj (:63-94) + ccw_face_normal (utils/ccw_face_normal.rs:21-27).
+// ccw_face_normal (utils/ccw_face_normal.rs:21-27): None becomes the zero vector.
+__device__ __forceinline__ f3 tri_normal(f3 va, f3 vb, f3 vc) {
+  f3 nn;
+  float nl;
+  if (try_normalize(cross(vb - va, vc - va), NCX_FLT_EPS, &nn, &nl)) return nn;
+  return f3{0.f, 0.f, 0.f};
+}
+
+// A face keeps only its vertex and neighbour ids.  Its normal and the barycentric coordinates
+// of the origin's projection are recomputed from the vertices when needed, with the operations
+// Face::new used, so they are the same bits as stored ones would be.
+__device__ __forceinline__ f3 face_normal(const Arena& A, const FaceD& f) {
+  return tri_normal(vpt(A, f.pts[0]), vpt(A, f.pts[1]), vpt(A, f.pts[2]));
+}
+
+// Face::new (epa3.rs:96-115) + new_with_proj (:63-94): fills f, returns whether the origin
+// projects inside the triangle (proj_is_inside) and the face normal in n.
 __device__ __forceinline__ bool make_face(const Arena& A, uint32_t p0, uint32_t p1, uint32_t p2, uint32_t a0,
-                                          uint32_t a1, uint32_t a2, FaceD& f) {
+                                          uint32_t a1, uint32_t a2, FaceD& f, f3& n) {
   f3 va = vpt(A, p0), vb = vpt(A, p1), vc = vpt(A, p2);
   Loc loc;
   proj_triangle(va, vb, vc, loc);
   f.pts[0] = (uint16_t)p0; f.pts[1] = (uint16_t)p1; f.pts[2] = (uint16_t)p2;
   f.adj[0] = (uint16_t)a0; f.adj[1] = (uint16_t)a1; f.adj[2] = (uint16_t)a2;
   f.deleted = 0;
-  f3 nn;
-  float nl;
-  if (try_normalize(cross(vb - va, vc - va), NCX_FLT_EPS, &nn, &nl)) { f.n[0] = nn.x; f.n[1] = nn.y; f.n[2] = nn.z; }
-  else { f.n[0] = 0.f; f.n[1] = 0.f; f.n[2] = 0.f; }
-  if (loc.kind == LOC_FACE) {
-    f.bc[0] = loc.bc0; f.bc[1] = loc.bc1; f.bc[2] = loc.bc2;
-    return true;
-  }
-  f.bc[0] = f.bc[1] = f.bc[2] = 0.f;
-  return false;
+  f.pad = 0;
+  n = tri_normal(va, vb, vc);
+  return loc.kind == LOC_FACE;
 }
 
-__device__ __forceinline__ f3 fnormal(const FaceD& f) { return f3{f.n[0], f.n[1], f.n[2]}; }
-
+// Face::closest_points with the barycentric coordinates new_with_proj stored (zero unless the
+// projection was inside the face).
 __device__ __forceinline__ void face_points(const Arena& A, const FaceD& f, f3& p1, f3& p2) {
   const CSO a = vget(A, f.pts[0]);
   const CSO b = vget(A, f.pts[1]);
   const CSO c = vget(A, f.pts[2]);
-  p1 = a.o1 * f.bc[0] + b.o1 * f.bc[1] + c.o1 * f.bc[2];
-  p2 = a.o2 * f.bc[0] + b.o2 * f.bc[1] + c.o2 * f.bc[2];
+  Loc loc;
+  proj_triangle(a.p, b.p, c.p, loc);
+  float b0 = 0.f, b1 = 0.f, b2 = 0.f;
+  if (loc.kind == LOC_FACE) { b0 = loc.bc0; b1 = loc.bc1; b2 = loc.bc2; }
+  p1 = a.o1 * b0 + b.o1 * b1 + c.o1 * b2;
+  p2 = a.o2 * b0 + b.o2 * b1 + c.o2 * b2;
 }
 
 __device__ __forceinline__ uint32_t next_ccw(const FaceD& f, uint32_t id) {
@@ -681,11 +692,13 @@ __device__ __forceinline__ bool affinely_dependent(f3 p1, f3 p2, f3 p3) {
 
 // Face::can_be_seen_by (epa3.rs:143-156)
 __device__ __forceinline__ bool can_be_seen(const Arena& A, const FaceD& f, uint32_t point, uint32_t opp) {
-  f3 p0 = vpt(A, f.pts[opp]);
-  f3 p1 = vpt(A, f.pts[(opp + 1) % 3]);
-  f3 p2 = vpt(A, f.pts[(opp + 2) % 3]);
+  f3 v0 = vpt(A, f.pts[0]), v1 = vpt(A, f.pts[1]), v2 = vpt(A, f.pts[2]);
+  f3 n = tri_normal(v0, v1, v2);
+  f3 p0 = sel3(opp == 0, v0, sel3(opp == 1, v1, v2));
+  f3 p1 = sel3(opp == 0, v1, sel3(opp == 1, v2, v0));
+  f3 p2 = sel3(opp == 0, v2, sel3(opp == 1, v0, v1));
   f3 pt = vpt(A, point);
-  return dot(pt - p0, fnormal(f)) >= -(NCX_FLT_EPS * 10.f) || affinely_dependent(p1, p2, pt);
+  return dot(pt - p0, n) >= -(NCX_FLT_EPS * 10.f) || affinely_dependent(p1, p2, pt);
 }
 
 // EPA::closest_points (epa3.rs:219-430).  compute_silhouette's recursion
@@ -718,14 +731,17 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
     const uint32_t PT[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {0, 3, 1}};
     const uint32_t AD[4][3] = {{3, 1, 2}, {3, 2, 0}, {0, 1, 3}, {2, 1, 0}};
     bool inside[4];
+    f3 fn[4];
+#pragma unroll
     for (int f = 0; f < 4; ++f) {
       FaceD fd;
-      inside[f] = make_face(A, PT[f][0], PT[f][1], PT[f][2], AD[f][0], AD[f][1], AD[f][2], fd);
+      inside[f] = make_face(A, PT[f][0], PT[f][1], PT[f][2], AD[f][0], AD[f][1], AD[f][2], fd, fn[f]);
       A.faces[nf++] = fd;
     }
+#pragma unroll
     for (int f = 0; f < 4; ++f) {
       if (inside[f]) {
-        float dist = dot(fnormal(A.faces[f]), vpt(A, f));
+        float dist = dot(fn[f], vpt(A, f));
         if (-dist > gjk_tol) return EPA_NONE;
         A.heap[hl] = HeapE{-dist, (uint32_t)f};
         heap_sift_up(A.heap, hl);
@@ -741,8 +757,9 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
       vset(A, nv++, cso_from<G>(c, cross(a, dpt), P, gl));
     }
     FaceD f1, f2;
-    make_face(A, 0, 1, 2, 1, 1, 1, f1);
-    make_face(A, 0, 2, 1, 0, 0, 0, f2);
+    f3 n1, n2;
+    make_face(A, 0, 1, 2, 1, 1, 1, f1, n1);
+    make_face(A, 0, 2, 1, 0, 0, 0, f2, n2);
     A.faces[nf++] = f1;
     A.faces[nf++] = f2;
     A.heap[hl] = HeapE{0.f, 0u}; heap_sift_up(A.heap, hl); hl++;
@@ -757,17 +774,18 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
     FaceD face = A.faces[fid.id];
     if (face.deleted) continue;
     if (nv >= A.vcap) return EPA_OVERFLOW;
-    CSO cso = cso_from<G>(c, fnormal(face), P, gl);
+    const f3 face_n = face_normal(A, face);
+    CSO cso = cso_from<G>(c, face_n, P, gl);
     iters += 1;
     uint32_t sp_id = nv;
     vset(A, nv++, cso);
-    float cand = dot(cso.p, fnormal(face));
+    float cand = dot(cso.p, face_n);
     if (cand < max_dist) { best = fid; max_dist = cand; }
     float curr_dist = -fid.neg_dist;
     if (max_dist - curr_dist < eps_tol) {
       FaceD bf = A.faces[best.id];
       face_points(A, bf, p1, p2);
-      n = fnormal(bf);
+      n = face_normal(A, bf);
       return EPA_OK;
     }
     A.faces[fid.id].deleted = 1;
@@ -805,14 +823,15 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
         uint32_t pt1 = A.faces[edge.face].pts[(edge.opp + 2) % 3];
         uint32_t pt2 = A.faces[edge.face].pts[(edge.opp + 1) % 3];
         FaceD nfc;
-        bool inside = make_face(A, pt1, pt2, sp_id, edge.face, new_id + 1, new_id - 1, nfc);
+        f3 nfn;
+        bool inside = make_face(A, pt1, pt2, sp_id, edge.face, new_id + 1, new_id - 1, nfc, nfn);
         A.faces[edge.face].adj[(edge.opp + 1) % 3] = (uint16_t)new_id;
         A.faces[nf++] = nfc;
         if (inside) {
-          float dist = dot(fnormal(nfc), vpt(A, nfc.pts[0]));
+          float dist = dot(nfn, vpt(A, nfc.pts[0]));
           if (dist < curr_dist) {
             face_points(A, face, p1, p2);
-            n = fnormal(face);
+            n = face_n;
             return EPA_OK;
           }
           if (-dist > gjk_tol) return EPA_NONE;
@@ -830,7 +849,7 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
   }
   FaceD bf = A.faces[best.id];
   face_points(A, bf, p1, p2);
-  n = fnormal(bf);
+  n = face_normal(A, bf);
   return EPA_OK;
 }
 
@@ -1075,7 +1094,7 @@ struct KArgs {
 // larger arenas.  VC == 0: the arena is the group's slice of a.arena (a.vcap, a.fcap) in device
 // memory.
 template <int G, int BS, uint32_t VC, uint32_t FC, uint32_t SC, uint32_t STC>
-__global__ __launch_bounds__(BS) void k_pairs(KArgs a) {
+__global__ __launch_bounds__(BS, 2) void k_pairs(KArgs a) {
   const int gl = threadIdx.x & (G - 1);
   const size_t group = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const size_t ngroups = ((size_t)gridDim.x * blockDim.x) / G;
@@ -1242,20 +1261,21 @@ extern "C" int ncx_shapes_count(ncx_shapes* s, size_t* n) {
   return NCX_OK;
 }
 
-// Arena tiers.  T1: 16 lanes per pair, 8 pairs per 128-thread block, LDS arenas of 36 vertices
-// and 72 faces (4.5 KiB per pair; 4 blocks fit a CU).  T2: 16 lanes per pair, 4 pairs per
-// 64-thread block, LDS arenas of 128 vertices and 256 faces (15.5 KiB per pair).  Beyond that,
-// device-memory arenas from 1024 vertices upward, 64 lanes per pair, host-driven.
+// Arena tiers.  T1: 16 lanes per pair, 8 pairs per 128-thread block, LDS arenas of 44 vertices
+// and 144 faces (4.7 KiB per pair; 4 blocks fit a CU).  T2: 16 lanes per pair, 4 pairs per
+// 64-thread block, LDS arenas of 128 vertices and 384 faces (12.5 KiB per pair; 3 blocks per
+// CU).  Beyond that, device-memory arenas from 1024 vertices upward, 64 lanes per pair,
+// host-driven.
 #define T1_G 16
 #define T1_BS 128
-#define T1_VC 36u
-#define T1_FC 72u
+#define T1_VC 44u
+#define T1_FC 144u
 #define T1_SC 32u
 #define T1_STC 32u
 #define T2_G 16
 #define T2_BS 64
 #define T2_VC 128u
-#define T2_FC 256u
+#define T2_FC 384u
 #define T2_SC 64u
 #define T2_STC 64u
 
@@ -1373,7 +1393,7 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
   b.ovf_count = ovf + NCX_OVF_T2_COUNT;
   b.ovf_list = T.t2_list();
   b.work = ovf + NCX_OVF_T2_WORK;
-  hipLaunchKernelGGL((k_pairs<T2_G, T2_BS, T2_VC, T2_FC, T2_SC, T2_STC>), dim3(ctx->num_cus * 2), dim3(T2_BS), 0,
+  hipLaunchKernelGGL((k_pairs<T2_G, T2_BS, T2_VC, T2_FC, T2_SC, T2_STC>), dim3(ctx->num_cus * 3), dim3(T2_BS), 0,
                      ctx->stream, b);
   NCX_HIP_CHECK(hipGetLastError());
   if (!T.last) T.last = new KArgs;

@@ -19,7 +19,77 @@ struct ncx_mesh {
   DevBuf pts, idx;  // uploaded mesh
   DevBuf overflow;  // spill area for traversal stacks deeper than the LDS part
   DevBuf err;       // error flag (stack exhausted)
+  DevBuf work;      // ray counter of the persistent cast
+  DevBuf qnodes;    // QNode[ntri - 1]: the LBVH with 16-bit quantised child boxes (cast kernel)
+  DevBuf qframe;    // float[6]: quantisation origin (3) and step (3)
 };
+
+// Internal node for the cast kernel, 32 bytes (two 16-byte loads per visit instead of four):
+// both child boxes as 16-bit integers on one grid over the root box,
+// coordinate = origin + q * step (a multiply and an add, not fused).  Each minimum is rounded
+// down and each maximum up, checked with that same decode, so every decoded box contains the
+// exact one and culling stays conservative; only more boxes pass.
+struct alignas(16) QNode {
+  uint16_t l[6];  // left child: min x, y, z, max x, y, z
+  uint16_t r[6];  // right child
+  uint32_t child_l, child_r;
+};
+
+__device__ __forceinline__ float qdec(uint32_t q, float o, float st) { return o + (float)q * st; }
+
+// Lowest q whose decode is <= x (minimum) or highest-needed q whose decode is >= x (maximum).
+__device__ __forceinline__ uint16_t qenc(float x, float o, float st, float inv, bool is_max) {
+  if (!(st > 0.f)) return 0;
+  float t = (x - o) * inv;
+  int q = is_max ? (int)ceilf(t) : (int)floorf(t);
+  q = q < 0 ? 0 : (q > 65535 ? 65535 : q);
+  if (is_max) {
+    while (q < 65535 && qdec((uint32_t)q, o, st) < x) ++q;
+  } else {
+    while (q > 0 && qdec((uint32_t)q, o, st) > x) --q;
+  }
+  return (uint16_t)q;
+}
+
+// Quantisation frame: origin = root box minimum, step = extent / 65535 enlarged by 2^-20 so the
+// top code decodes at or above the root maximum (the kernel below checks every box).
+__global__ void k_qframe(const LNode* __restrict__ nodes, float* __restrict__ frame) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const LNode& r = nodes[0];
+  float mn[3] = {fminf(r.lminx, r.rminx), fminf(r.lminy, r.rminy), fminf(r.lminz, r.rminz)};
+  float mx[3] = {fmaxf(r.lmaxx, r.rmaxx), fmaxf(r.lmaxy, r.rmaxy), fmaxf(r.lmaxz, r.rmaxz)};
+  for (int d = 0; d < 3; ++d) {
+    frame[d] = mn[d];
+    frame[3 + d] = (mx[d] - mn[d]) / 65535.f * (1.f + 9.5367431640625e-07f);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_quantize(const LNode* __restrict__ nodes, size_t ninternal,
+                                                  const float* __restrict__ frame, QNode* __restrict__ out,
+                                                  uint32_t* __restrict__ err) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninternal) return;
+  const LNode& a = nodes[i];
+  float o[3] = {frame[0], frame[1], frame[2]}, st[3] = {frame[3], frame[4], frame[5]};
+  float inv[3];
+  for (int d = 0; d < 3; ++d) inv[d] = st[d] > 0.f ? 1.f / st[d] : 0.f;
+  float lb[6] = {a.lminx, a.lminy, a.lminz, a.lmaxx, a.lmaxy, a.lmaxz};
+  float rb[6] = {a.rminx, a.rminy, a.rminz, a.rmaxx, a.rmaxy, a.rmaxz};
+  QNode q;
+  bool ok = true;
+  for (int k = 0; k < 6; ++k) {
+    int d = k % 3;
+    bool is_max = k >= 3;
+    q.l[k] = qenc(lb[k], o[d], st[d], inv[d], is_max);
+    q.r[k] = qenc(rb[k], o[d], st[d], inv[d], is_max);
+    float dl = qdec(q.l[k], o[d], st[d]), dr = qdec(q.r[k], o[d], st[d]);
+    ok = ok && (is_max ? (dl >= lb[k] && dr >= rb[k]) : (dl <= lb[k] && dr <= rb[k]));
+  }
+  q.child_l = a.child_l;
+  q.child_r = a.child_r;
+  out[i] = q;
+  if (!ok) atomicOr(err, 2u);
+}
 
 #define RC_BLOCK 128
 #define RC_LDS_STACK 24
@@ -202,6 +272,171 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast(const LNode* __restrict__ nod
   }
 }
 
+// Persistent closest-hit cast.  The per-ray traversal is k_cast's, one node per loop
+// iteration; what changes is the scheduling.  k_cast keeps a wavefront busy until its slowest
+// ray ends (a 64-ray wave of this workload runs ~3x the mean node count).  Here a lane that
+// finishes its ray takes the next one at once from its wavefront's chunk of RC_CHUNK ray
+// indices: each iteration the lanes without a ray are counted by a ballot and take consecutive
+// indices by their rank among them (mbcnt).  A wavefront whose chunk runs out takes the next
+// chunk from the launch's work counter with one atomicAdd, so the counter sees one atomic per
+// RC_CHUNK rays (one per ray serialises on that address and runs 8x slower).  The wave exits
+// when no lane holds a ray and the counter is past the batch.
+#define RC_CHUNK 64u
+__global__ __launch_bounds__(RC_BLOCK) void k_cast_pt(const QNode* __restrict__ nodes, const float* __restrict__ qframe,
+                                                       const float4* __restrict__ tris,
+                                                       uint32_t nleaves, uint32_t ntri_total, ncx_pose pose,
+                                                       const float* __restrict__ ox, const float* __restrict__ oy,
+                                                       const float* __restrict__ oz, const float* __restrict__ dx,
+                                                       const float* __restrict__ dy, const float* __restrict__ dz,
+                                                       uint32_t nrays, float max_toi, float* __restrict__ toi_out,
+                                                       float* __restrict__ nrm_out, uint32_t* __restrict__ feat_out,
+                                                       uint32_t* __restrict__ ovf, uint32_t* __restrict__ err,
+                                                       uint32_t* __restrict__ work) {
+  __shared__ uint32_t lstack[RC_LDS_STACK * RC_BLOCK];
+  const int tid = threadIdx.x;
+  const int lane = __lane_id();
+  const size_t gtid = (size_t)blockIdx.x * RC_BLOCK + tid;
+  uint32_t* gstack = ovf + gtid * RC_GLOBAL_STACK;
+  iso m;
+  m.t = f3{pose.tx, pose.ty, pose.tz};
+  m.q = quat{pose.qi, pose.qj, pose.qk, pose.qw};
+  uint32_t r = 0;
+  bool has_ray = false, exhausted = false;
+  f3 o = f3{0.f, 0.f, 0.f}, d = o, inv = o, best_n = o;
+  bool zx = false, zy = false, zz = false;
+  float best_t = max_toi;
+  uint32_t best_f = 0xFFFFFFFFu, node = 0, top = 0;
+  int sp = 0;
+  bool top_valid = false;
+  // The quantisation frame exists only for meshes with internal nodes (more than one triangle).
+  f3 qo = f3{0.f, 0.f, 0.f}, qs = qo;
+  if (nleaves > 1) {
+    qo = f3{qframe[0], qframe[1], qframe[2]};
+    qs = f3{qframe[3], qframe[4], qframe[5]};
+  }
+  uint32_t next = 0, end = 0;  // the wavefront's chunk [next, end) (the same in every lane)
+  bool pool_empty = false;
+  for (;;) {
+    // refill lanes without a ray
+    const bool want = !has_ray && !exhausted;
+    const uint64_t need = __ballot(want);
+    if (need) {
+      if (next >= end && !pool_empty) {
+        const int leader = __ffsll((unsigned long long)need) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(work, RC_CHUNK);
+        base = __shfl(base, leader, 64);
+        if (base >= nrays) {
+          pool_empty = true;
+        } else {
+          next = base;
+          end = base + RC_CHUNK < nrays ? base + RC_CHUNK : nrays;
+        }
+      }
+      const uint32_t avail = end - next;
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+      if (want) {
+        if (rank < avail) {
+          r = next + rank;
+          // Ray::inverse_transform_by (ray.rs:36-41)
+          o = inv_xform_pt(m, f3{ox[r], oy[r], oz[r]});
+          d = inv_xform_vec(m, f3{dx[r], dy[r], dz[r]});
+          zx = d.x == 0.f; zy = d.y == 0.f; zz = d.z == 0.f;
+          inv = f3{1.f / d.x, 1.f / d.y, 1.f / d.z};
+          best_t = max_toi;
+          best_f = 0xFFFFFFFFu;
+          best_n = f3{0.f, 0.f, 0.f};
+          node = 0;
+          sp = 0;
+          top_valid = false;
+          has_ray = true;
+        } else if (pool_empty) {
+          exhausted = true;
+        }
+      }
+      const uint32_t cnt = (uint32_t)__popcll(need);
+      next += cnt < avail ? cnt : avail;
+    }
+    if (__ballot(has_ray) == 0) break;
+    if (!has_ray) continue;
+    auto test_leaf = [&](uint32_t k) {
+      float4 A = tris[3 * k], B = tris[3 * k + 1], C = tris[3 * k + 2];
+      float t;
+      f3 nn;
+      bool back;
+      if (ray_tri(f3{A.x, A.y, A.z}, f3{B.x, B.y, B.z}, f3{C.x, C.y, C.z}, o, d, &t, &nn, &back) && t <= max_toi) {
+        uint32_t f = __float_as_uint(A.w);
+        uint32_t feat = back ? f + ntri_total : f;
+        if (t < best_t || (t == best_t && feat < best_f)) {
+          best_t = t;
+          best_f = feat;
+          best_n = nn;
+        }
+      }
+    };
+    bool done = false;
+    if (nleaves == 1) {
+      test_leaf(0);
+      done = true;
+    } else {
+      const uint4* q = (const uint4*)(nodes + node);
+      const uint4 qa = q[0], qb = q[1];
+      const uint32_t cl = qb.z, cr = qb.w;
+      float tl = slab(qdec(qa.x & 0xFFFFu, qo.x, qs.x), qdec(qa.x >> 16, qo.y, qs.y), qdec(qa.y & 0xFFFFu, qo.z, qs.z),
+                      qdec(qa.y >> 16, qo.x, qs.x), qdec(qa.z & 0xFFFFu, qo.y, qs.y), qdec(qa.z >> 16, qo.z, qs.z),
+                      o, inv, zx, zy, zz, max_toi);
+      float tr = slab(qdec(qa.w & 0xFFFFu, qo.x, qs.x), qdec(qa.w >> 16, qo.y, qs.y), qdec(qb.x & 0xFFFFu, qo.z, qs.z),
+                      qdec(qb.x >> 16, qo.x, qs.x), qdec(qb.y & 0xFFFFu, qo.y, qs.y), qdec(qb.y >> 16, qo.z, qs.z),
+                      o, inv, zx, zy, zz, max_toi);
+      bool hl = tl <= best_t, hr = tr <= best_t;
+      if (hl && (cl & NCX_LEAF_BIT)) { test_leaf(cl & ~NCX_LEAF_BIT); hl = false; }
+      if (hr && (cr & NCX_LEAF_BIT)) { test_leaf(cr & ~NCX_LEAF_BIT); hr = false; }
+      if (hl && hr) {
+        uint32_t nearc = tl <= tr ? cl : cr, farc = tl <= tr ? cr : cl;
+        // The stack's top entry stays in a register; pushing over it moves it to memory.
+        if (top_valid) {
+          if (sp < RC_LDS_STACK) lstack[sp * RC_BLOCK + tid] = top;
+          else if (sp < RC_LDS_STACK + RC_GLOBAL_STACK) gstack[sp - RC_LDS_STACK] = top;
+          else { atomicOr(err, 1u); done = true; }
+          sp++;
+        }
+        top = farc;
+        top_valid = true;
+        node = nearc;
+      } else if (hl) {
+        node = cl;
+      } else if (hr) {
+        node = cr;
+      } else if (top_valid) {
+        node = top;
+        top_valid = false;
+      } else if (sp == 0) {
+        done = true;
+      } else {
+        sp--;
+        node = sp < RC_LDS_STACK ? lstack[sp * RC_BLOCK + tid] : gstack[sp - RC_LDS_STACK];
+      }
+    }
+    if (done) {
+      if (best_f != 0xFFFFFFFFu) {
+        f3 wn = xform_vec(m, best_n);  // res.normal = m * res.normal (ray_trimesh.rs:47)
+        toi_out[r] = best_t;
+        nrm_out[3 * r] = wn.x;
+        nrm_out[3 * r + 1] = wn.y;
+        nrm_out[3 * r + 2] = wn.z;
+      } else {
+        toi_out[r] = -1.f;
+        nrm_out[3 * r] = 0.f;
+        nrm_out[3 * r + 1] = 0.f;
+        nrm_out[3 * r + 2] = 0.f;
+      }
+      feat_out[r] = best_f;
+      has_ray = false;
+    }
+  }
+}
+
 extern "C" int ncx_trimesh_create(ncx_ctx* ctx, const float* points_xyz, size_t npoints, const uint32_t* indices,
                                   size_t ntriangles, ncx_mesh** out) {
   NCX_REQUIRE(ctx && out, "ncx_trimesh_create: null argument");
@@ -214,10 +449,16 @@ extern "C" int ncx_trimesh_create(ncx_ctx* ctx, const float* points_xyz, size_t 
   m->npoints = npoints;
   m->ntri = ntriangles;
   int rc = 0;
-  auto fail = [&](int code) { m->bvh.release(); delete m; return code; };
+  auto fail = [&](int code) {
+    m->bvh.release();
+    m->tris.release(); m->boxes.release(); m->pts.release(); m->idx.release(); m->err.release(); m->work.release();
+    m->qnodes.release(); m->qframe.release();
+    delete m;
+    return code;
+  };
   if ((rc = m->pts.reserve(npoints * 12)) || (rc = m->idx.reserve(ntriangles * 12)) ||
       (rc = m->boxes.reserve(ntriangles * 24)) || (rc = m->tris.reserve(ntriangles * 48)) ||
-      (rc = m->err.reserve(4)))
+      (rc = m->err.reserve(4)) || (rc = m->work.reserve(4)))
     return fail(rc);
   if (hipMemcpyAsync(m->pts.p, points_xyz, npoints * 12, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
       hipMemcpyAsync(m->idx.p, indices, ntriangles * 12, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
@@ -234,8 +475,21 @@ extern "C" int ncx_trimesh_create(ncx_ctx* ctx, const float* points_xyz, size_t 
   if ((rc = fill_u32(ctx, m->err.as<uint32_t>(), 1, 0u))) return fail(rc);
   hipLaunchKernelGGL(k_tri_reorder, dim3(ceil_div(ntriangles, 256)), dim3(256), 0, ctx->stream, m->pts.as<float>(),
                      m->idx.as<uint32_t>(), m->bvh.order.as<uint32_t>(), ntriangles, m->tris.as<float4>());
+  if ((rc = m->qframe.reserve(64)) || (rc = fill_u32(ctx, m->qframe.as<uint32_t>(), 16, 0u))) return fail(rc);
+  if (ntriangles > 1) {
+    if ((rc = m->qnodes.reserve((ntriangles - 1) * sizeof(QNode)))) return fail(rc);
+    hipLaunchKernelGGL(k_qframe, dim3(1), dim3(64), 0, ctx->stream, m->bvh.nodes.as<LNode>(), m->qframe.as<float>());
+    hipLaunchKernelGGL(k_quantize, dim3(ceil_div(ntriangles - 1, 256)), dim3(256), 0, ctx->stream,
+                       m->bvh.nodes.as<LNode>(), ntriangles - 1, m->qframe.as<float>(), m->qnodes.as<QNode>(),
+                       m->err.as<uint32_t>());
+  }
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
     ncx_set_error("ncx_trimesh_create: build failed");
+    return fail(NCX_E_HIP);
+  }
+  uint32_t qerr = 0;
+  if (hipMemcpy(&qerr, m->err.p, 4, hipMemcpyDeviceToHost) != hipSuccess || qerr != 0) {
+    ncx_set_error("ncx_trimesh_create: a quantised node box does not contain its exact box");
     return fail(NCX_E_HIP);
   }
   *out = m;
@@ -245,7 +499,8 @@ extern "C" int ncx_trimesh_create(ncx_ctx* ctx, const float* points_xyz, size_t 
 extern "C" int ncx_trimesh_destroy(ncx_mesh* m) {
   if (!m) return NCX_OK;
   m->bvh.release();
-  m->tris.release(); m->boxes.release(); m->pts.release(); m->idx.release(); m->overflow.release(); m->err.release();
+  m->tris.release(); m->boxes.release(); m->pts.release(); m->idx.release(); m->overflow.release(); m->err.release(); m->work.release();
+  m->qnodes.release(); m->qframe.release();
   delete m;
   return NCX_OK;
 }
@@ -253,6 +508,7 @@ extern "C" int ncx_trimesh_destroy(ncx_mesh* m) {
 static int cast_impl(ncx_mesh* m, const ncx_pose* pose, const float* const r[6], size_t n, float max_toi, float* toi,
                      float* nrm, uint32_t* feat, uint32_t* sn, uint32_t* st, bool stats, bool sync) {
   NCX_REQUIRE(m && pose && r, "ncx_trimesh_cast_rays: null argument");
+  NCX_REQUIRE(n <= 0x7FFFFFFFull, "ncx_trimesh_cast_rays: more than 2^31 - 1 rays in one call");
   if (n == 0) return NCX_OK;
   for (int d = 0; d < 6; ++d) NCX_REQUIRE(r[d], "ncx_trimesh_cast_rays: null ray array");
   ncx_ctx* ctx = m->ctx;
@@ -264,10 +520,16 @@ static int cast_impl(ncx_mesh* m, const ncx_pose* pose, const float* const r[6],
     hipLaunchKernelGGL(k_cast<true>, dim3(blocks), dim3(RC_BLOCK), 0, ctx->stream, m->bvh.nodes.as<LNode>(),
                        m->tris.as<float4>(), (uint32_t)m->ntri, (uint32_t)m->ntri, *pose, r[0], r[1], r[2], r[3], r[4],
                        r[5], n, max_toi, toi, nrm, feat, sn, st, m->overflow.as<uint32_t>(), m->err.as<uint32_t>());
-  else
-    hipLaunchKernelGGL(k_cast<false>, dim3(blocks), dim3(RC_BLOCK), 0, ctx->stream, m->bvh.nodes.as<LNode>(),
-                       m->tris.as<float4>(), (uint32_t)m->ntri, (uint32_t)m->ntri, *pose, r[0], r[1], r[2], r[3], r[4],
-                       r[5], n, max_toi, toi, nrm, feat, sn, st, m->overflow.as<uint32_t>(), m->err.as<uint32_t>());
+  else {
+    // persistent grid: about as many blocks as are resident (LDS stacks allow 13 per CU)
+    unsigned pblocks = std::min<unsigned>(RC_GRID_BLOCKS, (unsigned)ctx->num_cus * 13);
+    NCX_REQUIRE(m->qframe.p && (m->ntri == 1 || m->qnodes.p), "ncx_trimesh_cast_rays: mesh nodes missing");
+    if ((rc = fill_u32(ctx, m->work.as<uint32_t>(), 1, 0u))) return rc;
+    hipLaunchKernelGGL(k_cast_pt, dim3(pblocks), dim3(RC_BLOCK), 0, ctx->stream, m->qnodes.as<QNode>(),
+                       m->qframe.as<float>(), m->tris.as<float4>(), (uint32_t)m->ntri, (uint32_t)m->ntri, *pose, r[0], r[1], r[2], r[3], r[4],
+                       r[5], (uint32_t)n, max_toi, toi, nrm, feat, m->overflow.as<uint32_t>(), m->err.as<uint32_t>(),
+                       m->work.as<uint32_t>());
+  }
   NCX_HIP_CHECK(hipGetLastError());
   if (!sync) return NCX_OK;
   return ncx_trimesh_check(m);

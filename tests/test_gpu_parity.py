@@ -320,3 +320,37 @@ def test_warm_started_distances_match_oracle(ctx, orc):
     err = np.abs(d.astype(np.float64) - rd)
     assert np.all(err <= REL * np.abs(rd) + ABS), f"max err {err.max()}"
     print(f"warm-started distance: bit-exact fraction {np.mean(d == rd):.4f}")
+
+
+def test_error_against_f64_reference(ctx, orc):
+    """The error of the f32 GPU results against an f64 instantiation of the same algorithms on the
+    same (f32-exact) inputs, reported per quantity.  The GPU equals the f32 oracle; this measures
+    what f32 itself costs.  Bounds are loose sanity bounds (GJK's own stopping rule is relative
+    1e-3 in f32), the printed figures are the report."""
+    w = workloads.hull_pairs(seed=15, npairs=3000)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    g = ncx.query.contact(ss, poses, w["ia"], w["ib"], 0.0)
+    d, _ = ncx.query.distance(ss, poses, w["ia"], w["ib"])
+    o64 = oracle_shapes(orc, w["records"], w["pool"], dtype=np.float64)
+    p64 = to_oracle_poses(w["poses"]).astype(np.float64)
+    r = orc.contact(o64, p64, w["ia"], w["ib"], 0.0, nthreads=8)
+    rd, _ = orc.distance(o64, p64, w["ia"], w["ib"], nthreads=8)
+    has = g["has_contact"].cpu().numpy()
+    both = has & r["has"]
+    agree = np.mean(has == r["has"])
+    dg = g["depth"].cpu().numpy()[both].astype(np.float64)
+    de = np.abs(dg - r["depth"][both])
+    ng = g["normal"].cpu().numpy()[both].astype(np.float64)
+    cosang = np.clip(np.sum(ng * r["normal"][both], 1) / np.linalg.norm(ng, axis=1), -1, 1)
+    ang = np.arccos(cosang)
+    dd = d.cpu().numpy().astype(np.float64)
+    sep = rd > 0
+    rel = np.abs(dd[sep] - rd[sep]) / rd[sep]
+    print(f"f32 GPU vs f64: has_contact agreement {agree:.4f}; depth |err| median {np.median(de):.2e} "
+          f"p99 {np.percentile(de, 99):.2e} max {de.max():.2e}; normal angle median {np.median(ang):.2e} "
+          f"p99 {np.percentile(ang, 99):.2e} rad; distance rel err median {np.median(rel):.2e} "
+          f"p99 {np.percentile(rel, 99):.2e} max {rel.max():.2e}")
+    assert agree > 0.99
+    assert np.percentile(de, 99) < 1e-3 and np.percentile(ang, 99) < 1e-2
+    assert np.percentile(rel, 99) < 1e-2
