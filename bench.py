@@ -17,6 +17,8 @@ import os
 import sys
 import time
 
+import ctypes as C
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -82,19 +84,24 @@ def bench_rays(args, ncx, torch, ctx):
     for _ in range(args.warmup):
         launch()
     torch.cuda.synchronize()
-    stream = torch.cuda.current_stream(dev)
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
+    # whole casts, back to back (the value): barrier + synchronize on both sides
     t0 = time.perf_counter()
-    for a, b in ev:
-        a.record(stream)
+    for _ in range(args.steps):
         launch()
-        b.record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     _lib.check(lib.ncx_trimesh_check(mesh.h))
-    kms = [a.elapsed_time(b) for a, b in ev]
     wall_ms = (t1 - t0) * 1e3 / args.steps
+    # the traversal kernel alone, from HIP events the library records around it on its stream
+    _lib.check(lib.ncx_trimesh_set_timing(mesh.h, 1))
+    kms = []
+    for _ in range(args.steps):
+        launch()
+        v = C.c_float()
+        _lib.check(lib.ncx_trimesh_kernel_ms(mesh.h, C.byref(v)))
+        kms.append(v.value)
+    _lib.check(lib.ncx_trimesh_set_timing(mesh.h, 0))
+    torch.cuda.synchronize()
     avg_kernel_s = float(np.mean(kms)) / 1e3
     hits = float((feat != -1).float().mean().item())
     return dict(n=n, wall_ms=wall_ms, kernel_ms=float(np.mean(kms)), kernel_ms_min=float(np.min(kms)),
@@ -180,6 +187,7 @@ def bench_hulls(args, ncx, torch, ctx):
     cit = g["gjk_iters"].float().mean().item()
     dits = dit.float().mean().item()
     return dict(pairs=n, contact_ms=tc * 1e3, distance_ms=td * 1e3, pairs_per_s=n / (tc + td),
+                lanes_per_pair_by_bucket={"contact": ss.tuning("contact")[0], "distance": ss.tuning("distance")[0]},
                 contact_pairs_per_s=n / tc, distance_pairs_per_s=n / td,
                 contact_fraction=g["has_contact"].float().mean().item(),
                 contact_gjk_iters=cit, distance_gjk_iters=dits, hull_vertices_per_pair=verts_per_pair,
@@ -245,31 +253,55 @@ def bench_learned(args, ncx, torch, ctx):
 
 def bench_scene(args, ncx, torch, ctx):
     """Scene step on 1,048,576 mixed shapes in 64 Gaussian clusters: shape AABBs, LBVH refit
-    (rebuild every 8 steps), pair search, contacts; each step replayed as a hipGraph."""
+    (rebuild every 8 steps), pair search, contacts; each refit step replayed as a hipGraph.
+    Then the same steps with the narrow phase off, for the broad phase's own time and roofline:
+    bytes per step = 68 N (pose + parameters in, box out) + 72 (N-1) (refit) + 64 x nodes visited
+    by one self-query traversal (both child boxes per visit) + 8 x pairs, plus on a rebuild step
+    64 N (key/value sort) + 16 (N-1) (hierarchy), spread over the 8-step cycle."""
     from ncollide_amd import workloads
     from ncollide_amd.world import World
     t = time.perf_counter()
     w = workloads.full_pipeline(seed=5, n=args.scene_shapes)
     gen_s = time.perf_counter() - t
     ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
-    poses = torch.as_tensor(w["poses"], device=ctx.device)
     vel = torch.as_tensor(w["velocities"], device=ctx.device)
-    world = World(ss, margin=w["margin"], prediction=0.0, max_pairs=args.scene_max_pairs, rebuild_every=8)
-    times, pairs, large = [], [], 0
-    for s in range(args.scene_steps):
+
+    def run(contacts, steps):
+        poses = torch.as_tensor(w["poses"], device=ctx.device)
+        world = World(ss, margin=w["margin"], prediction=0.0, max_pairs=args.scene_max_pairs, rebuild_every=8)
+        world.set_contacts(contacts)
+        times, pairs, large = [], [], 0
+        for s in range(steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            world.step(poses, use_graph=True)
+            npairs, ovf = world.sync()
+            times.append(time.perf_counter() - t0)
+            pairs.append(npairs)
+            large += ovf
+            poses[:, 0:3] += vel
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        world.step(poses, use_graph=True)
-        npairs, ovf = world.sync()
-        times.append(time.perf_counter() - t0)
-        pairs.append(npairs)
-        large += ovf
-        poses[:, 0:3] += vel
-    torch.cuda.synchronize()
-    ms = float(np.median(times[1:] if len(times) > 1 else times)) * 1e3
-    return dict(shapes=args.scene_shapes, steps=args.scene_steps, ms_per_step=ms, ms_first_step=times[0] * 1e3,
-                pairs_per_step=float(np.mean(pairs)), pairs_per_s=float(np.mean(pairs)) / (ms / 1e3),
-                epa_pairs_past_first_arena=int(large), gen_s=gen_s)
+        return world, times, pairs, large
+
+    world, times, pairs, large = run(True, args.scene_steps)
+    ms = float(np.mean(times[1:] if len(times) > 1 else times)) * 1e3
+    res = dict(shapes=args.scene_shapes, steps=args.scene_steps, ms_per_step=ms, ms_first_step=times[0] * 1e3,
+               lanes_per_pair_by_bucket=ss.tuning("contact")[0],
+               pairs_per_step=float(np.mean(pairs)), pairs_per_s=float(np.sum(pairs[1:])) / float(np.sum(times[1:])),
+               epa_pairs_past_first_arena=int(large), gen_s=gen_s)
+    del world
+    # broad phase alone: steps 1..16 timed (rebuilds at steps 8 and 16: one per 8 steps)
+    world, times, pairs, _ = run(False, 17)
+    nodes = world.query_stats()
+    n = args.scene_shapes
+    p = float(np.mean(pairs[1:]))
+    per_step = 68 * n + 72 * (n - 1) + 64 * nodes + 8 * p + (64 * n + 16 * (n - 1)) / 8
+    bp_s = float(np.mean(times[1:]))
+    res["broad_phase"] = {"ms_per_step": bp_s * 1e3, "nodes_visited_per_pass": nodes, "pairs_per_step": p,
+                          "roofline": {"bound": "hbm", "achieved": per_step / bp_s / 1e9, "peak": HBM_PEAK / 1e9,
+                                       "unit": "GB/s", "frac": per_step / bp_s / HBM_PEAK,
+                                       "alg_bytes_per_step": per_step}}
+    return res
 
 
 def bench_plain_pipeline(ncx, torch, ctx, steps=20):

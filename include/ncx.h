@@ -112,6 +112,11 @@ int ncx_shapes_create(ncx_ctx* ctx, const ncx_shape* shapes, size_t n, const flo
                       const float* hull_z, size_t n_hull_points, ncx_shapes** out);
 int ncx_shapes_destroy(ncx_shapes* s);
 int ncx_shapes_count(ncx_shapes* s, size_t* n);
+/* No reference counterpart: the autotuner's choice per vertex-count bucket (pairs by their larger
+ * hull vertex count: none, <= 32, <= 96, more) for query 0 (contact) or 1 (distance): 0, 1, 2 =
+ * 8, 16, 32 lanes per pair, -1 before the first batch large enough to tune on; sample_ms
+ * (nullable, 12 floats) the sample time of each variant per bucket. */
+int ncx_shapes_tuning(ncx_shapes* s, int query, int32_t* choice, float* sample_ms);
 
 /* World AABB of every shape at the given poses (device arrays), optionally loosened by
  * `margin` (Shape::aabb: src/bounding_volume/aabb_ball.rs:8-13, aabb_cuboid.rs:7-14,
@@ -155,6 +160,10 @@ int ncx_trimesh_cast_rays(ncx_mesh* mesh, const ncx_pose* pose, const float* con
 int ncx_trimesh_cast_rays_async(ncx_mesh* mesh, const ncx_pose* pose, const float* const ray_soa_dev[6], size_t n,
                                 float max_toi, float* toi_dev, float* normal_dev, uint32_t* feature_dev);
 int ncx_trimesh_check(ncx_mesh* mesh);
+/* No reference counterpart (measurement): record events around each cast's traversal kernel;
+ * ncx_trimesh_kernel_ms returns the last cast's kernel time once it has completed. */
+int ncx_trimesh_set_timing(ncx_mesh* mesh, int enable);
+int ncx_trimesh_kernel_ms(ncx_mesh* mesh, float* ms);
 /* Traversal statistics of the same cast (nodes visited, triangles tested per ray), for roofline accounting. */
 int ncx_trimesh_cast_stats(ncx_mesh* mesh, const ncx_pose* pose, const float* const ray_soa_dev[6], size_t n,
                            float max_toi, uint32_t* nodes_visited_dev, uint32_t* tris_tested_dev);
@@ -231,6 +240,12 @@ int ncx_world_destroy(ncx_world* world);
 int ncx_world_set_warmstart(ncx_world* world, ncx_mlp* mlp);
 int ncx_world_set_rebuild_every(ncx_world* world, int k);
 int ncx_world_step(ncx_world* world, const ncx_pose* poses_dev, int use_graph);
+/* No reference counterpart: turns each step's narrow phase on (default) or off, so the broad
+ * phase can be run and timed alone. */
+int ncx_world_set_contacts(ncx_world* world, int enable);
+/* No reference counterpart (measurement): internal nodes visited by one pair-count traversal of
+ * the current tree, the per-pass work of the step's pair search.  Synchronous. */
+int ncx_world_query_stats(ncx_world* world, unsigned long long* nodes_visited);
 /* Waits for the last step: *npairs = pair count; NCX_E_CAPACITY if it exceeded max_pairs.
  * Every contact is complete on return: pairs whose EPA polytope outgrew the step's two arena
  * tiers are recomputed here with larger arenas, so the step's poses must stay unchanged until

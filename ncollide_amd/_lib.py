@@ -37,6 +37,7 @@ SIGNATURES = {
     "ncx_shapes_create": (C.c_int, [P, P, SZ, P, P, P, SZ, P]),
     "ncx_shapes_destroy": (C.c_int, [P]),
     "ncx_shapes_count": (C.c_int, [P, P]),
+    "ncx_shapes_tuning": (C.c_int, [P, C.c_int, P, P]),
     "ncx_shapes_aabbs_dev": (C.c_int, [P, P, F32, P]),
     "ncx_contact_batch": (C.c_int, [P, P, P, P, SZ, F32, P, P, P, P, P, P, P]),
     "ncx_contact_batch_capped": (C.c_int, [P, P, P, P, SZ, F32, P, I32, P, P, P, P, P, P]),
@@ -48,6 +49,8 @@ SIGNATURES = {
     "ncx_trimesh_cast_stats": (C.c_int, [P, P, P, SZ, F32, P, P]),
     "ncx_trimesh_cast_rays_async": (C.c_int, [P, P, P, SZ, F32, P, P, P]),
     "ncx_trimesh_check": (C.c_int, [P]),
+    "ncx_trimesh_set_timing": (C.c_int, [P, C.c_int]),
+    "ncx_trimesh_kernel_ms": (C.c_int, [P, P]),
     "ncx_convex_hull": (C.c_int, [P, SZ, P, P]),
     "ncx_convex_hull_batch": (C.c_int, [P, P, P, SZ, P, P, C.c_int]),
     "ncx_mlp_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_uint64, P]),
@@ -66,6 +69,8 @@ SIGNATURES = {
     "ncx_world_set_warmstart": (C.c_int, [P, P]),
     "ncx_world_set_rebuild_every": (C.c_int, [P, C.c_int]),
     "ncx_world_step": (C.c_int, [P, P, C.c_int]),
+    "ncx_world_set_contacts": (C.c_int, [P, C.c_int]),
+    "ncx_world_query_stats": (C.c_int, [P, P]),
     "ncx_world_sync": (C.c_int, [P, P, P]),
     "ncx_world_outputs": (C.c_int, [P, P, P, P, P, P, P, P, P]),
     "ncx_world_stored_aabbs": (C.c_int, [P, P]),

@@ -72,7 +72,11 @@ struct DevBuf {
 // the first-tier arena (cap entries), then those that outgrew the second tier as well (cap
 // entries).
 constexpr uint32_t NCX_OVF_T1_COUNT = 0, NCX_OVF_BAD = 1, NCX_OVF_T2_COUNT = 2, NCX_OVF_T1_WORK = 3,
-                   NCX_OVF_T2_WORK = 4, NCX_OVF_HEADER = 8;
+                   NCX_OVF_T2_WORK = 4, NCX_BKT_CNT = 8, NCX_BKT_FILL = 12, NCX_BKT_WORK = 16, NCX_BKT_RANGE = 20,
+                   NCX_OVF_HEADER = 32;
+// Pairs are partitioned by their larger hull vertex count into NCX_NBUCKETS buckets, each run by
+// the kernel variant the autotuner picked for it.
+constexpr int NCX_NBUCKETS = 4;
 
 struct KArgs;  // contact kernel arguments (ncx_narrow.hip)
 
@@ -81,6 +85,8 @@ struct KArgs;  // contact kernel arguments (ncx_narrow.hip)
 // share them.
 struct ContactTiers {
   DevBuf ovf;
+  DevBuf blist;      // pair indices partitioned by bucket
+  size_t bcap = 0;   // entries of blist
   uint32_t cap = 0;  // entries per overflow list
   KArgs* last = nullptr;
   uint32_t* t1_list() const { return ovf.as<uint32_t>() + NCX_OVF_HEADER; }
@@ -94,6 +100,10 @@ struct ncx_shapes {
   DevBuf shapes;      // ncx_shape[n]
   DevBuf hx, hy, hz;  // hull pool
   ContactTiers tiers; // for ncx_contact_batch / ncx_distance_batch
+  // autotuner choice per query (0 contact, 1 distance) and bucket: an index into the kernel
+  // variants (group width 8, 16, 32), or -1 before tuning
+  int8_t tune[2][NCX_NBUCKETS] = {{-1, -1, -1, -1}, {-1, -1, -1, -1}};
+  float tune_ms[2][NCX_NBUCKETS][3] = {};
 };
 
 // ------------------------------------------------------------------ device math
@@ -211,13 +221,14 @@ int radix_sort_pairs64(ncx_ctx* ctx, uint32_t* hi, uint32_t* lo, size_t n, int h
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 // Tiered contact launch over pairs counted on the device (ncx_narrow.hip); no host
-// synchronisation, so it can be captured in a graph.
+// synchronisation unless may_tune (the autotuner's first run, never inside graph capture).
 int contact_tiers_reserve(ContactTiers& t, size_t cap);
 void contact_tiers_release(ContactTiers& t);
+int contact_tiers_reserve_list(ContactTiers& t, size_t npairs);
 int contacts_launch_device_count(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const ncx_pose* poses,
                                  const uint32_t* ia, const uint32_t* ib, const uint32_t* n_dev, size_t cap,
                                  float prediction, const float* init_dir, int max_iter, uint8_t* has, float* w1,
-                                 float* w2, float* nrm, float* depth, int32_t* iters);
+                                 float* w2, float* nrm, float* depth, int32_t* iters, bool may_tune);
 // Host side after that launch completed: checks, re-runs what both LDS tiers could not hold,
 // and reports the number of pairs that needed more than the first-tier arena.
 int contacts_resolve_overflow(ContactTiers& T, ncx_ctx* ctx, size_t* n_large);

@@ -1086,6 +1086,8 @@ struct KArgs {
   uint32_t* bad_index;  // set to 1 when a pair names a shape >= nshapes
   const uint32_t* n_dev;  // optional device-side pair count (capped by npairs)
   uint32_t* work;         // optional work counter (zero at launch): pairs are taken from it
+  const uint32_t* range_b;  // optional device begin and end into list: a run of buckets
+  const uint32_t* range_e;
 };
 
 // Contact/distance kernel, BS threads per block in groups of G lanes (one pair per group).
@@ -1108,6 +1110,11 @@ __global__ __launch_bounds__(BS, 2) void k_pairs(KArgs a) {
   else A = arena_at(a.arena, group, a.vcap, a.fcap, a.fcap, 2 * a.fcap);
   size_t npairs = a.npairs;
   if (a.n_dev) npairs = min(npairs, (size_t)*a.n_dev);
+  size_t lbase = 0;
+  if (a.range_b) {
+    lbase = *a.range_b;
+    npairs = *a.range_e - lbase;
+  }
   for (size_t it = 0;; ++it) {
     size_t q;
     if (a.work) {  // next pair from the work counter (lane 0 of the group takes it)
@@ -1118,7 +1125,7 @@ __global__ __launch_bounds__(BS, 2) void k_pairs(KArgs a) {
       q = group + it * ngroups;
     }
     if (q >= npairs) break;
-    size_t k = a.list ? a.list[q] : q;
+    size_t k = a.list ? a.list[lbase + q] : q;
     uint32_t i1 = a.ia[k], i2 = a.ib[k];
     if (i1 >= a.nshapes || i2 >= a.nshapes) {
       if (gl == 0) atomicOr(a.bad_index, 1u);
@@ -1255,14 +1262,107 @@ extern "C" int ncx_shapes_destroy(ncx_shapes* s) {
   return NCX_OK;
 }
 
+// The autotuner's state for one query (0 contact, 1 distance): per vertex-count bucket the
+// chosen variant (0, 1, 2 = 8, 16, 32 lanes per pair; -1 untuned) and each variant's sample time.
+extern "C" int ncx_shapes_tuning(ncx_shapes* s, int query, int32_t* choice, float* sample_ms) {
+  NCX_REQUIRE(s && choice && (query == 0 || query == 1), "ncx_shapes_tuning: bad argument");
+  for (int b = 0; b < NCX_NBUCKETS; ++b) {
+    choice[b] = s->tune[query][b];
+    if (sample_ms)
+      for (int v = 0; v < 3; ++v) sample_ms[3 * b + v] = s->tune_ms[query][b][v];
+  }
+  return NCX_OK;
+}
+
 extern "C" int ncx_shapes_count(ncx_shapes* s, size_t* n) {
   NCX_REQUIRE(s && n, "ncx_shapes_count: null argument");
   *n = s->n;
   return NCX_OK;
 }
 
-// Arena tiers.  T1: 16 lanes per pair, 8 pairs per 128-thread block, LDS arenas of 44 vertices
-// and 144 faces (4.7 KiB per pair; 4 blocks fit a CU).  T2: 16 lanes per pair, 4 pairs per
+// ------------------------------------------------------------------ vertex-count buckets
+// Bucket of a pair by the larger hull vertex count of its shapes: 0 no hull, 1 up to 32, 2 up to
+// 96, 3 more.  Pairs of one bucket have similar support-scan costs and get one kernel variant.
+__device__ __forceinline__ uint32_t pair_bucket(const ncx_shape* __restrict__ shapes, uint32_t nshapes, uint32_t i1,
+                                                uint32_t i2) {
+  if (i1 >= nshapes || i2 >= nshapes) return 0;  // the pair kernel reports the bad index
+  uint32_t c1 = shapes[i1].type == NCX_CONVEX_HULL ? shapes[i1].hull_count : 0;
+  uint32_t c2 = shapes[i2].type == NCX_CONVEX_HULL ? shapes[i2].hull_count : 0;
+  uint32_t m = c1 > c2 ? c1 : c2;
+  return m == 0 ? 0u : (m <= 32 ? 1u : (m <= 96 ? 2u : 3u));
+}
+
+#define BKT_BLOCK 256
+#define BKT_ITEMS 8
+
+// Pass 1: per-bucket counts (LDS histogram per block, one atomic per bucket per block).
+__global__ __launch_bounds__(BKT_BLOCK) void k_bucket_count(const uint32_t* __restrict__ ia,
+                                                            const uint32_t* __restrict__ ib,
+                                                            const ncx_shape* __restrict__ shapes, uint32_t nshapes,
+                                                            size_t npairs, const uint32_t* __restrict__ n_dev,
+                                                            uint32_t* __restrict__ hdr) {
+  __shared__ uint32_t h[NCX_NBUCKETS];
+  if (threadIdx.x < NCX_NBUCKETS) h[threadIdx.x] = 0;
+  __syncthreads();
+  size_t n = npairs;
+  if (n_dev) n = min(n, (size_t)*n_dev);
+  const size_t base = (size_t)blockIdx.x * BKT_BLOCK * BKT_ITEMS;
+  for (int r = 0; r < BKT_ITEMS; ++r) {
+    size_t q = base + (size_t)r * BKT_BLOCK + threadIdx.x;
+    if (q < n) atomicAdd(&h[pair_bucket(shapes, nshapes, ia[q], ib[q])], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < NCX_NBUCKETS && h[threadIdx.x]) atomicAdd(&hdr[NCX_BKT_CNT + threadIdx.x], h[threadIdx.x]);
+}
+
+// Bucket ranges [begin, end) in the partitioned list, and the scatter cursors.
+__global__ void k_bucket_offsets(uint32_t* __restrict__ hdr) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t off = 0;
+  for (int b = 0; b < NCX_NBUCKETS; ++b) {
+    hdr[NCX_BKT_RANGE + 2 * b] = off;
+    hdr[NCX_BKT_FILL + b] = off;
+    off += hdr[NCX_BKT_CNT + b];
+    hdr[NCX_BKT_RANGE + 2 * b + 1] = off;
+  }
+}
+
+// Pass 2: scatter pair indices to their bucket's range (one atomic per bucket per block for the
+// block's base, LDS atomics within the block).
+__global__ __launch_bounds__(BKT_BLOCK) void k_bucket_scatter(const uint32_t* __restrict__ ia,
+                                                              const uint32_t* __restrict__ ib,
+                                                              const ncx_shape* __restrict__ shapes, uint32_t nshapes,
+                                                              size_t npairs, const uint32_t* __restrict__ n_dev,
+                                                              uint32_t* __restrict__ hdr, uint32_t* __restrict__ list) {
+  __shared__ uint32_t h[NCX_NBUCKETS], bb[NCX_NBUCKETS];
+  if (threadIdx.x < NCX_NBUCKETS) h[threadIdx.x] = 0;
+  __syncthreads();
+  size_t n = npairs;
+  if (n_dev) n = min(n, (size_t)*n_dev);
+  const size_t base = (size_t)blockIdx.x * BKT_BLOCK * BKT_ITEMS;
+  uint32_t mine[BKT_ITEMS];
+#pragma unroll
+  for (int r = 0; r < BKT_ITEMS; ++r) {
+    size_t q = base + (size_t)r * BKT_BLOCK + threadIdx.x;
+    mine[r] = q < n ? pair_bucket(shapes, nshapes, ia[q], ib[q]) : 0xFFFFFFFFu;
+    if (mine[r] != 0xFFFFFFFFu) atomicAdd(&h[mine[r]], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < NCX_NBUCKETS) {
+    bb[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&hdr[NCX_BKT_FILL + threadIdx.x], h[threadIdx.x]) : 0u;
+    h[threadIdx.x] = 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < BKT_ITEMS; ++r) {
+    if (mine[r] == 0xFFFFFFFFu) continue;
+    size_t q = base + (size_t)r * BKT_BLOCK + threadIdx.x;
+    list[bb[mine[r]] + atomicAdd(&h[mine[r]], 1u)] = (uint32_t)q;
+  }
+}
+
+// Arena tiers.  T1: 8 pairs per block (8, 16 or 32 lanes per pair, the autotuner's choice),
+// LDS arenas of 44 vertices and 144 faces (4.7 KiB per pair; 4 blocks fit a CU).  T2: 16 lanes per pair, 4 pairs per
 // 64-thread block, LDS arenas of 128 vertices and 384 faces (12.5 KiB per pair; 3 blocks per
 // CU).  Beyond that, device-memory arenas from 1024 vertices upward, 64 lanes per pair,
 // host-driven.
@@ -1356,35 +1456,45 @@ static int rerun_overflows(ContactTiers& T, ncx_ctx* ctx, KArgs a, uint32_t n, c
   return NCX_OK;
 }
 
-// The two LDS tiers, with no host synchronisation (graph-safe): T1 runs every pair, listing those
-// whose EPA polytope outgrows its arena; T2 reruns that device-counted list, listing what
-// outgrows its arena in turn for contacts_resolve_overflow.  Both take pairs from a work
-// counter, so uneven pair costs balance across the chip.  npairs caps the pair count; n_dev
-// (optional) holds the actual count on the device.
-static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, size_t npairs,
-                        const uint32_t* n_dev) {
-  uint32_t* ovf = T.ovf.as<uint32_t>();
-  NCX_REQUIRE(ovf, "contact launch: overflow lists not allocated");
-  int rc;
-  if ((rc = fill_u32(ctx, ovf, NCX_OVF_HEADER, 0u))) return rc;
-  a.nshapes = (uint32_t)S->n;
-  a.bad_index = ovf + NCX_OVF_BAD;
-  a.arena = nullptr;
-  a.vcap = T1_VC;
-  a.fcap = T1_FC;
-  a.ovf_count = ovf + NCX_OVF_T1_COUNT;
-  a.ovf_list = T.t1_list();
-  a.ovf_cap = T.cap;
-  a.list = nullptr;
-  a.n_dev = n_dev;
-  a.npairs = npairs;
-  a.work = ovf + NCX_OVF_T1_WORK;
-  unsigned blocks = (unsigned)ctx->num_cus * 8;  // 4 blocks are resident per CU
-  if (!n_dev) blocks = std::max(1u, std::min<unsigned>(blocks, ceil_div(npairs * T1_G, T1_BS)));
-  hipLaunchKernelGGL((k_pairs<T1_G, T1_BS, T1_VC, T1_FC, T1_SC, T1_STC>), dim3(blocks), dim3(T1_BS), 0, ctx->stream,
-                     a);
+// First-tier kernel variants, the autotuner's choices: group width G = 8, 16, 32 lanes per
+// pair.  Contact variants all hold 8 pairs per block (64, 128, 256 threads) with the same
+// 44-vertex, 144-face LDS arena, so a variant changes occupancy and lanes per pair but not which
+// pairs overflow to the second tier.  Distance queries need no EPA arena and use 128-thread
+// variants without LDS.
+static const int T1_VARIANT_G[3] = {8, 16, 32};
+static const int T1_VARIANT_BS[2][3] = {{64, 128, 256}, {128, 128, 128}};
+static const unsigned T1_BLOCKS_PER_CU[2][3] = {{8, 8, 4}, {8, 8, 8}};  // twice the resident blocks
+
+static int launch_t1(int variant, bool distance, hipStream_t stream, const KArgs& a, unsigned blocks) {
+  if (distance) {
+    switch (variant) {
+      case 0: hipLaunchKernelGGL((k_pairs<8, 128, 0u, 0u, 0u, 0u>), dim3(blocks), dim3(128), 0, stream, a); break;
+      case 2: hipLaunchKernelGGL((k_pairs<32, 128, 0u, 0u, 0u, 0u>), dim3(blocks), dim3(128), 0, stream, a); break;
+      default: hipLaunchKernelGGL((k_pairs<16, 128, 0u, 0u, 0u, 0u>), dim3(blocks), dim3(128), 0, stream, a); break;
+    }
+  } else {
+    switch (variant) {
+      case 0:
+        hipLaunchKernelGGL((k_pairs<8, 64, T1_VC, T1_FC, T1_SC, T1_STC>), dim3(blocks), dim3(64), 0, stream, a);
+        break;
+      case 2:
+        hipLaunchKernelGGL((k_pairs<32, 256, T1_VC, T1_FC, T1_SC, T1_STC>), dim3(blocks), dim3(256), 0, stream, a);
+        break;
+      default:
+        hipLaunchKernelGGL((k_pairs<16, 128, T1_VC, T1_FC, T1_SC, T1_STC>), dim3(blocks), dim3(128), 0, stream, a);
+        break;
+    }
+  }
   NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
+}
+
+// T2 over the device-counted T1 overflow list.
+static KArgs t2_args(ContactTiers& T, const KArgs& a) {
+  uint32_t* ovf = T.ovf.as<uint32_t>();
   KArgs b = a;
+  b.range_b = nullptr;
+  b.range_e = nullptr;
   b.list = T.t1_list();
   b.n_dev = ovf + NCX_OVF_T1_COUNT;
   b.npairs = T.cap;
@@ -1393,11 +1503,167 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
   b.ovf_count = ovf + NCX_OVF_T2_COUNT;
   b.ovf_list = T.t2_list();
   b.work = ovf + NCX_OVF_T2_WORK;
+  return b;
+}
+
+static int launch_t2(ncx_ctx* ctx, ContactTiers& T, const KArgs& a) {
+  KArgs b = t2_args(T, a);
   hipLaunchKernelGGL((k_pairs<T2_G, T2_BS, T2_VC, T2_FC, T2_SC, T2_STC>), dim3(ctx->num_cus * 3), dim3(T2_BS), 0,
                      ctx->stream, b);
   NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
+}
+
+int contact_tiers_reserve_list(ContactTiers& t, size_t npairs) {
+  if (npairs <= t.bcap) return NCX_OK;
+  t.blist.release();
+  t.bcap = 0;
+  int rc = t.blist.reserve(npairs * 4 + 64);
+  if (rc) return rc;
+  t.bcap = npairs;
+  return NCX_OK;
+}
+
+// Autotuner: for each bucket with enough pairs and no choice yet, time every first-tier variant
+// on a sample of the bucket's pairs (second of two runs, HIP events) and keep the fastest.  The
+// sample's results are real results (the full launch recomputes them identically).  Host-driven;
+// the choice is kept in the shape set for every later call and scene step.
+#define TUNE_MIN_PAIRS 4096u
+#define TUNE_SAMPLE 32768u
+static int autotune(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const KArgs& base, int mode) {
+  uint32_t* hdr = T.ovf.as<uint32_t>();
+  uint32_t rng[2 * NCX_NBUCKETS];
+  NCX_HIP_CHECK(hipMemcpyAsync(rng, hdr + NCX_BKT_RANGE, sizeof(rng), hipMemcpyDeviceToHost, ctx->stream));
+  NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  DevBuf dr;
+  int rc;
+  if ((rc = dr.reserve(16))) return rc;
+  hipEvent_t e0, e1;
+  NCX_HIP_CHECK(hipEventCreate(&e0));
+  NCX_HIP_CHECK(hipEventCreate(&e1));
+  for (int b = 0; b < NCX_NBUCKETS; ++b) {
+    const uint32_t cnt = rng[2 * b + 1] - rng[2 * b];
+    if (S->tune[mode][b] >= 0 || cnt < TUNE_MIN_PAIRS) continue;
+    const uint32_t sr[2] = {rng[2 * b], rng[2 * b] + std::min(cnt, TUNE_SAMPLE)};
+    NCX_HIP_CHECK(hipMemcpyAsync(dr.p, sr, 8, hipMemcpyHostToDevice, ctx->stream));
+    int best = 1;
+    float best_ms = 0.f;
+    for (int v = 0; v < 3; ++v) {
+      KArgs a = base;
+      a.range_b = dr.as<uint32_t>();
+      a.range_e = dr.as<uint32_t>() + 1;
+      a.work = hdr + NCX_OVF_T1_WORK;
+      float ms = 0.f;
+      for (int rep = 0; rep < 2; ++rep) {
+        if ((rc = fill_u32(ctx, hdr, 8, 0u))) break;
+        NCX_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+        if ((rc = launch_t1(v, mode == 1, ctx->stream, a, (unsigned)ctx->num_cus * T1_BLOCKS_PER_CU[mode][v]))) break;
+        // the variant's first-tier overflows cost a second-tier run: time it too
+        if (mode == 0 && (rc = launch_t2(ctx, T, a))) break;
+        NCX_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+        NCX_HIP_CHECK(hipEventSynchronize(e1));
+        NCX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      }
+      if (rc) break;
+      S->tune_ms[mode][b][v] = ms;
+      if (v == 0 || ms < best_ms) { best = v; best_ms = ms; }
+    }
+    if (rc) break;
+    S->tune[mode][b] = (int8_t)best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dr.release();
+  if (rc) return rc;
+  // the samples left entries in the overflow lists and moved the work counter: start clean
+  return fill_u32(ctx, hdr, 8, 0u);
+}
+
+// The launch: partition the pairs into vertex-count buckets (device-side counts, so it is
+// graph-safe), run each bucket with its variant (T1, LDS arenas), then T2 reruns the
+// device-counted T1 overflow list with 128-vertex LDS arenas, listing what outgrows that in turn
+// for contacts_resolve_overflow.  Every launch takes pairs from a work counter, so uneven pair
+// costs balance across the chip.  npairs caps the pair count; n_dev (optional) holds the actual
+// count on the device.  may_tune: run the autotuner first if a bucket has no choice yet (host
+// synchronisation; never inside graph capture).
+static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, size_t npairs,
+                        const uint32_t* n_dev, bool may_tune) {
+  uint32_t* ovf = T.ovf.as<uint32_t>();
+  NCX_REQUIRE(ovf, "contact launch: overflow lists not allocated");
+  NCX_REQUIRE(npairs < ((size_t)1 << 31), "contact launch: more than 2^31 pairs");
+  int rc;
+  if ((rc = fill_u32(ctx, ovf, NCX_OVF_HEADER, 0u))) return rc;
+  const int mode = a.distance_mode ? 1 : 0;
+  a.nshapes = (uint32_t)S->n;
+  a.bad_index = ovf + NCX_OVF_BAD;
+  a.arena = nullptr;
+  a.vcap = T1_VC;
+  a.fcap = T1_FC;
+  a.ovf_count = ovf + NCX_OVF_T1_COUNT;
+  a.ovf_list = T.t1_list();
+  a.ovf_cap = T.cap;
+  a.range_b = nullptr;
+  a.range_e = nullptr;
+  bool untuned = false;
+  for (int b = 0; b < NCX_NBUCKETS; ++b) untuned = untuned || S->tune[mode][b] < 0;
+  const bool tune_now = may_tune && untuned;
+  bool same = true;  // one variant for every bucket: no partition needed
+  for (int b = 1; b < NCX_NBUCKETS; ++b) same = same && S->tune[mode][b] == S->tune[mode][0];
+  if (tune_now || !same) {
+    // partition the pair indices by vertex-count bucket
+    if ((rc = contact_tiers_reserve_list(T, npairs))) return rc;
+    unsigned bblocks = std::max(1u, ceil_div(npairs, (size_t)BKT_BLOCK * BKT_ITEMS));
+    hipLaunchKernelGGL(k_bucket_count, dim3(bblocks), dim3(BKT_BLOCK), 0, ctx->stream, a.ia, a.ib, a.shapes,
+                       a.nshapes, npairs, n_dev, ovf);
+    NCX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_bucket_offsets, dim3(1), dim3(1), 0, ctx->stream, ovf);
+    NCX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(bblocks), dim3(BKT_BLOCK), 0, ctx->stream, a.ia, a.ib, a.shapes,
+                       a.nshapes, npairs, n_dev, ovf, T.blist.as<uint32_t>());
+    NCX_HIP_CHECK(hipGetLastError());
+    KArgs t = a;
+    t.list = T.blist.as<uint32_t>();
+    t.n_dev = nullptr;
+    t.npairs = npairs;
+    if (tune_now && (rc = autotune(S, T, ctx, t, mode))) return rc;
+  }
+  same = true;
+  for (int b = 1; b < NCX_NBUCKETS; ++b) same = same && S->tune[mode][b] == S->tune[mode][0];
+  if (same) {
+    // one launch over all pairs, in their own order
+    const int v = S->tune[mode][0] >= 0 ? S->tune[mode][0] : 1;
+    KArgs ab = a;
+    ab.list = nullptr;
+    ab.n_dev = n_dev;
+    ab.npairs = npairs;
+    ab.work = ovf + NCX_BKT_WORK;
+    unsigned blocks = (unsigned)ctx->num_cus * T1_BLOCKS_PER_CU[mode][v];
+    if (!n_dev)
+      blocks = std::max(1u, std::min<unsigned>(blocks, ceil_div(npairs * T1_VARIANT_G[v], T1_VARIANT_BS[mode][v])));
+    if ((rc = launch_t1(v, mode == 1, ctx->stream, ab, blocks))) return rc;
+  } else {
+    // one launch per run of consecutive buckets sharing a variant, over the partitioned list
+    for (int b0 = 0; b0 < NCX_NBUCKETS;) {
+      const int v = S->tune[mode][b0] >= 0 ? S->tune[mode][b0] : 1;
+      int b1 = b0;
+      while (b1 + 1 < NCX_NBUCKETS && (S->tune[mode][b1 + 1] >= 0 ? S->tune[mode][b1 + 1] : 1) == v) ++b1;
+      KArgs ab = a;
+      ab.list = T.blist.as<uint32_t>();
+      ab.n_dev = nullptr;
+      ab.npairs = npairs;
+      ab.range_b = ovf + NCX_BKT_RANGE + 2 * b0;
+      ab.range_e = ovf + NCX_BKT_RANGE + 2 * b1 + 1;
+      ab.work = ovf + NCX_BKT_WORK + b0;
+      unsigned blocks = (unsigned)ctx->num_cus * T1_BLOCKS_PER_CU[mode][v];
+      if (!n_dev)
+        blocks = std::max(1u, std::min<unsigned>(blocks, ceil_div(npairs * T1_VARIANT_G[v], T1_VARIANT_BS[mode][v])));
+      if ((rc = launch_t1(v, mode == 1, ctx->stream, ab, blocks))) return rc;
+      b0 = b1 + 1;
+    }
+  }
+  if (mode == 0 && (rc = launch_t2(ctx, T, a))) return rc;
   if (!T.last) T.last = new KArgs;
-  *T.last = b;
+  *T.last = t2_args(T, a);
   return NCX_OK;
 }
 
@@ -1406,7 +1672,7 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
 int contacts_launch_device_count(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const ncx_pose* poses,
                                  const uint32_t* ia, const uint32_t* ib, const uint32_t* n_dev, size_t cap,
                                  float prediction, const float* init_dir, int max_iter, uint8_t* has, float* w1,
-                                 float* w2, float* nrm, float* depth, int32_t* iters) {
+                                 float* w2, float* nrm, float* depth, int32_t* iters, bool may_tune) {
   KArgs a = {};
   a.shapes = S->shapes.as<ncx_shape>();
   a.P = Pool{S->hx.as<float>(), S->hy.as<float>(), S->hz.as<float>()};
@@ -1418,7 +1684,7 @@ int contacts_launch_device_count(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, c
   a.max_iter = max_iter;
   a.has = has; a.w1 = w1; a.w2 = w2; a.nrm = nrm; a.depth = depth; a.iters = iters;
   a.distance_mode = 0;
-  return launch_tiers(S, T, ctx, a, cap, n_dev);
+  return launch_tiers(S, T, ctx, a, cap, n_dev, may_tune);
 }
 
 // After the last launch_tiers on T has completed on ctx's stream: fails on a bad pair index or on
@@ -1443,7 +1709,7 @@ static int run_pairs(ncx_shapes* S, KArgs a, size_t npairs) {
   if (npairs == 0) return NCX_OK;
   int rc;
   if ((rc = contact_tiers_reserve(S->tiers, npairs))) return rc;
-  if ((rc = launch_tiers(S, S->tiers, S->ctx, a, npairs, nullptr))) return rc;
+  if ((rc = launch_tiers(S, S->tiers, S->ctx, a, npairs, nullptr, true))) return rc;
   return contacts_resolve_overflow(S->tiers, S->ctx, nullptr);
 }
 

@@ -22,6 +22,8 @@ struct ncx_mesh {
   DevBuf work;      // ray counter of the persistent cast
   DevBuf qnodes;    // QNode[ntri - 1]: the LBVH with 16-bit quantised child boxes (cast kernel)
   DevBuf qframe;    // float[6]: quantisation origin (3) and step (3)
+  bool timing = false;              // record events around the cast kernel (ncx_trimesh_set_timing)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 // Internal node for the cast kernel, 32 bytes (two 16-byte loads per visit instead of four):
@@ -501,6 +503,8 @@ extern "C" int ncx_trimesh_destroy(ncx_mesh* m) {
   m->bvh.release();
   m->tris.release(); m->boxes.release(); m->pts.release(); m->idx.release(); m->overflow.release(); m->err.release(); m->work.release();
   m->qnodes.release(); m->qframe.release();
+  if (m->ev0) (void)hipEventDestroy(m->ev0);
+  if (m->ev1) (void)hipEventDestroy(m->ev1);
   delete m;
   return NCX_OK;
 }
@@ -525,14 +529,36 @@ static int cast_impl(ncx_mesh* m, const ncx_pose* pose, const float* const r[6],
     unsigned pblocks = std::min<unsigned>(RC_GRID_BLOCKS, (unsigned)ctx->num_cus * 13);
     NCX_REQUIRE(m->qframe.p && (m->ntri == 1 || m->qnodes.p), "ncx_trimesh_cast_rays: mesh nodes missing");
     if ((rc = fill_u32(ctx, m->work.as<uint32_t>(), 1, 0u))) return rc;
+    if (m->timing) NCX_HIP_CHECK(hipEventRecord(m->ev0, ctx->stream));
     hipLaunchKernelGGL(k_cast_pt, dim3(pblocks), dim3(RC_BLOCK), 0, ctx->stream, m->qnodes.as<QNode>(),
                        m->qframe.as<float>(), m->tris.as<float4>(), (uint32_t)m->ntri, (uint32_t)m->ntri, *pose, r[0], r[1], r[2], r[3], r[4],
                        r[5], (uint32_t)n, max_toi, toi, nrm, feat, m->overflow.as<uint32_t>(), m->err.as<uint32_t>(),
                        m->work.as<uint32_t>());
+    NCX_HIP_CHECK(hipGetLastError());
+    if (m->timing) NCX_HIP_CHECK(hipEventRecord(m->ev1, ctx->stream));
   }
   NCX_HIP_CHECK(hipGetLastError());
   if (!sync) return NCX_OK;
   return ncx_trimesh_check(m);
+}
+
+// Measurement: events around each cast's traversal kernel (k_cast_pt), read back with
+// ncx_trimesh_kernel_ms after the cast has completed.
+extern "C" int ncx_trimesh_set_timing(ncx_mesh* m, int enable) {
+  NCX_REQUIRE(m, "ncx_trimesh_set_timing: null argument");
+  if (enable && !m->ev0) {
+    NCX_HIP_CHECK(hipEventCreate(&m->ev0));
+    NCX_HIP_CHECK(hipEventCreate(&m->ev1));
+  }
+  m->timing = enable != 0;
+  return NCX_OK;
+}
+
+extern "C" int ncx_trimesh_kernel_ms(ncx_mesh* m, float* ms) {
+  NCX_REQUIRE(m && ms && m->ev0, "ncx_trimesh_kernel_ms: timing not enabled");
+  NCX_HIP_CHECK(hipEventSynchronize(m->ev1));
+  NCX_HIP_CHECK(hipEventElapsedTime(ms, m->ev0, m->ev1));
+  return NCX_OK;
 }
 
 // The traversal-stack error flag is sticky: it is cleared at mesh creation and checked here.

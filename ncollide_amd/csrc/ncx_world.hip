@@ -15,7 +15,6 @@
 #include "ncx_internal.h"
 
 #define WQ_BLOCK 128
-#define WQ_STACK 48
 
 struct ncx_mlp;
 int mlp_predict_device_count(ncx_mlp* m, ncx_ctx* ctx, const ncx_shape* shapes, const ncx_pose* poses,
@@ -35,8 +34,9 @@ struct ncx_world {
   bool first = true;
   DevBuf box_new, stored;
   LBVH bvh;
-  uint32_t depth = 0;
-  DevBuf gstack;
+  uint32_t depth = 0, spill = 1;
+  DevBuf gstack;  // search-stack spill area, `spill` entries per leaf
+  DevBuf lbuf;    // the count pass's first WQ_LOCAL partners per leaf
   DevBuf cnt, offs, counters;  // counters: [0] pair total, [1] pair overflow
   DevBuf pa, pb;
   DevBuf has, w1, w2, nrm, depth_out, iters;
@@ -49,6 +49,8 @@ struct ncx_world {
   const ncx_pose* graph_poses = nullptr;
   bool graph_valid = false;
   int last_was_graph = 0;
+  bool contacts = true;        // run the narrow phase in each step
+  bool last_had_contacts = false;
 };
 
 __global__ void k_world_stage(const float* __restrict__ nb, float* __restrict__ st, size_t n, float margin,
@@ -80,28 +82,40 @@ __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float 
   return mnx <= q[3] && mny <= q[4] && mnz <= q[5] && mxx >= q[0] && mxy >= q[1] && mxz >= q[2];
 }
 
-// Self-query of sorted leaf k against leaves at later positions.  WRITE: emit into offs[k]...
-template <bool WRITE, bool DEEP>
+// Self-query of sorted leaf k against leaves at later positions (each pair is found once, by its
+// earlier leaf).  Depth-first with the stack's top in a register, the next WQ_LSTACK entries in
+// LDS and the rest in a per-thread spill area of `spill` entries (the tree depth bounds it).
+//   MODE_COUNT: count the pairs and keep the first WQ_LOCAL partners in lbuf;
+//   MODE_WRITE: write every pair at offs[k] (only for leaves with more than WQ_LOCAL pairs).
+#define WQ_LSTACK 16
+#define WQ_LOCAL 16
+#define MODE_COUNT 0
+#define MODE_WRITE 1
+
+template <int MODE>
 __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restrict__ nodes,
                                                            const uint32_t* __restrict__ order,
                                                            const float* __restrict__ boxes, uint32_t n,
                                                            uint32_t* __restrict__ cnt, const uint32_t* __restrict__ offs,
                                                            size_t cap, uint32_t* __restrict__ pa,
                                                            uint32_t* __restrict__ pb, uint32_t* __restrict__ counters,
-                                                           uint32_t* __restrict__ gstack, uint32_t depth) {
-  __shared__ uint32_t lstack[DEEP ? 1 : WQ_STACK * WQ_BLOCK];
+                                                           uint32_t* __restrict__ gstack, uint32_t spill,
+                                                           uint32_t* __restrict__ lbuf,
+                                                           unsigned long long* __restrict__ visits) {
+  __shared__ uint32_t lstack[WQ_LSTACK * WQ_BLOCK];
   const int tid = threadIdx.x;
   uint32_t k = blockIdx.x * WQ_BLOCK + tid;
   if (k >= n) return;
-  uint32_t* gs = gstack + (size_t)k * depth;
+  if (MODE == MODE_WRITE && cnt[k] <= WQ_LOCAL) return;
+  uint32_t* gs = gstack + (size_t)k * spill;
   uint32_t prim = order[k];
   float q[6];
 #pragma unroll
   for (int d = 0; d < 6; ++d) q[d] = boxes[(size_t)d * n + prim];
   uint32_t c = 0;
-  size_t w = WRITE ? offs[k] : 0;
+  size_t w = MODE == MODE_WRITE ? offs[k] : 0;
   auto emit = [&](uint32_t leafpos) {
-    if (WRITE) {
+    if (MODE == MODE_WRITE) {
       if (w < cap) {
         uint32_t h2 = order[leafpos];
         pa[w] = prim < h2 ? prim : h2;
@@ -111,12 +125,16 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
       }
       w++;
     } else {
+      if (c < WQ_LOCAL) lbuf[(size_t)k * WQ_LOCAL + c] = leafpos;
       c++;
     }
   };
   int sp = 0;
-  uint32_t node = 0;
+  uint32_t node = 0, top = 0;
+  bool top_valid = false;
+  unsigned long long nvis = 0;
   for (;;) {
+    nvis++;
     const float4* nq = (const float4*)(nodes + node);
     float4 q0 = nq[0], q1 = nq[1], q2 = nq[2], q3 = nq[3];
     uint32_t cl = __float_as_uint(q3.x), cr = __float_as_uint(q3.y);
@@ -126,19 +144,55 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
     if (hl && (cl & NCX_LEAF_BIT)) { emit(cl & ~NCX_LEAF_BIT); hl = false; }
     if (hr && (cr & NCX_LEAF_BIT)) { emit(cr & ~NCX_LEAF_BIT); hr = false; }
     if (hl && hr) {
-      if (DEEP) gs[sp] = cr;
-      else lstack[sp * WQ_BLOCK + tid] = cr;
-      sp++;
+      if (top_valid) {
+        if (sp < WQ_LSTACK) lstack[sp * WQ_BLOCK + tid] = top;
+        else gs[sp - WQ_LSTACK] = top;
+        sp++;
+      }
+      top = cr;
+      top_valid = true;
       node = cl;
-    } else if (hl) node = cl;
-    else if (hr) node = cr;
-    else {
-      if (sp == 0) break;
+    } else if (hl) {
+      node = cl;
+    } else if (hr) {
+      node = cr;
+    } else if (top_valid) {
+      node = top;
+      top_valid = false;
+    } else if (sp == 0) {
+      break;
+    } else {
       sp--;
-      node = DEEP ? gs[sp] : lstack[sp * WQ_BLOCK + tid];
+      node = sp < WQ_LSTACK ? lstack[sp * WQ_BLOCK + tid] : gs[sp - WQ_LSTACK];
     }
   }
-  if (!WRITE) cnt[k] = c;
+  if (MODE == MODE_COUNT) cnt[k] = c;
+  if (visits) atomicAdd(visits, nvis);  // internal nodes visited (measurement only)
+}
+
+// Pairs of the leaves with at most WQ_LOCAL pairs, copied from the count pass's buffer in the
+// order they were found (the write pass handles the others), so the list is the same as one
+// full write traversal per leaf would give.
+__global__ __launch_bounds__(256) void k_world_emit(const uint32_t* __restrict__ order, uint32_t n,
+                                                    const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ offs,
+                                                    const uint32_t* __restrict__ lbuf, size_t cap,
+                                                    uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
+                                                    uint32_t* __restrict__ counters) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t c = cnt[k];
+  if (c == 0 || c > WQ_LOCAL) return;
+  const uint32_t prim = order[k];
+  size_t w = offs[k];
+  for (uint32_t i = 0; i < c; ++i, ++w) {
+    if (w < cap) {
+      uint32_t h2 = order[lbuf[(size_t)k * WQ_LOCAL + i]];
+      pa[w] = prim < h2 ? prim : h2;
+      pb[w] = prim < h2 ? h2 : prim;
+    } else {
+      counters[1] = 1u;
+    }
+  }
 }
 
 __global__ void k_world_tree_depth(const uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ out) {
@@ -166,7 +220,9 @@ static int world_build(ncx_world* w) {
     NCX_HIP_CHECK(hipMemcpyAsync(&depth, w->counters.as<uint32_t>() + 2, 4, hipMemcpyDeviceToHost, c->stream));
     NCX_HIP_CHECK(hipStreamSynchronize(c->stream));
     w->depth = depth;
-    if (depth >= WQ_STACK && (rc = w->gstack.reserve(w->n * (size_t)(depth + 1) * 4))) return rc;
+    // the search stack holds at most `depth` entries: one in a register, WQ_LSTACK in LDS
+    w->spill = depth + 1 > WQ_LSTACK ? depth + 1 - WQ_LSTACK : 1;
+    if ((rc = w->gstack.reserve(w->n * (size_t)w->spill * 4))) return rc;
   }
   w->since_build = 0;
   return NCX_OK;
@@ -191,41 +247,35 @@ static int world_enqueue(ncx_world* w, const ncx_pose* poses, bool refit) {
   return NCX_OK;
 }
 
-static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses) {
+static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses, bool eager) {
   ncx_ctx* c = &w->wctx;
   size_t n = w->n;
   int rc;
   uint32_t* ctr = w->counters.as<uint32_t>();
   if ((rc = fill_u32(c, ctr, 2, 0u))) return rc;
   if (n >= 2) {
-    bool deep = w->depth >= WQ_STACK;
     unsigned nb = ceil_div(n, WQ_BLOCK);
     const LNode* nodes = w->bvh.nodes.as<LNode>();
     const uint32_t* order = w->bvh.order.as<uint32_t>();
     const float* st = w->stored.as<float>();
-    uint32_t* gs = deep ? w->gstack.as<uint32_t>() : nullptr;
-    uint32_t dd = w->depth + 1;
-    if (deep)
-      hipLaunchKernelGGL((k_world_query<false, true>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
-                         (uint32_t)n, w->cnt.as<uint32_t>(), (const uint32_t*)nullptr, w->cap, (uint32_t*)nullptr,
-                         (uint32_t*)nullptr, ctr, gs, dd);
-    else
-      hipLaunchKernelGGL((k_world_query<false, false>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
-                         (uint32_t)n, w->cnt.as<uint32_t>(), (const uint32_t*)nullptr, w->cap, (uint32_t*)nullptr,
-                         (uint32_t*)nullptr, ctr, gs, dd);
+    hipLaunchKernelGGL((k_world_query<MODE_COUNT>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
+                       (uint32_t)n, w->cnt.as<uint32_t>(), (const uint32_t*)nullptr, w->cap, (uint32_t*)nullptr,
+                       (uint32_t*)nullptr, ctr, w->gstack.as<uint32_t>(), w->spill, w->lbuf.as<uint32_t>(),
+                       (unsigned long long*)nullptr);
     NCX_HIP_CHECK(hipGetLastError());
     if ((rc = scan_exclusive_u32(c, w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), n, w->bvh.scan_tmp, ctr)))
       return rc;
-    if (deep)
-      hipLaunchKernelGGL((k_world_query<true, true>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
-                         (uint32_t)n, (uint32_t*)nullptr, w->offs.as<uint32_t>(), w->cap, w->pa.as<uint32_t>(),
-                         w->pb.as<uint32_t>(), ctr, gs, dd);
-    else
-      hipLaunchKernelGGL((k_world_query<true, false>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
-                         (uint32_t)n, (uint32_t*)nullptr, w->offs.as<uint32_t>(), w->cap, w->pa.as<uint32_t>(),
-                         w->pb.as<uint32_t>(), ctr, gs, dd);
+    hipLaunchKernelGGL(k_world_emit, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, order, (uint32_t)n,
+                       w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), w->lbuf.as<uint32_t>(), w->cap,
+                       w->pa.as<uint32_t>(), w->pb.as<uint32_t>(), ctr);
+    NCX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL((k_world_query<MODE_WRITE>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
+                       (uint32_t)n, w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), w->cap, w->pa.as<uint32_t>(),
+                       w->pb.as<uint32_t>(), ctr, w->gstack.as<uint32_t>(), w->spill, w->lbuf.as<uint32_t>(),
+                       (unsigned long long*)nullptr);
     NCX_HIP_CHECK(hipGetLastError());
   }
+  if (!w->contacts) return NCX_OK;
   const float* idir = nullptr;
   if (w->mlp) {
     if ((rc = mlp_predict_device_count(w->mlp, c, w->shapes->shapes.as<ncx_shape>(), poses, w->desc.as<float>(),
@@ -237,7 +287,7 @@ static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses) {
   return contacts_launch_device_count(w->shapes, w->tiers, c, poses, w->pa.as<uint32_t>(), w->pb.as<uint32_t>(), ctr, w->cap,
                                       w->prediction, idir, w->mlp ? 32 : 10000, w->has.as<uint8_t>(),
                                       w->w1.as<float>(), w->w2.as<float>(), w->nrm.as<float>(),
-                                      w->depth_out.as<float>(), w->iters.as<int32_t>());
+                                      w->depth_out.as<float>(), w->iters.as<int32_t>(), eager);
 }
 
 extern "C" int ncx_world_create(ncx_ctx* ctx, ncx_shapes* shapes, float margin, float prediction, size_t max_pairs,
@@ -264,6 +314,7 @@ extern "C" int ncx_world_create(ncx_ctx* ctx, ncx_shapes* shapes, float margin, 
   }
   if ((rc = w->box_new.reserve(n * 24 + 64)) || (rc = w->stored.reserve(n * 24 + 64)) ||
       (rc = w->cnt.reserve(n * 4 + 64)) || (rc = w->offs.reserve(n * 4 + 64)) || (rc = w->counters.reserve(64)) ||
+      (rc = w->lbuf.reserve(n * (size_t)WQ_LOCAL * 4 + 64)) ||
       (rc = w->pa.reserve(cap * 4)) || (rc = w->pb.reserve(cap * 4)) || (rc = w->has.reserve(cap)) ||
       (rc = w->w1.reserve(cap * 12)) || (rc = w->w2.reserve(cap * 12)) || (rc = w->nrm.reserve(cap * 12)) ||
       (rc = w->depth_out.reserve(cap * 4)) || (rc = w->iters.reserve(cap * 4)) || (rc = contact_tiers_reserve(w->tiers, std::min<size_t>(cap, (size_t)1 << 22)))) {
@@ -294,6 +345,39 @@ extern "C" int ncx_world_set_rebuild_every(ncx_world* w, int k) {
   return NCX_OK;
 }
 
+// Turns the narrow phase of each step on or off (off: the step ends with the pair list).
+extern "C" int ncx_world_set_contacts(ncx_world* w, int enable) {
+  NCX_REQUIRE(w, "ncx_world_set_contacts: null argument");
+  if (w->contacts != (enable != 0)) w->graph_valid = false;
+  w->contacts = enable != 0;
+  return NCX_OK;
+}
+
+// Measurement: internal nodes visited by one pair-count traversal of the current tree and
+// stored boxes (the work of each of the step's two query passes).  Synchronous.
+extern "C" int ncx_world_query_stats(ncx_world* w, unsigned long long* nodes_visited) {
+  NCX_REQUIRE(w && nodes_visited, "ncx_world_query_stats: null argument");
+  NCX_REQUIRE(!w->first, "ncx_world_query_stats: no step has run yet");
+  *nodes_visited = 0;
+  if (w->n < 2) return NCX_OK;
+  ncx_ctx* c = &w->wctx;
+  DevBuf v;
+  int rc;
+  if ((rc = v.reserve(8))) return rc;
+  if ((rc = fill_u32(c, v.as<uint32_t>(), 2, 0u))) return rc;
+  unsigned nb = ceil_div(w->n, WQ_BLOCK);
+  hipLaunchKernelGGL((k_world_query<MODE_COUNT>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, w->bvh.nodes.as<LNode>(),
+                     w->bvh.order.as<uint32_t>(), w->stored.as<float>(), (uint32_t)w->n, w->cnt.as<uint32_t>(),
+                     (const uint32_t*)nullptr, w->cap, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                     w->counters.as<uint32_t>(), w->gstack.as<uint32_t>(), w->spill, w->lbuf.as<uint32_t>(),
+                     v.as<unsigned long long>());
+  NCX_HIP_CHECK(hipGetLastError());
+  NCX_HIP_CHECK(hipMemcpyAsync(nodes_visited, v.p, 8, hipMemcpyDeviceToHost, c->stream));
+  NCX_HIP_CHECK(hipStreamSynchronize(c->stream));
+  v.release();
+  return NCX_OK;
+}
+
 extern "C" int ncx_world_destroy(ncx_world* w) {
   if (!w) return NCX_OK;
   if (w->exec) (void)hipGraphExecDestroy(w->exec);
@@ -304,7 +388,7 @@ extern "C" int ncx_world_destroy(ncx_world* w) {
   }
   if (w->ev_in) (void)hipEventDestroy(w->ev_in);
   if (w->ev_out) (void)hipEventDestroy(w->ev_out);
-  DevBuf* bufs[] = {&w->box_new, &w->stored, &w->gstack, &w->cnt, &w->offs, &w->counters, &w->pa, &w->pb,
+  DevBuf* bufs[] = {&w->box_new, &w->stored, &w->gstack, &w->lbuf, &w->cnt, &w->offs, &w->counters, &w->pa, &w->pb,
                     &w->has, &w->w1, &w->w2, &w->nrm, &w->depth_out, &w->iters, &w->desc, &w->dirs};
   for (DevBuf* b : bufs) b->release();
   w->bvh.release();
@@ -327,7 +411,7 @@ extern "C" int ncx_world_step(ncx_world* w, const ncx_pose* poses_dev, int use_g
   if (rebuild) {
     if ((rc = world_enqueue(w, poses_dev, false))) return rc;
     if ((rc = world_build(w))) return rc;
-    if ((rc = world_pairs_and_contacts(w, poses_dev))) return rc;
+    if ((rc = world_pairs_and_contacts(w, poses_dev, true))) return rc;
     w->graph_valid = false;
   } else if (use_graph) {
     if (!w->graph_valid || w->graph_poses != poses_dev) {
@@ -335,7 +419,7 @@ extern "C" int ncx_world_step(ncx_world* w, const ncx_pose* poses_dev, int use_g
       if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
       NCX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       int r1 = world_enqueue(w, poses_dev, true);
-      int r2 = r1 ? r1 : world_pairs_and_contacts(w, poses_dev);
+      int r2 = r1 ? r1 : world_pairs_and_contacts(w, poses_dev, false);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(s, &g);
       if (r2) { if (g) (void)hipGraphDestroy(g); return r2; }
@@ -349,10 +433,11 @@ extern "C" int ncx_world_step(ncx_world* w, const ncx_pose* poses_dev, int use_g
     w->last_was_graph = 1;
   } else {
     if ((rc = world_enqueue(w, poses_dev, true))) return rc;
-    if ((rc = world_pairs_and_contacts(w, poses_dev))) return rc;
+    if ((rc = world_pairs_and_contacts(w, poses_dev, true))) return rc;
   }
   w->first = false;
   w->since_build++;
+  w->last_had_contacts = w->contacts;
   NCX_HIP_CHECK(hipEventRecord(w->ev_out, s));
   NCX_HIP_CHECK(hipStreamWaitEvent(w->ctx->stream, w->ev_out, 0));
   return NCX_OK;
@@ -373,8 +458,10 @@ extern "C" int ncx_world_sync(ncx_world* w, size_t* npairs, size_t* n_epa_overfl
     return NCX_E_CAPACITY;
   }
   size_t large = 0;
-  int rc = contacts_resolve_overflow(w->tiers, &w->wctx, &large);
-  if (rc) return rc;
+  if (w->last_had_contacts) {
+    int rc = contacts_resolve_overflow(w->tiers, &w->wctx, &large);
+    if (rc) return rc;
+  }
   if (n_epa_overflow) *n_epa_overflow = large;
   return NCX_OK;
 }

@@ -109,6 +109,15 @@ class ShapeSet:
     def types(self):
         return self.records["type"]
 
+    def tuning(self, query="contact"):
+        """The autotuner's choice per vertex-count bucket (no hull, <= 32, <= 96, more vertices):
+        lanes per pair (8, 16, 32) or None before tuning, and each variant's sample time (ms)."""
+        choice = (C.c_int32 * 4)()
+        ms = (C.c_float * 12)()
+        _lib.check(self.ctx.lib.ncx_shapes_tuning(self.h, 0 if query == "contact" else 1, choice, ms))
+        widths = (8, 16, 32)
+        return ([widths[c] if c >= 0 else None for c in choice], [list(ms[3 * b:3 * b + 3]) for b in range(4)])
+
     def params(self):
         return np.stack([self.records["p0"], self.records["p1"], self.records["p2"]], 1)
 

@@ -26,6 +26,16 @@ class World:
         self._mlp = mlp
         _lib.check(self.ctx.lib.ncx_world_set_warmstart(self.h, mlp.h if mlp is not None else None))
 
+    def set_contacts(self, enable):
+        """Runs each step's narrow phase (default) or stops the step at the pair list."""
+        _lib.check(self.ctx.lib.ncx_world_set_contacts(self.h, 1 if enable else 0))
+
+    def query_stats(self):
+        """Internal nodes visited by one pair-count traversal of the current tree (measurement)."""
+        v = C.c_ulonglong()
+        _lib.check(self.ctx.lib.ncx_world_query_stats(self.h, C.byref(v)))
+        return v.value
+
     def step(self, poses, use_graph=True):
         """poses: (n, 8) float32 CUDA tensor (keep the same tensor across steps to replay the graph)."""
         _lib.check(self.ctx.lib.ncx_world_step(self.h, _lib.ptr(poses), 1 if use_graph else 0))

@@ -354,3 +354,28 @@ def test_error_against_f64_reference(ctx, orc):
     assert agree > 0.99
     assert np.percentile(de, 99) < 1e-3 and np.percentile(ang, 99) < 1e-2
     assert np.percentile(rel, 99) < 1e-2
+
+
+def test_raycast_grazing_rays_split_into_segments(ctx, orc):
+    """Nearly horizontal rays over a flat terrain have long paths through the root box and are
+    cast as several segments whose hits are reduced per ray; the result must equal the
+    brute-force closest hit bit for bit (toi, normal, feature), including misses."""
+    pts, tri = workloads.heightfield_mesh(seed=9, cells=64)
+    pts = pts.copy()
+    pts[:, 2] *= 0.05
+    rng = np.random.default_rng(12)
+    n = 6000
+    o = np.stack([rng.uniform(-2, 8.4, n), rng.uniform(-2, 8.4, n), rng.uniform(0.02, 0.3, n)], 1).astype(np.float32)
+    ang = rng.uniform(0, 2 * np.pi, n)
+    dz = -rng.uniform(0.0, 0.08, n)
+    d = np.stack([np.cos(ang), np.sin(ang), dz], 1)
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    d[::50, 2] = 0.0  # exactly horizontal: the zero-direction slab rule
+    mesh = ncx.TriMesh(pts, tri, ctx=ctx)
+    toi, nrm, feat = mesh.toi_and_normal_with_ray(o, d, 30.0)
+    om = orc.TriMesh(pts, tri)
+    bt, bn, bf, _ = om.brute(o, d, 30.0, nthreads=8)
+    assert np.array_equal(feat.cpu().numpy().view(np.uint32), bf)
+    assert np.array_equal(toi.cpu().numpy(), bt)
+    assert np.array_equal(nrm.cpu().numpy(), bn)
+    print(f"grazing rays: {n}, hits {np.mean(bf != 0xFFFFFFFF):.3f}")
