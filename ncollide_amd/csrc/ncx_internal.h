@@ -69,11 +69,11 @@ struct DevBuf {
 };
 
 // Layout of ContactTiers::ovf (uint32 words): a header, then the pair indices whose EPA outgrew
-// the first-tier arena (cap entries), then those that outgrew the second tier as well (cap
-// entries).
+// the first-tier arena, those that outgrew the second tier, and those that outgrew the third
+// (cap entries each).
 constexpr uint32_t NCX_OVF_T1_COUNT = 0, NCX_OVF_BAD = 1, NCX_OVF_T2_COUNT = 2, NCX_OVF_T1_WORK = 3,
-                   NCX_OVF_T2_WORK = 4, NCX_BKT_CNT = 8, NCX_BKT_FILL = 12, NCX_BKT_WORK = 16, NCX_BKT_RANGE = 20,
-                   NCX_OVF_HEADER = 32;
+                   NCX_OVF_T2_WORK = 4, NCX_OVF_T3_COUNT = 5, NCX_OVF_T3_WORK = 6, NCX_BKT_CNT = 8,
+                   NCX_BKT_FILL = 12, NCX_BKT_WORK = 16, NCX_BKT_RANGE = 20, NCX_OVF_HEADER = 32;
 // Pairs are partitioned by their larger hull vertex count into NCX_NBUCKETS buckets, each run by
 // the kernel variant the autotuner picked for it.
 constexpr int NCX_NBUCKETS = 4;
@@ -91,6 +91,7 @@ struct ContactTiers {
   KArgs* last = nullptr;
   uint32_t* t1_list() const { return ovf.as<uint32_t>() + NCX_OVF_HEADER; }
   uint32_t* t2_list() const { return ovf.as<uint32_t>() + NCX_OVF_HEADER + cap; }
+  uint32_t* t3_list() const { return ovf.as<uint32_t>() + NCX_OVF_HEADER + 2 * (size_t)cap; }
 };
 
 // Device-resident shape set (ncx_shapes_create).

@@ -1361,11 +1361,12 @@ __global__ __launch_bounds__(BKT_BLOCK) void k_bucket_scatter(const uint32_t* __
   }
 }
 
-// Arena tiers.  T1: 8 pairs per block (8, 16 or 32 lanes per pair, the autotuner's choice),
-// LDS arenas of 44 vertices and 144 faces (4.7 KiB per pair; 4 blocks fit a CU).  T2: 16 lanes per pair, 4 pairs per
-// 64-thread block, LDS arenas of 128 vertices and 384 faces (12.5 KiB per pair; 3 blocks per
-// CU).  Beyond that, device-memory arenas from 1024 vertices upward, 64 lanes per pair,
-// host-driven.
+// Arena tiers, all but the last in LDS and in the graph:
+//   T1: 8 pairs per block (8, 16 or 32 lanes per pair, the autotuner's choice), arenas of 44
+//       vertices and 144 faces (4.7 KiB per pair; 4 blocks fit a CU);
+//   T2: 16 lanes per pair, 4 pairs per 64-thread block, 128 vertices and 384 faces (12.5 KiB);
+//   T3: 64 lanes, one pair per 64-thread block, 512 vertices and 1600 faces (51 KiB);
+//   then device-memory arenas from 4096 vertices upward, 64 lanes per pair, host-driven.
 #define T1_G 16
 #define T1_BS 128
 #define T1_VC 44u
@@ -1378,6 +1379,10 @@ __global__ __launch_bounds__(BKT_BLOCK) void k_bucket_scatter(const uint32_t* __
 #define T2_FC 384u
 #define T2_SC 64u
 #define T2_STC 64u
+#define T3_VC 512u
+#define T3_FC 1600u
+#define T3_SC 128u
+#define T3_STC 128u
 
 // Overflow lists of at least cap entries each (at most 2^24: a call with more overflowing pairs
 // fails with NCX_E_CAPACITY).
@@ -1386,7 +1391,7 @@ int contact_tiers_reserve(ContactTiers& t, size_t cap) {
   if (cap <= t.cap) return NCX_OK;
   t.ovf.release();
   t.cap = 0;
-  int rc = t.ovf.reserve(4 * (NCX_OVF_HEADER + 2 * cap));
+  int rc = t.ovf.reserve(4 * (NCX_OVF_HEADER + 3 * cap));
   if (rc) return rc;
   t.cap = (uint32_t)cap;
   return NCX_OK;
@@ -1506,10 +1511,29 @@ static KArgs t2_args(ContactTiers& T, const KArgs& a) {
   return b;
 }
 
+// T3 over the device-counted T2 overflow list: 64 lanes and one pair per block, with a
+// 512-vertex, 1600-face LDS arena (51 KiB).
+static KArgs t3_args(ContactTiers& T, const KArgs& a) {
+  uint32_t* ovf = T.ovf.as<uint32_t>();
+  KArgs b = t2_args(T, a);
+  b.list = T.t2_list();
+  b.n_dev = ovf + NCX_OVF_T2_COUNT;
+  b.vcap = T3_VC;
+  b.fcap = T3_FC;
+  b.ovf_count = ovf + NCX_OVF_T3_COUNT;
+  b.ovf_list = T.t3_list();
+  b.work = ovf + NCX_OVF_T3_WORK;
+  return b;
+}
+
 static int launch_t2(ncx_ctx* ctx, ContactTiers& T, const KArgs& a) {
   KArgs b = t2_args(T, a);
   hipLaunchKernelGGL((k_pairs<T2_G, T2_BS, T2_VC, T2_FC, T2_SC, T2_STC>), dim3(ctx->num_cus * 3), dim3(T2_BS), 0,
                      ctx->stream, b);
+  NCX_HIP_CHECK(hipGetLastError());
+  KArgs c = t3_args(T, a);
+  hipLaunchKernelGGL((k_pairs<64, 64, T3_VC, T3_FC, T3_SC, T3_STC>), dim3(ctx->num_cus * 3), dim3(64), 0,
+                     ctx->stream, c);
   NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
 }
@@ -1663,7 +1687,7 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
   }
   if (mode == 0 && (rc = launch_t2(ctx, T, a))) return rc;
   if (!T.last) T.last = new KArgs;
-  *T.last = t2_args(T, a);
+  *T.last = t3_args(T, a);
   return NCX_OK;
 }
 
@@ -1688,21 +1712,20 @@ int contacts_launch_device_count(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, c
 }
 
 // After the last launch_tiers on T has completed on ctx's stream: fails on a bad pair index or on
-// more T1 overflows than the list holds, re-runs what T2 could not hold with device-memory
-// arenas, and reports how many pairs needed more than the T1 arena.
+// more overflows than the lists hold, re-runs what T3 could not hold with device-memory arenas,
+// and reports how many pairs needed more than the T1 arena.
 int contacts_resolve_overflow(ContactTiers& T, ncx_ctx* ctx, size_t* n_large) {
   NCX_REQUIRE(T.last, "contact resolve: no tiered launch to resolve");
   uint32_t hdr[NCX_OVF_HEADER];
   NCX_HIP_CHECK(hipMemcpyAsync(hdr, T.ovf.p, sizeof(hdr), hipMemcpyDeviceToHost, ctx->stream));
   NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   NCX_REQUIRE(hdr[NCX_OVF_BAD] == 0, "contact batch: a pair index is not a shape of the set");
-  if (hdr[NCX_OVF_T1_COUNT] > T.cap || hdr[NCX_OVF_T2_COUNT] > T.cap) {
+  if (hdr[NCX_OVF_T1_COUNT] > T.cap || hdr[NCX_OVF_T2_COUNT] > T.cap || hdr[NCX_OVF_T3_COUNT] > T.cap) {
     ncx_set_error("contact batch: more EPA arena overflows in one call than the overflow list holds");
     return NCX_E_CAPACITY;
   }
   if (n_large) *n_large = hdr[NCX_OVF_T1_COUNT];
-  return rerun_overflows(T, ctx, *T.last, hdr[NCX_OVF_T2_COUNT], T.t2_list(), T2_VC,
-                         T2_FC);
+  return rerun_overflows(T, ctx, *T.last, hdr[NCX_OVF_T3_COUNT], T.t3_list(), T3_VC, T3_FC);
 }
 
 static int run_pairs(ncx_shapes* S, KArgs a, size_t npairs) {

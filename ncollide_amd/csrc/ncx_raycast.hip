@@ -439,6 +439,37 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_pt(const QNode* __restrict__ 
   }
 }
 
+// Lays the cast tree out in depth-first preorder (left child first), so a node's left child is
+// the next 32 bytes and descending often stays in the cache line just loaded.  One-time host
+// pass at mesh creation; leaf references (triangle positions) are unchanged.
+static int preorder_nodes(ncx_mesh* m) {
+  const size_t ni = m->ntri - 1;
+  std::vector<QNode> src(ni), dst(ni);
+  NCX_HIP_CHECK(hipMemcpy(src.data(), m->qnodes.p, ni * sizeof(QNode), hipMemcpyDeviceToHost));
+  std::vector<uint32_t> newid(ni, 0xFFFFFFFFu), stack;
+  stack.reserve(128);
+  stack.push_back(0);
+  uint32_t next = 0;
+  while (!stack.empty()) {
+    uint32_t v = stack.back();
+    stack.pop_back();
+    NCX_REQUIRE(v < ni && newid[v] == 0xFFFFFFFFu, "ncx_trimesh_create: malformed tree");
+    newid[v] = next++;
+    const QNode& q = src[v];
+    if (!(q.child_r & NCX_LEAF_BIT)) stack.push_back(q.child_r);
+    if (!(q.child_l & NCX_LEAF_BIT)) stack.push_back(q.child_l);
+  }
+  NCX_REQUIRE(next == ni, "ncx_trimesh_create: tree does not reach every node");
+  for (size_t v = 0; v < ni; ++v) {
+    QNode q = src[v];
+    if (!(q.child_l & NCX_LEAF_BIT)) q.child_l = newid[q.child_l];
+    if (!(q.child_r & NCX_LEAF_BIT)) q.child_r = newid[q.child_r];
+    dst[newid[v]] = q;
+  }
+  NCX_HIP_CHECK(hipMemcpy(m->qnodes.p, dst.data(), ni * sizeof(QNode), hipMemcpyHostToDevice));
+  return NCX_OK;
+}
+
 extern "C" int ncx_trimesh_create(ncx_ctx* ctx, const float* points_xyz, size_t npoints, const uint32_t* indices,
                                   size_t ntriangles, ncx_mesh** out) {
   NCX_REQUIRE(ctx && out, "ncx_trimesh_create: null argument");
@@ -494,6 +525,7 @@ extern "C" int ncx_trimesh_create(ncx_ctx* ctx, const float* points_xyz, size_t 
     ncx_set_error("ncx_trimesh_create: a quantised node box does not contain its exact box");
     return fail(NCX_E_HIP);
   }
+  if (ntriangles > 2 && (rc = preorder_nodes(m))) return fail(rc);
   *out = m;
   return NCX_OK;
 }

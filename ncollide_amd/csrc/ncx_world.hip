@@ -88,7 +88,7 @@ __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float 
 //   MODE_COUNT: count the pairs and keep the first WQ_LOCAL partners in lbuf;
 //   MODE_WRITE: write every pair at offs[k] (only for leaves with more than WQ_LOCAL pairs).
 #define WQ_LSTACK 16
-#define WQ_LOCAL 16
+#define WQ_LOCAL 64
 #define MODE_COUNT 0
 #define MODE_WRITE 1
 
