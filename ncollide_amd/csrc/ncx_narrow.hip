@@ -1095,8 +1095,8 @@ struct KArgs {
 // entries) lives in LDS; a pair whose polytope outgrows it is listed in ovf for a launch with
 // larger arenas.  VC == 0: the arena is the group's slice of a.arena (a.vcap, a.fcap) in device
 // memory.
-template <int G, int BS, uint32_t VC, uint32_t FC, uint32_t SC, uint32_t STC>
-__global__ __launch_bounds__(BS, 2) void k_pairs(KArgs a) {
+template <int G, int BS, uint32_t VC, uint32_t FC, uint32_t SC, uint32_t STC, int MODE>
+__global__ __launch_bounds__(BS, MODE == 1 ? 3 : 2) void k_pairs(KArgs a) {
   const int gl = threadIdx.x & (G - 1);
   const size_t group = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const size_t ngroups = ((size_t)gridDim.x * blockDim.x) / G;
@@ -1136,7 +1136,7 @@ __global__ __launch_bounds__(BS, 2) void k_pairs(KArgs a) {
     c.g2 = load_shape(a.shapes, i2);
     c.m1 = load_pose(a.poses, i1);
     c.m2 = load_pose(a.poses, i2);
-    if (a.distance_mode) {
+    if (MODE == 1) {  // distance (a compile-time branch: each kernel holds one query's code)
       float d = 0.f;
       int it = 0;
       bool ovf = false;
@@ -1407,7 +1407,7 @@ void contact_tiers_release(ContactTiers& t) {
 static int launch_global_tier(hipStream_t stream, KArgs a, size_t npairs) {
   a.npairs = npairs;
   a.work = nullptr;
-  hipLaunchKernelGGL((k_pairs<64, 128, 0u, 0u, 0u, 0u>), dim3(ceil_div(npairs * 64, 128)), dim3(128), 0, stream, a);
+  hipLaunchKernelGGL((k_pairs<64, 128, 0u, 0u, 0u, 0u, 0>), dim3(ceil_div(npairs * 64, 128)), dim3(128), 0, stream, a);
   NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
 }
@@ -1473,20 +1473,20 @@ static const unsigned T1_BLOCKS_PER_CU[2][3] = {{8, 8, 4}, {8, 8, 8}};  // twice
 static int launch_t1(int variant, bool distance, hipStream_t stream, const KArgs& a, unsigned blocks) {
   if (distance) {
     switch (variant) {
-      case 0: hipLaunchKernelGGL((k_pairs<8, 128, 0u, 0u, 0u, 0u>), dim3(blocks), dim3(128), 0, stream, a); break;
-      case 2: hipLaunchKernelGGL((k_pairs<32, 128, 0u, 0u, 0u, 0u>), dim3(blocks), dim3(128), 0, stream, a); break;
-      default: hipLaunchKernelGGL((k_pairs<16, 128, 0u, 0u, 0u, 0u>), dim3(blocks), dim3(128), 0, stream, a); break;
+      case 0: hipLaunchKernelGGL((k_pairs<8, 128, 0u, 0u, 0u, 0u, 1>), dim3(blocks), dim3(128), 0, stream, a); break;
+      case 2: hipLaunchKernelGGL((k_pairs<32, 128, 0u, 0u, 0u, 0u, 1>), dim3(blocks), dim3(128), 0, stream, a); break;
+      default: hipLaunchKernelGGL((k_pairs<16, 128, 0u, 0u, 0u, 0u, 1>), dim3(blocks), dim3(128), 0, stream, a); break;
     }
   } else {
     switch (variant) {
       case 0:
-        hipLaunchKernelGGL((k_pairs<8, 64, T1_VC, T1_FC, T1_SC, T1_STC>), dim3(blocks), dim3(64), 0, stream, a);
+        hipLaunchKernelGGL((k_pairs<8, 64, T1_VC, T1_FC, T1_SC, T1_STC, 0>), dim3(blocks), dim3(64), 0, stream, a);
         break;
       case 2:
-        hipLaunchKernelGGL((k_pairs<32, 256, T1_VC, T1_FC, T1_SC, T1_STC>), dim3(blocks), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL((k_pairs<32, 256, T1_VC, T1_FC, T1_SC, T1_STC, 0>), dim3(blocks), dim3(256), 0, stream, a);
         break;
       default:
-        hipLaunchKernelGGL((k_pairs<16, 128, T1_VC, T1_FC, T1_SC, T1_STC>), dim3(blocks), dim3(128), 0, stream, a);
+        hipLaunchKernelGGL((k_pairs<16, 128, T1_VC, T1_FC, T1_SC, T1_STC, 0>), dim3(blocks), dim3(128), 0, stream, a);
         break;
     }
   }
@@ -1528,11 +1528,11 @@ static KArgs t3_args(ContactTiers& T, const KArgs& a) {
 
 static int launch_t2(ncx_ctx* ctx, ContactTiers& T, const KArgs& a) {
   KArgs b = t2_args(T, a);
-  hipLaunchKernelGGL((k_pairs<T2_G, T2_BS, T2_VC, T2_FC, T2_SC, T2_STC>), dim3(ctx->num_cus * 3), dim3(T2_BS), 0,
+  hipLaunchKernelGGL((k_pairs<T2_G, T2_BS, T2_VC, T2_FC, T2_SC, T2_STC, 0>), dim3(ctx->num_cus * 3), dim3(T2_BS), 0,
                      ctx->stream, b);
   NCX_HIP_CHECK(hipGetLastError());
   KArgs c = t3_args(T, a);
-  hipLaunchKernelGGL((k_pairs<64, 64, T3_VC, T3_FC, T3_SC, T3_STC>), dim3(ctx->num_cus * 3), dim3(64), 0,
+  hipLaunchKernelGGL((k_pairs<64, 64, T3_VC, T3_FC, T3_SC, T3_STC, 0>), dim3(ctx->num_cus * 3), dim3(64), 0,
                      ctx->stream, c);
   NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
