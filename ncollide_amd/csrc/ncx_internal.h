@@ -69,14 +69,26 @@ struct DevBuf {
 };
 
 // Layout of ContactTiers::ovf (uint32 words): a header, then the pair indices whose EPA outgrew
-// the first-tier arena, those that outgrew the second tier, and those that outgrew the third
-// (cap entries each).
-constexpr uint32_t NCX_OVF_T1_COUNT = 0, NCX_OVF_BAD = 1, NCX_OVF_T2_COUNT = 2, NCX_OVF_T1_WORK = 3,
-                   NCX_OVF_T2_WORK = 4, NCX_OVF_T3_COUNT = 5, NCX_OVF_T3_WORK = 6, NCX_BKT_CNT = 8,
-                   NCX_BKT_FILL = 12, NCX_BKT_WORK = 16, NCX_BKT_RANGE = 20, NCX_OVF_HEADER = 32;
+// the first-tier arena, those that outgrew the second tier, the third, and the fourth (cap
+// entries each).
+// Header words [0, NCX_OVF_TIER_WORDS) are the tiers' counts (cleared between tuning samples),
+// [NCX_OVF_TIER_WORDS, NCX_OVF_COUNTS) the vertex-count buckets' (which the samples read), and
+// from NCX_WORK_BASE on the work counters of the launches (the tuning samples' first tier, the
+// second, third and fourth tiers, one per bucket), each on its own cache line.
+constexpr uint32_t NCX_OVF_T1_COUNT = 0, NCX_OVF_BAD = 1, NCX_OVF_T2_COUNT = 2, NCX_OVF_T3_COUNT = 3,
+                   NCX_OVF_T4_COUNT = 4, NCX_OVF_TIER_WORDS = 8, NCX_BKT_CNT = 8, NCX_BKT_FILL = 12,
+                   NCX_BKT_RANGE = 16, NCX_OVF_COUNTS = 32;
+constexpr uint32_t NCX_WORK_SET = 64, NCX_WORK_BASE = NCX_OVF_COUNTS, NCX_WORK_TUNE = 0, NCX_WORK_T2 = 1, NCX_WORK_T3 = 2,
+                   NCX_WORK_T4 = 3, NCX_WORK_BKT = 4, NCX_WORK_SETS = 8,
+                   NCX_OVF_HEADER = NCX_WORK_BASE + NCX_WORK_SETS * NCX_WORK_SET;
 // Pairs are partitioned by their larger hull vertex count into NCX_NBUCKETS buckets, each run by
 // the kernel variant the autotuner picked for it.
 constexpr int NCX_NBUCKETS = 4;
+static_assert(NCX_OVF_T4_COUNT < NCX_OVF_TIER_WORDS && NCX_OVF_TIER_WORDS <= NCX_BKT_CNT &&
+                  NCX_BKT_RANGE + 2 * NCX_NBUCKETS <= NCX_OVF_COUNTS && NCX_WORK_BKT + NCX_NBUCKETS <= NCX_WORK_SETS,
+              "overflow header layout");
+// The work counter of launch k.
+inline uint32_t* ncx_work_set(uint32_t* ovf, uint32_t k) { return ovf + NCX_WORK_BASE + k * NCX_WORK_SET; }
 
 struct KArgs;  // contact kernel arguments (ncx_narrow.hip)
 
@@ -92,6 +104,7 @@ struct ContactTiers {
   uint32_t* t1_list() const { return ovf.as<uint32_t>() + NCX_OVF_HEADER; }
   uint32_t* t2_list() const { return ovf.as<uint32_t>() + NCX_OVF_HEADER + cap; }
   uint32_t* t3_list() const { return ovf.as<uint32_t>() + NCX_OVF_HEADER + 2 * (size_t)cap; }
+  uint32_t* t4_list() const { return ovf.as<uint32_t>() + NCX_OVF_HEADER + 3 * (size_t)cap; }
 };
 
 // Device-resident shape set (ncx_shapes_create).

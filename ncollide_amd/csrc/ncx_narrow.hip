@@ -1102,7 +1102,7 @@ __global__ __launch_bounds__(BS, MODE == 1 ? 3 : 2) void k_pairs(KArgs a) {
   const size_t ngroups = ((size_t)gridDim.x * blockDim.x) / G;
   constexpr bool LDS = VC > 0;
   constexpr size_t LDS_BYTES = LDS ? (BS / G) * arena_bytes(VC, FC, SC, STC) : 16;
-  static_assert(LDS_BYTES <= 65536, "LDS arena exceeds 64 KiB per block");
+  static_assert(LDS_BYTES <= 163840, "LDS arena exceeds the 160 KiB a gfx950 workgroup may hold");
   static_assert(!LDS || (VC < 65536 && FC < 65536 && SC >= 3 && STC >= 3), "arena capacities");
   __shared__ __align__(16) uint8_t lds_arena[LDS_BYTES];
   Arena A;
@@ -1366,7 +1366,9 @@ __global__ __launch_bounds__(BKT_BLOCK) void k_bucket_scatter(const uint32_t* __
 //       vertices and 144 faces (4.7 KiB per pair; 4 blocks fit a CU);
 //   T2: 16 lanes per pair, 4 pairs per 64-thread block, 128 vertices and 384 faces (12.5 KiB);
 //   T3: 64 lanes, one pair per 64-thread block, 512 vertices and 1600 faces (51 KiB);
-//   then device-memory arenas from 4096 vertices upward, 64 lanes per pair, host-driven.
+//   T4: 64 lanes, one pair per 64-thread block, T4_VC vertices and T4_FC faces (up to the
+//       160 KiB a gfx950 workgroup may hold; one block per CU);
+//   then device-memory arenas 8x larger than T4's, 64 lanes per pair, host-driven.
 #define T1_G 16
 #define T1_BS 128
 #define T1_VC 44u
@@ -1383,6 +1385,10 @@ __global__ __launch_bounds__(BKT_BLOCK) void k_bucket_scatter(const uint32_t* __
 #define T3_FC 1600u
 #define T3_SC 128u
 #define T3_STC 128u
+#define T4_VC 1536u
+#define T4_FC 4800u
+#define T4_SC 256u
+#define T4_STC 256u
 
 // Overflow lists of at least cap entries each (at most 2^24: a call with more overflowing pairs
 // fails with NCX_E_CAPACITY).
@@ -1391,7 +1397,7 @@ int contact_tiers_reserve(ContactTiers& t, size_t cap) {
   if (cap <= t.cap) return NCX_OK;
   t.ovf.release();
   t.cap = 0;
-  int rc = t.ovf.reserve(4 * (NCX_OVF_HEADER + 3 * cap));
+  int rc = t.ovf.reserve(4 * (NCX_OVF_HEADER + 4 * cap));
   if (rc) return rc;
   t.cap = (uint32_t)cap;
   return NCX_OK;
@@ -1507,7 +1513,7 @@ static KArgs t2_args(ContactTiers& T, const KArgs& a) {
   b.fcap = T2_FC;
   b.ovf_count = ovf + NCX_OVF_T2_COUNT;
   b.ovf_list = T.t2_list();
-  b.work = ovf + NCX_OVF_T2_WORK;
+  b.work = ncx_work_set(ovf, NCX_WORK_T2);
   return b;
 }
 
@@ -1522,7 +1528,22 @@ static KArgs t3_args(ContactTiers& T, const KArgs& a) {
   b.fcap = T3_FC;
   b.ovf_count = ovf + NCX_OVF_T3_COUNT;
   b.ovf_list = T.t3_list();
-  b.work = ovf + NCX_OVF_T3_WORK;
+  b.work = ncx_work_set(ovf, NCX_WORK_T3);
+  return b;
+}
+
+// T4 over the device-counted T3 overflow list: one pair per 64-thread block with the largest
+// LDS arena a workgroup may hold.
+static KArgs t4_args(ContactTiers& T, const KArgs& a) {
+  uint32_t* ovf = T.ovf.as<uint32_t>();
+  KArgs b = t3_args(T, a);
+  b.list = T.t3_list();
+  b.n_dev = ovf + NCX_OVF_T3_COUNT;
+  b.vcap = T4_VC;
+  b.fcap = T4_FC;
+  b.ovf_count = ovf + NCX_OVF_T4_COUNT;
+  b.ovf_list = T.t4_list();
+  b.work = ncx_work_set(ovf, NCX_WORK_T4);
   return b;
 }
 
@@ -1534,6 +1555,10 @@ static int launch_t2(ncx_ctx* ctx, ContactTiers& T, const KArgs& a) {
   KArgs c = t3_args(T, a);
   hipLaunchKernelGGL((k_pairs<64, 64, T3_VC, T3_FC, T3_SC, T3_STC, 0>), dim3(ctx->num_cus * 3), dim3(64), 0,
                      ctx->stream, c);
+  NCX_HIP_CHECK(hipGetLastError());
+  KArgs d = t4_args(T, a);
+  hipLaunchKernelGGL((k_pairs<64, 64, T4_VC, T4_FC, T4_SC, T4_STC, 0>), dim3(ctx->num_cus), dim3(64), 0,
+                     ctx->stream, d);
   NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
 }
@@ -1552,6 +1577,14 @@ int contact_tiers_reserve_list(ContactTiers& t, size_t npairs) {
 // on a sample of the bucket's pairs (second of two runs, HIP events) and keep the fastest.  The
 // sample's results are real results (the full launch recomputes them identically).  Host-driven;
 // the choice is kept in the shape set for every later call and scene step.
+// Clears the tiers' counts and work counters (not the buckets').
+static int clear_tier_counters(ncx_ctx* ctx, uint32_t* hdr) {
+  int rc = fill_u32(ctx, hdr, NCX_OVF_TIER_WORDS, 0u);
+  if (rc) return rc;
+  static_assert(NCX_WORK_TUNE == 0 && NCX_WORK_T2 == 1 && NCX_WORK_T3 == 2 && NCX_WORK_T4 == 3, "work sets");
+  return fill_u32(ctx, ncx_work_set(hdr, 0), 4 * NCX_WORK_SET, 0u);
+}
+
 #define TUNE_MIN_PAIRS 4096u
 #define TUNE_SAMPLE 32768u
 static int autotune(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const KArgs& base, int mode) {
@@ -1576,10 +1609,10 @@ static int autotune(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const KArgs& b
       KArgs a = base;
       a.range_b = dr.as<uint32_t>();
       a.range_e = dr.as<uint32_t>() + 1;
-      a.work = hdr + NCX_OVF_T1_WORK;
+      a.work = ncx_work_set(hdr, NCX_WORK_TUNE);
       float ms = 0.f;
       for (int rep = 0; rep < 2; ++rep) {
-        if ((rc = fill_u32(ctx, hdr, 8, 0u))) break;
+        if ((rc = clear_tier_counters(ctx, hdr))) break;
         NCX_HIP_CHECK(hipEventRecord(e0, ctx->stream));
         if ((rc = launch_t1(v, mode == 1, ctx->stream, a, (unsigned)ctx->num_cus * T1_BLOCKS_PER_CU[mode][v]))) break;
         // the variant's first-tier overflows cost a second-tier run: time it too
@@ -1600,13 +1633,13 @@ static int autotune(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const KArgs& b
   dr.release();
   if (rc) return rc;
   // the samples left entries in the overflow lists and moved the work counter: start clean
-  return fill_u32(ctx, hdr, 8, 0u);
+  return clear_tier_counters(ctx, hdr);
 }
 
 // The launch: partition the pairs into vertex-count buckets (device-side counts, so it is
-// graph-safe), run each bucket with its variant (T1, LDS arenas), then T2 reruns the
-// device-counted T1 overflow list with 128-vertex LDS arenas, listing what outgrows that in turn
-// for contacts_resolve_overflow.  Every launch takes pairs from a work counter, so uneven pair
+// graph-safe), run each bucket with its variant (T1, LDS arenas), then T2, T3 and T4
+// rerun the device-counted overflow list of the tier before with larger LDS arenas, the last
+// listing what outgrows it for contacts_resolve_overflow.  Every launch takes pairs from a work counter, so uneven pair
 // costs balance across the chip.  npairs caps the pair count; n_dev (optional) holds the actual
 // count on the device.  may_tune: run the autotuner first if a bucket has no choice yet (host
 // synchronisation; never inside graph capture).
@@ -1660,7 +1693,7 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
     ab.list = nullptr;
     ab.n_dev = n_dev;
     ab.npairs = npairs;
-    ab.work = ovf + NCX_BKT_WORK;
+    ab.work = ncx_work_set(ovf, NCX_WORK_BKT);
     unsigned blocks = (unsigned)ctx->num_cus * T1_BLOCKS_PER_CU[mode][v];
     if (!n_dev)
       blocks = std::max(1u, std::min<unsigned>(blocks, ceil_div(npairs * T1_VARIANT_G[v], T1_VARIANT_BS[mode][v])));
@@ -1677,7 +1710,7 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
       ab.npairs = npairs;
       ab.range_b = ovf + NCX_BKT_RANGE + 2 * b0;
       ab.range_e = ovf + NCX_BKT_RANGE + 2 * b1 + 1;
-      ab.work = ovf + NCX_BKT_WORK + b0;
+      ab.work = ncx_work_set(ovf, NCX_WORK_BKT + b0);
       unsigned blocks = (unsigned)ctx->num_cus * T1_BLOCKS_PER_CU[mode][v];
       if (!n_dev)
         blocks = std::max(1u, std::min<unsigned>(blocks, ceil_div(npairs * T1_VARIANT_G[v], T1_VARIANT_BS[mode][v])));
@@ -1687,7 +1720,7 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
   }
   if (mode == 0 && (rc = launch_t2(ctx, T, a))) return rc;
   if (!T.last) T.last = new KArgs;
-  *T.last = t3_args(T, a);
+  *T.last = t4_args(T, a);
   return NCX_OK;
 }
 
@@ -1712,20 +1745,21 @@ int contacts_launch_device_count(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, c
 }
 
 // After the last launch_tiers on T has completed on ctx's stream: fails on a bad pair index or on
-// more overflows than the lists hold, re-runs what T3 could not hold with device-memory arenas,
+// more overflows than the lists hold, re-runs what T4 could not hold with device-memory arenas,
 // and reports how many pairs needed more than the T1 arena.
 int contacts_resolve_overflow(ContactTiers& T, ncx_ctx* ctx, size_t* n_large) {
   NCX_REQUIRE(T.last, "contact resolve: no tiered launch to resolve");
-  uint32_t hdr[NCX_OVF_HEADER];
+  uint32_t hdr[NCX_OVF_COUNTS];
   NCX_HIP_CHECK(hipMemcpyAsync(hdr, T.ovf.p, sizeof(hdr), hipMemcpyDeviceToHost, ctx->stream));
   NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   NCX_REQUIRE(hdr[NCX_OVF_BAD] == 0, "contact batch: a pair index is not a shape of the set");
-  if (hdr[NCX_OVF_T1_COUNT] > T.cap || hdr[NCX_OVF_T2_COUNT] > T.cap || hdr[NCX_OVF_T3_COUNT] > T.cap) {
+  if (hdr[NCX_OVF_T1_COUNT] > T.cap || hdr[NCX_OVF_T2_COUNT] > T.cap || hdr[NCX_OVF_T3_COUNT] > T.cap ||
+      hdr[NCX_OVF_T4_COUNT] > T.cap) {
     ncx_set_error("contact batch: more EPA arena overflows in one call than the overflow list holds");
     return NCX_E_CAPACITY;
   }
   if (n_large) *n_large = hdr[NCX_OVF_T1_COUNT];
-  return rerun_overflows(T, ctx, *T.last, hdr[NCX_OVF_T3_COUNT], T.t3_list(), T3_VC, T3_FC);
+  return rerun_overflows(T, ctx, *T.last, hdr[NCX_OVF_T4_COUNT], T.t4_list(), T4_VC, T4_FC);
 }
 
 static int run_pairs(ncx_shapes* S, KArgs a, size_t npairs) {
