@@ -28,7 +28,7 @@ struct ncx_mesh {
 
 // Internal node for the cast kernel, 32 bytes (two 16-byte loads per visit instead of four):
 // both child boxes as 16-bit integers on one grid over the root box,
-// coordinate = origin + q * step (a multiply and an add, not fused).  Each minimum is rounded
+// coordinate = fma(q, step, origin) (one fused operation).  Each minimum is rounded
 // down and each maximum up, checked with that same decode, so every decoded box contains the
 // exact one and culling stays conservative; only more boxes pass.
 struct alignas(16) QNode {
@@ -37,7 +37,7 @@ struct alignas(16) QNode {
   uint32_t child_l, child_r;
 };
 
-__device__ __forceinline__ float qdec(uint32_t q, float o, float st) { return o + (float)q * st; }
+__device__ __forceinline__ float qdec(uint32_t q, float o, float st) { return __builtin_fmaf((float)q, st, o); }
 
 // Lowest q whose decode is <= x (minimum) or highest-needed q whose decode is >= x (maximum).
 __device__ __forceinline__ uint16_t qenc(float x, float o, float st, float inv, bool is_max) {
@@ -161,16 +161,38 @@ __device__ __forceinline__ bool ray_tri(f3 a, f3 b, f3 c, f3 o, f3 dir, float* t
 
 // Slab test of one box; axes with a zero direction component use the
 // containment test of AABB::toi_with_ray (ray_aabb.rs:20-23).  Returns the entry
-// parameter, or NaN on a miss (so every `t <= x` test on it is false).
+// parameter, or NaN on a miss (so every `t <= x` test on it is false).  Written without
+// branches: the interval of an axis with a zero component is left unchanged by a select, and
+// its containment test only decides the miss, so the result is the same as returning early.
 __device__ __forceinline__ float slab(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, f3 o, f3 inv,
                                       bool zx, bool zy, bool zz, float tmax0) {
+  const float ax = (mnx - o.x) * inv.x, bx = (mxx - o.x) * inv.x;
+  const float ay = (mny - o.y) * inv.y, by = (mxy - o.y) * inv.y;
+  const float az = (mnz - o.z) * inv.z, bz = (mxz - o.z) * inv.z;
+  const bool miss = (zx & ((o.x < mnx) | (o.x > mxx))) | (zy & ((o.y < mny) | (o.y > mxy))) |
+                    (zz & ((o.z < mnz) | (o.z > mxz)));
   float tmin = 0.f, tmax = tmax0;
-  if (zx) { if (o.x < mnx || o.x > mxx) return __builtin_nanf(""); }
-  else { float a = (mnx - o.x) * inv.x, b = (mxx - o.x) * inv.x; tmin = fmaxf(tmin, fminf(a, b)); tmax = fminf(tmax, fmaxf(a, b)); }
-  if (zy) { if (o.y < mny || o.y > mxy) return __builtin_nanf(""); }
-  else { float a = (mny - o.y) * inv.y, b = (mxy - o.y) * inv.y; tmin = fmaxf(tmin, fminf(a, b)); tmax = fminf(tmax, fmaxf(a, b)); }
-  if (zz) { if (o.z < mnz || o.z > mxz) return __builtin_nanf(""); }
-  else { float a = (mnz - o.z) * inv.z, b = (mxz - o.z) * inv.z; tmin = fmaxf(tmin, fminf(a, b)); tmax = fminf(tmax, fmaxf(a, b)); }
+  tmin = zx ? tmin : fmaxf(tmin, fminf(ax, bx));
+  tmax = zx ? tmax : fminf(tmax, fmaxf(ax, bx));
+  tmin = zy ? tmin : fmaxf(tmin, fminf(ay, by));
+  tmax = zy ? tmax : fminf(tmax, fmaxf(ay, by));
+  tmin = zz ? tmin : fmaxf(tmin, fminf(az, bz));
+  tmax = zz ? tmax : fminf(tmax, fmaxf(az, bz));
+  return (!miss & (tmin <= tmax)) ? tmin : __builtin_nanf("");
+}
+
+// slab for a ray with no zero direction component: the same operations in the same order
+// without the containment tests.
+__device__ __forceinline__ float slab_nz(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, f3 o, f3 inv,
+                                         float tmax0) {
+  const float ax = (mnx - o.x) * inv.x, bx = (mxx - o.x) * inv.x;
+  const float ay = (mny - o.y) * inv.y, by = (mxy - o.y) * inv.y;
+  const float az = (mnz - o.z) * inv.z, bz = (mxz - o.z) * inv.z;
+  float tmin = fmaxf(0.f, fminf(ax, bx)), tmax = fminf(tmax0, fmaxf(ax, bx));
+  tmin = fmaxf(tmin, fminf(ay, by));
+  tmax = fminf(tmax, fmaxf(ay, by));
+  tmin = fmaxf(tmin, fminf(az, bz));
+  tmax = fminf(tmax, fmaxf(az, bz));
   return tmin <= tmax ? tmin : __builtin_nanf("");
 }
 
@@ -448,12 +470,20 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_pt(const QNode* __restrict__ 
     const uint4* q = (const uint4*)(nodes + node);
     const uint4 qa = q[0], qb = q[1];
     const uint32_t cl = qb.z, cr = qb.w;
-    float tl = slab(qdec(qa.x & 0xFFFFu, qo.x, qs.x), qdec(qa.x >> 16, qo.y, qs.y), qdec(qa.y & 0xFFFFu, qo.z, qs.z),
-                    qdec(qa.y >> 16, qo.x, qs.x), qdec(qa.z & 0xFFFFu, qo.y, qs.y), qdec(qa.z >> 16, qo.z, qs.z),
-                    o, inv, zx, zy, zz, max_toi);
-    float tr = slab(qdec(qa.w & 0xFFFFu, qo.x, qs.x), qdec(qa.w >> 16, qo.y, qs.y), qdec(qb.x & 0xFFFFu, qo.z, qs.z),
-                    qdec(qb.x >> 16, qo.x, qs.x), qdec(qb.y & 0xFFFFu, qo.y, qs.y), qdec(qb.y >> 16, qo.z, qs.z),
-                    o, inv, zx, zy, zz, max_toi);
+    const float lx0 = qdec(qa.x & 0xFFFFu, qo.x, qs.x), ly0 = qdec(qa.x >> 16, qo.y, qs.y);
+    const float lz0 = qdec(qa.y & 0xFFFFu, qo.z, qs.z), lx1 = qdec(qa.y >> 16, qo.x, qs.x);
+    const float ly1 = qdec(qa.z & 0xFFFFu, qo.y, qs.y), lz1 = qdec(qa.z >> 16, qo.z, qs.z);
+    const float rx0 = qdec(qa.w & 0xFFFFu, qo.x, qs.x), ry0 = qdec(qa.w >> 16, qo.y, qs.y);
+    const float rz0 = qdec(qb.x & 0xFFFFu, qo.z, qs.z), rx1 = qdec(qb.x >> 16, qo.x, qs.x);
+    const float ry1 = qdec(qb.y & 0xFFFFu, qo.y, qs.y), rz1 = qdec(qb.y >> 16, qo.z, qs.z);
+    float tl, tr;
+    if (__ballot(zx | zy | zz) == 0) {  // no lane's ray has a zero direction component (uniform)
+      tl = slab_nz(lx0, ly0, lz0, lx1, ly1, lz1, o, inv, max_toi);
+      tr = slab_nz(rx0, ry0, rz0, rx1, ry1, rz1, o, inv, max_toi);
+    } else {
+      tl = slab(lx0, ly0, lz0, lx1, ly1, lz1, o, inv, zx, zy, zz, max_toi);
+      tr = slab(rx0, ry0, rz0, rx1, ry1, rz1, o, inv, zx, zy, zz, max_toi);
+    }
     bool hl = tl <= best_t, hr = tr <= best_t;
     if (hl && (cl & NCX_LEAF_BIT)) { pl0 = cl & ~NCX_LEAF_BIT; hl = false; }
     if (hr && (cr & NCX_LEAF_BIT)) {

@@ -65,6 +65,50 @@ def test_raycast_heightfield_matches_oracle(ctx, orc):
     print(f"rays: {len(o)}, hits {np.mean(feat != 0xFFFFFFFF):.3f}, tie-resolved differences {len(diff)}")
 
 
+def test_raycast_axis_aligned_rays_on_grid_lines(ctx, orc):
+    # Rays with zero direction components through a many-triangle mesh (the traversal's boxes
+    # take the containment branch of the slab test), origins exactly on vertex coordinates and
+    # cell edges, where box faces lie on the ray.
+    pts, tri = workloads.heightfield_mesh(seed=4, cells=32)
+    rng = np.random.default_rng(7)
+    xs = np.unique(pts[:, 0])
+    n = 3000
+    o = np.empty((n, 3), np.float32)
+    d = np.zeros((n, 3), np.float32)
+    o[:, 0] = rng.choice(xs, n)
+    o[:, 1] = rng.choice(xs, n)
+    o[:, 2] = 3.0
+    half = n // 3
+    o[half:2 * half, 0] += rng.uniform(0, 0.1, half).astype(np.float32)  # on a grid line in y only
+    d[:2 * half, 2] = -1.0                                              # straight down
+    k = n - 2 * half
+    axis = rng.integers(0, 2, k)                                        # along x or y, from outside
+    sign = rng.choice([-1.0, 1.0], k)
+    o[2 * half:, 2] = rng.uniform(-0.6, 0.6, k)
+    o[2 * half:, 0] = np.where(axis == 0, np.where(sign > 0, -1.0, 4.2), o[2 * half:, 0])
+    o[2 * half:, 1] = np.where(axis == 1, np.where(sign > 0, -1.0, 4.2), o[2 * half:, 1])
+    d[2 * half:, 0] = np.where(axis == 0, sign, 0.0)
+    d[2 * half:, 1] = np.where(axis == 1, sign, 0.0)
+    mesh = ncx.TriMesh(pts, tri, ctx=ctx)
+    toi, nrm, feat = mesh.toi_and_normal_with_ray(o, d, 100.0)
+    toi, nrm, feat = toi.cpu().numpy(), nrm.cpu().numpy(), feat.cpu().numpy().view(np.uint32)
+    om = orc.TriMesh(pts, tri)
+    bt, bn, bf, _ = om.brute(o, d, 100.0, nthreads=8)
+    assert np.array_equal(toi, bt), "toi differs from the least hit over all triangles"
+    assert np.array_equal(feat, bf)
+    assert np.array_equal(nrm, bn)
+    # The reference's best-first search (bvh.rs:101-160) keeps the first hit found and stops once
+    # no box can hold a smaller cost.  A ray through a mesh vertex hits the triangles around it
+    # at parameters a rounding apart, and the search may stop on one 1-2 ulps past the least
+    # (its box's entry parameter already exceeds the better hit): only there may it differ.
+    rt, rn, rf, _ = om.cast(o, d, 100.0, nthreads=8)
+    diff = np.nonzero(toi != rt)[0]
+    ulps = rt[diff].view(np.int32).astype(np.int64) - toi[diff].view(np.int32).astype(np.int64)
+    assert np.all((ulps >= 1) & (ulps <= 2)), ulps
+    assert np.all(diff < half), "a reference difference away from a mesh vertex"
+    print(f"rays {n}, hits {np.mean(feat != 0xFFFFFFFF):.3f}, reference search 1-2 ulps farther on {len(diff)}")
+
+
 def test_raycast_edge_cases(ctx, orc):
     # single triangle, rays along axes (zero direction components), back faces, max_toi cut
     pts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
