@@ -45,6 +45,8 @@ def parse():
     p.add_argument("--c4-steps", type=int, default=64)
     p.add_argument("--scene-shapes", type=int, default=1 << 20)
     p.add_argument("--scene-steps", type=int, default=32)
+    p.add_argument("--rebuild-every", type=int, default=2,
+                   help="scene step: LBVH rebuilt every k steps, refit in between")
     p.add_argument("--scene-max-pairs", type=int, default=40_000_000)
     return p.parse_args()
 
@@ -253,11 +255,12 @@ def bench_learned(args, ncx, torch, ctx):
 
 def bench_scene(args, ncx, torch, ctx):
     """Scene step on 1,048,576 mixed shapes in 64 Gaussian clusters: shape AABBs, LBVH refit
-    (rebuild every 8 steps), pair search, contacts; each refit step replayed as a hipGraph.
+    (rebuilt every --rebuild-every steps, default 2), pair search, contacts; each refit step replayed
+    as a hipGraph.
     Then the same steps with the narrow phase off, for the broad phase's own time and roofline:
     bytes per step = 68 N (pose + parameters in, box out) + 72 (N-1) (refit) + 64 x nodes visited
     by one self-query traversal (both child boxes per visit) + 8 x pairs, plus on a rebuild step
-    64 N (key/value sort) + 16 (N-1) (hierarchy), spread over the 8-step cycle."""
+    64 N (key/value sort) + 16 (N-1) (hierarchy), spread over the rebuild cycle."""
     from ncollide_amd import workloads
     from ncollide_amd.world import World
     t = time.perf_counter()
@@ -268,7 +271,8 @@ def bench_scene(args, ncx, torch, ctx):
 
     def run(contacts, steps):
         poses = torch.as_tensor(w["poses"], device=ctx.device)
-        world = World(ss, margin=w["margin"], prediction=0.0, max_pairs=args.scene_max_pairs, rebuild_every=8)
+        world = World(ss, margin=w["margin"], prediction=0.0, max_pairs=args.scene_max_pairs,
+                      rebuild_every=args.rebuild_every)
         world.set_contacts(contacts)
         times, pairs, large = [], [], 0
         for s in range(steps):
@@ -290,12 +294,12 @@ def bench_scene(args, ncx, torch, ctx):
                pairs_per_step=float(np.mean(pairs)), pairs_per_s=float(np.sum(pairs[1:])) / float(np.sum(times[1:])),
                epa_pairs_past_first_arena=int(large), gen_s=gen_s)
     del world
-    # broad phase alone: steps 1..16 timed (rebuilds at steps 8 and 16: one per 8 steps)
+    # broad phase alone: steps 1..16 timed
     world, times, pairs, _ = run(False, 17)
     nodes = world.query_stats()
     n = args.scene_shapes
     p = float(np.mean(pairs[1:]))
-    per_step = 68 * n + 72 * (n - 1) + 64 * nodes + 8 * p + (64 * n + 16 * (n - 1)) / 8
+    per_step = 68 * n + 72 * (n - 1) + 64 * nodes + 8 * p + (64 * n + 16 * (n - 1)) / args.rebuild_every
     bp_s = float(np.mean(times[1:]))
     res["broad_phase"] = {"ms_per_step": bp_s * 1e3, "nodes_visited_per_pass": nodes, "pairs_per_step": p,
                           "roofline": {"bound": "hbm", "achieved": per_step / bp_s / 1e9, "peak": HBM_PEAK / 1e9,

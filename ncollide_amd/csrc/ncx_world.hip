@@ -6,7 +6,8 @@
 //   2. the DBVTBroadPhase stored-box rule: a proxy's box is replaced by box.loosened(margin) only
 //      when the new box is not contained in it (dbvt_broad_phase.rs:325-347); the first step
 //      stores the boxes as given (create_proxy, :275-280),
-//   3. LBVH refit over the stored boxes (rebuilt every `rebuild_every` steps),
+//   3. LBVH refit over the stored boxes (rebuilt every `rebuild_every` steps, default 2: the
+//      refit tree's query cost grows each step as shapes move, ~2.5x after 7 refits),
 //   4. all intersecting stored-box pairs, counted, prefix-summed and written on the device,
 //   5. query::contact on every pair (contact_shape_shape.rs:10-48), optionally with the learned
 //      init_dir on polytope pairs.
@@ -29,13 +30,14 @@ struct ncx_world {
   ncx_shapes* shapes;
   size_t n, cap;
   float margin, prediction;
-  int rebuild_every = 8;
+  int rebuild_every = 2;
   int since_build = 0;
   bool first = true;
   DevBuf box_new, stored;
   LBVH bvh;
   uint32_t depth = 0, spill = 1;
-  DevBuf gstack;  // search-stack spill area, `spill` entries per leaf
+  DevBuf gstack;  // search-stack spill area, `spill` entries per query thread
+  DevBuf qwork;   // the query passes' work counters (WQ_PARTS per pass)
   DevBuf lbuf;    // the count pass's first WQ_LOCAL partners per leaf
   DevBuf cnt, offs, counters;  // counters: [0] pair total, [1] pair overflow
   DevBuf pa, pb;
@@ -87,8 +89,17 @@ __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float 
 // LDS and the rest in a per-thread spill area of `spill` entries (the tree depth bounds it).
 //   MODE_COUNT: count the pairs and keep the first WQ_LOCAL partners in lbuf;
 //   MODE_WRITE: write every pair at offs[k] (only for leaves with more than WQ_LOCAL pairs).
+// Persistent: a lane that finishes a leaf takes the next one at once from its wavefront's chunk of
+// WQ_CHUNK leaves (ballot + rank), and a wavefront whose chunk is used up takes the next chunk of
+// its part of the leaves with one atomicAdd on that part's counter (WQ_PARTS counters, each on
+// its own cache line, so the atomics do not serialise on one line); a wave that finds its part
+// used up moves on to the next parts.
 #define WQ_LSTACK 16
 #define WQ_LOCAL 64
+#define WQ_CHUNK 64u
+#define WQ_PARTS 8u
+#define WQ_PART_STRIDE 64u
+#define WQ_BLOCKS_PER_CU 8u
 #define MODE_COUNT 0
 #define MODE_WRITE 1
 
@@ -101,19 +112,25 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
                                                            uint32_t* __restrict__ pb, uint32_t* __restrict__ counters,
                                                            uint32_t* __restrict__ gstack, uint32_t spill,
                                                            uint32_t* __restrict__ lbuf,
-                                                           unsigned long long* __restrict__ visits) {
+                                                           unsigned long long* __restrict__ visits,
+                                                           uint32_t* __restrict__ work) {
   __shared__ uint32_t lstack[WQ_LSTACK * WQ_BLOCK];
   const int tid = threadIdx.x;
-  uint32_t k = blockIdx.x * WQ_BLOCK + tid;
-  if (k >= n) return;
-  if (MODE == MODE_WRITE && cnt[k] <= WQ_LOCAL) return;
-  uint32_t* gs = gstack + (size_t)k * spill;
-  uint32_t prim = order[k];
-  float q[6];
-#pragma unroll
-  for (int d = 0; d < 6; ++d) q[d] = boxes[(size_t)d * n + prim];
-  uint32_t c = 0;
-  size_t w = MODE == MODE_WRITE ? offs[k] : 0;
+  const int lane = __lane_id();
+  const uint32_t gtid = blockIdx.x * WQ_BLOCK + tid;
+  uint32_t* gs = gstack + (size_t)gtid * spill;
+  const uint32_t wave = gtid / 64;
+  const uint32_t part0 = wave % WQ_PARTS;
+  uint32_t part_k = 0;
+  uint32_t next = 0, end = 0;  // the wavefront's chunk [next, end) (the same in every lane)
+  bool pool_empty = false, exhausted = false, has = false;
+  uint32_t k = 0, prim = 0, c = 0;
+  size_t w = 0;
+  float q[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int sp = 0;
+  uint32_t node = 0, top = 0;
+  bool top_valid = false;
+  unsigned long long nvis = 0;
   auto emit = [&](uint32_t leafpos) {
     if (MODE == MODE_WRITE) {
       if (w < cap) {
@@ -129,11 +146,58 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
       c++;
     }
   };
-  int sp = 0;
-  uint32_t node = 0, top = 0;
-  bool top_valid = false;
-  unsigned long long nvis = 0;
   for (;;) {
+    // refill lanes without a leaf
+    const bool want = !has && !exhausted;
+    const uint64_t need = __ballot(want);
+    if (need) {
+      if (next >= end && !pool_empty) {
+        const int leader = __ffsll((unsigned long long)need) - 1;
+        while (part_k < WQ_PARTS) {
+          const uint32_t p = (part0 + part_k) % WQ_PARTS;
+          const uint32_t lo = (uint32_t)(((uint64_t)n * p) / WQ_PARTS);
+          const uint32_t hi = (uint32_t)(((uint64_t)n * (p + 1)) / WQ_PARTS);
+          uint32_t base = 0;
+          if (lane == leader) base = atomicAdd(work + p * WQ_PART_STRIDE, WQ_CHUNK);
+          base = __shfl(base, leader, 64);
+          if (base < hi - lo) {
+            next = lo + base;
+            end = hi - next > WQ_CHUNK ? next + WQ_CHUNK : hi;
+            break;
+          }
+          ++part_k;
+        }
+        if (part_k >= WQ_PARTS) pool_empty = true;
+      }
+      const uint32_t avail = end - next;
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+      if (want) {
+        if (rank < avail) {
+          k = next + rank;
+          if (MODE == MODE_COUNT || cnt[k] > WQ_LOCAL) {
+            prim = order[k];
+#pragma unroll
+            for (int d = 0; d < 6; ++d) q[d] = boxes[(size_t)d * n + prim];
+            c = 0;
+            w = MODE == MODE_WRITE ? offs[k] : 0;
+            sp = 0;
+            node = 0;
+            top_valid = false;
+            has = true;
+          }
+        } else if (pool_empty) {
+          exhausted = true;
+        }
+      }
+      const uint32_t taken = (uint32_t)__popcll(need);
+      next += taken < avail ? taken : avail;
+    }
+    if (__ballot(has) == 0) {
+      if (__ballot(!exhausted) == 0) break;
+      continue;
+    }
+    if (!has) continue;
     nvis++;
     const float4* nq = (const float4*)(nodes + node);
     float4 q0 = nq[0], q1 = nq[1], q2 = nq[2], q3 = nq[3];
@@ -143,6 +207,7 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
     bool hr = lr > k && wbox_overlap(q, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
     if (hl && (cl & NCX_LEAF_BIT)) { emit(cl & ~NCX_LEAF_BIT); hl = false; }
     if (hr && (cr & NCX_LEAF_BIT)) { emit(cr & ~NCX_LEAF_BIT); hr = false; }
+    bool done = false;
     if (hl && hr) {
       if (top_valid) {
         if (sp < WQ_LSTACK) lstack[sp * WQ_BLOCK + tid] = top;
@@ -160,14 +225,24 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
       node = top;
       top_valid = false;
     } else if (sp == 0) {
-      break;
+      done = true;
     } else {
       sp--;
       node = sp < WQ_LSTACK ? lstack[sp * WQ_BLOCK + tid] : gs[sp - WQ_LSTACK];
     }
+    if (done) {
+      if (MODE == MODE_COUNT) cnt[k] = c;
+      has = false;
+    }
   }
-  if (MODE == MODE_COUNT) cnt[k] = c;
   if (visits) atomicAdd(visits, nvis);  // internal nodes visited (measurement only)
+}
+
+// Persistent grid of the query kernels: about as many blocks as are resident, at most one lane
+// per leaf.
+static unsigned world_query_blocks(const ncx_ctx* c, size_t n) {
+  const unsigned resident = (unsigned)c->num_cus * WQ_BLOCKS_PER_CU;
+  return std::max(1u, std::min<unsigned>(resident, ceil_div(n, WQ_BLOCK)));
 }
 
 // Pairs of the leaves with at most WQ_LOCAL pairs, copied from the count pass's buffer in the
@@ -222,7 +297,7 @@ static int world_build(ncx_world* w) {
     w->depth = depth;
     // the search stack holds at most `depth` entries: one in a register, WQ_LSTACK in LDS
     w->spill = depth + 1 > WQ_LSTACK ? depth + 1 - WQ_LSTACK : 1;
-    if ((rc = w->gstack.reserve(w->n * (size_t)w->spill * 4))) return rc;
+    if ((rc = w->gstack.reserve((size_t)ceil_div(w->n, WQ_BLOCK) * WQ_BLOCK * w->spill * 4))) return rc;
   }
   w->since_build = 0;
   return NCX_OK;
@@ -254,14 +329,16 @@ static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses, bool ea
   uint32_t* ctr = w->counters.as<uint32_t>();
   if ((rc = fill_u32(c, ctr, 2, 0u))) return rc;
   if (n >= 2) {
-    unsigned nb = ceil_div(n, WQ_BLOCK);
+    unsigned nb = world_query_blocks(c, n);
     const LNode* nodes = w->bvh.nodes.as<LNode>();
     const uint32_t* order = w->bvh.order.as<uint32_t>();
     const float* st = w->stored.as<float>();
+    uint32_t* qw = w->qwork.as<uint32_t>();
+    if ((rc = fill_u32(c, qw, 2 * WQ_PARTS * WQ_PART_STRIDE, 0u))) return rc;
     hipLaunchKernelGGL((k_world_query<MODE_COUNT>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
                        (uint32_t)n, w->cnt.as<uint32_t>(), (const uint32_t*)nullptr, w->cap, (uint32_t*)nullptr,
                        (uint32_t*)nullptr, ctr, w->gstack.as<uint32_t>(), w->spill, w->lbuf.as<uint32_t>(),
-                       (unsigned long long*)nullptr);
+                       (unsigned long long*)nullptr, qw);
     NCX_HIP_CHECK(hipGetLastError());
     if ((rc = scan_exclusive_u32(c, w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), n, w->bvh.scan_tmp, ctr)))
       return rc;
@@ -272,7 +349,7 @@ static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses, bool ea
     hipLaunchKernelGGL((k_world_query<MODE_WRITE>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
                        (uint32_t)n, w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), w->cap, w->pa.as<uint32_t>(),
                        w->pb.as<uint32_t>(), ctr, w->gstack.as<uint32_t>(), w->spill, w->lbuf.as<uint32_t>(),
-                       (unsigned long long*)nullptr);
+                       (unsigned long long*)nullptr, qw + WQ_PARTS * WQ_PART_STRIDE);
     NCX_HIP_CHECK(hipGetLastError());
   }
   if (!w->contacts) return NCX_OK;
@@ -315,6 +392,7 @@ extern "C" int ncx_world_create(ncx_ctx* ctx, ncx_shapes* shapes, float margin, 
   if ((rc = w->box_new.reserve(n * 24 + 64)) || (rc = w->stored.reserve(n * 24 + 64)) ||
       (rc = w->cnt.reserve(n * 4 + 64)) || (rc = w->offs.reserve(n * 4 + 64)) || (rc = w->counters.reserve(64)) ||
       (rc = w->lbuf.reserve(n * (size_t)WQ_LOCAL * 4 + 64)) ||
+      (rc = w->qwork.reserve(2 * WQ_PARTS * WQ_PART_STRIDE * 4)) ||
       (rc = w->pa.reserve(cap * 4)) || (rc = w->pb.reserve(cap * 4)) || (rc = w->has.reserve(cap)) ||
       (rc = w->w1.reserve(cap * 12)) || (rc = w->w2.reserve(cap * 12)) || (rc = w->nrm.reserve(cap * 12)) ||
       (rc = w->depth_out.reserve(cap * 4)) || (rc = w->iters.reserve(cap * 4)) || (rc = contact_tiers_reserve(w->tiers, std::min<size_t>(cap, (size_t)1 << 22)))) {
@@ -365,12 +443,13 @@ extern "C" int ncx_world_query_stats(ncx_world* w, unsigned long long* nodes_vis
   int rc;
   if ((rc = v.reserve(8))) return rc;
   if ((rc = fill_u32(c, v.as<uint32_t>(), 2, 0u))) return rc;
-  unsigned nb = ceil_div(w->n, WQ_BLOCK);
+  unsigned nb = world_query_blocks(c, w->n);
+  if ((rc = fill_u32(c, w->qwork.as<uint32_t>(), WQ_PARTS * WQ_PART_STRIDE, 0u))) return rc;
   hipLaunchKernelGGL((k_world_query<MODE_COUNT>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, w->bvh.nodes.as<LNode>(),
                      w->bvh.order.as<uint32_t>(), w->stored.as<float>(), (uint32_t)w->n, w->cnt.as<uint32_t>(),
                      (const uint32_t*)nullptr, w->cap, (uint32_t*)nullptr, (uint32_t*)nullptr,
                      w->counters.as<uint32_t>(), w->gstack.as<uint32_t>(), w->spill, w->lbuf.as<uint32_t>(),
-                     v.as<unsigned long long>());
+                     v.as<unsigned long long>(), w->qwork.as<uint32_t>());
   NCX_HIP_CHECK(hipGetLastError());
   NCX_HIP_CHECK(hipMemcpyAsync(nodes_visited, v.p, 8, hipMemcpyDeviceToHost, c->stream));
   NCX_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -388,7 +467,7 @@ extern "C" int ncx_world_destroy(ncx_world* w) {
   }
   if (w->ev_in) (void)hipEventDestroy(w->ev_in);
   if (w->ev_out) (void)hipEventDestroy(w->ev_out);
-  DevBuf* bufs[] = {&w->box_new, &w->stored, &w->gstack, &w->lbuf, &w->cnt, &w->offs, &w->counters, &w->pa, &w->pb,
+  DevBuf* bufs[] = {&w->box_new, &w->stored, &w->gstack, &w->qwork, &w->lbuf, &w->cnt, &w->offs, &w->counters, &w->pa, &w->pb,
                     &w->has, &w->w1, &w->w2, &w->nrm, &w->depth_out, &w->iters, &w->desc, &w->dirs};
   for (DevBuf* b : bufs) b->release();
   w->bvh.release();

@@ -11,7 +11,7 @@ from . import _lib
 
 
 class World:
-    def __init__(self, shapes, margin, prediction=0.0, max_pairs=1 << 20, rebuild_every=8):
+    def __init__(self, shapes, margin, prediction=0.0, max_pairs=1 << 20, rebuild_every=2):
         self.ctx = shapes.ctx
         self.shapes = shapes
         h = C.c_void_p()
