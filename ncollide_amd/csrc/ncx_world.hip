@@ -87,7 +87,7 @@ __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float 
 // Self-query of sorted leaf k against leaves at later positions (each pair is found once, by its
 // earlier leaf).  Depth-first with the stack's top in a register, the next WQ_LSTACK entries in
 // LDS and the rest in a per-thread spill area of `spill` entries (the tree depth bounds it).
-//   MODE_COUNT: count the pairs and keep the first WQ_LOCAL partners in lbuf;
+//   MODE_COUNT: count the pairs and keep the first WQ_LOCAL partners (shape ids) in lbuf;
 //   MODE_WRITE: write every pair at offs[k] (only for leaves with more than WQ_LOCAL pairs).
 // Persistent: a lane that finishes a leaf takes the next one at once from its wavefront's chunk of
 // WQ_CHUNK leaves (ballot + rank), and a wavefront whose chunk is used up takes the next chunk of
@@ -95,7 +95,7 @@ __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float 
 // its own cache line, so the atomics do not serialise on one line); a wave that finds its part
 // used up moves on to the next parts.
 #define WQ_LSTACK 16
-#define WQ_LOCAL 64
+#define WQ_LOCAL 128
 #define WQ_CHUNK 64u
 #define WQ_PARTS 8u
 #define WQ_PART_STRIDE 64u
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
       }
       w++;
     } else {
-      if (c < WQ_LOCAL) lbuf[(size_t)k * WQ_LOCAL + c] = leafpos;
+      if (c < WQ_LOCAL) lbuf[(size_t)k * WQ_LOCAL + c] = order[leafpos];
       c++;
     }
   };
@@ -247,21 +247,53 @@ static unsigned world_query_blocks(const ncx_ctx* c, size_t n) {
 
 // Pairs of the leaves with at most WQ_LOCAL pairs, copied from the count pass's buffer in the
 // order they were found (the write pass handles the others), so the list is the same as one
-// full write traversal per leaf would give.
-__global__ __launch_bounds__(256) void k_world_emit(const uint32_t* __restrict__ order, uint32_t n,
-                                                    const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ offs,
-                                                    const uint32_t* __restrict__ lbuf, size_t cap,
-                                                    uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
-                                                    uint32_t* __restrict__ counters) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const uint32_t c = cnt[k];
-  if (c == 0 || c > WQ_LOCAL) return;
-  const uint32_t prim = order[k];
-  size_t w = offs[k];
-  for (uint32_t i = 0; i < c; ++i, ++w) {
+// full write traversal per leaf would give.  A block takes WQ_EMIT_BLOCK consecutive leaves; its
+// threads walk the concatenation of those leaves' pairs (a block-wide prefix sum over the
+// counts, then a binary search for each pair's leaf), so consecutive threads write consecutive
+// pair slots.
+#define WQ_EMIT_BLOCK 256
+__global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __restrict__ order, uint32_t n,
+                                                              const uint32_t* __restrict__ cnt,
+                                                              const uint32_t* __restrict__ offs,
+                                                              const uint32_t* __restrict__ lbuf, size_t cap,
+                                                              uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
+                                                              uint32_t* __restrict__ counters) {
+  __shared__ uint32_t s_pre[WQ_EMIT_BLOCK + 1];
+  __shared__ uint32_t s_warp[WQ_EMIT_BLOCK / 64];
+  const uint32_t t = threadIdx.x;
+  const uint32_t k = blockIdx.x * WQ_EMIT_BLOCK + t;
+  uint32_t c = k < n ? cnt[k] : 0u;
+  if (c > WQ_LOCAL) c = 0;  // written by the write pass
+  // block-wide inclusive prefix sum of c: within each wavefront, then across wavefronts
+  uint32_t x = c;
+  const int lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) s_warp[t / 64] = x;
+  __syncthreads();
+  uint32_t wbase = 0;
+  for (uint32_t i = 0; i < t / 64; ++i) wbase += s_warp[i];
+  s_pre[t + 1] = wbase + x;
+  if (t == 0) s_pre[0] = 0;
+  __syncthreads();
+  const uint32_t total = s_pre[WQ_EMIT_BLOCK];
+  for (uint32_t j = t; j < total; j += WQ_EMIT_BLOCK) {
+    // the leaf l with s_pre[l] <= j < s_pre[l + 1]
+    uint32_t lo = 0, hi = WQ_EMIT_BLOCK;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (s_pre[mid] <= j) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t kl = blockIdx.x * WQ_EMIT_BLOCK + lo;
+    const uint32_t i = j - s_pre[lo];
+    const size_t w = (size_t)offs[kl] + i;
     if (w < cap) {
-      uint32_t h2 = order[lbuf[(size_t)k * WQ_LOCAL + i]];
+      const uint32_t prim = order[kl];
+      const uint32_t h2 = lbuf[(size_t)kl * WQ_LOCAL + i];
       pa[w] = prim < h2 ? prim : h2;
       pb[w] = prim < h2 ? h2 : prim;
     } else {
@@ -270,15 +302,22 @@ __global__ __launch_bounds__(256) void k_world_emit(const uint32_t* __restrict__
   }
 }
 
-__global__ void k_world_tree_depth(const uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ out) {
+// Tree depth: the longest leaf-to-root path, reduced within each wavefront before one atomicMax.
+__global__ __launch_bounds__(256) void k_world_tree_depth(const uint32_t* __restrict__ parent, uint32_t n,
+                                                          uint32_t* __restrict__ out) {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  uint32_t d = 1, p = parent[(n - 1) + k] & ~NCX_RIGHT_BIT;
-  while (p != 0 && d <= n) {
-    p = parent[p] & ~NCX_RIGHT_BIT;
-    d++;
+  uint32_t d = 0;
+  if (k < n) {
+    d = 1;
+    uint32_t p = parent[(n - 1) + k] & ~NCX_RIGHT_BIT;
+    while (p != 0 && d <= n) {
+      p = parent[p] & ~NCX_RIGHT_BIT;
+      d++;
+    }
   }
-  atomicMax(out, d);
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) d = max(d, (uint32_t)__shfl_xor((int)d, s, 64));
+  if (__lane_id() == 0 && d) atomicMax(out, d);
 }
 
 static int world_build(ncx_world* w) {
@@ -342,7 +381,7 @@ static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses, bool ea
     NCX_HIP_CHECK(hipGetLastError());
     if ((rc = scan_exclusive_u32(c, w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), n, w->bvh.scan_tmp, ctr)))
       return rc;
-    hipLaunchKernelGGL(k_world_emit, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, order, (uint32_t)n,
+    hipLaunchKernelGGL(k_world_emit, dim3(ceil_div(n, WQ_EMIT_BLOCK)), dim3(WQ_EMIT_BLOCK), 0, c->stream, order, (uint32_t)n,
                        w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), w->lbuf.as<uint32_t>(), w->cap,
                        w->pa.as<uint32_t>(), w->pb.as<uint32_t>(), ctr);
     NCX_HIP_CHECK(hipGetLastError());
