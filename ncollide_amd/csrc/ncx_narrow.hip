@@ -703,7 +703,7 @@ __device__ __forceinline__ bool can_be_seen(const Arena& A, const FaceD& f, uint
 
 // EPA::closest_points (epa3.rs:219-430).  compute_silhouette's recursion
 // (:432-454) becomes an explicit stack visiting faces in the same order.
-template <int G>
+template <int G, bool PAR>
 __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const Arena& A, f3& p1, f3& p2, f3& n,
                    int& iters) {
   const float eps_tol = NCX_FLT_EPS * 100.f;
@@ -815,20 +815,51 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
     }
     uint32_t first_new = nf;
     if (ns == 0) return EPA_NONE;
-    for (uint32_t q = 0; q < ns; ++q) {
-      SilE edge = A.sil[q];
-      if (!A.faces[edge.face].deleted) {
-        if (nf >= A.fcap || hl >= A.fcap) return EPA_OVERFLOW;
-        uint32_t new_id = nf;
-        uint32_t pt1 = A.faces[edge.face].pts[(edge.opp + 2) % 3];
-        uint32_t pt2 = A.faces[edge.face].pts[(edge.opp + 1) % 3];
-        FaceD nfc;
-        f3 nfn;
-        bool inside = make_face(A, pt1, pt2, sp_id, edge.face, new_id + 1, new_id - 1, nfc, nfn);
-        A.faces[edge.face].adj[(edge.opp + 1) % 3] = (uint16_t)new_id;
-        A.faces[nf++] = nfc;
-        if (inside) {
-          float dist = dot(nfn, vpt(A, nfc.pts[0]));
+    if constexpr (PAR) {
+      // The new faces are made in parallel, silhouette edge q by lane q mod G of the group, and
+      // then checked and pushed in edge order as the loop below does.  Which edges get a face
+      // (their silhouette face still live), and so every new face's id, is known before any face
+      // is made: making faces only appends faces and rewrites adjacency slots, never the deleted
+      // flags or vertex ids read here.  Each edge's results go to its silhouette entry (bit 14:
+      // has a face, bit 15: origin projects inside) and to the emptied search stack (the
+      // distance).  LDS arenas only: lanes of one wavefront see each other's LDS writes in order.
+      const int lane = __lane_id();
+      const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << (lane & ~(G - 1)));
+      const uint64_t below = (1ull << lane) - 1ull;
+      float* sdist = (float*)A.stack;
+      uint32_t nvalid = 0;
+      for (uint32_t base = 0; base < ns; base += G) {
+        const uint32_t q = base + (uint32_t)gl;
+        SilE edge = SilE{0, 0};
+        bool valid = false;
+        if (q < ns) {
+          edge = A.sil[q];
+          valid = !A.faces[edge.face].deleted;
+        }
+        const uint64_t vm = __ballot(valid) & gmask;
+        const uint32_t new_id = nf + nvalid + (uint32_t)__popcll(vm & below);
+        __builtin_amdgcn_wave_barrier();
+        if (valid && new_id < A.fcap) {
+          const uint32_t pt1 = A.faces[edge.face].pts[(edge.opp + 2) % 3];
+          const uint32_t pt2 = A.faces[edge.face].pts[(edge.opp + 1) % 3];
+          FaceD nfc;
+          f3 nfn;
+          const bool inside = make_face(A, pt1, pt2, sp_id, edge.face, new_id + 1, new_id - 1, nfc, nfn);
+          A.faces[edge.face].adj[(edge.opp + 1) % 3] = (uint16_t)new_id;
+          A.faces[new_id] = nfc;
+          sdist[q] = inside ? dot(nfn, vpt(A, pt1)) : 0.f;
+          A.sil[q].opp = (uint16_t)(edge.opp | 0x4000u | (inside ? 0x8000u : 0u));
+        }
+        nvalid += (uint32_t)__popcll(vm);
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (nf + nvalid > A.fcap) return EPA_OVERFLOW;
+      for (uint32_t q = 0; q < ns; ++q) {
+        const uint32_t fl = A.sil[q].opp;
+        if (!(fl & 0x4000u)) continue;
+        const uint32_t new_id = nf++;
+        if (fl & 0x8000u) {
+          const float dist = sdist[q];
           if (dist < curr_dist) {
             face_points(A, face, p1, p2);
             n = face_n;
@@ -838,6 +869,33 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
           A.heap[hl] = HeapE{-dist, new_id};
           heap_sift_up(A.heap, hl);
           hl++;
+        }
+      }
+    } else {
+      for (uint32_t q = 0; q < ns; ++q) {
+        SilE edge = A.sil[q];
+        if (!A.faces[edge.face].deleted) {
+          if (nf >= A.fcap || hl >= A.fcap) return EPA_OVERFLOW;
+          uint32_t new_id = nf;
+          uint32_t pt1 = A.faces[edge.face].pts[(edge.opp + 2) % 3];
+          uint32_t pt2 = A.faces[edge.face].pts[(edge.opp + 1) % 3];
+          FaceD nfc;
+          f3 nfn;
+          bool inside = make_face(A, pt1, pt2, sp_id, edge.face, new_id + 1, new_id - 1, nfc, nfn);
+          A.faces[edge.face].adj[(edge.opp + 1) % 3] = (uint16_t)new_id;
+          A.faces[nf++] = nfc;
+          if (inside) {
+            float dist = dot(nfn, vpt(A, nfc.pts[0]));
+            if (dist < curr_dist) {
+              face_points(A, face, p1, p2);
+              n = face_n;
+              return EPA_OK;
+            }
+            if (-dist > gjk_tol) return EPA_NONE;
+            A.heap[hl] = HeapE{-dist, new_id};
+            heap_sift_up(A.heap, hl);
+            hl++;
+          }
         }
       }
     }
@@ -862,7 +920,7 @@ struct PairOut {
 };
 
 // contact_support_map_support_map_with_params (contact_support_map_support_map.rs:38-88)
-template <int G>
+template <int G, bool PAR>
 __device__ __forceinline__ PairOut contact_sm(const Ctx2& c, float prediction, bool has_init, f3 init_dir, const Pool& P,
                               int gl, const Arena& A, int max_iter) {
   PairOut o;
@@ -887,7 +945,7 @@ __device__ __forceinline__ PairOut contact_sm(const Ctx2& c, float prediction, b
   if (r.status == GJK_S_CLOSEST) { w1 = r.p1; w2 = r.p2; n = r.dir; }
   else if (r.status == GJK_S_INTERSECTION) {
     int it;
-    int e = epa<G>(c, s, P, gl, A, w1, w2, n, it);
+    int e = epa<G, PAR>(c, s, P, gl, A, w1, w2, n, it);
     if (e == EPA_OVERFLOW) { o.has = 2; return o; }
     if (e != EPA_OK) return o;
   } else {
@@ -959,7 +1017,7 @@ __device__ __forceinline__ f3 cuboid_feature_normal(int fkind, int fid) {
 }
 
 // contact_ball_convex_polyhedron (contact_ball_convex_polyhedron.rs:11-58), Cuboid or ConvexHull.
-template <int G>
+template <int G, bool PAR>
 __device__ __forceinline__ PairOut contact_ball_convex(f3 c1, float r, const iso& m2, const ShapeD& g2, float prediction,
                                        const Pool& P, int gl, const Arena& A) {
   PairOut o;
@@ -992,7 +1050,7 @@ __device__ __forceinline__ PairOut contact_ball_convex(f3 c1, float r, const iso
     } else {
       f3 p1, p2, nn;
       int it;
-      int e = r2.status == GJK_S_INTERSECTION ? epa<G>(c, s, P, gl, A, p1, p2, nn, it) : EPA_NONE;
+      int e = r2.status == GJK_S_INTERSECTION ? epa<G, PAR>(c, s, P, gl, A, p1, p2, nn, it) : EPA_NONE;
       if (e == EPA_OVERFLOW) { o.has = 2; return o; }
       inside = true;
       world2 = e == EPA_OK ? p1 + c1 : c1;
@@ -1021,7 +1079,7 @@ __device__ __forceinline__ PairOut contact_ball_convex(f3 c1, float r, const iso
 }
 
 // query::contact dispatch (contact_shape_shape.rs:10-48).
-template <int G>
+template <int G, bool PAR>
 __device__ __forceinline__ PairOut contact_pair(const Ctx2& c, float prediction, bool has_init, f3 init_dir, const Pool& P,
                                 int gl, const Arena& A, int max_iter) {
   bool b1 = c.g1.type == NCX_BALL, b2 = c.g2.type == NCX_BALL;
@@ -1046,16 +1104,16 @@ __device__ __forceinline__ PairOut contact_pair(const Ctx2& c, float prediction,
     }
     return o;
   }
-  if (b1 && poly2) return contact_ball_convex<G>(c.m1.t, c.g1.a, c.m2, c.g2, prediction, P, gl, A);
+  if (b1 && poly2) return contact_ball_convex<G, PAR>(c.m1.t, c.g1.a, c.m2, c.g2, prediction, P, gl, A);
   if (poly1 && b2) {
-    PairOut o = contact_ball_convex<G>(c.m2.t, c.g2.a, c.m1, c.g1, prediction, P, gl, A);
+    PairOut o = contact_ball_convex<G, PAR>(c.m2.t, c.g2.a, c.m1, c.g1, prediction, P, gl, A);
     if (o.has == 1) {  // Contact::flip (contact.rs:52-55)
       f3 t = o.w1; o.w1 = o.w2; o.w2 = t;
       o.n = -o.n;
     }
     return o;
   }
-  return contact_sm<G>(c, prediction, has_init, init_dir, P, gl, A, max_iter);
+  return contact_sm<G, PAR>(c, prediction, has_init, init_dir, P, gl, A, max_iter);
 }
 
 struct KArgs {
@@ -1191,12 +1249,12 @@ __global__ __launch_bounds__(BS, MODE == 1 ? 3 : 2) void k_pairs(KArgs a) {
       // An all-zero entry means None for this pair (the reference's default direction).
       has_init = idir.x != 0.f || idir.y != 0.f || idir.z != 0.f;
     }
-    PairOut o = contact_pair<G>(c, a.prediction, has_init, idir, a.P, gl, A, has_init ? a.max_iter : 10000);
+    PairOut o = contact_pair<G, LDS>(c, a.prediction, has_init, idir, a.P, gl, A, has_init ? a.max_iter : 10000);
     int extra_iters = 0;
     if (o.has == 3) {
       // Warm start hit its iteration cap: plain GJK (init_dir = None) judges the pair.
       extra_iters = o.gjk_iters;
-      o = contact_pair<G>(c, a.prediction, false, idir, a.P, gl, A, 10000);
+      o = contact_pair<G, LDS>(c, a.prediction, false, idir, a.P, gl, A, 10000);
     }
     if (o.has == 2) {
       if (gl == 0) {
