@@ -270,6 +270,43 @@ def test_contact_mixed_shapes(ctx, orc):
         assert_contacts_match(g, r, f"mixed shapes prediction {pred}")
 
 
+def test_long_epa_runs_through_every_lds_tier(ctx, orc):
+    # Isolated Ball-Capsule and Capsule-Capsule pairs at random overlaps: curved surfaces make EPA
+    # run long (tens to ~1,000 iterations), so pairs finish in every arena tier (44, 128, 512 and
+    # 1,536 vertices).  Every output must be bit-identical to the f32 oracle.
+    rng = np.random.default_rng(11)
+    n = 400000
+    rec = np.zeros(2 * n, workloads.SHAPE_DTYPE)
+    kind = rng.integers(0, 2, n)
+    rec["type"][0::2] = np.where(kind == 0, workloads.BALL, workloads.CAPSULE)
+    rec["type"][1::2] = workloads.CAPSULE
+    rec["p0"][0::2] = rng.uniform(0.1, 0.5, n)
+    rec["p0"][1::2] = rng.uniform(0.1, 0.5, n)
+    rec["p1"][1::2] = rng.uniform(0.05, 0.3, n)
+    rec["p1"][0::2] = np.where(kind == 1, rng.uniform(0.05, 0.3, n), 0.0)
+    t = np.zeros((2 * n, 3), np.float32)
+    t[1::2] = (workloads.unit_vectors(rng, n) * rng.uniform(0.0, 0.6, (n, 1))).astype(np.float32)
+    poses8 = workloads.poses8(t, workloads.uniform_quats(rng, 2 * n))
+    pool = np.zeros((1, 3), np.float32)
+    ia = np.arange(0, 2 * n, 2, dtype=np.uint32)
+    ib = ia + 1
+    r = orc.contact(oracle_shapes(orc, rec, pool), to_oracle_poses(poses8), ia, ib, 0.0, nthreads=8)
+    e = r["epa_iters"]
+    tiers = [int(np.sum((e > a) & (e <= b))) for a, b in ((0, 40), (40, 124), (124, 508), (508, 1532))]
+    assert tiers[3] > 0 and tiers[2] > 0, tiers
+    sel = np.nonzero(e > 40)[0]
+    sel = np.concatenate([sel, rng.choice(np.nonzero(e <= 40)[0], 20000, replace=False)])
+    ss = ncx.ShapeSet(records=rec, pool=pool, ctx=ctx)
+    g = ncx.query.contact(ss, torch.as_tensor(poses8, device=ctx.device), ia[sel], ib[sel], 0.0)
+    assert np.array_equal(g["has_contact"].cpu().numpy(), r["has"][sel])
+    for k, rk in (("depth", "depth"), ("normal", "normal"), ("world1", "world1"), ("world2", "world2")):
+        gv = g[k].cpu().numpy()
+        rv = r[rk][sel]
+        bad = np.nonzero(np.any((gv != rv).reshape(len(sel), -1), 1))[0]
+        assert len(bad) == 0, f"{k} differs on {len(bad)} pairs, EPA iterations {e[sel][bad][:10]}"
+    print(f"pairs by EPA iterations (<=40, <=124, <=508, <=1532): {tiers}, checked {len(sel)}, max {e.max()}")
+
+
 def test_distance_mixed_shapes(ctx, orc):
     w = workloads.full_pipeline(seed=6, n=2000, box=20.0, nclusters=2)
     ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
