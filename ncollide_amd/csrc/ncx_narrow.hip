@@ -791,26 +791,71 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
     A.faces[fid.id].deleted = 1;
     // compute_silhouette for the three neighbours, depth first.
     uint32_t ns = 0, sp = 0;
-    for (int k = 2; k >= 0; --k) {
-      uint32_t adj = face.adj[k];
-      A.stack[sp++] = SilE{(uint16_t)adj, (uint16_t)next_ccw(A.faces[adj], face.pts[k])};
-    }
-    while (sp > 0) {
-      SilE e = A.stack[--sp];
-      FaceD& fa = A.faces[e.face];
-      if (fa.deleted) continue;
-      if (!can_be_seen(A, fa, sp_id, e.opp)) {
-        if (ns >= A.scap) return EPA_OVERFLOW;
-        A.sil[ns++] = e;
-      } else {
-        fa.deleted = 1;
-        uint32_t i1 = (e.opp + 2) % 3, i2 = e.opp;
-        uint32_t adj1 = fa.adj[i1], adj2 = fa.adj[i2];
-        uint32_t o1 = next_ccw(A.faces[adj1], fa.pts[i1]);
-        uint32_t o2 = next_ccw(A.faces[adj2], fa.pts[i2]);
-        if (sp + 2 > A.stcap) return EPA_OVERFLOW;
-        A.stack[sp++] = SilE{(uint16_t)adj2, (uint16_t)o2};
-        A.stack[sp++] = SilE{(uint16_t)adj1, (uint16_t)o1};
+    if constexpr (PAR) {
+      // Each stack entry's visibility (can_be_seen through the entry's edge) is computed when it
+      // is pushed, the entries pushed together by different lanes of the group at once, and kept
+      // in bit 15 of the entry: it depends only on the face's vertices, the new point and the
+      // edge, none of which change during the search.  The pops, the deleted-face skips and the
+      // pushes are the search's own, so the silhouette comes out in the same order.
+      const int lane = __lane_id();
+      const int gbase = lane & ~(G - 1);
+      {
+        const int k = 2 - (gl % 3);  // lanes 0, 1, 2 take the entries in push order
+        // value selects, not an index into the register copy of the face (that would spill it)
+        const uint32_t adj = k == 0 ? face.adj[0] : (k == 1 ? face.adj[1] : face.adj[2]);
+        const uint32_t ptk = k == 0 ? face.pts[0] : (k == 1 ? face.pts[1] : face.pts[2]);
+        const bool v = can_be_seen(A, A.faces[adj], sp_id, next_ccw(A.faces[adj], ptk));
+        const uint64_t vb = __ballot(v) >> gbase;
+        for (int j = 0; j < 3; ++j) {
+          const uint32_t adjj = face.adj[2 - j];
+          const uint32_t oj = next_ccw(A.faces[adjj], face.pts[2 - j]);
+          A.stack[sp++] = SilE{(uint16_t)adjj, (uint16_t)(oj | (((vb >> j) & 1ull) ? 0x8000u : 0u))};
+        }
+      }
+      while (sp > 0) {
+        const SilE e = A.stack[--sp];
+        const uint32_t opp = e.opp & 3u;
+        FaceD& fa = A.faces[e.face];
+        if (fa.deleted) continue;
+        if (!(e.opp & 0x8000u)) {
+          if (ns >= A.scap) return EPA_OVERFLOW;
+          A.sil[ns++] = SilE{e.face, (uint16_t)opp};
+        } else {
+          fa.deleted = 1;
+          const uint32_t i1 = (opp + 2) % 3, i2 = opp;
+          const uint32_t adj1 = fa.adj[i1], adj2 = fa.adj[i2];
+          const uint32_t o1 = next_ccw(A.faces[adj1], fa.pts[i1]);
+          const uint32_t o2 = next_ccw(A.faces[adj2], fa.pts[i2]);
+          if (sp + 2 > A.stcap) return EPA_OVERFLOW;
+          const bool odd = (gl & 1) != 0;  // even lanes: adj2 (pushed first), odd lanes: adj1
+          const bool v = can_be_seen(A, A.faces[odd ? adj1 : adj2], sp_id, odd ? o1 : o2);
+          const uint64_t vb = __ballot(v) >> gbase;
+          A.stack[sp++] = SilE{(uint16_t)adj2, (uint16_t)(o2 | ((vb & 1ull) ? 0x8000u : 0u))};
+          A.stack[sp++] = SilE{(uint16_t)adj1, (uint16_t)(o1 | ((vb & 2ull) ? 0x8000u : 0u))};
+        }
+      }
+    } else {
+      for (int k = 2; k >= 0; --k) {
+        uint32_t adj = face.adj[k];
+        A.stack[sp++] = SilE{(uint16_t)adj, (uint16_t)next_ccw(A.faces[adj], face.pts[k])};
+      }
+      while (sp > 0) {
+        SilE e = A.stack[--sp];
+        FaceD& fa = A.faces[e.face];
+        if (fa.deleted) continue;
+        if (!can_be_seen(A, fa, sp_id, e.opp)) {
+          if (ns >= A.scap) return EPA_OVERFLOW;
+          A.sil[ns++] = e;
+        } else {
+          fa.deleted = 1;
+          uint32_t i1 = (e.opp + 2) % 3, i2 = e.opp;
+          uint32_t adj1 = fa.adj[i1], adj2 = fa.adj[i2];
+          uint32_t o1 = next_ccw(A.faces[adj1], fa.pts[i1]);
+          uint32_t o2 = next_ccw(A.faces[adj2], fa.pts[i2]);
+          if (sp + 2 > A.stcap) return EPA_OVERFLOW;
+          A.stack[sp++] = SilE{(uint16_t)adj2, (uint16_t)o2};
+          A.stack[sp++] = SilE{(uint16_t)adj1, (uint16_t)o1};
+        }
       }
     }
     uint32_t first_new = nf;
