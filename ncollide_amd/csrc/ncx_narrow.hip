@@ -1677,7 +1677,8 @@ int contact_tiers_reserve_list(ContactTiers& t, size_t npairs) {
 }
 
 // Autotuner: for each bucket with enough pairs and no choice yet, time every first-tier variant
-// on a sample of the bucket's pairs (second of two runs, HIP events) and keep the fastest.  The
+// on a sample of the bucket's pairs (HIP events; the least of three rounds after a warm-up
+// round, the variants interleaved within each round) and keep the fastest.  The
 // sample's results are real results (the full launch recomputes them identically).  Host-driven;
 // the choice is kept in the shape set for every later call and scene step.
 // Clears the tiers' counts and work counters (not the buckets').
@@ -1689,7 +1690,8 @@ static int clear_tier_counters(ncx_ctx* ctx, uint32_t* hdr) {
 }
 
 #define TUNE_MIN_PAIRS 4096u
-#define TUNE_SAMPLE 32768u
+#define TUNE_SAMPLE 131072u
+#define TUNE_ROUNDS 4
 static int autotune(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const KArgs& base, int mode) {
   uint32_t* hdr = T.ovf.as<uint32_t>();
   uint32_t rng[2 * NCX_NBUCKETS];
@@ -1706,15 +1708,15 @@ static int autotune(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const KArgs& b
     if (S->tune[mode][b] >= 0 || cnt < TUNE_MIN_PAIRS) continue;
     const uint32_t sr[2] = {rng[2 * b], rng[2 * b] + std::min(cnt, TUNE_SAMPLE)};
     NCX_HIP_CHECK(hipMemcpyAsync(dr.p, sr, 8, hipMemcpyHostToDevice, ctx->stream));
-    int best = 1;
-    float best_ms = 0.f;
-    for (int v = 0; v < 3; ++v) {
-      KArgs a = base;
-      a.range_b = dr.as<uint32_t>();
-      a.range_e = dr.as<uint32_t>() + 1;
-      a.work = ncx_work_set(hdr, NCX_WORK_TUNE);
-      float ms = 0.f;
-      for (int rep = 0; rep < 2; ++rep) {
+    // Rounds over the three variants, so clock ramps and noise fall on every variant alike; the
+    // first round warms up, each variant keeps its least time of the others.
+    float ms_v[3] = {0.f, 0.f, 0.f};
+    for (int rep = 0; rep < TUNE_ROUNDS && !rc; ++rep) {
+      for (int v = 0; v < 3; ++v) {
+        KArgs a = base;
+        a.range_b = dr.as<uint32_t>();
+        a.range_e = dr.as<uint32_t>() + 1;
+        a.work = ncx_work_set(hdr, NCX_WORK_TUNE);
         if ((rc = clear_tier_counters(ctx, hdr))) break;
         NCX_HIP_CHECK(hipEventRecord(e0, ctx->stream));
         if ((rc = launch_t1(v, mode == 1, ctx->stream, a, (unsigned)ctx->num_cus * T1_BLOCKS_PER_CU[mode][v]))) break;
@@ -1722,11 +1724,15 @@ static int autotune(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const KArgs& b
         if (mode == 0 && (rc = launch_t2(ctx, T, a))) break;
         NCX_HIP_CHECK(hipEventRecord(e1, ctx->stream));
         NCX_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
         NCX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (rep == 1 || (rep > 1 && ms < ms_v[v])) ms_v[v] = ms;
       }
-      if (rc) break;
-      S->tune_ms[mode][b][v] = ms;
-      if (v == 0 || ms < best_ms) { best = v; best_ms = ms; }
+    }
+    int best = 1;
+    for (int v = 0; v < 3 && !rc; ++v) {
+      S->tune_ms[mode][b][v] = ms_v[v];
+      if (ms_v[v] < ms_v[best]) best = v;
     }
     if (rc) break;
     S->tune[mode][b] = (int8_t)best;
