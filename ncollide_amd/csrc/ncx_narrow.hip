@@ -599,7 +599,34 @@ __device__ __forceinline__ void heap_sift_up(HeapE* h, uint32_t pos) {
   }
   h[pos] = e;
 }
-__device__ __forceinline__ bool heap_pop(HeapE* h, uint32_t& len, HeapE& out) {
+// heap_sift_up with the group's lanes: lane j reads the element's j-th ancestor (0 = the parent;
+// in 1-based positions ancestor j of k is k >> (j + 1)), the sequential loop's break test
+// `e <= ancestor` is evaluated on every ancestor at once, and the first ancestor where it holds
+// (nearest first) is where the climb stops.  The ancestors below it each move down one level
+// along the path and the element takes the last one's slot, which is what the loop leaves.
+// Needs as many lanes as the heap has levels above pos.
+template <int G>
+__device__ __forceinline__ void heap_sift_up_par(HeapE* h, uint32_t pos, int gl) {
+  const HeapE e = h[pos];
+  const uint32_t k = pos + 1;
+  const uint32_t depth = 31u - (uint32_t)__builtin_clz(k);  // ancestors of pos
+  const bool have = (uint32_t)gl < depth;
+  const uint32_t anc = have ? (k >> (gl + 1)) - 1u : 0u;
+  HeapE a = HeapE{0.f, 0u};
+  if (have) a = h[anc];
+  const bool stop = have && e.neg_dist <= a.neg_dist;
+  const int gbase = __lane_id() & ~(G - 1);
+  const uint64_t sm = (__ballot(stop) >> gbase) & (G == 64 ? ~0ull : ((1ull << G) - 1ull));
+  const uint32_t s = sm ? (uint32_t)__builtin_ctzll(sm) : depth;
+  __builtin_amdgcn_wave_barrier();
+  if ((uint32_t)gl < s) h[gl == 0 ? pos : (k >> gl) - 1u] = a;
+  __builtin_amdgcn_wave_barrier();
+  if (gl == 0) h[s == 0 ? pos : (k >> s) - 1u] = e;
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int G, bool PAR>
+__device__ __forceinline__ bool heap_pop(HeapE* h, uint32_t& len, HeapE& out, int gl) {
   if (len == 0) return false;
   len -= 1;
   HeapE item = h[len];
@@ -617,7 +644,8 @@ __device__ __forceinline__ bool heap_pop(HeapE* h, uint32_t& len, HeapE& out) {
     }
     if (child == end - 1) { h[pos] = h[child]; pos = child; }
     h[pos] = e;
-    heap_sift_up(h, pos);
+    if constexpr (PAR) heap_sift_up_par<G>(h, pos, gl);
+    else heap_sift_up(h, pos);
   }
   out = item;
   return true;
@@ -770,7 +798,7 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
   float max_dist = NCX_FLT_MAX;
   HeapE best = A.heap[0];
   HeapE fid;
-  while (heap_pop(A.heap, hl, fid)) {
+  while (heap_pop<G, PAR>(A.heap, hl, fid, gl)) {
     FaceD face = A.faces[fid.id];
     if (face.deleted) continue;
     if (nv >= A.vcap) return EPA_OVERFLOW;
@@ -912,7 +940,7 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
           }
           if (-dist > gjk_tol) return EPA_NONE;
           A.heap[hl] = HeapE{-dist, new_id};
-          heap_sift_up(A.heap, hl);
+          heap_sift_up_par<G>(A.heap, hl, gl);
           hl++;
         }
       }
@@ -1198,6 +1226,8 @@ struct KArgs {
 // entries) lives in LDS; a pair whose polytope outgrows it is listed in ovf for a launch with
 // larger arenas.  VC == 0: the arena is the group's slice of a.arena (a.vcap, a.fcap) in device
 // memory.
+constexpr uint32_t ilog2_floor(uint32_t x) { return x <= 1 ? 0u : 1u + ilog2_floor(x >> 1); }
+
 template <int G, int BS, uint32_t VC, uint32_t FC, uint32_t SC, uint32_t STC, int MODE>
 __global__ __launch_bounds__(BS, MODE == 1 ? 3 : 2) void k_pairs(KArgs a) {
   const int gl = threadIdx.x & (G - 1);
@@ -1207,6 +1237,7 @@ __global__ __launch_bounds__(BS, MODE == 1 ? 3 : 2) void k_pairs(KArgs a) {
   constexpr size_t LDS_BYTES = LDS ? (BS / G) * arena_bytes(VC, FC, SC, STC) : 16;
   static_assert(LDS_BYTES <= 163840, "LDS arena exceeds the 160 KiB a gfx950 workgroup may hold");
   static_assert(!LDS || (VC < 65536 && FC < 65536 && SC >= 3 && STC >= 3), "arena capacities");
+  static_assert(!LDS || ilog2_floor(FC) <= (uint32_t)G, "the parallel heap climb needs a lane per heap level");
   __shared__ __align__(16) uint8_t lds_arena[LDS_BYTES];
   Arena A;
   if constexpr (LDS) A = arena_at(lds_arena, threadIdx.x / G, VC, FC, SC, STC);
