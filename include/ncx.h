@@ -152,7 +152,9 @@ int ncx_trimesh_destroy(ncx_mesh* mesh);
  * n rays (device SoA origins ox/oy/oz and directions dx/dy/dz), mesh at `pose`.
  * Hit: toi, world normal (3 floats per ray), feature = face id, or face id +
  * ntriangles for a back-face hit (ray_trimesh.rs:41-47).  Miss: feature =
- * UINT32_MAX, toi = -1.  Among hits with equal toi the smallest feature wins. */
+ * UINT32_MAX, toi = -1.  The hit kept is the least (toi, feature) over all triangles; the
+ * reference's best-first search (src/partitioning/bvh.rs:101-160) can stop on a hit 1-2 ulps
+ * farther when a ray passes exactly through a mesh vertex. */
 int ncx_trimesh_cast_rays(ncx_mesh* mesh, const ncx_pose* pose, const float* const ray_soa_dev[6], size_t n,
                           float max_toi, float* toi_dev, float* normal_dev, uint32_t* feature_dev);
 /* Same as ncx_trimesh_cast_rays, but only enqueues the kernel on the context stream (no host
@@ -238,6 +240,7 @@ int ncx_world_destroy(ncx_world* world);
 /* Learned init_dir (ncx_mlp) for polytope-polytope pairs, GJK capped at 32 iterations with plain
  * GJK on the cap; NULL turns it off. */
 int ncx_world_set_warmstart(ncx_world* world, ncx_mlp* mlp);
+/* LBVH rebuilt every k >= 1 steps and refit in between (default 2). */
 int ncx_world_set_rebuild_every(ncx_world* world, int k);
 int ncx_world_step(ncx_world* world, const ncx_pose* poses_dev, int use_graph);
 /* No reference counterpart: turns each step's narrow phase on (default) or off, so the broad
