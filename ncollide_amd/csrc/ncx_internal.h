@@ -101,6 +101,7 @@ struct ContactTiers {
   size_t bcap = 0;   // entries of blist
   uint32_t cap = 0;  // entries per overflow list
   KArgs* last = nullptr;
+  DevBuf sus[4];     // saved EPA runs handed on: T1 -> T2, T2 -> T3, T3 -> T4, T4 -> device-memory tier
   uint32_t* t1_list() const { return ovf.as<uint32_t>() + NCX_OVF_HEADER; }
   uint32_t* t2_list() const { return ovf.as<uint32_t>() + NCX_OVF_HEADER + cap; }
   uint32_t* t3_list() const { return ovf.as<uint32_t>() + NCX_OVF_HEADER + 2 * (size_t)cap; }

@@ -729,15 +729,89 @@ __device__ __forceinline__ bool can_be_seen(const Arena& A, const FaceD& f, uint
   return dot(pt - p0, n) >= -(NCX_FLT_EPS * 10.f) || affinely_dependent(p1, p2, pt);
 }
 
+// Suspending and resuming an EPA run between arena tiers.  A run that is about to outgrow its
+// arena at an iteration boundary (the vertex capacity reached, or fewer than SUSPEND_MARGIN free
+// face slots) saves its polytope -- vertices, faces, heap and loop scalars, bit for bit -- to
+// slot i of the next tier's input, where i is its entry in the overflow list, and marks the
+// entry with ENTRY_SAVED.  The next tier replays the pair's GJK, which rebuilds the same simplex,
+// and continues the EPA from the saved state instead of from that simplex, so only a run that
+// outgrows a tier mid-iteration (a silhouette longer than the margin), or finds no free slot,
+// starts again.
+struct EpaIO {
+  const uint8_t* load;  // saved run to continue (null: build the polytope from the simplex)
+  uint8_t* save;        // slots for runs this launch suspends (null: it does not suspend)
+  size_t save_bytes;    // bytes per slot
+  uint32_t save_cap;    // slots
+  uint32_t* count;      // the overflow list's counter and entries
+  uint32_t* list;
+  uint32_t list_cap;
+  uint32_t k;           // the pair's index
+};
+#define EPA_SUSPENDED 3
+#define SUSPEND_MARGIN 16u
+#define ENTRY_SAVED 0x80000000u
+#define SAVE_HEADER_WORDS 8u
+
+__host__ __device__ constexpr size_t epa_save_bytes(uint32_t vcap, uint32_t fcap) {
+  return 4 * (size_t)SAVE_HEADER_WORDS + (size_t)vcap * sizeof(VertA) + (size_t)fcap * (sizeof(FaceD) + sizeof(HeapE));
+}
+
+// Word copy split over the group's lanes.
+template <int G>
+__device__ __forceinline__ void copy_words(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint32_t n,
+                                           int gl) {
+  for (uint32_t w = (uint32_t)gl; w < n; w += G) dst[w] = src[w];
+}
+
+template <int G>
+__device__ __forceinline__ void epa_save(uint8_t* slot, const Arena& A, int gl, uint32_t nv, uint32_t nf, uint32_t hl,
+                                         int niter, int iters, HeapE best, float max_dist) {
+  uint32_t* d = (uint32_t*)slot;
+  if (gl == 0) {
+    d[0] = nv; d[1] = nf; d[2] = hl; d[3] = (uint32_t)niter; d[4] = (uint32_t)iters;
+    d[5] = __float_as_uint(best.neg_dist); d[6] = best.id; d[7] = __float_as_uint(max_dist);
+  }
+  d += SAVE_HEADER_WORDS;
+  copy_words<G>(d, (const uint32_t*)A.verts, nv * (uint32_t)(sizeof(VertA) / 4), gl);
+  d += nv * (sizeof(VertA) / 4);
+  copy_words<G>(d, (const uint32_t*)A.faces, nf * (uint32_t)(sizeof(FaceD) / 4), gl);
+  d += nf * (sizeof(FaceD) / 4);
+  copy_words<G>(d, (const uint32_t*)A.heap, hl * (uint32_t)(sizeof(HeapE) / 4), gl);
+}
+
+// Loads a saved run into A (whose capacities are at least the saving tier's); the caller
+// orders these writes before the reads of other lanes.
+template <int G>
+__device__ __forceinline__ void epa_load(const uint8_t* slot, const Arena& A, int gl, uint32_t& nv, uint32_t& nf,
+                                         uint32_t& hl, int& niter, int& iters, HeapE& best, float& max_dist) {
+  const uint32_t* s = (const uint32_t*)slot;
+  nv = s[0]; nf = s[1]; hl = s[2]; niter = (int)s[3]; iters = (int)s[4];
+  best = HeapE{__uint_as_float(s[5]), s[6]};
+  max_dist = __uint_as_float(s[7]);
+  s += SAVE_HEADER_WORDS;
+  copy_words<G>((uint32_t*)A.verts, s, nv * (uint32_t)(sizeof(VertA) / 4), gl);
+  s += nv * (sizeof(VertA) / 4);
+  copy_words<G>((uint32_t*)A.faces, s, nf * (uint32_t)(sizeof(FaceD) / 4), gl);
+  s += nf * (sizeof(FaceD) / 4);
+  copy_words<G>((uint32_t*)A.heap, s, hl * (uint32_t)(sizeof(HeapE) / 4), gl);
+}
+
 // EPA::closest_points (epa3.rs:219-430).  compute_silhouette's recursion
 // (:432-454) becomes an explicit stack visiting faces in the same order.
 template <int G, bool PAR>
 __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const Arena& A, f3& p1, f3& p2, f3& n,
-                   int& iters) {
+                   int& iters, const EpaIO& io) {
   const float eps_tol = NCX_FLT_EPS * 100.f;
   const float gjk_tol = NCX_FLT_EPS * 10.f;
   iters = 0;
   uint32_t nv = 0, nf = 0, hl = 0;
+  int niter = 0;
+  float max_dist = NCX_FLT_MAX;
+  HeapE best;
+  if (io.load) {
+    epa_load<G>(io.load, A, gl, nv, nf, hl, niter, iters, best, max_dist);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  } else {
   vset(A, nv++, s.v0);
   if (s.dim >= 1) vset(A, nv++, s.v1);
   if (s.dim >= 2) vset(A, nv++, s.v2);
@@ -794,11 +868,22 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
     A.heap[hl] = HeapE{0.f, 1u}; heap_sift_up(A.heap, hl); hl++;
   }
   if (hl == 0) return EPA_NONE;
-  int niter = 0;
-  float max_dist = NCX_FLT_MAX;
-  HeapE best = A.heap[0];
+  best = A.heap[0];
+  }
   HeapE fid;
-  while (heap_pop<G, PAR>(A.heap, hl, fid, gl)) {
+  for (;;) {
+    if constexpr (PAR) {
+      if (io.save && (nv >= A.vcap || nf + SUSPEND_MARGIN > A.fcap)) {
+        uint32_t slot = 0;
+        if (gl == 0) slot = atomicAdd(io.count, 1u);
+        slot = (uint32_t)__shfl((int)slot, __lane_id() & ~(G - 1), 64);
+        const bool saved = slot < io.save_cap;
+        if (saved) epa_save<G>(io.save + (size_t)slot * io.save_bytes, A, gl, nv, nf, hl, niter, iters, best, max_dist);
+        if (gl == 0 && slot < io.list_cap) io.list[slot] = io.k | (saved ? ENTRY_SAVED : 0u);
+        return EPA_SUSPENDED;
+      }
+    }
+    if (!heap_pop<G, PAR>(A.heap, hl, fid, gl)) break;
     FaceD face = A.faces[fid.id];
     if (face.deleted) continue;
     if (nv >= A.vcap) return EPA_OVERFLOW;
@@ -986,7 +1071,7 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
 
 // ------------------------------------------------------------------ per-pair queries
 struct PairOut {
-  int has;  // 1 = Some(contact); 0 = None; 2 = arena overflow (re-run)
+  int has;  // 1 = Some(contact); 0 = None; 2 = arena overflow (re-run); 3 = warm-start cap; 4 = suspended (listed)
   f3 w1, w2, n;
   float depth;
   int gjk_iters;
@@ -995,7 +1080,7 @@ struct PairOut {
 // contact_support_map_support_map_with_params (contact_support_map_support_map.rs:38-88)
 template <int G, bool PAR>
 __device__ __forceinline__ PairOut contact_sm(const Ctx2& c, float prediction, bool has_init, f3 init_dir, const Pool& P,
-                              int gl, const Arena& A, int max_iter) {
+                              int gl, const Arena& A, int max_iter, const EpaIO& io) {
   PairOut o;
   o.has = 0;
   o.gjk_iters = 0;
@@ -1018,8 +1103,9 @@ __device__ __forceinline__ PairOut contact_sm(const Ctx2& c, float prediction, b
   if (r.status == GJK_S_CLOSEST) { w1 = r.p1; w2 = r.p2; n = r.dir; }
   else if (r.status == GJK_S_INTERSECTION) {
     int it;
-    int e = epa<G, PAR>(c, s, P, gl, A, w1, w2, n, it);
+    int e = epa<G, PAR>(c, s, P, gl, A, w1, w2, n, it, io);
     if (e == EPA_OVERFLOW) { o.has = 2; return o; }
+    if (e == EPA_SUSPENDED) { o.has = 4; return o; }
     if (e != EPA_OK) return o;
   } else {
     return o;
@@ -1092,7 +1178,7 @@ __device__ __forceinline__ f3 cuboid_feature_normal(int fkind, int fid) {
 // contact_ball_convex_polyhedron (contact_ball_convex_polyhedron.rs:11-58), Cuboid or ConvexHull.
 template <int G, bool PAR>
 __device__ __forceinline__ PairOut contact_ball_convex(f3 c1, float r, const iso& m2, const ShapeD& g2, float prediction,
-                                       const Pool& P, int gl, const Arena& A) {
+                                       const Pool& P, int gl, const Arena& A, const EpaIO& io) {
   PairOut o;
   o.has = 0;
   o.gjk_iters = 0;
@@ -1123,8 +1209,9 @@ __device__ __forceinline__ PairOut contact_ball_convex(f3 c1, float r, const iso
     } else {
       f3 p1, p2, nn;
       int it;
-      int e = r2.status == GJK_S_INTERSECTION ? epa<G, PAR>(c, s, P, gl, A, p1, p2, nn, it) : EPA_NONE;
+      int e = r2.status == GJK_S_INTERSECTION ? epa<G, PAR>(c, s, P, gl, A, p1, p2, nn, it, io) : EPA_NONE;
       if (e == EPA_OVERFLOW) { o.has = 2; return o; }
+      if (e == EPA_SUSPENDED) { o.has = 4; return o; }
       inside = true;
       world2 = e == EPA_OK ? p1 + c1 : c1;
     }
@@ -1154,7 +1241,7 @@ __device__ __forceinline__ PairOut contact_ball_convex(f3 c1, float r, const iso
 // query::contact dispatch (contact_shape_shape.rs:10-48).
 template <int G, bool PAR>
 __device__ __forceinline__ PairOut contact_pair(const Ctx2& c, float prediction, bool has_init, f3 init_dir, const Pool& P,
-                                int gl, const Arena& A, int max_iter) {
+                                int gl, const Arena& A, int max_iter, const EpaIO& io) {
   bool b1 = c.g1.type == NCX_BALL, b2 = c.g2.type == NCX_BALL;
   bool poly1 = c.g1.type == NCX_CUBOID || c.g1.type == NCX_CONVEX_HULL;
   bool poly2 = c.g2.type == NCX_CUBOID || c.g2.type == NCX_CONVEX_HULL;
@@ -1177,16 +1264,16 @@ __device__ __forceinline__ PairOut contact_pair(const Ctx2& c, float prediction,
     }
     return o;
   }
-  if (b1 && poly2) return contact_ball_convex<G, PAR>(c.m1.t, c.g1.a, c.m2, c.g2, prediction, P, gl, A);
+  if (b1 && poly2) return contact_ball_convex<G, PAR>(c.m1.t, c.g1.a, c.m2, c.g2, prediction, P, gl, A, io);
   if (poly1 && b2) {
-    PairOut o = contact_ball_convex<G, PAR>(c.m2.t, c.g2.a, c.m1, c.g1, prediction, P, gl, A);
+    PairOut o = contact_ball_convex<G, PAR>(c.m2.t, c.g2.a, c.m1, c.g1, prediction, P, gl, A, io);
     if (o.has == 1) {  // Contact::flip (contact.rs:52-55)
       f3 t = o.w1; o.w1 = o.w2; o.w2 = t;
       o.n = -o.n;
     }
     return o;
   }
-  return contact_sm<G, PAR>(c, prediction, has_init, init_dir, P, gl, A, max_iter);
+  return contact_sm<G, PAR>(c, prediction, has_init, init_dir, P, gl, A, max_iter, io);
 }
 
 struct KArgs {
@@ -1219,6 +1306,11 @@ struct KArgs {
   uint32_t* work;         // optional work counter (zero at launch): pairs are taken from it
   const uint32_t* range_b;  // optional device begin and end into list: a run of buckets
   const uint32_t* range_e;
+  const uint8_t* sus_load;  // saved EPA runs of the list's ENTRY_SAVED entries, slot = list position
+  size_t sus_load_bytes;
+  uint8_t* sus_save;        // slots for runs this launch suspends (LDS tiers only; null: none)
+  size_t sus_save_bytes;
+  uint32_t sus_save_cap;
 };
 
 // Contact/distance kernel, BS threads per block in groups of G lanes (one pair per group).
@@ -1226,6 +1318,28 @@ struct KArgs {
 // entries) lives in LDS; a pair whose polytope outgrows it is listed in ovf for a launch with
 // larger arenas.  VC == 0: the arena is the group's slice of a.arena (a.vcap, a.fcap) in device
 // memory.
+// Arena tiers (capacities; see the launch code below).
+#define T1_G 16
+#define T1_BS 128
+#define T1_VC 44u
+#define T1_FC 144u
+#define T1_SC 32u
+#define T1_STC 32u
+#define T2_G 16
+#define T2_BS 64
+#define T2_VC 128u
+#define T2_FC 384u
+#define T2_SC 64u
+#define T2_STC 64u
+#define T3_VC 512u
+#define T3_FC 1600u
+#define T3_SC 128u
+#define T3_STC 128u
+#define T4_VC 1536u
+#define T4_FC 4800u
+#define T4_SC 256u
+#define T4_STC 256u
+
 constexpr uint32_t ilog2_floor(uint32_t x) { return x <= 1 ? 0u : 1u + ilog2_floor(x >> 1); }
 
 template <int G, int BS, uint32_t VC, uint32_t FC, uint32_t SC, uint32_t STC, int MODE>
@@ -1259,7 +1373,22 @@ __global__ __launch_bounds__(BS, MODE == 1 ? 3 : 2) void k_pairs(KArgs a) {
       q = group + it * ngroups;
     }
     if (q >= npairs) break;
-    size_t k = a.list ? a.list[lbase + q] : q;
+    const uint32_t entry = a.list ? a.list[lbase + q] : (uint32_t)q;
+    size_t k = entry & ~ENTRY_SAVED;
+    EpaIO io;
+    // Runs are saved by the second to fourth LDS tiers and continued by the third, fourth and
+    // device-memory tiers; the first tier, where nearly all pairs finish, has neither path.
+    constexpr bool SAVES = LDS && VC > T1_VC;
+    constexpr bool LOADS = !LDS || VC > T2_VC;
+    io.load = (LOADS && a.list && (entry & ENTRY_SAVED) && a.sus_load) ? a.sus_load + (size_t)q * a.sus_load_bytes
+                                                                       : nullptr;
+    io.save = SAVES ? a.sus_save : nullptr;
+    io.save_bytes = a.sus_save_bytes;
+    io.save_cap = a.sus_save_cap;
+    io.count = a.ovf_count;
+    io.list = a.ovf_list;
+    io.list_cap = a.ovf_cap;
+    io.k = (uint32_t)k;
     uint32_t i1 = a.ia[k], i2 = a.ib[k];
     if (i1 >= a.nshapes || i2 >= a.nshapes) {
       if (gl == 0) atomicOr(a.bad_index, 1u);
@@ -1325,13 +1454,14 @@ __global__ __launch_bounds__(BS, MODE == 1 ? 3 : 2) void k_pairs(KArgs a) {
       // An all-zero entry means None for this pair (the reference's default direction).
       has_init = idir.x != 0.f || idir.y != 0.f || idir.z != 0.f;
     }
-    PairOut o = contact_pair<G, LDS>(c, a.prediction, has_init, idir, a.P, gl, A, has_init ? a.max_iter : 10000);
+    PairOut o = contact_pair<G, LDS>(c, a.prediction, has_init, idir, a.P, gl, A, has_init ? a.max_iter : 10000, io);
     int extra_iters = 0;
     if (o.has == 3) {
       // Warm start hit its iteration cap: plain GJK (init_dir = None) judges the pair.
       extra_iters = o.gjk_iters;
-      o = contact_pair<G, LDS>(c, a.prediction, false, idir, a.P, gl, A, 10000);
+      o = contact_pair<G, LDS>(c, a.prediction, false, idir, a.P, gl, A, 10000, io);
     }
+    if (o.has == 4) continue;  // suspended: listed (and saved) for the next tier
     if (o.has == 2) {
       if (gl == 0) {
         uint32_t slot = atomicAdd(a.ovf_count, 1u);
@@ -1503,30 +1633,23 @@ __global__ __launch_bounds__(BKT_BLOCK) void k_bucket_scatter(const uint32_t* __
 //   T4: 64 lanes, one pair per 64-thread block, T4_VC vertices and T4_FC faces (up to the
 //       160 KiB a gfx950 workgroup may hold; one block per CU);
 //   then device-memory arenas 8x larger than T4's, 64 lanes per pair, host-driven.
-#define T1_G 16
-#define T1_BS 128
-#define T1_VC 44u
-#define T1_FC 144u
-#define T1_SC 32u
-#define T1_STC 32u
-#define T2_G 16
-#define T2_BS 64
-#define T2_VC 128u
-#define T2_FC 384u
-#define T2_SC 64u
-#define T2_STC 64u
-#define T3_VC 512u
-#define T3_FC 1600u
-#define T3_SC 128u
-#define T3_STC 128u
-#define T4_VC 1536u
-#define T4_FC 4800u
-#define T4_SC 256u
-#define T4_STC 256u
+
+// Saved-run slots handed from each LDS tier to the next (the first SUS_CAP[t] suspended runs of
+// a launch are saved; later ones start again in the next tier) and their sizes.  The first tier
+// saves nothing (slot count 0): replaying its short runs costs less than carrying the save path
+// in its kernel.
+static const uint32_t SUS_CAP[4] = {0, 4096, 1024, 256};
+static const size_t SUS_BYTES[4] = {epa_save_bytes(T1_VC, T1_FC), epa_save_bytes(T2_VC, T2_FC),
+                                    epa_save_bytes(T3_VC, T3_FC), epa_save_bytes(T4_VC, T4_FC)};
 
 // Overflow lists of at least cap entries each (at most 2^24: a call with more overflowing pairs
-// fails with NCX_E_CAPACITY).
+// fails with NCX_E_CAPACITY), and the saved-run slots.
 int contact_tiers_reserve(ContactTiers& t, size_t cap) {
+  for (int i = 0; i < 4; ++i) {
+    if (t.sus[i].p || SUS_CAP[i] == 0) continue;
+    int rc = t.sus[i].reserve((size_t)SUS_CAP[i] * SUS_BYTES[i]);
+    if (rc) return rc;
+  }
   cap = std::min<size_t>(std::max<size_t>(cap, 4096), (size_t)1 << 24);
   if (cap <= t.cap) return NCX_OK;
   t.ovf.release();
@@ -1539,6 +1662,7 @@ int contact_tiers_reserve(ContactTiers& t, size_t cap) {
 
 void contact_tiers_release(ContactTiers& t) {
   t.ovf.release();
+  for (int i = 0; i < 4; ++i) t.sus[i].release();
   delete t.last;
   t.last = nullptr;
 }
@@ -1554,9 +1678,10 @@ static int launch_global_tier(hipStream_t stream, KArgs a, size_t npairs) {
 
 // Re-runs the n pairs listed at list_src (device) with device-memory arenas 8x larger per round
 // (starting from 8 x the last tier's vcap/fcap) until none overflows; host-driven (synchronises
-// the stream).  Chunks keep each round's arenas <= 2 GiB.
+// the stream).  Chunks keep each round's arenas <= 2 GiB.  The first round continues the runs
+// the last LDS tier saved (sus_src, slot = list position); device-memory rounds do not suspend.
 static int rerun_overflows(ContactTiers& T, ncx_ctx* ctx, KArgs a, uint32_t n, const uint32_t* list_src,
-                           uint32_t vcap, uint32_t fcap) {
+                           uint32_t vcap, uint32_t fcap, const uint8_t* sus_src, size_t sus_bytes) {
   if (n == 0) return NCX_OK;
   uint32_t* ovf = T.ovf.as<uint32_t>();
   DevBuf lst;
@@ -1584,8 +1709,14 @@ static int rerun_overflows(ContactTiers& T, ncx_ctx* ctx, KArgs a, uint32_t n, c
       b.ovf_count = ovf + NCX_OVF_T2_COUNT;
       b.ovf_list = T.t2_list();
       b.ovf_cap = T.cap;
+      b.sus_load = sus_src ? sus_src + off * sus_bytes : nullptr;
+      b.sus_load_bytes = sus_bytes;
+      b.sus_save = nullptr;
+      b.sus_save_bytes = 0;
+      b.sus_save_cap = 0;
       if ((rc = launch_global_tier(ctx->stream, b, m))) return rc;
     }
+    sus_src = nullptr;  // later rounds list this tier's own overflows, with nothing saved
     uint32_t left = 0;
     NCX_HIP_CHECK(hipMemcpyAsync(&left, ovf + NCX_OVF_T2_COUNT, 4, hipMemcpyDeviceToHost, ctx->stream));
     NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -1634,6 +1765,16 @@ static int launch_t1(int variant, bool distance, hipStream_t stream, const KArgs
   return NCX_OK;
 }
 
+// Saved-run buffers of LDS tier `tier` (0 = T1): it continues the runs T(tier) saved and saves
+// its own suspended runs for the next tier.
+static void set_sus(ContactTiers& T, KArgs& b, int tier) {
+  b.sus_load = tier > 0 ? T.sus[tier - 1].as<uint8_t>() : nullptr;
+  b.sus_load_bytes = tier > 0 ? SUS_BYTES[tier - 1] : 0;
+  b.sus_save = T.sus[tier].as<uint8_t>();
+  b.sus_save_bytes = SUS_BYTES[tier];
+  b.sus_save_cap = T.sus[tier].p ? SUS_CAP[tier] : 0u;
+}
+
 // T2 over the device-counted T1 overflow list.
 static KArgs t2_args(ContactTiers& T, const KArgs& a) {
   uint32_t* ovf = T.ovf.as<uint32_t>();
@@ -1648,6 +1789,7 @@ static KArgs t2_args(ContactTiers& T, const KArgs& a) {
   b.ovf_count = ovf + NCX_OVF_T2_COUNT;
   b.ovf_list = T.t2_list();
   b.work = ncx_work_set(ovf, NCX_WORK_T2);
+  set_sus(T, b, 1);
   return b;
 }
 
@@ -1663,6 +1805,7 @@ static KArgs t3_args(ContactTiers& T, const KArgs& a) {
   b.ovf_count = ovf + NCX_OVF_T3_COUNT;
   b.ovf_list = T.t3_list();
   b.work = ncx_work_set(ovf, NCX_WORK_T3);
+  set_sus(T, b, 2);
   return b;
 }
 
@@ -1678,6 +1821,7 @@ static KArgs t4_args(ContactTiers& T, const KArgs& a) {
   b.ovf_count = ovf + NCX_OVF_T4_COUNT;
   b.ovf_list = T.t4_list();
   b.work = ncx_work_set(ovf, NCX_WORK_T4);
+  set_sus(T, b, 3);
   return b;
 }
 
@@ -1799,6 +1943,7 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
   a.ovf_count = ovf + NCX_OVF_T1_COUNT;
   a.ovf_list = T.t1_list();
   a.ovf_cap = T.cap;
+  set_sus(T, a, 0);
   a.range_b = nullptr;
   a.range_e = nullptr;
   bool untuned = false;
@@ -1899,7 +2044,8 @@ int contacts_resolve_overflow(ContactTiers& T, ncx_ctx* ctx, size_t* n_large) {
     return NCX_E_CAPACITY;
   }
   if (n_large) *n_large = hdr[NCX_OVF_T1_COUNT];
-  return rerun_overflows(T, ctx, *T.last, hdr[NCX_OVF_T4_COUNT], T.t4_list(), T4_VC, T4_FC);
+  return rerun_overflows(T, ctx, *T.last, hdr[NCX_OVF_T4_COUNT], T.t4_list(), T4_VC, T4_FC, T.sus[3].as<uint8_t>(),
+                         SUS_BYTES[3]);
 }
 
 static int run_pairs(ncx_shapes* S, KArgs a, size_t npairs) {
