@@ -812,63 +812,63 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
     epa_load<G>(io.load, A, gl, nv, nf, hl, niter, iters, best, max_dist);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   } else {
-  vset(A, nv++, s.v0);
-  if (s.dim >= 1) vset(A, nv++, s.v1);
-  if (s.dim >= 2) vset(A, nv++, s.v2);
-  if (s.dim >= 3) vset(A, nv++, s.v3);
-  if (s.dim == 0) {
-    p1 = f3{0.f, 0.f, 0.f};
-    p2 = f3{0.f, 0.f, 0.f};
-    n = f3{0.f, 1.f, 0.f};
-    return EPA_OK;
-  } else if (s.dim == 3) {
-    f3 dp1 = vpt(A, 1) - vpt(A, 0);
-    f3 dp2 = vpt(A, 2) - vpt(A, 0);
-    f3 dp3 = vpt(A, 3) - vpt(A, 0);
-    if (dot(cross(dp1, dp2), dp3) > 0.f) {
-      CSO t1 = vget(A, 1), t2 = vget(A, 2);
-      vset(A, 1, t2);
-      vset(A, 2, t1);
-    }
-    const uint32_t PT[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {0, 3, 1}};
-    const uint32_t AD[4][3] = {{3, 1, 2}, {3, 2, 0}, {0, 1, 3}, {2, 1, 0}};
-    bool inside[4];
-    f3 fn[4];
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      FaceD fd;
-      inside[f] = make_face(A, PT[f][0], PT[f][1], PT[f][2], AD[f][0], AD[f][1], AD[f][2], fd, fn[f]);
-      A.faces[nf++] = fd;
-    }
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      if (inside[f]) {
-        float dist = dot(fn[f], vpt(A, f));
-        if (-dist > gjk_tol) return EPA_NONE;
-        A.heap[hl] = HeapE{-dist, (uint32_t)f};
-        heap_sift_up(A.heap, hl);
-        hl++;
+    vset(A, nv++, s.v0);
+    if (s.dim >= 1) vset(A, nv++, s.v1);
+    if (s.dim >= 2) vset(A, nv++, s.v2);
+    if (s.dim >= 3) vset(A, nv++, s.v3);
+    if (s.dim == 0) {
+      p1 = f3{0.f, 0.f, 0.f};
+      p2 = f3{0.f, 0.f, 0.f};
+      n = f3{0.f, 1.f, 0.f};
+      return EPA_OK;
+    } else if (s.dim == 3) {
+      f3 dp1 = vpt(A, 1) - vpt(A, 0);
+      f3 dp2 = vpt(A, 2) - vpt(A, 0);
+      f3 dp3 = vpt(A, 3) - vpt(A, 0);
+      if (dot(cross(dp1, dp2), dp3) > 0.f) {
+        CSO t1 = vget(A, 1), t2 = vget(A, 2);
+        vset(A, 1, t2);
+        vset(A, 2, t1);
       }
+      const uint32_t PT[4][3] = {{0, 1, 2}, {1, 3, 2}, {0, 2, 3}, {0, 3, 1}};
+      const uint32_t AD[4][3] = {{3, 1, 2}, {3, 2, 0}, {0, 1, 3}, {2, 1, 0}};
+      bool inside[4];
+      f3 fn[4];
+  #pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        FaceD fd;
+        inside[f] = make_face(A, PT[f][0], PT[f][1], PT[f][2], AD[f][0], AD[f][1], AD[f][2], fd, fn[f]);
+        A.faces[nf++] = fd;
+      }
+  #pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        if (inside[f]) {
+          float dist = dot(fn[f], vpt(A, f));
+          if (-dist > gjk_tol) return EPA_NONE;
+          A.heap[hl] = HeapE{-dist, (uint32_t)f};
+          heap_sift_up(A.heap, hl);
+          hl++;
+        }
+      }
+    } else {
+      if (s.dim == 1) {
+        f3 dpt = vpt(A, 1) - vpt(A, 0);
+        // nalgebra orthonormal_subspace_basis for one vector; epa3.rs:299-305 keeps a x dpt.
+        f3 a = fabsf(dpt.x) > fabsf(dpt.y) ? f3{dpt.z, 0.f, -dpt.x} : f3{0.f, -dpt.z, dpt.y};
+        a = normalize(a);
+        vset(A, nv++, cso_from<G>(c, cross(a, dpt), P, gl));
+      }
+      FaceD f1, f2;
+      f3 n1, n2;
+      make_face(A, 0, 1, 2, 1, 1, 1, f1, n1);
+      make_face(A, 0, 2, 1, 0, 0, 0, f2, n2);
+      A.faces[nf++] = f1;
+      A.faces[nf++] = f2;
+      A.heap[hl] = HeapE{0.f, 0u}; heap_sift_up(A.heap, hl); hl++;
+      A.heap[hl] = HeapE{0.f, 1u}; heap_sift_up(A.heap, hl); hl++;
     }
-  } else {
-    if (s.dim == 1) {
-      f3 dpt = vpt(A, 1) - vpt(A, 0);
-      // nalgebra orthonormal_subspace_basis for one vector; epa3.rs:299-305 keeps a x dpt.
-      f3 a = fabsf(dpt.x) > fabsf(dpt.y) ? f3{dpt.z, 0.f, -dpt.x} : f3{0.f, -dpt.z, dpt.y};
-      a = normalize(a);
-      vset(A, nv++, cso_from<G>(c, cross(a, dpt), P, gl));
-    }
-    FaceD f1, f2;
-    f3 n1, n2;
-    make_face(A, 0, 1, 2, 1, 1, 1, f1, n1);
-    make_face(A, 0, 2, 1, 0, 0, 0, f2, n2);
-    A.faces[nf++] = f1;
-    A.faces[nf++] = f2;
-    A.heap[hl] = HeapE{0.f, 0u}; heap_sift_up(A.heap, hl); hl++;
-    A.heap[hl] = HeapE{0.f, 1u}; heap_sift_up(A.heap, hl); hl++;
-  }
-  if (hl == 0) return EPA_NONE;
-  best = A.heap[0];
+    if (hl == 0) return EPA_NONE;
+    best = A.heap[0];
   }
   HeapE fid;
   for (;;) {
