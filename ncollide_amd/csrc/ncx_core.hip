@@ -483,75 +483,71 @@ __global__ __launch_bounds__(1024) void k_refit_upper(LNode* __restrict__ nodes,
   }
 }
 
-// Depth of every internal node and whether its leaf range spans refit workgroups.
+// Stage-2 nodes (leaf range spanning refit workgroups) and their depth; LBVH_MAX_DEPTH marks the
+// others.  Counts stage-2 nodes per depth in cnt[0..LBVH_MAX_DEPTH).
 __global__ __launch_bounds__(256) void k_upper_mark(const LNode* __restrict__ nodes, const uint32_t* __restrict__ parent,
                                                      const uint32_t* __restrict__ first, int n,
-                                                     uint32_t* __restrict__ is_upper, uint32_t* __restrict__ depth) {
+                                                     uint32_t* __restrict__ depth, uint32_t* __restrict__ cnt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
-  uint32_t d = 0, p = (uint32_t)i;
-  while (p != 0 && d < (uint32_t)n) {
-    p = parent[p] & ~NCX_RIGHT_BIT;
-    d++;
+  const bool upper = first[i] / REFIT_B != nodes[i].last_r / REFIT_B;
+  uint32_t d = LBVH_MAX_DEPTH;
+  if (upper) {
+    d = 0;
+    uint32_t p = (uint32_t)i;
+    while (p != 0 && d < LBVH_MAX_DEPTH - 1) {
+      p = parent[p] & ~NCX_RIGHT_BIT;
+      d++;
+    }
+    atomicAdd(&cnt[d], 1u);
   }
   depth[i] = d;
-  is_upper[i] = (first[i] / REFIT_B != nodes[i].last_r / REFIT_B) ? 1u : 0u;
 }
 
-__global__ __launch_bounds__(256) void k_upper_compact(const uint32_t* __restrict__ is_upper,
-                                                        const uint32_t* __restrict__ pos,
-                                                        const uint32_t* __restrict__ depth, int m,
-                                                        uint32_t* __restrict__ out) {
+// Level offsets, deepest level first: off[lv] for lv = 0..LBVH_MAX_DEPTH (level lv holds depth
+// LBVH_MAX_DEPTH - 1 - lv), and each depth's scatter cursor.
+__global__ void k_upper_offsets(const uint32_t* __restrict__ cnt, uint32_t* __restrict__ off,
+                                uint32_t* __restrict__ cursor) {
+  if (threadIdx.x != 0) return;
+  uint32_t o = 0;
+  for (uint32_t lv = 0; lv < LBVH_MAX_DEPTH; ++lv) {
+    const uint32_t d = LBVH_MAX_DEPTH - 1 - lv;
+    off[lv] = o;
+    cursor[d] = o;
+    o += cnt[d];
+  }
+  off[LBVH_MAX_DEPTH] = o;
+}
+
+// The stage-2 list, grouped by level (order within a level is free: a level's nodes write
+// distinct parent slots).
+__global__ __launch_bounds__(256) void k_upper_scatter(const uint32_t* __restrict__ depth, int m,
+                                                        uint32_t* __restrict__ cursor, uint32_t* __restrict__ upper) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m || !is_upper[i]) return;
-  out[2 * pos[i]] = (uint32_t)i;
-  out[2 * pos[i] + 1] = depth[i];
+  if (i >= m) return;
+  const uint32_t d = depth[i];
+  if (d < LBVH_MAX_DEPTH) upper[atomicAdd(&cursor[d], 1u)] = (uint32_t)i;
 }
 
+// Stage-2 node list for a new topology, on the device (no host synchronisation).
 static int refit_prepare(ncx_ctx* ctx, LBVH& bvh) {
-  size_t n = bvh.n;
-  size_t m = n - 1;
+  const size_t m = bvh.n - 1;
   int rc;
-  DevBuf isu, pos, dep, total, lst;
-  if ((rc = isu.reserve(m * 4)) || (rc = pos.reserve(m * 4)) || (rc = dep.reserve(m * 4)) ||
-      (rc = total.reserve(16)))
+  if ((rc = bvh.node_depth.reserve(m * 4)) || (rc = bvh.upper.reserve(m * 4)) ||
+      (rc = bvh.upper_off.reserve((3 * (size_t)LBVH_MAX_DEPTH + 1) * 4)))
     return rc;
+  uint32_t* cnt = bvh.upper_off.as<uint32_t>() + LBVH_MAX_DEPTH + 1;
+  uint32_t* cursor = cnt + LBVH_MAX_DEPTH;
+  if ((rc = fill_u32(ctx, cnt, LBVH_MAX_DEPTH, 0u))) return rc;
   hipLaunchKernelGGL(k_upper_mark, dim3(ceil_div(m, 256)), dim3(256), 0, ctx->stream, bvh.nodes.as<LNode>(),
-                     bvh.parent.as<uint32_t>(), bvh.first.as<uint32_t>(), (int)n, isu.as<uint32_t>(),
-                     dep.as<uint32_t>());
+                     bvh.parent.as<uint32_t>(), bvh.first.as<uint32_t>(), (int)bvh.n, bvh.node_depth.as<uint32_t>(),
+                     cnt);
   NCX_HIP_CHECK(hipGetLastError());
-  if ((rc = scan_exclusive_u32(ctx, isu.as<uint32_t>(), pos.as<uint32_t>(), m, bvh.scan_tmp, total.as<uint32_t>())))
-    return rc;
-  uint32_t nu = 0;
-  NCX_HIP_CHECK(hipMemcpyAsync(&nu, total.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-  NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  std::vector<uint32_t> pairs(2 * (size_t)nu);
-  if (nu) {
-    if ((rc = lst.reserve((size_t)nu * 8))) return rc;
-    hipLaunchKernelGGL(k_upper_compact, dim3(ceil_div(m, 256)), dim3(256), 0, ctx->stream, isu.as<uint32_t>(),
-                       pos.as<uint32_t>(), dep.as<uint32_t>(), (int)m, lst.as<uint32_t>());
-    NCX_HIP_CHECK(hipGetLastError());
-    NCX_HIP_CHECK(hipMemcpyAsync(pairs.data(), lst.p, (size_t)nu * 8, hipMemcpyDeviceToHost, ctx->stream));
-    NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  }
-  // deepest level first
-  uint32_t maxd = 0;
-  for (uint32_t i = 0; i < nu; ++i) maxd = std::max(maxd, pairs[2 * i + 1]);
-  std::vector<std::vector<uint32_t>> byd(nu ? maxd + 1 : 0);
-  for (uint32_t i = 0; i < nu; ++i) byd[pairs[2 * i + 1]].push_back(pairs[2 * i]);
-  std::vector<uint32_t> order, off(1, 0);
-  for (int d = (int)byd.size() - 1; d >= 0; --d) {
-    if (byd[d].empty()) continue;
-    order.insert(order.end(), byd[d].begin(), byd[d].end());
-    off.push_back((uint32_t)order.size());
-  }
-  bvh.n_levels = (int)off.size() - 1;
-  if ((rc = bvh.upper.reserve((order.size() + 1) * 4)) || (rc = bvh.upper_off.reserve(off.size() * 4))) return rc;
-  if (!order.empty())
-    NCX_HIP_CHECK(hipMemcpyAsync(bvh.upper.p, order.data(), order.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-  NCX_HIP_CHECK(hipMemcpyAsync(bvh.upper_off.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-  NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  isu.release(); pos.release(); dep.release(); total.release(); lst.release();
+  hipLaunchKernelGGL(k_upper_offsets, dim3(1), dim3(64), 0, ctx->stream, cnt, bvh.upper_off.as<uint32_t>(), cursor);
+  NCX_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_upper_scatter, dim3(ceil_div(m, 256)), dim3(256), 0, ctx->stream,
+                     bvh.node_depth.as<uint32_t>(), (int)m, cursor, bvh.upper.as<uint32_t>());
+  NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
 }
 
@@ -563,12 +559,10 @@ static int refit_launch(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], floa
                      bvh.order.as<uint32_t>(), (int)n, bvh.nodes.as<LNode>(), bvh.parent.as<uint32_t>(),
                      bvh.first.as<uint32_t>(), expand);
   NCX_HIP_CHECK(hipGetLastError());
-  if (bvh.n_levels > 0) {
-    hipLaunchKernelGGL(k_refit_upper, dim3(1), dim3(1024), 0, ctx->stream, bvh.nodes.as<LNode>(),
-                       bvh.parent.as<uint32_t>(), bvh.upper.as<uint32_t>(), bvh.upper_off.as<uint32_t>(),
-                       bvh.n_levels);
-    NCX_HIP_CHECK(hipGetLastError());
-  }
+  hipLaunchKernelGGL(k_refit_upper, dim3(1), dim3(1024), 0, ctx->stream, bvh.nodes.as<LNode>(),
+                     bvh.parent.as<uint32_t>(), bvh.upper.as<uint32_t>(), bvh.upper_off.as<uint32_t>(),
+                     (int)LBVH_MAX_DEPTH);
+  NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
 }
 

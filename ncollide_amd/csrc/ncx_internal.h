@@ -208,13 +208,17 @@ struct LBVH {
   DevBuf tmp_keys, tmp_vals, hist, flags, bounds, scan_tmp;
   DevBuf first;          // uint32 first sorted leaf position under each internal node
   DevBuf upper, upper_off;  // refit stage-2 node list (deepest level first) and level offsets
-  int n_levels = 0;
+  DevBuf node_depth;        // depth of each internal node (or LBVH_MAX_DEPTH when not in stage 2)
   void release() {
-    first.release(); upper.release(); upper_off.release();
+    first.release(); upper.release(); upper_off.release(); node_depth.release();
     nodes.release(); parent.release(); order.release(); keys.release();
     tmp_keys.release(); tmp_vals.release(); hist.release(); flags.release(); bounds.release(); scan_tmp.release();
   }
 };
+
+// Depth bound of an LBVH over 30-bit Morton codes with index tie-breaks: along any root-to-leaf
+// path the split position (the common prefix length of the 62-bit key:index) strictly grows.
+constexpr uint32_t LBVH_MAX_DEPTH = 64;
 
 // Builds the hierarchy over n boxes given as device SoA.  `expand` > 0 grows every
 // stored box outward by expand * (1 + |coordinate|) (used for conservative ray culling).

@@ -35,7 +35,7 @@ struct ncx_world {
   bool first = true;
   DevBuf box_new, stored;
   LBVH bvh;
-  uint32_t depth = 0, spill = 1;
+  uint32_t spill = 1;
   DevBuf gstack;  // search-stack spill area, `spill` entries per query thread
   DevBuf qwork;   // the query passes' work counters (WQ_PARTS per pass)
   DevBuf lbuf;    // the count pass's first WQ_LOCAL partners per leaf
@@ -302,24 +302,6 @@ __global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __
   }
 }
 
-// Tree depth: the longest leaf-to-root path, reduced within each wavefront before one atomicMax.
-__global__ __launch_bounds__(256) void k_world_tree_depth(const uint32_t* __restrict__ parent, uint32_t n,
-                                                          uint32_t* __restrict__ out) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t d = 0;
-  if (k < n) {
-    d = 1;
-    uint32_t p = parent[(n - 1) + k] & ~NCX_RIGHT_BIT;
-    while (p != 0 && d <= n) {
-      p = parent[p] & ~NCX_RIGHT_BIT;
-      d++;
-    }
-  }
-#pragma unroll
-  for (int s = 32; s >= 1; s >>= 1) d = max(d, (uint32_t)__shfl_xor((int)d, s, 64));
-  if (__lane_id() == 0 && d) atomicMax(out, d);
-}
-
 static int world_build(ncx_world* w) {
   ncx_ctx* c = &w->wctx;
   const float* box[6];
@@ -327,16 +309,10 @@ static int world_build(ncx_world* w) {
   int rc = lbvh_build(c, w->bvh, box, w->n, 0.f);
   if (rc) return rc;
   if (w->n > 1) {
-    if ((rc = fill_u32(c, w->counters.as<uint32_t>() + 2, 1, 0u))) return rc;
-    hipLaunchKernelGGL(k_world_tree_depth, dim3(ceil_div(w->n, 256)), dim3(256), 0, c->stream,
-                       w->bvh.parent.as<uint32_t>(), (uint32_t)w->n, w->counters.as<uint32_t>() + 2);
-    uint32_t depth = 0;
-    NCX_HIP_CHECK(hipMemcpyAsync(&depth, w->counters.as<uint32_t>() + 2, 4, hipMemcpyDeviceToHost, c->stream));
-    NCX_HIP_CHECK(hipStreamSynchronize(c->stream));
-    w->depth = depth;
-    // the search stack holds at most `depth` entries: one in a register, WQ_LSTACK in LDS
-    w->spill = depth + 1 > WQ_LSTACK ? depth + 1 - WQ_LSTACK : 1;
-    if ((rc = w->gstack.reserve((size_t)ceil_div(w->n, WQ_BLOCK) * WQ_BLOCK * w->spill * 4))) return rc;
+    // The search stack holds fewer entries than the tree has levels (at most LBVH_MAX_DEPTH): one
+    // in a register, WQ_LSTACK in LDS, the rest here.  A fixed bound needs no host read-back.
+    w->spill = LBVH_MAX_DEPTH + 1 - WQ_LSTACK;
+    if ((rc = w->gstack.reserve((size_t)world_query_blocks(c, w->n) * WQ_BLOCK * w->spill * 4))) return rc;
   }
   w->since_build = 0;
   return NCX_OK;
