@@ -231,8 +231,9 @@ int ncx_mlp_predict_dirs(ncx_mlp* mlp, const ncx_pose* poses_dev, const float* d
  * shapes are all broad-phase proxies: world AABBs loosened by `prediction`, the DBVTBroadPhase
  * stored-box rule with `margin` (dbvt_broad_phase.rs:325-347; the first step stores boxes as
  * given, :275-280), LBVH refit (rebuilt every k steps), every intersecting stored-box pair, and
- * query::contact at `prediction` on each pair.  No host synchronisation inside a step: refit steps
- * replay a captured hipGraph when use_graph != 0.  Outputs are capacity-`max_pairs` device arrays. */
+ * query::contact at `prediction` on each pair.  No host synchronisation inside a step: with
+ * use_graph != 0, refit steps and rebuild steps (after the first step) each replay a captured
+ * hipGraph.  Outputs are capacity-`max_pairs` device arrays. */
 typedef struct ncx_world ncx_world;
 int ncx_world_create(ncx_ctx* ctx, ncx_shapes* shapes, float margin, float prediction, size_t max_pairs,
                      ncx_world** world);

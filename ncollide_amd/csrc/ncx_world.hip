@@ -11,8 +11,8 @@
 //   4. all intersecting stored-box pairs, counted, prefix-summed and written on the device,
 //   5. query::contact on every pair (contact_shape_shape.rs:10-48), optionally with the learned
 //      init_dir on polytope pairs.
-// Steps 1-5 contain no host synchronisation, so a refit step is captured once in a hipGraph and
-// replayed; rebuild steps run eagerly and invalidate the graph.
+// Steps 1-5 contain no host synchronisation, so a refit step and a rebuild step are each captured
+// once in a hipGraph and replayed (the first step, which tunes and allocates, runs eagerly).
 #include "ncx_internal.h"
 
 #define WQ_BLOCK 128
@@ -48,6 +48,10 @@ struct ncx_world {
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipGraph_t graph_rb = nullptr;       // the rebuild step's graph
+  hipGraphExec_t exec_rb = nullptr;
+  const ncx_pose* rb_poses = nullptr;
+  bool rb_valid = false;
   const ncx_pose* graph_poses = nullptr;
   bool graph_valid = false;
   int last_was_graph = 0;
@@ -302,6 +306,7 @@ __global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __
   }
 }
 
+// Enqueues the LBVH build (no host synchronisation, so a rebuild step can be captured).
 static int world_build(ncx_world* w) {
   ncx_ctx* c = &w->wctx;
   const float* box[6];
@@ -314,7 +319,6 @@ static int world_build(ncx_world* w) {
     w->spill = LBVH_MAX_DEPTH + 1 - WQ_LSTACK;
     if ((rc = w->gstack.reserve((size_t)world_query_blocks(c, w->n) * WQ_BLOCK * w->spill * 4))) return rc;
   }
-  w->since_build = 0;
   return NCX_OK;
 }
 
@@ -423,6 +427,7 @@ extern "C" int ncx_world_set_warmstart(ncx_world* w, ncx_mlp* mlp) {
   NCX_REQUIRE(w, "ncx_world_set_warmstart: null world");
   w->mlp = mlp;
   w->graph_valid = false;
+  w->rb_valid = false;
   if (mlp) {
     int rc;
     if ((rc = w->desc.reserve(w->n * 16 * 4 + 64)) || (rc = w->dirs.reserve(w->cap * 12))) return rc;
@@ -441,7 +446,10 @@ extern "C" int ncx_world_set_rebuild_every(ncx_world* w, int k) {
 // Turns the narrow phase of each step on or off (off: the step ends with the pair list).
 extern "C" int ncx_world_set_contacts(ncx_world* w, int enable) {
   NCX_REQUIRE(w, "ncx_world_set_contacts: null argument");
-  if (w->contacts != (enable != 0)) w->graph_valid = false;
+  if (w->contacts != (enable != 0)) {
+    w->graph_valid = false;
+    w->rb_valid = false;
+  }
   w->contacts = enable != 0;
   return NCX_OK;
 }
@@ -476,6 +484,8 @@ extern "C" int ncx_world_destroy(ncx_world* w) {
   if (!w) return NCX_OK;
   if (w->exec) (void)hipGraphExecDestroy(w->exec);
   if (w->graph) (void)hipGraphDestroy(w->graph);
+  if (w->exec_rb) (void)hipGraphExecDestroy(w->exec_rb);
+  if (w->graph_rb) (void)hipGraphDestroy(w->graph_rb);
   if (w->wctx.stream) {
     (void)hipStreamSynchronize(w->wctx.stream);
     (void)hipStreamDestroy(w->wctx.stream);
@@ -491,9 +501,27 @@ extern "C" int ncx_world_destroy(ncx_world* w) {
   return NCX_OK;
 }
 
-// One step.  use_graph != 0: refit steps replay a captured hipGraph (captured on first use, and
-// again after a rebuild or when `poses_dev` changes).  The step is ordered after prior work on the
-// caller's stream and later caller work waits for it; ncx_world_sync waits for completion.
+// Captures the step's work (enqueue) into *exec on stream s.
+template <class F>
+static int capture_step(hipStream_t s, hipGraph_t* graph, hipGraphExec_t* exec, F enqueue) {
+  if (*exec) { (void)hipGraphExecDestroy(*exec); *exec = nullptr; }
+  if (*graph) { (void)hipGraphDestroy(*graph); *graph = nullptr; }
+  NCX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int r = enqueue();
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (r) { if (g) (void)hipGraphDestroy(g); return r; }
+  NCX_HIP_CHECK(e);
+  *graph = g;
+  NCX_HIP_CHECK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return NCX_OK;
+}
+
+// One step.  use_graph != 0: refit steps and rebuild steps (after the first step, which tunes
+// and allocates) each replay a captured hipGraph, captured on first use and again when
+// `poses_dev` changes.  Neither depends on the tree's topology, so a rebuild does not invalidate
+// the refit graph.  The step is ordered after prior work on the caller's stream and later caller
+// work waits for it; ncx_world_sync waits for completion.
 extern "C" int ncx_world_step(ncx_world* w, const ncx_pose* poses_dev, int use_graph) {
   NCX_REQUIRE(w && poses_dev, "ncx_world_step: null argument");
   hipStream_t s = w->wctx.stream;
@@ -502,24 +530,32 @@ extern "C" int ncx_world_step(ncx_world* w, const ncx_pose* poses_dev, int use_g
   int rc;
   bool rebuild = w->first || w->since_build >= w->rebuild_every;
   w->last_was_graph = 0;
-  if (rebuild) {
+  if (rebuild && (w->first || !use_graph)) {
     if ((rc = world_enqueue(w, poses_dev, false))) return rc;
     if ((rc = world_build(w))) return rc;
     if ((rc = world_pairs_and_contacts(w, poses_dev, true))) return rc;
-    w->graph_valid = false;
+  } else if (rebuild) {
+    if (!w->rb_valid || w->rb_poses != poses_dev) {
+      rc = capture_step(s, &w->graph_rb, &w->exec_rb, [&]() {
+        int r = world_enqueue(w, poses_dev, false);
+        if (!r) r = world_build(w);
+        if (!r) r = world_pairs_and_contacts(w, poses_dev, false);
+        return r;
+      });
+      if (rc) return rc;
+      w->rb_valid = true;
+      w->rb_poses = poses_dev;
+    }
+    NCX_HIP_CHECK(hipGraphLaunch(w->exec_rb, s));
+    w->last_was_graph = 1;
   } else if (use_graph) {
     if (!w->graph_valid || w->graph_poses != poses_dev) {
-      if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
-      if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
-      NCX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int r1 = world_enqueue(w, poses_dev, true);
-      int r2 = r1 ? r1 : world_pairs_and_contacts(w, poses_dev, false);
-      hipGraph_t g = nullptr;
-      hipError_t e = hipStreamEndCapture(s, &g);
-      if (r2) { if (g) (void)hipGraphDestroy(g); return r2; }
-      NCX_HIP_CHECK(e);
-      w->graph = g;
-      NCX_HIP_CHECK(hipGraphInstantiate(&w->exec, g, nullptr, nullptr, 0));
+      rc = capture_step(s, &w->graph, &w->exec, [&]() {
+        int r = world_enqueue(w, poses_dev, true);
+        if (!r) r = world_pairs_and_contacts(w, poses_dev, false);
+        return r;
+      });
+      if (rc) return rc;
       w->graph_valid = true;
       w->graph_poses = poses_dev;
     }
@@ -529,6 +565,7 @@ extern "C" int ncx_world_step(ncx_world* w, const ncx_pose* poses_dev, int use_g
     if ((rc = world_enqueue(w, poses_dev, true))) return rc;
     if ((rc = world_pairs_and_contacts(w, poses_dev, true))) return rc;
   }
+  if (rebuild) w->since_build = 0;
   w->first = false;
   w->since_build++;
   w->last_had_contacts = w->contacts;
