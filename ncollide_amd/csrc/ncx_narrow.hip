@@ -1325,8 +1325,9 @@ struct KArgs {
 #define T1_FC 144u
 #define T1_SC 32u
 #define T1_STC 32u
-#define T2_G 16
+#define T2_G 32
 #define T2_BS 64
+#define T2_BLOCKS_PER_CU (160u * 1024u / ((T2_BS / T2_G) * (uint32_t)arena_bytes(T2_VC, T2_FC, T2_SC, T2_STC)))
 #define T2_VC 128u
 #define T2_FC 384u
 #define T2_SC 64u
@@ -1628,7 +1629,8 @@ __global__ __launch_bounds__(BKT_BLOCK) void k_bucket_scatter(const uint32_t* __
 // Arena tiers, all but the last in LDS and in the graph:
 //   T1: 8 pairs per block (8, 16 or 32 lanes per pair, the autotuner's choice), arenas of 44
 //       vertices and 144 faces (4.7 KiB per pair; 4 blocks fit a CU);
-//   T2: 16 lanes per pair, 4 pairs per 64-thread block, 128 vertices and 384 faces (12.5 KiB);
+//   T2: 32 lanes per pair, 2 pairs per 64-thread block, 128 vertices and 384 faces (12.5 KiB;
+//       with 16 lanes and 4 pairs per block only 3 blocks fit a CU's LDS, one SIMD idle);
 //   T3: 64 lanes, one pair per 64-thread block, 512 vertices and 1600 faces (51 KiB);
 //   T4: 64 lanes, one pair per 64-thread block, T4_VC vertices and T4_FC faces (up to the
 //       160 KiB a gfx950 workgroup may hold; one block per CU);
@@ -1827,7 +1829,7 @@ static KArgs t4_args(ContactTiers& T, const KArgs& a) {
 
 static int launch_t2(ncx_ctx* ctx, ContactTiers& T, const KArgs& a) {
   KArgs b = t2_args(T, a);
-  hipLaunchKernelGGL((k_pairs<T2_G, T2_BS, T2_VC, T2_FC, T2_SC, T2_STC, 0>), dim3(ctx->num_cus * 3), dim3(T2_BS), 0,
+  hipLaunchKernelGGL((k_pairs<T2_G, T2_BS, T2_VC, T2_FC, T2_SC, T2_STC, 0>), dim3(ctx->num_cus * T2_BLOCKS_PER_CU), dim3(T2_BS), 0,
                      ctx->stream, b);
   NCX_HIP_CHECK(hipGetLastError());
   KArgs c = t3_args(T, a);
