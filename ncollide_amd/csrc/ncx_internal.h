@@ -72,20 +72,23 @@ struct DevBuf {
 // the first-tier arena, those that outgrew the second tier, the third, and the fourth (cap
 // entries each).
 // Header words [0, NCX_OVF_TIER_WORDS) are the tiers' counts (cleared between tuning samples),
-// [NCX_OVF_TIER_WORDS, NCX_OVF_COUNTS) the vertex-count buckets' (which the samples read), and
-// from NCX_WORK_BASE on the work counters of the launches (the tuning samples' first tier, the
+// [NCX_OVF_TIER_WORDS, NCX_OVF_COUNTS) the partition's (which the samples read): each vertex-count
+// bucket's range in the partitioned list, and each sort key's count and scatter cursor.  From
+// NCX_WORK_BASE on come the work counters of the launches (the tuning samples' first tier, the
 // second, third and fourth tiers, one per bucket), each on its own cache line.
+constexpr uint32_t NCX_NKEYS = 64;  // sort keys: vertex-count bucket x 16 shape-type pairs
 constexpr uint32_t NCX_OVF_T1_COUNT = 0, NCX_OVF_BAD = 1, NCX_OVF_T2_COUNT = 2, NCX_OVF_T3_COUNT = 3,
-                   NCX_OVF_T4_COUNT = 4, NCX_OVF_TIER_WORDS = 8, NCX_BKT_CNT = 8, NCX_BKT_FILL = 12,
-                   NCX_BKT_RANGE = 16, NCX_OVF_COUNTS = 32;
+                   NCX_OVF_T4_COUNT = 4, NCX_OVF_TIER_WORDS = 8, NCX_BKT_RANGE = 8, NCX_KEY_CNT = 16,
+                   NCX_KEY_FILL = NCX_KEY_CNT + NCX_NKEYS, NCX_OVF_COUNTS = NCX_KEY_FILL + NCX_NKEYS;
 constexpr uint32_t NCX_WORK_SET = 64, NCX_WORK_BASE = NCX_OVF_COUNTS, NCX_WORK_TUNE = 0, NCX_WORK_T2 = 1, NCX_WORK_T3 = 2,
                    NCX_WORK_T4 = 3, NCX_WORK_BKT = 4, NCX_WORK_SETS = 8,
                    NCX_OVF_HEADER = NCX_WORK_BASE + NCX_WORK_SETS * NCX_WORK_SET;
 // Pairs are partitioned by their larger hull vertex count into NCX_NBUCKETS buckets, each run by
 // the kernel variant the autotuner picked for it.
 constexpr int NCX_NBUCKETS = 4;
-static_assert(NCX_OVF_T4_COUNT < NCX_OVF_TIER_WORDS && NCX_OVF_TIER_WORDS <= NCX_BKT_CNT &&
-                  NCX_BKT_RANGE + 2 * NCX_NBUCKETS <= NCX_OVF_COUNTS && NCX_WORK_BKT + NCX_NBUCKETS <= NCX_WORK_SETS,
+static_assert(NCX_OVF_T4_COUNT < NCX_OVF_TIER_WORDS && NCX_OVF_TIER_WORDS <= NCX_BKT_RANGE &&
+                  NCX_BKT_RANGE + 2 * NCX_NBUCKETS <= NCX_KEY_CNT && NCX_NKEYS == 16 * NCX_NBUCKETS &&
+                  NCX_WORK_BKT + NCX_NBUCKETS <= NCX_WORK_SETS,
               "overflow header layout");
 // The work counter of launch k.
 inline uint32_t* ncx_work_set(uint32_t* ovf, uint32_t k) { return ovf + NCX_WORK_BASE + k * NCX_WORK_SET; }

