@@ -76,7 +76,7 @@ struct DevBuf {
 // bucket's range in the partitioned list, and each sort key's count and scatter cursor.  From
 // NCX_WORK_BASE on come the work counters of the launches (the tuning samples' first tier, the
 // second, third and fourth tiers, one per bucket), each on its own cache line.
-constexpr uint32_t NCX_NKEYS = 64;  // sort keys: vertex-count bucket x 16 shape-type pairs
+constexpr uint32_t NCX_NKEYS = 512;  // sort keys: vertex-count bucket x 16 shape-type pairs x 8 overlap bins
 constexpr uint32_t NCX_OVF_T1_COUNT = 0, NCX_OVF_BAD = 1, NCX_OVF_T2_COUNT = 2, NCX_OVF_T3_COUNT = 3,
                    NCX_OVF_T4_COUNT = 4, NCX_OVF_TIER_WORDS = 8, NCX_BKT_RANGE = 8, NCX_KEY_CNT = 16,
                    NCX_KEY_FILL = NCX_KEY_CNT + NCX_NKEYS, NCX_OVF_COUNTS = NCX_KEY_FILL + NCX_NKEYS;
@@ -87,7 +87,7 @@ constexpr uint32_t NCX_WORK_SET = 64, NCX_WORK_BASE = NCX_OVF_COUNTS, NCX_WORK_T
 // the kernel variant the autotuner picked for it.
 constexpr int NCX_NBUCKETS = 4;
 static_assert(NCX_OVF_T4_COUNT < NCX_OVF_TIER_WORDS && NCX_OVF_TIER_WORDS <= NCX_BKT_RANGE &&
-                  NCX_BKT_RANGE + 2 * NCX_NBUCKETS <= NCX_KEY_CNT && NCX_NKEYS == 16 * NCX_NBUCKETS &&
+                  NCX_BKT_RANGE + 2 * NCX_NBUCKETS <= NCX_KEY_CNT && NCX_NKEYS == 128 * NCX_NBUCKETS &&
                   NCX_WORK_BKT + NCX_NBUCKETS <= NCX_WORK_SETS,
               "overflow header layout");
 // The work counter of launch k.
@@ -100,8 +100,9 @@ struct KArgs;  // contact kernel arguments (ncx_narrow.hip)
 // share them.
 struct ContactTiers {
   DevBuf ovf;
-  DevBuf blist;      // pair indices partitioned by bucket
-  size_t bcap = 0;   // entries of blist
+  DevBuf blist;      // pair indices partitioned by sort key
+  DevBuf bkey;       // each pair's sort key (count pass -> scatter pass)
+  size_t bcap = 0;   // entries of blist and bkey
   uint32_t cap = 0;  // entries per overflow list
   KArgs* last = nullptr;
   DevBuf sus[4];     // saved EPA runs handed on: T1 -> T2, T2 -> T3, T3 -> T4, T4 -> device-memory tier

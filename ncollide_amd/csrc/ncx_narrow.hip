@@ -616,7 +616,7 @@ __device__ __forceinline__ void heap_sift_up_par(HeapE* h, uint32_t pos, int gl)
   if (have) a = h[anc];
   const bool stop = have && e.neg_dist <= a.neg_dist;
   const int gbase = __lane_id() & ~(G - 1);
-  const uint64_t sm = (__ballot(stop) >> gbase) & (G == 64 ? ~0ull : ((1ull << G) - 1ull));
+  const uint64_t sm = (__ballot(stop) >> gbase) & (G == 64 ? ~0ull : ((1ull << (G & 63)) - 1ull));
   const uint32_t s = sm ? (uint32_t)__builtin_ctzll(sm) : depth;
   __builtin_amdgcn_wave_barrier();
   if ((uint32_t)gl < s) h[gl == 0 ? pos : (k >> gl) - 1u] = a;
@@ -982,7 +982,7 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
       // has a face, bit 15: origin projects inside) and to the emptied search stack (the
       // distance).  LDS arenas only: lanes of one wavefront see each other's LDS writes in order.
       const int lane = __lane_id();
-      const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << (lane & ~(G - 1)));
+      const uint64_t gmask = G == 64 ? ~0ull : (((1ull << (G & 63)) - 1ull) << (lane & ~(G - 1)));
       const uint64_t below = (1ull << lane) - 1ull;
       float* sdist = (float*)A.stack;
       uint32_t nvalid = 0;
@@ -1489,6 +1489,8 @@ __global__ __launch_bounds__(BS, MODE == 1 ? 3 : 2) void k_pairs(KArgs a) {
 }
 
 // ------------------------------------------------------------------ host side
+__global__ void k_shape_radius(ncx_shape* __restrict__ shapes, size_t n, Pool P);
+
 extern "C" int ncx_shapes_create(ncx_ctx* ctx, const ncx_shape* shapes, size_t n, const float* hx, const float* hy,
                                  const float* hz, size_t npts, ncx_shapes** out) {
   NCX_REQUIRE(ctx && out && (shapes || n == 0), "ncx_shapes_create: null argument");
@@ -1514,6 +1516,11 @@ extern "C" int ncx_shapes_create(ncx_ctx* ctx, const ncx_shape* shapes, size_t n
     NCX_HIP_CHECK(hipMemcpyAsync(S->hx.p, hx, npts * 4, hipMemcpyHostToDevice, ctx->stream));
     NCX_HIP_CHECK(hipMemcpyAsync(S->hy.p, hy, npts * 4, hipMemcpyHostToDevice, ctx->stream));
     NCX_HIP_CHECK(hipMemcpyAsync(S->hz.p, hz, npts * 4, hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (n) {
+    hipLaunchKernelGGL(k_shape_radius, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, S->shapes.as<ncx_shape>(), n,
+                       Pool{S->hx.as<float>(), S->hy.as<float>(), S->hz.as<float>()});
+    NCX_HIP_CHECK(hipGetLastError());
   }
   NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   *out = S;
@@ -1548,11 +1555,15 @@ extern "C" int ncx_shapes_count(ncx_shapes* s, size_t* n) {
 // ------------------------------------------------------------------ vertex-count buckets
 // Sort key of a pair: its vertex-count bucket (by the larger hull vertex count of its shapes: 0 no
 // hull, 1 up to 32, 2 up to 96, 3 more; pairs of one bucket have similar support-scan costs and
-// get one kernel variant) times 16, plus its pair of shape types.  Pairs of one type pair take the
-// same dispatch branch, so sorting by type pair keeps the groups of a wavefront on the same code
-// path (on a 1M-shape scene step, contacts over the sorted pairs took 131 ms instead of 144).
-__device__ __forceinline__ uint32_t pair_key(const ncx_shape* __restrict__ shapes, uint32_t nshapes, uint32_t i1,
-                                             uint32_t i2) {
+// get one kernel variant) times 128, plus 8 x its pair of shape types, plus an overlap bin:
+// floor(8 x centre distance / (sum of the shapes' bounding radii)), at most 7.  Pairs of one
+// type pair take the same dispatch branch, and pairs of one bin are about as likely to need EPA
+// (on a 1M-shape scene: bins 0-2 ~100% penetrating, 4: 80%, 6: 31%, 8 and up: none), so the groups
+// of a wavefront stay on one code path.  On that scene's 10.2 M pairs contacts took 144 ms in
+// their own order, 133 ms sorted by type pair, 121 ms by type pair and bin.  The radius of shape i
+// is kept in its record's pad[0] (ncx_shapes_create).
+__device__ __forceinline__ uint32_t pair_key(const ncx_shape* __restrict__ shapes, uint32_t nshapes,
+                                             const ncx_pose* __restrict__ poses, uint32_t i1, uint32_t i2) {
   if (i1 >= nshapes || i2 >= nshapes) return 0;  // the pair kernel reports the bad index
   const ncx_shape s1 = shapes[i1], s2 = shapes[i2];
   uint32_t c1 = s1.type == NCX_CONVEX_HULL ? s1.hull_count : 0;
@@ -1560,31 +1571,61 @@ __device__ __forceinline__ uint32_t pair_key(const ncx_shape* __restrict__ shape
   uint32_t m = c1 > c2 ? c1 : c2;
   uint32_t b = m == 0 ? 0u : (m <= 32 ? 1u : (m <= 96 ? 2u : 3u));
   uint32_t t1 = s1.type < 3u ? s1.type : 3u, t2 = s2.type < 3u ? s2.type : 3u;
-  return 16u * b + 4u * (t1 < t2 ? t1 : t2) + (t1 < t2 ? t2 : t1);
+  const float dx = poses[i1].tx - poses[i2].tx, dy = poses[i1].ty - poses[i2].ty, dz = poses[i1].tz - poses[i2].tz;
+  const float rs = __uint_as_float(s1.pad[0]) + __uint_as_float(s2.pad[0]);
+  const float bin = 8.f * __builtin_sqrtf(dx * dx + dy * dy + dz * dz) / rs;
+  const uint32_t o = bin < 7.f ? (uint32_t)bin : 7u;  // NaN (rs = 0) and large go to bin 7
+  return 128u * b + 8u * (4u * (t1 < t2 ? t1 : t2) + (t1 < t2 ? t2 : t1)) + o;
+}
+
+// Bounding radius of each shape about its local origin, into pad[0] of its device record (a
+// sort heuristic only: nothing else reads it).
+__global__ __launch_bounds__(256) void k_shape_radius(ncx_shape* __restrict__ shapes, size_t n, Pool P) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const ncx_shape s = shapes[i];
+  float r = 0.f;
+  if (s.type == NCX_BALL) r = s.p0;
+  else if (s.type == NCX_CUBOID) r = __builtin_sqrtf(s.p0 * s.p0 + s.p1 * s.p1 + s.p2 * s.p2);
+  else if (s.type == NCX_CAPSULE) r = s.p0 + s.p1;
+  else {
+    for (uint32_t k = 0; k < s.hull_count; ++k) {
+      const uint32_t j = s.hull_offset + k;
+      r = fmaxf(r, P.x[j] * P.x[j] + P.y[j] * P.y[j] + P.z[j] * P.z[j]);
+    }
+    r = __builtin_sqrtf(r);
+  }
+  shapes[i].pad[0] = __float_as_uint(r);
 }
 
 #define BKT_BLOCK 256
 #define BKT_ITEMS 8
-static_assert(BKT_BLOCK >= NCX_NKEYS, "one thread per sort key");
 
-// Pass 1: per-key counts (LDS histogram per block, one atomic per key per block).
+// Pass 1: each pair's key, and per-key counts (LDS histogram per block, one atomic per key per
+// block).
 __global__ __launch_bounds__(BKT_BLOCK) void k_bucket_count(const uint32_t* __restrict__ ia,
                                                             const uint32_t* __restrict__ ib,
                                                             const ncx_shape* __restrict__ shapes, uint32_t nshapes,
-                                                            size_t npairs, const uint32_t* __restrict__ n_dev,
-                                                            uint32_t* __restrict__ hdr) {
+                                                            const ncx_pose* __restrict__ poses, size_t npairs,
+                                                            const uint32_t* __restrict__ n_dev,
+                                                            uint32_t* __restrict__ hdr, uint32_t* __restrict__ keys) {
   __shared__ uint32_t h[NCX_NKEYS];
-  if (threadIdx.x < NCX_NKEYS) h[threadIdx.x] = 0;
+  for (uint32_t k = threadIdx.x; k < NCX_NKEYS; k += BKT_BLOCK) h[k] = 0;
   __syncthreads();
   size_t n = npairs;
   if (n_dev) n = min(n, (size_t)*n_dev);
   const size_t base = (size_t)blockIdx.x * BKT_BLOCK * BKT_ITEMS;
   for (int r = 0; r < BKT_ITEMS; ++r) {
     size_t q = base + (size_t)r * BKT_BLOCK + threadIdx.x;
-    if (q < n) atomicAdd(&h[pair_key(shapes, nshapes, ia[q], ib[q])], 1u);
+    if (q < n) {
+      const uint32_t key = pair_key(shapes, nshapes, poses, ia[q], ib[q]);
+      keys[q] = key;
+      atomicAdd(&h[key], 1u);
+    }
   }
   __syncthreads();
-  if (threadIdx.x < NCX_NKEYS && h[threadIdx.x]) atomicAdd(&hdr[NCX_KEY_CNT + threadIdx.x], h[threadIdx.x]);
+  for (uint32_t k = threadIdx.x; k < NCX_NKEYS; k += BKT_BLOCK)
+    if (h[k]) atomicAdd(&hdr[NCX_KEY_CNT + k], h[k]);
 }
 
 // Key offsets (the scatter cursors) and each bucket's range [begin, end) in the partitioned list.
@@ -1592,22 +1633,21 @@ __global__ void k_bucket_offsets(uint32_t* __restrict__ hdr) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint32_t off = 0;
   for (uint32_t key = 0; key < NCX_NKEYS; ++key) {
-    if (key % 16 == 0) hdr[NCX_BKT_RANGE + 2 * (key / 16)] = off;
+    constexpr uint32_t per_bucket = NCX_NKEYS / NCX_NBUCKETS;
+    if (key % per_bucket == 0) hdr[NCX_BKT_RANGE + 2 * (key / per_bucket)] = off;
     hdr[NCX_KEY_FILL + key] = off;
     off += hdr[NCX_KEY_CNT + key];
-    if (key % 16 == 15) hdr[NCX_BKT_RANGE + 2 * (key / 16) + 1] = off;
+    if (key % per_bucket == per_bucket - 1) hdr[NCX_BKT_RANGE + 2 * (key / per_bucket) + 1] = off;
   }
 }
 
 // Pass 2: scatter pair indices to their key's range (one atomic per key per block for the
 // block's base, LDS atomics within the block).
-__global__ __launch_bounds__(BKT_BLOCK) void k_bucket_scatter(const uint32_t* __restrict__ ia,
-                                                              const uint32_t* __restrict__ ib,
-                                                              const ncx_shape* __restrict__ shapes, uint32_t nshapes,
-                                                              size_t npairs, const uint32_t* __restrict__ n_dev,
+__global__ __launch_bounds__(BKT_BLOCK) void k_bucket_scatter(const uint32_t* __restrict__ keys, size_t npairs,
+                                                              const uint32_t* __restrict__ n_dev,
                                                               uint32_t* __restrict__ hdr, uint32_t* __restrict__ list) {
   __shared__ uint32_t h[NCX_NKEYS], bb[NCX_NKEYS];
-  if (threadIdx.x < NCX_NKEYS) h[threadIdx.x] = 0;
+  for (uint32_t k = threadIdx.x; k < NCX_NKEYS; k += BKT_BLOCK) h[k] = 0;
   __syncthreads();
   size_t n = npairs;
   if (n_dev) n = min(n, (size_t)*n_dev);
@@ -1616,13 +1656,13 @@ __global__ __launch_bounds__(BKT_BLOCK) void k_bucket_scatter(const uint32_t* __
 #pragma unroll
   for (int r = 0; r < BKT_ITEMS; ++r) {
     size_t q = base + (size_t)r * BKT_BLOCK + threadIdx.x;
-    mine[r] = q < n ? pair_key(shapes, nshapes, ia[q], ib[q]) : 0xFFFFFFFFu;
+    mine[r] = q < n ? keys[q] : 0xFFFFFFFFu;
     if (mine[r] != 0xFFFFFFFFu) atomicAdd(&h[mine[r]], 1u);
   }
   __syncthreads();
-  if (threadIdx.x < NCX_NKEYS) {
-    bb[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&hdr[NCX_KEY_FILL + threadIdx.x], h[threadIdx.x]) : 0u;
-    h[threadIdx.x] = 0;
+  for (uint32_t k = threadIdx.x; k < NCX_NKEYS; k += BKT_BLOCK) {
+    bb[k] = h[k] ? atomicAdd(&hdr[NCX_KEY_FILL + k], h[k]) : 0u;
+    h[k] = 0;
   }
   __syncthreads();
 #pragma unroll
@@ -1671,6 +1711,10 @@ int contact_tiers_reserve(ContactTiers& t, size_t cap) {
 
 void contact_tiers_release(ContactTiers& t) {
   t.ovf.release();
+  t.blist.release();
+  t.bkey.release();
+  t.bcap = 0;
+  t.cap = 0;
   for (int i = 0; i < 4; ++i) t.sus[i].release();
   delete t.last;
   t.last = nullptr;
@@ -1853,8 +1897,10 @@ static int launch_t2(ncx_ctx* ctx, ContactTiers& T, const KArgs& a) {
 int contact_tiers_reserve_list(ContactTiers& t, size_t npairs) {
   if (npairs <= t.bcap) return NCX_OK;
   t.blist.release();
+  t.bkey.release();
   t.bcap = 0;
   int rc = t.blist.reserve(npairs * 4 + 64);
+  if (!rc) rc = t.bkey.reserve(npairs * 4 + 64);
   if (rc) return rc;
   t.bcap = npairs;
   return NCX_OK;
@@ -1982,12 +2028,12 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
     if ((rc = contact_tiers_reserve_list(T, npairs))) return rc;
     unsigned bblocks = std::max(1u, ceil_div(npairs, (size_t)BKT_BLOCK * BKT_ITEMS));
     hipLaunchKernelGGL(k_bucket_count, dim3(bblocks), dim3(BKT_BLOCK), 0, ctx->stream, a.ia, a.ib, a.shapes,
-                       a.nshapes, npairs, n_dev, ovf);
+                       a.nshapes, a.poses, npairs, n_dev, ovf, T.bkey.as<uint32_t>());
     NCX_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_bucket_offsets, dim3(1), dim3(1), 0, ctx->stream, ovf);
     NCX_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_bucket_scatter, dim3(bblocks), dim3(BKT_BLOCK), 0, ctx->stream, a.ia, a.ib, a.shapes,
-                       a.nshapes, npairs, n_dev, ovf, T.blist.as<uint32_t>());
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(bblocks), dim3(BKT_BLOCK), 0, ctx->stream, T.bkey.as<uint32_t>(),
+                       npairs, n_dev, ovf, T.blist.as<uint32_t>());
     NCX_HIP_CHECK(hipGetLastError());
     KArgs t = a;
     t.list = T.blist.as<uint32_t>();
