@@ -76,7 +76,8 @@ struct DevBuf {
 // bucket's range in the partitioned list, and each sort key's count and scatter cursor.  From
 // NCX_WORK_BASE on come the work counters of the launches (the tuning samples' first tier, the
 // second, third and fourth tiers, one per bucket), each on its own cache line.
-constexpr uint32_t NCX_NKEYS = 512;  // sort keys: vertex-count bucket x 16 shape-type pairs x 8 overlap bins
+constexpr uint32_t NCX_OVERLAP_BINS = 8;  // 4 and 16 bins ordered the scene's pairs no better
+constexpr uint32_t NCX_NKEYS = 64 * NCX_OVERLAP_BINS;  // sort keys: vertex-count bucket x 16 shape-type pairs x overlap bins
 constexpr uint32_t NCX_OVF_T1_COUNT = 0, NCX_OVF_BAD = 1, NCX_OVF_T2_COUNT = 2, NCX_OVF_T3_COUNT = 3,
                    NCX_OVF_T4_COUNT = 4, NCX_OVF_TIER_WORDS = 8, NCX_BKT_RANGE = 8, NCX_KEY_CNT = 16,
                    NCX_KEY_FILL = NCX_KEY_CNT + NCX_NKEYS, NCX_OVF_COUNTS = NCX_KEY_FILL + NCX_NKEYS;
@@ -87,7 +88,7 @@ constexpr uint32_t NCX_WORK_SET = 64, NCX_WORK_BASE = NCX_OVF_COUNTS, NCX_WORK_T
 // the kernel variant the autotuner picked for it.
 constexpr int NCX_NBUCKETS = 4;
 static_assert(NCX_OVF_T4_COUNT < NCX_OVF_TIER_WORDS && NCX_OVF_TIER_WORDS <= NCX_BKT_RANGE &&
-                  NCX_BKT_RANGE + 2 * NCX_NBUCKETS <= NCX_KEY_CNT && NCX_NKEYS == 128 * NCX_NBUCKETS &&
+                  NCX_BKT_RANGE + 2 * NCX_NBUCKETS <= NCX_KEY_CNT && NCX_NKEYS == 16 * NCX_OVERLAP_BINS * NCX_NBUCKETS &&
                   NCX_WORK_BKT + NCX_NBUCKETS <= NCX_WORK_SETS,
               "overflow header layout");
 // The work counter of launch k.

@@ -1573,9 +1573,9 @@ __device__ __forceinline__ uint32_t pair_key(const ncx_shape* __restrict__ shape
   uint32_t t1 = s1.type < 3u ? s1.type : 3u, t2 = s2.type < 3u ? s2.type : 3u;
   const float dx = poses[i1].tx - poses[i2].tx, dy = poses[i1].ty - poses[i2].ty, dz = poses[i1].tz - poses[i2].tz;
   const float rs = __uint_as_float(s1.pad[0]) + __uint_as_float(s2.pad[0]);
-  const float bin = 8.f * __builtin_sqrtf(dx * dx + dy * dy + dz * dz) / rs;
-  const uint32_t o = bin < 7.f ? (uint32_t)bin : 7u;  // NaN (rs = 0) and large go to bin 7
-  return 128u * b + 8u * (4u * (t1 < t2 ? t1 : t2) + (t1 < t2 ? t2 : t1)) + o;
+  const float bin = (float)NCX_OVERLAP_BINS * __builtin_sqrtf(dx * dx + dy * dy + dz * dz) / rs;
+  const uint32_t o = bin < (float)(NCX_OVERLAP_BINS - 1) ? (uint32_t)bin : NCX_OVERLAP_BINS - 1;  // NaN and large: last bin
+  return 16u * NCX_OVERLAP_BINS * b + NCX_OVERLAP_BINS * (4u * (t1 < t2 ? t1 : t2) + (t1 < t2 ? t2 : t1)) + o;
 }
 
 // Bounding radius of each shape about its local origin, into pad[0] of its device record (a
