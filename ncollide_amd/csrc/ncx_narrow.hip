@@ -605,7 +605,7 @@ __device__ __forceinline__ void heap_sift_up(HeapE* h, uint32_t pos) {
 // (nearest first) is where the climb stops.  The ancestors below it each move down one level
 // along the path and the element takes the last one's slot, which is what the loop leaves.
 // Needs as many lanes as the heap has levels above pos.
-template <int G>
+template <int G, bool GMEM>
 __device__ __forceinline__ void heap_sift_up_par(HeapE* h, uint32_t pos, int gl) {
   const HeapE e = h[pos];
   const uint32_t k = pos + 1;
@@ -623,9 +623,11 @@ __device__ __forceinline__ void heap_sift_up_par(HeapE* h, uint32_t pos, int gl)
   __builtin_amdgcn_wave_barrier();
   if (gl == 0) h[s == 0 ? pos : (k >> s) - 1u] = e;
   __builtin_amdgcn_wave_barrier();
+  // A device-memory heap: the other lanes' stores must have landed before anyone reads the heap.
+  if constexpr (GMEM) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
 }
 
-template <int G, bool PAR>
+template <int G, int PAR>
 __device__ __forceinline__ bool heap_pop(HeapE* h, uint32_t& len, HeapE& out, int gl) {
   if (len == 0) return false;
   len -= 1;
@@ -644,7 +646,7 @@ __device__ __forceinline__ bool heap_pop(HeapE* h, uint32_t& len, HeapE& out, in
     }
     if (child == end - 1) { h[pos] = h[child]; pos = child; }
     h[pos] = e;
-    if constexpr (PAR) heap_sift_up_par<G>(h, pos, gl);
+    if constexpr (PAR != 0) heap_sift_up_par<G, PAR == 2>(h, pos, gl);
     else heap_sift_up(h, pos);
   }
   out = item;
@@ -797,8 +799,10 @@ __device__ __forceinline__ void epa_load(const uint8_t* slot, const Arena& A, in
 }
 
 // EPA::closest_points (epa3.rs:219-430).  compute_silhouette's recursion
-// (:432-454) becomes an explicit stack visiting faces in the same order.
-template <int G, bool PAR>
+// (:432-454) becomes an explicit stack visiting faces in the same order.  PAR picks how the
+// group's lanes share the work: 0 every lane runs the same sequential steps, 1 the lane-parallel
+// steps on an LDS arena, 2 the same on a device-memory arena (with workgroup fences).
+template <int G, int PAR>
 __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& P, int gl, const Arena& A, f3& p1, f3& p2, f3& n,
                    int& iters, const EpaIO& io) {
   const float eps_tol = NCX_FLT_EPS * 100.f;
@@ -872,7 +876,7 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
   }
   HeapE fid;
   for (;;) {
-    if constexpr (PAR) {
+    if constexpr (PAR != 0) {
       if (io.save && (nv >= A.vcap || nf + SUSPEND_MARGIN > A.fcap)) {
         uint32_t slot = 0;
         if (gl == 0) slot = atomicAdd(io.count, 1u);
@@ -904,7 +908,7 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
     A.faces[fid.id].deleted = 1;
     // compute_silhouette for the three neighbours, depth first.
     uint32_t ns = 0, sp = 0;
-    if constexpr (PAR) {
+    if constexpr (PAR != 0) {
       // Each stack entry's visibility (can_be_seen through the entry's edge) is computed when it
       // is pushed, the entries pushed together by different lanes of the group at once, and kept
       // in bit 15 of the entry: it depends only on the face's vertices, the new point and the
@@ -973,14 +977,15 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
     }
     uint32_t first_new = nf;
     if (ns == 0) return EPA_NONE;
-    if constexpr (PAR) {
+    if constexpr (PAR != 0) {
       // The new faces are made in parallel, silhouette edge q by lane q mod G of the group, and
       // then checked and pushed in edge order as the loop below does.  Which edges get a face
       // (their silhouette face still live), and so every new face's id, is known before any face
       // is made: making faces only appends faces and rewrites adjacency slots, never the deleted
       // flags or vertex ids read here.  Each edge's results go to its silhouette entry (bit 14:
       // has a face, bit 15: origin projects inside) and to the emptied search stack (the
-      // distance).  LDS arenas only: lanes of one wavefront see each other's LDS writes in order.
+      // distance).  Lanes of one wavefront see each other's LDS writes in order; a device-memory
+      // arena (PAR 2) fences at workgroup scope before the edges' results are read back.
       const int lane = __lane_id();
       const uint64_t gmask = G == 64 ? ~0ull : (((1ull << (G & 63)) - 1ull) << (lane & ~(G - 1)));
       const uint64_t below = (1ull << lane) - 1ull;
@@ -1011,6 +1016,7 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
         nvalid += (uint32_t)__popcll(vm);
       }
       __builtin_amdgcn_wave_barrier();
+      if constexpr (PAR == 2) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
       if (nf + nvalid > A.fcap) return EPA_OVERFLOW;
       for (uint32_t q = 0; q < ns; ++q) {
         const uint32_t fl = A.sil[q].opp;
@@ -1025,7 +1031,7 @@ __device__ __forceinline__ int epa(const Ctx2& c, const Simplex& s, const Pool& 
           }
           if (-dist > gjk_tol) return EPA_NONE;
           A.heap[hl] = HeapE{-dist, new_id};
-          heap_sift_up_par<G>(A.heap, hl, gl);
+          heap_sift_up_par<G, PAR == 2>(A.heap, hl, gl);
           hl++;
         }
       }
@@ -1078,7 +1084,7 @@ struct PairOut {
 };
 
 // contact_support_map_support_map_with_params (contact_support_map_support_map.rs:38-88)
-template <int G, bool PAR>
+template <int G, int PAR>
 __device__ __forceinline__ PairOut contact_sm(const Ctx2& c, float prediction, bool has_init, f3 init_dir, const Pool& P,
                               int gl, const Arena& A, int max_iter, const EpaIO& io) {
   PairOut o;
@@ -1176,7 +1182,7 @@ __device__ __forceinline__ f3 cuboid_feature_normal(int fkind, int fid) {
 }
 
 // contact_ball_convex_polyhedron (contact_ball_convex_polyhedron.rs:11-58), Cuboid or ConvexHull.
-template <int G, bool PAR>
+template <int G, int PAR>
 __device__ __forceinline__ PairOut contact_ball_convex(f3 c1, float r, const iso& m2, const ShapeD& g2, float prediction,
                                        const Pool& P, int gl, const Arena& A, const EpaIO& io) {
   PairOut o;
@@ -1239,7 +1245,7 @@ __device__ __forceinline__ PairOut contact_ball_convex(f3 c1, float r, const iso
 }
 
 // query::contact dispatch (contact_shape_shape.rs:10-48).
-template <int G, bool PAR>
+template <int G, int PAR>
 __device__ __forceinline__ PairOut contact_pair(const Ctx2& c, float prediction, bool has_init, f3 init_dir, const Pool& P,
                                 int gl, const Arena& A, int max_iter, const EpaIO& io) {
   bool b1 = c.g1.type == NCX_BALL, b2 = c.g2.type == NCX_BALL;
@@ -1455,12 +1461,12 @@ __global__ __launch_bounds__(BS, MODE == 1 ? 3 : 2) void k_pairs(KArgs a) {
       // An all-zero entry means None for this pair (the reference's default direction).
       has_init = idir.x != 0.f || idir.y != 0.f || idir.z != 0.f;
     }
-    PairOut o = contact_pair<G, LDS>(c, a.prediction, has_init, idir, a.P, gl, A, has_init ? a.max_iter : 10000, io);
+    PairOut o = contact_pair<G, LDS ? 1 : 2>(c, a.prediction, has_init, idir, a.P, gl, A, has_init ? a.max_iter : 10000, io);
     int extra_iters = 0;
     if (o.has == 3) {
       // Warm start hit its iteration cap: plain GJK (init_dir = None) judges the pair.
       extra_iters = o.gjk_iters;
-      o = contact_pair<G, LDS>(c, a.prediction, false, idir, a.P, gl, A, 10000, io);
+      o = contact_pair<G, LDS ? 1 : 2>(c, a.prediction, false, idir, a.P, gl, A, 10000, io);
     }
     if (o.has == 4) continue;  // suspended: listed (and saved) for the next tier
     if (o.has == 2) {
