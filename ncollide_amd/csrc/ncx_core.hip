@@ -443,6 +443,8 @@ __global__ __launch_bounds__(REFIT_B) void k_refit_local(BoxPtrs bx, const uint3
     uint32_t par = pe & ~NCX_RIGHT_BIT;
     bool right = (pe & NCX_RIGHT_BIT) != 0;
     put_slot(&nodes[par], right, b);
+    // The next parent is fetched with the range test's loads, so a level costs one round trip.
+    const uint32_t pe_next = parent[par];
     int fl = (int)first[par], la = (int)nodes[par].last_r;
     if (fl < b0 || la > b1) return;  // spans workgroups: stage 2
     int slot = (int)par - b0;
@@ -456,38 +458,49 @@ __global__ __launch_bounds__(REFIT_B) void k_refit_local(BoxPtrs bx, const uint3
       b[3 + d] = fmaxf(lbox[slot][3 + d], lbox[slot][9 + d]);
     }
     if (par == 0) return;
-    pe = parent[par];
+    pe = pe_next;
   }
 }
 
-__global__ __launch_bounds__(1024) void k_refit_upper(LNode* __restrict__ nodes, const uint32_t* __restrict__ parent,
-                                                       const uint32_t* __restrict__ upper,
-                                                       const uint32_t* __restrict__ level_off, int nlevels) {
-  for (int lv = 0; lv < nlevels; ++lv) {
-    uint32_t a = level_off[lv], e = level_off[lv + 1];
-    for (uint32_t i = a + threadIdx.x; i < e; i += blockDim.x) {
-      uint32_t p = upper[i];
-      const float* f = (const float*)&nodes[p];
+// Stage-2 entries held in LDS by k_refit_upper (64 KiB); entries past it are read from memory.
+#define UPPER_LDS 8192
+
+// Stage 2: level by level, deepest first; each entry is (node, the node's parent entry).  The
+// level offsets and the entries are loaded once, so a level costs one round trip (the node's
+// child slots, written by stage 1 or by the level below) and a barrier; empty levels cost nothing.
+__global__ __launch_bounds__(1024) void k_refit_upper(LNode* __restrict__ nodes, const uint2* __restrict__ upper,
+                                                       const uint32_t* __restrict__ level_off) {
+  __shared__ uint32_t off[LBVH_MAX_DEPTH + 1];
+  __shared__ uint2 ent[UPPER_LDS];
+  const uint32_t tid = threadIdx.x;
+  if (tid <= LBVH_MAX_DEPTH) off[tid] = level_off[tid];
+  __syncthreads();
+  const uint32_t total = off[LBVH_MAX_DEPTH];
+  for (uint32_t i = tid; i < total && i < UPPER_LDS; i += blockDim.x) ent[i] = upper[i];
+  __syncthreads();
+  for (int lv = 0; lv < (int)LBVH_MAX_DEPTH; ++lv) {
+    const uint32_t a = off[lv], e = off[lv + 1];
+    if (a == e) continue;  // uniform across the workgroup
+    for (uint32_t i = a + tid; i < e; i += blockDim.x) {
+      const uint2 en = i < UPPER_LDS ? ent[i] : upper[i];
+      const float* f = (const float*)&nodes[en.x];
       float b[6];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         b[d] = fminf(f[d], f[6 + d]);
         b[3 + d] = fmaxf(f[3 + d], f[9 + d]);
       }
-      if (p != 0) {
-        uint32_t pe = parent[p];
-        put_slot(&nodes[pe & ~NCX_RIGHT_BIT], (pe & NCX_RIGHT_BIT) != 0, b);
-      }
+      if (en.x != 0) put_slot(&nodes[en.y & ~NCX_RIGHT_BIT], (en.y & NCX_RIGHT_BIT) != 0, b);
     }
     __syncthreads();
   }
 }
 
-// Stage-2 nodes (leaf range spanning refit workgroups) and their depth; LBVH_MAX_DEPTH marks the
-// others.  Counts stage-2 nodes per depth in cnt[0..LBVH_MAX_DEPTH).
+// Stage-2 nodes (leaf range spanning refit workgroups): flag and depth (LBVH_MAX_DEPTH for the
+// others).  No atomics: a few thousand nodes counting into 64 shared counters serialise.
 __global__ __launch_bounds__(256) void k_upper_mark(const LNode* __restrict__ nodes, const uint32_t* __restrict__ parent,
                                                      const uint32_t* __restrict__ first, int n,
-                                                     uint32_t* __restrict__ depth, uint32_t* __restrict__ cnt) {
+                                                     uint32_t* __restrict__ depth, uint32_t* __restrict__ flag) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const bool upper = first[i] / REFIT_B != nodes[i].last_r / REFIT_B;
@@ -499,54 +512,72 @@ __global__ __launch_bounds__(256) void k_upper_mark(const LNode* __restrict__ no
       p = parent[p] & ~NCX_RIGHT_BIT;
       d++;
     }
-    atomicAdd(&cnt[d], 1u);
   }
   depth[i] = d;
+  flag[i] = upper ? 1u : 0u;
 }
 
-// Level offsets, deepest level first: off[lv] for lv = 0..LBVH_MAX_DEPTH (level lv holds depth
-// LBVH_MAX_DEPTH - 1 - lv), and each depth's scatter cursor.
-__global__ void k_upper_offsets(const uint32_t* __restrict__ cnt, uint32_t* __restrict__ off,
-                                uint32_t* __restrict__ cursor) {
-  if (threadIdx.x != 0) return;
-  uint32_t o = 0;
-  for (uint32_t lv = 0; lv < LBVH_MAX_DEPTH; ++lv) {
-    const uint32_t d = LBVH_MAX_DEPTH - 1 - lv;
-    off[lv] = o;
-    cursor[d] = o;
-    o += cnt[d];
-  }
-  off[LBVH_MAX_DEPTH] = o;
-}
-
-// The stage-2 list, grouped by level (order within a level is free: a level's nodes write
-// distinct parent slots).
-__global__ __launch_bounds__(256) void k_upper_scatter(const uint32_t* __restrict__ depth, int m,
-                                                        uint32_t* __restrict__ cursor, uint32_t* __restrict__ upper) {
+__global__ __launch_bounds__(256) void k_upper_compact(const uint32_t* __restrict__ flag,
+                                                        const uint32_t* __restrict__ pos, int m,
+                                                        uint32_t* __restrict__ list) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  const uint32_t d = depth[i];
-  if (d < LBVH_MAX_DEPTH) upper[atomicAdd(&cursor[d], 1u)] = (uint32_t)i;
+  if (i < m && flag[i]) list[pos[i]] = (uint32_t)i;
+}
+
+// One workgroup: counting sort of the compacted stage-2 nodes by depth, deepest first, into
+// (node, parent entry) pairs, with level offsets off[lv] for lv = 0..LBVH_MAX_DEPTH (level lv
+// holds depth LBVH_MAX_DEPTH - 1 - lv).  Order within a level is free: a level's nodes write
+// distinct parent slots.
+__global__ __launch_bounds__(1024) void k_upper_sort(const uint32_t* __restrict__ list, const uint32_t* __restrict__ total_p,
+                                                      const uint32_t* __restrict__ depth,
+                                                      const uint32_t* __restrict__ parent, uint2* __restrict__ upper,
+                                                      uint32_t* __restrict__ off) {
+  __shared__ uint32_t cnt[LBVH_MAX_DEPTH], cur[LBVH_MAX_DEPTH];
+  const uint32_t tid = threadIdx.x;
+  if (tid < LBVH_MAX_DEPTH) cnt[tid] = 0u;
+  __syncthreads();
+  const uint32_t total = *total_p;
+  for (uint32_t i = tid; i < total; i += blockDim.x) atomicAdd(&cnt[depth[list[i]]], 1u);
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t o = 0;
+    for (uint32_t lv = 0; lv < LBVH_MAX_DEPTH; ++lv) {
+      const uint32_t d = LBVH_MAX_DEPTH - 1 - lv;
+      off[lv] = o;
+      cur[d] = o;
+      o += cnt[d];
+    }
+    off[LBVH_MAX_DEPTH] = o;
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < total; i += blockDim.x) {
+    const uint32_t p = list[i];
+    const uint32_t slot = atomicAdd(&cur[depth[p]], 1u);
+    upper[slot] = make_uint2(p, p != 0 ? parent[p] : 0u);
+  }
 }
 
 // Stage-2 node list for a new topology, on the device (no host synchronisation).
 static int refit_prepare(ncx_ctx* ctx, LBVH& bvh) {
   const size_t m = bvh.n - 1;
   int rc;
-  if ((rc = bvh.node_depth.reserve(m * 4)) || (rc = bvh.upper.reserve(m * 4)) ||
-      (rc = bvh.upper_off.reserve((3 * (size_t)LBVH_MAX_DEPTH + 1) * 4)))
+  if ((rc = bvh.node_depth.reserve(m * 4)) || (rc = bvh.upper.reserve(m * 8)) ||
+      (rc = bvh.upper_off.reserve(((size_t)LBVH_MAX_DEPTH + 2) * 4)) || (rc = bvh.flags.reserve(m * 4)) ||
+      (rc = bvh.tmp_keys.reserve(m * 4)) || (rc = bvh.tmp_vals.reserve(m * 4)))
     return rc;
-  uint32_t* cnt = bvh.upper_off.as<uint32_t>() + LBVH_MAX_DEPTH + 1;
-  uint32_t* cursor = cnt + LBVH_MAX_DEPTH;
-  if ((rc = fill_u32(ctx, cnt, LBVH_MAX_DEPTH, 0u))) return rc;
+  uint32_t* total = bvh.upper_off.as<uint32_t>() + LBVH_MAX_DEPTH + 1;
   hipLaunchKernelGGL(k_upper_mark, dim3(ceil_div(m, 256)), dim3(256), 0, ctx->stream, bvh.nodes.as<LNode>(),
                      bvh.parent.as<uint32_t>(), bvh.first.as<uint32_t>(), (int)bvh.n, bvh.node_depth.as<uint32_t>(),
-                     cnt);
+                     bvh.flags.as<uint32_t>());
   NCX_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_upper_offsets, dim3(1), dim3(64), 0, ctx->stream, cnt, bvh.upper_off.as<uint32_t>(), cursor);
+  if ((rc = scan_exclusive_u32(ctx, bvh.flags.as<uint32_t>(), bvh.tmp_keys.as<uint32_t>(), m, bvh.scan_tmp, total)))
+    return rc;
+  hipLaunchKernelGGL(k_upper_compact, dim3(ceil_div(m, 256)), dim3(256), 0, ctx->stream, bvh.flags.as<uint32_t>(),
+                     bvh.tmp_keys.as<uint32_t>(), (int)m, bvh.tmp_vals.as<uint32_t>());
   NCX_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_upper_scatter, dim3(ceil_div(m, 256)), dim3(256), 0, ctx->stream,
-                     bvh.node_depth.as<uint32_t>(), (int)m, cursor, bvh.upper.as<uint32_t>());
+  hipLaunchKernelGGL(k_upper_sort, dim3(1), dim3(1024), 0, ctx->stream, bvh.tmp_vals.as<uint32_t>(), total,
+                     bvh.node_depth.as<uint32_t>(), bvh.parent.as<uint32_t>(), bvh.upper.as<uint2>(),
+                     bvh.upper_off.as<uint32_t>());
   NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
 }
@@ -560,8 +591,7 @@ static int refit_launch(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], floa
                      bvh.first.as<uint32_t>(), expand);
   NCX_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_refit_upper, dim3(1), dim3(1024), 0, ctx->stream, bvh.nodes.as<LNode>(),
-                     bvh.parent.as<uint32_t>(), bvh.upper.as<uint32_t>(), bvh.upper_off.as<uint32_t>(),
-                     (int)LBVH_MAX_DEPTH);
+                     bvh.upper.as<uint2>(), bvh.upper_off.as<uint32_t>());
   NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
 }
