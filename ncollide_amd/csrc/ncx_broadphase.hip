@@ -286,9 +286,9 @@ __global__ void k_tree_depth(const uint32_t* __restrict__ parent, uint32_t n, ui
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   uint32_t d = 1;
-  uint32_t p = parent[(n - 1) + k] & ~NCX_RIGHT_BIT;
+  uint32_t p = parent[(n - 1) + k] & NCX_PARENT_MASK;
   while (p != 0 && d <= n) {
-    p = parent[p] & ~NCX_RIGHT_BIT;
+    p = parent[p] & NCX_PARENT_MASK;
     d++;
   }
   atomicMax(maxdepth, d);

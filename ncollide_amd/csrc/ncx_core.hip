@@ -399,7 +399,7 @@ int fill_u32(ncx_ctx* ctx, uint32_t* p, size_t n, uint32_t v) {
 // range lies inside the workgroup's leaves, with flags and boxes in LDS (the second child to
 // arrive merges).  Stage 2 (one workgroup) refits the few nodes whose range spans workgroups,
 // deepest level first, with a barrier between levels.  No device-scope synchronisation is needed.
-#define REFIT_B 256
+#define REFIT_B 1024
 
 __device__ __forceinline__ void leaf_box(const float* const* box, uint32_t p, float expand, float b[6]) {
 #pragma unroll
@@ -425,8 +425,7 @@ struct BoxPtrs {
 
 __global__ __launch_bounds__(REFIT_B) void k_refit_local(BoxPtrs bx, const uint32_t* __restrict__ order, int n,
                                                           LNode* __restrict__ nodes,
-                                                          const uint32_t* __restrict__ parent,
-                                                          const uint32_t* __restrict__ first, float expand) {
+                                                          const uint32_t* __restrict__ parent, float expand) {
   __shared__ uint32_t lflag[REFIT_B];
   __shared__ float lbox[REFIT_B][12];
   const int tid = threadIdx.x;
@@ -435,18 +434,15 @@ __global__ __launch_bounds__(REFIT_B) void k_refit_local(BoxPtrs bx, const uint3
   const int b0 = blockIdx.x * REFIT_B;
   const int k = b0 + tid;
   if (k >= n) return;
-  const int b1 = min(b0 + REFIT_B, n) - 1;
   float b[6];
   leaf_box(bx.p, order[k], expand, b);
   uint32_t pe = parent[(n - 1) + k];
   for (;;) {
-    uint32_t par = pe & ~NCX_RIGHT_BIT;
+    uint32_t par = pe & NCX_PARENT_MASK;
     bool right = (pe & NCX_RIGHT_BIT) != 0;
     put_slot(&nodes[par], right, b);
-    // The next parent is fetched with the range test's loads, so a level costs one round trip.
-    const uint32_t pe_next = parent[par];
-    int fl = (int)first[par], la = (int)nodes[par].last_r;
-    if (fl < b0 || la > b1) return;  // spans workgroups: stage 2
+    // The span bit (the parent's leaf range leaves this workgroup's) saves loading the range.
+    if (pe & NCX_SPAN_BIT) return;  // stage 2
     int slot = (int)par - b0;
 #pragma unroll
     for (int q = 0; q < 6; ++q) lbox[slot][(right ? 6 : 0) + q] = b[q];
@@ -458,47 +454,94 @@ __global__ __launch_bounds__(REFIT_B) void k_refit_local(BoxPtrs bx, const uint3
       b[3 + d] = fmaxf(lbox[slot][3 + d], lbox[slot][9 + d]);
     }
     if (par == 0) return;
-    pe = pe_next;
+    pe = parent[par];
   }
 }
 
 // Stage-2 entries held in LDS by k_refit_upper (64 KiB); entries past it are read from memory.
-#define UPPER_LDS 8192
+#define UPPER_LDS 3072
+#define UPPER_THREADS 1024
+#define UPPER_NONE 0xFFFFFFFFu
 
-// Stage 2: level by level, deepest first; each entry is (node, the node's parent entry).  The
-// level offsets and the entries are loaded once, so a level costs one round trip (the node's
-// child slots, written by stage 1 or by the level below) and a barrier; empty levels cost nothing.
-__global__ __launch_bounds__(1024) void k_refit_upper(LNode* __restrict__ nodes, const uint2* __restrict__ upper,
-                                                       const uint32_t* __restrict__ level_off) {
+__device__ __forceinline__ void merge_slots(const float* f, float b[6]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    b[d] = fminf(f[d], f[6 + d]);
+    b[3 + d] = fmaxf(f[3 + d], f[9 + d]);
+  }
+}
+
+// Stage 2, one workgroup, level by level, deepest first.  Entry = (node, the node's parent entry,
+// the stage-2 entry of its left child, of its right child; UPPER_NONE for a child refitted by
+// stage 1).  Up to UPPER_LDS entries: every node's child slots are fetched at the start (one
+// round trip for all levels), a stage-2 child's box comes from LDS, and a level costs a barrier;
+// every box is also stored into its parent's slot for the traversals.  Past UPPER_LDS entries:
+// the same levels through device memory, one round trip per level.
+__global__ __launch_bounds__(UPPER_THREADS) void k_refit_upper(LNode* __restrict__ nodes, const uint4* __restrict__ upper,
+                                                                const uint32_t* __restrict__ level_off) {
   __shared__ uint32_t off[LBVH_MAX_DEPTH + 1];
-  __shared__ uint2 ent[UPPER_LDS];
+  __shared__ uint4 ent[UPPER_LDS];
+  __shared__ float mb[UPPER_LDS][6];
+  constexpr int J = UPPER_LDS / UPPER_THREADS;
   const uint32_t tid = threadIdx.x;
   if (tid <= LBVH_MAX_DEPTH) off[tid] = level_off[tid];
   __syncthreads();
   const uint32_t total = off[LBVH_MAX_DEPTH];
-  for (uint32_t i = tid; i < total && i < UPPER_LDS; i += blockDim.x) ent[i] = upper[i];
-  __syncthreads();
+  if (total <= UPPER_LDS) {
+    float slot[J][12];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const uint32_t i = tid + j * UPPER_THREADS;
+      if (i < total) {
+        const uint4 en = upper[i];
+        ent[i] = en;
+        const float4* f = (const float4*)&nodes[en.x];
+        const float4 f0 = f[0], f1 = f[1], f2 = f[2];
+        slot[j][0] = f0.x; slot[j][1] = f0.y; slot[j][2] = f0.z; slot[j][3] = f0.w;
+        slot[j][4] = f1.x; slot[j][5] = f1.y; slot[j][6] = f1.z; slot[j][7] = f1.w;
+        slot[j][8] = f2.x; slot[j][9] = f2.y; slot[j][10] = f2.z; slot[j][11] = f2.w;
+      }
+    }
+    __syncthreads();
+    for (int lv = 0; lv < (int)LBVH_MAX_DEPTH; ++lv) {
+      const uint32_t a = off[lv], e = off[lv + 1];
+      if (a == e) continue;  // uniform across the workgroup
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const uint32_t i = tid + j * UPPER_THREADS;
+        if (i < a || i >= e) continue;
+        const uint4 en = ent[i];
+        if (en.z != UPPER_NONE)
+          for (int q = 0; q < 6; ++q) slot[j][q] = mb[en.z][q];
+        if (en.w != UPPER_NONE)
+          for (int q = 0; q < 6; ++q) slot[j][6 + q] = mb[en.w][q];
+        float b[6];
+        merge_slots(slot[j], b);
+        for (int q = 0; q < 6; ++q) mb[i][q] = b[q];
+        if (en.x != 0) put_slot(&nodes[en.y & NCX_PARENT_MASK], (en.y & NCX_RIGHT_BIT) != 0, b);
+      }
+      __syncthreads();
+    }
+    return;
+  }
   for (int lv = 0; lv < (int)LBVH_MAX_DEPTH; ++lv) {
     const uint32_t a = off[lv], e = off[lv + 1];
-    if (a == e) continue;  // uniform across the workgroup
-    for (uint32_t i = a + tid; i < e; i += blockDim.x) {
-      const uint2 en = i < UPPER_LDS ? ent[i] : upper[i];
-      const float* f = (const float*)&nodes[en.x];
+    if (a == e) continue;
+    for (uint32_t i = a + tid; i < e; i += UPPER_THREADS) {
+      const uint4 en = upper[i];
       float b[6];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        b[d] = fminf(f[d], f[6 + d]);
-        b[3 + d] = fmaxf(f[3 + d], f[9 + d]);
-      }
-      if (en.x != 0) put_slot(&nodes[en.y & ~NCX_RIGHT_BIT], (en.y & NCX_RIGHT_BIT) != 0, b);
+      merge_slots((const float*)&nodes[en.x], b);
+      if (en.x != 0) put_slot(&nodes[en.y & NCX_PARENT_MASK], (en.y & NCX_RIGHT_BIT) != 0, b);
     }
     __syncthreads();
   }
 }
 
 // Stage-2 nodes (leaf range spanning refit workgroups): flag and depth (LBVH_MAX_DEPTH for the
-// others).  No atomics: a few thousand nodes counting into 64 shared counters serialise.
-__global__ __launch_bounds__(256) void k_upper_mark(const LNode* __restrict__ nodes, const uint32_t* __restrict__ parent,
+// others), and the span bit in their children's parent entries.  No atomics: a few thousand
+// nodes counting into 64 shared counters serialise.  Each parent entry is rewritten only by its
+// parent's thread; the depth walks read entries concurrently and mask the bit off.
+__global__ __launch_bounds__(256) void k_upper_mark(const LNode* __restrict__ nodes, uint32_t* __restrict__ parent,
                                                      const uint32_t* __restrict__ first, int n,
                                                      uint32_t* __restrict__ depth, uint32_t* __restrict__ flag) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -506,10 +549,15 @@ __global__ __launch_bounds__(256) void k_upper_mark(const LNode* __restrict__ no
   const bool upper = first[i] / REFIT_B != nodes[i].last_r / REFIT_B;
   uint32_t d = LBVH_MAX_DEPTH;
   if (upper) {
+    const uint32_t cl = nodes[i].child_l, cr = nodes[i].child_r;
+    const uint32_t el = (cl & NCX_LEAF_BIT) ? (uint32_t)(n - 1) + (cl & ~NCX_LEAF_BIT) : cl;
+    const uint32_t er = (cr & NCX_LEAF_BIT) ? (uint32_t)(n - 1) + (cr & ~NCX_LEAF_BIT) : cr;
+    parent[el] |= NCX_SPAN_BIT;
+    parent[er] |= NCX_SPAN_BIT;
     d = 0;
     uint32_t p = (uint32_t)i;
     while (p != 0 && d < LBVH_MAX_DEPTH - 1) {
-      p = parent[p] & ~NCX_RIGHT_BIT;
+      p = parent[p] & NCX_PARENT_MASK;
       d++;
     }
   }
@@ -518,42 +566,77 @@ __global__ __launch_bounds__(256) void k_upper_mark(const LNode* __restrict__ no
 }
 
 __global__ __launch_bounds__(256) void k_upper_compact(const uint32_t* __restrict__ flag,
-                                                        const uint32_t* __restrict__ pos, int m,
-                                                        uint32_t* __restrict__ list) {
+                                                        const uint32_t* __restrict__ pos,
+                                                        const uint32_t* __restrict__ depth,
+                                                        const uint32_t* __restrict__ parent, int m,
+                                                        uint4* __restrict__ list) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m && flag[i]) list[pos[i]] = (uint32_t)i;
+  if (i < m && flag[i]) list[pos[i]] = make_uint4((uint32_t)i, depth[i], i != 0 ? parent[i] : 0u, 0u);
 }
 
-// One workgroup: counting sort of the compacted stage-2 nodes by depth, deepest first, into
-// (node, parent entry) pairs, with level offsets off[lv] for lv = 0..LBVH_MAX_DEPTH (level lv
-// holds depth LBVH_MAX_DEPTH - 1 - lv).  Order within a level is free: a level's nodes write
-// distinct parent slots.
-__global__ __launch_bounds__(1024) void k_upper_sort(const uint32_t* __restrict__ list, const uint32_t* __restrict__ total_p,
-                                                      const uint32_t* __restrict__ depth,
-                                                      const uint32_t* __restrict__ parent, uint2* __restrict__ upper,
+// One workgroup: counting sort of the compacted stage-2 nodes (node, depth, parent entry) by
+// depth, deepest first, into k_refit_upper's entries, with level offsets off[lv] for lv =
+// 0..LBVH_MAX_DEPTH (level lv holds depth LBVH_MAX_DEPTH - 1 - lv).  Order within a level is
+// free: a level's nodes write distinct parent slots.  Each thread loads its entries of a chunk
+// at once, then counts or places them; pos[] (node -> entry) then links each entry to its
+// children's.
+#define SORT_PER_THREAD 8
+__global__ __launch_bounds__(1024) void k_upper_sort(const uint4* __restrict__ list, const uint32_t* __restrict__ total_p,
+                                                      const LNode* __restrict__ nodes, const uint32_t* __restrict__ flag,
+                                                      uint32_t* __restrict__ pos, uint4* __restrict__ upper,
                                                       uint32_t* __restrict__ off) {
   __shared__ uint32_t cnt[LBVH_MAX_DEPTH], cur[LBVH_MAX_DEPTH];
   const uint32_t tid = threadIdx.x;
+  const uint32_t chunk = SORT_PER_THREAD * blockDim.x;
   if (tid < LBVH_MAX_DEPTH) cnt[tid] = 0u;
   __syncthreads();
   const uint32_t total = *total_p;
-  for (uint32_t i = tid; i < total; i += blockDim.x) atomicAdd(&cnt[depth[list[i]]], 1u);
+  for (uint32_t c0 = 0; c0 < total; c0 += chunk) {
+    uint32_t d[SORT_PER_THREAD];
+#pragma unroll
+    for (int j = 0; j < SORT_PER_THREAD; ++j) {
+      const uint32_t i = c0 + tid + j * blockDim.x;
+      d[j] = i < total ? list[i].y : LBVH_MAX_DEPTH;
+    }
+#pragma unroll
+    for (int j = 0; j < SORT_PER_THREAD; ++j)
+      if (d[j] < LBVH_MAX_DEPTH) atomicAdd(&cnt[d[j]], 1u);
+  }
   __syncthreads();
   if (tid == 0) {
     uint32_t o = 0;
     for (uint32_t lv = 0; lv < LBVH_MAX_DEPTH; ++lv) {
-      const uint32_t d = LBVH_MAX_DEPTH - 1 - lv;
+      const uint32_t dd = LBVH_MAX_DEPTH - 1 - lv;
       off[lv] = o;
-      cur[d] = o;
-      o += cnt[d];
+      cur[dd] = o;
+      o += cnt[dd];
     }
     off[LBVH_MAX_DEPTH] = o;
   }
   __syncthreads();
-  for (uint32_t i = tid; i < total; i += blockDim.x) {
-    const uint32_t p = list[i];
-    const uint32_t slot = atomicAdd(&cur[depth[p]], 1u);
-    upper[slot] = make_uint2(p, p != 0 ? parent[p] : 0u);
+  for (uint32_t c0 = 0; c0 < total; c0 += chunk) {
+    uint4 e[SORT_PER_THREAD];
+#pragma unroll
+    for (int j = 0; j < SORT_PER_THREAD; ++j) {
+      const uint32_t i = c0 + tid + j * blockDim.x;
+      e[j] = i < total ? list[i] : make_uint4(0u, LBVH_MAX_DEPTH, 0u, 0u);
+    }
+#pragma unroll
+    for (int j = 0; j < SORT_PER_THREAD; ++j)
+      if (e[j].y < LBVH_MAX_DEPTH) {
+        const uint32_t slot = atomicAdd(&cur[e[j].y], 1u);
+        upper[slot] = make_uint4(e[j].x, e[j].z, UPPER_NONE, UPPER_NONE);
+        pos[e[j].x] = slot;
+      }
+  }
+  __syncthreads();  // pos[] and the entries, written above by this workgroup, are read below
+  for (uint32_t s = tid; s < total; s += blockDim.x) {
+    const uint32_t p = upper[s].x;
+    const uint32_t cl = nodes[p].child_l, cr = nodes[p].child_r;
+    const uint32_t el = (!(cl & NCX_LEAF_BIT) && flag[cl]) ? pos[cl] : UPPER_NONE;
+    const uint32_t er = (!(cr & NCX_LEAF_BIT) && flag[cr]) ? pos[cr] : UPPER_NONE;
+    upper[s].z = el;
+    upper[s].w = er;
   }
 }
 
@@ -561,9 +644,9 @@ __global__ __launch_bounds__(1024) void k_upper_sort(const uint32_t* __restrict_
 static int refit_prepare(ncx_ctx* ctx, LBVH& bvh) {
   const size_t m = bvh.n - 1;
   int rc;
-  if ((rc = bvh.node_depth.reserve(m * 4)) || (rc = bvh.upper.reserve(m * 8)) ||
+  if ((rc = bvh.node_depth.reserve(m * 4)) || (rc = bvh.upper.reserve(m * 16)) ||
       (rc = bvh.upper_off.reserve(((size_t)LBVH_MAX_DEPTH + 2) * 4)) || (rc = bvh.flags.reserve(m * 4)) ||
-      (rc = bvh.tmp_keys.reserve(m * 4)) || (rc = bvh.tmp_vals.reserve(m * 4)))
+      (rc = bvh.tmp_keys.reserve(m * 4)) || (rc = bvh.tmp_vals.reserve(m * 16)))
     return rc;
   uint32_t* total = bvh.upper_off.as<uint32_t>() + LBVH_MAX_DEPTH + 1;
   hipLaunchKernelGGL(k_upper_mark, dim3(ceil_div(m, 256)), dim3(256), 0, ctx->stream, bvh.nodes.as<LNode>(),
@@ -573,11 +656,12 @@ static int refit_prepare(ncx_ctx* ctx, LBVH& bvh) {
   if ((rc = scan_exclusive_u32(ctx, bvh.flags.as<uint32_t>(), bvh.tmp_keys.as<uint32_t>(), m, bvh.scan_tmp, total)))
     return rc;
   hipLaunchKernelGGL(k_upper_compact, dim3(ceil_div(m, 256)), dim3(256), 0, ctx->stream, bvh.flags.as<uint32_t>(),
-                     bvh.tmp_keys.as<uint32_t>(), (int)m, bvh.tmp_vals.as<uint32_t>());
+                     bvh.tmp_keys.as<uint32_t>(), bvh.node_depth.as<uint32_t>(), bvh.parent.as<uint32_t>(), (int)m,
+                     bvh.tmp_vals.as<uint4>());
   NCX_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_upper_sort, dim3(1), dim3(1024), 0, ctx->stream, bvh.tmp_vals.as<uint32_t>(), total,
-                     bvh.node_depth.as<uint32_t>(), bvh.parent.as<uint32_t>(), bvh.upper.as<uint2>(),
-                     bvh.upper_off.as<uint32_t>());
+  hipLaunchKernelGGL(k_upper_sort, dim3(1), dim3(1024), 0, ctx->stream, bvh.tmp_vals.as<uint4>(), total,
+                     bvh.nodes.as<LNode>(), bvh.flags.as<uint32_t>(), bvh.tmp_keys.as<uint32_t>(),
+                     bvh.upper.as<uint4>(), bvh.upper_off.as<uint32_t>());
   NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
 }
@@ -587,18 +671,17 @@ static int refit_launch(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], floa
   BoxPtrs bp;
   for (int d = 0; d < 6; ++d) bp.p[d] = box[d];
   hipLaunchKernelGGL(k_refit_local, dim3(ceil_div(n, REFIT_B)), dim3(REFIT_B), 0, ctx->stream, bp,
-                     bvh.order.as<uint32_t>(), (int)n, bvh.nodes.as<LNode>(), bvh.parent.as<uint32_t>(),
-                     bvh.first.as<uint32_t>(), expand);
+                     bvh.order.as<uint32_t>(), (int)n, bvh.nodes.as<LNode>(), bvh.parent.as<uint32_t>(), expand);
   NCX_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_refit_upper, dim3(1), dim3(1024), 0, ctx->stream, bvh.nodes.as<LNode>(),
-                     bvh.upper.as<uint2>(), bvh.upper_off.as<uint32_t>());
+  hipLaunchKernelGGL(k_refit_upper, dim3(1), dim3(UPPER_THREADS), 0, ctx->stream, bvh.nodes.as<LNode>(),
+                     bvh.upper.as<uint4>(), bvh.upper_off.as<uint32_t>());
   NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
 }
 
 int lbvh_build(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], size_t n, float expand) {
   int rc;
-  NCX_REQUIRE(n < (size_t)0x7FFFFFFF, "lbvh_build: too many primitives");
+  NCX_REQUIRE(n <= (size_t)NCX_PARENT_MASK, "lbvh_build: too many primitives");
   bvh.n = n;
   if (n == 0) return NCX_OK;
   if ((rc = bvh.order.reserve(n * 4))) return rc;

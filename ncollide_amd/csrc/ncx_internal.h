@@ -195,8 +195,11 @@ __device__ __forceinline__ iso iso_identity() {
 // four 16-byte loads.  child_l/child_r: internal index, or (leaf position | LEAF_BIT).
 // last_l/last_r: the largest sorted leaf position under each child.
 #define NCX_LEAF_BIT 0x80000000u
-// In LBVH::parent entries the same bit means "right child of".
+// In LBVH::parent entries the same bit means "right child of", bit 30 that the parent's leaf
+// range spans refit workgroups (set once per topology), and the low 30 bits are the parent.
 #define NCX_RIGHT_BIT 0x80000000u
+#define NCX_SPAN_BIT 0x40000000u
+#define NCX_PARENT_MASK 0x3FFFFFFFu
 struct alignas(16) LNode {
   float lminx, lminy, lminz, lmaxx;
   float lmaxy, lmaxz, rminx, rminy;
@@ -207,7 +210,7 @@ struct alignas(16) LNode {
 struct LBVH {
   size_t n = 0;          // number of leaves
   DevBuf nodes;          // LNode[n-1]
-  DevBuf parent;         // uint32 parent of internal nodes [n-1] then leaves [n]; side bit 31
+  DevBuf parent;         // uint32 parent of internal nodes [n-1] then leaves [n]; side bit 31, span bit 30
   DevBuf order;          // uint32 sorted position -> primitive index
   DevBuf keys;           // Morton codes (sorted)
   DevBuf tmp_keys, tmp_vals, hist, flags, bounds, scan_tmp;

@@ -99,7 +99,7 @@ __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float 
 // its own cache line, so the atomics do not serialise on one line); a wave that finds its part
 // used up moves on to the next parts.
 #define WQ_LSTACK 16
-#define WQ_LOCAL 128
+#define WQ_LOCAL 512
 #define WQ_CHUNK 64u
 #define WQ_PARTS 8u
 #define WQ_PART_STRIDE 64u

@@ -84,3 +84,33 @@ def test_world_graph_replay_equals_eager(ctx):
             assert torch.equal(ra[k], rb[k]), (step, k)
         pa[:, 0:3] += vel
         pb[:, 0:3] += vel
+
+
+def test_world_refit_matches_rebuild_at_4m_shapes(ctx):
+    """Refitted trees (one topology kept for 5 steps) give the pair sets of trees rebuilt every
+    step.  At 4M shapes the refit's cross-workgroup node list exceeds what its one-workgroup
+    stage keeps in LDS, so the device-memory levels run; the smaller scenes above cover the LDS
+    levels."""
+    n = 1 << 22
+    w = workloads.plain_pipeline(seed=31, n=n)
+    rng = np.random.default_rng(31)
+    poses_np = w["poses"]
+    poses_np[:, 0:3] = rng.uniform(0.0, 320.0, (n, 3)).astype(np.float32)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    pa = torch.as_tensor(poses_np, device=ctx.device)
+    pb = pa.clone()
+    wa = World(ss, margin=0.02, max_pairs=1 << 23, rebuild_every=1)
+    wb = World(ss, margin=0.02, max_pairs=1 << 23, rebuild_every=6)
+    wa.set_contacts(False)
+    wb.set_contacts(False)
+    vel = torch.as_tensor(rng.uniform(-0.4, 0.4, (n, 3)).astype(np.float32), device=ctx.device)
+    for step in range(5):
+        wa.step(pa, use_graph=False)
+        wb.step(pb, use_graph=False)
+        ka = np.sort(keys_of(wa.results()))
+        kb = np.sort(keys_of(wb.results()))
+        print(f"step {step}: {len(ka)} pairs", flush=True)
+        assert len(ka) > 100000
+        assert np.array_equal(ka, kb), f"pairs differ at step {step}"
+        pa[:, 0:3] += vel
+        pb[:, 0:3] += vel
