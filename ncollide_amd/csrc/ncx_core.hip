@@ -401,9 +401,7 @@ int fill_u32(ncx_ctx* ctx, uint32_t* p, size_t n, uint32_t v) {
 // deepest level first, with a barrier between levels.  No device-scope synchronisation is needed.
 #define REFIT_B 1024
 
-__device__ __forceinline__ void leaf_box(const float* const* box, uint32_t p, float expand, float b[6]) {
-#pragma unroll
-  for (int d = 0; d < 6; ++d) b[d] = box[d][p];
+__device__ __forceinline__ void expand_box(float expand, float b[6]) {
   if (expand > 0.f) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -411,6 +409,12 @@ __device__ __forceinline__ void leaf_box(const float* const* box, uint32_t p, fl
       b[3 + d] += expand * (1.f + fabsf(b[3 + d]));
     }
   }
+}
+
+__device__ __forceinline__ void leaf_box(const float* const* box, uint32_t p, float expand, float b[6]) {
+#pragma unroll
+  for (int d = 0; d < 6; ++d) b[d] = box[d][p];
+  expand_box(expand, b);
 }
 
 __device__ __forceinline__ void put_slot(LNode* nd, bool right, const float b[6]) {
@@ -423,7 +427,9 @@ struct BoxPtrs {
   const float* p[6];
 };
 
-__global__ __launch_bounds__(REFIT_B) void k_refit_local(BoxPtrs bx, const uint32_t* __restrict__ order, int n,
+template <bool AOS>
+__global__ __launch_bounds__(REFIT_B) void k_refit_local(BoxPtrs bx, const float4* __restrict__ aos,
+                                                          const uint32_t* __restrict__ order, int n,
                                                           LNode* __restrict__ nodes,
                                                           const uint32_t* __restrict__ parent, float expand) {
   __shared__ uint32_t lflag[REFIT_B];
@@ -435,7 +441,14 @@ __global__ __launch_bounds__(REFIT_B) void k_refit_local(BoxPtrs bx, const uint3
   const int k = b0 + tid;
   if (k >= n) return;
   float b[6];
-  leaf_box(bx.p, order[k], expand, b);
+  if constexpr (AOS) {
+    const uint32_t p = order[k];
+    const float4 a0 = aos[2 * (size_t)p], a1 = aos[2 * (size_t)p + 1];
+    b[0] = a0.x; b[1] = a0.y; b[2] = a0.z; b[3] = a0.w; b[4] = a1.x; b[5] = a1.y;
+    expand_box(expand, b);
+  } else {
+    leaf_box(bx.p, order[k], expand, b);
+  }
   uint32_t pe = parent[(n - 1) + k];
   for (;;) {
     uint32_t par = pe & NCX_PARENT_MASK;
@@ -666,12 +679,16 @@ static int refit_prepare(ncx_ctx* ctx, LBVH& bvh) {
   return NCX_OK;
 }
 
-static int refit_launch(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], float expand) {
+static int refit_launch(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], float expand, const float4* aos) {
   size_t n = bvh.n;
   BoxPtrs bp;
   for (int d = 0; d < 6; ++d) bp.p[d] = box[d];
-  hipLaunchKernelGGL(k_refit_local, dim3(ceil_div(n, REFIT_B)), dim3(REFIT_B), 0, ctx->stream, bp,
-                     bvh.order.as<uint32_t>(), (int)n, bvh.nodes.as<LNode>(), bvh.parent.as<uint32_t>(), expand);
+  if (aos)
+    hipLaunchKernelGGL(k_refit_local<true>, dim3(ceil_div(n, REFIT_B)), dim3(REFIT_B), 0, ctx->stream, bp, aos,
+                       bvh.order.as<uint32_t>(), (int)n, bvh.nodes.as<LNode>(), bvh.parent.as<uint32_t>(), expand);
+  else
+    hipLaunchKernelGGL(k_refit_local<false>, dim3(ceil_div(n, REFIT_B)), dim3(REFIT_B), 0, ctx->stream, bp, aos,
+                       bvh.order.as<uint32_t>(), (int)n, bvh.nodes.as<LNode>(), bvh.parent.as<uint32_t>(), expand);
   NCX_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_refit_upper, dim3(1), dim3(UPPER_THREADS), 0, ctx->stream, bvh.nodes.as<LNode>(),
                      bvh.upper.as<uint4>(), bvh.upper_off.as<uint32_t>());
@@ -679,7 +696,7 @@ static int refit_launch(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], floa
   return NCX_OK;
 }
 
-int lbvh_build(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], size_t n, float expand) {
+int lbvh_build(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], size_t n, float expand, const float4* aos) {
   int rc;
   NCX_REQUIRE(n <= (size_t)NCX_PARENT_MASK, "lbvh_build: too many primitives");
   bvh.n = n;
@@ -708,11 +725,11 @@ int lbvh_build(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], size_t n, flo
                      (int)n, bvh.nodes.as<LNode>(), bvh.parent.as<uint32_t>(), bvh.first.as<uint32_t>());
   NCX_HIP_CHECK(hipGetLastError());
   if ((rc = refit_prepare(ctx, bvh))) return rc;
-  return refit_launch(ctx, bvh, box, expand);
+  return refit_launch(ctx, bvh, box, expand, aos);
 }
 
 // Refit only (same leaf order and topology), used when boxes move but the proxy set is unchanged.
-int lbvh_refit(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], float expand) {
+int lbvh_refit(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], float expand, const float4* aos) {
   if (bvh.n <= 1) return NCX_OK;
-  return refit_launch(ctx, bvh, box, expand);
+  return refit_launch(ctx, bvh, box, expand, aos);
 }

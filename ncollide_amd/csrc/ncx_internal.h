@@ -230,9 +230,12 @@ constexpr uint32_t LBVH_MAX_DEPTH = 64;
 
 // Builds the hierarchy over n boxes given as device SoA.  `expand` > 0 grows every
 // stored box outward by expand * (1 + |coordinate|) (used for conservative ray culling).
-int lbvh_build(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], size_t n, float expand);
+// `aos`, when given, holds the same boxes as two float4 per box (min xyz, max x | max yz, pad),
+// which the refit gathers instead of the six SoA arrays.
+int lbvh_build(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], size_t n, float expand,
+               const float4* aos = nullptr);
 // Recomputes every node box for new primitive boxes, keeping order and topology.
-int lbvh_refit(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], float expand);
+int lbvh_refit(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], float expand, const float4* aos = nullptr);
 
 // Fills n uint32 with v on the context stream (a kernel, so it is safe inside graph capture).
 int fill_u32(ncx_ctx* ctx, uint32_t* p, size_t n, uint32_t v);

@@ -34,6 +34,7 @@ struct ncx_world {
   int since_build = 0;
   bool first = true;
   DevBuf box_new, stored;
+  DevBuf sbox;  // the stored boxes again, two float4 per proxy (min xyz, max x | max yz), for gathers
   LBVH bvh;
   uint32_t spill = 1;
   DevBuf gstack;  // search-stack spill area, `spill` entries per query thread
@@ -59,28 +60,31 @@ struct ncx_world {
   bool last_had_contacts = false;
 };
 
-__global__ void k_world_stage(const float* __restrict__ nb, float* __restrict__ st, size_t n, float margin,
-                              int first) {
+// The stored boxes are kept twice: SoA (the interface's layout, read coalesced here) and as two
+// float4 per proxy, which the refit and the pair query gather by sorted leaf: one cache line
+// per box instead of six.
+__global__ void k_world_stage(const float* __restrict__ nb, float* __restrict__ st, float4* __restrict__ sb, size_t n,
+                              float margin, int first) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float b[6];
 #pragma unroll
   for (int d = 0; d < 6; ++d) b[d] = nb[d * n + i];
-  if (first) {
-#pragma unroll
-    for (int d = 0; d < 6; ++d) st[d * n + i] = b[d];
-    return;
-  }
-  // AABB::contains (aabb.rs:161-163)
-  bool contained = st[0 * n + i] <= b[0] && st[1 * n + i] <= b[1] && st[2 * n + i] <= b[2] &&
-                   st[3 * n + i] >= b[3] && st[4 * n + i] >= b[4] && st[5 * n + i] >= b[5];
-  if (!contained) {
+  if (!first) {
+    // AABB::contains (aabb.rs:161-163)
+    const bool contained = st[0 * n + i] <= b[0] && st[1 * n + i] <= b[1] && st[2 * n + i] <= b[2] &&
+                           st[3 * n + i] >= b[3] && st[4 * n + i] >= b[4] && st[5 * n + i] >= b[5];
+    if (contained) return;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      st[d * n + i] = b[d] + (-margin);
-      st[(3 + d) * n + i] = b[3 + d] + margin;
+      b[d] = b[d] + (-margin);
+      b[3 + d] = b[3 + d] + margin;
     }
   }
+#pragma unroll
+  for (int d = 0; d < 6; ++d) st[d * n + i] = b[d];
+  sb[2 * i] = make_float4(b[0], b[1], b[2], b[3]);
+  sb[2 * i + 1] = make_float4(b[4], b[5], 0.f, 0.f);
 }
 
 __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float mny, float mnz, float mxx, float mxy,
@@ -110,7 +114,7 @@ __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float 
 template <int MODE>
 __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restrict__ nodes,
                                                            const uint32_t* __restrict__ order,
-                                                           const float* __restrict__ boxes, uint32_t n,
+                                                           const float4* __restrict__ sbox, uint32_t n,
                                                            uint32_t* __restrict__ cnt, const uint32_t* __restrict__ offs,
                                                            size_t cap, uint32_t* __restrict__ pa,
                                                            uint32_t* __restrict__ pb, uint32_t* __restrict__ counters,
@@ -181,8 +185,8 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
           k = next + rank;
           if (MODE == MODE_COUNT || cnt[k] > WQ_LOCAL) {
             prim = order[k];
-#pragma unroll
-            for (int d = 0; d < 6; ++d) q[d] = boxes[(size_t)d * n + prim];
+            const float4 b0 = sbox[2 * (size_t)prim], b1 = sbox[2 * (size_t)prim + 1];
+            q[0] = b0.x; q[1] = b0.y; q[2] = b0.z; q[3] = b0.w; q[4] = b1.x; q[5] = b1.y;
             c = 0;
             w = MODE == MODE_WRITE ? offs[k] : 0;
             sp = 0;
@@ -311,7 +315,7 @@ static int world_build(ncx_world* w) {
   ncx_ctx* c = &w->wctx;
   const float* box[6];
   for (int d = 0; d < 6; ++d) box[d] = w->stored.as<float>() + d * w->n;
-  int rc = lbvh_build(c, w->bvh, box, w->n, 0.f);
+  int rc = lbvh_build(c, w->bvh, box, w->n, 0.f, w->sbox.as<float4>());
   if (rc) return rc;
   if (w->n > 1) {
     // The search stack holds fewer entries than the tree has levels (at most LBVH_MAX_DEPTH): one
@@ -331,12 +335,12 @@ static int world_enqueue(ncx_world* w, const ncx_pose* poses, bool refit) {
   for (int d = 0; d < 6; ++d) nb[d] = w->box_new.as<float>() + d * n;
   if ((rc = shapes_aabbs_on(w->shapes, c, poses, w->prediction, nb))) return rc;
   hipLaunchKernelGGL(k_world_stage, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, w->box_new.as<float>(),
-                     w->stored.as<float>(), n, w->margin, w->first ? 1 : 0);
+                     w->stored.as<float>(), w->sbox.as<float4>(), n, w->margin, w->first ? 1 : 0);
   NCX_HIP_CHECK(hipGetLastError());
   if (refit) {
     const float* box[6];
     for (int d = 0; d < 6; ++d) box[d] = w->stored.as<float>() + d * n;
-    if ((rc = lbvh_refit(c, w->bvh, box, 0.f))) return rc;
+    if ((rc = lbvh_refit(c, w->bvh, box, 0.f, w->sbox.as<float4>()))) return rc;
   }
   return NCX_OK;
 }
@@ -351,7 +355,7 @@ static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses, bool ea
     unsigned nb = world_query_blocks(c, n);
     const LNode* nodes = w->bvh.nodes.as<LNode>();
     const uint32_t* order = w->bvh.order.as<uint32_t>();
-    const float* st = w->stored.as<float>();
+    const float4* st = w->sbox.as<float4>();
     uint32_t* qw = w->qwork.as<uint32_t>();
     if ((rc = fill_u32(c, qw, 2 * WQ_PARTS * WQ_PART_STRIDE, 0u))) return rc;
     hipLaunchKernelGGL((k_world_query<MODE_COUNT>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
@@ -409,6 +413,7 @@ extern "C" int ncx_world_create(ncx_ctx* ctx, ncx_shapes* shapes, float margin, 
     return NCX_E_HIP;
   }
   if ((rc = w->box_new.reserve(n * 24 + 64)) || (rc = w->stored.reserve(n * 24 + 64)) ||
+      (rc = w->sbox.reserve(n * 32 + 64)) ||
       (rc = w->cnt.reserve(n * 4 + 64)) || (rc = w->offs.reserve(n * 4 + 64)) || (rc = w->counters.reserve(64)) ||
       (rc = w->lbuf.reserve(n * (size_t)WQ_LOCAL * 4 + 64)) ||
       (rc = w->qwork.reserve(2 * WQ_PARTS * WQ_PART_STRIDE * 4)) ||
@@ -469,7 +474,7 @@ extern "C" int ncx_world_query_stats(ncx_world* w, unsigned long long* nodes_vis
   unsigned nb = world_query_blocks(c, w->n);
   if ((rc = fill_u32(c, w->qwork.as<uint32_t>(), WQ_PARTS * WQ_PART_STRIDE, 0u))) return rc;
   hipLaunchKernelGGL((k_world_query<MODE_COUNT>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, w->bvh.nodes.as<LNode>(),
-                     w->bvh.order.as<uint32_t>(), w->stored.as<float>(), (uint32_t)w->n, w->cnt.as<uint32_t>(),
+                     w->bvh.order.as<uint32_t>(), w->sbox.as<float4>(), (uint32_t)w->n, w->cnt.as<uint32_t>(),
                      (const uint32_t*)nullptr, w->cap, (uint32_t*)nullptr, (uint32_t*)nullptr,
                      w->counters.as<uint32_t>(), w->gstack.as<uint32_t>(), w->spill, w->lbuf.as<uint32_t>(),
                      v.as<unsigned long long>(), w->qwork.as<uint32_t>());
@@ -492,7 +497,7 @@ extern "C" int ncx_world_destroy(ncx_world* w) {
   }
   if (w->ev_in) (void)hipEventDestroy(w->ev_in);
   if (w->ev_out) (void)hipEventDestroy(w->ev_out);
-  DevBuf* bufs[] = {&w->box_new, &w->stored, &w->gstack, &w->qwork, &w->lbuf, &w->cnt, &w->offs, &w->counters, &w->pa, &w->pb,
+  DevBuf* bufs[] = {&w->box_new, &w->stored, &w->sbox, &w->gstack, &w->qwork, &w->lbuf, &w->cnt, &w->offs, &w->counters, &w->pa, &w->pb,
                     &w->has, &w->w1, &w->w2, &w->nrm, &w->depth_out, &w->iters, &w->desc, &w->dirs};
   for (DevBuf* b : bufs) b->release();
   w->bvh.release();
