@@ -2220,58 +2220,120 @@ extern "C" int ncx_distance_batch_capped(ncx_shapes* s, const ncx_pose* poses, c
 }
 
 // ------------------------------------------------------------------ shape AABBs
+// One thread per shape, except hulls: a wavefront takes its hull lanes four at a time, 16 lanes
+// per hull, each lane transforming four of the hull's points per pass with independent loads
+// (coalesced across the lanes: the point pool is SoA), and reduces min/max over its 16 lanes.
+// A thread looping over its own hull paid one memory round trip per point; one hull per
+// wavefront at a time paid one per hull.  The min/max of a set does not depend on the visiting
+// order (a point read twice, to keep the loads unconditional, changes nothing).
+__device__ __forceinline__ float grp16_min(float x) {
+#pragma unroll
+  for (int s = 8; s >= 1; s >>= 1) x = fminf(x, __shfl_xor(x, s, 64));
+  return x;
+}
+__device__ __forceinline__ float grp16_max(float x) {
+#pragma unroll
+  for (int s = 8; s >= 1; s >>= 1) x = fmaxf(x, __shfl_xor(x, s, 64));
+  return x;
+}
+
 __global__ __launch_bounds__(256) void k_shape_aabbs(const ncx_shape* __restrict__ shapes, Pool P,
                                                       const ncx_pose* __restrict__ poses, size_t n, float margin,
                                                       float* b0, float* b1, float* b2, float* b3, float* b4,
                                                       float* b5) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  ShapeD g = load_shape(shapes, (uint32_t)i);
-  iso m = load_pose(poses, (uint32_t)i);
-  f3 mn, mx;
-  switch (g.type) {
-    case NCX_BALL:  // ball_aabb (aabb_ball.rs:8-13)
-      mn = m.t + f3{-g.a, -g.a, -g.a};
-      mx = m.t + f3{g.a, g.a, g.a};
-      break;
-    case NCX_CUBOID: {  // center +- |R| h (aabb_cuboid.rs:7-14, isometry_ops.rs:43-45)
-      float qi = m.q.i, qj = m.q.j, qk = m.q.k, qw = m.q.w;
-      float ww = qw * qw, ii = qi * qi, jj = qj * qj, kk = qk * qk;
-      float ij = qi * qj * 2.f, wk = qw * qk * 2.f, wj = qw * qj * 2.f;
-      float ik = qi * qk * 2.f, jk = qj * qk * 2.f, wi = qw * qi * 2.f;
-      float M[3][3] = {{ww + ii - jj - kk, ij - wk, wj + ik},
-                       {wk + ij, ww - ii + jj - kk, jk - wi},
-                       {ik - wj, wi + jk, ww - ii - jj + kk}};
-      f3 ws;
-      ws.x = (fabsf(M[0][0]) * g.a + fabsf(M[0][1]) * g.b) + fabsf(M[0][2]) * g.c;
-      ws.y = (fabsf(M[1][0]) * g.a + fabsf(M[1][1]) * g.b) + fabsf(M[1][2]) * g.c;
-      ws.z = (fabsf(M[2][0]) * g.a + fabsf(M[2][1]) * g.b) + fabsf(M[2][2]) * g.c;
-      mn = m.t - ws;
-      mx = m.t + ws;
-      break;
-    }
-    case NCX_CAPSULE: {  // support_map_aabb (aabb_utils.rs:9-31)
-      for (int d = 0; d < 3; ++d) {
-        f3 e = f3{0.f, 0.f, 0.f};
-        set_comp(e, d, 1.f);
-        set_comp(mx, d, comp(support<1>(g, m, e, P, 0), d));
-        set_comp(e, d, -1.f);
-        set_comp(mn, d, comp(support<1>(g, m, e, P, 0), d));
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < n;  // no early exit: every lane takes part in the hull loop
+  const int lane = __lane_id();
+  ShapeD g = {};
+  iso m = {};
+  f3 mn = f3{0.f, 0.f, 0.f}, mx = f3{0.f, 0.f, 0.f};
+  bool hull = false;
+  if (live) {
+    g = load_shape(shapes, (uint32_t)i);
+    m = load_pose(poses, (uint32_t)i);
+    switch (g.type) {
+      case NCX_BALL:  // ball_aabb (aabb_ball.rs:8-13)
+        mn = m.t + f3{-g.a, -g.a, -g.a};
+        mx = m.t + f3{g.a, g.a, g.a};
+        break;
+      case NCX_CUBOID: {  // center +- |R| h (aabb_cuboid.rs:7-14, isometry_ops.rs:43-45)
+        float qi = m.q.i, qj = m.q.j, qk = m.q.k, qw = m.q.w;
+        float ww = qw * qw, ii = qi * qi, jj = qj * qj, kk = qk * qk;
+        float ij = qi * qj * 2.f, wk = qw * qk * 2.f, wj = qw * qj * 2.f;
+        float ik = qi * qk * 2.f, jk = qj * qk * 2.f, wi = qw * qi * 2.f;
+        float M[3][3] = {{ww + ii - jj - kk, ij - wk, wj + ik},
+                         {wk + ij, ww - ii + jj - kk, jk - wi},
+                         {ik - wj, wi + jk, ww - ii - jj + kk}};
+        f3 ws;
+        ws.x = (fabsf(M[0][0]) * g.a + fabsf(M[0][1]) * g.b) + fabsf(M[0][2]) * g.c;
+        ws.y = (fabsf(M[1][0]) * g.a + fabsf(M[1][1]) * g.b) + fabsf(M[1][2]) * g.c;
+        ws.z = (fabsf(M[2][0]) * g.a + fabsf(M[2][1]) * g.b) + fabsf(M[2][2]) * g.c;
+        mn = m.t - ws;
+        mx = m.t + ws;
+        break;
       }
-      break;
-    }
-    default: {  // point_cloud_aabb (aabb_utils.rs:59-79)
-      f3 w = xform_pt(m, f3{P.x[g.off], P.y[g.off], P.z[g.off]});
-      mn = w;
-      mx = w;
-      for (uint32_t v = 1; v < g.cnt; ++v) {
-        f3 p = xform_pt(m, f3{P.x[g.off + v], P.y[g.off + v], P.z[g.off + v]});
-        mn = f3{fminf(mn.x, p.x), fminf(mn.y, p.y), fminf(mn.z, p.z)};
-        mx = f3{fmaxf(mx.x, p.x), fmaxf(mx.y, p.y), fmaxf(mx.z, p.z)};
+      case NCX_CAPSULE: {  // support_map_aabb (aabb_utils.rs:9-31)
+        for (int d = 0; d < 3; ++d) {
+          f3 e = f3{0.f, 0.f, 0.f};
+          set_comp(e, d, 1.f);
+          set_comp(mx, d, comp(support<1>(g, m, e, P, 0), d));
+          set_comp(e, d, -1.f);
+          set_comp(mn, d, comp(support<1>(g, m, e, P, 0), d));
+        }
+        break;
       }
-      break;
+      default:
+        hull = true;
+        break;
     }
   }
+  // point_cloud_aabb (aabb_utils.rs:59-79): the componentwise min and max of the transformed points
+  uint64_t hm = __ballot(hull);
+  const int grp = lane >> 4, gl = lane & 15;
+  while (hm) {
+    // group grp takes the grp-th lowest remaining hull lane (none when fewer remain)
+    uint64_t t = hm;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if (q < grp) t &= t - 1;
+    const int h = t ? __ffsll((unsigned long long)t) - 1 : lane;
+    // shuffles outside any condition: a lane inactive at a shuffle does not supply its value
+    const uint32_t off = (uint32_t)__shfl((int)g.off, h, 64);
+    const uint32_t cnt_h = (uint32_t)__shfl((int)g.cnt, h, 64);
+    const uint32_t cnt = t ? cnt_h : 0u;
+    iso mh;
+    mh.t = f3{__shfl(m.t.x, h, 64), __shfl(m.t.y, h, 64), __shfl(m.t.z, h, 64)};
+    mh.q.i = __shfl(m.q.i, h, 64);
+    mh.q.j = __shfl(m.q.j, h, 64);
+    mh.q.k = __shfl(m.q.k, h, 64);
+    mh.q.w = __shfl(m.q.w, h, 64);
+    float lx = __builtin_inff(), ly = __builtin_inff(), lz = __builtin_inff();
+    float ux = -__builtin_inff(), uy = -__builtin_inff(), uz = -__builtin_inff();
+    for (uint32_t v0 = (uint32_t)gl; v0 < cnt; v0 += 64) {
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t v = v0 + 16u * u < cnt ? v0 + 16u * u : v0;
+        const f3 p = xform_pt(mh, f3{P.x[off + v], P.y[off + v], P.z[off + v]});
+        lx = fminf(lx, p.x); ly = fminf(ly, p.y); lz = fminf(lz, p.z);
+        ux = fmaxf(ux, p.x); uy = fmaxf(uy, p.y); uz = fmaxf(uz, p.z);
+      }
+    }
+    lx = grp16_min(lx); ly = grp16_min(ly); lz = grp16_min(lz);
+    ux = grp16_max(ux); uy = grp16_max(uy); uz = grp16_max(uz);
+    // a hull lane among the four taken fetches its box from the first lane of its group
+    const bool mine = ((hm >> lane) & 1ull) != 0;
+    const int rank = mine ? __popcll(hm & ((1ull << lane) - 1ull)) : 4;
+    const int src = rank < 4 ? rank * 16 : lane;
+    const float rlx = __shfl(lx, src, 64), rly = __shfl(ly, src, 64), rlz = __shfl(lz, src, 64);
+    const float rux = __shfl(ux, src, 64), ruy = __shfl(uy, src, 64), ruz = __shfl(uz, src, 64);
+    if (rank < 4) {
+      mn = f3{rlx, rly, rlz};
+      mx = f3{rux, ruy, ruz};
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) hm &= hm - 1;
+  }
+  if (!live) return;
   if (margin != 0.f) {  // AABB::loosened (aabb.rs:190-199)
     mn = mn + f3{-margin, -margin, -margin};
     mx = mx + f3{margin, margin, margin};

@@ -155,6 +155,18 @@ def test_shape_aabbs_bit_exact(ctx, orc):
     assert np.array_equal(bb, ref)
 
 
+def test_shape_aabbs_of_large_hulls_bit_exact(ctx, orc):
+    # hulls of up to 256 points: a wavefront's 64 lanes take several passes over each
+    w = workloads.hull_pairs(seed=9, npairs=1024)
+    assert w["records"]["hull_count"].max() > 128
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    bb = ss.aabbs(poses, margin=0.01).cpu().numpy().T
+    ref = orc.shape_aabbs(oracle_shapes(orc, w["records"], w["pool"]), to_oracle_poses(w["poses"]))
+    ref = ref + np.array([-0.01, -0.01, -0.01, 0.01, 0.01, 0.01], np.float32)
+    assert np.array_equal(bb, ref)
+
+
 def test_broad_phase_plain_pipeline(ctx, orc):
     w = workloads.plain_pipeline(seed=1)
     ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
