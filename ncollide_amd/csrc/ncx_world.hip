@@ -260,6 +260,7 @@ static unsigned world_query_blocks(const ncx_ctx* c, size_t n) {
 // counts, then a binary search for each pair's leaf), so consecutive threads write consecutive
 // pair slots.
 #define WQ_EMIT_BLOCK 256
+#define EMIT_UNROLL 4
 __global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __restrict__ order, uint32_t n,
                                                               const uint32_t* __restrict__ cnt,
                                                               const uint32_t* __restrict__ offs,
@@ -288,24 +289,39 @@ __global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __
   if (t == 0) s_pre[0] = 0;
   __syncthreads();
   const uint32_t total = s_pre[WQ_EMIT_BLOCK];
-  for (uint32_t j = t; j < total; j += WQ_EMIT_BLOCK) {
-    // the leaf l with s_pre[l] <= j < s_pre[l + 1]
-    uint32_t lo = 0, hi = WQ_EMIT_BLOCK;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) / 2;
-      if (s_pre[mid] <= j) lo = mid;
-      else hi = mid;
+  // EMIT_UNROLL pairs per thread per pass, their loads issued together: a block over a dense
+  // cluster walks tens of thousands of pairs, and one load round trip per pair was its bound.
+  for (uint32_t j0 = t; j0 < total; j0 += EMIT_UNROLL * WQ_EMIT_BLOCK) {
+    uint32_t kl[EMIT_UNROLL], ii[EMIT_UNROLL], w0[EMIT_UNROLL], prim[EMIT_UNROLL], h2[EMIT_UNROLL];
+#pragma unroll
+    for (int u = 0; u < EMIT_UNROLL; ++u) {
+      // the leaf l with s_pre[l] <= j < s_pre[l + 1] (a pair past the end repeats the first one)
+      const uint32_t j = j0 + u * WQ_EMIT_BLOCK < total ? j0 + u * WQ_EMIT_BLOCK : j0;
+      uint32_t lo = 0, hi = WQ_EMIT_BLOCK;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (s_pre[mid] <= j) lo = mid;
+        else hi = mid;
+      }
+      kl[u] = blockIdx.x * WQ_EMIT_BLOCK + lo;
+      ii[u] = j - s_pre[lo];
     }
-    const uint32_t kl = blockIdx.x * WQ_EMIT_BLOCK + lo;
-    const uint32_t i = j - s_pre[lo];
-    const size_t w = (size_t)offs[kl] + i;
-    if (w < cap) {
-      const uint32_t prim = order[kl];
-      const uint32_t h2 = lbuf[(size_t)kl * WQ_LOCAL + i];
-      pa[w] = prim < h2 ? prim : h2;
-      pb[w] = prim < h2 ? h2 : prim;
-    } else {
-      counters[1] = 1u;
+#pragma unroll
+    for (int u = 0; u < EMIT_UNROLL; ++u) {
+      w0[u] = offs[kl[u]];
+      prim[u] = order[kl[u]];
+      h2[u] = ii[u] < WQ_LOCAL ? lbuf[(size_t)kl[u] * WQ_LOCAL + ii[u]] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < EMIT_UNROLL; ++u) {
+      if (j0 + u * WQ_EMIT_BLOCK >= total) break;
+      const size_t w = (size_t)w0[u] + ii[u];
+      if (w < cap) {
+        pa[w] = prim[u] < h2[u] ? prim[u] : h2[u];
+        pb[w] = prim[u] < h2[u] ? h2[u] : prim[u];
+      } else {
+        counters[1] = 1u;
+      }
     }
   }
 }
