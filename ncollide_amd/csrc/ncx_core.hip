@@ -488,7 +488,7 @@ __device__ __forceinline__ void merge_slots(const float* f, float b[6]) {
 // the stage-2 entry of its left child, of its right child; UPPER_NONE for a child refitted by
 // stage 1).  Up to UPPER_LDS entries: every node's child slots are fetched at the start (one
 // round trip for all levels), a stage-2 child's box comes from LDS, and a level costs a barrier;
-// every box is also stored into its parent's slot for the traversals.  Past UPPER_LDS entries:
+// at the end every box is stored into its parent's slot for the traversals.  Past UPPER_LDS entries:
 // the same levels through device memory, one round trip per level.
 __global__ __launch_bounds__(UPPER_THREADS) void k_refit_upper(LNode* __restrict__ nodes, const uint4* __restrict__ upper,
                                                                 const uint32_t* __restrict__ level_off) {
@@ -531,9 +531,19 @@ __global__ __launch_bounds__(UPPER_THREADS) void k_refit_upper(LNode* __restrict
         float b[6];
         merge_slots(slot[j], b);
         for (int q = 0; q < 6; ++q) mb[i][q] = b[q];
-        if (en.x != 0) put_slot(&nodes[en.y & NCX_PARENT_MASK], (en.y & NCX_RIGHT_BIT) != 0, b);
       }
       __syncthreads();
+    }
+    // the boxes go to their parents' slots once all levels are done: a store pending at a
+    // level's barrier would make the barrier wait for it
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const uint32_t i = tid + j * UPPER_THREADS;
+      if (i < total && ent[i].x != 0) {
+        float b[6];
+        for (int q = 0; q < 6; ++q) b[q] = mb[i][q];
+        put_slot(&nodes[ent[i].y & NCX_PARENT_MASK], (ent[i].y & NCX_RIGHT_BIT) != 0, b);
+      }
     }
     return;
   }
@@ -591,11 +601,9 @@ __global__ __launch_bounds__(256) void k_upper_compact(const uint32_t* __restric
 // depth, deepest first, into k_refit_upper's entries, with level offsets off[lv] for lv =
 // 0..LBVH_MAX_DEPTH (level lv holds depth LBVH_MAX_DEPTH - 1 - lv).  Order within a level is
 // free: a level's nodes write distinct parent slots.  Each thread loads its entries of a chunk
-// at once, then counts or places them; pos[] (node -> entry) then links each entry to its
-// children's.
+// at once, then counts or places them, and records pos[] (node -> entry) for k_upper_link.
 #define SORT_PER_THREAD 8
 __global__ __launch_bounds__(1024) void k_upper_sort(const uint4* __restrict__ list, const uint32_t* __restrict__ total_p,
-                                                      const LNode* __restrict__ nodes, const uint32_t* __restrict__ flag,
                                                       uint32_t* __restrict__ pos, uint4* __restrict__ upper,
                                                       uint32_t* __restrict__ off) {
   __shared__ uint32_t cnt[LBVH_MAX_DEPTH], cur[LBVH_MAX_DEPTH];
@@ -642,8 +650,16 @@ __global__ __launch_bounds__(1024) void k_upper_sort(const uint4* __restrict__ l
         pos[e[j].x] = slot;
       }
   }
-  __syncthreads();  // pos[] and the entries, written above by this workgroup, are read below
-  for (uint32_t s = tid; s < total; s += blockDim.x) {
+}
+
+// Links each stage-2 entry to its children's entries (pos[] maps node -> entry); a grid of its
+// own, so the dependent loads of all entries are in flight at once.
+#define UPPER_LINK_BLOCKS 64
+__global__ __launch_bounds__(256) void k_upper_link(const uint32_t* __restrict__ total_p, const LNode* __restrict__ nodes,
+                                                     const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                                     uint4* __restrict__ upper) {
+  const uint32_t total = *total_p;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < total; s += gridDim.x * blockDim.x) {
     const uint32_t p = upper[s].x;
     const uint32_t cl = nodes[p].child_l, cr = nodes[p].child_r;
     const uint32_t el = (!(cl & NCX_LEAF_BIT) && flag[cl]) ? pos[cl] : UPPER_NONE;
@@ -673,8 +689,10 @@ static int refit_prepare(ncx_ctx* ctx, LBVH& bvh) {
                      bvh.tmp_vals.as<uint4>());
   NCX_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_upper_sort, dim3(1), dim3(1024), 0, ctx->stream, bvh.tmp_vals.as<uint4>(), total,
-                     bvh.nodes.as<LNode>(), bvh.flags.as<uint32_t>(), bvh.tmp_keys.as<uint32_t>(),
-                     bvh.upper.as<uint4>(), bvh.upper_off.as<uint32_t>());
+                     bvh.tmp_keys.as<uint32_t>(), bvh.upper.as<uint4>(), bvh.upper_off.as<uint32_t>());
+  NCX_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_upper_link, dim3(UPPER_LINK_BLOCKS), dim3(256), 0, ctx->stream, total, bvh.nodes.as<LNode>(),
+                     bvh.flags.as<uint32_t>(), bvh.tmp_keys.as<uint32_t>(), bvh.upper.as<uint4>());
   NCX_HIP_CHECK(hipGetLastError());
   return NCX_OK;
 }
