@@ -233,7 +233,9 @@ int ncx_mlp_predict_dirs(ncx_mlp* mlp, const ncx_pose* poses_dev, const float* d
  * given, :275-280), LBVH refit (rebuilt every k steps), every intersecting stored-box pair, and
  * query::contact at `prediction` on each pair.  No host synchronisation inside a step: with
  * use_graph != 0, refit steps and rebuild steps (after the first step) each replay a captured
- * hipGraph.  Outputs are capacity-`max_pairs` device arrays. */
+ * hipGraph.  Outputs are capacity-`max_pairs` device arrays.  Device memory: about 2.3 KB per
+ * shape (2 KB of it buffers each proxy's first 512 pair partners for the pair search) plus the
+ * per-pair outputs and contact work lists. */
 typedef struct ncx_world ncx_world;
 int ncx_world_create(ncx_ctx* ctx, ncx_shapes* shapes, float margin, float prediction, size_t max_pairs,
                      ncx_world** world);
@@ -251,9 +253,9 @@ int ncx_world_set_contacts(ncx_world* world, int enable);
  * the current tree, the per-pass work of the step's pair search.  Synchronous. */
 int ncx_world_query_stats(ncx_world* world, unsigned long long* nodes_visited);
 /* Waits for the last step: *npairs = pair count; NCX_E_CAPACITY if it exceeded max_pairs.
- * Every contact is complete on return: pairs whose EPA polytope outgrew the step's two arena
- * tiers are recomputed here with larger arenas, so the step's poses must stay unchanged until
- * this returns.  *n_epa_overflow = pairs of the step that needed more than the first tier. */
+ * Every contact is complete on return: pairs whose EPA polytope outgrew the step's LDS arena
+ * tiers are continued here with device-memory arenas, so the step's poses must stay unchanged
+ * until this returns.  *n_epa_overflow = pairs of the step that needed more than the first tier. */
 int ncx_world_sync(ncx_world* world, size_t* npairs, size_t* n_epa_overflow);
 int ncx_world_outputs(ncx_world* world, uint32_t** ia, uint32_t** ib, uint8_t** has_contact, float** world1,
                       float** world2, float** normal, float** depth, int32_t** gjk_iters);
