@@ -39,7 +39,7 @@ struct ncx_world {
   uint32_t spill = 1;
   DevBuf gstack;  // search-stack spill area, `spill` entries per query thread
   DevBuf qwork;   // the query passes' work counters (WQ_PARTS per pass)
-  DevBuf lbuf;    // the count pass's first WQ_LOCAL partners per leaf
+  DevBuf lbuf;    // the count pass's first WQ_LOCAL partners per leaf (sorted leaf positions)
   DevBuf cnt, offs, counters;  // counters: [0] pair total, [1] pair overflow
   DevBuf pa, pb;
   DevBuf has, w1, w2, nrm, depth_out, iters;
@@ -95,7 +95,8 @@ __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float 
 // Self-query of sorted leaf k against leaves at later positions (each pair is found once, by its
 // earlier leaf).  Depth-first with the stack's top in a register, the next WQ_LSTACK entries in
 // LDS and the rest in a per-thread spill area of `spill` entries (the tree depth bounds it).
-//   MODE_COUNT: count the pairs and keep the first WQ_LOCAL partners (shape ids) in lbuf;
+//   MODE_COUNT: count the pairs and keep the first WQ_LOCAL partners (leaf positions: no
+//               dependent gather inside the traversal) in lbuf;
 //   MODE_WRITE: write every pair at offs[k] (only for leaves with more than WQ_LOCAL pairs).
 // Persistent: a lane that finishes a leaf takes the next one at once from its wavefront's chunk of
 // WQ_CHUNK leaves (ballot + rank), and a wavefront whose chunk is used up takes the next chunk of
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
       }
       w++;
     } else {
-      if (c < WQ_LOCAL) lbuf[(size_t)k * WQ_LOCAL + c] = order[leafpos];
+      if (c < WQ_LOCAL) lbuf[(size_t)k * WQ_LOCAL + c] = leafpos;  // the emit kernel maps it to a shape
       c++;
     }
   };
@@ -310,7 +311,7 @@ __global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __
     for (int u = 0; u < EMIT_UNROLL; ++u) {
       w0[u] = offs[kl[u]];
       prim[u] = order[kl[u]];
-      h2[u] = ii[u] < WQ_LOCAL ? lbuf[(size_t)kl[u] * WQ_LOCAL + ii[u]] : 0u;
+      h2[u] = order[lbuf[(size_t)kl[u] * WQ_LOCAL + ii[u]]];
     }
 #pragma unroll
     for (int u = 0; u < EMIT_UNROLL; ++u) {
