@@ -102,7 +102,9 @@ __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float 
 // WQ_CHUNK leaves (ballot + rank), and a wavefront whose chunk is used up takes the next chunk of
 // its part of the leaves with one atomicAdd on that part's counter (WQ_PARTS counters, each on
 // its own cache line, so the atomics do not serialise on one line); a wave that finds its part
-// used up moves on to the next parts.
+// used up moves on to the next parts.  A block starts on part blockIdx % WQ_PARTS: blocks are
+// dealt to the 8 XCDs in turn, so each XCD's L2 first serves one contiguous (spatially close)
+// eighth of the leaves and the tree nodes above them.
 #define WQ_LSTACK 16
 #define WQ_LOCAL 512
 #define WQ_CHUNK 64u
@@ -128,8 +130,7 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
   const int lane = __lane_id();
   const uint32_t gtid = blockIdx.x * WQ_BLOCK + tid;
   uint32_t* gs = gstack + (size_t)gtid * spill;
-  const uint32_t wave = gtid / 64;
-  const uint32_t part0 = wave % WQ_PARTS;
+  const uint32_t part0 = blockIdx.x % WQ_PARTS;
   uint32_t part_k = 0;
   uint32_t next = 0, end = 0;  // the wavefront's chunk [next, end) (the same in every lane)
   bool pool_empty = false, exhausted = false, has = false;
