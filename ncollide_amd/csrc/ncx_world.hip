@@ -97,7 +97,8 @@ __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float 
 // LDS and the rest in a per-thread spill area of `spill` entries (the tree depth bounds it).
 //   MODE_COUNT: count the pairs and keep the first WQ_LOCAL partners (leaf positions: no
 //               dependent gather inside the traversal) in lbuf;
-//   MODE_WRITE: write every pair at offs[k] (only for leaves with more than WQ_LOCAL pairs).
+//   MODE_WRITE: for leaves with more than WQ_LOCAL pairs, the partners past the first WQ_LOCAL
+//               (leaf positions) at their slots offs[k] + index.
 // Persistent: a lane that finishes a leaf takes the next one at once from its wavefront's chunk of
 // WQ_CHUNK leaves (ballot + rank), and a wavefront whose chunk is used up takes the next chunk of
 // its part of the leaves with one atomicAdd on that part's counter (WQ_PARTS counters, each on
@@ -143,14 +144,14 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
   unsigned long long nvis = 0;
   auto emit = [&](uint32_t leafpos) {
     if (MODE == MODE_WRITE) {
-      if (w < cap) {
-        uint32_t h2 = order[leafpos];
-        pa[w] = prim < h2 ? prim : h2;
-        pb[w] = prim < h2 ? h2 : prim;
-      } else {
-        counters[1] = 1u;
+      // a pair past the buffered ones: its leaf position goes to pb[w], which the copy kernel,
+      // run after this pass, maps to a shape and orders (no dependent gather in here)
+      if (c >= WQ_LOCAL) {
+        if (w < cap) pb[w] = leafpos;
+        else counters[1] = 1u;
       }
       w++;
+      c++;
     } else {
       if (c < WQ_LOCAL) lbuf[(size_t)k * WQ_LOCAL + c] = leafpos;  // the emit kernel maps it to a shape
       c++;
@@ -255,9 +256,10 @@ static unsigned world_query_blocks(const ncx_ctx* c, size_t n) {
   return std::max(1u, std::min<unsigned>(resident, ceil_div(n, WQ_BLOCK)));
 }
 
-// Pairs of the leaves with at most WQ_LOCAL pairs, copied from the count pass's buffer in the
-// order they were found (the write pass handles the others), so the list is the same as one
-// full write traversal per leaf would give.  A block takes WQ_EMIT_BLOCK consecutive leaves; its
+// Every pair, in the order the traversal found it, so the list is the same as one full write
+// traversal per leaf would give: a leaf's first WQ_LOCAL partners from the count pass's buffer,
+// the rest from the leaf positions the write pass left in pb; each is mapped to its shape and the
+// pair ordered (a < b).  A block takes WQ_EMIT_BLOCK consecutive leaves; its
 // threads walk the concatenation of those leaves' pairs (a block-wide prefix sum over the
 // counts, then a binary search for each pair's leaf), so consecutive threads write consecutive
 // pair slots.
@@ -274,7 +276,8 @@ __global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __
   const uint32_t t = threadIdx.x;
   const uint32_t k = blockIdx.x * WQ_EMIT_BLOCK + t;
   uint32_t c = k < n ? cnt[k] : 0u;
-  if (c > WQ_LOCAL) c = 0;  // written by the write pass
+  // every pair of the block's leaves: the first WQ_LOCAL of a leaf from the count pass's buffer,
+  // the rest from the leaf positions the write pass left in pb
   // block-wide inclusive prefix sum of c: within each wavefront, then across wavefronts
   uint32_t x = c;
   const int lane = __lane_id();
@@ -312,7 +315,9 @@ __global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __
     for (int u = 0; u < EMIT_UNROLL; ++u) {
       w0[u] = offs[kl[u]];
       prim[u] = order[kl[u]];
-      h2[u] = order[lbuf[(size_t)kl[u] * WQ_LOCAL + ii[u]]];
+      const size_t w = (size_t)w0[u] + ii[u];
+      const uint32_t lp = ii[u] < WQ_LOCAL ? lbuf[(size_t)kl[u] * WQ_LOCAL + ii[u]] : (w < cap ? pb[w] : 0u);
+      h2[u] = order[lp];
     }
 #pragma unroll
     for (int u = 0; u < EMIT_UNROLL; ++u) {
@@ -383,14 +388,15 @@ static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses, bool ea
     NCX_HIP_CHECK(hipGetLastError());
     if ((rc = scan_exclusive_u32(c, w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), n, w->bvh.scan_tmp, ctr)))
       return rc;
-    hipLaunchKernelGGL(k_world_emit, dim3(ceil_div(n, WQ_EMIT_BLOCK)), dim3(WQ_EMIT_BLOCK), 0, c->stream, order, (uint32_t)n,
-                       w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), w->lbuf.as<uint32_t>(), w->cap,
-                       w->pa.as<uint32_t>(), w->pb.as<uint32_t>(), ctr);
-    NCX_HIP_CHECK(hipGetLastError());
+    // the write pass first: it leaves leaf positions in pb for the copy kernel to map
     hipLaunchKernelGGL((k_world_query<MODE_WRITE>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
                        (uint32_t)n, w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), w->cap, w->pa.as<uint32_t>(),
                        w->pb.as<uint32_t>(), ctr, w->gstack.as<uint32_t>(), w->spill, w->lbuf.as<uint32_t>(),
                        (unsigned long long*)nullptr, qw + WQ_PARTS * WQ_PART_STRIDE);
+    NCX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_world_emit, dim3(ceil_div(n, WQ_EMIT_BLOCK)), dim3(WQ_EMIT_BLOCK), 0, c->stream, order, (uint32_t)n,
+                       w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), w->lbuf.as<uint32_t>(), w->cap,
+                       w->pa.as<uint32_t>(), w->pb.as<uint32_t>(), ctr);
     NCX_HIP_CHECK(hipGetLastError());
   }
   if (!w->contacts) return NCX_OK;
