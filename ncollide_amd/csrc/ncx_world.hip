@@ -259,11 +259,12 @@ static unsigned world_query_blocks(const ncx_ctx* c, size_t n) {
 // Every pair, in the order the traversal found it, so the list is the same as one full write
 // traversal per leaf would give: a leaf's first WQ_LOCAL partners from the count pass's buffer,
 // the rest from the leaf positions the write pass left in pb; each is mapped to its shape and the
-// pair ordered (a < b).  A block takes WQ_EMIT_BLOCK consecutive leaves; its
+// pair ordered (a < b).  A block takes EMIT_LEAVES consecutive leaves; its
 // threads walk the concatenation of those leaves' pairs (a block-wide prefix sum over the
 // counts, then a binary search for each pair's leaf), so consecutive threads write consecutive
 // pair slots.
 #define WQ_EMIT_BLOCK 256
+#define EMIT_LEAVES 64  // leaves per block: 256 leaves per block left dense clusters to few blocks
 #define EMIT_UNROLL 4
 __global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __restrict__ order, uint32_t n,
                                                               const uint32_t* __restrict__ cnt,
@@ -271,29 +272,25 @@ __global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __
                                                               const uint32_t* __restrict__ lbuf, size_t cap,
                                                               uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
                                                               uint32_t* __restrict__ counters) {
-  __shared__ uint32_t s_pre[WQ_EMIT_BLOCK + 1];
-  __shared__ uint32_t s_warp[WQ_EMIT_BLOCK / 64];
+  __shared__ uint32_t s_pre[EMIT_LEAVES + 1];
   const uint32_t t = threadIdx.x;
-  const uint32_t k = blockIdx.x * WQ_EMIT_BLOCK + t;
-  uint32_t c = k < n ? cnt[k] : 0u;
+  const uint32_t k0 = blockIdx.x * EMIT_LEAVES;
   // every pair of the block's leaves: the first WQ_LOCAL of a leaf from the count pass's buffer,
   // the rest from the leaf positions the write pass left in pb
-  // block-wide inclusive prefix sum of c: within each wavefront, then across wavefronts
-  uint32_t x = c;
-  const int lane = __lane_id();
+  if (t < EMIT_LEAVES) {  // the first wavefront, all lanes active: inclusive prefix sum of the counts
+    const uint32_t k = k0 + t;
+    uint32_t x = k < n ? cnt[k] : 0u;
+    const int lane = __lane_id();
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
-    if (lane >= d) x += y;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+      if (lane >= d) x += y;
+    }
+    s_pre[t + 1] = x;
+    if (t == 0) s_pre[0] = 0;
   }
-  if (lane == 63) s_warp[t / 64] = x;
   __syncthreads();
-  uint32_t wbase = 0;
-  for (uint32_t i = 0; i < t / 64; ++i) wbase += s_warp[i];
-  s_pre[t + 1] = wbase + x;
-  if (t == 0) s_pre[0] = 0;
-  __syncthreads();
-  const uint32_t total = s_pre[WQ_EMIT_BLOCK];
+  const uint32_t total = s_pre[EMIT_LEAVES];
   // EMIT_UNROLL pairs per thread per pass, their loads issued together: a block over a dense
   // cluster walks tens of thousands of pairs, and one load round trip per pair was its bound.
   for (uint32_t j0 = t; j0 < total; j0 += EMIT_UNROLL * WQ_EMIT_BLOCK) {
@@ -302,13 +299,13 @@ __global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __
     for (int u = 0; u < EMIT_UNROLL; ++u) {
       // the leaf l with s_pre[l] <= j < s_pre[l + 1] (a pair past the end repeats the first one)
       const uint32_t j = j0 + u * WQ_EMIT_BLOCK < total ? j0 + u * WQ_EMIT_BLOCK : j0;
-      uint32_t lo = 0, hi = WQ_EMIT_BLOCK;
+      uint32_t lo = 0, hi = EMIT_LEAVES;
       while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) / 2;
         if (s_pre[mid] <= j) lo = mid;
         else hi = mid;
       }
-      kl[u] = blockIdx.x * WQ_EMIT_BLOCK + lo;
+      kl[u] = k0 + lo;
       ii[u] = j - s_pre[lo];
     }
 #pragma unroll
@@ -394,7 +391,7 @@ static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses, bool ea
                        w->pb.as<uint32_t>(), ctr, w->gstack.as<uint32_t>(), w->spill, w->lbuf.as<uint32_t>(),
                        (unsigned long long*)nullptr, qw + WQ_PARTS * WQ_PART_STRIDE);
     NCX_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_world_emit, dim3(ceil_div(n, WQ_EMIT_BLOCK)), dim3(WQ_EMIT_BLOCK), 0, c->stream, order, (uint32_t)n,
+    hipLaunchKernelGGL(k_world_emit, dim3(ceil_div(n, EMIT_LEAVES)), dim3(WQ_EMIT_BLOCK), 0, c->stream, order, (uint32_t)n,
                        w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), w->lbuf.as<uint32_t>(), w->cap,
                        w->pa.as<uint32_t>(), w->pb.as<uint32_t>(), ctr);
     NCX_HIP_CHECK(hipGetLastError());
