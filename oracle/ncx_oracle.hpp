@@ -586,6 +586,18 @@ template <class R> inline bool is_affinely_dependent_triangle(V3<R> p1, V3<R> p2
   return diff <= largest * Eps<R>::eps();
 }
 
+// The first direction Vector::orthonormal_subspace_basis(&[v], f) hands to its callback for one
+// 3D vector (nalgebra): a = (v.z, 0, -v.x) if |v.x| > |v.y| else (0, -v.z, v.y), a /= |a|, and the
+// callback receives a x v; epa3.rs:299-305 stops after that first one.  v is not normalised by the
+// caller, so the direction has length |v|: it is orthogonal to v and to a.
+template <class R> inline V3<R> orthonormal_subspace_first(V3<R> v) {
+  V3<R> a;
+  if (std::fabs(v.x) > std::fabs(v.y)) a = V3<R>(v.z, R(0), -v.x);
+  else a = V3<R>(R(0), -v.z, v.y);
+  a = normalize(a);
+  return cross(a, v);
+}
+
 template <class R> struct EPA {
   std::vector<CSOPoint<R>> vertices;
   std::vector<EPAFace<R>> faces;
@@ -676,16 +688,7 @@ template <class R> struct EPA {
     } else {
       if (simplex.dim == 1) {
         V3<R> dpt = vertices[1].point - vertices[0].point;
-        // Vector::orthonormal_subspace_basis(&[dpt], f) (nalgebra, 3D, one vector):
-        // a = (v.z, 0, -v.x) if |v.x| > |v.y| else (0, -v.z, v.y); a /= |a|;
-        // the first callback receives a x v (epa3.rs:299-305 stops after it).
-        // dpt is not normalised by the caller, so the direction is not unit.
-        V3<R> a;
-        if (std::fabs(dpt.x) > std::fabs(dpt.y)) a = V3<R>(dpt.z, R(0), -dpt.x);
-        else a = V3<R>(R(0), -dpt.z, dpt.y);
-        a = normalize(a);
-        V3<R> b = cross(a, dpt);
-        vertices.push_back(cso_from_shapes(m1, g1, m2, g2, b));
+        vertices.push_back(cso_from_shapes(m1, g1, m2, g2, orthonormal_subspace_first(dpt)));
       }
       const uint32_t P1[3] = {0, 1, 2}, P2[3] = {0, 2, 1};
       const uint32_t A1[3] = {1, 1, 1}, A2[3] = {0, 0, 0};

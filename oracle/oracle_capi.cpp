@@ -329,6 +329,20 @@ int orc_project_f64(int kind, const double* pts, double* proj, int* loc_kind, in
   return 0;
 }
 
+// ---- single-function probes for the property tests
+// orthonormal_subspace_first (EPA's third point direction on a 1-simplex); returns the vector.
+void orc_orthonormal_first_f64(const double* v, double* out) {
+  V3<double> r = orthonormal_subspace_first(V3<double>(v[0], v[1], v[2]));
+  out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+int orc_affinely_dependent_f64(const double* p9) {
+  return is_affinely_dependent_triangle(V3<double>(p9[0], p9[1], p9[2]), V3<double>(p9[3], p9[4], p9[5]),
+                                        V3<double>(p9[6], p9[7], p9[8])) ? 1 : 0;
+}
+int orc_affinely_dependent_f32(const float* p9) {
+  return is_affinely_dependent_triangle(V3<float>(p9[0], p9[1], p9[2]), V3<float>(p9[3], p9[4], p9[5]),
+                                        V3<float>(p9[6], p9[7], p9[8])) ? 1 : 0;
+}
 // ---- DBVTBroadPhase
 void* orc_bp_create(double margin) { return new DBVTBroadPhase<float>((float)margin); }
 void orc_bp_free(void* h) { delete (DBVTBroadPhase<float>*)h; }

@@ -56,6 +56,9 @@ def _declare(L):
     sig("orc_ray_triangle_f64", C.c_int, P, P, P, P, P, P)
     sig("orc_ray_aabb_f64", C.c_int, P, P, P, C.c_double, C.c_int, P)
     sig("orc_project_f64", C.c_int, C.c_int, P, P, P, P, P)
+    sig("orc_orthonormal_first_f64", None, P, P)
+    sig("orc_affinely_dependent_f64", C.c_int, P)
+    sig("orc_affinely_dependent_f32", C.c_int, P)
     sig("orc_bp_create", P, C.c_double)
     sig("orc_bp_free", None, P)
     sig("orc_bp_create_proxy", C.c_uint32, P, P)
@@ -226,6 +229,20 @@ def project_origin(points):
     proj = np.zeros(3); lk = np.zeros(1, np.int32); li = np.zeros(1, np.int32); bc = np.zeros(3)
     lib().orc_project_f64(len(points) - 1, _p(pts), _p(proj), _p(lk), _p(li), _p(bc))
     return proj, int(lk[0]), int(li[0]), bc
+
+
+def orthonormal_subspace_first(v):
+    """The direction EPA's 1-simplex start adds its third point along (f64)."""
+    v = np.ascontiguousarray(v, np.float64)
+    out = np.zeros(3)
+    lib().orc_orthonormal_first_f64(_p(v), _p(out))
+    return out
+
+
+def is_affinely_dependent_triangle(p1, p2, p3, dtype=np.float64):
+    pts = np.ascontiguousarray(np.concatenate([p1, p2, p3]), dtype)
+    f = lib().orc_affinely_dependent_f32 if np.dtype(dtype) == np.float32 else lib().orc_affinely_dependent_f64
+    return bool(f(_p(pts)))
 
 
 class BroadPhase:

@@ -428,3 +428,148 @@ def test_broad_phase_matches_brute_force(orc):
     bp.update()
     assert np.array_equal(bp.pairs(), orc.pairs_brute(bb))
     assert 200 < bp.num_interferences() < 450
+
+
+# ------------------------------------------------------------------ shape AABBs against closed forms
+def _quat_mat(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def test_shape_aabbs_closed_forms(orc):
+    """aabb_ball.rs:8-13 (centre +- r), aabb_cuboid.rs:7-14 (the box of the 8 world corners, which
+    centre +- |R| h equals) and aabb_support_map.rs:35-45 on a capsule (the box of the two world
+    segment end points, grown by the radius), in f64 with random rotations.  A cuboid box with R in
+    place of |R|, or a capsule box missing the radius or the rotation, fails."""
+    rng = np.random.default_rng(21)
+    q = workloads.uniform_quats(rng, 60).astype(np.float64)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    t = rng.normal(size=(60, 3)) * 5
+    for i in range(60):
+        r, h, hh = rng.uniform(0.1, 1.0), rng.uniform(0.1, 1.0, 3), rng.uniform(0.1, 1.0)
+        ss = shapeset(orc, [("ball", r), ("cuboid", h), ("capsule", hh, r)])
+        poses = np.stack([pose7(t[i], q[i])] * 3)
+        bb = orc.shape_aabbs(ss, poses)
+        assert np.array_equal(bb[0], np.concatenate([t[i] - r, t[i] + r]))
+        M = _quat_mat(q[i])
+        corners = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]) * h
+        w = corners @ M.T + t[i]
+        assert np.allclose(bb[1], np.concatenate([w.min(0), w.max(0)]), rtol=0, atol=1e-12)
+        ends = np.array([[0, hh, 0], [0, -hh, 0]]) @ M.T + t[i]
+        assert np.allclose(bb[2], np.concatenate([ends.min(0) - r, ends.max(0) + r]), rtol=0, atol=1e-12)
+
+
+# ------------------------------------------------------------------ DBVTBroadPhase stored-box rule
+def _f32_boxes(rng, n, spread=6.0):
+    c = rng.uniform(0, spread, (n, 3)).astype(np.float32)
+    e = rng.uniform(0.1, 0.6, (n, 3)).astype(np.float32)
+    return np.concatenate([c - e, c + e], 1).astype(np.float32)
+
+
+def _brute_keys(stored):
+    hs = sorted(stored)
+    keys = []
+    for i, a in enumerate(hs):
+        for b in hs[i + 1:]:
+            A, B = stored[a], stored[b]
+            if np.all(A[:3] <= B[3:]) and np.all(B[:3] <= A[3:]):
+                keys.append((min(a, b) << 32) | max(a, b))
+    return set(keys)
+
+
+def test_stored_box_rule_independent_recurrence(orc):
+    """dbvt_broad_phase.rs:275-280 (create_proxy stores the box as given) and :325-347
+    (deferred_set_bounding_volume stores box.loosened(margin) only when the stored box does not
+    contain the new one, aabb.rs:161-163, 190-199), restated here in numpy f32 and run beside the
+    oracle over moving steps, a removal and re-created proxies (slab handles reused LIFO).  Stored
+    boxes must agree bit for bit (a proxy created since the last update has no stored box, so a
+    deferred_set_bounding_volume on it always stores the loosened box); pairs must equal brute
+    force over the stored boxes; started /
+    stopped events must be the reference's: pairs of removed proxies stop inside remove, the rest
+    by set difference at update."""
+    rng = np.random.default_rng(31)
+    margin = np.float32(0.05)
+    bp = orc.BroadPhase(float(margin))
+    stored = {}
+    for b in _f32_boxes(rng, 120):
+        h = bp.create_proxy(b)
+        stored[h] = b.copy()
+    bp.update()
+    prev = _brute_keys(stored)
+    st, sp = bp.events()
+    assert set(st.tolist()) == prev and len(sp) == 0 and len(prev) > 50
+    for step in range(8):
+        stopped_at_remove = set()
+        detached = set()
+        if step in (3, 6):
+            gone = sorted(rng.choice(sorted(stored), 15, replace=False).tolist())
+            bp.remove(gone)
+            for h in gone:
+                del stored[h]
+            stopped_at_remove = {k for k in prev if (k >> 32) in gone or (k & 0xFFFFFFFF) in gone}
+            for b in _f32_boxes(rng, 10):  # re-created proxies take the freed handles, last freed first
+                h = bp.create_proxy(b)
+                assert h in gone
+                stored[h] = b.copy()
+                detached.add(h)
+        for h in sorted(stored):
+            if rng.random() < 0.5:
+                cur = stored[h]
+                c = (cur[:3] + cur[3:]) * np.float32(0.5) + rng.normal(0, 0.04, 3).astype(np.float32)
+                e = rng.uniform(0.1, 0.6, 3).astype(np.float32) if rng.random() < 0.2 else (cur[3:] - cur[:3]) * 0.45
+                b = np.concatenate([c - e, c + e]).astype(np.float32)
+                bp.deferred_set_bounding_volume(h, b)
+                # a proxy created since the last update has no stored box yet: always requeued
+                if h in detached or not (np.all(cur[:3] <= b[:3]) and np.all(b[3:] <= cur[3:])):
+                    stored[h] = np.concatenate([b[:3] + (-margin), b[3:] + margin]).astype(np.float32)
+        bp.update()
+        for h, b in stored.items():
+            assert np.array_equal(bp.stored_bv(h), b), f"step {step}: stored box of {h}"
+        now = _brute_keys(stored)
+        assert set(bp.pairs().tolist()) == now
+        st, sp = bp.events()
+        kept = prev - stopped_at_remove
+        assert sorted(st.tolist()) == sorted(now - kept), f"step {step}: started"
+        assert sorted(sp.tolist()) == sorted(stopped_at_remove | (kept - now)), f"step {step}: stopped"
+        prev = now
+
+
+# ------------------------------------------------------------------ EPA's 1-simplex start and the triangle test
+def test_orthonormal_subspace_first_is_orthogonal(orc):
+    """The direction EPA adds its third point along from a segment (nalgebra's
+    orthonormal_subspace_basis, epa3.rs:295-305) is orthogonal to the segment and as long as it
+    (a unit vector crossed with it), on both branches (|x| > |y| and not)."""
+    rng = np.random.default_rng(41)
+    for _ in range(400):
+        v = rng.normal(size=3) * 10 ** rng.uniform(-3, 3)
+        r = orc.orthonormal_subspace_first(v)
+        nv = np.linalg.norm(v)
+        assert abs(r @ v) <= 1e-13 * nv * nv
+        assert abs(np.linalg.norm(r) - nv) <= 1e-13 * nv
+
+
+def test_is_affinely_dependent_triangle_threshold(orc):
+    """utils/triangle.rs:85-102: relative_eq!(|p1p2 x p1p3|^2, 0, epsilon = (100 eps)^2) with
+    approx's default max_relative (eps) holds exactly when |p1p2 x p1p3|^2 <= (100 eps)^2 (the
+    relative test |x| <= eps |x| never holds for x > 0).  Checked in f64 and f32 on triangles
+    scaled across the threshold (cases within 0.1% of it are skipped) and on collinear points."""
+    rng = np.random.default_rng(47)
+    for dt, eps in ((np.float64, np.finfo(np.float64).eps), (np.float32, np.finfo(np.float32).eps)):
+        thr = (100 * float(eps)) ** 2
+        tested = [0, 0]
+        for _ in range(600):
+            p = rng.normal(size=(3, 3))
+            c2 = np.sum(np.cross(p[1] - p[0], p[2] - p[0]) ** 2)
+            s = (thr / c2) ** 0.25 * 10 ** rng.uniform(-0.25, 0.25)  # |cross|^2 grows as s^4
+            p = (p * s).astype(dt)
+            x = np.sum(np.cross((p[1] - p[0]).astype(np.float64), (p[2] - p[0]).astype(np.float64)) ** 2)
+            if abs(x / thr - 1) < 1e-3:
+                continue
+            got = orc.is_affinely_dependent_triangle(p[0], p[1], p[2], dtype=dt)
+            assert got == (x <= thr), (dt, x, thr)
+            tested[int(got)] += 1
+        assert min(tested) > 100
+        p0, d = rng.normal(size=3), rng.normal(size=3)
+        assert orc.is_affinely_dependent_triangle(p0, p0 + d, p0 + 2 * d, dtype=dt)
