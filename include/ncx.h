@@ -162,6 +162,16 @@ int ncx_trimesh_cast_rays(ncx_mesh* mesh, const ncx_pose* pose, const float* con
 int ncx_trimesh_cast_rays_async(ncx_mesh* mesh, const ncx_pose* pose, const float* const ray_soa_dev[6], size_t n,
                                 float max_toi, float* toi_dev, float* normal_dev, uint32_t* feature_dev);
 int ncx_trimesh_check(ncx_mesh* mesh);
+/* No reference counterpart: the same cast as ncx_trimesh_cast_rays_async for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no allocating routine: its work counters and traversal spill stacks live
+ * in a 16 MiB device array of the library, one per GPU, held until the process ends.  Enqueue
+ * only; ncx_trimesh_check reports a traversal failure.  Every call on one GPU shares that array,
+ * so calls that overlap on one GPU (different streams or host threads) race.  Returns
+ * NCX_E_CAPACITY when the mesh's tree is too deep for the scratch to hold one block of spill
+ * stacks per CU. */
+int ncx_trimesh_cast_rays_noalloc(ncx_mesh* mesh, const ncx_pose* pose, const float* const ray_soa_dev[6], size_t n,
+                                  float max_toi, float* toi_dev, float* normal_dev, uint32_t* feature_dev);
 /* No reference counterpart (measurement): record events around each cast's traversal kernel;
  * ncx_trimesh_kernel_ms returns the last cast's kernel time once it has completed. */
 int ncx_trimesh_set_timing(ncx_mesh* mesh, int enable);

@@ -49,6 +49,7 @@ SIGNATURES = {
     "ncx_trimesh_cast_stats": (C.c_int, [P, P, P, SZ, F32, P, P]),
     "ncx_trimesh_cast_rays_async": (C.c_int, [P, P, P, SZ, F32, P, P, P]),
     "ncx_trimesh_check": (C.c_int, [P]),
+    "ncx_trimesh_cast_rays_noalloc": (C.c_int, [P, P, P, SZ, F32, P, P, P]),
     "ncx_trimesh_set_timing": (C.c_int, [P, C.c_int]),
     "ncx_trimesh_kernel_ms": (C.c_int, [P, P]),
     "ncx_convex_hull": (C.c_int, [P, SZ, P, P]),

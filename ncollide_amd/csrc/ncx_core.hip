@@ -21,6 +21,8 @@ extern "C" int ncx_init(ncx_ctx** out, int device, void* stream) {
   NCX_HIP_CHECK(hipSetDevice(device));
   hipDeviceProp_t prop;
   NCX_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+  int rc = rc_scratch_setup(device);
+  if (rc) return rc;
   ncx_ctx* c = new ncx_ctx;
   c->device = device;
   c->stream = (hipStream_t)stream;

@@ -239,6 +239,8 @@ int lbvh_refit(ncx_ctx* ctx, LBVH& bvh, const float* const box[6], float expand,
 
 // Fills n uint32 with v on the context stream (a kernel, so it is safe inside graph capture).
 int fill_u32(ncx_ctx* ctx, uint32_t* p, size_t n, uint32_t v);
+// Per-device address of the allocation-free ray cast's scratch array (ncx_raycast.hip).
+int rc_scratch_setup(int device);
 // Device-wide exclusive scan of n uint32 (in != out); returns total through *total_dev (device) if non-null.
 int scan_exclusive_u32(ncx_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n, DevBuf& tmp, uint32_t* total_dev);
 // Stable LSD radix sort of (key, val) uint32 pairs on bits [begin_bit, end_bit).

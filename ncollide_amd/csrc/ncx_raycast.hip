@@ -8,6 +8,7 @@
 // ray_intersection_with_triangle (src/query/ray/ray_triangle.rs:32-114) with the
 // same operation order, so every per-triangle toi is bit-identical to the
 // reference's; the hit kept is the (toi, feature) lexicographic minimum.
+#include <mutex>
 #include "ncx_internal.h"
 
 struct ncx_mesh {
@@ -22,6 +23,7 @@ struct ncx_mesh {
   DevBuf work;      // ray counters of the persistent cast (one per part)
   DevBuf qnodes;    // QNode[ntri - 1]: the LBVH with 16-bit quantised child boxes (cast kernel)
   DevBuf qframe;    // float[6]: quantisation origin (3) and step (3)
+  uint32_t levels = 1;  // internal-node levels of the cast tree (root = level 1)
   bool timing = false;              // record events around the cast kernel (ncx_trimesh_set_timing)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -320,13 +322,13 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_pt(const QNode* __restrict__ 
                                                        const float* __restrict__ dy, const float* __restrict__ dz,
                                                        uint32_t nrays, float max_toi, float* __restrict__ toi_out,
                                                        float* __restrict__ nrm_out, uint32_t* __restrict__ feat_out,
-                                                       uint32_t* __restrict__ ovf, uint32_t* __restrict__ err,
-                                                       uint32_t* __restrict__ work) {
+                                                       uint32_t* __restrict__ ovf, uint32_t gdepth,
+                                                       uint32_t* __restrict__ err, uint32_t* __restrict__ work) {
   __shared__ uint32_t lstack[RC_LDS_STACK * RC_BLOCK];
   const int tid = threadIdx.x;
   const int lane = __lane_id();
   const size_t gtid = (size_t)blockIdx.x * RC_BLOCK + tid;
-  uint32_t* gstack = ovf + gtid * RC_GLOBAL_STACK;
+  uint32_t* gstack = ovf + gtid * gdepth;
   iso m;
   m.t = f3{pose.tx, pose.ty, pose.tz};
   m.q = quat{pose.qi, pose.qj, pose.qk, pose.qw};
@@ -496,7 +498,7 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_pt(const QNode* __restrict__ 
       // The stack's top entry stays in a register; pushing over it moves it to memory.
       if (top_valid) {
         if (sp < RC_LDS_STACK) lstack[sp * RC_BLOCK + tid] = top;
-        else if (sp < RC_LDS_STACK + RC_GLOBAL_STACK) gstack[sp - RC_LDS_STACK] = top;
+        else if (sp < RC_LDS_STACK + (int)gdepth) gstack[sp - RC_LDS_STACK] = top;
         else { atomicOr(err, 1u); fin = true; }
         sp++;
       }
@@ -527,19 +529,21 @@ static int preorder_nodes(ncx_mesh* m) {
   const size_t ni = m->ntri - 1;
   std::vector<QNode> src(ni), dst(ni);
   NCX_HIP_CHECK(hipMemcpy(src.data(), m->qnodes.p, ni * sizeof(QNode), hipMemcpyDeviceToHost));
-  std::vector<uint32_t> newid(ni, 0xFFFFFFFFu), stack;
+  std::vector<uint32_t> newid(ni, 0xFFFFFFFFu), stack, depth(ni, 0);
   stack.reserve(128);
   stack.push_back(0);
-  uint32_t next = 0;
+  uint32_t next = 0, max_depth = 0;
   while (!stack.empty()) {
     uint32_t v = stack.back();
     stack.pop_back();
     NCX_REQUIRE(v < ni && newid[v] == 0xFFFFFFFFu, "ncx_trimesh_create: malformed tree");
     newid[v] = next++;
+    max_depth = std::max(max_depth, depth[v]);
     const QNode& q = src[v];
-    if (!(q.child_r & NCX_LEAF_BIT)) stack.push_back(q.child_r);
-    if (!(q.child_l & NCX_LEAF_BIT)) stack.push_back(q.child_l);
+    if (!(q.child_r & NCX_LEAF_BIT)) { depth[q.child_r] = depth[v] + 1; stack.push_back(q.child_r); }
+    if (!(q.child_l & NCX_LEAF_BIT)) { depth[q.child_l] = depth[v] + 1; stack.push_back(q.child_l); }
   }
+  m->levels = max_depth + 1;
   NCX_REQUIRE(next == ni, "ncx_trimesh_create: tree does not reach every node");
   for (size_t v = 0; v < ni; ++v) {
     QNode q = src[v];
@@ -645,8 +649,8 @@ static int cast_impl(ncx_mesh* m, const ncx_pose* pose, const float* const r[6],
     if (m->timing) NCX_HIP_CHECK(hipEventRecord(m->ev0, ctx->stream));
     hipLaunchKernelGGL(k_cast_pt, dim3(pblocks), dim3(RC_BLOCK), 0, ctx->stream, m->qnodes.as<QNode>(),
                        m->qframe.as<float>(), m->tris.as<float4>(), (uint32_t)m->ntri, (uint32_t)m->ntri, *pose, r[0], r[1], r[2], r[3], r[4],
-                       r[5], (uint32_t)n, max_toi, toi, nrm, feat, m->overflow.as<uint32_t>(), m->err.as<uint32_t>(),
-                       m->work.as<uint32_t>());
+                       r[5], (uint32_t)n, max_toi, toi, nrm, feat, m->overflow.as<uint32_t>(), (uint32_t)RC_GLOBAL_STACK,
+                       m->err.as<uint32_t>(), m->work.as<uint32_t>());
     NCX_HIP_CHECK(hipGetLastError());
     if (m->timing) NCX_HIP_CHECK(hipEventRecord(m->ev1, ctx->stream));
   }
@@ -704,4 +708,72 @@ extern "C" int ncx_trimesh_cast_stats(ncx_mesh* m, const ncx_pose* pose, const f
                                       float max_toi, uint32_t* nodes_dev, uint32_t* tris_dev) {
   NCX_REQUIRE(nodes_dev && tris_dev, "ncx_trimesh_cast_stats: null output");
   return cast_impl(m, pose, ray_soa_dev, n, max_toi, nullptr, nullptr, nullptr, nodes_dev, tris_dev, true, true);
+}
+
+// ------------------------------------------------------------------ allocation-free cast
+// Device scratch of ncx_trimesh_cast_rays_noalloc: the work counters of the persistent cast
+// (RC_PARTS counters, RC_PART_STRIDE words apart) followed by the traversal spill stacks.  It is
+// a file-scope device array, so each GPU the library runs on holds its own copy, at its own
+// address, until the process ends.  Host code reaches it only through the per-device address
+// that hipGetSymbolAddress returns (the array's name on the host is not a device address).
+#define RC_SCRATCH_BYTES (16u << 20)
+#define RC_SCRATCH_WORDS (RC_SCRATCH_BYTES / 4u)
+#define RC_SCRATCH_MAX_DEVICES 64
+static __device__ __align__(256) uint32_t scratch[RC_SCRATCH_WORDS];
+static uint32_t* g_scratch_ptr[RC_SCRATCH_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// Looks up the scratch address on `device` once; ncx_init calls it with the device current.
+int rc_scratch_setup(int device) {
+  NCX_REQUIRE(device >= 0 && device < RC_SCRATCH_MAX_DEVICES, "ncx_init: device index beyond 64");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return NCX_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    ncx_set_error(std::string("ncx_init: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    return NCX_E_HIP;
+  }
+  g_scratch_ptr[device] = (uint32_t*)p;
+  return NCX_OK;
+}
+
+extern "C" int ncx_trimesh_cast_rays_noalloc(ncx_mesh* m, const ncx_pose* pose, const float* const r[6], size_t n,
+                                             float max_toi, float* toi, float* nrm, uint32_t* feat) {
+  NCX_REQUIRE(m && pose && r && toi && nrm && feat, "ncx_trimesh_cast_rays_noalloc: null argument");
+  NCX_REQUIRE(n <= 0x7FFFFFFFull, "ncx_trimesh_cast_rays_noalloc: more than 2^31 - 1 rays in one call");
+  if (n == 0) return NCX_OK;
+  for (int d = 0; d < 6; ++d) NCX_REQUIRE(r[d], "ncx_trimesh_cast_rays_noalloc: null ray array");
+  ncx_ctx* ctx = m->ctx;
+  NCX_REQUIRE(ctx->device >= 0 && ctx->device < RC_SCRATCH_MAX_DEVICES, "ncx_trimesh_cast_rays_noalloc: bad device");
+  uint32_t* sp;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    sp = g_scratch_ptr[ctx->device];
+  }
+  NCX_REQUIRE(sp, "ncx_trimesh_cast_rays_noalloc: scratch not set up on this device (ncx_init)");
+  NCX_REQUIRE(m->qframe.p && (m->ntri == 1 || m->qnodes.p), "ncx_trimesh_cast_rays_noalloc: mesh nodes missing");
+  // A traversal keeps at most `levels` pending subtrees: one in a register, RC_LDS_STACK in LDS,
+  // the rest in its spill stack.
+  const uint32_t gdepth = m->levels + 1 > RC_LDS_STACK ? m->levels + 1 - RC_LDS_STACK : 1u;
+  const size_t counters = (size_t)RC_PARTS * RC_PART_STRIDE;
+  const size_t per_block = (size_t)RC_BLOCK * gdepth;
+  unsigned pblocks = std::min<unsigned>(RC_GRID_BLOCKS, (unsigned)ctx->num_cus * 13);
+  pblocks = std::min<unsigned>(pblocks, ceil_div(n, RC_BLOCK));
+  const size_t fit = (RC_SCRATCH_WORDS - counters) / per_block;
+  if (fit < pblocks) pblocks = (unsigned)fit;
+  if (pblocks < std::min<unsigned>((unsigned)ctx->num_cus, ceil_div(n, RC_BLOCK))) {
+    ncx_set_error("ncx_trimesh_cast_rays_noalloc: the mesh's " + std::to_string(m->levels) +
+                  "-level tree needs " + std::to_string(per_block * 4) + " bytes of spill stack per block; " +
+                  std::to_string(RC_SCRATCH_BYTES) + " bytes of scratch do not hold one block per CU");
+    return NCX_E_CAPACITY;
+  }
+  int rc;
+  if ((rc = fill_u32(ctx, sp, counters, 0u))) return rc;
+  hipLaunchKernelGGL(k_cast_pt, dim3(pblocks), dim3(RC_BLOCK), 0, ctx->stream, m->qnodes.as<QNode>(),
+                     m->qframe.as<float>(), m->tris.as<float4>(), (uint32_t)m->ntri, (uint32_t)m->ntri, *pose, r[0],
+                     r[1], r[2], r[3], r[4], r[5], (uint32_t)n, max_toi, toi, nrm, feat, sp + counters, gdepth,
+                     m->err.as<uint32_t>(), sp);
+  NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
 }
