@@ -89,6 +89,31 @@ int ncx_bp_set_aabbs_dev(ncx_bp* bp, const uint32_t* handles_dev, const float* c
 int ncx_bp_set_aabbs(ncx_bp* bp, const uint32_t* handles, const float* const aabb_soa_host[6], size_t n);
 /* remove (:282-323). */
 int ncx_bp_remove(ncx_bp* bp, const uint32_t* handles, size_t n);
+/* remove (:282-323) with its handler: the pairs of the current set that contain a removed proxy are
+ * dropped and written to removed_keys (sorted, host; nullable), *n_removed = their number.  They
+ * are not reported again as stopped by the next update's ncx_bp_events. */
+int ncx_bp_remove_with_pairs(ncx_bp* bp, const uint32_t* handles, size_t n, uint64_t* removed_keys, size_t cap,
+                             size_t* n_removed);
+/* The pair filter BroadPhaseInterferenceHandler::is_interference_allowed (broad_phase.rs:31) in the
+ * form the world gives it: per-proxy CollisionGroups (src/pipeline/object/collision_groups.rs:
+ * membership, whitelist, blacklist masks over 30 groups); a pair is kept when
+ * can_interact_with_groups (:353-359) holds.  Proxies start with CollisionGroups::new() (all
+ * groups).  Host arrays; takes effect at the next update, which recomputes every pair. */
+int ncx_bp_set_collision_groups(ncx_bp* bp, const uint32_t* handles, const uint32_t* membership,
+                                const uint32_t* whitelist, const uint32_t* blacklist, size_t n);
+/* interferences_with_bounding_volume / _with_point / _with_ray (:387-432), batched: the proxies of
+ * the last update's tree (less those removed since) whose stored box intersects query box q
+ * (aabb.rs:156-158), contains point q (aabb.rs:138-146), or is hit by ray q within max_toi
+ * (AABB::intersects_ray, src/query/ray/ray_aabb.rs:13-50).  Queries are device SoA; the handles of
+ * query q go to handles_dev[offsets_dev[q] .. offsets_dev[q + 1]) (offsets_dev: nq + 1 entries,
+ * device), in no particular order.  *total = the number of handles; NCX_E_CAPACITY (nothing
+ * written) when it exceeds cap. */
+int ncx_bp_interferences_with_aabbs(ncx_bp* bp, const float* const aabb_soa_dev[6], size_t nq, uint32_t* offsets_dev,
+                                    uint32_t* handles_dev, size_t cap, size_t* total);
+int ncx_bp_interferences_with_points(ncx_bp* bp, const float* const point_soa_dev[3], size_t nq,
+                                     uint32_t* offsets_dev, uint32_t* handles_dev, size_t cap, size_t* total);
+int ncx_bp_interferences_with_rays(ncx_bp* bp, const float* const ray_soa_dev[6], size_t nq, float max_toi,
+                                   uint32_t* offsets_dev, uint32_t* handles_dev, size_t cap, size_t* total);
 /* update (:174-259): rebuild the pair set on the GPU. */
 int ncx_bp_update(ncx_bp* bp);
 /* num_interferences (:97-99). */
@@ -141,6 +166,28 @@ int ncx_contact_batch(ncx_shapes* s, const ncx_pose* poses_dev, const uint32_t* 
 /* Batched query::distance (src/query/distance/distance.rs:9-33). */
 int ncx_distance_batch(ncx_shapes* s, const ncx_pose* poses_dev, const uint32_t* ia_dev, const uint32_t* ib_dev,
                        size_t npairs, float* dist_dev, int32_t* gjk_iters_dev);
+
+/* ---------------------------------------------------------------- time of impact
+ * Batched query::time_of_impact (src/query/time_of_impact/time_of_impact.rs:66-158) under
+ * translational motion over pairs (ia[k], ib[k]) of one shape set: velocities per shape (vel_dev,
+ * 3 floats per shape, like poses_dev).  Ball-Ball: time_of_impact_ball_ball
+ * (time_of_impact_ball_ball.rs:7-61); every other pair: time_of_impact_support_map_support_map
+ * (time_of_impact_support_map_support_map.rs:10-116), GJK's ray cast on the Minkowski difference
+ * (src/query/algorithms/gjk.rs:214-365).  Outputs per pair: has = 1 for Some(TOI); toi; witness1,
+ * witness2, normal1, normal2 (3 floats each, local frames of the two shapes); status 0
+ * OutOfIterations, 1 Converged, 2 Failed, 3 Penetrating (time_of_impact.rs:7-25). */
+int ncx_time_of_impact_batch(ncx_shapes* s, const ncx_pose* poses_dev, const float* vel_dev, const uint32_t* ia_dev,
+                             const uint32_t* ib_dev, size_t npairs, float max_toi, float target_distance,
+                             uint8_t* has_dev, float* toi_dev, float* witness1_dev, float* witness2_dev,
+                             float* normal1_dev, float* normal2_dev, uint8_t* status_dev);
+/* Batched RayCast::toi_and_normal_with_ray (src/query/ray/ray.rs:125-160) of shape shape_dev[k] at
+ * its pose against ray k (device SoA origins and directions): Ball (ray_ball.rs:32-43), Cuboid
+ * (ray_cuboid.rs:17-26 -> ray_aabb.rs:45-61, 178-290), Capsule and ConvexHull (ray_support_map.rs:
+ * 15-62, 117-163, GJK ray cast).  Hit: toi, world normal, feature = face id (cuboid faces 0-5, 0 for
+ * a ball, 0xFFFFFFFE for FeatureId::Unknown); miss: toi = -1, feature = UINT32_MAX. */
+int ncx_shapes_cast_rays(ncx_shapes* s, const ncx_pose* poses_dev, const uint32_t* shape_dev,
+                         const float* const ray_soa_dev[6], size_t n, float max_toi, int solid, float* toi_dev,
+                         float* normal_dev, uint32_t* feature_dev);
 
 /* ---------------------------------------------------------------- TriMesh ray casting
  * TriMesh::new (src/shape/trimesh.rs:100-197) with a GPU-built LBVH in place of
@@ -253,6 +300,21 @@ int ncx_world_destroy(ncx_world* world);
 /* Learned init_dir (ncx_mlp) for polytope-polytope pairs, GJK capped at 32 iterations with plain
  * GJK on the cap; NULL turns it off. */
 int ncx_world_set_warmstart(ncx_world* world, ncx_mlp* mlp);
+/* The init_dir of each pair of the last step under the learned warm start (3 floats per pair, in
+ * the order of ncx_world_outputs; all-zero = None), valid until the next step. */
+int ncx_world_warmstart_dirs(ncx_world* world, const float** dirs_dev);
+/* The persistent narrow phase's warm start: ConvexPolyhedronConvexPolyhedronManifoldGenerator's
+ * last_gjk_dir (src/pipeline/narrow_phase/contact_generator/
+ * convex_polyhedron_convex_polyhedron_manifold_generator.rs:24-33, 96-145) kept per pair of
+ * polyhedra (Cuboid / ConvexHull on both sides) across steps on the device: a pair's GJK starts from
+ * its previous step's result direction (GJK uncapped, as the reference), a pair new to the broad
+ * phase starts from None.  Exclusive with ncx_world_set_warmstart.  Device memory: 24 bytes per
+ * table slot (a power of two, at least 2 * max_pairs) plus 24 bytes per pair. */
+int ncx_world_set_temporal_warmstart(ncx_world* world, int enable);
+/* Under the temporal warm start: each pair's init_dir of the last step (all-zero = None) and its
+ * result direction (the next step's last_gjk_dir), 3 floats per pair in the order of
+ * ncx_world_outputs; valid after ncx_world_sync until the next step. */
+int ncx_world_temporal_dirs(ncx_world* world, const float** init_dev, const float** result_dev);
 /* LBVH rebuilt every k >= 1 steps and refit in between (default 2). */
 int ncx_world_set_rebuild_every(ncx_world* world, int k);
 int ncx_world_step(ncx_world* world, const ncx_pose* poses_dev, int use_graph);
@@ -270,6 +332,28 @@ int ncx_world_sync(ncx_world* world, size_t* npairs, size_t* n_epa_overflow);
 int ncx_world_outputs(ncx_world* world, uint32_t** ia, uint32_t** ib, uint8_t** has_contact, float** world1,
                       float** world2, float** normal, float** depth, int32_t** gjk_iters);
 int ncx_world_stored_aabbs(ncx_world* world, const float** soa_dev);
+/* World-level queries after a step, through the scene's broad phase (handles are shape indices).
+ * ncx_world_interferences_with_rays: glue::interferences_with_ray (src/pipeline/glue/query.rs:
+ * 13-66): the shapes whose stored box the ray enters within max_toi and whose shape (at poses_dev,
+ * the last step's poses) the ray hits, RayCast::toi_and_normal_with_ray with solid = true; per hit
+ * the handle, toi and world normal (toi_dev / normal_dev nullable), in CSR form as the
+ * ncx_bp_interferences_* calls.  aabb_only != 0 gives BroadPhase::interferences_with_ray (stored
+ * boxes only; poses_dev may be NULL).
+ * ncx_world_first_interference_with_rays: first_interference_with_ray (glue/query.rs:200-224 ->
+ * dbvt_broad_phase.rs:433-460, RayIntersectionCostFnVisitor ray_intersection_cost_fn_visitor.rs:
+ * 64-107): per ray the hit with the least toi (then the least handle: the reference keeps the
+ * first of equal-toi hits its heap pops), handle UINT32_MAX and toi -1 on a miss.
+ * ncx_world_interferences_with_aabbs: glue::interferences_with_aabb (glue/query.rs) = the stored
+ * boxes intersecting each query box. */
+int ncx_world_interferences_with_rays(ncx_world* world, const ncx_pose* poses_dev, const float* const ray_soa_dev[6],
+                                      size_t n, float max_toi, int aabb_only, uint32_t* offsets_dev,
+                                      uint32_t* handles_dev, float* toi_dev, float* normal_dev, size_t cap,
+                                      size_t* total);
+int ncx_world_first_interference_with_rays(ncx_world* world, const ncx_pose* poses_dev,
+                                           const float* const ray_soa_dev[6], size_t n, float max_toi,
+                                           uint32_t* handle_dev, float* toi_dev, float* normal_dev);
+int ncx_world_interferences_with_aabbs(ncx_world* world, const float* const aabb_soa_dev[6], size_t n,
+                                       uint32_t* offsets_dev, uint32_t* handles_dev, size_t cap, size_t* total);
 
 #ifdef __cplusplus
 }

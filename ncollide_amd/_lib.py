@@ -43,6 +43,8 @@ SIGNATURES = {
     "ncx_contact_batch_capped": (C.c_int, [P, P, P, P, SZ, F32, P, I32, P, P, P, P, P, P]),
     "ncx_distance_batch": (C.c_int, [P, P, P, P, SZ, P, P]),
     "ncx_distance_batch_capped": (C.c_int, [P, P, P, P, SZ, P, C.c_int32, P, P]),
+    "ncx_time_of_impact_batch": (C.c_int, [P, P, P, P, P, SZ, F32, F32, P, P, P, P, P, P, P]),
+    "ncx_shapes_cast_rays": (C.c_int, [P, P, P, P, SZ, F32, C.c_int, P, P, P]),
     "ncx_trimesh_create": (C.c_int, [P, P, SZ, P, SZ, P]),
     "ncx_trimesh_destroy": (C.c_int, [P]),
     "ncx_trimesh_cast_rays": (C.c_int, [P, P, P, SZ, F32, P, P, P]),
@@ -75,6 +77,17 @@ SIGNATURES = {
     "ncx_world_sync": (C.c_int, [P, P, P]),
     "ncx_world_outputs": (C.c_int, [P, P, P, P, P, P, P, P, P]),
     "ncx_world_stored_aabbs": (C.c_int, [P, P]),
+    "ncx_world_warmstart_dirs": (C.c_int, [P, P]),
+    "ncx_world_set_temporal_warmstart": (C.c_int, [P, C.c_int]),
+    "ncx_world_temporal_dirs": (C.c_int, [P, P, P]),
+    "ncx_world_interferences_with_rays": (C.c_int, [P, P, P, SZ, F32, C.c_int, P, P, P, P, SZ, P]),
+    "ncx_world_first_interference_with_rays": (C.c_int, [P, P, P, SZ, F32, P, P, P]),
+    "ncx_world_interferences_with_aabbs": (C.c_int, [P, P, SZ, P, P, SZ, P]),
+    "ncx_bp_remove_with_pairs": (C.c_int, [P, P, SZ, P, SZ, P]),
+    "ncx_bp_set_collision_groups": (C.c_int, [P, P, P, P, P, SZ]),
+    "ncx_bp_interferences_with_aabbs": (C.c_int, [P, P, SZ, P, P, SZ, P]),
+    "ncx_bp_interferences_with_points": (C.c_int, [P, P, SZ, P, P, SZ, P]),
+    "ncx_bp_interferences_with_rays": (C.c_int, [P, P, SZ, F32, P, P, SZ, P]),
 }
 
 
@@ -120,3 +133,16 @@ def ptr(t):
 def ptr6(arrs):
     return PP6(*[C.c_void_p(a.data_ptr()) if hasattr(a, "data_ptr") else a.ctypes.data_as(C.c_void_p)
                  for a in arrs])
+
+
+def ptrs(arrs):
+    """Array of raw pointers (device tensors or host arrays) for `const float* const x[k]` arguments."""
+    return (P * len(arrs))(*[C.c_void_p(a.data_ptr()) if hasattr(a, "data_ptr") else a.ctypes.data_as(C.c_void_p)
+                             for a in arrs])
+
+
+def soa_columns(rows, k, device):
+    """(n, k) host or device array -> k contiguous float32 device columns."""
+    import torch
+    t = torch.as_tensor(rows, device=device, dtype=torch.float32).reshape(-1, k)
+    return [t[:, d].contiguous() for d in range(k)]

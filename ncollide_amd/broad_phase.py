@@ -52,8 +52,55 @@ class DBVTBroadPhase:
                                                      handles_dev.numel()))
 
     def remove(self, handles):
+        """BroadPhase::remove: returns the sorted keys of the pairs it drops (the reference hands
+        each to its handler); they are not reported again as stopped by the next update."""
         hs = np.ascontiguousarray(handles, np.uint32)
-        _lib.check(self.ctx.lib.ncx_bp_remove(self.h, _lib.ptr(hs), len(hs)))
+        n = C.c_size_t()
+        cap = max(1, self.num_interferences())
+        out = np.zeros(cap, np.uint64)
+        _lib.check(self.ctx.lib.ncx_bp_remove_with_pairs(self.h, _lib.ptr(hs), len(hs), _lib.ptr(out), cap,
+                                                         C.byref(n)))
+        return out[:n.value]
+
+    def set_collision_groups(self, handles, membership, whitelist, blacklist):
+        """Per-proxy CollisionGroups masks, the pair filter (can_interact_with_groups)."""
+        hs = np.ascontiguousarray(handles, np.uint32)
+        m = np.ascontiguousarray(membership, np.uint32)
+        w = np.ascontiguousarray(whitelist, np.uint32)
+        b = np.ascontiguousarray(blacklist, np.uint32)
+        _lib.check(self.ctx.lib.ncx_bp_set_collision_groups(self.h, _lib.ptr(hs), _lib.ptr(m), _lib.ptr(w),
+                                                            _lib.ptr(b), len(hs)))
+
+    # -- interference queries (batched; CSR results: offsets (nq + 1,), handles)
+    def _collect(self, fn, cols, nq, *extra):
+        dev = self.ctx.device
+        offs = torch.empty(nq + 1, dtype=torch.int32, device=dev)
+        total = C.c_size_t()
+        cap = 0
+        handles = torch.empty(1, dtype=torch.int32, device=dev)
+        while True:
+            rc = fn(self.h, _lib.ptrs(cols), nq, *extra, _lib.ptr(offs), _lib.ptr(handles), cap, C.byref(total))
+            if rc == -3 and total.value > cap:
+                cap = total.value
+                handles = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+                continue
+            _lib.check(rc)
+            break
+        return offs, handles[:total.value]
+
+    def interferences_with_bounding_volumes(self, aabbs):
+        """interferences_with_bounding_volume for each box of (nq, 6): (offsets, handles) CUDA tensors."""
+        cols = _lib.soa_columns(aabbs, 6, self.ctx.device)
+        return self._collect(self.ctx.lib.ncx_bp_interferences_with_aabbs, cols, cols[0].numel())
+
+    def interferences_with_points(self, points):
+        cols = _lib.soa_columns(points, 3, self.ctx.device)
+        return self._collect(self.ctx.lib.ncx_bp_interferences_with_points, cols, cols[0].numel())
+
+    def interferences_with_rays(self, origins, dirs, max_toi):
+        cols = _lib.soa_columns(np.hstack([np.asarray(origins, np.float32).reshape(-1, 3),
+                                           np.asarray(dirs, np.float32).reshape(-1, 3)]), 6, self.ctx.device)
+        return self._collect(self.ctx.lib.ncx_bp_interferences_with_rays, cols, cols[0].numel(), C.c_float(max_toi))
 
     # -- update and results
     def update(self):

@@ -11,7 +11,7 @@
 // produced once), instead of re-inserting leaves into a DBVT.  The set equals
 // the reference's HashMap contents after update (:174-259): intersection tests
 // are exact f32 comparisons and node boxes are exact min/max unions.
-#include "ncx_internal.h"
+#include "ncx_bvh_query.h"
 #include <algorithm>
 
 struct ncx_bp {
@@ -42,7 +42,27 @@ struct ncx_bp {
   DevBuf ev_flags, ev_offs;
   DevBuf staging_h, staging_b;
   bool have_prev = false;
+  // pairs dropped by remove() since the last update, reported there (the reference's remove
+  // handler); they are taken out of the previous pair set when the next update's events are made
+  std::vector<uint64_t> removed_pending, removed_prev;
+  DevBuf qcnt, qscan, qtot;  // interference queries
+  // collision groups per handle (membership, whitelist, blacklist), the pair filter; off until set
+  bool groups_on = false, groups_dirty = false;
+  std::vector<uint32_t> groups_host;
+  DevBuf groups;
 };
+
+// CollisionGroups::new() (src/pipeline/object/collision_groups.rs:54-60): member of and
+// interacting with all 30 groups.
+#define NCX_ALL_GROUPS ((1u << 30) - 1u)
+
+// CollisionGroups::can_interact_with_groups (collision_groups.rs:353-359).
+__device__ __forceinline__ bool groups_allow(const uint32_t* __restrict__ g, uint32_t a, uint32_t b) {
+  if (!g) return true;
+  const uint32_t ma = g[3 * a], wa = g[3 * a + 1], ba = g[3 * a + 2];
+  const uint32_t mb = g[3 * b], wb = g[3 * b + 1], bb = g[3 * b + 2];
+  return (ma & bb) == 0 && (mb & ba) == 0 && (ma & wb) != 0 && (mb & wa) != 0;
+}
 
 static int bp_grow(ncx_bp* bp, size_t need) {
   if (need <= bp->cap) return NCX_OK;
@@ -91,7 +111,8 @@ extern "C" int ncx_bp_destroy(ncx_bp* bp) {
   DevBuf* bufs[] = {&bp->stored, &bp->pending, &bp->pending_flag, &bp->in_tree, &bp->alive, &bp->boxes,
                     &bp->cnt, &bp->offs, &bp->total, &bp->pa, &bp->pb, &bp->prev_a, &bp->prev_b,
                     &bp->sort_t1, &bp->sort_t2, &bp->sort_t3, &bp->sort_hist, &bp->sort_scan, &bp->ev_flags,
-                    &bp->ev_offs, &bp->staging_h, &bp->staging_b};
+                    &bp->ev_offs, &bp->staging_h, &bp->staging_b, &bp->qcnt, &bp->qscan, &bp->qtot,
+                    &bp->groups};
   for (DevBuf* b : bufs) b->release();
   bp->bvh.release();
   delete bp;
@@ -109,6 +130,13 @@ static uint32_t slab_insert(ncx_bp* bp) {
     bp->occupied[key] = 1;
   }
   bp->slab_len++;
+  if (bp->groups_on) {  // a new proxy starts with CollisionGroups::new()
+    if (bp->groups_host.size() < 3 * bp->occupied.size()) bp->groups_host.resize(3 * bp->occupied.size(), 0u);
+    bp->groups_host[3 * key] = NCX_ALL_GROUPS;
+    bp->groups_host[3 * key + 1] = NCX_ALL_GROUPS;
+    bp->groups_host[3 * key + 2] = 0u;
+    bp->groups_dirty = true;
+  }
   return key;
 }
 
@@ -186,7 +214,7 @@ __global__ __launch_bounds__(128) void k_bp_query(const LNode* __restrict__ node
                                                   const uint32_t* __restrict__ alive, const float* __restrict__ boxes,
                                                   uint32_t n, uint32_t* __restrict__ cnt,
                                                   const uint32_t* __restrict__ offs, uint32_t* __restrict__ pa,
-                                                  uint32_t* __restrict__ pb) {
+                                                  uint32_t* __restrict__ pb, const uint32_t* __restrict__ groups) {
   __shared__ uint32_t stack[32 * 128];
   const int tid = threadIdx.x;
   uint32_t k = blockIdx.x * 128 + tid;
@@ -198,8 +226,9 @@ __global__ __launch_bounds__(128) void k_bp_query(const LNode* __restrict__ node
   uint32_t hk = alive[prim];
   uint32_t c = 0, w = WRITE ? offs[k] : 0u;
   auto emit = [&](uint32_t leafpos) {
+    uint32_t h2 = alive[order[leafpos]];
+    if (!groups_allow(groups, hk, h2)) return;  // is_interference_allowed (broad_phase.rs:31)
     if (WRITE) {
-      uint32_t h2 = alive[order[leafpos]];
       pa[w] = hk < h2 ? hk : h2;
       pb[w] = hk < h2 ? h2 : hk;
       w++;
@@ -241,7 +270,8 @@ __global__ __launch_bounds__(128) void k_bp_query_deep(const LNode* __restrict__
                                                        const float* __restrict__ boxes, uint32_t n,
                                                        uint32_t* __restrict__ cnt, const uint32_t* __restrict__ offs,
                                                        uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
-                                                       uint32_t* __restrict__ gstack_all, uint32_t depth) {
+                                                       uint32_t* __restrict__ gstack_all, uint32_t depth,
+                                                       const uint32_t* __restrict__ groups) {
   uint32_t k = blockIdx.x * 128 + threadIdx.x;
   if (k >= n) return;
   uint32_t* stack = gstack_all + (size_t)k * depth;
@@ -262,12 +292,14 @@ __global__ __launch_bounds__(128) void k_bp_query_deep(const LNode* __restrict__
       uint32_t ch = side ? cr : cl;
       bool h = side ? hr : hl;
       if (h && (ch & NCX_LEAF_BIT)) {
-        if (WRITE) {
-          uint32_t h2 = alive[order[ch & ~NCX_LEAF_BIT]];
-          pa[w] = hk < h2 ? hk : h2;
-          pb[w] = hk < h2 ? h2 : hk;
-          w++;
-        } else c++;
+        uint32_t h2 = alive[order[ch & ~NCX_LEAF_BIT]];
+        if (groups_allow(groups, hk, h2)) {
+          if (WRITE) {
+            pa[w] = hk < h2 ? hk : h2;
+            pb[w] = hk < h2 ? h2 : hk;
+            w++;
+          } else c++;
+        }
         if (side) hr = false; else hl = false;
       }
     }
@@ -358,11 +390,39 @@ extern "C" int ncx_bp_set_aabbs(ncx_bp* bp, const uint32_t* handles, const float
   return NCX_OK;
 }
 
-extern "C" int ncx_bp_remove(ncx_bp* bp, const uint32_t* handles, size_t n) {
+static int bp_sort_current(ncx_bp* bp, bool prev);
+static int download_keys(ncx_bp* bp, const DevBuf& a, const DevBuf& b, size_t n, uint64_t* out);
+
+// BroadPhase::remove (dbvt_broad_phase.rs:282-323): every pair of the current set that contains a
+// removed proxy is dropped and handed to the caller (the reference's handler), sorted.
+extern "C" int ncx_bp_remove_with_pairs(ncx_bp* bp, const uint32_t* handles, size_t n, uint64_t* removed_keys,
+                                        size_t cap, size_t* n_removed) {
   NCX_REQUIRE(bp && (handles || n == 0), "ncx_bp_remove: null argument");
   for (size_t i = 0; i < n; ++i)
     NCX_REQUIRE(handles[i] < bp->occupied.size() && bp->occupied[handles[i]],
                 "Attempting to remove an object that does not exist.");
+  std::vector<uint64_t> dropped;
+  if (bp->npairs) {
+    int rc;
+    if ((rc = bp_sort_current(bp, false))) return rc;
+    std::vector<uint64_t> cur(bp->npairs);
+    if ((rc = download_keys(bp, bp->pa, bp->pb, bp->npairs, cur.data()))) return rc;
+    std::vector<uint8_t> gone(bp->occupied.size(), 0);
+    for (size_t i = 0; i < n; ++i) gone[handles[i]] = 1;
+    std::vector<uint64_t> already = bp->removed_pending;
+    std::sort(already.begin(), already.end());
+    for (uint64_t k : cur) {
+      const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+      if ((gone[a] || gone[b]) && !std::binary_search(already.begin(), already.end(), k)) dropped.push_back(k);
+    }
+  }
+  if (n_removed) *n_removed = dropped.size();
+  if (removed_keys && cap < dropped.size()) {
+    ncx_set_error("ncx_bp_remove_with_pairs: capacity too small (nothing removed)");
+    return NCX_E_CAPACITY;
+  }
+  if (removed_keys) std::copy(dropped.begin(), dropped.end(), removed_keys);
+  bp->removed_pending.insert(bp->removed_pending.end(), dropped.begin(), dropped.end());
   ncx_ctx* ctx = bp->ctx;
   std::vector<uint32_t> zero(1, 0);
   for (size_t i = 0; i < n; ++i) {
@@ -374,6 +434,40 @@ extern "C" int ncx_bp_remove(ncx_bp* bp, const uint32_t* handles, size_t n) {
   }
   NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   bp->topology_dirty = true;
+  return NCX_OK;
+}
+
+extern "C" int ncx_bp_remove(ncx_bp* bp, const uint32_t* handles, size_t n) {
+  return ncx_bp_remove_with_pairs(bp, handles, n, nullptr, 0, nullptr);
+}
+
+// Per-proxy CollisionGroups (src/pipeline/object/collision_groups.rs): the pair filter of
+// BroadPhaseInterferenceHandler::is_interference_allowed (broad_phase.rs:31) as the world applies
+// it (can_interact_with_groups, collision_groups.rs:353-359).  Host arrays.
+extern "C" int ncx_bp_set_collision_groups(ncx_bp* bp, const uint32_t* handles, const uint32_t* membership,
+                                           const uint32_t* whitelist, const uint32_t* blacklist, size_t n) {
+  NCX_REQUIRE(bp && (n == 0 || (handles && membership && whitelist && blacklist)),
+              "ncx_bp_set_collision_groups: null argument");
+  for (size_t i = 0; i < n; ++i) {
+    NCX_REQUIRE(handles[i] < bp->occupied.size() && bp->occupied[handles[i]],
+                "ncx_bp_set_collision_groups: no such proxy");
+    NCX_REQUIRE(membership[i] <= NCX_ALL_GROUPS && blacklist[i] <= NCX_ALL_GROUPS,
+                "ncx_bp_set_collision_groups: there are at most 30 groups");
+  }
+  if (!bp->groups_on) {
+    bp->groups_on = true;
+    bp->groups_host.assign(3 * bp->occupied.size(), 0u);
+    for (size_t h = 0; h < bp->occupied.size(); ++h) {
+      bp->groups_host[3 * h] = NCX_ALL_GROUPS;
+      bp->groups_host[3 * h + 1] = NCX_ALL_GROUPS;
+    }
+  }
+  for (size_t i = 0; i < n; ++i) {
+    bp->groups_host[3 * handles[i]] = membership[i];
+    bp->groups_host[3 * handles[i] + 1] = whitelist[i];
+    bp->groups_host[3 * handles[i] + 2] = blacklist[i];
+  }
+  bp->groups_dirty = true;
   return NCX_OK;
 }
 
@@ -398,11 +492,12 @@ static int bp_query(ncx_bp* bp, uint32_t n) {
   if (!deep)
     hipLaunchKernelGGL(k_bp_query<false>, dim3(nb), dim3(128), 0, ctx->stream, nodes, order, bp->alive.as<uint32_t>(),
                        bp->boxes.as<float>(), n, bp->cnt.as<uint32_t>(), (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                       (uint32_t*)nullptr);
+                       (uint32_t*)nullptr, bp->groups_on ? bp->groups.as<uint32_t>() : (const uint32_t*)nullptr);
   else
     hipLaunchKernelGGL(k_bp_query_deep<false>, dim3(nb), dim3(128), 0, ctx->stream, nodes, order,
                        bp->alive.as<uint32_t>(), bp->boxes.as<float>(), n, bp->cnt.as<uint32_t>(),
-                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, gst.as<uint32_t>(), depth + 1);
+                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, gst.as<uint32_t>(), depth + 1,
+                       bp->groups_on ? bp->groups.as<uint32_t>() : (const uint32_t*)nullptr);
   NCX_HIP_CHECK(hipGetLastError());
   if ((rc = scan_exclusive_u32(ctx, bp->cnt.as<uint32_t>(), bp->offs.as<uint32_t>(), n, bp->sort_scan,
                                bp->total.as<uint32_t>())))
@@ -414,12 +509,12 @@ static int bp_query(ncx_bp* bp, uint32_t n) {
   if (!deep)
     hipLaunchKernelGGL(k_bp_query<true>, dim3(nb), dim3(128), 0, ctx->stream, nodes, order, bp->alive.as<uint32_t>(),
                        bp->boxes.as<float>(), n, (uint32_t*)nullptr, bp->offs.as<uint32_t>(), bp->pa.as<uint32_t>(),
-                       bp->pb.as<uint32_t>());
+                       bp->pb.as<uint32_t>(), bp->groups_on ? bp->groups.as<uint32_t>() : (const uint32_t*)nullptr);
   else
     hipLaunchKernelGGL(k_bp_query_deep<true>, dim3(nb), dim3(128), 0, ctx->stream, nodes, order,
                        bp->alive.as<uint32_t>(), bp->boxes.as<float>(), n, (uint32_t*)nullptr,
                        bp->offs.as<uint32_t>(), bp->pa.as<uint32_t>(), bp->pb.as<uint32_t>(), gst.as<uint32_t>(),
-                       depth + 1);
+                       depth + 1, bp->groups_on ? bp->groups.as<uint32_t>() : (const uint32_t*)nullptr);
   NCX_HIP_CHECK(hipGetLastError());
   bp->npairs = total;
   return NCX_OK;
@@ -437,6 +532,16 @@ extern "C" int ncx_bp_update(ncx_bp* bp) {
   bp->have_prev = true;
   bp->npairs = 0;
   bp->sorted = false;
+  bp->removed_prev.swap(bp->removed_pending);
+  bp->removed_pending.clear();
+  if (bp->groups_on && bp->groups_dirty) {
+    if (bp->groups_host.size() < 3 * bp->occupied.size()) bp->groups_host.resize(3 * bp->occupied.size(), 0u);
+    if ((rc = bp->groups.reserve(bp->groups_host.size() * 4 + 4))) return rc;
+    if (!bp->groups_host.empty())
+      NCX_HIP_CHECK(hipMemcpyAsync(bp->groups.p, bp->groups_host.data(), bp->groups_host.size() * 4,
+                                   hipMemcpyHostToDevice, ctx->stream));
+    bp->groups_dirty = false;
+  }
   if (bp->topology_dirty) {
     bp->alive_host.clear();
     for (uint32_t h = 0; h < bp->occupied.size(); ++h)
@@ -554,6 +659,12 @@ extern "C" int ncx_bp_events(ncx_bp* bp, uint64_t* started, size_t cap_s, size_t
   std::vector<uint64_t> cur(bp->npairs), prev(bp->prev_npairs);
   if ((rc = download_keys(bp, bp->pa, bp->pb, bp->npairs, cur.data()))) return rc;
   if ((rc = download_keys(bp, bp->prev_a, bp->prev_b, bp->prev_npairs, prev.data()))) return rc;
+  if (!bp->removed_prev.empty()) {  // reported by remove(), not again as stopped
+    std::vector<uint64_t> rm = bp->removed_prev, kept;
+    std::sort(rm.begin(), rm.end());
+    std::set_difference(prev.begin(), prev.end(), rm.begin(), rm.end(), std::back_inserter(kept));
+    prev.swap(kept);
+  }
   std::vector<uint64_t> s, t;
   std::set_difference(cur.begin(), cur.end(), prev.begin(), prev.end(), std::back_inserter(s));
   std::set_difference(prev.begin(), prev.end(), cur.begin(), cur.end(), std::back_inserter(t));
@@ -580,4 +691,57 @@ extern "C" int ncx_bp_proxy_aabb(ncx_bp* bp, uint32_t handle, float out[6]) {
   NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   NCX_REQUIRE(in_tree, "ncx_bp_proxy_aabb: proxy not yet inserted by an update");
   return NCX_OK;
+}
+
+// ------------------------------------------------------------------ interference queries
+// BroadPhase::interferences_with_bounding_volume / _with_point / _with_ray (dbvt_broad_phase.rs:
+// 387-432) over the tree of the last update, excluding proxies removed since: every proxy whose
+// stored box intersects the query box (aabb.rs:156-158), contains the point (aabb.rs:138-146), or
+// is hit by the ray within max_toi (AABB::intersects_ray, solid; ray_aabb.rs:13-50).  Results per
+// query are handles at offsets_dev[q] .. offsets_dev[q + 1] (offsets_dev: nq + 1 entries).
+static int bp_interferences(ncx_bp* bp, int kind, const float* const* q, int ncomp, size_t nq, float max_toi,
+                            uint32_t* offsets_dev, uint32_t* handles_dev, size_t cap, size_t* total) {
+  NCX_REQUIRE(bp && q && offsets_dev && total, "ncx_bp_interferences: null argument");
+  NCX_REQUIRE(nq < 0xFFFFFFFFull, "ncx_bp_interferences: too many queries");
+  for (int d = 0; d < ncomp; ++d) NCX_REQUIRE(q[d], "ncx_bp_interferences: null query array");
+  ncx_ctx* ctx = bp->ctx;
+  const size_t n = bp->alive_host.size();
+  if (n == 0 || bp->bvh.n != n || nq == 0) {
+    *total = 0;
+    int rc;
+    if (nq && (rc = fill_u32(ctx, offsets_dev, nq + 1, 0u))) return rc;
+    NCX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return NCX_OK;
+  }
+  NCX_REQUIRE(handles_dev || cap == 0, "ncx_bp_interferences: null output");
+  BQArgs a = {};
+  a.nodes = bp->bvh.nodes.as<LNode>();
+  a.order = bp->bvh.order.as<uint32_t>();
+  a.alive = bp->alive.as<uint32_t>();
+  a.in_tree = bp->in_tree.as<uint32_t>();
+  a.leaf_soa = bp->boxes.as<float>();
+  a.nleaves = (uint32_t)n;
+  for (int d = 0; d < ncomp; ++d) a.q[d] = q[d];
+  a.nq = (uint32_t)nq;
+  a.max_toi = max_toi;
+  a.handles = handles_dev;
+  return bq_collect_run(ctx, a, kind, offsets_dev, cap, total, bp->qcnt, bp->qscan, bp->qtot);
+}
+
+extern "C" int ncx_bp_interferences_with_aabbs(ncx_bp* bp, const float* const aabb_soa_dev[6], size_t nq,
+                                               uint32_t* offsets_dev, uint32_t* handles_dev, size_t cap,
+                                               size_t* total) {
+  return bp_interferences(bp, BQ_BOX, aabb_soa_dev, 6, nq, 0.f, offsets_dev, handles_dev, cap, total);
+}
+
+extern "C" int ncx_bp_interferences_with_points(ncx_bp* bp, const float* const point_soa_dev[3], size_t nq,
+                                                uint32_t* offsets_dev, uint32_t* handles_dev, size_t cap,
+                                                size_t* total) {
+  return bp_interferences(bp, BQ_POINT, point_soa_dev, 3, nq, 0.f, offsets_dev, handles_dev, cap, total);
+}
+
+extern "C" int ncx_bp_interferences_with_rays(ncx_bp* bp, const float* const ray_soa_dev[6], size_t nq,
+                                              float max_toi, uint32_t* offsets_dev, uint32_t* handles_dev,
+                                              size_t cap, size_t* total) {
+  return bp_interferences(bp, BQ_RAY, ray_soa_dev, 6, nq, max_toi, offsets_dev, handles_dev, cap, total);
 }

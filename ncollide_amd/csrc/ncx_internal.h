@@ -120,6 +120,7 @@ struct ncx_shapes {
   DevBuf shapes;      // ncx_shape[n]
   DevBuf hx, hy, hz;  // hull pool
   ContactTiers tiers; // for ncx_contact_batch / ncx_distance_batch
+  DevBuf qerr;        // error flag of the time-of-impact, ray-cast and composite queries
   // autotuner choice per query (0 contact, 1 distance) and bucket: an index into the kernel
   // variants (group width 8, 16, 32), or -1 before tuning
   int8_t tune[2][NCX_NBUCKETS] = {{-1, -1, -1, -1}, {-1, -1, -1, -1}};
@@ -260,7 +261,8 @@ int contact_tiers_reserve_list(ContactTiers& t, size_t npairs);
 int contacts_launch_device_count(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const ncx_pose* poses,
                                  const uint32_t* ia, const uint32_t* ib, const uint32_t* n_dev, size_t cap,
                                  float prediction, const float* init_dir, int max_iter, uint8_t* has, float* w1,
-                                 float* w2, float* nrm, float* depth, int32_t* iters, bool may_tune);
+                                 float* w2, float* nrm, float* depth, int32_t* iters, bool may_tune,
+                                 float* fdir = nullptr);
 // Host side after that launch completed: checks, re-runs what both LDS tiers could not hold,
 // and reports the number of pairs that needed more than the first-tier arena.
 int contacts_resolve_overflow(ContactTiers& T, ncx_ctx* ctx, size_t* n_large);

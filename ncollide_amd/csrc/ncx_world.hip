@@ -13,7 +13,7 @@
 //      init_dir on polytope pairs.
 // Steps 1-5 contain no host synchronisation, so a refit step and a rebuild step are each captured
 // once in a hipGraph and replayed (the first step, which tunes and allocates, runs eagerly).
-#include "ncx_internal.h"
+#include "ncx_bvh_query.h"
 
 #define WQ_BLOCK 128
 
@@ -58,7 +58,83 @@ struct ncx_world {
   int last_was_graph = 0;
   bool contacts = true;        // run the narrow phase in each step
   bool last_had_contacts = false;
+  DevBuf qcnt, qscan, qtot;    // interference queries: per-query counts, scan scratch, total
+  // the polyhedron manifold generator's last_gjk_dir kept per pair across steps
+  bool temporal = false;
+  bool synced = true;          // ncx_world_sync has run since the last step
+  uint64_t tw_mask = 0;        // hash table slots - 1 (a power of two, at least twice max_pairs)
+  DevBuf tw_keys, tw_dirs;     // pair key (UINT64_MAX = empty) and direction (float4) per slot
+  DevBuf tw_init, tw_fdir;     // per pair: the looked-up init_dir, the step's result direction
 };
+
+// ------------------------------------------------------------------ last_gjk_dir per pair
+// ConvexPolyhedronConvexPolyhedronManifoldGenerator keeps, per pair of polyhedra (Cuboid or
+// ConvexHull on both sides), the direction of its last GJK/EPA result and starts the next step's
+// GJK from it (convex_polyhedron_convex_polyhedron_manifold_generator.rs:24-33, 96-145); the
+// generator lives while the broad phase keeps the pair.  Here the directions live in an
+// open-addressing table keyed by the pair key: at the start of a step the previous step's pairs
+// and result directions are inserted (after the last step's contacts, including those finished
+// by the host-driven tier at sync), then each pair of the new step looks its direction up (none
+// when the pair is new).
+#define TW_EMPTY 0xFFFFFFFFFFFFFFFFull
+__device__ __forceinline__ uint64_t tw_hash(uint64_t k) {  // a 64-bit mix (MurmurHash3 finaliser)
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+__device__ __forceinline__ bool tw_poly(const ncx_shape* __restrict__ sh, uint32_t a, uint32_t b) {
+  const uint32_t ta = sh[a].type, tb = sh[b].type;
+  return (ta == NCX_CUBOID || ta == NCX_CONVEX_HULL) && (tb == NCX_CUBOID || tb == NCX_CONVEX_HULL);
+}
+
+__global__ void k_tw_insert(const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+                            const uint32_t* __restrict__ n_dev, size_t cap, const ncx_shape* __restrict__ sh,
+                            const float* __restrict__ fdir, unsigned long long* __restrict__ keys,
+                            float4* __restrict__ dirs, uint64_t mask) {
+  const size_t n = min(cap, (size_t)*n_dev);
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t a = pa[k], b = pb[k];
+    const float dx = fdir[3 * k], dy = fdir[3 * k + 1], dz = fdir[3 * k + 2];
+    if (!tw_poly(sh, a, b) || (dx == 0.f && dy == 0.f && dz == 0.f)) continue;
+    const unsigned long long key = ((unsigned long long)a << 32) | b;
+    uint64_t h = tw_hash(key) & mask;
+    for (uint64_t probe = 0; probe <= mask; ++probe) {
+      const unsigned long long prev = atomicCAS(&keys[h], TW_EMPTY, key);
+      if (prev == TW_EMPTY || prev == key) {
+        dirs[h] = make_float4(dx, dy, dz, 0.f);
+        break;
+      }
+      h = (h + 1) & mask;
+    }
+  }
+}
+
+__global__ void k_tw_lookup(const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+                            const uint32_t* __restrict__ n_dev, size_t cap, const ncx_shape* __restrict__ sh,
+                            const unsigned long long* __restrict__ keys, const float4* __restrict__ dirs,
+                            uint64_t mask, float* __restrict__ init) {
+  const size_t n = min(cap, (size_t)*n_dev);
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t a = pa[k], b = pb[k];
+    float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (tw_poly(sh, a, b)) {
+      const unsigned long long key = ((unsigned long long)a << 32) | b;
+      uint64_t h = tw_hash(key) & mask;
+      for (uint64_t probe = 0; probe <= mask; ++probe) {
+        const unsigned long long kk = keys[h];
+        if (kk == key) { d = dirs[h]; break; }
+        if (kk == TW_EMPTY) break;
+        h = (h + 1) & mask;
+      }
+    }
+    init[3 * k] = d.x;
+    init[3 * k + 1] = d.y;
+    init[3 * k + 2] = d.z;
+  }
+}
 
 // The stored boxes are kept twice: SoA (the interface's layout, read coalesced here) and as two
 // float4 per proxy, which the refit and the pair query gather by sorted leaf: one cache line
@@ -370,6 +446,15 @@ static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses, bool ea
   size_t n = w->n;
   int rc;
   uint32_t* ctr = w->counters.as<uint32_t>();
+  const unsigned tw_blocks = std::max(1u, std::min<unsigned>(ceil_div(w->cap, 256), (unsigned)c->num_cus * 8));
+  if (w->temporal && !w->first) {
+    // the previous step's pairs (still in pa / pb, counted in ctr[0]) and result directions
+    if ((rc = fill_u32(c, w->tw_keys.as<uint32_t>(), 2 * (w->tw_mask + 1), 0xFFFFFFFFu))) return rc;
+    hipLaunchKernelGGL(k_tw_insert, dim3(tw_blocks), dim3(256), 0, c->stream, w->pa.as<uint32_t>(),
+                       w->pb.as<uint32_t>(), ctr, w->cap, w->shapes->shapes.as<ncx_shape>(), w->tw_fdir.as<float>(),
+                       w->tw_keys.as<unsigned long long>(), w->tw_dirs.as<float4>(), w->tw_mask);
+    NCX_HIP_CHECK(hipGetLastError());
+  }
   if ((rc = fill_u32(c, ctr, 2, 0u))) return rc;
   if (n >= 2) {
     unsigned nb = world_query_blocks(c, n);
@@ -398,6 +483,22 @@ static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses, bool ea
   }
   if (!w->contacts) return NCX_OK;
   const float* idir = nullptr;
+  if (w->temporal) {
+    if (w->first) {
+      if ((rc = fill_u32(c, w->tw_init.as<uint32_t>(), 3 * w->cap, 0u))) return rc;
+    } else {
+      hipLaunchKernelGGL(k_tw_lookup, dim3(tw_blocks), dim3(256), 0, c->stream, w->pa.as<uint32_t>(),
+                         w->pb.as<uint32_t>(), ctr, w->cap, w->shapes->shapes.as<ncx_shape>(),
+                         w->tw_keys.as<unsigned long long>(), w->tw_dirs.as<float4>(), w->tw_mask,
+                         w->tw_init.as<float>());
+      NCX_HIP_CHECK(hipGetLastError());
+    }
+    return contacts_launch_device_count(w->shapes, w->tiers, c, poses, w->pa.as<uint32_t>(), w->pb.as<uint32_t>(),
+                                        ctr, w->cap, w->prediction, w->tw_init.as<float>(), 10000,
+                                        w->has.as<uint8_t>(), w->w1.as<float>(), w->w2.as<float>(),
+                                        w->nrm.as<float>(), w->depth_out.as<float>(), w->iters.as<int32_t>(), eager,
+                                        w->tw_fdir.as<float>());
+  }
   if (w->mlp) {
     if ((rc = mlp_predict_device_count(w->mlp, c, w->shapes->shapes.as<ncx_shape>(), poses, w->desc.as<float>(),
                                        w->pa.as<uint32_t>(), w->pb.as<uint32_t>(), ctr, w->cap, w->dirs.as<float>())))
@@ -451,6 +552,7 @@ extern "C" int ncx_world_create(ncx_ctx* ctx, ncx_shapes* shapes, float margin, 
 
 extern "C" int ncx_world_set_warmstart(ncx_world* w, ncx_mlp* mlp) {
   NCX_REQUIRE(w, "ncx_world_set_warmstart: null world");
+  NCX_REQUIRE(!(mlp && w->temporal), "ncx_world_set_warmstart: the temporal warm start is on");
   w->mlp = mlp;
   w->graph_valid = false;
   w->rb_valid = false;
@@ -519,7 +621,8 @@ extern "C" int ncx_world_destroy(ncx_world* w) {
   if (w->ev_in) (void)hipEventDestroy(w->ev_in);
   if (w->ev_out) (void)hipEventDestroy(w->ev_out);
   DevBuf* bufs[] = {&w->box_new, &w->stored, &w->sbox, &w->gstack, &w->qwork, &w->lbuf, &w->cnt, &w->offs, &w->counters, &w->pa, &w->pb,
-                    &w->has, &w->w1, &w->w2, &w->nrm, &w->depth_out, &w->iters, &w->desc, &w->dirs};
+                    &w->has, &w->w1, &w->w2, &w->nrm, &w->depth_out, &w->iters, &w->desc, &w->dirs,
+                    &w->qcnt, &w->qscan, &w->qtot, &w->tw_keys, &w->tw_dirs, &w->tw_init, &w->tw_fdir};
   for (DevBuf* b : bufs) b->release();
   w->bvh.release();
   contact_tiers_release(w->tiers);
@@ -548,8 +651,18 @@ static int capture_step(hipStream_t s, hipGraph_t* graph, hipGraphExec_t* exec, 
 // `poses_dev` changes.  Neither depends on the tree's topology, so a rebuild does not invalidate
 // the refit graph.  The step is ordered after prior work on the caller's stream and later caller
 // work waits for it; ncx_world_sync waits for completion.
+extern "C" int ncx_world_sync(ncx_world* w, size_t* npairs, size_t* n_epa_overflow);
+
 extern "C" int ncx_world_step(ncx_world* w, const ncx_pose* poses_dev, int use_graph) {
   NCX_REQUIRE(w && poses_dev, "ncx_world_step: null argument");
+  if (w->temporal && !w->first && !w->synced) {
+    // every result direction of the last step (including pairs the host-driven EPA tier finishes
+    // at sync) must be in place before this step reads them
+    size_t np = 0;
+    int r = ncx_world_sync(w, &np, nullptr);
+    if (r) return r;
+  }
+  w->synced = false;
   hipStream_t s = w->wctx.stream;
   NCX_HIP_CHECK(hipEventRecord(w->ev_in, w->ctx->stream));
   NCX_HIP_CHECK(hipStreamWaitEvent(s, w->ev_in, 0));
@@ -620,6 +733,7 @@ extern "C" int ncx_world_sync(ncx_world* w, size_t* npairs, size_t* n_epa_overfl
     if (rc) return rc;
   }
   if (n_epa_overflow) *n_epa_overflow = large;
+  w->synced = true;
   return NCX_OK;
 }
 
@@ -638,8 +752,126 @@ extern "C" int ncx_world_outputs(ncx_world* w, uint32_t** ia, uint32_t** ib, uin
 }
 
 // The stored (fat) box of every proxy after the last step, device SoA (6 x n floats).
+extern "C" int ncx_world_set_temporal_warmstart(ncx_world* w, int enable) {
+  NCX_REQUIRE(w, "ncx_world_set_temporal_warmstart: null argument");
+  NCX_REQUIRE(!(enable && w->mlp), "ncx_world_set_temporal_warmstart: the learned warm start is set");
+  if (enable && !w->temporal) {
+    uint64_t slots = 1;
+    while (slots < 2 * (uint64_t)w->cap) slots <<= 1;
+    int rc;
+    if ((rc = w->tw_keys.reserve(slots * 8)) || (rc = w->tw_dirs.reserve(slots * 16)) ||
+        (rc = w->tw_init.reserve(w->cap * 12 + 16)) || (rc = w->tw_fdir.reserve(w->cap * 12 + 16)))
+      return rc;
+    w->tw_mask = slots - 1;
+    // no pair has a direction yet: the first temporal step inserts nothing
+    if ((rc = fill_u32(&w->wctx, w->tw_fdir.as<uint32_t>(), 3 * w->cap, 0u))) return rc;
+    NCX_HIP_CHECK(hipStreamSynchronize(w->wctx.stream));
+  }
+  if ((enable != 0) != w->temporal) {
+    w->temporal = enable != 0;
+    w->graph_valid = false;  // the captured steps hold the other narrow phase
+    w->rb_valid = false;
+  }
+  return NCX_OK;
+}
+
+extern "C" int ncx_world_temporal_dirs(ncx_world* w, const float** init_dev, const float** result_dev) {
+  NCX_REQUIRE(w && init_dev && result_dev, "ncx_world_temporal_dirs: null argument");
+  NCX_REQUIRE(w->temporal, "ncx_world_temporal_dirs: the temporal warm start is off");
+  *init_dev = w->tw_init.as<float>();
+  *result_dev = w->tw_fdir.as<float>();
+  return NCX_OK;
+}
+
+extern "C" int ncx_world_warmstart_dirs(ncx_world* w, const float** dirs_dev) {
+  NCX_REQUIRE(w && dirs_dev, "ncx_world_warmstart_dirs: null argument");
+  NCX_REQUIRE(w->mlp, "ncx_world_warmstart_dirs: no learned warm start set");
+  *dirs_dev = w->dirs.as<float>();
+  return NCX_OK;
+}
+
 extern "C" int ncx_world_stored_aabbs(ncx_world* w, const float** soa) {
   NCX_REQUIRE(w && soa, "ncx_world_stored_aabbs: null argument");
   *soa = w->stored.as<float>();
+  return NCX_OK;
+}
+
+// ------------------------------------------------------------------ interference queries of a scene
+// On the caller's stream, after the last step (which that stream waits for): the tree and the
+// stored boxes are those of the last step, the shapes are at poses_dev.
+static BQArgs world_bq_args(ncx_world* w, const ncx_pose* poses) {
+  BQArgs a = {};
+  a.nodes = w->bvh.nodes.as<LNode>();
+  a.order = w->bvh.order.as<uint32_t>();
+  a.alive = nullptr;
+  a.in_tree = nullptr;
+  a.leaf_soa = w->stored.as<float>();
+  a.nleaves = (uint32_t)w->n;
+  a.shapes = w->shapes->shapes.as<ncx_shape>();
+  a.P = Pool{w->shapes->hx.as<float>(), w->shapes->hy.as<float>(), w->shapes->hz.as<float>()};
+  a.poses = poses;
+  return a;
+}
+
+extern "C" int ncx_world_interferences_with_rays(ncx_world* w, const ncx_pose* poses_dev,
+                                                 const float* const ray_soa_dev[6], size_t n, float max_toi,
+                                                 int aabb_only, uint32_t* offsets_dev, uint32_t* handles_dev,
+                                                 float* toi_dev, float* normal_dev, size_t cap, size_t* total) {
+  NCX_REQUIRE(w && ray_soa_dev && offsets_dev && handles_dev && total && (aabb_only || poses_dev),
+              "ncx_world_interferences_with_rays: null argument");
+  NCX_REQUIRE(!w->first, "ncx_world_interferences_with_rays: no step has run");
+  NCX_REQUIRE(n < 0xFFFFFFFFull, "ncx_world_interferences_with_rays: too many rays");
+  for (int d = 0; d < 6; ++d) NCX_REQUIRE(ray_soa_dev[d], "ncx_world_interferences_with_rays: null ray array");
+  BQArgs a = world_bq_args(w, poses_dev);
+  for (int d = 0; d < 6; ++d) a.q[d] = ray_soa_dev[d];
+  a.nq = (uint32_t)n;
+  a.max_toi = max_toi;
+  a.cast_shapes = aabb_only ? 0 : 1;
+  a.handles = handles_dev;
+  a.toi = toi_dev;
+  a.nrm = normal_dev;
+  return bq_collect_run(w->ctx, a, BQ_RAY, offsets_dev, cap, total, w->qcnt, w->qscan, w->qtot);
+}
+
+extern "C" int ncx_world_interferences_with_aabbs(ncx_world* w, const float* const aabb_soa_dev[6], size_t n,
+                                                  uint32_t* offsets_dev, uint32_t* handles_dev, size_t cap,
+                                                  size_t* total) {
+  NCX_REQUIRE(w && aabb_soa_dev && offsets_dev && handles_dev && total,
+              "ncx_world_interferences_with_aabbs: null argument");
+  NCX_REQUIRE(!w->first, "ncx_world_interferences_with_aabbs: no step has run");
+  NCX_REQUIRE(n < 0xFFFFFFFFull, "ncx_world_interferences_with_aabbs: too many boxes");
+  for (int d = 0; d < 6; ++d) NCX_REQUIRE(aabb_soa_dev[d], "ncx_world_interferences_with_aabbs: null box array");
+  BQArgs a = world_bq_args(w, nullptr);
+  for (int d = 0; d < 6; ++d) a.q[d] = aabb_soa_dev[d];
+  a.nq = (uint32_t)n;
+  a.handles = handles_dev;
+  return bq_collect_run(w->ctx, a, BQ_BOX, offsets_dev, cap, total, w->qcnt, w->qscan, w->qtot);
+}
+
+extern "C" int ncx_world_first_interference_with_rays(ncx_world* w, const ncx_pose* poses_dev,
+                                                      const float* const ray_soa_dev[6], size_t n, float max_toi,
+                                                      uint32_t* handle_dev, float* toi_dev, float* normal_dev) {
+  NCX_REQUIRE(w && poses_dev && ray_soa_dev && handle_dev && toi_dev && normal_dev,
+              "ncx_world_first_interference_with_rays: null argument");
+  NCX_REQUIRE(!w->first, "ncx_world_first_interference_with_rays: no step has run");
+  NCX_REQUIRE(n < 0xFFFFFFFFull, "ncx_world_first_interference_with_rays: too many rays");
+  for (int d = 0; d < 6; ++d) NCX_REQUIRE(ray_soa_dev[d], "ncx_world_first_interference_with_rays: null ray array");
+  if (n == 0) return NCX_OK;
+  BQArgs a = world_bq_args(w, poses_dev);
+  for (int d = 0; d < 6; ++d) a.q[d] = ray_soa_dev[d];
+  a.nq = (uint32_t)n;
+  a.max_toi = max_toi;
+  a.handles = handle_dev;
+  a.toi = toi_dev;
+  a.nrm = normal_dev;
+  int rc;
+  if ((rc = w->qtot.reserve(16)) || (rc = fill_u32(w->ctx, w->qtot.as<uint32_t>() + 1, 1, 0u))) return rc;
+  a.err = w->qtot.as<uint32_t>() + 1;
+  hipLaunchKernelGGL(k_bq_first_ray, dim3(ceil_div(n, BQ_BLOCK)), dim3(BQ_BLOCK), 0, w->ctx->stream, a);
+  NCX_HIP_CHECK(hipGetLastError());
+  uint32_t err = 0;
+  NCX_HIP_CHECK(hipMemcpyAsync(&err, w->qtot.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, w->ctx->stream));
+  NCX_HIP_CHECK(hipStreamSynchronize(w->ctx->stream));
+  NCX_REQUIRE(err == 0, "ncx_world_first_interference_with_rays: traversal stack overflow");
   return NCX_OK;
 }

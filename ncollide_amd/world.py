@@ -67,6 +67,34 @@ class World:
         out["epa_overflow"] = ovf
         return out
 
+    def set_temporal_warmstart(self, enable=True):
+        """The polyhedron manifold generator's last_gjk_dir kept per pair across steps (the
+        reference's own warm start of polytope pairs)."""
+        _lib.check(self.ctx.lib.ncx_world_set_temporal_warmstart(self.h, 1 if enable else 0))
+
+    def temporal_dirs(self):
+        """(init_dir, result direction) per pair of the last step under the temporal warm start."""
+        n, _ = self.sync()
+        pi, pr = C.c_void_p(), C.c_void_p()
+        _lib.check(self.ctx.lib.ncx_world_temporal_dirs(self.h, C.byref(pi), C.byref(pr)))
+        out = []
+        for p in (pi, pr):
+            t = torch.empty((max(n, 1), 3), dtype=torch.float32, device=self.ctx.device)
+            if n:
+                _memcpy_d2d(t.data_ptr(), p.value, 12 * n)
+            out.append(t[:n])
+        return out[0], out[1]
+
+    def warmstart_dirs(self):
+        """The learned init_dir of each pair of the last step ((npairs, 3) CUDA tensor, zero rows = None)."""
+        n, _ = self.sync()
+        p = C.c_void_p()
+        _lib.check(self.ctx.lib.ncx_world_warmstart_dirs(self.h, C.byref(p)))
+        t = torch.empty((max(n, 1), 3), dtype=torch.float32, device=self.ctx.device)
+        if n:
+            _memcpy_d2d(t.data_ptr(), p.value, 12 * n)
+        return t[:n]
+
     def stored_aabbs(self):
         p = C.c_void_p()
         _lib.check(self.ctx.lib.ncx_world_stored_aabbs(self.h, C.byref(p)))
@@ -74,6 +102,69 @@ class World:
         t = torch.empty((6, n), dtype=torch.float32, device=self.ctx.device)
         _memcpy_d2d(t.data_ptr(), p.value, 24 * n)
         return t
+
+    # -- world-level queries through the broad phase (after a step; handles are shape indices)
+    def _rays(self, origins, dirs):
+        return _lib.soa_columns(torch.cat([torch.as_tensor(origins, dtype=torch.float32).reshape(-1, 3).cpu(),
+                                           torch.as_tensor(dirs, dtype=torch.float32).reshape(-1, 3).cpu()], 1),
+                                6, self.ctx.device)
+
+    def first_interference_with_rays(self, poses, origins, dirs, max_toi):
+        """CollisionWorld::first_interference_with_ray per ray (solid casts): (handle, toi, normal)
+        CUDA tensors; handle -1 (0xFFFFFFFF) and toi -1 on a miss.  poses: the last step's poses."""
+        cols = self._rays(origins, dirs)
+        n = cols[0].numel()
+        dev = self.ctx.device
+        h = torch.empty(n, dtype=torch.int32, device=dev)
+        t = torch.empty(n, dtype=torch.float32, device=dev)
+        nr = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        _lib.check(self.ctx.lib.ncx_world_first_interference_with_rays(self.h, _lib.ptr(poses), _lib.ptrs(cols), n,
+                                                                       float(max_toi), _lib.ptr(h), _lib.ptr(t),
+                                                                       _lib.ptr(nr)))
+        return h, t, nr
+
+    def interferences_with_rays(self, poses, origins, dirs, max_toi, aabb_only=False):
+        """CollisionWorld::interferences_with_ray per ray: CSR (offsets (n+1,), handles, toi, normal)."""
+        cols = self._rays(origins, dirs)
+        n = cols[0].numel()
+        dev = self.ctx.device
+        offs = torch.empty(n + 1, dtype=torch.int32, device=dev)
+        total = C.c_size_t()
+        cap = 0
+        while True:
+            hs = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+            ts = torch.empty(max(cap, 1), dtype=torch.float32, device=dev)
+            ns = torch.empty((max(cap, 1), 3), dtype=torch.float32, device=dev)
+            rc = self.ctx.lib.ncx_world_interferences_with_rays(self.h, _lib.ptr(poses), _lib.ptrs(cols), n,
+                                                                float(max_toi), 1 if aabb_only else 0, _lib.ptr(offs),
+                                                                _lib.ptr(hs), _lib.ptr(ts), _lib.ptr(ns), cap,
+                                                                C.byref(total))
+            if rc == -3 and total.value > cap:
+                cap = total.value
+                continue
+            _lib.check(rc)
+            break
+        k = total.value
+        return offs, hs[:k], ts[:k], ns[:k]
+
+    def interferences_with_aabbs(self, aabbs):
+        """CollisionWorld::interferences_with_aabb per box of (n, 6): CSR (offsets, handles)."""
+        cols = _lib.soa_columns(aabbs, 6, self.ctx.device)
+        n = cols[0].numel()
+        dev = self.ctx.device
+        offs = torch.empty(n + 1, dtype=torch.int32, device=dev)
+        total = C.c_size_t()
+        cap = 0
+        while True:
+            hs = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+            rc = self.ctx.lib.ncx_world_interferences_with_aabbs(self.h, _lib.ptrs(cols), n, _lib.ptr(offs),
+                                                                 _lib.ptr(hs), cap, C.byref(total))
+            if rc == -3 and total.value > cap:
+                cap = total.value
+                continue
+            _lib.check(rc)
+            break
+        return offs, hs[:total.value]
 
     def __del__(self):
         try:

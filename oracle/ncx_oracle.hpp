@@ -1493,4 +1493,463 @@ template <class R> struct DBVTBroadPhase {
   }
 };
 
+
+// ================================================================ time of impact and ray casts on shapes
+// approx::relative_eq!(x, 0) with the default epsilon and max_relative (both EPSILON of the type):
+// |x - 0| <= EPSILON, and the relative test |x| <= |x| * EPSILON never holds for finite x != 0.
+template <class R> inline bool relative_eq_zero(R x) { return std::fabs(x) <= Eps<R>::eps(); }
+
+// SupportMap::support_point_toward (shape/support_map.rs:31-34): `u` is unit and used as given;
+// Ball overrides it (ball.rs:36-39), Capsule's local_support_point_toward does not normalise
+// (capsule.rs:79-83), Cuboid and ConvexHull fall back to support_point.
+template <class R> inline V3<R> support_point_toward(const Shape<R>& g, const Iso<R>& m, V3<R> u) {
+  switch (g.type) {
+    case BALL: return m.t + u * g.a;
+    case CAPSULE: {
+      V3<R> ld = inv_xform_vec(m, u);
+      V3<R> res(R(0), std::copysign(g.a, ld.y), R(0));
+      return xform_pt(m, res + ld * g.b);
+    }
+    default: return support_point(g, m, u);
+  }
+}
+
+// A support map, dilated by `radius` when `dilated`: DilatedShape (special_support_maps.rs:32-59),
+// whose support_point normalises the direction and adds the ball's support to the shape's.
+template <class R> struct SMap {
+  const Shape<R>* g;
+  R radius;
+  bool dilated;
+};
+template <class R> inline SMap<R> smap(const Shape<R>& g) { return SMap<R>{&g, R(0), false}; }
+template <class R> inline V3<R> smap_support(const SMap<R>& s, const Iso<R>& m, V3<R> dir) {
+  if (!s.dilated) return support_point(*s.g, m, dir);
+  V3<R> u = normalize(dir);
+  return support_point_toward(*s.g, m, u) + u * s.radius;
+}
+template <class R> inline CSOPoint<R> cso_from_smaps(const Iso<R>& m1, const SMap<R>& g1, const Iso<R>& m2,
+                                                     const SMap<R>& g2, V3<R> dir) {
+  CSOPoint<R> c;
+  c.orig1 = smap_support(g1, m1, dir);
+  c.orig2 = smap_support(g2, m2, -dir);
+  c.point = c.orig1 - c.orig2;
+  return c;
+}
+template <class R> inline CSOPoint<R> cso_translated(const CSOPoint<R>& c, V3<R> d) {  // cso_point.rs:88-90
+  CSOPoint<R> r = c;
+  r.point = c.point + d;
+  return r;
+}
+
+// gjk::minkowski_ray_cast (gjk.rs:245-365) on m1*g1 - m2*g2; false is None.  *iters counts the
+// support evaluations.  The improved_fixed_point_support feature is off (the crate's default).
+template <class R> inline bool minkowski_ray_cast(const Iso<R>& m1, const SMap<R>& g1, const Iso<R>& m2,
+                                                  const SMap<R>& g2, const Ray<R>& ray, R max_toi,
+                                                  VoronoiSimplex<R>& simplex, R* toi, V3<R>* normal,
+                                                  int* iters = nullptr) {
+  const R eps_tol = gjk_eps_tol<R>();
+  const R eps_rel = std::sqrt(eps_tol);
+  const R maxv = Eps<R>::maxv();
+  int it = 0;
+  if (iters) *iters = 0;
+  R ray_length = norm(ray.dir);
+  if (relative_eq_zero(ray_length)) return false;
+  R ltoi = R(0);
+  V3<R> co = ray.origin, cd = ray.dir / ray_length;
+  V3<R> dir = -cd;
+  V3<R> ldir = dir;
+  CSOPoint<R> sp = cso_from_smaps(m1, g1, m2, g2, dir);
+  it++;
+  simplex.reset(cso_translated(sp, -co));
+  V3<R> proj = simplex.project_origin_and_reduce();
+  R max_bound = maxv;
+  int niter = 0;
+  bool last_chance = false;
+  auto hit = [&]() {
+    *toi = ltoi / ray_length;
+    *normal = ldir;
+    if (iters) *iters = it;
+    return true;
+  };
+  auto miss = [&]() {
+    if (iters) *iters = it;
+    return false;
+  };
+  for (;;) {
+    R old_max_bound = max_bound;
+    {
+      V3<R> nd;
+      R dist;
+      if (try_normalize(-proj, eps_tol, &nd, &dist)) { dir = nd; max_bound = dist; }
+      else return hit();
+    }
+    if (max_bound >= old_max_bound) {
+      last_chance = true;  // CSOPoint::single_point (cso_point.rs:42-44)
+      V3<R> p = proj + co;
+      sp.point = p; sp.orig1 = p; sp.orig2 = V3<R>(0, 0, 0);
+    } else {
+      sp = cso_from_smaps(m1, g1, m2, g2, dir);
+      it++;
+    }
+    if (last_chance && ltoi > R(0)) return hit();
+    // query::ray_toi_with_plane(support point, dir, curr_ray) (ray_plane.rs:8-37)
+    R denom = dot(dir, cd);
+    bool has_t = false;
+    R t = R(0);
+    if (!relative_eq_zero(denom)) {
+      t = dot(dir, sp.point - co) / denom;
+      has_t = t >= R(0);
+    }
+    if (has_t) {
+      if (dot(dir, cd) < R(0) && t > R(0)) {
+        ldir = dir;
+        ltoi += t;
+        if (ltoi / ray_length > max_toi) return miss();
+        V3<R> shift = cd * t;
+        co = co + shift;
+        max_bound = maxv;
+        for (int i = 0; i < simplex.dim + 1; ++i) simplex.vertices[i].point = simplex.vertices[i].point + (-shift);
+        last_chance = false;
+      }
+    } else if (dot(dir, cd) > eps_tol) {
+      return miss();
+    }
+    if (last_chance) return miss();
+    R min_bound = -dot(dir, sp.point - co);
+    if (max_bound - min_bound <= eps_rel * max_bound) return miss();
+    (void)simplex.add_point(cso_translated(sp, -co));
+    proj = simplex.project_origin_and_reduce();
+    if (simplex.dim == 3) {
+      if (min_bound >= eps_tol) return miss();
+      return hit();
+    }
+    niter += 1;
+    if (niter == 10000) return miss();
+  }
+}
+
+// TOI status (time_of_impact.rs:7-25).
+enum TOIStatus { TOI_OUT_OF_ITERATIONS = 0, TOI_CONVERGED = 1, TOI_FAILED = 2, TOI_PENETRATING = 3 };
+template <class R> struct TOIResult {
+  R toi;
+  V3<R> witness1, witness2, normal1, normal2;  // local frames of shape 1 and 2
+  int status;
+};
+
+// ray_toi_with_ball (ray_ball.rs:83-124); *inside reports the origin inside the ball.
+template <class R> inline bool ray_toi_with_ball(V3<R> center, R radius, const Ray<R>& ray, bool solid, bool* inside,
+                                                 R* toi) {
+  V3<R> dcenter = ray.origin - center;
+  R a = norm_sq(ray.dir);
+  R b = dot(dcenter, ray.dir);
+  R c = norm_sq(dcenter) - radius * radius;
+  *inside = false;
+  if (a == R(0)) {
+    if (c > R(0)) return false;
+    *inside = true;
+    *toi = R(0);
+    return true;
+  }
+  if (c > R(0) && b > R(0)) return false;
+  R delta = b * b - a * c;
+  if (delta < R(0)) return false;
+  R t = (-b - std::sqrt(delta)) / a;
+  if (t <= R(0)) {
+    *inside = true;
+    *toi = solid ? R(0) : (-b + std::sqrt(delta)) / a;
+    return true;
+  }
+  *toi = t;
+  return true;
+}
+
+// time_of_impact_ball_ball (time_of_impact_ball_ball.rs:7-61) followed by the frame change of
+// query::time_of_impact (time_of_impact.rs:75-98).
+template <class R> inline bool toi_ball_ball(const Iso<R>& m1, V3<R> vel1, R r1, const Iso<R>& m2, V3<R> vel2, R r2,
+                                             R max_toi, R target_distance, TOIResult<R>* out) {
+  V3<R> vel = vel1 - vel2;
+  R rsum = r1 + r2;
+  R radius = rsum + target_distance;
+  V3<R> center = m1.t + (-m2.t);
+  Ray<R> ray{V3<R>(0, 0, 0), -vel};
+  bool inside;
+  R toi;
+  if (!ray_toi_with_ball(center, radius, ray, true, &inside, &toi)) return false;
+  if (toi > max_toi) return false;
+  V3<R> dpt = (ray.origin + ray.dir * toi) - center;
+  V3<R> n1, n2, w1, w2;
+  if (radius == R(0)) {
+    n1 = V3<R>(1, 0, 0); n2 = V3<R>(1, 0, 0); w1 = V3<R>(0, 0, 0); w2 = V3<R>(0, 0, 0);
+  } else {
+    n1 = dpt / radius;
+    n2 = -n1;
+    w1 = n1 * r1;
+    w2 = n2 * r2;
+  }
+  out->toi = toi;
+  out->status = (inside && norm_sq(center) < rsum * rsum) ? TOI_PENETRATING : TOI_CONVERGED;
+  out->witness1 = qrot_inv(m1.q, w1);
+  out->witness2 = qrot_inv(m2.q, w2);
+  out->normal1 = qrot_inv(m1.q, n1);
+  out->normal2 = qrot_inv(m2.q, n2);
+  return true;
+}
+
+// time_of_impact_support_map_support_map (time_of_impact_support_map_support_map.rs:10-116)
+// through gjk::directional_distance (gjk.rs:214-242).
+template <class R> inline bool toi_support_map_support_map(const Iso<R>& m1, V3<R> vel1, const Shape<R>& g1,
+                                                           const Iso<R>& m2, V3<R> vel2, const Shape<R>& g2,
+                                                           R max_toi, R target_distance, TOIResult<R>* out,
+                                                           int* iters = nullptr) {
+  V3<R> dvel = vel2 - vel1;
+  SMap<R> s1 = smap(g1), s2 = smap(g2);
+  if (target_distance != R(0)) { s1.radius = target_distance; s1.dilated = true; }
+  VoronoiSimplex<R> simplex;
+  Ray<R> ray{V3<R>(0, 0, 0), dvel};
+  R toi;
+  V3<R> normal;
+  if (!minkowski_ray_cast(m1, s1, m2, s2, ray, Eps<R>::maxv(), simplex, &toi, &normal, iters)) return false;
+  V3<R> w1(0, 0, 0), w2(0, 0, 0);
+  if (toi != R(0)) gjk_result(simplex, simplex.dim == 3, &w1, &w2);
+  if (toi > max_toi) return false;
+  out->toi = toi;
+  if (target_distance == R(0)) {
+    out->normal1 = inv_xform_vec(m1, normal);
+    out->normal2 = inv_xform_vec(m2, -normal);
+    out->witness1 = inv_xform_pt(m1, w1);
+    out->witness2 = inv_xform_pt(m2, w2);
+    out->status = toi == R(0) ? TOI_PENETRATING : TOI_CONVERGED;
+    return true;
+  }
+  int status = TOI_CONVERGED;
+  if (toi == R(0)) {
+    // closest_points_support_map_support_map_with_params(.., target_distance, Some(normal))
+    // (closest_points_support_map_support_map.rs:40-67)
+    VoronoiSimplex<R> s2x;
+    V3<R> d;
+    if (try_normalize(normal, Eps<R>::eps(), &d)) s2x.reset(cso_from_shapes(m1, g1, m2, g2, d));
+    else s2x.reset(cso_from_shapes(m1, g1, m2, g2, V3<R>(1, 0, 0)));
+    GJKResult<R> r = gjk_closest_points(m1, g1, m2, g2, target_distance, true, s2x);
+    if (r.status == GJK_CLOSEST) {
+      V3<R> nn = normalize(r.p2 - r.p1);
+      out->normal1 = inv_xform_vec(m1, nn);
+      out->normal2 = inv_xform_vec(m2, -nn);
+      out->witness1 = inv_xform_pt(m1, w1);
+      out->witness2 = inv_xform_pt(m2, w2);
+      out->status = TOI_CONVERGED;
+      return true;
+    } else if (r.status == GJK_INTERSECTION) {
+      status = TOI_PENETRATING;
+    }
+  }
+  V3<R> n1 = inv_xform_vec(m1, normal);
+  out->normal1 = n1;
+  out->normal2 = inv_xform_vec(m2, -normal);
+  out->witness1 = inv_xform_pt(m1, w1) - n1 * target_distance;
+  out->witness2 = inv_xform_pt(m2, w2);
+  out->status = status;
+  return true;
+}
+
+// query::time_of_impact (time_of_impact.rs:66-158) for {Ball, Cuboid, Capsule, ConvexHull}:
+// Ball-Ball closed form, every other pair through the support maps.
+template <class R> inline bool time_of_impact(const Iso<R>& m1, V3<R> vel1, const Shape<R>& g1, const Iso<R>& m2,
+                                              V3<R> vel2, const Shape<R>& g2, R max_toi, R target_distance,
+                                              TOIResult<R>* out, int* iters = nullptr) {
+  if (iters) *iters = 0;
+  if (g1.type == BALL && g2.type == BALL)
+    return toi_ball_ball(m1, vel1, g1.a, m2, vel2, g2.a, max_toi, target_distance, out);
+  return toi_support_map_support_map(m1, vel1, g1, m2, vel2, g2, max_toi, target_distance, out, iters);
+}
+
+// RayIntersection of a shape: toi, world normal, feature (FEAT_FACE0 + face id; FEAT_UNKNOWN for
+// FeatureId::Unknown).
+constexpr uint32_t FEAT_UNKNOWN = 0xFFFFFFFEu;
+template <class R> struct RayHit {
+  R toi;
+  V3<R> normal;
+  uint32_t feature;
+};
+
+// AABB ray cast with the entry/exit normals: clip_line (ray_aabb.rs:178-268) and ray_aabb
+// (:270-290), local frame.  *side: +(i+1) / -(i+1) as in the reference.
+template <class R> inline bool ray_aabb_normal(const AABB<R>& bb, const Ray<R>& ray, R max_toi, bool solid, R* toi,
+                                               V3<R>* normal, int* side) {
+  R tmax = Eps<R>::maxv();
+  R tmin = -tmax;
+  int near_side = 0, far_side = 0;
+  bool near_diag = false, far_diag = false;
+  for (int i = 0; i < 3; ++i) {
+    if (ray.dir[i] == R(0)) {
+      if (ray.origin[i] < bb.mins[i] || ray.origin[i] > bb.maxs[i]) return false;
+    } else {
+      R denom = R(1) / ray.dir[i];
+      bool flip;
+      R tn = (bb.mins[i] - ray.origin[i]) * denom;
+      R tf = (bb.maxs[i] - ray.origin[i]) * denom;
+      if (tn > tf) { flip = true; std::swap(tn, tf); } else flip = false;
+      if (tn > tmin) { tmin = tn; near_side = flip ? -(i + 1) : i + 1; near_diag = false; }
+      else if (tn == tmin) near_diag = true;
+      if (tf < tmax) { tmax = tf; far_side = !flip ? -(i + 1) : i + 1; far_diag = false; }
+      else if (tf == tmax) far_diag = true;
+      if (tmax < R(0) || tmin > tmax) return false;
+    }
+  }
+  V3<R> nn(0, 0, 0), nf(0, 0, 0);
+  if (near_diag) nn = -normalize(ray.dir);
+  else if (near_side < 0) nn[-near_side - 1] = R(1);
+  else nn[near_side - 1] = R(-1);
+  if (far_diag) nf = -normalize(ray.dir);
+  else if (far_side < 0) nf[-far_side - 1] = R(-1);
+  else nf[far_side - 1] = R(1);
+  if (tmin < R(0)) {
+    if (solid) { *toi = R(0); *normal = V3<R>(0, 0, 0); *side = far_side; return true; }
+    if (tmax <= max_toi) { *toi = tmax; *normal = nf; *side = far_side; return true; }
+    return false;
+  }
+  if (tmin <= max_toi) { *toi = tmin; *normal = nn; *side = near_side; return true; }
+  return false;
+}
+
+// ray_intersection_with_support_map_with_params (ray_support_map.rs:15-62), shape at identity.
+template <class R> inline bool ray_support_map(const Shape<R>& g, const Ray<R>& ray, R max_toi, bool solid, R* toi,
+                                               V3<R>* normal) {
+  Iso<R> id;
+  id.t = V3<R>(0, 0, 0);
+  id.q = Quat<R>{R(0), R(0), R(0), R(1)};
+  Shape<R> origin;
+  origin.type = ORIGIN; origin.a = origin.b = origin.c = R(0); origin.pts = nullptr; origin.npts = 0;
+  VoronoiSimplex<R> simplex;  // the reset with the -dir support point is overwritten by cast_ray
+  R t;
+  V3<R> n;
+  if (!minkowski_ray_cast(id, smap(g), id, smap(origin), ray, max_toi, simplex, &t, &n)) return false;
+  if (solid || t != R(0)) { *toi = t; *normal = n; return true; }
+  // the ray starts inside the shape: cast back from a point past the far side
+  V3<R> ndir = normalize(ray.dir);
+  V3<R> supp = support_point(g, id, ndir);
+  R eps = R(0.001);
+  R shift = dot(supp - ray.origin, ndir) + eps;
+  Ray<R> nr{ray.origin + ndir * shift, -ray.dir};
+  VoronoiSimplex<R> s2;
+  if (!minkowski_ray_cast(id, smap(g), id, smap(origin), nr, shift + eps, s2, &t, &n)) return false;
+  R tt = shift - t;
+  if (!(tt <= max_toi)) return false;
+  *toi = tt;
+  *normal = n;
+  return true;
+}
+
+// RayCast::toi_and_normal_with_ray of Ball (ray_ball.rs:32-43, 127-142), Cuboid (ray_cuboid.rs:17-26
+// -> ray_aabb.rs:45-61), Capsule and ConvexHull (ray_support_map.rs:117-163: local-frame ray,
+// normal back to world).
+template <class R> inline bool shape_ray_cast(const Shape<R>& g, const Iso<R>& m, const Ray<R>& ray, R max_toi,
+                                              bool solid, RayHit<R>* out) {
+  switch (g.type) {
+    case BALL: {
+      bool inside;
+      R t;
+      if (!ray_toi_with_ball(m.t, g.a, ray, solid, &inside, &t)) return false;
+      V3<R> pos = (ray.origin + ray.dir * t) - m.t;
+      V3<R> n = normalize(pos);
+      if (!(t <= max_toi)) return false;
+      out->toi = t;
+      out->normal = inside ? -n : n;
+      out->feature = 0;
+      return true;
+    }
+    case CUBOID: {
+      Ray<R> ls{inv_xform_pt(m, ray.origin), inv_xform_vec(m, ray.dir)};
+      AABB<R> bb{-V3<R>(g.a, g.b, g.c), V3<R>(g.a, g.b, g.c)};
+      R t;
+      V3<R> n;
+      int side;
+      if (!ray_aabb_normal(bb, ls, max_toi, solid, &t, &n, &side)) return false;
+      out->toi = t;
+      out->normal = xform_vec(m, n);
+      out->feature = side < 0 ? (uint32_t)(-side - 1 + 3) : (uint32_t)(side - 1);
+      return true;
+    }
+    case CAPSULE:
+    case HULL: {
+      Ray<R> ls{inv_xform_pt(m, ray.origin), inv_xform_vec(m, ray.dir)};
+      R t;
+      V3<R> n;
+      if (!ray_support_map(g, ls, max_toi, solid, &t, &n)) return false;
+      out->toi = t;
+      out->normal = xform_vec(m, n);
+      out->feature = FEAT_UNKNOWN;
+      return true;
+    }
+    default:
+      return false;
+  }
+}
+
+
+// ================================================================ world ray queries (brute force)
+// first_interference_with_ray (glue/query.rs:200-224, dbvt_broad_phase.rs:433-460) over proxies
+// with stored boxes `boxes`: the best-first search returns the least-toi shape hit among the
+// proxies whose box the ray enters (AABB::toi_with_ray, solid) no later than the best so far; that
+// is the least toi over every proxy whose box the ray enters within max_toi.  Ties: least handle.
+template <class R> inline bool world_first_ray(const std::vector<Shape<R>>& shapes, const std::vector<Iso<R>>& poses,
+                                               const AABB<R>* boxes, const Ray<R>& ray, R max_toi, uint32_t* handle,
+                                               RayHit<R>* hit) {
+  bool found = false;
+  for (size_t h = 0; h < shapes.size(); ++h) {
+    R bt;
+    if (!ray_aabb_toi(boxes[h], ray, max_toi, true, &bt)) continue;
+    RayHit<R> r;
+    if (!shape_ray_cast(shapes[h], poses[h], ray, max_toi, true, &r)) continue;
+    if (!found || r.toi < hit->toi) {  // ascending handles: the first of equal tois is the least handle
+      *hit = r;
+      *handle = (uint32_t)h;
+      found = true;
+    }
+  }
+  return found;
+}
+
+
+// ================================================================ persistent narrow phase
+// ConvexPolyhedronConvexPolyhedronManifoldGenerator::generate_contacts, its GJK part
+// (pipeline/narrow_phase/contact_generator/convex_polyhedron_convex_polyhedron_manifold_generator.rs:
+// 96-145): contact_support_map_support_map_with_params (contact_support_map_support_map.rs:38-76)
+// from last_gjk_dir (None when `last_dir` is null), then last_gjk_dir <- the result's direction:
+// ClosestPoints(.., dir) (EPA's normal for a penetrating pair) and NoIntersection(dir) (GJK's
+// separating direction; +x when EPA fails or GJK stops at its iteration cap) set it; nothing else
+// changes it (*dir_set = false).  The contact is Contact::new_wo_depth of ClosestPoints.
+template <class R> inline bool polyhedra_gjk_step(const Iso<R>& m1, const Shape<R>& g1, const Iso<R>& m2,
+                                                  const Shape<R>& g2, R prediction, const V3<R>* last_dir,
+                                                  Contact<R>* out, V3<R>* dir_out, bool* dir_set) {
+  V3<R> dir;
+  if (last_dir) dir = *last_dir;
+  else if (!try_normalize(m2.t - m1.t, Eps<R>::eps(), &dir)) dir = V3<R>(1, 0, 0);
+  VoronoiSimplex<R> simplex;
+  simplex.reset(cso_from_shapes(m1, g1, m2, g2, dir));
+  GJKResult<R> r = gjk_closest_points(m1, g1, m2, g2, prediction, true, simplex);
+  *dir_set = false;
+  V3<R> w1, w2, n;
+  if (r.status == GJK_CLOSEST) {
+    w1 = r.p1; w2 = r.p2; n = r.dir;
+  } else if (r.status == GJK_INTERSECTION) {
+    EPA<R> epa;
+    if (!epa.closest_points(m1, g1, m2, g2, simplex, &w1, &w2, &n)) {
+      *dir_out = V3<R>(1, 0, 0);
+      *dir_set = true;
+      return false;
+    }
+  } else {
+    if (r.status == GJK_NO_INTERSECTION) {
+      *dir_out = r.dir;
+      *dir_set = true;
+    }
+    return false;
+  }
+  *dir_out = n;
+  *dir_set = true;
+  out->world1 = w1; out->world2 = w2; out->normal = n;
+  out->depth = -dot(n, w2 - w1);
+  return true;
+}
+
 }  // namespace ncxo

@@ -8,6 +8,7 @@
 //            hull_off[n], hull_cnt[n] index into hull_pts[3 * total] (xyz AoS)
 //   poses  : 7 per shape, (tx, ty, tz, qi, qj, qk, qw)
 //   pairs  : ia[np], ib[np] shape indices
+#include <memory>
 #include <thread>
 #include <vector>
 #include <algorithm>
@@ -218,6 +219,145 @@ template <class R> inline AABB<R> aabb_from6(const R* b) {
 
 }  // namespace
 
+
+template <class R>
+int toi_batch(int64_t ns, const uint32_t* types, const R* params, const uint32_t* hoff, const uint32_t* hcnt,
+              const R* hpts, int64_t npts_total, const R* poses, const R* vel, const uint32_t* ia, const uint32_t* ib,
+              int64_t np, R max_toi, R target, uint8_t* has, R* toi, R* w1, R* w2, R* n1, R* n2, uint8_t* status,
+              int32_t* iters, int nthreads) {
+  ShapeSet<R> ss;
+  ss.load(ns, types, params, hoff, hcnt, hpts, npts_total);
+  parallel_for(np, nthreads, [&](int64_t k) {
+    uint32_t a = ia[k], b = ib[k];
+    TOIResult<R> r;
+    int it = 0;
+    V3<R> v1(vel[3 * a], vel[3 * a + 1], vel[3 * a + 2]), v2(vel[3 * b], vel[3 * b + 1], vel[3 * b + 2]);
+    bool ok = time_of_impact(pose_at(poses, a), v1, ss.shapes[a], pose_at(poses, b), v2, ss.shapes[b], max_toi, target,
+                             &r, &it);
+    has[k] = ok ? 1 : 0;
+    if (iters) iters[k] = it;
+    if (!ok) { r.toi = 0; r.witness1 = r.witness2 = r.normal1 = r.normal2 = V3<R>(0, 0, 0); r.status = 0; }
+    toi[k] = r.toi;
+    status[k] = (uint8_t)r.status;
+    for (int d = 0; d < 3; ++d) {
+      w1[3 * k + d] = r.witness1[d]; w2[3 * k + d] = r.witness2[d];
+      n1[3 * k + d] = r.normal1[d]; n2[3 * k + d] = r.normal2[d];
+    }
+  });
+  return 0;
+}
+
+template <class R>
+int shape_rays(int64_t ns, const uint32_t* types, const R* params, const uint32_t* hoff, const uint32_t* hcnt,
+               const R* hpts, int64_t npts_total, const R* poses, const uint32_t* idx, const R* orig, const R* dir,
+               int64_t n, R max_toi, int solid, R* toi, R* nrm, uint32_t* feat, int nthreads) {
+  ShapeSet<R> ss;
+  ss.load(ns, types, params, hoff, hcnt, hpts, npts_total);
+  parallel_for(n, nthreads, [&](int64_t k) {
+    Ray<R> ray{V3<R>(orig[3 * k], orig[3 * k + 1], orig[3 * k + 2]), V3<R>(dir[3 * k], dir[3 * k + 1], dir[3 * k + 2])};
+    RayHit<R> h;
+    uint32_t i = idx[k];
+    if (shape_ray_cast(ss.shapes[i], pose_at(poses, i), ray, max_toi, solid != 0, &h)) {
+      toi[k] = h.toi;
+      for (int d = 0; d < 3; ++d) nrm[3 * k + d] = h.normal[d];
+      feat[k] = h.feature;
+    } else {
+      toi[k] = R(-1);
+      for (int d = 0; d < 3; ++d) nrm[3 * k + d] = R(0);
+      feat[k] = 0xFFFFFFFFu;
+    }
+  });
+  return 0;
+}
+
+
+template <class R>
+int64_t world_rays(int64_t ns, const uint32_t* types, const R* params, const uint32_t* hoff, const uint32_t* hcnt,
+                   const R* hpts, int64_t npts_total, const R* poses, const R* boxes6, const R* orig, const R* dir,
+                   int64_t nr, R max_toi, int mode, uint32_t* handles, R* toi, R* nrm, int64_t* counts, int64_t cap,
+                   int nthreads) {
+  ShapeSet<R> ss;
+  ss.load(ns, types, params, hoff, hcnt, hpts, npts_total);
+  std::vector<Iso<R>> ps((size_t)ns);
+  std::vector<AABB<R>> bb((size_t)ns);
+  for (int64_t i = 0; i < ns; ++i) {
+    ps[i] = pose_at(poses, i);
+    bb[i] = aabb_from6(boxes6 + 6 * i);
+  }
+  auto ray_at = [&](int64_t k) {
+    return Ray<R>{V3<R>(orig[3 * k], orig[3 * k + 1], orig[3 * k + 2]), V3<R>(dir[3 * k], dir[3 * k + 1], dir[3 * k + 2])};
+  };
+  if (mode == 0) {  // first interference
+    parallel_for(nr, nthreads, [&](int64_t k) {
+      uint32_t h = 0xFFFFFFFFu;
+      RayHit<R> r;
+      if (world_first_ray(ss.shapes, ps, bb.data(), ray_at(k), max_toi, &h, &r)) {
+        handles[k] = h; toi[k] = r.toi;
+        for (int d = 0; d < 3; ++d) nrm[3 * k + d] = r.normal[d];
+      } else {
+        handles[k] = 0xFFFFFFFFu; toi[k] = R(-1);
+        for (int d = 0; d < 3; ++d) nrm[3 * k + d] = R(0);
+      }
+    });
+    return nr;
+  }
+  // mode 1: interferences_with_ray with shape casts; mode 2: stored boxes only.  Handles ascending.
+  std::vector<std::vector<uint32_t>> hh((size_t)nr);
+  std::vector<std::vector<RayHit<R>>> rr((size_t)nr);
+  parallel_for(nr, nthreads, [&](int64_t k) {
+    Ray<R> ray = ray_at(k);
+    for (int64_t h = 0; h < ns; ++h) {
+      R bt;
+      if (!ray_aabb_toi(bb[h], ray, max_toi, true, &bt)) continue;
+      RayHit<R> r{R(0), V3<R>(0, 0, 0), 0};
+      if (mode == 1 && !shape_ray_cast(ss.shapes[h], ps[h], ray, max_toi, true, &r)) continue;
+      hh[k].push_back((uint32_t)h);
+      rr[k].push_back(r);
+    }
+  });
+  int64_t total = 0;
+  for (int64_t k = 0; k < nr; ++k) { counts[k] = (int64_t)hh[k].size(); total += counts[k]; }
+  if (total <= cap) {
+    int64_t o = 0;
+    for (int64_t k = 0; k < nr; ++k)
+      for (size_t j = 0; j < hh[k].size(); ++j, ++o) {
+        handles[o] = hh[k][j]; toi[o] = rr[k][j].toi;
+        for (int d = 0; d < 3; ++d) nrm[3 * o + d] = rr[k][j].normal[d];
+      }
+  }
+  return total;
+}
+
+
+template <class R>
+int polyhedra_step_batch(int64_t ns, const uint32_t* types, const R* params, const uint32_t* hoff, const uint32_t* hcnt,
+                         const R* hpts, int64_t npts_total, const R* poses, const uint32_t* ia, const uint32_t* ib,
+                         int64_t np, R prediction, const R* last_dirs, uint8_t* has, R* w1, R* w2, R* nrm, R* depth,
+                         R* dir_out, uint8_t* dir_set, int nthreads) {
+  ShapeSet<R> ss;
+  ss.load(ns, types, params, hoff, hcnt, hpts, npts_total);
+  parallel_for(np, nthreads, [&](int64_t k) {
+    uint32_t a = ia[k], b = ib[k];
+    V3<R> ld(last_dirs[3 * k], last_dirs[3 * k + 1], last_dirs[3 * k + 2]);
+    const bool some = ld.x != R(0) || ld.y != R(0) || ld.z != R(0);
+    Contact<R> c;
+    V3<R> nd(0, 0, 0);
+    bool set = false;
+    bool ok = polyhedra_gjk_step(pose_at(poses, a), ss.shapes[a], pose_at(poses, b), ss.shapes[b], prediction,
+                                 some ? &ld : nullptr, &c, &nd, &set);
+    has[k] = ok ? 1 : 0;
+    dir_set[k] = set ? 1 : 0;
+    for (int d = 0; d < 3; ++d) {
+      w1[3 * k + d] = ok ? c.world1[d] : R(0);
+      w2[3 * k + d] = ok ? c.world2[d] : R(0);
+      nrm[3 * k + d] = ok ? c.normal[d] : R(0);
+      dir_out[3 * k + d] = nd[d];
+    }
+    depth[k] = ok ? c.depth : R(0);
+  });
+  return 0;
+}
+
 extern "C" {
 
 int orc_contact_f32(int64_t ns, const uint32_t* types, const float* params, const uint32_t* hoff, const uint32_t* hcnt,
@@ -355,6 +495,27 @@ void orc_bp_remove(void* h, const uint32_t* hs, int64_t n) {
   ((DBVTBroadPhase<float>*)h)->remove(v);
 }
 void orc_bp_update(void* h) { ((DBVTBroadPhase<float>*)h)->update(); }
+// CollisionGroups per handle for the pair filter: can_interact_with_groups
+// (pipeline/object/collision_groups.rs:353-359); handles never set keep CollisionGroups::new().
+void orc_bp_set_groups(void* h, uint32_t handle, uint32_t membership, uint32_t whitelist, uint32_t blacklist) {
+  auto* bp = (DBVTBroadPhase<float>*)h;
+  static thread_local std::vector<std::pair<void*, std::shared_ptr<std::vector<uint32_t>>>> tables;
+  std::shared_ptr<std::vector<uint32_t>> t;
+  for (auto& e : tables) if (e.first == h) t = e.second;
+  if (!t) {
+    t = std::make_shared<std::vector<uint32_t>>();
+    tables.push_back({h, t});
+    bp->allowed = [t](uint32_t a, uint32_t b) {
+      const uint32_t all = (1u << 30) - 1u;
+      auto get = [&](uint32_t x, int k) { return (size_t)3 * x + k < t->size() ? (*t)[3 * x + k] : (k == 2 ? 0u : all); };
+      uint32_t ma = get(a, 0), wa = get(a, 1), ba = get(a, 2), mb = get(b, 0), wb = get(b, 1), bb = get(b, 2);
+      return (ma & bb) == 0 && (mb & ba) == 0 && (ma & wb) != 0 && (mb & wa) != 0;
+    };
+  }
+  const uint32_t all = (1u << 30) - 1u;
+  while (t->size() < (size_t)3 * (handle + 1)) { t->push_back(all); t->push_back(all); t->push_back(0u); }
+  (*t)[3 * handle] = membership; (*t)[3 * handle + 1] = whitelist; (*t)[3 * handle + 2] = blacklist;
+}
 int64_t orc_bp_num_pairs(void* h) { return (int64_t)((DBVTBroadPhase<float>*)h)->pairs.size(); }
 // Sorted (a << 32 | b) keys with a < b.
 int64_t orc_bp_pairs(void* h, uint64_t* out, int64_t cap) {
@@ -399,6 +560,41 @@ int64_t orc_pairs_brute(const float* bb6, int64_t n, uint64_t* out, int64_t cap)
     }
   }
   return cnt;
+}
+
+#define ORC_TOI(T, R)                                                                                             \
+  int orc_toi_##T(int64_t ns, const uint32_t* types, const R* params, const uint32_t* hoff, const uint32_t* hcnt,  \
+                  const R* hpts, int64_t npts, const R* poses, const R* vel, const uint32_t* ia, const uint32_t* ib, \
+                  int64_t np, R max_toi, R target, uint8_t* has, R* toi, R* w1, R* w2, R* n1, R* n2,               \
+                  uint8_t* status, int32_t* iters, int nthreads) {                                                 \
+    return toi_batch<R>(ns, types, params, hoff, hcnt, hpts, npts, poses, vel, ia, ib, np, max_toi, target, has,   \
+                        toi, w1, w2, n1, n2, status, iters, nthreads);                                             \
+  }                                                                                                              \
+  int orc_shape_rays_##T(int64_t ns, const uint32_t* types, const R* params, const uint32_t* hoff,                \
+                         const uint32_t* hcnt, const R* hpts, int64_t npts, const R* poses, const uint32_t* idx,   \
+                         const R* orig, const R* dir, int64_t n, R max_toi, int solid, R* toi, R* nrm,              \
+                         uint32_t* feat, int nthreads) {                                                           \
+    return shape_rays<R>(ns, types, params, hoff, hcnt, hpts, npts, poses, idx, orig, dir, n, max_toi, solid, toi, \
+                         nrm, feat, nthreads);                                                                     \
+  }
+ORC_TOI(f32, float)
+ORC_TOI(f64, double)
+
+int64_t orc_world_rays_f32(int64_t ns, const uint32_t* types, const float* params, const uint32_t* hoff,
+                           const uint32_t* hcnt, const float* hpts, int64_t npts, const float* poses, const float* boxes6,
+                           const float* orig, const float* dir, int64_t nr, float max_toi, int mode, uint32_t* handles,
+                           float* toi, float* nrm, int64_t* counts, int64_t cap, int nthreads) {
+  return world_rays<float>(ns, types, params, hoff, hcnt, hpts, npts, poses, boxes6, orig, dir, nr, max_toi, mode,
+                           handles, toi, nrm, counts, cap, nthreads);
+}
+
+int orc_polyhedra_step_f32(int64_t ns, const uint32_t* types, const float* params, const uint32_t* hoff,
+                           const uint32_t* hcnt, const float* hpts, int64_t npts, const float* poses, const uint32_t* ia,
+                           const uint32_t* ib, int64_t np, float prediction, const float* last_dirs, uint8_t* has,
+                           float* w1, float* w2, float* nrm, float* depth, float* dir_out, uint8_t* dir_set,
+                           int nthreads) {
+  return polyhedra_step_batch<float>(ns, types, params, hoff, hcnt, hpts, npts, poses, ia, ib, np, prediction,
+                                     last_dirs, has, w1, w2, nrm, depth, dir_out, dir_set, nthreads);
 }
 
 }  // extern "C"

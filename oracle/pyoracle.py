@@ -53,6 +53,13 @@ def _declare(L):
         sig(f"orc_trimesh_free_{t}", None, P)
         sig(f"orc_trimesh_cast_{t}", C.c_int, P, P, P, P, I64, R, P, P, P, P, C.c_int)
         sig(f"orc_trimesh_brute_{t}", C.c_int, P, P, I64, P, P, P, I64, R, P, P, P, P, C.c_int)
+        sig(f"orc_toi_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P, P, P, I64, R, R, P, P, P, P, P, P, P, P,
+            C.c_int)
+        sig(f"orc_shape_rays_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P, P, P, I64, R, C.c_int, P, P, P, C.c_int)
+    sig("orc_world_rays_f32", I64, I64, P, P, P, P, P, I64, P, P, P, P, I64, C.c_float, C.c_int, P, P, P, P, I64,
+        C.c_int)
+    sig("orc_polyhedra_step_f32", C.c_int, I64, P, P, P, P, P, I64, P, P, P, I64, C.c_float, P, P, P, P, P, P, P,
+        P, C.c_int)
     sig("orc_ray_triangle_f64", C.c_int, P, P, P, P, P, P)
     sig("orc_ray_aabb_f64", C.c_int, P, P, P, C.c_double, C.c_int, P)
     sig("orc_project_f64", C.c_int, C.c_int, P, P, P, P, P)
@@ -65,6 +72,7 @@ def _declare(L):
     sig("orc_bp_set_bv", None, P, C.c_uint32, P)
     sig("orc_bp_remove", None, P, P, I64)
     sig("orc_bp_update", None, P)
+    sig("orc_bp_set_groups", None, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32)
     sig("orc_bp_num_pairs", I64, P)
     sig("orc_bp_pairs", I64, P, P, I64)
     sig("orc_bp_events", I64, P, C.c_int, P, I64)
@@ -272,6 +280,9 @@ class BroadPhase:
     def update(self):
         lib().orc_bp_update(self.h)
 
+    def set_collision_groups(self, handle, membership, whitelist, blacklist):
+        lib().orc_bp_set_groups(self.h, int(handle), int(membership), int(whitelist), int(blacklist))
+
     def num_interferences(self):
         return int(lib().orc_bp_num_pairs(self.h))
 
@@ -303,3 +314,85 @@ def pairs_brute(bb6):
     out = np.zeros(max(n, 1), np.uint64)
     lib().orc_pairs_brute(_p(bb), bb.shape[0], _p(out), n)
     return out[:n]
+
+
+def time_of_impact(ss, poses, vel, ia, ib, max_toi, target_distance, nthreads=1):
+    """query::time_of_impact per pair; velocities per shape (n, 3)."""
+    L = lib()
+    R = ss.R
+    poses = np.ascontiguousarray(poses, dtype=R).reshape(-1, 7)
+    vel = np.ascontiguousarray(vel, dtype=R).reshape(-1, 3)
+    ia = np.ascontiguousarray(ia, dtype=np.uint32)
+    ib = np.ascontiguousarray(ib, dtype=np.uint32)
+    n = len(ia)
+    has = np.zeros(n, np.uint8)
+    toi = np.zeros(n, R)
+    w1, w2, n1, n2 = (np.zeros((n, 3), R) for _ in range(4))
+    status = np.zeros(n, np.uint8)
+    it = np.zeros(n, np.int32)
+    getattr(L, f"orc_toi_{ss.tag}")(*ss.args(), _p(poses), _p(vel), _p(ia), _p(ib), n, max_toi, target_distance,
+                                     _p(has), _p(toi), _p(w1), _p(w2), _p(n1), _p(n2), _p(status), _p(it), nthreads)
+    return dict(has=has.astype(bool), toi=toi, witness1=w1, witness2=w2, normal1=n1, normal2=n2, status=status,
+                iters=it)
+
+
+def shape_rays(ss, poses, idx, origins, dirs, max_toi, solid, nthreads=1):
+    """RayCast::toi_and_normal_with_ray of shape idx[k] against ray k; miss: toi -1, feature UINT32_MAX."""
+    L = lib()
+    R = ss.R
+    poses = np.ascontiguousarray(poses, dtype=R).reshape(-1, 7)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    o = np.ascontiguousarray(origins, dtype=R).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, dtype=R).reshape(-1, 3)
+    n = len(idx)
+    toi = np.zeros(n, R)
+    nrm = np.zeros((n, 3), R)
+    feat = np.zeros(n, np.uint32)
+    getattr(L, f"orc_shape_rays_{ss.tag}")(*ss.args(), _p(poses), _p(idx), _p(o), _p(d), n, max_toi, int(solid),
+                                            _p(toi), _p(nrm), _p(feat), nthreads)
+    return toi, nrm, feat
+
+
+def world_rays(ss, poses, boxes6, origins, dirs, max_toi, mode, nthreads=8):
+    """Brute-force world ray queries over proxies with stored boxes (n, 6) (f32 only).
+    mode 0: first_interference_with_ray -> (handle, toi, normal) per ray;
+    mode 1: interferences_with_ray with solid shape casts, mode 2: stored boxes only ->
+    (counts per ray, handles, toi, normal), handles ascending per ray."""
+    L = lib()
+    assert ss.R == np.float32
+    poses = np.ascontiguousarray(poses, np.float32).reshape(-1, 7)
+    boxes6 = np.ascontiguousarray(boxes6, np.float32).reshape(-1, 6)
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    nr = len(o)
+    counts = np.zeros(nr, np.int64)
+    if mode == 0:
+        h = np.zeros(nr, np.uint32); t = np.zeros(nr, np.float32); n = np.zeros((nr, 3), np.float32)
+        L.orc_world_rays_f32(*ss.args(), _p(poses), _p(boxes6), _p(o), _p(d), nr, max_toi, 0, _p(h), _p(t), _p(n),
+                             _p(counts), nr, nthreads)
+        return h, t, n
+    total = L.orc_world_rays_f32(*ss.args(), _p(poses), _p(boxes6), _p(o), _p(d), nr, max_toi, mode, None, None,
+                                 None, _p(counts), 0, nthreads)
+    h = np.zeros(max(total, 1), np.uint32); t = np.zeros(max(total, 1), np.float32)
+    n = np.zeros((max(total, 1), 3), np.float32)
+    L.orc_world_rays_f32(*ss.args(), _p(poses), _p(boxes6), _p(o), _p(d), nr, max_toi, mode, _p(h), _p(t), _p(n),
+                         _p(counts), total, nthreads)
+    return counts, h[:total], t[:total], n[:total]
+
+
+def polyhedra_gjk_step(ss, poses, ia, ib, prediction, last_dirs, nthreads=8):
+    """ConvexPolyhedronConvexPolyhedronManifoldGenerator's GJK part per pair (f32): contact from
+    last_gjk_dir (zero rows = None) and the new last_gjk_dir (dir_set False: unchanged)."""
+    L = lib()
+    assert ss.R == np.float32
+    poses = np.ascontiguousarray(poses, np.float32).reshape(-1, 7)
+    ia = np.ascontiguousarray(ia, np.uint32)
+    ib = np.ascontiguousarray(ib, np.uint32)
+    n = len(ia)
+    ld = np.ascontiguousarray(last_dirs, np.float32).reshape(n, 3)
+    has = np.zeros(n, np.uint8); dset = np.zeros(n, np.uint8)
+    w1, w2, nr, nd = (np.zeros((n, 3), np.float32) for _ in range(4))
+    depth = np.zeros(n, np.float32)
+    L.orc_polyhedra_step_f32(*ss.args(), _p(poses), _p(ia), _p(ib), n, prediction, _p(ld), _p(has), _p(w1), _p(w2),
+                             _p(nr), _p(depth), _p(nd), _p(dset), nthreads)
+    return dict(has=has.astype(bool), world1=w1, world2=w2, normal=nr, depth=depth, dir=nd, dir_set=dset.astype(bool))

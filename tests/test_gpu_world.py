@@ -114,3 +114,56 @@ def test_world_refit_matches_rebuild_at_4m_shapes(ctx):
         assert np.array_equal(ka, kb), f"pairs differ at step {step}"
         pa[:, 0:3] += vel
         pb[:, 0:3] += vel
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_world_temporal_warm_start_matches_generator(ctx, orc, use_graph):
+    """The persistent narrow phase's warm start over moving steps: every pair of polyhedra starts
+    GJK from its last_gjk_dir (None when the pair is new to the broad phase), as
+    ConvexPolyhedronConvexPolyhedronManifoldGenerator does, and keeps its result direction for the
+    next step.  The oracle's generator step is driven with a per-pair dict kept across steps; the
+    scene's contacts, init directions and result directions must equal it bit for bit."""
+    w = workloads.full_pipeline(seed=71, n=5000, box=32.0, nclusters=3)
+    rec = w["records"]
+    ss = ncx.ShapeSet(records=rec, pool=w["pool"], ctx=ctx)
+    world = World(ss, margin=0.05, prediction=0.0, max_pairs=1 << 20, rebuild_every=2)
+    world.set_temporal_warmstart(True)
+    p_np = w["poses"].copy()
+    poses = torch.as_tensor(p_np, device=ctx.device)
+    oss = oracle_shapes(orc, rec, w["pool"])
+    vel = w["velocities"] * 0.15
+    types = rec["type"]
+    last = {}  # pair key -> last_gjk_dir (the generators alive after the previous step)
+    reused = 0
+    for step in range(6):
+        world.step(poses, use_graph=use_graph)
+        r = world.results()
+        init, fdir = (t.cpu().numpy() for t in world.temporal_dirs())
+        ia, ib = r["ia"].cpu().numpy().astype(np.uint32), r["ib"].cpu().numpy().astype(np.uint32)
+        keys = (ia.astype(np.uint64) << np.uint64(32)) | ib.astype(np.uint64)
+        poly = np.isin(types[ia], (1, 3)) & np.isin(types[ib], (1, 3))
+        want_init = np.zeros((len(ia), 3), np.float32)
+        for k in np.nonzero(poly)[0]:
+            d = last.get(int(keys[k]))
+            if d is not None:
+                want_init[k] = d
+        assert np.array_equal(init[poly].view(np.uint32), want_init[poly].view(np.uint32)), f"step {step}: init dirs"
+        reused += int(np.any(want_init[poly] != 0, 1).sum())
+        op = to_oracle_poses(p_np)
+        ref = orc.contact(oss, op, ia, ib, 0.0, nthreads=8)
+        gen = orc.polyhedra_gjk_step(oss, op, ia[poly], ib[poly], 0.0, want_init[poly])
+        for key in ("has", "world1", "world2", "normal", "depth"):
+            ref[key][poly] = gen[key]
+        has = r["has_contact"].cpu().numpy()
+        assert np.array_equal(has, ref["has"]), f"step {step}: has_contact differs"
+        for key in ("depth", "normal", "world1", "world2"):
+            assert np.array_equal(r[key].cpu().numpy()[has].view(np.uint32), ref[key][has].view(np.uint32)), \
+                f"step {step}: {key} differs"
+        new_dir = np.where(gen["dir_set"][:, None], gen["dir"], want_init[poly])
+        assert np.array_equal(fdir[poly].view(np.uint32), new_dir.view(np.uint32)), f"step {step}: result dirs"
+        # the generators of this step's pairs carry over (pairs that left the broad phase drop theirs)
+        last = {int(kk): d for kk, d in zip(keys[poly], new_dir) if np.any(d != 0)}
+        p_np[:, :3] += vel
+        poses.copy_(torch.as_tensor(p_np, device=ctx.device))
+    print(f"temporal warm start: {reused} pair-steps started from last_gjk_dir over 6 steps, bit-exact")
+    assert reused > 1000
