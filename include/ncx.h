@@ -227,6 +227,26 @@ int ncx_trimesh_kernel_ms(ncx_mesh* mesh, float* ms);
 int ncx_trimesh_cast_stats(ncx_mesh* mesh, const ncx_pose* pose, const float* const ray_soa_dev[6], size_t n,
                            float max_toi, uint32_t* nodes_visited_dev, uint32_t* tris_tested_dev);
 
+/* ---------------------------------------------------------------- composite shape (TriMesh) against convex shapes
+ * contact_composite_shape_shape (src/query/contact/contact_composite_shape_shape.rs:9-45) of the mesh
+ * at `mesh_pose` (host) and shape shape_dev[k] of `s` at poses_dev: the triangles whose box
+ * intersects the shape's box in the mesh frame loosened by `prediction`, query::contact(mesh pose,
+ * triangle, shape pose, shape, prediction) on each (a ball takes the closed-form point projection on
+ * the triangle, contact_ball_convex_polyhedron.rs:11-77; every other shape GJK/EPA with the triangle's
+ * support map, triangle.rs:290-307), and the deepest contact kept (world points, normal from the
+ * mesh toward the shape, depth; face_dev (nullable): its triangle id, UINT32_MAX when none).  The
+ * reference keeps the first of equally deep contacts in its BVT's order; here the least triangle id.
+ * The mesh and the shape set must live on the same device. */
+int ncx_mesh_contact_batch(ncx_mesh* mesh, const ncx_pose* mesh_pose, ncx_shapes* s, const ncx_pose* poses_dev,
+                           const uint32_t* shape_dev, size_t n, float prediction, uint8_t* has_dev,
+                           float* world1_dev, float* world2_dev, float* normal_dev, float* depth_dev,
+                           uint32_t* face_dev);
+/* distance_composite_shape_shape (src/query/distance/distance_composite_shape_shape.rs:9-101): the
+ * least query::distance (GJK) between the mesh and shape shape_dev[k], found by a search of the mesh's
+ * tree that skips boxes whose Minkowski-sum distance is not below the best so far. */
+int ncx_mesh_distance_batch(ncx_mesh* mesh, const ncx_pose* mesh_pose, ncx_shapes* s, const ncx_pose* poses_dev,
+                            const uint32_t* shape_dev, size_t n, float* dist_dev);
+
 /* ---------------------------------------------------------------- input preparation (host)
  * ConvexHull::try_from_points (src/shape/convex.rs:88-97 -> transformation::convex_hull3,
  * src/transformation/convex_hull3.rs:28-149): writes the indices of the input points that are

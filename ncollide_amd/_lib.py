@@ -45,6 +45,8 @@ SIGNATURES = {
     "ncx_distance_batch_capped": (C.c_int, [P, P, P, P, SZ, P, C.c_int32, P, P]),
     "ncx_time_of_impact_batch": (C.c_int, [P, P, P, P, P, SZ, F32, F32, P, P, P, P, P, P, P]),
     "ncx_shapes_cast_rays": (C.c_int, [P, P, P, P, SZ, F32, C.c_int, P, P, P]),
+    "ncx_mesh_contact_batch": (C.c_int, [P, P, P, P, P, SZ, F32, P, P, P, P, P, P]),
+    "ncx_mesh_distance_batch": (C.c_int, [P, P, P, P, P, SZ, P]),
     "ncx_trimesh_create": (C.c_int, [P, P, SZ, P, SZ, P]),
     "ncx_trimesh_destroy": (C.c_int, [P]),
     "ncx_trimesh_cast_rays": (C.c_int, [P, P, P, SZ, F32, P, P, P]),

@@ -40,7 +40,22 @@ struct Pool {
   const float* __restrict__ x;
   const float* __restrict__ y;
   const float* __restrict__ z;
+  // TriMesh triangles (three float4 per triangle: a with the face id in a.w, b, c), the parts of
+  // a composite shape; SHAPE_TRIANGLE records index them by `off`
+  const float4* __restrict__ tri;
 };
+
+// A part of a composite shape: triangle `off` of Pool::tri (TriMesh::triangle_at,
+// src/shape/trimesh.rs:471-475), in the mesh's frame.
+#define SHAPE_TRIANGLE 5u
+
+__device__ __forceinline__ void tri_vertices(const Pool& P, uint32_t k, f3& a, f3& b, f3& c) {
+  const float4* t = P.tri + 3 * (size_t)k;
+  const float4 A = t[0], B = t[1], C = t[2];
+  a = f3{A.x, A.y, A.z};
+  b = f3{B.x, B.y, B.z};
+  c = f3{C.x, C.y, C.z};
+}
 
 // point_cloud_support_point_id (src/utils/point_cloud_support_point.rs:6-24), split over G lanes.
 // The reference keeps index 0 when dot(p_0, dir) is NaN and otherwise returns the first index of
@@ -116,6 +131,14 @@ __device__ __forceinline__ f3 support(const ShapeD& g, const iso& m, f3 dir, con
       f3 ld = inv_xform_vec(m, dir);
       uint32_t id = hull_support_id<G>(P, g.off, g.cnt, ld, gl);
       f3 p = f3{P.x[g.off + id], P.y[g.off + id], P.z[g.off + id]};
+      return xform_pt(m, p);
+    }
+    case SHAPE_TRIANGLE: {  // Triangle::local_support_point (src/shape/triangle.rs:290-307)
+      f3 ld = inv_xform_vec(m, dir);
+      f3 a, b, c;
+      tri_vertices(P, g.off, a, b, c);
+      const float d1 = dot(a, ld), d2 = dot(b, ld), d3 = dot(c, ld);
+      const f3 p = d1 > d2 ? (d1 > d3 ? a : c) : (d2 > d3 ? b : c);
       return xform_pt(m, p);
     }
     default:
@@ -770,4 +793,147 @@ __device__ __forceinline__ bool shape_ray_cast(const ShapeD& g, const iso& m, f3
     return true;
   }
   return false;
+}
+
+// ------------------------------------------------------------------ triangles and isometries for composite shapes
+// approx::relative_eq!(x, y) with the default epsilon and max_relative (f32::EPSILON).
+__device__ __forceinline__ bool relative_eq_f(float x, float y) {
+  if (x == y) return true;
+  if (__builtin_isinf(x) || __builtin_isinf(y)) return false;
+  const float d = fabsf(x - y);
+  if (d <= NCX_FLT_EPS) return true;
+  const float ax = fabsf(x), ay = fabsf(y);
+  return d <= (ay > ax ? ay : ax) * NCX_FLT_EPS;
+}
+
+// Triangle::project_point_with_location with solid = true (src/query/point/point_triangle.rs:61-240)
+// for the triangle (a, b, c) placed by m: the projection in the world frame; *inside is the
+// PointProjection's is_inside (relative_eq!(projection, pt) in 3D, :6-19; true on the solid
+// branch).
+__device__ __forceinline__ f3 proj_triangle_at(const iso& m, f3 a, f3 b, f3 c, f3 pt, Loc& loc, bool& inside) {
+  const f3 p = inv_xform_pt(m, pt);
+  const f3 ab = b - a, ac = c - a, ap = p - a;
+  const float ab_ap = dot(ab, ap), ac_ap = dot(ac, ap);
+  f3 res;
+  if (ab_ap <= 0.f && ac_ap <= 0.f) { loc.kind = LOC_VERTEX; loc.id = 0; res = xform_pt(m, a); }
+  else {
+    const f3 bp = p - b;
+    const float ab_bp = dot(ab, bp), ac_bp = dot(ac, bp);
+    if (ab_bp >= 0.f && ac_bp <= ab_bp) { loc.kind = LOC_VERTEX; loc.id = 1; res = xform_pt(m, b); }
+    else {
+      const f3 cp = p - c;
+      const float ab_cp = dot(ab, cp), ac_cp = dot(ac, cp);
+      if (ac_cp >= 0.f && ab_cp <= ac_cp) { loc.kind = LOC_VERTEX; loc.id = 2; res = xform_pt(m, c); }
+      else {
+        const f3 bc = c - b;
+        const f3 n = cross(ab, ac);
+        const float vc = dot(n, cross(ab, ap));
+        const float vb = -dot(n, cross(ac, cp));
+        const float va = dot(n, cross(bc, bp));
+        if (vc < 0.f && ab_ap >= 0.f && ab_bp <= 0.f) {
+          const float v = ab_ap / norm_sq(ab);
+          loc.kind = LOC_EDGE; loc.id = 0; loc.bc0 = 1.f - v; loc.bc1 = v;
+          res = xform_pt(m, a + ab * v);
+        } else if (vb < 0.f && ac_ap >= 0.f && ac_cp <= 0.f) {
+          const float w = ac_ap / norm_sq(ac);
+          loc.kind = LOC_EDGE; loc.id = 2; loc.bc0 = 1.f - w; loc.bc1 = w;
+          res = xform_pt(m, a + ac * w);
+        } else if (va < 0.f && ac_bp - ab_bp >= 0.f && ab_cp - ac_cp >= 0.f) {
+          const float w = dot(bc, bp) / norm_sq(bc);
+          loc.kind = LOC_EDGE; loc.id = 1; loc.bc0 = 1.f - w; loc.bc1 = w;
+          res = xform_pt(m, b + bc * w);
+        } else if (va + vb + vc != 0.f) {
+          const float denom = 1.f / (va + vb + vc);
+          const float v = vb * denom, w = vc * denom;
+          loc.kind = LOC_FACE; loc.id = dot(n, ap) >= 0.f ? 0 : 1;
+          loc.bc0 = 1.f - v - w; loc.bc1 = v; loc.bc2 = w;
+          res = xform_pt(m, a + ab * v + ac * w);
+        } else {
+          loc.kind = LOC_SOLID; loc.id = 0;
+          inside = true;
+          return pt;
+        }
+      }
+    }
+  }
+  inside = relative_eq_f(res.x, pt.x) && relative_eq_f(res.y, pt.y) && relative_eq_f(res.z, pt.z);
+  return res;
+}
+
+// Quaternion product (nalgebra's Quaternion * Quaternion), coordinates (i, j, k, w).
+__device__ __forceinline__ quat quat_mul(quat a, quat b) {
+  quat r;
+  r.w = ((a.w * b.w - a.i * b.i) - a.j * b.j) - a.k * b.k;
+  r.i = ((a.w * b.i + a.i * b.w) + a.j * b.k) - a.k * b.j;
+  r.j = ((a.w * b.j - a.i * b.k) + a.j * b.w) + a.k * b.i;
+  r.k = ((a.w * b.k + a.i * b.j) - a.j * b.i) + a.k * b.w;
+  return r;
+}
+
+// m1.inverse() * m2 (nalgebra Isometry::inverse then Isometry * Isometry): rotation q1^-1 q2,
+// translation q1^-1 (-t1) + q1^-1 t2.
+__device__ __forceinline__ iso iso_inv_mul(const iso& m1, const iso& m2) {
+  const quat qi = quat{-m1.q.i, -m1.q.j, -m1.q.k, m1.q.w};
+  iso r;
+  const f3 t_inv = qrot(qi, -m1.t);
+  r.t = t_inv + qrot(qi, m2.t);
+  r.q = quat_mul(qi, m2.q);
+  return r;
+}
+
+// |R| h of a cuboid rotated by q (isometry_ops.rs:43-45 through UnitQuaternion::to_rotation_matrix).
+__device__ __forceinline__ f3 cuboid_world_half(quat q, float hx, float hy, float hz) {
+  const float qi = q.i, qj = q.j, qk = q.k, qw = q.w;
+  const float ww = qw * qw, ii = qi * qi, jj = qj * qj, kk = qk * qk;
+  const float ij = qi * qj * 2.f, wk = qw * qk * 2.f, wj = qw * qj * 2.f;
+  const float ik = qi * qk * 2.f, jk = qj * qk * 2.f, wi = qw * qi * 2.f;
+  const float M[3][3] = {{ww + ii - jj - kk, ij - wk, wj + ik},
+                         {wk + ij, ww - ii + jj - kk, jk - wi},
+                         {ik - wj, wi + jk, ww - ii - jj + kk}};
+  f3 ws;
+  ws.x = (fabsf(M[0][0]) * hx + fabsf(M[0][1]) * hy) + fabsf(M[0][2]) * hz;
+  ws.y = (fabsf(M[1][0]) * hx + fabsf(M[1][1]) * hy) + fabsf(M[1][2]) * hz;
+  ws.z = (fabsf(M[2][0]) * hx + fabsf(M[2][1]) * hy) + fabsf(M[2][2]) * hz;
+  return ws;
+}
+
+// Shape::aabb at m (aabb_ball.rs:8-13, aabb_cuboid.rs:7-14, aabb_support_map.rs:35-45,
+// aabb_convex.rs:7-11 -> aabb_utils.rs:59-79), every lane of a group of G the same result (the
+// hull's points split over the lanes; a min/max does not depend on the order).
+template <int G>
+__device__ __forceinline__ void shape_aabb_g(const ShapeD& g, const iso& m, const Pool& P, int gl, f3& mn, f3& mx) {
+  if (g.type == NCX_BALL) {
+    mn = m.t + f3{-g.a, -g.a, -g.a};
+    mx = m.t + f3{g.a, g.a, g.a};
+  } else if (g.type == NCX_CUBOID) {
+    const f3 ws = cuboid_world_half(m.q, g.a, g.b, g.c);
+    mn = m.t - ws;
+    mx = m.t + ws;
+  } else if (g.type == NCX_CAPSULE) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      f3 e = f3{0.f, 0.f, 0.f};
+      set_comp(e, d, 1.f);
+      set_comp(mx, d, comp(support<1>(g, m, e, P, 0), d));
+      set_comp(e, d, -1.f);
+      set_comp(mn, d, comp(support<1>(g, m, e, P, 0), d));
+    }
+  } else {
+    f3 lo = f3{__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    f3 hi = -lo;
+    for (uint32_t v = gl; v < g.cnt; v += G) {
+      const f3 p = xform_pt(m, f3{P.x[g.off + v], P.y[g.off + v], P.z[g.off + v]});
+      lo = f3{fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z)};
+      hi = f3{fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z)};
+    }
+#pragma unroll
+    for (int o = G / 2; o >= 1; o >>= 1) {
+      lo.x = fminf(lo.x, __shfl_xor(lo.x, o, 64)); lo.y = fminf(lo.y, __shfl_xor(lo.y, o, 64));
+      lo.z = fminf(lo.z, __shfl_xor(lo.z, o, 64));
+      hi.x = fmaxf(hi.x, __shfl_xor(hi.x, o, 64)); hi.y = fmaxf(hi.y, __shfl_xor(hi.y, o, 64));
+      hi.z = fmaxf(hi.z, __shfl_xor(hi.z, o, 64));
+    }
+    mn = lo;
+    mx = hi;
+  }
 }

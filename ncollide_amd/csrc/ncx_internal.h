@@ -121,6 +121,7 @@ struct ncx_shapes {
   DevBuf hx, hy, hz;  // hull pool
   ContactTiers tiers; // for ncx_contact_batch / ncx_distance_batch
   DevBuf qerr;        // error flag of the time-of-impact, ray-cast and composite queries
+  const float4* tri = nullptr;  // triangles of the Triangle parts (a mesh's combined set only)
   // autotuner choice per query (0 contact, 1 distance) and bucket: an index into the kernel
   // variants (group width 8, 16, 32), or -1 before tuning
   int8_t tune[2][NCX_NBUCKETS] = {{-1, -1, -1, -1}, {-1, -1, -1, -1}};
@@ -252,6 +253,31 @@ int radix_sort_pairs64(ncx_ctx* ctx, uint32_t* hi, uint32_t* lo, size_t n, int h
                        DevBuf& t2, DevBuf& t3, DevBuf& hist, DevBuf& scan_tmp);
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// TriMesh on the device (ncx_raycast.hip builds it; ncx_composite.hip queries it).
+struct ncx_mesh {
+  ncx_ctx* ctx;
+  size_t npoints = 0, ntri = 0;
+  LBVH bvh;
+  DevBuf tris;      // float4[3 * ntri] in leaf order: a (w = face id bits), b, c
+  DevBuf boxes;     // 6 SoA arrays of triangle boxes (build input)
+  DevBuf pts, idx;  // uploaded mesh
+  DevBuf overflow;  // spill area for traversal stacks deeper than the LDS part
+  DevBuf err;       // error flag (stack exhausted)
+  DevBuf work;      // ray counters of the persistent cast (one per part)
+  DevBuf qnodes;    // QNode[ntri - 1]: the LBVH with 16-bit quantised child boxes (cast kernel)
+  DevBuf qframe;    // float[6]: quantisation origin (3) and step (3)
+  uint32_t levels = 1;  // internal-node levels of the cast tree (root = level 1)
+  bool timing = false;              // record events around the cast kernel (ncx_trimesh_set_timing)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // composite queries against shape sets (ncx_composite.hip): the shapes and the mesh's triangles
+  // as one shape set (rebuilt when another shape set comes), poses of both, candidate pairs
+  ncx_shapes* comp = nullptr;
+  const ncx_shapes* comp_src = nullptr;
+  size_t comp_src_n = 0;
+  DevBuf cposes, ccnt, coffs, cscan, ctot, ca, cb, chas, cw1, cw2, cn, cdepth, citers;
+  void release_composite();
+};
 
 // Tiered contact launch over pairs counted on the device (ncx_narrow.hip); no host
 // synchronisation unless may_tune (the autotuner's first run, never inside graph capture).

@@ -11,22 +11,6 @@
 #include <mutex>
 #include "ncx_internal.h"
 
-struct ncx_mesh {
-  ncx_ctx* ctx;
-  size_t npoints = 0, ntri = 0;
-  LBVH bvh;
-  DevBuf tris;      // float4[3 * ntri] in leaf order: a (w = face id bits), b, c
-  DevBuf boxes;     // 6 SoA arrays of triangle boxes (build input)
-  DevBuf pts, idx;  // uploaded mesh
-  DevBuf overflow;  // spill area for traversal stacks deeper than the LDS part
-  DevBuf err;       // error flag (stack exhausted)
-  DevBuf work;      // ray counters of the persistent cast (one per part)
-  DevBuf qnodes;    // QNode[ntri - 1]: the LBVH with 16-bit quantised child boxes (cast kernel)
-  DevBuf qframe;    // float[6]: quantisation origin (3) and step (3)
-  uint32_t levels = 1;  // internal-node levels of the cast tree (root = level 1)
-  bool timing = false;              // record events around the cast kernel (ncx_trimesh_set_timing)
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-};
 
 // Internal node for the cast kernel, 32 bytes (two 16-byte loads per visit instead of four):
 // both child boxes as 16-bit integers on one grid over the root box,
@@ -620,6 +604,7 @@ extern "C" int ncx_trimesh_destroy(ncx_mesh* m) {
   m->bvh.release();
   m->tris.release(); m->boxes.release(); m->pts.release(); m->idx.release(); m->overflow.release(); m->err.release(); m->work.release();
   m->qnodes.release(); m->qframe.release();
+  m->release_composite();
   if (m->ev0) (void)hipEventDestroy(m->ev0);
   if (m->ev1) (void)hipEventDestroy(m->ev1);
   delete m;

@@ -69,6 +69,36 @@ class TriMesh:
                                                        _lib.ptr(nodes), _lib.ptr(tris)))
         return nodes, tris
 
+    # -- the mesh as a composite shape against convex shapes of a ShapeSet
+    def contact_with_shapes(self, shapes, poses, shape_idx, prediction, pose=None):
+        """contact_composite_shape_shape(this mesh at `pose`, shape shape_idx[k] at poses) per k:
+        dict of CUDA tensors has_contact, world1, world2, normal (mesh toward shape), depth, face."""
+        dev = self.ctx.device
+        poses = torch.as_tensor(poses, device=dev, dtype=torch.float32).contiguous()
+        idx = torch.as_tensor(shape_idx, device=dev).to(torch.int32).contiguous()
+        n = idx.numel()
+        has = torch.empty(n, dtype=torch.uint8, device=dev)
+        w1, w2, nr = (torch.empty((n, 3), dtype=torch.float32, device=dev) for _ in range(3))
+        depth = torch.empty(n, dtype=torch.float32, device=dev)
+        face = torch.empty(n, dtype=torch.int32, device=dev)
+        p = self._pose(pose)
+        _lib.check(self.ctx.lib.ncx_mesh_contact_batch(self.h, _lib.ptr(p), shapes.h, _lib.ptr(poses), _lib.ptr(idx),
+                                                       n, float(prediction), _lib.ptr(has), _lib.ptr(w1),
+                                                       _lib.ptr(w2), _lib.ptr(nr), _lib.ptr(depth), _lib.ptr(face)))
+        return dict(has_contact=has.bool(), world1=w1, world2=w2, normal=nr, depth=depth, face=face)
+
+    def distance_to_shapes(self, shapes, poses, shape_idx, pose=None):
+        """distance_composite_shape_shape(this mesh at `pose`, shape shape_idx[k]) per k (CUDA tensor)."""
+        dev = self.ctx.device
+        poses = torch.as_tensor(poses, device=dev, dtype=torch.float32).contiguous()
+        idx = torch.as_tensor(shape_idx, device=dev).to(torch.int32).contiguous()
+        n = idx.numel()
+        d = torch.empty(n, dtype=torch.float32, device=dev)
+        p = self._pose(pose)
+        _lib.check(self.ctx.lib.ncx_mesh_distance_batch(self.h, _lib.ptr(p), shapes.h, _lib.ptr(poses),
+                                                        _lib.ptr(idx), n, _lib.ptr(d)))
+        return d
+
     def __del__(self):
         try:
             if getattr(self, "h", None):

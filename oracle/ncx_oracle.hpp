@@ -99,7 +99,9 @@ template <class R> inline void quat_to_mat(const Quat<R>& q, R m[3][3]) {
 template <class R> inline R gjk_eps_tol() { return Eps<R>::eps() * R(10); }  // gjk.rs:26-29
 
 // ---------------------------------------------------------------- shapes
-enum ShapeType : uint32_t { BALL = 0, CUBOID = 1, CAPSULE = 2, HULL = 3, ORIGIN = 4 };
+enum ShapeType : uint32_t { BALL = 0, CUBOID = 1, CAPSULE = 2, HULL = 3, ORIGIN = 4, TRIANGLE = 5 };
+// TRIANGLE: a part of a TriMesh (TriMesh::triangle_at, shape/trimesh.rs:471-475); pts = its three
+// vertices a, b, c in the mesh frame.
 
 template <class R> struct Shape {
   uint32_t type;
@@ -144,6 +146,12 @@ template <class R> inline V3<R> support_point(const Shape<R>& g, const Iso<R>& m
       V3<R> ld = inv_xform_vec(m, dir);
       uint32_t id = point_cloud_support_id(ld, g.pts, g.npts);
       return xform_pt(m, g.pts[id]);
+    }
+    case TRIANGLE: {  // Triangle::local_support_point (shape/triangle.rs:290-307)
+      V3<R> ld = inv_xform_vec(m, dir);
+      R d1 = dot(g.pts[0], ld), d2 = dot(g.pts[1], ld), d3 = dot(g.pts[2], ld);
+      V3<R> p = d1 > d2 ? (d1 > d3 ? g.pts[0] : g.pts[2]) : (d2 > d3 ? g.pts[1] : g.pts[2]);
+      return xform_pt(m, p);
     }
     default:
       return V3<R>(0, 0, 0);
@@ -914,6 +922,55 @@ template <class R> inline V3<R> cuboid_feature_normal(int fkind, int fid) {
   }
 }
 
+// approx::relative_eq!(x, y) with the default epsilon and max_relative (EPSILON of the type).
+template <class R> inline bool relative_eq_r(R x, R y) {
+  if (x == y) return true;
+  if (std::isinf(x) || std::isinf(y)) return false;
+  R d = std::fabs(x - y);
+  if (d <= Eps<R>::eps()) return true;
+  R ax = std::fabs(x), ay = std::fabs(y);
+  return d <= (ay > ax ? ay : ax) * Eps<R>::eps();
+}
+
+// Triangle::project_point_with_location with solid = true for a triangle placed by m
+// (query/point/point_triangle.rs:61-240); *inside = relative_eq!(projection, pt) (3D, :6-19), true
+// on the solid branch.  Returns the projection in the world frame.
+template <class R> inline V3<R> project_triangle_at(const Iso<R>& m, V3<R> a, V3<R> b, V3<R> c, V3<R> pt, bool* inside) {
+  V3<R> p = inv_xform_pt(m, pt);
+  V3<R> ab = b - a, ac = c - a, ap = p - a;
+  R ab_ap = dot(ab, ap), ac_ap = dot(ac, ap);
+  V3<R> res;
+  bool done = false;
+  if (ab_ap <= R(0) && ac_ap <= R(0)) { res = xform_pt(m, a); done = true; }
+  V3<R> bp = p - b;
+  R ab_bp = dot(ab, bp), ac_bp = dot(ac, bp);
+  if (!done && ab_bp >= R(0) && ac_bp <= ab_bp) { res = xform_pt(m, b); done = true; }
+  V3<R> cp = p - c;
+  R ab_cp = dot(ab, cp), ac_cp = dot(ac, cp);
+  if (!done && ac_cp >= R(0) && ab_cp <= ac_cp) { res = xform_pt(m, c); done = true; }
+  if (!done) {
+    V3<R> bc = c - b;
+    V3<R> n = cross(ab, ac);
+    R vc = dot(n, cross(ab, ap));
+    R vb = -dot(n, cross(ac, cp));
+    R va = dot(n, cross(bc, bp));
+    if (vc < R(0) && ab_ap >= R(0) && ab_bp <= R(0)) res = xform_pt(m, a + ab * (ab_ap / norm_sq(ab)));
+    else if (vb < R(0) && ac_ap >= R(0) && ac_cp <= R(0)) res = xform_pt(m, a + ac * (ac_ap / norm_sq(ac)));
+    else if (va < R(0) && ac_bp - ab_bp >= R(0) && ab_cp - ac_cp >= R(0))
+      res = xform_pt(m, b + bc * (dot(bc, bp) / norm_sq(bc)));
+    else if (va + vb + vc != R(0)) {
+      R denom = R(1) / (va + vb + vc);
+      R v = vb * denom, w = vc * denom;
+      res = xform_pt(m, a + ab * v + ac * w);
+    } else {
+      *inside = true;
+      return pt;
+    }
+  }
+  *inside = relative_eq_r(res.x, pt.x) && relative_eq_r(res.y, pt.y) && relative_eq_r(res.z, pt.z);
+  return res;
+}
+
 // contact_ball_convex_polyhedron (contact_ball_convex_polyhedron.rs:11-58) for Cuboid and ConvexHull.
 // For ConvexHull, the reference's zero-distance branch needs the hull's face/edge adjacency
 // (convex.rs:388-415); this restatement returns None there (ball centre within eps of the surface).
@@ -921,14 +978,22 @@ template <class R> inline bool contact_ball_convex(V3<R> c1, R r, const Iso<R>& 
   bool inside;
   V3<R> world2;
   int fkind = 3, fid = 0;
+  V3<R> tri_n(0, 1, 0);  // Triangle::feature_normal: its normal, or +y when degenerate (triangle.rs:367-374)
   if (g2.type == CUBOID) world2 = project_point_cuboid(m2, V3<R>(g2.a, g2.b, g2.c), c1, &inside, &fkind, &fid);
-  else world2 = project_point_support_map(m2, g2, c1, &inside);
+  else if (g2.type == TRIANGLE) {
+    world2 = project_triangle_at(m2, g2.pts[0], g2.pts[1], g2.pts[2], c1, &inside);
+    V3<R> nn;
+    if (try_normalize(cross(g2.pts[1] - g2.pts[0], g2.pts[2] - g2.pts[0]), Eps<R>::eps(), &nn)) tri_n = nn;
+  } else world2 = project_point_support_map(m2, g2, c1, &inside);
   V3<R> dpt = world2 - c1;
   V3<R> dir; R dist;
   R depth; V3<R> normal;
   if (try_normalize(dpt, Eps<R>::eps(), &dir, &dist)) {
     if (inside) { depth = dist + r; normal = -dir; }
     else { depth = -dist + r; normal = dir; }
+  } else if (g2.type == TRIANGLE) {  // every triangle location names a feature
+    depth = r;
+    normal = -tri_n;
   } else {
     if (g2.type != CUBOID || fkind == 3) return false;
     depth = r;
@@ -950,7 +1015,8 @@ template <class R> inline bool contact(const Iso<R>& m1, const Shape<R>& g1, con
   if (gjk_iters) *gjk_iters = 0;
   if (epa_iters) *epa_iters = 0;
   bool b1 = g1.type == BALL, b2 = g2.type == BALL;
-  bool poly1 = g1.type == CUBOID || g1.type == HULL, poly2 = g2.type == CUBOID || g2.type == HULL;
+  bool poly1 = g1.type == CUBOID || g1.type == HULL || g1.type == TRIANGLE;
+  bool poly2 = g2.type == CUBOID || g2.type == HULL || g2.type == TRIANGLE;
   if (b1 && b2) return contact_ball_ball(m1.t, g1.a, m2.t, g2.a, prediction, out);
   if (b1 && poly2) return contact_ball_convex(m1.t, g1.a, m2, g2, prediction, out);
   if (poly1 && b2) {
@@ -1950,6 +2016,90 @@ template <class R> inline bool polyhedra_gjk_step(const Iso<R>& m1, const Shape<
   out->world1 = w1; out->world2 = w2; out->normal = n;
   out->depth = -dot(n, w2 - w1);
   return true;
+}
+
+
+// ================================================================ composite shape (TriMesh) against a convex shape
+// nalgebra Quaternion product, coordinates (i, j, k, w).
+template <class R> inline Quat<R> quat_mul(const Quat<R>& a, const Quat<R>& b) {
+  Quat<R> r;
+  r.w = ((a.w * b.w - a.i * b.i) - a.j * b.j) - a.k * b.k;
+  r.i = ((a.w * b.i + a.i * b.w) + a.j * b.k) - a.k * b.j;
+  r.j = ((a.w * b.j - a.i * b.k) + a.j * b.w) + a.k * b.i;
+  r.k = ((a.w * b.k + a.i * b.j) - a.j * b.i) + a.k * b.w;
+  return r;
+}
+// m1.inverse() * m2 (Isometry::inverse, Isometry * Isometry).
+template <class R> inline Iso<R> iso_inv_mul(const Iso<R>& m1, const Iso<R>& m2) {
+  Quat<R> qi{-m1.q.i, -m1.q.j, -m1.q.k, m1.q.w};
+  Iso<R> r;
+  V3<R> t_inv = qrot(qi, -m1.t);
+  r.t = t_inv + qrot(qi, m2.t);
+  r.q = quat_mul(qi, m2.q);
+  return r;
+}
+
+template <class R> inline AABB<R> triangle_aabb(const V3<R>* t) {  // local AABB: the vertices' min and max
+  return AABB<R>{vmin(vmin(t[0], t[1]), t[2]), vmax(vmax(t[0], t[1]), t[2])};
+}
+
+// contact_composite_shape_shape (query/contact/contact_composite_shape_shape.rs:9-45): the
+// triangles whose box intersects g2's box in the mesh frame loosened by the prediction
+// (BoundingVolumeInterferencesCollector over the BVT), query::contact on each, the deepest kept
+// (replace when deeper; triangles in ascending id order, so the least id among equally deep).
+// tri: 3 vertices per triangle (mesh frame).
+template <class R> inline bool contact_mesh_shape(const Iso<R>& m1, const V3<R>* tri, size_t ntri, const Iso<R>& m2,
+                                                  const Shape<R>& g2, R prediction, Contact<R>* out,
+                                                  uint32_t* face) {
+  Iso<R> ls = iso_inv_mul(m1, m2);
+  AABB<R> box = aabb_loosened(shape_aabb(g2, ls), prediction);
+  bool found = false;
+  for (size_t k = 0; k < ntri; ++k) {
+    if (!aabb_intersects(triangle_aabb(tri + 3 * k), box)) continue;
+    Shape<R> t;
+    t.type = TRIANGLE; t.a = t.b = t.c = R(0); t.pts = tri + 3 * k; t.npts = 3;
+    Contact<R> c;
+    c.depth = R(0);
+    if (!contact(m1, t, m2, g2, prediction, &c)) continue;
+    if (!found || c.depth > out->depth) {
+      *out = c;
+      *face = (uint32_t)k;
+      found = true;
+    }
+  }
+  return found;
+}
+
+// distance_composite_shape_shape (query/distance/distance_composite_shape_shape.rs:9-101): the
+// best-first search over the BVT returns the least query::distance among the triangles whose
+// Minkowski-sum box distance (the visitor's cost, :70-79) is below the best so far; visiting the
+// leaves by increasing cost and stopping at the first cost not below the best gives that value.
+template <class R> inline R distance_mesh_shape(const Iso<R>& m1, const V3<R>* tri, size_t ntri, const Iso<R>& m2,
+                                                const Shape<R>& g2) {
+  Iso<R> ls = iso_inv_mul(m1, m2);
+  AABB<R> b2 = shape_aabb(g2, ls);
+  V3<R> shift = -aabb_center(b2);
+  V3<R> margin = (b2.maxs - b2.mins) * R(0.5);
+  std::vector<std::pair<R, size_t>> order(ntri);
+  for (size_t k = 0; k < ntri; ++k) {
+    AABB<R> t = triangle_aabb(tri + 3 * k);
+    V3<R> lo = (t.mins + shift) + (-margin), hi = (t.maxs + shift) + margin;
+    V3<R> z(0, 0, 0);
+    V3<R> mins_pt = lo - z, pt_maxs = z - hi;
+    V3<R> sh(std::fmax(std::fmax(mins_pt.x, pt_maxs.x), R(0)), std::fmax(std::fmax(mins_pt.y, pt_maxs.y), R(0)),
+             std::fmax(std::fmax(mins_pt.z, pt_maxs.z), R(0)));
+    order[k] = {norm(sh), k};
+  }
+  std::sort(order.begin(), order.end());
+  R best = Eps<R>::maxv();
+  for (auto& e : order) {
+    if (!(e.first < best)) break;
+    Shape<R> t;
+    t.type = TRIANGLE; t.a = t.b = t.c = R(0); t.pts = tri + 3 * e.second; t.npts = 3;
+    R d = distance(m1, t, m2, g2);
+    if (d < best) best = d;
+  }
+  return best;
 }
 
 }  // namespace ncxo

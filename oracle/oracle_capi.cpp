@@ -358,6 +358,39 @@ int polyhedra_step_batch(int64_t ns, const uint32_t* types, const R* params, con
   return 0;
 }
 
+
+template <class R>
+int mesh_queries(const R* tri_pts, int64_t ntri, const R* mesh_pose7, int64_t ns, const uint32_t* types, const R* params,
+                 const uint32_t* hoff, const uint32_t* hcnt, const R* hpts, int64_t npts_total, const R* poses,
+                 const uint32_t* idx, int64_t nq, R prediction, int mode, uint8_t* has, R* w1, R* w2, R* nrm, R* depth,
+                 uint32_t* face, R* dist, int nthreads) {
+  ShapeSet<R> ss;
+  ss.load(ns, types, params, hoff, hcnt, hpts, npts_total);
+  std::vector<V3<R>> tri((size_t)ntri * 3);
+  for (int64_t i = 0; i < 3 * ntri; ++i) tri[i] = V3<R>(tri_pts[3 * i], tri_pts[3 * i + 1], tri_pts[3 * i + 2]);
+  Iso<R> m1 = pose_at(mesh_pose7, 0);
+  parallel_for(nq, nthreads, [&](int64_t k) {
+    uint32_t s = idx[k];
+    if (mode == 0) {
+      Contact<R> c;
+      c.depth = R(0);
+      uint32_t f = 0xFFFFFFFFu;
+      bool ok = contact_mesh_shape(m1, tri.data(), (size_t)ntri, pose_at(poses, s), ss.shapes[s], prediction, &c, &f);
+      has[k] = ok ? 1 : 0;
+      face[k] = f;
+      for (int d = 0; d < 3; ++d) {
+        w1[3 * k + d] = ok ? c.world1[d] : R(0);
+        w2[3 * k + d] = ok ? c.world2[d] : R(0);
+        nrm[3 * k + d] = ok ? c.normal[d] : R(0);
+      }
+      depth[k] = ok ? c.depth : R(0);
+    } else {
+      dist[k] = distance_mesh_shape(m1, tri.data(), (size_t)ntri, pose_at(poses, s), ss.shapes[s]);
+    }
+  });
+  return 0;
+}
+
 extern "C" {
 
 int orc_contact_f32(int64_t ns, const uint32_t* types, const float* params, const uint32_t* hoff, const uint32_t* hcnt,
@@ -596,5 +629,16 @@ int orc_polyhedra_step_f32(int64_t ns, const uint32_t* types, const float* param
   return polyhedra_step_batch<float>(ns, types, params, hoff, hcnt, hpts, npts, poses, ia, ib, np, prediction,
                                      last_dirs, has, w1, w2, nrm, depth, dir_out, dir_set, nthreads);
 }
+
+#define ORC_MESH(T, R)                                                                                           \
+  int orc_mesh_queries_##T(const R* tri_pts, int64_t ntri, const R* mesh_pose7, int64_t ns, const uint32_t* types, \
+                           const R* params, const uint32_t* hoff, const uint32_t* hcnt, const R* hpts, int64_t npts,  \
+                           const R* poses, const uint32_t* idx, int64_t nq, R prediction, int mode, uint8_t* has,     \
+                           R* w1, R* w2, R* nrm, R* depth, uint32_t* face, R* dist, int nthreads) {                  \
+    return mesh_queries<R>(tri_pts, ntri, mesh_pose7, ns, types, params, hoff, hcnt, hpts, npts, poses, idx, nq,       \
+                           prediction, mode, has, w1, w2, nrm, depth, face, dist, nthreads);                          \
+  }
+ORC_MESH(f32, float)
+ORC_MESH(f64, double)
 
 }  // extern "C"

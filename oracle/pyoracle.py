@@ -60,6 +60,9 @@ def _declare(L):
         C.c_int)
     sig("orc_polyhedra_step_f32", C.c_int, I64, P, P, P, P, P, I64, P, P, P, I64, C.c_float, P, P, P, P, P, P, P,
         P, C.c_int)
+    for t, R in (("f32", C.c_float), ("f64", C.c_double)):
+        sig(f"orc_mesh_queries_{t}", C.c_int, P, I64, P, I64, P, P, P, P, P, I64, P, P, I64, R, C.c_int, P, P, P, P, P,
+            P, P, C.c_int)
     sig("orc_ray_triangle_f64", C.c_int, P, P, P, P, P, P)
     sig("orc_ray_aabb_f64", C.c_int, P, P, P, C.c_double, C.c_int, P)
     sig("orc_project_f64", C.c_int, C.c_int, P, P, P, P, P)
@@ -396,3 +399,42 @@ def polyhedra_gjk_step(ss, poses, ia, ib, prediction, last_dirs, nthreads=8):
     L.orc_polyhedra_step_f32(*ss.args(), _p(poses), _p(ia), _p(ib), n, prediction, _p(ld), _p(has), _p(w1), _p(w2),
                              _p(nr), _p(depth), _p(nd), _p(dset), nthreads)
     return dict(has=has.astype(bool), world1=w1, world2=w2, normal=nr, depth=depth, dir=nd, dir_set=dset.astype(bool))
+
+
+def _mesh_args(ss, points, indices, mesh_pose7):
+    R = ss.R
+    pts = np.asarray(points, R).reshape(-1, 3)
+    idx = np.asarray(indices, np.int64).reshape(-1, 3)
+    tri = np.ascontiguousarray(pts[idx].reshape(-1, 9))
+    mp = np.ascontiguousarray(mesh_pose7, R).reshape(1, 7)
+    return tri, len(idx), mp
+
+
+def mesh_contact(ss, points, indices, mesh_pose7, poses, shape_idx, prediction, nthreads=8):
+    """contact_composite_shape_shape(mesh at mesh_pose7, shape shape_idx[k]) per query."""
+    L = lib()
+    R = ss.R
+    tri, ntri, mp = _mesh_args(ss, points, indices, mesh_pose7)
+    poses = np.ascontiguousarray(poses, R).reshape(-1, 7)
+    qi = np.ascontiguousarray(shape_idx, np.uint32)
+    n = len(qi)
+    has = np.zeros(n, np.uint8); face = np.zeros(n, np.uint32)
+    w1, w2, nr = (np.zeros((n, 3), R) for _ in range(3))
+    depth = np.zeros(n, R)
+    getattr(L, f"orc_mesh_queries_{ss.tag}")(_p(tri), ntri, _p(mp), *ss.args(), _p(poses), _p(qi), n, prediction, 0,
+                                              _p(has), _p(w1), _p(w2), _p(nr), _p(depth), _p(face), None, nthreads)
+    return dict(has=has.astype(bool), world1=w1, world2=w2, normal=nr, depth=depth, face=face)
+
+
+def mesh_distance(ss, points, indices, mesh_pose7, poses, shape_idx, nthreads=8):
+    """distance_composite_shape_shape(mesh at mesh_pose7, shape shape_idx[k]) per query."""
+    L = lib()
+    R = ss.R
+    tri, ntri, mp = _mesh_args(ss, points, indices, mesh_pose7)
+    poses = np.ascontiguousarray(poses, R).reshape(-1, 7)
+    qi = np.ascontiguousarray(shape_idx, np.uint32)
+    n = len(qi)
+    dist = np.zeros(n, R)
+    getattr(L, f"orc_mesh_queries_{ss.tag}")(_p(tri), ntri, _p(mp), *ss.args(), _p(poses), _p(qi), n, 0.0, 1, None,
+                                              None, None, None, None, None, _p(dist), nthreads)
+    return dist
