@@ -33,7 +33,7 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--quick", action="store_true", help="skip the secondary workloads")
-    p.add_argument("--only", choices=["hulls", "learned", "scene", "plain"], default=None,
+    p.add_argument("--only", choices=["hulls", "learned", "scene", "plain", "queries"], default=None,
                    help="run one secondary workload alone (after the ray-cast headline); for profiling")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--copy-calibration", action="store_true",
@@ -238,9 +238,15 @@ def bench_learned(args, ncx, torch, ctx):
     fwd = _timed(torch, lambda: mlp.forward(feat), 5)
     flops = 14208.0 * nf
     res = learned.evaluate_coherent(ctx, mlp, seed=4, npairs=args.hull_pairs, steps=args.c4_steps,
-                                    variants=("plain", "learned"))
-    p, l = res["plain"], res["learned"]
+                                    variants=("plain", "learned", "temporal", "oracle_dir"))
+    p, l, tp, ex = res["plain"], res["learned"], res["temporal"], res["oracle_dir"]
     return dict(pairs=res["pairs"], steps=res["steps"], train_samples=int(nf), train_set_s=gen_s, train_s=train_s,
+                _mlp=mlp,
+                temporal_pairs_per_s=tp["pairs_per_s"], exact_direction_pairs_per_s=ex["pairs_per_s"],
+                temporal_iters={"contact": tp["contact_gjk_iters_per_pair"],
+                                "distance": tp["distance_gjk_iters_per_pair"]},
+                exact_direction_iters={"contact": ex["contact_gjk_iters_per_pair"],
+                                       "distance": ex["distance_gjk_iters_per_pair"]},
                 loss_first=hist[0], loss_last=hist[-1],
                 plain_pairs_per_s=p["pairs_per_s"], learned_pairs_per_s=l["pairs_per_s"],
                 speedup=l["pairs_per_s"] / p["pairs_per_s"],
@@ -253,7 +259,41 @@ def bench_learned(args, ncx, torch, ctx):
                              "frac_of_fp16_dense_peak": flops / fwd / 2.5e15})
 
 
-def bench_scene(args, ncx, torch, ctx):
+def cpu_accuracy_learned(ncx, torch, ctx, mlp, nsample=16384):
+    """Error against the f64 oracle (the error reference) of plain and learned-warm-start contact
+    depth and distance on the learned workload's hull distribution: relative errors' median and
+    p99 (depth on pairs in contact in both, distance on separated pairs)."""
+    from oracle import pyoracle as orc
+    from ncollide_amd import workloads
+    from ncollide_amd.math import to_oracle_poses
+    from ncollide_amd.warmstart import descriptors
+    w = workloads.hull_pairs(seed=44, npairs=nsample)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    dirs = mlp.predict_dirs(ss, poses, descriptors(ss), w["ia"], w["ib"])
+    runs = {"plain": (ncx.query.contact(ss, poses, w["ia"], w["ib"], 0.0), ncx.query.distance(ss, poses, w["ia"], w["ib"])[0]),
+            "learned": (ncx.query.contact(ss, poses, w["ia"], w["ib"], 0.0, init_dir=dirs, max_iter=32),
+                        ncx.query.distance(ss, poses, w["ia"], w["ib"], init_dir=dirs, max_iter=32)[0])}
+    rec = w["records"]
+    params = np.stack([rec["p0"], rec["p1"], rec["p2"]], 1)
+    o64 = orc.ShapeSet(rec["type"], params, rec["hull_offset"], rec["hull_count"], w["pool"], dtype=np.float64)
+    op = to_oracle_poses(w["poses"]).astype(np.float64)
+    threads = min(os.cpu_count() or 1, 64)
+    td, _ = orc.distance(o64, op, w["ia"], w["ib"], nthreads=threads)
+    tc = orc.contact(o64, op, w["ia"], w["ib"], 0.0, nthreads=threads)
+    out = {"pairs": nsample}
+    for key, (c, d) in runs.items():
+        d = d.cpu().numpy().astype(np.float64)
+        sep = td > 0
+        ed = np.abs(d[sep] - td[sep]) / np.maximum(td[sep], 1e-3)
+        h = c["has_contact"].cpu().numpy() & tc["has"]
+        ec = np.abs(c["depth"].cpu().numpy().astype(np.float64)[h] - tc["depth"][h]) / np.maximum(np.abs(tc["depth"][h]), 1e-3)
+        out[key] = {"distance_rel_err_p50": float(np.median(ed)), "distance_rel_err_p99": float(np.percentile(ed, 99)),
+                    "depth_rel_err_p50": float(np.median(ec)), "depth_rel_err_p99": float(np.percentile(ec, 99))}
+    return out
+
+
+def bench_scene(args, ncx, torch, ctx, mlp=None):
     """Scene step on 1,048,576 mixed shapes in 64 Gaussian clusters: shape AABBs, LBVH refit
     (rebuilt every --rebuild-every steps, default 2), pair search, contacts; each refit step replayed
     as a hipGraph.
@@ -269,11 +309,15 @@ def bench_scene(args, ncx, torch, ctx):
     ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
     vel = torch.as_tensor(w["velocities"], device=ctx.device)
 
-    def run(contacts, steps):
+    def run(contacts, steps, warm=None):
         poses = torch.as_tensor(w["poses"], device=ctx.device)
         world = World(ss, margin=w["margin"], prediction=0.0, max_pairs=args.scene_max_pairs,
                       rebuild_every=args.rebuild_every)
         world.set_contacts(contacts)
+        if warm == "learned":
+            world.set_warmstart(mlp)
+        elif warm == "temporal":
+            world.set_temporal_warmstart(True)
         times, pairs, large = [], [], 0
         for s in range(steps):
             torch.cuda.synchronize()
@@ -294,6 +338,14 @@ def bench_scene(args, ncx, torch, ctx):
                pairs_per_step=float(np.mean(pairs)), pairs_per_s=float(np.sum(pairs[1:])) / float(np.sum(times[1:])),
                epa_pairs_past_first_arena=int(large), gen_s=gen_s)
     del world
+    # the same steps with the persistent narrow phase's last_gjk_dir, and with the learned init_dir
+    for warm in ("temporal", "learned"):
+        if warm == "learned" and mlp is None:
+            continue
+        world, times, pairs, _ = run(True, args.scene_steps, warm=warm)
+        res[f"{warm}_warm_start"] = {"ms_per_step": float(np.mean(times[1:])) * 1e3,
+                                     "pairs_per_s": float(np.sum(pairs[1:])) / float(np.sum(times[1:]))}
+        del world
     # broad phase alone: steps 1..16 timed
     world, times, pairs, _ = run(False, 17)
     nodes = world.query_stats()
@@ -333,6 +385,134 @@ def bench_plain_pipeline(ncx, torch, ctx, steps=20):
         n, c = step()
     dt = (time.perf_counter() - t0) / steps
     return dict(ms_per_step=dt * 1e3, pairs=n, contacts=c)
+
+
+def cpu_baseline_plain(nsteps=20):
+    """The plain pipeline on the reference algorithms (the oracle's DBVTBroadPhase, then
+    query::contact on every pair), one thread as the reference runs: ms per step."""
+    from oracle import pyoracle as orc
+    from ncollide_amd import workloads
+    from ncollide_amd.math import to_oracle_poses
+    w = workloads.plain_pipeline(seed=1)
+    rec = w["records"]
+    params = np.stack([rec["p0"], rec["p1"], rec["p2"]], 1)
+    oss = orc.ShapeSet(rec["type"], params, rec["hull_offset"], rec["hull_count"], w["pool"], dtype=np.float32)
+    op = to_oracle_poses(w["poses"])
+    t = time.perf_counter()
+    for _ in range(nsteps):
+        bb = orc.shape_aabbs(oss, op) + np.array([-1, -1, -1, 1, 1, 1], np.float32) * np.float32(w["margin"])
+        bp = orc.BroadPhase(w["margin"])
+        for b in bb:
+            bp.create_proxy(b)
+        bp.update()
+        keys = bp.pairs()
+        ia = (keys >> np.uint64(32)).astype(np.uint32)
+        ib = (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+        orc.contact(oss, op, ia, ib, 0.0, nthreads=1)
+    ms = (time.perf_counter() - t) / nsteps * 1e3
+    return dict(value=ms, unit="ms per step", cores=1, kind="port",
+                sample="the whole 1,024-shape step (shape AABBs, DBVTBroadPhase update, query::contact per pair)")
+
+
+def cpu_baseline_scene(ncx, torch, ctx, args, nsample=65536):
+    """query::contact on a sample of one scene step's pairs on the oracle, all host threads."""
+    from oracle import pyoracle as orc
+    from ncollide_amd import workloads
+    from ncollide_amd.math import to_oracle_poses
+    from ncollide_amd.world import World
+    w = workloads.full_pipeline(seed=5, n=args.scene_shapes)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    world = World(ss, margin=w["margin"], prediction=0.0, max_pairs=args.scene_max_pairs)
+    world.set_contacts(False)
+    world.step(torch.as_tensor(w["poses"], device=ctx.device), use_graph=False)
+    world.sync()
+    r = world.results()
+    ia, ib = r["ia"].cpu().numpy().astype(np.uint32), r["ib"].cpu().numpy().astype(np.uint32)
+    rng = np.random.default_rng(0)
+    sel = rng.choice(len(ia), min(nsample, len(ia)), replace=False)
+    rec = w["records"]
+    params = np.stack([rec["p0"], rec["p1"], rec["p2"]], 1)
+    oss = orc.ShapeSet(rec["type"], params, rec["hull_offset"], rec["hull_count"], w["pool"], dtype=np.float32)
+    threads = min(os.cpu_count() or 1, 64)
+    t = time.perf_counter()
+    orc.contact(oss, to_oracle_poses(w["poses"]), ia[sel], ib[sel], 0.0, nthreads=threads)
+    dt = time.perf_counter() - t
+    return dict(value=len(sel) / dt, unit="pairs/s (query::contact)", cores=threads, kind="port",
+                sample=f"{len(sel)} random pairs of the first step's {len(ia)}, query::contact per pair")
+
+
+def bench_queries(args, ncx, torch, ctx):
+    """Throughput of the other batched queries on the same workloads: time of impact on the hull
+    pairs (with velocities toward each other), ray casts on the scene's shapes, first ray hits
+    among the 1,048,576-shape scene's proxies, and contact / distance of shapes against the
+    257x257 heightfield TriMesh."""
+    from ncollide_amd import workloads, query
+    from ncollide_amd.world import World
+    out = {}
+    dev = ctx.device
+    # time of impact: hull pairs of the batched-contact distribution, shape b moving back toward a
+    w = workloads.hull_pairs(seed=3, npairs=args.hull_pairs)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    rng = np.random.default_rng(7)
+    vel = np.zeros((len(w["records"]), 3), np.float32)
+    vel[1::2] = (-w["dirs"] * rng.uniform(0.2, 1.0, (len(w["dirs"]), 1))).astype(np.float32)
+    poses = torch.as_tensor(w["poses"], device=dev)
+    velt = torch.as_tensor(vel, device=dev)
+    ia = torch.as_tensor(w["ia"].astype(np.int32), device=dev)
+    ib = torch.as_tensor(w["ib"].astype(np.int32), device=dev)
+    t = _timed(torch, lambda: query.time_of_impact(ss, poses, velt, ia, ib, 10.0, 0.0), 5)
+    out["time_of_impact"] = {"pairs": len(w["ia"]), "ms": t * 1e3, "pairs_per_s": len(w["ia"]) / t}
+    # scene shapes: ray casts, world first interference
+    sw = workloads.full_pipeline(seed=5, n=args.scene_shapes)
+    sss = ncx.ShapeSet(records=sw["records"], pool=sw["pool"], ctx=ctx)
+    sp = torch.as_tensor(sw["poses"], device=dev)
+    nr = 1 << 20
+    idx = rng.integers(0, args.scene_shapes, nr).astype(np.int32)
+    c = sw["poses"][idx, :3]
+    o = (c + rng.normal(0, 2.0, (nr, 3))).astype(np.float32)
+    d = (c + rng.uniform(-0.2, 0.2, (nr, 3)) - o).astype(np.float32)
+    ot, dt_, it = (torch.as_tensor(a, device=dev) for a in (o, d, idx))
+    t = _timed(torch, lambda: query.cast_rays(sss, sp, it, ot, dt_, 50.0, True), 5)
+    out["shape_ray_casts"] = {"rays": nr, "ms": t * 1e3, "rays_per_s": nr / t}
+    world = World(sss, margin=sw["margin"], prediction=0.0, max_pairs=args.scene_max_pairs)
+    world.set_contacts(False)
+    world.step(sp, use_graph=False)
+    world.sync()
+    ro = np.concatenate([rng.uniform(0, 200, (nr, 2)), np.full((nr, 1), 210.0)], 1).astype(np.float32)
+    rd = np.tile(np.array([[0.05, 0.02, -1.0]], np.float32), (nr, 1))
+    t = _timed(torch, lambda: world.first_interference_with_rays(sp, ro, rd, 400.0), 3)
+    h, _, _ = world.first_interference_with_rays(sp, ro, rd, 400.0)
+    out["world_first_interference"] = {"rays": nr, "ms": t * 1e3, "rays_per_s": nr / t,
+                                       "hit_fraction": float((h != -1).float().mean().item())}
+    del world
+    # composite: the heightfield TriMesh against scene shapes hovering over it
+    pts, tri = workloads.heightfield_mesh(seed=2)
+    mesh = ncx.TriMesh(pts, tri, ctx=ctx)
+    nq = 1 << 18
+    mp = sw["poses"].copy()  # poses are indexed by shape: the first nq shapes move over the terrain
+    mp[:nq, 0:2] = rng.uniform(0.5, 25.1, (nq, 2))
+    mp[:nq, 2] = rng.uniform(-0.6, 1.2, nq)
+    mpt = torch.as_tensor(mp, device=dev)
+    qi = torch.arange(nq, dtype=torch.int32, device=dev)
+    tcn = _timed(torch, lambda: mesh.contact_with_shapes(sss, mpt, qi, 0.0), 3)
+    g = mesh.contact_with_shapes(sss, mpt, qi, 0.0)
+    tdi = _timed(torch, lambda: mesh.distance_to_shapes(sss, mpt, qi), 3)
+    out["composite_trimesh"] = {"queries": nq, "triangles": len(tri), "contact_ms": tcn * 1e3,
+                                "contact_queries_per_s": nq / tcn, "distance_ms": tdi * 1e3,
+                                "distance_queries_per_s": nq / tdi,
+                                "contact_fraction": float(g["has_contact"].float().mean().item())}
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -381,6 +561,7 @@ def main():
         cb = cpu_baseline_rays(r["pts"], r["tri"], r["o"], r["d"])
         result["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         result["cpu_baseline"]["single_thread_rays_per_s"] = cb["single_thread"]
+        result["cpu_baseline"]["cpu_model"] = cpu_model()
     if not args.quick:
         sec = {}
         want = (lambda k: args.only is None or args.only == k)
@@ -390,15 +571,28 @@ def main():
             sec["hull_pairs"] = {k: v for k, v in h.items() if k not in ("w", "gen_s")}
             if not args.no_cpu:
                 sec["hull_pairs"]["cpu_baseline"] = cpu_baseline_hulls(h["w"])
+        mlp = None
         if want("learned"):
             log("learned warm start ...")
-            sec["learned_warm_start"] = bench_learned(args, ncx, torch, ctx)
+            lw = bench_learned(args, ncx, torch, ctx)
+            mlp = lw.pop("_mlp")
+            sec["learned_warm_start"] = lw
+            if not args.no_cpu:
+                log("learned accuracy vs f64 ...")
+                lw["accuracy_vs_f64"] = cpu_accuracy_learned(ncx, torch, ctx, mlp)
         if want("scene"):
             log("scene step ...")
-            sec["scene_step"] = bench_scene(args, ncx, torch, ctx)
+            sec["scene_step"] = bench_scene(args, ncx, torch, ctx, mlp)
+            if not args.no_cpu:
+                sec["scene_step"]["cpu_baseline"] = cpu_baseline_scene(ncx, torch, ctx, args)
         if want("plain"):
             log("plain pipeline ...")
             sec["plain_pipeline"] = bench_plain_pipeline(ncx, torch, ctx)
+            if not args.no_cpu:
+                sec["plain_pipeline"]["cpu_baseline"] = cpu_baseline_plain()
+        if want("queries"):
+            log("other queries ...")
+            sec["queries"] = bench_queries(args, ncx, torch, ctx)
         result["secondary"] = sec
     print(json.dumps(result))
 
