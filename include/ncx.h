@@ -256,6 +256,27 @@ int ncx_convex_hull(const float* points_xyz, size_t npoints, uint32_t* out_ids, 
  * vertex ids (local to the cloud) go to out_ids[offsets[i] ..], their number to out_counts[i]. */
 int ncx_convex_hull_batch(const float* points_xyz, const uint32_t* offsets, const uint32_t* counts, size_t nclouds,
                           uint32_t* out_ids, uint32_t* out_counts, int nthreads);
+/* The outward triangles (counter-clockwise seen from outside) of the convex hull of the points,
+ * indexing the input: the triangulation the contact-manifold generators' hull faces are built
+ * from (ConvexHull::try_from_points hands convex_hull3's triangles to try_new,
+ * src/shape/convex.rs:88-97).  tris = NULL: *ntri only.  *ntri = 0 for a flat cloud. */
+int ncx_hull_triangulate(const float* points_xyz, size_t npoints, uint32_t* tris, size_t cap, size_t* ntri);
+/* Whether ConvexHull::try_new succeeds on each hull of a pool (hull i: its vertices
+ * pool[offsets[i] .. offsets[i] + counts[i]), triangulated as ncx_hull_triangulate): ok[i] = 1
+ * or 0.  The reference has no ConvexHull where it fails (try_from_points returns None); the
+ * contact-manifold generators need its faces. */
+int ncx_hull_polyhedra_check(const float* pool_xyz, const uint32_t* offsets, const uint32_t* counts, size_t nhulls,
+                             uint8_t* ok, int nthreads);
+/* ConvexHull::try_new (src/shape/convex.rs:107-324) on a hull's triangles: polygonal faces
+ * (normal, run of the loop arrays), loops (vertex, outgoing edge), edges (end vertices, unit
+ * direction), per vertex a run of the adjacency arrays (face, the vertex's outgoing edge in that
+ * face).  Capacities: faces <= ntri, loop and adjacency entries <= 3 ntri, edges <= 3 ntri.
+ * NCX_E_INVALID where try_new returns None.  Any output pointer may be NULL. */
+int ncx_hull_polyhedron(const float* points_xyz, size_t npoints, const uint32_t* tris, size_t ntri,
+                        float* face_normals, uint32_t* face_first, uint32_t* face_count, size_t* nfaces,
+                        uint32_t* loop_vertex, uint32_t* loop_edge, size_t* nloops, uint32_t* edge_vertices,
+                        float* edge_dirs, size_t* nedges, uint32_t* vertex_first, uint32_t* vertex_count,
+                        uint32_t* adj_face, uint32_t* adj_edge);
 
 /* Warm-started contact batch: like ncx_contact_batch with init_dir, but GJK from init_dir stops
  * after max_iter iterations and such a pair is recomputed with init_dir = None (plain GJK).
@@ -374,6 +395,41 @@ int ncx_world_first_interference_with_rays(ncx_world* world, const ncx_pose* pos
                                            uint32_t* handle_dev, float* toi_dev, float* normal_dev);
 int ncx_world_interferences_with_aabbs(ncx_world* world, const float* const aabb_soa_dev[6], size_t n,
                                        uint32_t* offsets_dev, uint32_t* handles_dev, size_t cap, size_t* total);
+
+/* ---------------------------------------------------------------- contact manifolds
+ * The narrow phase's persistent contact-manifold generators for Ball, Cuboid, Capsule and
+ * ConvexHull pairs (src/pipeline/narrow_phase/contact_generator/: DefaultContactDispatcher,
+ * default_contact_dispatcher.rs:27-97; BallBall-, BallConvexPolyhedron-, CapsuleShape-,
+ * CapsuleCapsule- and ConvexPolyhedronConvexPolyhedronManifoldGenerator) and the ContactManifold
+ * they fill (src/query/contact/contact_manifold.rs, DistanceBased(0.02) tracking), one update
+ * per pair per call as NarrowPhase::update_contact (src/pipeline/narrow_phase/narrow_phase.rs:56-80).
+ *
+ * State: one slot per interaction (a generator and its manifold in the reference): the
+ * polyhedron generator's last_gjk_dir and the tracking points of the last update's contacts.
+ * The caller maps pairs to slots (e.g. by its pair table) and resets a slot when its interaction
+ * starts.  Slots must be distinct within one call.
+ *
+ * Feature ids (FeatureId): kind in the top two bits (0 vertex, 1 edge, 2 face, 3 unknown), the
+ * shape-local id below; a capsule's features are its preprocessor's (Face 0/1 the end caps,
+ * Face 2 the side).  Each pair's contacts are written in the manifold's cache order at
+ * [k * NCX_MANIFOLD_MAX, k * NCX_MANIFOLD_MAX + count[k]) of world1/world2/normal (xyz each),
+ * depth, feature1, feature2 (device arrays); the reference iterates the same set in slab order.
+ * A hull face of more than 16 vertices, or a manifold of more than NCX_MANIFOLD_MAX tracked
+ * contacts, fails the call with NCX_E_CAPACITY. */
+#define NCX_MANIFOLD_MAX 32
+typedef struct ncx_manifolds ncx_manifolds;
+int ncx_manifolds_create(ncx_ctx* ctx, size_t nslots, ncx_manifolds** out);
+int ncx_manifolds_destroy(ncx_manifolds* m);
+/* New interactions: clears last_gjk_dir and the tracked contacts of the slots (device array). */
+int ncx_manifolds_reset(ncx_manifolds* m, const uint32_t* slots_dev, size_t n);
+/* last_gjk_dir of the slots (device arrays; zero rows: None). */
+int ncx_manifolds_dirs(ncx_manifolds* m, const uint32_t* slots_dev, size_t n, float* dirs_dev);
+/* One ContactManifoldGenerator::generate_contacts per pair, prediction (linear, angular1,
+ * angular2) as ContactPrediction. */
+int ncx_contact_manifolds(ncx_manifolds* m, ncx_shapes* s, const ncx_pose* poses_dev, const uint32_t* ia_dev,
+                          const uint32_t* ib_dev, const uint32_t* slots_dev, size_t npairs, float linear,
+                          float angular1, float angular2, uint32_t* count_dev, float* world1_dev, float* world2_dev,
+                          float* normal_dev, float* depth_dev, uint32_t* feature1_dev, uint32_t* feature2_dev);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,8 @@ from .context import Context, default_context
 from .shape import Ball, Cuboid, Capsule, ConvexHull, ShapeSet, BALL, CUBOID, CAPSULE, CONVEX_HULL
 from .broad_phase import DBVTBroadPhase
 from .ray import TriMesh, NO_HIT
-from . import query, workloads, math
+from .manifold import ContactManifolds
+from . import query, workloads, math, manifold
 
 __all__ = ["NcxError", "load_library", "Context", "default_context", "Ball", "Cuboid", "Capsule", "ConvexHull",
            "ShapeSet", "DBVTBroadPhase", "TriMesh", "NO_HIT", "query", "workloads", "math",

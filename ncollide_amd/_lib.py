@@ -47,6 +47,14 @@ SIGNATURES = {
     "ncx_shapes_cast_rays": (C.c_int, [P, P, P, P, SZ, F32, C.c_int, P, P, P]),
     "ncx_mesh_contact_batch": (C.c_int, [P, P, P, P, P, SZ, F32, P, P, P, P, P, P]),
     "ncx_mesh_distance_batch": (C.c_int, [P, P, P, P, P, SZ, P]),
+    "ncx_hull_triangulate": (C.c_int, [P, SZ, P, SZ, P]),
+    "ncx_hull_polyhedra_check": (C.c_int, [P, P, P, SZ, P, C.c_int]),
+    "ncx_hull_polyhedron": (C.c_int, [P, SZ, P, SZ, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "ncx_manifolds_create": (C.c_int, [P, SZ, P]),
+    "ncx_manifolds_destroy": (C.c_int, [P]),
+    "ncx_manifolds_reset": (C.c_int, [P, P, SZ]),
+    "ncx_manifolds_dirs": (C.c_int, [P, P, SZ, P]),
+    "ncx_contact_manifolds": (C.c_int, [P, P, P, P, P, P, SZ, F32, F32, F32, P, P, P, P, P, P, P]),
     "ncx_trimesh_create": (C.c_int, [P, P, SZ, P, SZ, P]),
     "ncx_trimesh_destroy": (C.c_int, [P]),
     "ncx_trimesh_cast_rays": (C.c_int, [P, P, P, SZ, F32, P, P, P]),

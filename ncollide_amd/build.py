@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libncx.so")
 OBJ = os.path.join(CSRC, "_build")
 SOURCES = ["ncx_core.hip", "ncx_raycast.hip", "ncx_broadphase.hip", "ncx_narrow.hip", "ncx_mlp.hip",
-           "ncx_world.hip", "ncx_toi.hip", "ncx_composite.hip", "ncx_hull.cpp"]
+           "ncx_world.hip", "ncx_toi.hip", "ncx_composite.hip", "ncx_manifold.hip", "ncx_hull.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -700,6 +700,9 @@ __device__ __forceinline__ PairOut contact_ball_convex(f3 c1, float r, const iso
   f3 tri_n = f3{0.f, 1.f, 0.f};  // Triangle::feature_normal: its normal, or +y when degenerate
   if (g2.type == NCX_CUBOID) {
     world2 = project_cuboid(m2, f3{g2.a, g2.b, g2.c}, c1, inside, fkind, fid);
+  } else if (g2.type == SHAPE_SEGMENT) {
+    // a capsule's segment in the contact-manifold generators: Segment::project_point_with_feature
+    world2 = proj_segment_at(m2, f3{0.f, -g2.a, 0.f}, f3{0.f, g2.a, 0.f}, c1, fkind, fid, inside);
   } else if (g2.type == SHAPE_TRIANGLE) {
     // Triangle::project_point_with_feature, solid in 3D (src/query/point/point_triangle.rs:29-51);
     // every location names a feature, so the zero-distance branch below always has a normal
@@ -751,6 +754,9 @@ __device__ __forceinline__ PairOut contact_ball_convex(f3 c1, float r, const iso
     if (g2.type == SHAPE_TRIANGLE) {
       depth = r;
       normal = -tri_n;
+    } else if (g2.type == SHAPE_SEGMENT) {  // a segment location always names a feature
+      depth = r;
+      normal = -segment_feature_normal(f3{0.f, -g2.a, 0.f}, f3{0.f, g2.a, 0.f}, fkind, fid);
     } else {
       if (g2.type != NCX_CUBOID || fkind == 3) return o;
       depth = r;
@@ -772,8 +778,13 @@ template <int G, int PAR>
 __device__ __forceinline__ PairOut contact_pair(const Ctx2& c, float prediction, bool has_init, f3 init_dir, const Pool& P,
                                 int gl, const Arena& A, int max_iter, const EpaIO& io) {
   bool b1 = c.g1.type == NCX_BALL, b2 = c.g2.type == NCX_BALL;
-  bool poly1 = c.g1.type == NCX_CUBOID || c.g1.type == NCX_CONVEX_HULL || c.g1.type == SHAPE_TRIANGLE;
-  bool poly2 = c.g2.type == NCX_CUBOID || c.g2.type == NCX_CONVEX_HULL || c.g2.type == SHAPE_TRIANGLE;
+  // ConvexPolyhedron shapes: a ball against one takes the point-projection path (a capsule's
+  // segment appears only in the contact-manifold generators, where the ball-polyhedron
+  // generator handles it the same way)
+  bool poly1 = c.g1.type == NCX_CUBOID || c.g1.type == NCX_CONVEX_HULL || c.g1.type == SHAPE_TRIANGLE ||
+               c.g1.type == SHAPE_SEGMENT;
+  bool poly2 = c.g2.type == NCX_CUBOID || c.g2.type == NCX_CONVEX_HULL || c.g2.type == SHAPE_TRIANGLE ||
+               c.g2.type == SHAPE_SEGMENT;
   if (b1 && b2) {
     // contact_ball_ball (contact_ball_ball.rs:8-38)
     PairOut o;

@@ -48,6 +48,10 @@ struct Pool {
 // A part of a composite shape: triangle `off` of Pool::tri (TriMesh::triangle_at,
 // src/shape/trimesh.rs:471-475), in the mesh's frame.
 #define SHAPE_TRIANGLE 5u
+// The core segment of a capsule (Capsule::segment, src/shape/capsule.rs:54-61): a = (0, -a, 0),
+// b = (0, a, 0) with `a` the capsule's half height.  The contact-manifold generators replace a
+// capsule by it (capsule_shape_manifold_generator.rs:34-46).
+#define SHAPE_SEGMENT 6u
 
 __device__ __forceinline__ void tri_vertices(const Pool& P, uint32_t k, f3& a, f3& b, f3& c) {
   const float4* t = P.tri + 3 * (size_t)k;
@@ -140,6 +144,11 @@ __device__ __forceinline__ f3 support(const ShapeD& g, const iso& m, f3 dir, con
       const float d1 = dot(a, ld), d2 = dot(b, ld), d3 = dot(c, ld);
       const f3 p = d1 > d2 ? (d1 > d3 ? a : c) : (d2 > d3 ? b : c);
       return xform_pt(m, p);
+    }
+    case SHAPE_SEGMENT: {  // Segment::local_support_point (src/shape/segment.rs:182-191)
+      f3 ld = inv_xform_vec(m, dir);
+      const f3 a = f3{0.f, -g.a, 0.f}, b = f3{0.f, g.a, 0.f};
+      return xform_pt(m, dot(a, ld) > dot(b, ld) ? a : b);
     }
     default:
       return f3{0.f, 0.f, 0.f};
@@ -858,6 +867,44 @@ __device__ __forceinline__ f3 proj_triangle_at(const iso& m, f3 a, f3 b, f3 c, f
   }
   inside = relative_eq_f(res.x, pt.x) && relative_eq_f(res.y, pt.y) && relative_eq_f(res.z, pt.z);
   return res;
+}
+
+// Segment::project_point_with_location (src/query/point/point_segment.rs:49-91) for the segment
+// (a, b) placed by m, and project_point_with_feature's feature (:13-41): vertex 0 or 1 (kind
+// LOC_VERTEX) or the edge (LOC_EDGE).  *inside = relative_eq!(projection, pt).
+__device__ __forceinline__ f3 proj_segment_at(const iso& m, f3 a, f3 b, f3 pt, int& kind, int& id, bool& inside) {
+  const f3 ls = inv_xform_pt(m, pt);
+  const f3 ab = b - a, ap = ls - a;
+  const float ab_ap = dot(ab, ap), sqnab = norm_sq(ab);
+  f3 proj;
+  if (ab_ap <= 0.f) { kind = LOC_VERTEX; id = 0; proj = xform_pt(m, a); }
+  else if (ab_ap >= sqnab) { kind = LOC_VERTEX; id = 1; proj = xform_pt(m, b); }
+  else {
+    const float u = ab_ap / sqnab;
+    kind = LOC_EDGE; id = 0;
+    proj = xform_pt(m, a + ab * u);
+  }
+  inside = relative_eq_f(proj.x, pt.x) && relative_eq_f(proj.y, pt.y) && relative_eq_f(proj.z, pt.z);
+  return proj;
+}
+
+// Segment::feature_normal (src/shape/segment.rs:238-273), local frame, for the segment (a, b):
+// vertex 0 -> its direction, vertex 1 -> minus it, the edge -> the unit vector along the
+// direction's smallest component with the direction's part removed; +y when degenerate.
+__device__ __forceinline__ f3 segment_feature_normal(f3 a, f3 b, int kind, int id) {
+  f3 d;
+  float nl;
+  if (!try_normalize(b - a, NCX_FLT_EPS, &d, &nl)) return f3{0.f, 1.f, 0.f};
+  if (kind == LOC_VERTEX) return id == 0 ? d : -d;
+  // Vector::iamin: the first index of the smallest |component|
+  int im = 0;
+  float best = fabsf(d.x);
+  if (fabsf(d.y) < best) { best = fabsf(d.y); im = 1; }
+  if (fabsf(d.z) < best) im = 2;
+  f3 n = f3{0.f, 0.f, 0.f};
+  set_comp(n, im, 1.f);
+  n = n - d * comp(d, im);
+  return normalize(n);
 }
 
 // Quaternion product (nalgebra's Quaternion * Quaternion), coordinates (i, j, k, w).

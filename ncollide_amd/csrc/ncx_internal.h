@@ -122,6 +122,12 @@ struct ncx_shapes {
   ContactTiers tiers; // for ncx_contact_batch / ncx_distance_batch
   DevBuf qerr;        // error flag of the time-of-impact, ray-cast and composite queries
   const float4* tri = nullptr;  // triangles of the Triangle parts (a mesh's combined set only)
+  // polygonal faces, edges and adjacency of the hulls (ConvexHull::try_new), built on the first
+  // contact-manifold call: per shape (face base, face count, edge base, 0); per face normal and
+  // loop run; loop entries (vertex, edge); per edge direction and end vertices; per pool point
+  // the adjacency run; adjacency entries (face, edge).  Ids inside a hull are local.
+  bool hf_ready = false;
+  DevBuf hf_shape, hf_fn, hf_floop, hf_loop, hf_edir, hf_ev, hf_vadj, hf_adj;
   // autotuner choice per query (0 contact, 1 distance) and bucket: an index into the kernel
   // variants (group width 8, 16, 32), or -1 before tuning
   int8_t tune[2][NCX_NBUCKETS] = {{-1, -1, -1, -1}, {-1, -1, -1, -1}};
@@ -288,7 +294,7 @@ int contacts_launch_device_count(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, c
                                  const uint32_t* ia, const uint32_t* ib, const uint32_t* n_dev, size_t cap,
                                  float prediction, const float* init_dir, int max_iter, uint8_t* has, float* w1,
                                  float* w2, float* nrm, float* depth, int32_t* iters, bool may_tune,
-                                 float* fdir = nullptr);
+                                 float* fdir = nullptr, bool cap_seg = false);
 // Host side after that launch completed: checks, re-runs what both LDS tiers could not hold,
 // and reports the number of pairs that needed more than the first-tier arena.
 int contacts_resolve_overflow(ContactTiers& T, ncx_ctx* ctx, size_t* n_large);

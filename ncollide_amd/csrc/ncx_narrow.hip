@@ -30,6 +30,8 @@ struct KArgs {
   float prediction;
   const float* init_dir;
   int max_iter;
+  int cap_seg;  // contact-manifold mode: a capsule is its core segment, and each capsule of a pair
+                // adds its radius to the pair's prediction
   uint8_t* has;
   float* w1;
   float* w2;
@@ -145,6 +147,13 @@ __global__ __launch_bounds__(BS, MODE == 1 ? 3 : 2) void k_pairs(KArgs a) {
     c.g2 = load_shape(a.shapes, i2);
     c.m1 = load_pose(a.poses, i1);
     c.m2 = load_pose(a.poses, i2);
+    float pred = a.prediction;
+    if (MODE == 0 && a.cap_seg) {
+      // CapsuleShapeManifoldGenerator / CapsuleCapsuleManifoldGenerator
+      // (capsule_shape_manifold_generator.rs:34-37, capsule_capsule_manifold_generator.rs:36-40)
+      if (c.g1.type == NCX_CAPSULE) { pred = pred + c.g1.b; c.g1.type = SHAPE_SEGMENT; }
+      if (c.g2.type == NCX_CAPSULE) { pred = pred + c.g2.b; c.g2.type = SHAPE_SEGMENT; }
+    }
     if (MODE == 1) {  // distance (a compile-time branch: each kernel holds one query's code)
       float d = 0.f;
       int it = 0;
@@ -200,12 +209,12 @@ __global__ __launch_bounds__(BS, MODE == 1 ? 3 : 2) void k_pairs(KArgs a) {
       // An all-zero entry means None for this pair (the reference's default direction).
       has_init = idir.x != 0.f || idir.y != 0.f || idir.z != 0.f;
     }
-    PairOut o = contact_pair<G, LDS ? 1 : 2>(c, a.prediction, has_init, idir, a.P, gl, A, has_init ? a.max_iter : 10000, io);
+    PairOut o = contact_pair<G, LDS ? 1 : 2>(c, pred, has_init, idir, a.P, gl, A, has_init ? a.max_iter : 10000, io);
     int extra_iters = 0;
     if (o.has == 3) {
       // Warm start hit its iteration cap: plain GJK (init_dir = None) judges the pair.
       extra_iters = o.gjk_iters;
-      o = contact_pair<G, LDS ? 1 : 2>(c, a.prediction, false, idir, a.P, gl, A, 10000, io);
+      o = contact_pair<G, LDS ? 1 : 2>(c, pred, false, idir, a.P, gl, A, 10000, io);
     }
     if (o.has == 4) continue;  // suspended: listed (and saved) for the next tier
     if (o.has == 2) {
@@ -283,6 +292,8 @@ extern "C" int ncx_shapes_destroy(ncx_shapes* s) {
   if (!s) return NCX_OK;
   s->shapes.release(); s->hx.release(); s->hy.release(); s->hz.release(); contact_tiers_release(s->tiers);
   s->qerr.release();
+  s->hf_shape.release(); s->hf_fn.release(); s->hf_floop.release(); s->hf_loop.release();
+  s->hf_edir.release(); s->hf_ev.release(); s->hf_vadj.release(); s->hf_adj.release();
   delete s;
   return NCX_OK;
 }
@@ -837,9 +848,11 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
 int contacts_launch_device_count(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const ncx_pose* poses,
                                  const uint32_t* ia, const uint32_t* ib, const uint32_t* n_dev, size_t cap,
                                  float prediction, const float* init_dir, int max_iter, uint8_t* has, float* w1,
-                                 float* w2, float* nrm, float* depth, int32_t* iters, bool may_tune, float* fdir) {
+                                 float* w2, float* nrm, float* depth, int32_t* iters, bool may_tune, float* fdir,
+                                 bool cap_seg) {
   KArgs a = {};
   a.fdir = fdir;
+  a.cap_seg = cap_seg ? 1 : 0;
   a.shapes = S->shapes.as<ncx_shape>();
   a.P = Pool{S->hx.as<float>(), S->hy.as<float>(), S->hz.as<float>(), S->tri};
   a.poses = poses;

@@ -231,7 +231,32 @@ def step_poses(poses, v, w):
 
 
 # ------------------------------------------------------------------ full pipeline (mixed shapes in clusters)
-def full_pipeline(seed=5, n=1 << 20, nclusters=64, sigma=5.0, box=200.0, hull_nmax=64):
+def polyhedral_hulls(rng, nclouds, nmin=8, nmax=64, max_rounds=20):
+    """Hull pools like build_hulls(ellipsoid_clouds(...)) where every hull has a polyhedral
+    structure: a cloud whose ConvexHull::try_new fails (merged nearly coplanar faces break
+    Euler's characteristic; the reference's try_from_points returns None for it) is drawn
+    again.  Returns (pool, hull_offset, hull_count)."""
+    from .hull import polyhedra_ok
+    pts, offs, cnts = ellipsoid_clouds(rng, nclouds, nmin, nmax)
+    clouds = [pts[o:o + c] for o, c in zip(offs, cnts)]
+    for _ in range(max_rounds):
+        cnt = np.array([len(c) for c in clouds], np.uint32)
+        off = np.zeros(len(cnt), np.uint32)
+        off[1:] = np.cumsum(cnt)[:-1]
+        pool, hoff, hcnt = build_hulls(np.concatenate(clouds), off, cnt)
+        ok = polyhedra_ok(pool, hoff, hcnt)
+        if ok.all():
+            return pool, hoff, hcnt
+        bad = np.nonzero(~ok)[0]
+        p2, o2, c2 = ellipsoid_clouds(rng, len(bad), nmin, nmax)
+        for j, b in enumerate(bad):
+            clouds[b] = p2[o2[j]:o2[j] + c2[j]]
+    raise RuntimeError("hull clouds kept failing ConvexHull::try_new")
+
+
+def full_pipeline(seed=5, n=1 << 20, nclusters=64, sigma=5.0, box=200.0, hull_nmax=64, polyhedra=False):
+    """Mixed shapes in Gaussian clusters.  polyhedra: every hull has a polyhedral structure (the
+    contact-manifold generators need one; see polyhedral_hulls)."""
     rng = np.random.default_rng(seed)
     types = rng.integers(0, 4, n).astype(np.uint32)
     rec = np.zeros(n, SHAPE_DTYPE)
@@ -246,8 +271,11 @@ def full_pipeline(seed=5, n=1 << 20, nclusters=64, sigma=5.0, box=200.0, hull_nm
     rec["p1"][ncap] = rng.uniform(0.05, 0.3, ncap.sum())
     nh = types == CONVEX_HULL
     hidx = np.nonzero(nh)[0]
-    pts, offs, cnts = ellipsoid_clouds(rng, len(hidx), 8, hull_nmax)
-    pool, hoff, hcnt = build_hulls(pts, offs, cnts)
+    if polyhedra:
+        pool, hoff, hcnt = polyhedral_hulls(rng, len(hidx), 8, hull_nmax)
+    else:
+        pts, offs, cnts = ellipsoid_clouds(rng, len(hidx), 8, hull_nmax)
+        pool, hoff, hcnt = build_hulls(pts, offs, cnts)
     rec["hull_offset"][hidx] = hoff
     rec["hull_count"][hidx] = hcnt
     centres = rng.uniform(0, box, (nclusters, 3))
@@ -256,3 +284,90 @@ def full_pipeline(seed=5, n=1 << 20, nclusters=64, sigma=5.0, box=200.0, hull_nm
     q = uniform_quats(rng, n)
     vel = ball_samples(rng, n, 0.5).astype(np.float32)
     return dict(records=rec, pool=pool, poses=poses8(t, q), velocities=vel, margin=0.02, prediction=0.0)
+
+
+# ------------------------------------------------------------------ resting stacks (contact manifolds)
+def resting_stacks(seed=7, ncols=16, height=4, spacing=3.0, hull_share=0.25):
+    """Columns of shapes resting on a ground cuboid, each a little into the one below, with
+    small tilts: face-on-face contacts whose manifolds have several points.  Shape 0 is the
+    ground (a cuboid of half extents (ncols * spacing / 2 + 2, ncols * spacing / 2 + 2, 0.5),
+    top face at z = 0).  Columns stand on a ncols x ncols grid; each level is a cuboid (a hull
+    for hull_share of them) or, at the top, a ball or a lying capsule."""
+    rng = np.random.default_rng(seed)
+    ncol = ncols * ncols
+    n = 1 + ncol * height
+    rec = np.zeros(n, SHAPE_DTYPE)
+    half = ncols * spacing / 2 + 2.0
+    rec["type"][0] = CUBOID
+    rec["p0"][0], rec["p1"][0], rec["p2"][0] = half, half, 0.5
+    t = np.zeros((n, 3), np.float32)
+    q = np.zeros((n, 4), np.float32)
+    q[:, 3] = 1.0
+    t[0] = (half - 2.0 - spacing / 2, half - 2.0 - spacing / 2, -0.5)
+    gx, gy = np.meshgrid(np.arange(ncols), np.arange(ncols), indexing="ij")
+    cx, cy = (gx.ravel() * spacing).astype(np.float64), (gy.ravel() * spacing).astype(np.float64)
+    types = np.full((ncol, height), CUBOID, np.uint32)
+    hullmask = rng.random((ncol, height)) < hull_share
+    types[hullmask] = CONVEX_HULL
+    top = rng.integers(0, 3, ncol)  # the top level: 0 cuboid/hull as drawn, 1 ball, 2 capsule
+    types[top == 1, height - 1] = BALL
+    types[top == 2, height - 1] = CAPSULE
+    nh = int((types == CONVEX_HULL).sum())
+    # hulls: boxes with slightly perturbed corners plus a few points on the faces (flat faces)
+    hpts, hoff, hcnt = [], np.zeros(nh, np.uint32), np.zeros(nh, np.uint32)
+    he_h = rng.uniform(0.4, 0.9, (nh, 3))
+    corners = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], np.float64)
+    from .hull import convex_hull_vertices, polyhedra_ok
+    off = 0
+    for i in range(nh):
+        for _ in range(50):
+            c = corners * he_h[i] * (1 + rng.uniform(-0.01, 0.01, (8, 3)))
+            extra = rng.uniform(-1, 1, (6, 3)) * he_h[i] * 0.7
+            cl = np.concatenate([c, extra]).astype(np.float32)
+            ids = convex_hull_vertices(cl)
+            hp = cl[ids]
+            if polyhedra_ok(hp, np.zeros(1, np.uint32), np.array([len(hp)], np.uint32))[0]:
+                break
+        hpts.append(hp)
+        hoff[i], hcnt[i] = off, len(hp)
+        off += len(hp)
+    pool = np.concatenate(hpts).astype(np.float32) if nh else np.zeros((0, 3), np.float32)
+    hi = 0
+    idx = 1
+    for c in range(ncol):
+        z = 0.0
+        for lv in range(height):
+            ty = types[c, lv]
+            rec["type"][idx] = ty
+            tilt = rng.normal(0, 0.02, 3)
+            tilt[2] = rng.uniform(-np.pi, np.pi)  # yaw about z
+            if ty == CUBOID:
+                he = rng.uniform(0.4, 0.9, 3)
+                rec["p0"][idx], rec["p1"][idx], rec["p2"][idx] = he
+                hz = he[2]
+            elif ty == CONVEX_HULL:
+                rec["hull_offset"][idx], rec["hull_count"][idx] = hoff[hi], hcnt[hi]
+                hz = float(np.max(np.abs(pool[hoff[hi]:hoff[hi] + hcnt[hi], 2])))
+                hi += 1
+            elif ty == BALL:
+                r = rng.uniform(0.3, 0.7)
+                rec["p0"][idx] = r
+                hz = r
+            else:  # a capsule lying along x: rotate its local y axis onto x
+                hh, r = rng.uniform(0.3, 0.8), rng.uniform(0.2, 0.5)
+                rec["p0"][idx], rec["p1"][idx] = hh, r
+                hz = r
+                tilt = np.array([0.0, 0.0, -np.pi / 2]) + rng.normal(0, 0.02, 3)
+            pen = rng.uniform(0.0, 0.04)
+            zc = z + hz - pen
+            t[idx] = (cx[c] + rng.normal(0, 0.05), cy[c] + rng.normal(0, 0.05), zc)
+            # rotation: small tilts about x and y, then the yaw (axis-angle composition)
+            ang = np.linalg.norm(tilt)
+            ax = tilt / ang if ang > 0 else np.array([0.0, 0.0, 1.0])
+            q[idx, :3] = ax * np.sin(ang / 2)
+            q[idx, 3] = np.cos(ang / 2)
+            z = zc + hz
+            idx += 1
+    vel = rng.normal(0, 0.01, (n, 3)).astype(np.float32)
+    vel[0] = 0.0
+    return dict(records=rec, pool=pool, poses=poses8(t, q.astype(np.float32)), velocities=vel)

@@ -99,7 +99,10 @@ template <class R> inline void quat_to_mat(const Quat<R>& q, R m[3][3]) {
 template <class R> inline R gjk_eps_tol() { return Eps<R>::eps() * R(10); }  // gjk.rs:26-29
 
 // ---------------------------------------------------------------- shapes
-enum ShapeType : uint32_t { BALL = 0, CUBOID = 1, CAPSULE = 2, HULL = 3, ORIGIN = 4, TRIANGLE = 5 };
+// SEGMENT: a capsule's core segment (Capsule::segment, shape/capsule.rs:54-61), a = (0, -a, 0),
+// b = (0, a, 0); the contact-manifold generators use it in place of the capsule.
+enum ShapeType : uint32_t { BALL = 0, CUBOID = 1, CAPSULE = 2, HULL = 3, ORIGIN = 4, TRIANGLE = 5, SEGMENT = 6 };
+template <class R> struct HullPolyO;
 // TRIANGLE: a part of a TriMesh (TriMesh::triangle_at, shape/trimesh.rs:471-475); pts = its three
 // vertices a, b, c in the mesh frame.
 
@@ -108,6 +111,7 @@ template <class R> struct Shape {
   R a, b, c;              // ball: r; cuboid: hx,hy,hz; capsule: half_height, radius
   const V3<R>* pts;       // hull vertices (local frame)
   uint32_t npts;
+  const HullPolyO<R>* poly = nullptr;  // hull faces, edges and adjacency (contact manifolds only)
 };
 
 // utils/point_cloud_support_point.rs:6-24 (strict '>' keeps the first maximum)
@@ -152,6 +156,11 @@ template <class R> inline V3<R> support_point(const Shape<R>& g, const Iso<R>& m
       R d1 = dot(g.pts[0], ld), d2 = dot(g.pts[1], ld), d3 = dot(g.pts[2], ld);
       V3<R> p = d1 > d2 ? (d1 > d3 ? g.pts[0] : g.pts[2]) : (d2 > d3 ? g.pts[1] : g.pts[2]);
       return xform_pt(m, p);
+    }
+    case SEGMENT: {  // Segment::local_support_point (shape/segment.rs:182-191)
+      V3<R> ld = inv_xform_vec(m, dir);
+      V3<R> a(R(0), -g.a, R(0)), b(R(0), g.a, R(0));
+      return xform_pt(m, dot(a, ld) > dot(b, ld) ? a : b);
     }
     default:
       return V3<R>(0, 0, 0);
@@ -974,13 +983,26 @@ template <class R> inline V3<R> project_triangle_at(const Iso<R>& m, V3<R> a, V3
 // contact_ball_convex_polyhedron (contact_ball_convex_polyhedron.rs:11-58) for Cuboid and ConvexHull.
 // For ConvexHull, the reference's zero-distance branch needs the hull's face/edge adjacency
 // (convex.rs:388-415); this restatement returns None there (ball centre within eps of the surface).
-template <class R> inline bool contact_ball_convex(V3<R> c1, R r, const Iso<R>& m2, const Shape<R>& g2, R prediction, Contact<R>* out) {
+template <class R> inline bool contact_ball_convex(V3<R> c1, R r, const Iso<R>& m2, const Shape<R>& g2, R prediction,
+                                                   Contact<R>* out, int* fkind_out = nullptr, int* fid_out = nullptr,
+                                                   bool* inside_out = nullptr) {
   bool inside;
   V3<R> world2;
   int fkind = 3, fid = 0;
   V3<R> tri_n(0, 1, 0);  // Triangle::feature_normal: its normal, or +y when degenerate (triangle.rs:367-374)
   if (g2.type == CUBOID) world2 = project_point_cuboid(m2, V3<R>(g2.a, g2.b, g2.c), c1, &inside, &fkind, &fid);
-  else if (g2.type == TRIANGLE) {
+  else if (g2.type == SEGMENT) {
+    // Segment::project_point_with_feature (query/point/point_segment.rs:13-91): kind 2 vertex
+    // (id 0 or 1) or 1 edge (id 0), as project_point_cuboid's kinds
+    V3<R> a(R(0), -g2.a, R(0)), b(R(0), g2.a, R(0));
+    V3<R> ls = inv_xform_pt(m2, c1);
+    V3<R> ab = b - a, ap = ls - a;
+    R ab_ap = dot(ab, ap), sqnab = norm_sq(ab);
+    if (ab_ap <= R(0)) { fkind = 2; fid = 0; world2 = xform_pt(m2, a); }
+    else if (ab_ap >= sqnab) { fkind = 2; fid = 1; world2 = xform_pt(m2, b); }
+    else { R u = ab_ap / sqnab; fkind = 1; fid = 0; world2 = xform_pt(m2, a + ab * u); }
+    inside = relative_eq_r(world2.x, c1.x) && relative_eq_r(world2.y, c1.y) && relative_eq_r(world2.z, c1.z);
+  } else if (g2.type == TRIANGLE) {
     world2 = project_triangle_at(m2, g2.pts[0], g2.pts[1], g2.pts[2], c1, &inside);
     V3<R> nn;
     if (try_normalize(cross(g2.pts[1] - g2.pts[0], g2.pts[2] - g2.pts[0]), Eps<R>::eps(), &nn)) tri_n = nn;
@@ -994,11 +1016,29 @@ template <class R> inline bool contact_ball_convex(V3<R> c1, R r, const Iso<R>& 
   } else if (g2.type == TRIANGLE) {  // every triangle location names a feature
     depth = r;
     normal = -tri_n;
+  } else if (g2.type == SEGMENT) {  // Segment::feature_normal (shape/segment.rs:238-273), local frame
+    V3<R> d;
+    if (!try_normalize(V3<R>(R(0), g2.a, R(0)) - V3<R>(R(0), -g2.a, R(0)), Eps<R>::eps(), &d)) normal = V3<R>(0, 1, 0);
+    else if (fkind == 2) normal = fid == 0 ? d : -d;
+    else {
+      int im = 0;  // Vector::iamin: first index of the smallest |component|
+      R best = std::fabs(d.x);
+      if (std::fabs(d.y) < best) { best = std::fabs(d.y); im = 1; }
+      if (std::fabs(d.z) < best) im = 2;
+      V3<R> nn(0, 0, 0);
+      nn[im] = R(1);
+      normal = normalize(nn - d * d[im]);
+    }
+    depth = r;
+    normal = -normal;
   } else {
     if (g2.type != CUBOID || fkind == 3) return false;
     depth = r;
     normal = -cuboid_feature_normal<R>(fkind, fid);
   }
+  if (fkind_out) *fkind_out = fkind;
+  if (fid_out) *fid_out = fid;
+  if (inside_out) *inside_out = inside;
   if (depth >= -prediction) {
     out->world1 = c1 + normal * r;
     out->world2 = world2;
@@ -2100,6 +2140,610 @@ template <class R> inline R distance_mesh_shape(const Iso<R>& m1, const V3<R>* t
     if (d < best) best = d;
   }
   return best;
+}
+
+
+// ================================================================ contact manifolds
+// The narrow phase's contact-manifold generators for Ball, Cuboid, Capsule and ConvexHull
+// (pipeline/narrow_phase/contact_generator/*), the polygonal-feature clipping they use
+// (shape/convex_polygonal_feature3.rs) and ContactManifold's distance-based contact tracking
+// (query/contact/contact_manifold.rs).
+//
+// Feature ids: kind in the top two bits (0 vertex, 1 edge, 2 face, 3 unknown), id below.
+constexpr uint32_t FEAT_V = 0u, FEAT_E = 1u, FEAT_F = 2u;
+constexpr uint32_t FEAT_NONE = 0xC0000000u;  // FeatureId::Unknown
+inline uint32_t feat(uint32_t kind, uint32_t id) { return (kind << 30) | id; }
+inline uint32_t feat_kind(uint32_t f) { return f >> 30; }
+inline uint32_t feat_id(uint32_t f) { return f & 0x3FFFFFFFu; }
+
+// ConvexHull::try_new (shape/convex.rs:107-324): polygonal faces from the hull's outward
+// triangles.  An edge between triangles whose normals' dot exceeds 1 - sqrt(eps) is deleted; a
+// face is the contour walked around deleted edges, with its first triangle's normal.
+template <class R> struct HullPolyO {
+  std::vector<V3<R>> fn;                 // face normals
+  std::vector<uint32_t> ffirst, fnum;    // face loops in lvert / ledge
+  std::vector<uint32_t> lvert, ledge;    // vertices_adj_to_face, edges_adj_to_face
+  std::vector<uint32_t> ev0, ev1;        // edge end vertices
+  std::vector<V3<R>> edir;               // edge unit directions
+  std::vector<uint32_t> vfirst, vnum;    // per vertex: adjacency run
+  std::vector<uint32_t> vface, vedge;    // faces_adj_to_vertex, edges_adj_to_vertex
+};
+
+template <class R> inline bool hull_poly_build_o(const V3<R>* pts, uint32_t npts, const uint32_t* tris, uint32_t ntri,
+                                                 HullPolyO<R>& H) {
+  H = HullPolyO<R>();
+  const R eps = std::sqrt(Eps<R>::eps());
+  struct E { uint32_t v0, v1, f0, f1; V3<R> dir; bool del; };
+  struct T { uint32_t v[3], e[3]; V3<R> n; int64_t parent; };
+  std::vector<E> edges;
+  std::vector<T> tri(ntri);
+  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> emap(npts);  // SortedPair -> edge
+  for (uint32_t t = 0; t < ntri; ++t) {
+    const uint32_t* vtx = tris + 3 * (size_t)t;
+    for (int i = 0; i < 3; ++i)
+      if (vtx[i] >= npts) return false;
+    for (int i1 = 0; i1 < 3; ++i1) {
+      int i2 = (i1 + 1) % 3;
+      uint32_t lo = std::min(vtx[i1], vtx[i2]), hi = std::max(vtx[i1], vtx[i2]);
+      int64_t found = -1;
+      for (auto& e : emap[lo]) if (e.first == hi) found = e.second;
+      if (found >= 0) {
+        tri[t].e[i1] = (uint32_t)found;
+        edges[(size_t)found].f1 = t;
+      } else {
+        V3<R> d;
+        if (!try_normalize(pts[vtx[i2]] - pts[vtx[i1]], Eps<R>::eps(), &d)) return false;
+        tri[t].e[i1] = (uint32_t)edges.size();
+        emap[lo].push_back({hi, (uint32_t)edges.size()});
+        edges.push_back(E{vtx[i1], vtx[i2], t, 0, d, false});
+      }
+    }
+    V3<R> n;  // utils::ccw_face_normal (utils/ccw_face_normal.rs:21-27)
+    if (!try_normalize(cross(pts[vtx[1]] - pts[vtx[0]], pts[vtx[2]] - pts[vtx[0]]), Eps<R>::eps(), &n)) return false;
+    for (int k = 0; k < 3; ++k) tri[t].v[k] = vtx[k];
+    tri[t].n = n;
+    tri[t].parent = -1;
+  }
+  int64_t valid_edges = 0;
+  for (auto& e : edges) {
+    if (dot(tri[e.f0].n, tri[e.f1].n) > R(1) - eps) e.del = true;
+    else valid_edges++;
+  }
+  std::vector<uint32_t> face_tri;
+  for (uint32_t i = 0; i < ntri; ++i) {
+    if (tri[i].parent >= 0) continue;
+    for (int j1 = 0; j1 < 3; ++j1) {
+      if (edges[tri[i].e[j1]].del) continue;
+      uint32_t face_id = (uint32_t)H.ffirst.size(), first = (uint32_t)H.ledge.size(), num = 1;
+      H.ledge.push_back(tri[i].e[j1]);
+      H.lvert.push_back(tri[i].v[j1]);
+      uint32_t start = tri[i].v[j1], ct = i;
+      int ce = (j1 + 1) % 3;
+      size_t steps = 0;
+      while (tri[ct].v[ce] != start) {
+        if (++steps > 8 * (size_t)ntri + 16) return false;
+        uint32_t cur_e = tri[ct].e[ce], cur_v = tri[ct].v[ce];
+        tri[ct].parent = face_id;
+        if (!edges[cur_e].del) {
+          H.ledge.push_back(cur_e);
+          H.lvert.push_back(cur_v);
+          num++;
+          ce = (ce + 1) % 3;
+        } else {
+          ct = ct == edges[cur_e].f0 ? edges[cur_e].f1 : edges[cur_e].f0;  // Edge::other_triangle
+          int nx = -1;
+          for (int k = 0; k < 3; ++k) if (tri[ct].e[k] == cur_e) nx = (k + 1) % 3;  // Triangle::next_edge_id
+          if (nx < 0 || tri[ct].v[nx] != cur_v) return false;
+          ce = nx;
+        }
+      }
+      if (num > 2) { H.ffirst.push_back(first); H.fnum.push_back(num); face_tri.push_back(i); }
+      break;
+    }
+  }
+  for (uint32_t t : face_tri) H.fn.push_back(tri[t].n);
+  for (auto& e : edges) { H.ev0.push_back(e.v0); H.ev1.push_back(e.v1); H.edir.push_back(e.dir); }
+  H.vfirst.assign(npts, 0);
+  H.vnum.assign(npts, 0);
+  for (size_t f = 0; f < H.ffirst.size(); ++f)
+    for (uint32_t k = H.ffirst[f]; k < H.ffirst[f] + H.fnum[f]; ++k) H.vnum[H.lvert[k]]++;
+  uint32_t total = 0;
+  for (uint32_t v = 0; v < npts; ++v) { H.vfirst[v] = total; total += H.vnum[v]; H.vnum[v] = 0; }
+  H.vface.assign(total, 0);
+  H.vedge.assign(total, 0);
+  for (size_t f = 0; f < H.ffirst.size(); ++f)
+    for (uint32_t k = H.ffirst[f]; k < H.ffirst[f] + H.fnum[f]; ++k) {
+      uint32_t v = H.lvert[k];
+      H.vface[H.vfirst[v] + H.vnum[v]] = (uint32_t)f;
+      H.vedge[H.vfirst[v] + H.vnum[v]] = H.ledge[k];
+      H.vnum[v]++;
+    }
+  int64_t valid_vertices = 0;
+  for (uint32_t v = 0; v < npts; ++v) valid_vertices += H.vnum[v] != 0;
+  return valid_vertices + (int64_t)H.ffirst.size() - valid_edges == 2;  // Euler's characteristic
+}
+
+// ConvexPolygonalFeature (shape/convex_polygonal_feature3.rs:38-56); edge normals are left out
+// (only ConvexPolygonalFeature::project_point reads them, and no generator here calls it).
+template <class R> struct PolyFeat {
+  std::vector<V3<R>> v;
+  std::vector<uint32_t> vid, eid;
+  bool has_n = false;
+  V3<R> n;
+  uint32_t fid = FEAT_NONE;
+  void clear() { v.clear(); vid.clear(); eid.clear(); has_n = false; fid = FEAT_NONE; }
+  void push(V3<R> p, uint32_t id) { v.push_back(p); vid.push_back(id); }
+  size_t nedges() const { return v.size() == 1 ? 0 : (v.size() == 2 ? 1 : v.size()); }
+  void transform_by(const Iso<R>& m) {
+    for (auto& p : v) p = xform_pt(m, p);
+    if (has_n) n = xform_vec(m, n);
+  }
+  // ConvexPolygonalFeature::edge: the segment starting at the vertex whose edge id matches
+  bool edge(uint32_t id, V3<R>* a, V3<R>* b) const {
+    for (size_t i1 = 0; i1 < v.size() && i1 < eid.size(); ++i1)
+      if (eid[i1] == id) { *a = v[i1]; *b = v[(i1 + 1) % v.size()]; return true; }
+    return false;
+  }
+};
+
+// Cuboid::face (shape/cuboid.rs:185-276), 3D.
+template <class R> inline void cuboid_face(V3<R> h, uint32_t i, PolyFeat<R>& out) {
+  out.clear();
+  uint32_t i1;
+  R sign;
+  if (i < 3) { i1 = i; sign = R(1); } else { i1 = i - 3; sign = R(-1); }
+  uint32_t i2 = (i1 + 1) % 3, i3 = (i1 + 2) % 3;
+  uint32_t ei2 = sign > R(0) ? i2 : i3, ei3 = sign > R(0) ? i3 : i2;
+  uint32_t mask_i2 = ~(1u << ei2), mask_i3 = ~(1u << ei3);
+  V3<R> vertex = h;
+  vertex[i1] = vertex[i1] * sign;
+  uint32_t sbit = sign < R(0) ? 1u : 0u, msbit = sign < R(0) ? 0u : 1u;
+  uint32_t vertex_id = sbit << i1;
+  out.push(vertex, feat(FEAT_V, vertex_id));
+  out.eid.push_back(feat(FEAT_E, ei2 | ((vertex_id & mask_i2) << 2)));
+  vertex[i2] = -sign * h[i2];
+  vertex[i3] = sign * h[i3];
+  vertex_id |= (msbit << i2) | (sbit << i3);
+  out.push(vertex, feat(FEAT_V, vertex_id));
+  out.eid.push_back(feat(FEAT_E, ei3 | ((vertex_id & mask_i3) << 2)));
+  vertex[i2] = -h[i2];
+  vertex[i3] = -h[i3];
+  vertex_id |= (1u << i2) | (1u << i3);
+  out.push(vertex, feat(FEAT_V, vertex_id));
+  out.eid.push_back(feat(FEAT_E, ei2 | ((vertex_id & mask_i2) << 2)));
+  vertex[i2] = sign * h[i2];
+  vertex[i3] = -sign * h[i3];
+  vertex_id = (sbit << i1) | (sbit << i2) | (msbit << i3);
+  out.push(vertex, feat(FEAT_V, vertex_id));
+  out.eid.push_back(feat(FEAT_E, ei3 | ((vertex_id & mask_i3) << 2)));
+  V3<R> nn(0, 0, 0);
+  nn[i1] = sign;
+  out.has_n = true;
+  out.n = nn;
+  out.fid = feat(FEAT_F, sign > R(0) ? i1 : i1 + 3);
+}
+
+// ConvexHull::face (shape/convex.rs:443-460).
+template <class R> inline void hull_face(const Shape<R>& g, uint32_t f, PolyFeat<R>& out) {
+  out.clear();
+  const HullPolyO<R>& H = *g.poly;
+  for (uint32_t k = H.ffirst[f]; k < H.ffirst[f] + H.fnum[f]; ++k) {
+    out.push(g.pts[H.lvert[k]], feat(FEAT_V, H.lvert[k]));
+    out.eid.push_back(feat(FEAT_E, H.ledge[k]));
+  }
+  out.has_n = true;
+  out.n = H.fn[f];
+  out.fid = feat(FEAT_F, f);
+}
+
+// R::signum (Rust): +1 for +0 and positives, -1 for -0 and negatives, NaN for NaN.
+template <class R> inline R rsignum(R x) { return std::isnan(x) ? x : (std::signbit(x) ? R(-1) : R(1)); }
+
+// ConvexPolyhedron::support_face_toward for Cuboid (cuboid.rs:279-307), ConvexHull
+// (convex.rs:487-509) and Segment (segment.rs:291-304).
+template <class R> inline void support_face_toward(const Shape<R>& g, const Iso<R>& m, V3<R> dir, PolyFeat<R>& out) {
+  out.clear();
+  if (g.type == CUBOID) {
+    V3<R> ld = inv_xform_vec(m, dir);
+    int iamax = 0;
+    R amax = std::fabs(ld[0]);
+    for (int i = 1; i < 3; ++i) if (std::fabs(ld[i]) > amax) { amax = std::fabs(ld[i]); iamax = i; }
+    cuboid_face(V3<R>(g.a, g.b, g.c), ld[iamax] > R(0) ? (uint32_t)iamax : (uint32_t)iamax + 3, out);
+    out.transform_by(m);
+  } else if (g.type == HULL) {
+    V3<R> ld = inv_xform_vec(m, dir);
+    const HullPolyO<R>& H = *g.poly;
+    uint32_t best = 0;
+    R max_dot = dot(H.fn[0], ld);
+    for (uint32_t f = 1; f < H.fn.size(); ++f) {
+      R d = dot(H.fn[f], ld);
+      if (d > max_dot) { max_dot = d; best = f; }
+    }
+    hull_face(g, best, out);
+    out.transform_by(m);
+  } else {  // SEGMENT
+    out.push(V3<R>(R(0), -g.a, R(0)), feat(FEAT_V, 0));
+    out.push(V3<R>(R(0), g.a, R(0)), feat(FEAT_V, 1));
+    out.eid.push_back(feat(FEAT_E, 0));
+    out.fid = feat(FEAT_E, 0);
+    out.transform_by(m);
+  }
+}
+
+// ConvexHull::support_feature_id_toward_eps (convex.rs:388-415): sin and cos of the angle given.
+template <class R> inline uint32_t hull_support_feature_id(const Shape<R>& g, V3<R> ld, R seps, R ceps) {
+  const HullPolyO<R>& H = *g.poly;
+  uint32_t sv = point_cloud_support_id(ld, g.pts, g.npts);
+  for (uint32_t i = 0; i < H.vnum[sv]; ++i) {
+    uint32_t f = H.vface[H.vfirst[sv] + i];
+    if (dot(H.fn[f], ld) >= ceps) return feat(FEAT_F, f);
+  }
+  for (uint32_t i = 0; i < H.vnum[sv]; ++i) {
+    uint32_t e = H.vedge[H.vfirst[sv] + i];
+    if (std::fabs(dot(H.edir[e], ld)) <= seps) return feat(FEAT_E, e);
+  }
+  return feat(FEAT_V, sv);
+}
+
+// ConvexPolyhedron::support_feature_toward for Cuboid (cuboid.rs:309-405), ConvexHull
+// (convex.rs:511-540) and Segment (segment.rs:306-344); cang/sang: cos and sin of the angle.
+template <class R> inline void support_feature_toward(const Shape<R>& g, const Iso<R>& m, V3<R> dir, R cang, R sang,
+                                                      PolyFeat<R>& out) {
+  out.clear();
+  if (g.type == CUBOID) {
+    V3<R> h(g.a, g.b, g.c);
+    V3<R> ld = inv_xform_vec(m, dir);
+    V3<R> sp = h;
+    uint32_t spid = 0;
+    for (int i1 = 0; i1 < 3; ++i1) {
+      R sign = rsignum(ld[i1]);
+      if (sign * ld[i1] >= cang) {
+        cuboid_face(h, sign > R(0) ? (uint32_t)i1 : (uint32_t)i1 + 3, out);
+        out.transform_by(m);
+        return;
+      } else if (sign < R(0)) {
+        sp[i1] = sp[i1] * sign;
+        spid |= 1u << i1;
+      }
+    }
+    for (int i = 0; i < 3; ++i) {
+      R sign = rsignum(ld[i]);
+      if (sign * ld[i] <= sang) {
+        sp[i] = -h[i];
+        V3<R> p1 = sp;
+        sp[i] = h[i];
+        V3<R> p2 = sp;
+        uint32_t p2id = spid & ~(1u << i);
+        out.push(xform_pt(m, p1), feat(FEAT_V, spid | (1u << i)));
+        out.push(xform_pt(m, p2), feat(FEAT_V, p2id));
+        uint32_t e = feat(FEAT_E, (uint32_t)i | (p2id << 2));
+        out.eid.push_back(e);
+        out.fid = e;
+        return;
+      }
+    }
+    out.push(xform_pt(m, sp), feat(FEAT_V, spid));
+    out.fid = feat(FEAT_V, spid);
+  } else if (g.type == HULL) {
+    V3<R> ld = inv_xform_vec(m, dir);
+    uint32_t f = hull_support_feature_id(g, ld, sang, cang);
+    const HullPolyO<R>& H = *g.poly;
+    if (feat_kind(f) == FEAT_V) {
+      out.push(g.pts[feat_id(f)], f);
+      out.fid = f;
+    } else if (feat_kind(f) == FEAT_E) {
+      uint32_t e = feat_id(f);
+      out.push(g.pts[H.ev0[e]], feat(FEAT_V, H.ev0[e]));
+      out.push(g.pts[H.ev1[e]], feat(FEAT_V, H.ev1[e]));
+      out.fid = f;
+      out.eid.push_back(f);
+    } else {
+      hull_face(g, feat_id(f), out);
+    }
+    out.transform_by(m);
+  } else {  // SEGMENT: the segment placed by m
+    V3<R> a = xform_pt(m, V3<R>(R(0), -g.a, R(0))), b = xform_pt(m, V3<R>(R(0), g.a, R(0)));
+    V3<R> sd;
+    if (!try_normalize(b - a, Eps<R>::eps(), &sd)) return;
+    R c = dot(dir, sd);
+    if (c > sang) { out.fid = feat(FEAT_V, 1); out.push(b, feat(FEAT_V, 1)); }
+    else if (c < -sang) { out.fid = feat(FEAT_V, 0); out.push(a, feat(FEAT_V, 0)); }
+    else {
+      out.push(a, feat(FEAT_V, 0));
+      out.push(b, feat(FEAT_V, 1));
+      out.eid.push_back(feat(FEAT_E, 0));
+      out.fid = feat(FEAT_E, 0);
+    }
+  }
+}
+
+// 2D helpers of the clipping
+template <class R> struct P2 { R x, y; };
+// utils::point_in_poly2d (utils/point_in_poly2d.rs:5-26)
+template <class R> inline bool point_in_poly2d(P2<R> pt, const std::vector<P2<R>>& poly) {
+  if (poly.empty()) return false;
+  R sign = R(0);
+  for (size_t i1 = 0; i1 < poly.size(); ++i1) {
+    size_t i2 = (i1 + 1) % poly.size();
+    R sx = poly[i2].x - poly[i1].x, sy = poly[i2].y - poly[i1].y;
+    R dx = pt.x - poly[i1].x, dy = pt.y - poly[i1].y;
+    R perp = dx * sy - dy * sx;
+    if (sign == R(0)) sign = perp;
+    else if (sign * perp < R(0)) return false;
+  }
+  return true;
+}
+
+// approx::ulps_eq!(a, b) with the default epsilon (EPSILON) and max_ulps (4).
+inline bool ulps_eq_r(float a, float b) {
+  if ((a > b ? a - b : b - a) <= Eps<float>::eps()) return true;
+  if (std::signbit(a) != std::signbit(b) || std::isnan(a) || std::isnan(b)) return false;
+  uint32_t ia, ib;
+  std::memcpy(&ia, &a, 4);
+  std::memcpy(&ib, &b, 4);
+  return (ia <= ib ? ib - ia : ia - ib) <= 4u;
+}
+inline bool ulps_eq_r(double a, double b) {
+  if ((a > b ? a - b : b - a) <= Eps<double>::eps()) return true;
+  if (std::signbit(a) != std::signbit(b) || std::isnan(a) || std::isnan(b)) return false;
+  uint64_t ia, ib;
+  std::memcpy(&ia, &a, 8);
+  std::memcpy(&ib, &b, 8);
+  return (ia <= ib ? ib - ia : ia - ib) <= 4u;
+}
+// nalgebra::clamp
+template <class R> inline R na_clamp(R v, R lo, R hi) { return v > lo ? (v < hi ? v : hi) : lo; }
+
+// closest_points_segment_segment_with_locations_nD (query/closest_points/
+// closest_points_segment_segment.rs:44-141) in 2D: true when both points lie inside their
+// segments (SegmentPointLocation::OnEdge), with the parameters s and t.
+template <class R> inline bool seg_seg_2d_on_edges(P2<R> p1, P2<R> q1, P2<R> p2, P2<R> q2, R* s_out, R* t_out) {
+  const R eps = Eps<R>::eps();
+  R d1x = q1.x - p1.x, d1y = q1.y - p1.y, d2x = q2.x - p2.x, d2y = q2.y - p2.y;
+  R rx = p1.x - p2.x, ry = p1.y - p2.y;
+  R a = d1x * d1x + d1y * d1y, e = d2x * d2x + d2y * d2y, f = d2x * rx + d2y * ry;
+  R s, t;
+  if (a <= eps && e <= eps) { s = R(0); t = R(0); }
+  else if (a <= eps) { s = R(0); t = na_clamp(f / e, R(0), R(1)); }
+  else {
+    R c = d1x * rx + d1y * ry;
+    if (e <= eps) { t = R(0); s = na_clamp(-c / a, R(0), R(1)); }
+    else {
+      R b = d1x * d2x + d1y * d2y;
+      R ae = a * e, bb = b * b;
+      R denom = ae - bb;
+      bool parallel = denom <= eps || ulps_eq_r(ae, bb);
+      s = !parallel ? na_clamp((b * f - c * e) / denom, R(0), R(1)) : R(0);
+      t = (b * s + f) / e;
+      if (t < R(0)) { t = R(0); s = na_clamp(-c / a, R(0), R(1)); }
+      else if (t > R(1)) { t = R(1); s = na_clamp((b - c) / a, R(0), R(1)); }
+    }
+  }
+  *s_out = s;
+  *t_out = t;
+  return s != R(0) && s != R(1) && t != R(0) && t != R(1);
+}
+
+template <class R> struct ClipContact { Contact<R> c; uint32_t f1, f2; };
+
+// ConvexPolygonalFeature::clip (convex_polygonal_feature3.rs:217-352).
+template <class R> inline void clip_features(const PolyFeat<R>& s, const PolyFeat<R>& o, V3<R> normal, R linear,
+                                             std::vector<ClipContact<R>>& out) {
+  if (s.v.size() <= 2 && o.v.size() <= 2) return;
+  // Vector::orthonormal_subspace_basis(&[normal]): a x normal, then a
+  V3<R> a;
+  if (std::fabs(normal.x) > std::fabs(normal.y)) a = V3<R>(normal.z, R(0), -normal.x);
+  else a = V3<R>(R(0), -normal.z, normal.y);
+  a = normalize(a);
+  V3<R> b0 = cross(a, normal), b1 = a;
+  V3<R> ref = s.v[0];
+  std::vector<P2<R>> p1, p2;
+  for (auto& p : s.v) { V3<R> d = p - ref; p1.push_back(P2<R>{dot(b0, d), dot(b1, d)}); }
+  for (auto& p : o.v) { V3<R> d = p - ref; p2.push_back(P2<R>{dot(b0, d), dot(b1, d)}); }
+  auto emit = [&](V3<R> w1, V3<R> w2, uint32_t f1, uint32_t f2) {
+    Contact<R> c;  // Contact::new_wo_depth (contact.rs:43-46)
+    c.world1 = w1; c.world2 = w2; c.normal = normal;
+    c.depth = -dot(normal, w2 - w1);
+    if (-c.depth <= linear) out.push_back(ClipContact<R>{c, f1, f2});
+  };
+  // query::line_toi_with_plane (query/ray/ray_plane.rs:9-23)
+  auto line_toi = [&](V3<R> pc, V3<R> pn, V3<R> lo, V3<R> ld, R* toi) {
+    V3<R> dpos = pc - lo;
+    R denom = dot(pn, ld);
+    if (relative_eq_r(denom, R(0))) return false;
+    *toi = dot(pn, dpos) / denom;
+    return true;
+  };
+  if (p2.size() > 2) {
+    for (size_t i = 0; i < p1.size(); ++i) {
+      if (!point_in_poly2d(p1[i], p2)) continue;
+      V3<R> origin = (ref + b0 * p1[i].x) + b1 * p1[i].y;
+      R toi;
+      if (line_toi(o.v[0], o.n, origin, normal, &toi)) emit(s.v[i], origin + normal * toi, s.vid[i], o.fid);
+    }
+  }
+  if (p1.size() > 2) {
+    for (size_t i = 0; i < p2.size(); ++i) {
+      if (!point_in_poly2d(p2[i], p1)) continue;
+      V3<R> origin = (ref + b0 * p2[i].x) + b1 * p2[i].y;
+      R toi;
+      if (line_toi(s.v[0], s.n, origin, normal, &toi)) emit(origin + normal * toi, o.v[i], s.fid, o.vid[i]);
+    }
+  }
+  size_t ne1 = s.nedges(), ne2 = o.nedges();
+  for (size_t i1 = 0; i1 < ne1; ++i1) {
+    size_t j1 = (i1 + 1) % p1.size();
+    for (size_t i2 = 0; i2 < ne2; ++i2) {
+      size_t j2 = (i2 + 1) % p2.size();
+      R sp, tp;
+      if (!seg_seg_2d_on_edges(p1[i1], p1[j1], p2[i2], p2[j2], &sp, &tp)) continue;
+      // Segment::point_at(OnEdge([1 - s, s])) (shape/segment.rs:128-137)
+      V3<R> w1 = s.v[i1] * (R(1) - sp) + s.v[j1] * sp;
+      V3<R> w2 = o.v[i2] * (R(1) - tp) + o.v[j2] * tp;
+      emit(w1, w2, s.eid[i1], o.eid[i2]);
+    }
+  }
+}
+
+// ContactManifold with ContactTrackingMode::DistanceBased(0.02) (contact_manifold.rs:44-277):
+// the cache holds one tracking point per contact; save_cache_and_clear keeps the entries whose
+// contact was in the last update and marks them old.
+template <class R> struct TrackedEntry {
+  V3<R> track;
+  bool active;
+  Contact<R> c;
+  uint32_t f1, f2;
+};
+template <class R> struct ManifoldState {
+  bool has_dir = false;  // the convex-polyhedron generator's last_gjk_dir
+  V3<R> dir;
+  std::vector<V3<R>> cache;  // tracking points of the last update's contacts, in cache order
+};
+
+// A generator's contact preprocessor chain: the capsule's (capsule.rs:86-127) when the shape
+// is a capsule's segment (radius r > 0 marks it).
+template <class R> inline bool capsule_preprocess(R r, bool is_first, Contact<R>& c, uint32_t& f1, uint32_t& f2) {
+  uint32_t& f = is_first ? f1 : f2;
+  uint32_t k = feat_kind(f);
+  if (k == FEAT_V) f = feat(FEAT_F, feat_id(f));
+  else if (k == FEAT_E) f = feat(FEAT_F, 2);
+  else if (k == FEAT_F) f = feat(FEAT_F, 2 + feat_id(f));
+  else return false;
+  if (is_first) c.world1 = c.world1 + c.normal * r;
+  else c.world2 = c.world2 - c.normal * r;
+  c.depth = c.depth + r;
+  return true;
+}
+
+template <class R> inline void manifold_push(std::vector<TrackedEntry<R>>& cache, Contact<R> c, uint32_t f1, uint32_t f2,
+                                             V3<R> track, R cap1, R cap2) {
+  if (cap1 > R(0) && !capsule_preprocess(cap1, true, c, f1, f2)) return;
+  if (cap2 > R(0) && !capsule_preprocess(cap2, false, c, f1, f2)) return;
+  const R threshold = R(0.02);
+  size_t closest = cache.size();
+  R closest_dist = threshold * threshold;
+  for (size_t i = 0; i < cache.size(); ++i) {
+    R d = norm_sq(cache[i].track - track);
+    if (d < closest_dist) { closest_dist = d; closest = i; }
+  }
+  if (closest == cache.size()) {
+    cache.push_back(TrackedEntry<R>{track, true, c, f1, f2});
+    return;
+  }
+  TrackedEntry<R>& e = cache[closest];
+  if (e.active) {
+    if (c.depth <= e.c.depth) return;  // the contact already there is deeper
+  } else {
+    e.active = true;
+  }
+  e.c = c; e.f1 = f1; e.f2 = f2;
+  e.track = track;
+}
+
+// ConvexPolygonalFeature::add_contact_to_manifold (convex_polygonal_feature3.rs:355-400): the
+// contact is dropped when a feature is unknown or names a degenerate edge; the tracking point is
+// the contact's first point in the first shape's frame.
+template <class R> inline void add_contact_to_manifold(const PolyFeat<R>& F1, const PolyFeat<R>& F2, const ClipContact<R>& cc,
+                                                       const Iso<R>& m1, R cap1, R cap2,
+                                                       std::vector<TrackedEntry<R>>& cache) {
+  V3<R> local1 = inv_xform_pt(m1, cc.c.world1);
+  for (int side = 0; side < 2; ++side) {
+    uint32_t f = side == 0 ? cc.f1 : cc.f2;
+    const PolyFeat<R>& F = side == 0 ? F1 : F2;
+    if (f == FEAT_NONE) return;
+    if (feat_kind(f) == FEAT_E) {
+      V3<R> a, b, d;
+      if (!F.edge(f, &a, &b)) return;
+      if (!try_normalize(b - a, Eps<R>::eps(), &d)) return;
+    }
+  }
+  manifold_push(cache, cc.c, cc.f1, cc.f2, local1, cap1, cap2);
+}
+
+// BallConvexPolyhedronManifoldGenerator (ball_convex_polyhedron_manifold_generator.rs:28-121):
+// ball (center c, radius r) against a Cuboid, ConvexHull or capsule segment; flip: the ball is
+// the pair's second shape.  ceps1/seps1: cos and sin of one degree (the hull feature's angle).
+template <class R> inline void ball_polyhedron_generator(V3<R> c, R r, const Iso<R>& m2, const Shape<R>& g2, R linear,
+                                                         bool flip, R cap_poly, std::vector<TrackedEntry<R>>& cache) {
+  Contact<R> ct;
+  int fkind = 3, fid = 0;
+  bool inside = false;
+  if (!contact_ball_convex(c, r, m2, g2, linear, &ct, &fkind, &fid, &inside)) return;
+  uint32_t f2;
+  if (g2.type == HULL) {
+    // ConvexHull::project_point_with_feature (point_support_map.rs:96-118)
+    V3<R> dpt = c - ct.world2;
+    V3<R> ld = inside ? inv_xform_vec(m2, -dpt) : inv_xform_vec(m2, dpt);
+    V3<R> u;
+    if (!try_normalize(ld, Eps<R>::eps(), &u)) return;  // FeatureId::Unknown: the reference panics here
+    const R one_degree = R(3.14159265358979323846 / 180.0);
+    f2 = hull_support_feature_id(g2, u, std::sin(one_degree), std::cos(one_degree));
+  } else {
+    if (fkind == 3) return;  // FeatureId::Unknown: the reference panics here
+    f2 = fkind == 0 ? feat(FEAT_F, (uint32_t)fid) : fkind == 1 ? feat(FEAT_E, (uint32_t)fid) : feat(FEAT_V, (uint32_t)fid);
+  }
+  const uint32_t ball = feat(FEAT_F, 0);
+  if (!flip) manifold_push(cache, ct, ball, f2, V3<R>(0, 0, 0), R(0), cap_poly);
+  else {
+    Contact<R> fl;  // Contact::new(world2, world1, -normal, depth)
+    fl.world1 = ct.world2; fl.world2 = ct.world1; fl.normal = -ct.normal; fl.depth = ct.depth;
+    manifold_push(cache, fl, f2, ball, V3<R>(0, 0, 0), cap_poly, R(0));
+  }
+}
+
+// ConvexPolyhedronConvexPolyhedronManifoldGenerator::generate_contacts
+// (convex_polyhedron_convex_polyhedron_manifold_generator.rs:83-167).
+template <class R> inline void polyhedra_generator(const Iso<R>& m1, const Shape<R>& g1, const Iso<R>& m2, const Shape<R>& g2,
+                                                   R linear, R c1, R s1, R c2, R s2, R cap1, R cap2, ManifoldState<R>& st,
+                                                   std::vector<TrackedEntry<R>>& cache) {
+  Contact<R> ct;
+  V3<R> nd;
+  bool set = false;
+  bool ok = polyhedra_gjk_step(m1, g1, m2, g2, linear, st.has_dir ? &st.dir : nullptr, &ct, &nd, &set);
+  if (set) { st.has_dir = true; st.dir = nd; }
+  if (!ok) return;
+  PolyFeat<R> F1, F2;
+  std::vector<ClipContact<R>> nc;
+  if (ct.depth > R(0)) {
+    support_face_toward(g1, m1, ct.normal, F1);
+    support_face_toward(g2, m2, -ct.normal, F2);
+  } else {
+    support_feature_toward(g1, m1, ct.normal, c1, s1, F1);
+    support_feature_toward(g2, m2, -ct.normal, c2, s2, F2);
+  }
+  clip_features(F1, F2, ct.normal, linear, nc);
+  if (nc.empty()) nc.push_back(ClipContact<R>{ct, F1.fid, F2.fid});
+  for (auto& cc : nc) add_contact_to_manifold(F1, F2, cc, m1, cap1, cap2, cache);
+}
+
+// One update of a pair's manifold (NarrowPhase::update_contact, narrow_phase.rs:56-80, with the
+// generator DefaultContactDispatcher picks, default_contact_dispatcher.rs:27-97).  c1/s1, c2/s2:
+// cos and sin of the prediction's angular1 / angular2.
+template <class R> inline void manifold_update(const Iso<R>& m1, const Shape<R>& g1, const Iso<R>& m2, const Shape<R>& g2,
+                                               R linear, R c1, R s1, R c2, R s2, ManifoldState<R>& st,
+                                               std::vector<TrackedEntry<R>>& cache) {
+  cache.clear();  // save_cache_and_clear: the last update's contacts become old entries
+  for (auto& p : st.cache) cache.push_back(TrackedEntry<R>{p, false, Contact<R>{}, FEAT_NONE, FEAT_NONE});
+  Shape<R> a = g1, b = g2;
+  R cap1 = R(0), cap2 = R(0), lin = linear;
+  // CapsuleCapsuleManifoldGenerator / CapsuleShapeManifoldGenerator: capsules become their
+  // segments, the prediction grows by each radius, and each capsule adds its preprocessor
+  if (g1.type == CAPSULE) { a.type = SEGMENT; cap1 = g1.b; lin = lin + g1.b; }
+  if (g2.type == CAPSULE) { b.type = SEGMENT; cap2 = g2.b; lin = lin + g2.b; }
+  if (a.type == BALL && b.type == BALL) {
+    // BallBallManifoldGenerator (ball_ball_manifold_generator.rs:27-66)
+    Contact<R> ct;
+    if (contact_ball_ball(m1.t, g1.a, m2.t, g2.a, linear, &ct))
+      manifold_push(cache, ct, feat(FEAT_F, 0), feat(FEAT_F, 0), V3<R>(0, 0, 0), R(0), R(0));
+  } else if (a.type == BALL) {
+    ball_polyhedron_generator(m1.t, g1.a, m2, b, lin, false, cap2, cache);
+  } else if (b.type == BALL) {
+    ball_polyhedron_generator(m2.t, g2.a, m1, a, lin, true, cap1, cache);
+  } else {
+    polyhedra_generator(m1, a, m2, b, lin, c1, s1, c2, s2, cap1, cap2, st, cache);
+  }
+  st.cache.clear();
+  for (auto& e : cache) if (e.active) st.cache.push_back(e.track);
 }
 
 }  // namespace ncxo

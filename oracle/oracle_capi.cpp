@@ -217,7 +217,65 @@ template <class R> inline AABB<R> aabb_from6(const R* b) {
   return AABB<R>{V3<R>(b[0], b[1], b[2]), V3<R>(b[3], b[4], b[5])};
 }
 
+// Per-interaction state of the contact-manifold generators, kept across steps by slot.
+struct ManifoldStore {
+  std::vector<ManifoldState<float>> slots;
+};
+
+int manifold_step(ManifoldStore* st, int64_t ns, const uint32_t* types, const float* params, const uint32_t* hoff,
+                  const uint32_t* hcnt, const float* hpts, int64_t npts_total, const uint32_t* hull_tris,
+                  const uint32_t* tri_off, const uint32_t* tri_cnt, const float* poses, const uint32_t* ia,
+                  const uint32_t* ib, const uint32_t* slots, int64_t np, float linear, float ang1, float ang2,
+                  int64_t maxc, uint32_t* count, float* w1, float* w2, float* nrm, float* depth, uint32_t* f1,
+                  uint32_t* f2, float* dirs_out, int nthreads) {
+  ShapeSet<float> ss;
+  ss.load(ns, types, params, hoff, hcnt, hpts, npts_total);
+  std::vector<HullPolyO<float>> polys((size_t)ns);
+  std::vector<uint8_t> bad((size_t)ns, 0);
+  parallel_for(ns, nthreads, [&](int64_t i) {
+    Shape<float>& g = ss.shapes[i];
+    if (g.type != HULL) return;
+    if (!hull_poly_build_o(g.pts, g.npts, hull_tris + 3 * (size_t)tri_off[i], tri_cnt[i], polys[i])) bad[i] = 1;
+    g.poly = &polys[i];
+  });
+  for (int64_t i = 0; i < ns; ++i)
+    if (bad[i]) return -1;
+  for (int64_t k = 0; k < np; ++k)
+    if (slots[k] >= st->slots.size()) return -3;
+  const float c1 = std::cos(ang1), s1 = std::sin(ang1), c2 = std::cos(ang2), s2 = std::sin(ang2);
+  std::vector<uint8_t> over((size_t)np, 0);
+  parallel_for(np, nthreads, [&](int64_t k) {
+    uint32_t a = ia[k], b = ib[k];
+    ManifoldState<float>& ms = st->slots[slots[k]];
+    std::vector<TrackedEntry<float>> cache;
+    manifold_update(pose_at(poses, a), ss.shapes[a], pose_at(poses, b), ss.shapes[b], linear, c1, s1, c2, s2, ms, cache);
+    int64_t j = 0;
+    for (auto& e : cache) {
+      if (!e.active) continue;
+      if (j < maxc) {
+        size_t o = (size_t)k * maxc + j;
+        for (int d = 0; d < 3; ++d) {
+          w1[3 * o + d] = e.c.world1[d];
+          w2[3 * o + d] = e.c.world2[d];
+          nrm[3 * o + d] = e.c.normal[d];
+        }
+        depth[o] = e.c.depth;
+        f1[o] = e.f1;
+        f2[o] = e.f2;
+      }
+      j++;
+    }
+    count[k] = (uint32_t)j;
+    if (j > maxc) over[k] = 1;
+    for (int d = 0; d < 3; ++d) dirs_out[3 * k + d] = ms.has_dir ? ms.dir[d] : 0.f;
+  });
+  for (int64_t k = 0; k < np; ++k)
+    if (over[k]) return -2;
+  return 0;
+}
+
 }  // namespace
+
 
 
 template <class R>
@@ -640,5 +698,58 @@ int orc_polyhedra_step_f32(int64_t ns, const uint32_t* types, const float* param
   }
 ORC_MESH(f32, float)
 ORC_MESH(f64, double)
+
+
+void* orc_manifold_store_new(int64_t nslots) {
+  ManifoldStore* st = new ManifoldStore;
+  st->slots.resize((size_t)nslots);
+  return st;
+}
+void orc_manifold_store_free(void* p) { delete (ManifoldStore*)p; }
+int orc_manifold_reset(void* p, const uint32_t* slots, int64_t n) {
+  ManifoldStore* st = (ManifoldStore*)p;
+  for (int64_t k = 0; k < n; ++k) {
+    if (slots[k] >= st->slots.size()) return -3;
+    st->slots[slots[k]] = ManifoldState<float>();
+  }
+  return 0;
+}
+int orc_manifold_step_f32(void* store, int64_t ns, const uint32_t* types, const float* params, const uint32_t* hoff,
+                          const uint32_t* hcnt, const float* hpts, int64_t npts, const uint32_t* hull_tris,
+                          const uint32_t* tri_off, const uint32_t* tri_cnt, const float* poses, const uint32_t* ia,
+                          const uint32_t* ib, const uint32_t* slots, int64_t np, float linear, float ang1, float ang2,
+                          int64_t maxc, uint32_t* count, float* w1, float* w2, float* nrm, float* depth, uint32_t* f1,
+                          uint32_t* f2, float* dirs_out, int nthreads) {
+  return manifold_step((ManifoldStore*)store, ns, types, params, hoff, hcnt, hpts, npts, hull_tris, tri_off, tri_cnt,
+                       poses, ia, ib, slots, np, linear, ang1, ang2, maxc, count, w1, w2, nrm, depth, f1, f2, dirs_out,
+                       nthreads);
+}
+int orc_hull_polyhedron_f32(const float* pts, int64_t npts, const uint32_t* tris, int64_t ntri, float* face_normals,
+                            uint32_t* face_first, uint32_t* face_count, int64_t* nfaces, uint32_t* loop_vertex,
+                            uint32_t* loop_edge, int64_t* nloops, uint32_t* edge_vertices, float* edge_dirs,
+                            int64_t* nedges, uint32_t* vertex_first, uint32_t* vertex_count, uint32_t* adj_face,
+                            uint32_t* adj_edge) {
+  std::vector<V3<float>> P((size_t)npts);
+  for (int64_t i = 0; i < npts; ++i) P[i] = V3<float>(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+  HullPolyO<float> H;
+  if (!hull_poly_build_o(P.data(), (uint32_t)npts, tris, (uint32_t)ntri, H)) return -1;
+  *nfaces = (int64_t)H.ffirst.size();
+  *nloops = (int64_t)H.lvert.size();
+  *nedges = (int64_t)H.ev0.size();
+  for (size_t f = 0; f < H.ffirst.size(); ++f) {
+    for (int d = 0; d < 3; ++d) face_normals[3 * f + d] = H.fn[f][d];
+    face_first[f] = H.ffirst[f];
+    face_count[f] = H.fnum[f];
+  }
+  for (size_t k = 0; k < H.lvert.size(); ++k) { loop_vertex[k] = H.lvert[k]; loop_edge[k] = H.ledge[k]; }
+  for (size_t e = 0; e < H.ev0.size(); ++e) {
+    edge_vertices[2 * e] = H.ev0[e];
+    edge_vertices[2 * e + 1] = H.ev1[e];
+    for (int d = 0; d < 3; ++d) edge_dirs[3 * e + d] = H.edir[e][d];
+  }
+  for (int64_t v = 0; v < npts; ++v) { vertex_first[v] = H.vfirst[v]; vertex_count[v] = H.vnum[v]; }
+  for (size_t k = 0; k < H.vface.size(); ++k) { adj_face[k] = H.vface[k]; adj_edge[k] = H.vedge[k]; }
+  return 0;
+}
 
 }  // extern "C"

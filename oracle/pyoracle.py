@@ -58,6 +58,12 @@ def _declare(L):
         sig(f"orc_shape_rays_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P, P, P, I64, R, C.c_int, P, P, P, C.c_int)
     sig("orc_world_rays_f32", I64, I64, P, P, P, P, P, I64, P, P, P, P, I64, C.c_float, C.c_int, P, P, P, P, I64,
         C.c_int)
+    sig("orc_manifold_store_new", P, I64)
+    sig("orc_manifold_store_free", None, P)
+    sig("orc_manifold_reset", C.c_int, P, P, I64)
+    sig("orc_manifold_step_f32", C.c_int, P, I64, P, P, P, P, P, I64, P, P, P, P, P, P, P, I64, C.c_float,
+        C.c_float, C.c_float, I64, P, P, P, P, P, P, P, P, C.c_int)
+    sig("orc_hull_polyhedron_f32", C.c_int, P, I64, P, I64, P, P, P, P, P, P, P, P, P, P, P, P, P, P)
     sig("orc_polyhedra_step_f32", C.c_int, I64, P, P, P, P, P, I64, P, P, P, I64, C.c_float, P, P, P, P, P, P, P,
         P, C.c_int)
     for t, R in (("f32", C.c_float), ("f64", C.c_double)):
@@ -399,6 +405,78 @@ def polyhedra_gjk_step(ss, poses, ia, ib, prediction, last_dirs, nthreads=8):
     L.orc_polyhedra_step_f32(*ss.args(), _p(poses), _p(ia), _p(ib), n, prediction, _p(ld), _p(has), _p(w1), _p(w2),
                              _p(nr), _p(depth), _p(nd), _p(dset), nthreads)
     return dict(has=has.astype(bool), world1=w1, world2=w2, normal=nr, depth=depth, dir=nd, dir_set=dset.astype(bool))
+
+
+def hull_polyhedron(points, tris):
+    """ConvexHull::try_new's faces, edges and adjacency (f32) from a hull's outward triangles;
+    None where try_new returns None."""
+    L = lib()
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    tr = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+    n, nt = len(pts), len(tr)
+    fn = np.zeros((nt, 3), np.float32)
+    ff, fc = np.zeros(nt, np.uint32), np.zeros(nt, np.uint32)
+    lv, le = np.zeros(3 * nt, np.uint32), np.zeros(3 * nt, np.uint32)
+    ev, ed = np.zeros((3 * nt, 2), np.uint32), np.zeros((3 * nt, 3), np.float32)
+    vf, vc = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    af, ae = np.zeros(3 * nt, np.uint32), np.zeros(3 * nt, np.uint32)
+    nf, nl, ne = I64(0), I64(0), I64(0)
+    rc = L.orc_hull_polyhedron_f32(_p(pts), n, _p(tr), nt, _p(fn), _p(ff), _p(fc), C.byref(nf), _p(lv), _p(le),
+                                   C.byref(nl), _p(ev), _p(ed), C.byref(ne), _p(vf), _p(vc), _p(af), _p(ae))
+    if rc != 0:
+        return None
+    nf, nl, ne = nf.value, nl.value, ne.value
+    na = int(vc.sum())
+    return dict(face_normals=fn[:nf], face_first=ff[:nf], face_count=fc[:nf], loop_vertex=lv[:nl], loop_edge=le[:nl],
+                edge_vertices=ev[:ne], edge_dirs=ed[:ne], vertex_first=vf, vertex_count=vc, adj_face=af[:na],
+                adj_edge=ae[:na])
+
+
+class ManifoldStore:
+    """Per-interaction state of the contact-manifold generators (last_gjk_dir and the
+    ContactManifold's tracking cache), one slot per interaction, kept across steps."""
+
+    def __init__(self, nslots):
+        self._h = lib().orc_manifold_store_new(nslots)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().orc_manifold_store_free(self._h)
+                self._h = None
+        except Exception:  # interpreter shutdown
+            pass
+
+    def reset(self, slots):
+        slots = np.ascontiguousarray(slots, np.uint32)
+        assert lib().orc_manifold_reset(self._h, _p(slots), len(slots)) == 0
+
+    def step(self, ss, hull_tris, tri_off, tri_cnt, poses, ia, ib, slots, linear, angular1, angular2, maxc=32,
+             nthreads=8):
+        """One NarrowPhase::update_contact per pair (f32): the manifold's contacts after the
+        update, in cache order, at most maxc per pair."""
+        assert ss.R == np.float32
+        poses = np.ascontiguousarray(poses, np.float32).reshape(-1, 7)
+        ia = np.ascontiguousarray(ia, np.uint32)
+        ib = np.ascontiguousarray(ib, np.uint32)
+        slots = np.ascontiguousarray(slots, np.uint32)
+        tris = np.ascontiguousarray(hull_tris, np.uint32).reshape(-1)
+        if len(tris) == 0:
+            tris = np.zeros(3, np.uint32)
+        toff = np.ascontiguousarray(tri_off, np.uint32)
+        tcnt = np.ascontiguousarray(tri_cnt, np.uint32)
+        n = len(ia)
+        cnt = np.zeros(n, np.uint32)
+        w1, w2, nr = (np.zeros((n, maxc, 3), np.float32) for _ in range(3))
+        depth = np.zeros((n, maxc), np.float32)
+        f1, f2 = np.zeros((n, maxc), np.uint32), np.zeros((n, maxc), np.uint32)
+        dirs = np.zeros((n, 3), np.float32)
+        rc = lib().orc_manifold_step_f32(self._h, *ss.args(), _p(tris), _p(toff), _p(tcnt), _p(poses), _p(ia), _p(ib),
+                                         _p(slots), n, linear, angular1, angular2, maxc, _p(cnt), _p(w1), _p(w2),
+                                         _p(nr), _p(depth), _p(f1), _p(f2), _p(dirs), nthreads)
+        assert rc in (0, -2), f"manifold step failed ({rc})"
+        return dict(count=cnt, world1=w1, world2=w2, normal=nr, depth=depth, feature1=f1, feature2=f2, dirs=dirs,
+                    overflow=rc == -2)
 
 
 def _mesh_args(ss, points, indices, mesh_pose7):

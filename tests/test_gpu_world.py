@@ -167,3 +167,39 @@ def test_world_temporal_warm_start_matches_generator(ctx, orc, use_graph):
         poses.copy_(torch.as_tensor(p_np, device=ctx.device))
     print(f"temporal warm start: {reused} pair-steps started from last_gjk_dir over 6 steps, bit-exact")
     assert reused > 1000
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_world_dense_cluster_pairs_match_brute_force(ctx, orc, use_graph):
+    """A dense cluster: 3,000 shapes around one centre (sigma 1), so most shapes touch hundreds
+    to thousands of others.  Leaves with more partners than the count pass keeps per leaf (512)
+    take the write pass's re-traversal and the emit kernel's second source; the pair set of every
+    step must equal brute force over the world's own stored boxes, and the first step's must
+    equal the oracle's DBVTBroadPhase."""
+    w = workloads.full_pipeline(seed=73, n=3000, box=40.0, nclusters=1, sigma=1.0)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    world = World(ss, margin=0.05, prediction=0.0, max_pairs=1 << 22, rebuild_every=3)
+    world.set_contacts(False)
+    p_np = w["poses"].copy()
+    poses = torch.as_tensor(p_np, device=ctx.device)
+    vel = w["velocities"] * 0.1
+    oss = oracle_shapes(orc, w["records"], w["pool"])
+    for step in range(5):
+        world.step(poses, use_graph=use_graph)
+        got = np.sort(keys_of(world.results()))
+        stored = np.ascontiguousarray(world.stored_aabbs().cpu().numpy().T)
+        want = np.sort(orc.pairs_brute(stored))
+        assert np.array_equal(got, want), f"step {step}: {len(got)} pairs against {len(want)} by brute force"
+        if step == 0:
+            ref_bp = orc.BroadPhase(0.05)
+            for b in orc.shape_aabbs(oss, to_oracle_poses(p_np)):
+                ref_bp.create_proxy(b)
+            ref_bp.update()
+            assert np.array_equal(got, ref_bp.pairs())
+        deg = np.bincount(np.concatenate([(got >> np.uint64(32)).astype(np.int64),
+                                          (got & np.uint64(0xFFFFFFFF)).astype(np.int64)]), minlength=3000)
+        print(f"step {step}: {len(got)} pairs, partners per shape max {deg.max()}, "
+              f"{np.sum(deg > 512)} shapes above 512, {np.sum((deg >= 500) & (deg <= 530))} in 500..530")
+        assert deg.max() > 1500 and np.sum((deg >= 500) & (deg <= 530)) > 0
+        p_np[:, :3] += vel
+        poses.copy_(torch.as_tensor(p_np, device=ctx.device))
