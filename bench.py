@@ -33,7 +33,7 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--quick", action="store_true", help="skip the secondary workloads")
-    p.add_argument("--only", choices=["hulls", "learned", "scene", "plain", "queries"], default=None,
+    p.add_argument("--only", choices=["hulls", "learned", "scene", "plain", "queries", "manifolds"], default=None,
                    help="run one secondary workload alone (after the ray-cast headline); for profiling")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--copy-calibration", action="store_true",
@@ -504,6 +504,77 @@ def bench_queries(args, ncx, torch, ctx):
     return out
 
 
+def bench_manifolds(args, ncx, torch, ctx):
+    """Persistent contact manifolds (ncx_contact_manifolds) on the pairs of the 1,048,576-shape
+    scene, its hulls drawn so that each has a polyhedral structure (ConvexHull::try_new succeeds):
+    one slot per pair, four updates with the shapes moving, prediction 0.02 / 0.05 rad."""
+    from ncollide_amd import workloads
+    from ncollide_amd.world import World
+    dev = ctx.device
+    t0 = time.perf_counter()
+    sw = workloads.full_pipeline(seed=5, n=args.scene_shapes, polyhedra=True)
+    gen_s = time.perf_counter() - t0
+    ss = ncx.ShapeSet(records=sw["records"], pool=sw["pool"], ctx=ctx)
+    poses = torch.as_tensor(sw["poses"], device=dev)
+    world = World(ss, margin=sw["margin"], prediction=0.0, max_pairs=args.scene_max_pairs)
+    world.set_contacts(False)
+    world.step(poses, use_graph=False)
+    world.sync()
+    r = world.results()
+    ia, ib = r["ia"].clone(), r["ib"].clone()
+    del world
+    n = ia.numel()
+    slots = torch.arange(n, dtype=torch.int32, device=dev)
+    gm = ncx.ContactManifolds(ctx, n)
+    vel = torch.as_tensor(sw["velocities"] * 0.05, device=dev)
+    g = gm.update(ss, poses, ia, ib, slots, 0.02, 0.05, 0.05)  # first update (also builds the hull faces)
+    times, counts = [], None
+    for step in range(4):
+        poses[:, 0:3] += vel
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        g = gm.update(ss, poses, ia, ib, slots, 0.02, 0.05, 0.05, out=g)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+        counts = g["count"]
+    tmed = float(np.median(times))
+    c = counts.cpu().numpy()
+    out = {"pairs": n, "ms_per_update": tmed * 1e3, "pairs_per_s": n / tmed,
+           "contacts_per_s": float(c.sum()) / tmed, "contacts_per_manifold": float(c.mean()),
+           "manifolds_by_count": np.bincount(np.minimum(c, 9), minlength=10).tolist(), "gen_s": gen_s,
+           "data": "1,048,576-shape clustered scene (hulls redrawn where try_new fails), pairs of one broad-phase "
+                   "step, four persistent updates"}
+    if not args.no_cpu:
+        from oracle import pyoracle as orc
+        from ncollide_amd.hull import hull_triangles
+        from ncollide_amd.math import to_oracle_poses
+        rec, pool = sw["records"], sw["pool"]
+        k = min(65536, n)
+        sel = np.sort(np.random.default_rng(1).choice(n, k, replace=False))
+        sa, sb = ia.cpu().numpy()[sel].astype(np.uint32), ib.cpu().numpy()[sel].astype(np.uint32)
+        used = np.unique(np.concatenate([sa, sb]))
+        tris, toff, tcnt = [], np.zeros(len(rec), np.uint32), np.zeros(len(rec), np.uint32)
+        o = 0
+        for i in used[rec["type"][used] == ncx.CONVEX_HULL]:
+            t = hull_triangles(pool[rec["hull_offset"][i]:rec["hull_offset"][i] + rec["hull_count"][i]])
+            tris.append(t)
+            toff[i], tcnt[i] = o, len(t)
+            o += len(t)
+        params = np.stack([rec["p0"], rec["p1"], rec["p2"]], 1)
+        oss = orc.ShapeSet(rec["type"], params, rec["hull_offset"], rec["hull_count"], pool, dtype=np.float32)
+        store = orc.ManifoldStore(k)
+        nth = min(16, os.cpu_count() or 1)
+        op = to_oracle_poses(poses.cpu().numpy())
+        t = time.perf_counter()
+        store.step(oss, np.concatenate(tris), toff, tcnt, op, sa, sb, np.arange(k, dtype=np.uint32), 0.02, 0.05,
+                   0.05, nthreads=nth)
+        tc = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": k / tc, "unit": "pairs/s (one manifold update)", "cores": nth, "kind": "port",
+                               "sample": f"{k} random pairs of the step, one update from fresh slots (hull faces "
+                                         "built outside the timing)"}
+    return out
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -593,6 +664,12 @@ def main():
         if want("queries"):
             log("other queries ...")
             sec["queries"] = bench_queries(args, ncx, torch, ctx)
+        if want("manifolds"):
+            log("contact manifolds ...")
+            try:
+                sec["contact_manifolds"] = bench_manifolds(args, ncx, torch, ctx)
+            except Exception as e:  # reported, not fatal to the headline line
+                sec["contact_manifolds"] = {"error": str(e)}
         result["secondary"] = sec
     print(json.dumps(result))
 

@@ -56,11 +56,12 @@ class ContactManifolds:
         _lib.check(self.ctx.lib.ncx_manifolds_dirs(self.h, _lib.ptr(s), s.numel(), _lib.ptr(out)))
         return out
 
-    def update(self, shapes, poses, ia, ib, slots, linear, angular1=0.0, angular2=0.0):
+    def update(self, shapes, poses, ia, ib, slots, linear, angular1=0.0, angular2=0.0, out=None):
         """One generate_contacts per pair (ia[k], ib[k]) on slot slots[k] (distinct within a
         call).  Returns CUDA tensors: count (n,), and per pair MAX_CONTACTS rows of world1,
         world2, normal (n, MAX, 3), depth, feature1, feature2 (n, MAX); rows past count[k] are
-        unused.  Contacts are in the manifold's cache order."""
+        unused.  Contacts are in the manifold's cache order.  out: a previous result for the same
+        number of pairs, whose tensors are written again."""
         dev = self.ctx.device
         poses = torch.as_tensor(poses, device=dev, dtype=torch.float32).contiguous()
         ia = torch.as_tensor(ia, device=dev).to(torch.int32).contiguous()
@@ -70,13 +71,19 @@ class ContactManifolds:
         if ib.numel() != n or s.numel() != n:
             raise ValueError("ia, ib and slots differ in length")
         m = MAX_CONTACTS
-        count = torch.zeros(n, dtype=torch.int32, device=dev)
-        w1 = torch.zeros((n, m, 3), dtype=torch.float32, device=dev)
-        w2 = torch.zeros((n, m, 3), dtype=torch.float32, device=dev)
-        nr = torch.zeros((n, m, 3), dtype=torch.float32, device=dev)
-        depth = torch.zeros((n, m), dtype=torch.float32, device=dev)
-        f1 = torch.zeros((n, m), dtype=torch.int32, device=dev)
-        f2 = torch.zeros((n, m), dtype=torch.int32, device=dev)
+        if out is not None:
+            if out["count"].numel() != n:
+                raise ValueError("out holds another number of pairs")
+            count, w1, w2, nr = out["count"], out["world1"], out["world2"], out["normal"]
+            depth, f1, f2 = out["depth"], out["feature1"], out["feature2"]
+        else:
+            count = torch.zeros(n, dtype=torch.int32, device=dev)
+            w1 = torch.zeros((n, m, 3), dtype=torch.float32, device=dev)
+            w2 = torch.zeros((n, m, 3), dtype=torch.float32, device=dev)
+            nr = torch.zeros((n, m, 3), dtype=torch.float32, device=dev)
+            depth = torch.zeros((n, m), dtype=torch.float32, device=dev)
+            f1 = torch.zeros((n, m), dtype=torch.int32, device=dev)
+            f2 = torch.zeros((n, m), dtype=torch.int32, device=dev)
         p = _lib.ptr
         _lib.check(self.ctx.lib.ncx_contact_manifolds(self.h, shapes.h, p(poses), p(ia), p(ib), p(s), n,
                                                       float(linear), float(angular1), float(angular2), p(count),

@@ -231,10 +231,14 @@ int manifold_step(ManifoldStore* st, int64_t ns, const uint32_t* types, const fl
   ShapeSet<float> ss;
   ss.load(ns, types, params, hoff, hcnt, hpts, npts_total);
   std::vector<HullPolyO<float>> polys((size_t)ns);
-  std::vector<uint8_t> bad((size_t)ns, 0);
-  parallel_for(ns, nthreads, [&](int64_t i) {
+  std::vector<uint8_t> bad((size_t)ns, 0), used((size_t)ns, 0);
+  for (int64_t k = 0; k < np; ++k) {
+    if (ia[k] >= ns || ib[k] >= ns) return -4;
+    used[ia[k]] = used[ib[k]] = 1;
+  }
+  parallel_for(ns, nthreads, [&](int64_t i) {  // the hulls the pairs name
     Shape<float>& g = ss.shapes[i];
-    if (g.type != HULL) return;
+    if (g.type != HULL || !used[i]) return;
     if (!hull_poly_build_o(g.pts, g.npts, hull_tris + 3 * (size_t)tri_off[i], tri_cnt[i], polys[i])) bad[i] = 1;
     g.poly = &polys[i];
   });
