@@ -106,9 +106,31 @@ def bench_rays(args, ncx, torch, ctx):
     torch.cuda.synchronize()
     avg_kernel_s = float(np.mean(kms)) / 1e3
     hits = float((feat != -1).float().mean().item())
+    ns = min(n, 65536)
     return dict(n=n, wall_ms=wall_ms, kernel_ms=float(np.mean(kms)), kernel_ms_min=float(np.min(kms)),
                 alg_bytes=alg_bytes, nodes_per_ray=nodes / n, tris_per_ray=tris / n,
-                achieved=alg_bytes / avg_kernel_s, hits=hits, pts=pts, tri=tri, o=o, d=d)
+                achieved=alg_bytes / avg_kernel_s, hits=hits, pts=pts, tri=tri, o=o, d=d,
+                sample=(toi[:ns].cpu().numpy(), nrm[:ns].cpu().numpy(), feat[:ns].cpu().numpy().view(np.uint32)))
+
+
+def cpu_accuracy_rays(r):
+    """Error of the GPU's f32 hits against the reference algorithm in f64 (the oracle's BVT
+    best-first search with Real = double) on the first 65,536 rays: relative toi error, normal
+    angle, and how often the hit triangle and the hit/miss decision agree."""
+    from oracle import pyoracle as orc
+    toi, nrm, feat = r["sample"]
+    ns = len(toi)
+    m = orc.TriMesh(r["pts"].astype(np.float64), r["tri"], dtype=np.float64)
+    t64, n64, f64, _ = m.cast(r["o"][:ns].astype(np.float64), r["d"][:ns].astype(np.float64), 100.0,
+                              nthreads=min(os.cpu_count() or 1, 64))
+    hit = (feat != 0xFFFFFFFF) & (f64 != 0xFFFFFFFF)
+    rel = np.abs(toi[hit].astype(np.float64) - t64[hit]) / np.maximum(np.abs(t64[hit]), 1e-12)
+    cosang = np.clip(np.sum(nrm[hit].astype(np.float64) * n64[hit], 1), -1.0, 1.0)
+    ang = np.arccos(cosang)
+    return {"rays": int(ns), "hit_agreement": float(np.mean((feat != 0xFFFFFFFF) == (f64 != 0xFFFFFFFF))),
+            "triangle_agreement": float(np.mean(feat[hit] == f64[hit])),
+            "toi_rel_err_p50": float(np.percentile(rel, 50)), "toi_rel_err_p99": float(np.percentile(rel, 99)),
+            "toi_rel_err_max": float(rel.max()), "normal_angle_p99_rad": float(np.percentile(ang, 99))}
 
 
 def cpu_baseline_rays(pts, tri, o, d):
@@ -189,7 +211,7 @@ def bench_hulls(args, ncx, torch, ctx):
     cit = g["gjk_iters"].float().mean().item()
     dits = dit.float().mean().item()
     return dict(pairs=n, contact_ms=tc * 1e3, distance_ms=td * 1e3, pairs_per_s=n / (tc + td),
-                lanes_per_pair_by_bucket={"contact": ss.tuning("contact")[0], "distance": ss.tuning("distance")[0]},
+                first_tier_variant_by_bucket={"contact": ss.tuning("contact")[0], "distance": ss.tuning("distance")[0]},
                 contact_pairs_per_s=n / tc, distance_pairs_per_s=n / td,
                 contact_fraction=g["has_contact"].float().mean().item(),
                 contact_gjk_iters=cit, distance_gjk_iters=dits, hull_vertices_per_pair=verts_per_pair,
@@ -334,7 +356,8 @@ def bench_scene(args, ncx, torch, ctx, mlp=None):
     world, times, pairs, large = run(True, args.scene_steps)
     ms = float(np.mean(times[1:] if len(times) > 1 else times)) * 1e3
     res = dict(shapes=args.scene_shapes, steps=args.scene_steps, ms_per_step=ms, ms_first_step=times[0] * 1e3,
-               lanes_per_pair_by_bucket=ss.tuning("contact")[0],
+               first_tier_variant_by_bucket=ss.tuning("contact")[0],
+               first_tier_sample_ms_by_bucket=ss.tuning("contact")[1],
                pairs_per_step=float(np.mean(pairs)), pairs_per_s=float(np.sum(pairs[1:])) / float(np.sum(times[1:])),
                epa_pairs_past_first_arena=int(large), gen_s=gen_s)
     del world
@@ -633,6 +656,8 @@ def main():
         result["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         result["cpu_baseline"]["single_thread_rays_per_s"] = cb["single_thread"]
         result["cpu_baseline"]["cpu_model"] = cpu_model()
+        log("ray accuracy vs f64 ...")
+        result["accuracy_vs_f64"] = cpu_accuracy_rays(r)
     if not args.quick:
         sec = {}
         want = (lambda k: args.only is None or args.only == k)

@@ -139,9 +139,14 @@ int ncx_shapes_destroy(ncx_shapes* s);
 int ncx_shapes_count(ncx_shapes* s, size_t* n);
 /* No reference counterpart: the autotuner's choice per vertex-count bucket (pairs by their larger
  * hull vertex count: none, <= 32, <= 96, more) for query 0 (contact) or 1 (distance): 0, 1, 2 =
- * 8, 16, 32 lanes per pair, -1 before the first batch large enough to tune on; sample_ms
- * (nullable, 12 floats) the sample time of each variant per bucket. */
+ * 8, 16, 32 lanes per pair with the 44-vertex first-tier EPA arena, 3, 4 = 8, 16 lanes with the
+ * 24-vertex arena (3, 4 contact only), -1 before the first batch large enough to tune on;
+ * sample_ms (nullable, 20 floats) the sample time of each variant per bucket (0 for a variant not
+ * tried). */
 int ncx_shapes_tuning(ncx_shapes* s, int query, int32_t* choice, float* sample_ms);
+/* No reference counterpart: sets the first-tier variant per bucket (as ncx_shapes_tuning reports
+ * them; -1 = tune on the next large batch).  Results do not depend on the variant. */
+int ncx_shapes_set_tuning(ncx_shapes* s, int query, const int32_t* choice);
 
 /* World AABB of every shape at the given poses (device arrays), optionally loosened by
  * `margin` (Shape::aabb: src/bounding_volume/aabb_ball.rs:8-13, aabb_cuboid.rs:7-14,

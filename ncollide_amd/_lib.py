@@ -38,6 +38,7 @@ SIGNATURES = {
     "ncx_shapes_destroy": (C.c_int, [P]),
     "ncx_shapes_count": (C.c_int, [P, P]),
     "ncx_shapes_tuning": (C.c_int, [P, C.c_int, P, P]),
+    "ncx_shapes_set_tuning": (C.c_int, [P, C.c_int, P]),
     "ncx_shapes_aabbs_dev": (C.c_int, [P, P, F32, P]),
     "ncx_contact_batch": (C.c_int, [P, P, P, P, SZ, F32, P, P, P, P, P, P, P]),
     "ncx_contact_batch_capped": (C.c_int, [P, P, P, P, SZ, F32, P, I32, P, P, P, P, P, P]),

@@ -16,6 +16,7 @@
 #include "../../include/ncx.h"
 
 #define NCX_FLT_EPS 1.1920929e-07f
+#define NCX_T1_VARIANTS 5  // first-tier contact kernel variants (the autotuner's choices)
 #define NCX_FLT_MAX 3.40282347e+38f
 
 // ------------------------------------------------------------------ errors
@@ -128,10 +129,10 @@ struct ncx_shapes {
   // the adjacency run; adjacency entries (face, edge).  Ids inside a hull are local.
   bool hf_ready = false;
   DevBuf hf_shape, hf_fn, hf_floop, hf_loop, hf_edir, hf_ev, hf_vadj, hf_adj;
-  // autotuner choice per query (0 contact, 1 distance) and bucket: an index into the kernel
-  // variants (group width 8, 16, 32), or -1 before tuning
+  // autotuner choice per query (0 contact, 1 distance) and bucket: an index into the first-tier
+  // kernel variants, or -1 before tuning
   int8_t tune[2][NCX_NBUCKETS] = {{-1, -1, -1, -1}, {-1, -1, -1, -1}};
-  float tune_ms[2][NCX_NBUCKETS][3] = {};
+  float tune_ms[2][NCX_NBUCKETS][NCX_T1_VARIANTS] = {};
 };
 
 // ------------------------------------------------------------------ device math

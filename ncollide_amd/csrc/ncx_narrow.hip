@@ -72,6 +72,12 @@ struct KArgs {
 #define T1_FC 144u
 #define T1_SC 32u
 #define T1_STC 32u
+// A smaller first-tier arena (24 vertices, 80 faces, 2.7 KiB): nearly twice the arenas per CU
+// for pairs whose EPA ends within 20 iterations (most polytope pairs); the rest overflow to T2.
+#define T1S_VC 24u
+#define T1S_FC 80u
+#define T1S_SC 32u
+#define T1S_STC 32u
 #define T2_G 32
 #define T2_BS 64
 #define T2_BLOCKS_PER_CU (160u * 1024u / ((T2_BS / T2_G) * (uint32_t)arena_bytes(T2_VC, T2_FC, T2_SC, T2_STC)))
@@ -305,7 +311,7 @@ extern "C" int ncx_shapes_tuning(ncx_shapes* s, int query, int32_t* choice, floa
   for (int b = 0; b < NCX_NBUCKETS; ++b) {
     choice[b] = s->tune[query][b];
     if (sample_ms)
-      for (int v = 0; v < 3; ++v) sample_ms[3 * b + v] = s->tune_ms[query][b][v];
+      for (int v = 0; v < NCX_T1_VARIANTS; ++v) sample_ms[NCX_T1_VARIANTS * b + v] = s->tune_ms[query][b][v];
   }
   return NCX_OK;
 }
@@ -549,14 +555,27 @@ static int rerun_overflows(ContactTiers& T, ncx_ctx* ctx, KArgs a, uint32_t n, c
   return NCX_OK;
 }
 
-// First-tier kernel variants, the autotuner's choices: group width G = 8, 16, 32 lanes per
-// pair.  Contact variants all hold 8 pairs per block (64, 128, 256 threads) with the same
-// 44-vertex, 144-face LDS arena, so a variant changes occupancy and lanes per pair but not which
-// pairs overflow to the second tier.  Distance queries need no EPA arena and use 128-thread
-// variants without LDS.
-static const int T1_VARIANT_G[3] = {8, 16, 32};
-static const int T1_VARIANT_BS[2][3] = {{64, 128, 256}, {128, 128, 128}};
-static const unsigned T1_BLOCKS_PER_CU[2][3] = {{8, 8, 4}, {8, 8, 8}};  // twice the resident blocks
+// First-tier kernel variants, the autotuner's choices.  Contact: group width G = 8, 16, 32 lanes
+// per pair with the 44-vertex, 144-face LDS arena (variants 0-2), and G = 8, 16 with the
+// 24-vertex, 80-face arena (variants 3-4); every variant holds 8 pairs per block.  The wide
+// arena's variants differ in occupancy and lanes per pair only; the small arena's hold more pairs
+// per CU and send more pairs to the second tier, which the tuner times with them.  Distance
+// queries need no EPA arena and use 128-thread variants without LDS (variants 0-2).
+static const int T1_NVARIANTS[2] = {NCX_T1_VARIANTS, 3};
+static const int T1_VARIANT_G[NCX_T1_VARIANTS] = {8, 16, 32, 8, 16};
+static const int T1_VARIANT_BS[2][NCX_T1_VARIANTS] = {{64, 128, 256, 64, 128}, {128, 128, 128, 128, 128}};
+// twice the resident blocks (LDS: 4 wide-arena or 7 small-arena blocks per CU; VGPRs: 8 waves).
+// A middle arena (32 vertices, 112 faces, 8 lanes) was never the tuner's choice on the scene's
+// buckets (5.17 / 2.51 / 3.31 ms against 5.04 / 2.13 / 2.81 for the best) and is left out.
+static const unsigned T1_BLOCKS_PER_CU[2][NCX_T1_VARIANTS] = {{8, 8, 4, 14, 8}, {8, 8, 8, 8, 8}};
+
+extern "C" int ncx_shapes_set_tuning(ncx_shapes* s, int query, const int32_t* choice) {
+  NCX_REQUIRE(s && choice && (query == 0 || query == 1), "ncx_shapes_set_tuning: bad argument");
+  for (int b = 0; b < NCX_NBUCKETS; ++b)
+    NCX_REQUIRE(choice[b] >= -1 && choice[b] < T1_NVARIANTS[query], "ncx_shapes_set_tuning: no such variant");
+  for (int b = 0; b < NCX_NBUCKETS; ++b) s->tune[query][b] = (int8_t)choice[b];
+  return NCX_OK;
+}
 
 static int launch_t1(int variant, bool distance, hipStream_t stream, const KArgs& a, unsigned blocks) {
   if (distance) {
@@ -572,6 +591,13 @@ static int launch_t1(int variant, bool distance, hipStream_t stream, const KArgs
         break;
       case 2:
         hipLaunchKernelGGL((k_pairs<32, 256, T1_VC, T1_FC, T1_SC, T1_STC, 0>), dim3(blocks), dim3(256), 0, stream, a);
+        break;
+      case 3:
+        hipLaunchKernelGGL((k_pairs<8, 64, T1S_VC, T1S_FC, T1S_SC, T1S_STC, 0>), dim3(blocks), dim3(64), 0, stream, a);
+        break;
+      case 4:
+        hipLaunchKernelGGL((k_pairs<16, 128, T1S_VC, T1S_FC, T1S_SC, T1S_STC, 0>), dim3(blocks), dim3(128), 0, stream,
+                           a);
         break;
       default:
         hipLaunchKernelGGL((k_pairs<16, 128, T1_VC, T1_FC, T1_SC, T1_STC, 0>), dim3(blocks), dim3(128), 0, stream, a);
@@ -715,11 +741,12 @@ static int autotune(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const KArgs& b
     NCX_HIP_CHECK(hipGetLastError());
     const uint32_t sr[2] = {0, ns};
     NCX_HIP_CHECK(hipMemcpyAsync(dr.p, sr, 8, hipMemcpyHostToDevice, ctx->stream));
-    // Rounds over the three variants, so clock ramps and noise fall on every variant alike; the
-    // first round warms up, each variant keeps its least time of the others.
-    float ms_v[3] = {0.f, 0.f, 0.f};
+    // Rounds over the variants, so clock ramps and noise fall on every variant alike; the first
+    // round warms up, each variant keeps its least time of the others.
+    const int nv = T1_NVARIANTS[mode];
+    float ms_v[NCX_T1_VARIANTS] = {};
     for (int rep = 0; rep < TUNE_ROUNDS && !rc; ++rep) {
-      for (int v = 0; v < 3; ++v) {
+      for (int v = 0; v < nv; ++v) {
         KArgs a = base;
         a.list = smp.as<uint32_t>();
         a.range_b = dr.as<uint32_t>();
@@ -738,7 +765,7 @@ static int autotune(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, const KArgs& b
       }
     }
     int best = 1;
-    for (int v = 0; v < 3 && !rc; ++v) {
+    for (int v = 0; v < nv && !rc; ++v) {
       S->tune_ms[mode][b][v] = ms_v[v];
       if (ms_v[v] < ms_v[best]) best = v;
     }

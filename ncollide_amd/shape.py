@@ -109,14 +109,24 @@ class ShapeSet:
     def types(self):
         return self.records["type"]
 
+    # first-tier variants: (lanes per pair, EPA arena vertices)
+    VARIANTS = ((8, 44), (16, 44), (32, 44), (8, 24), (16, 24))
+
     def tuning(self, query="contact"):
         """The autotuner's choice per vertex-count bucket (no hull, <= 32, <= 96, more vertices):
-        lanes per pair (8, 16, 32) or None before tuning, and each variant's sample time (ms)."""
+        the variant (lanes per pair, first-tier EPA arena vertices) or None before tuning, and each
+        variant's sample time (ms; 0 for variants not tried)."""
+        nv = len(self.VARIANTS)
         choice = (C.c_int32 * 4)()
-        ms = (C.c_float * 12)()
+        ms = (C.c_float * (4 * nv))()
         _lib.check(self.ctx.lib.ncx_shapes_tuning(self.h, 0 if query == "contact" else 1, choice, ms))
-        widths = (8, 16, 32)
-        return ([widths[c] if c >= 0 else None for c in choice], [list(ms[3 * b:3 * b + 3]) for b in range(4)])
+        return ([self.VARIANTS[c] if c >= 0 else None for c in choice], [list(ms[nv * b:nv * b + nv]) for b in range(4)])
+
+    def set_tuning(self, variants, query="contact"):
+        """Pins the first-tier variant of each bucket: a list of 4 indices into VARIANTS (None =
+        tune on the next large batch).  Results do not depend on the variant."""
+        choice = (C.c_int32 * 4)(*[-1 if v is None else int(v) for v in variants])
+        _lib.check(self.ctx.lib.ncx_shapes_set_tuning(self.h, 0 if query == "contact" else 1, choice))
 
     def params(self):
         return np.stack([self.records["p0"], self.records["p1"], self.records["p2"]], 1)

@@ -282,6 +282,28 @@ def test_contact_mixed_shapes(ctx, orc):
         assert_contacts_match(g, r, f"mixed shapes prediction {pred}")
 
 
+@pytest.mark.parametrize("variant", range(5))
+def test_contact_every_first_tier_variant_matches_oracle(ctx, orc, variant):
+    """Each first-tier kernel variant (lanes per pair, EPA arena size), pinned on every bucket:
+    the same bit-exact contacts.  The 24-vertex arena sends the longer EPA runs to the second
+    tier, so this also runs the T1 -> T2 hand-over on many pairs."""
+    w = workloads.full_pipeline(seed=6, n=6000, box=30.0, nclusters=3)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    ss.set_tuning([variant] * 4)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    bb = ss.aabbs(poses, margin=0.02).cpu().numpy().T
+    keys = orc.pairs_brute(bb)
+    ia = (keys >> np.uint64(32)).astype(np.uint32)
+    ib = (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    g = ncx.query.contact(ss, poses, ia, ib, 0.0)
+    r = orc.contact(oracle_shapes(orc, w["records"], w["pool"]), to_oracle_poses(w["poses"]), ia, ib, 0.0,
+                    nthreads=8)
+    assert_contacts_match(g, r, f"variant {variant}")
+    assert ss.tuning("contact")[0] == [ss.VARIANTS[variant]] * 4
+    print(f"variant {ss.VARIANTS[variant]}: {len(ia)} pairs, {int(r['has'].sum())} contacts, "
+          f"{int(np.sum(r['epa_iters'] > 20))} EPA runs past 20 iterations")
+
+
 def _curved_pairs(seed, n):
     """n isolated Ball-Capsule / Capsule-Capsule pairs at random overlaps (shapes 2i, 2i + 1)."""
     rng = np.random.default_rng(seed)
