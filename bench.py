@@ -645,7 +645,7 @@ def main():
                    "rays": r["n"], "triangles": 131072, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": r["achieved"] / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": r["achieved"] / HBM_PEAK, "traffic": _traffic("raycast"),
-                     "kernel": "k_cast_pt", "kernel_ms": r["kernel_ms"],
+                     "kernel": "k_cast_wide", "kernel_ms": r["kernel_ms"],
                      "alg_bytes_per_launch": r["alg_bytes"], "nodes_per_ray": r["nodes_per_ray"],
                      "tris_per_ray": r["tris_per_ray"]},
         "hit_fraction": r["hits"],

@@ -272,9 +272,10 @@ struct ncx_mesh {
   DevBuf overflow;  // spill area for traversal stacks deeper than the LDS part
   DevBuf err;       // error flag (stack exhausted)
   DevBuf work;      // ray counters of the persistent cast (one per part)
-  DevBuf qnodes;    // QNode[ntri - 1]: the LBVH with 16-bit quantised child boxes (cast kernel)
   DevBuf qframe;    // float[6]: quantisation origin (3) and step (3)
-  uint32_t levels = 1;  // internal-node levels of the cast tree (root = level 1)
+  DevBuf qnodes4;   // the cast tree: the LBVH with levels merged into 4-wide nodes with 16-bit
+                    // quantised child boxes (QNode4), depth-first preorder
+  uint32_t levels4 = 1;  // levels of the cast tree (root = level 1)
   bool timing = false;              // record events around the cast kernel (ncx_trimesh_set_timing)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // composite queries against shape sets (ncx_composite.hip): the shapes and the mesh's triangles

@@ -8,36 +8,30 @@
 // ray_intersection_with_triangle (src/query/ray/ray_triangle.rs:32-114) with the
 // same operation order, so every per-triangle toi is bit-identical to the
 // reference's; the hit kept is the (toi, feature) lexicographic minimum.
+#include <cmath>
+#include <cstdlib>
 #include <mutex>
+#include <vector>
 #include "ncx_internal.h"
 
 
-// Internal node for the cast kernel, 32 bytes (two 16-byte loads per visit instead of four):
-// both child boxes as 16-bit integers on one grid over the root box,
-// coordinate = fma(q, step, origin) (one fused operation).  Each minimum is rounded
-// down and each maximum up, checked with that same decode, so every decoded box contains the
-// exact one and culling stays conservative; only more boxes pass.
-struct alignas(16) QNode {
-  uint16_t l[6];  // left child: min x, y, z, max x, y, z
-  uint16_t r[6];  // right child
-  uint32_t child_l, child_r;
+// Internal node of the cast tree, 64 bytes (four 16-byte loads per visit): up to four children,
+// the binary LBVH with levels merged.  Child boxes as 16-bit integers on one grid over the root
+// box, coordinate = fma(q, step, origin) (one fused operation); each minimum is rounded down and
+// each maximum up, checked with that same decode, so every decoded box contains the exact one
+// and culling stays conservative (only more boxes pass).  Per coordinate the four children side
+// by side: lo[c] holds children 0, 1 of coordinate c in its low and high halves, hi[c] children
+// 2, 3; coordinates min x, y, z, max x, y, z.
+// child: internal 4-wide node index, (leaf position | NCX_LEAF_BIT), or RC_EMPTY.
+#define RC_EMPTY 0xFFFFFFFFu
+struct alignas(16) QNode4 {
+  uint32_t lo[6];
+  uint32_t hi[6];
+  uint32_t child[4];
 };
+static_assert(sizeof(QNode4) == 64, "QNode4 is one 64-byte line");
 
 __device__ __forceinline__ float qdec(uint32_t q, float o, float st) { return __builtin_fmaf((float)q, st, o); }
-
-// Lowest q whose decode is <= x (minimum) or highest-needed q whose decode is >= x (maximum).
-__device__ __forceinline__ uint16_t qenc(float x, float o, float st, float inv, bool is_max) {
-  if (!(st > 0.f)) return 0;
-  float t = (x - o) * inv;
-  int q = is_max ? (int)ceilf(t) : (int)floorf(t);
-  q = q < 0 ? 0 : (q > 65535 ? 65535 : q);
-  if (is_max) {
-    while (q < 65535 && qdec((uint32_t)q, o, st) < x) ++q;
-  } else {
-    while (q > 0 && qdec((uint32_t)q, o, st) > x) --q;
-  }
-  return (uint16_t)q;
-}
 
 // Quantisation frame: origin = root box minimum, step = extent / 65535 enlarged by 2^-20 so the
 // top code decodes at or above the root maximum (the kernel below checks every box).
@@ -50,33 +44,6 @@ __global__ void k_qframe(const LNode* __restrict__ nodes, float* __restrict__ fr
     frame[d] = mn[d];
     frame[3 + d] = (mx[d] - mn[d]) / 65535.f * (1.f + 9.5367431640625e-07f);
   }
-}
-
-__global__ __launch_bounds__(256) void k_quantize(const LNode* __restrict__ nodes, size_t ninternal,
-                                                  const float* __restrict__ frame, QNode* __restrict__ out,
-                                                  uint32_t* __restrict__ err) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ninternal) return;
-  const LNode& a = nodes[i];
-  float o[3] = {frame[0], frame[1], frame[2]}, st[3] = {frame[3], frame[4], frame[5]};
-  float inv[3];
-  for (int d = 0; d < 3; ++d) inv[d] = st[d] > 0.f ? 1.f / st[d] : 0.f;
-  float lb[6] = {a.lminx, a.lminy, a.lminz, a.lmaxx, a.lmaxy, a.lmaxz};
-  float rb[6] = {a.rminx, a.rminy, a.rminz, a.rmaxx, a.rmaxy, a.rmaxz};
-  QNode q;
-  bool ok = true;
-  for (int k = 0; k < 6; ++k) {
-    int d = k % 3;
-    bool is_max = k >= 3;
-    q.l[k] = qenc(lb[k], o[d], st[d], inv[d], is_max);
-    q.r[k] = qenc(rb[k], o[d], st[d], inv[d], is_max);
-    float dl = qdec(q.l[k], o[d], st[d]), dr = qdec(q.r[k], o[d], st[d]);
-    ok = ok && (is_max ? (dl >= lb[k] && dr >= rb[k]) : (dl <= lb[k] && dr <= rb[k]));
-  }
-  q.child_l = a.child_l;
-  q.child_r = a.child_r;
-  out[i] = q;
-  if (!ok) atomicOr(err, 2u);
 }
 
 #define RC_BLOCK 128
@@ -298,7 +265,7 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast(const LNode* __restrict__ nod
 #define NONE_LEAF 0xFFFFFFFFu
 #define RC_LEAF_PHASE 32u
 static_assert(RC_LEAF_PHASE >= 1 && RC_LEAF_PHASE <= 64, "leaf phase threshold out of range");
-__global__ __launch_bounds__(RC_BLOCK) void k_cast_pt(const QNode* __restrict__ nodes, const float* __restrict__ qframe,
+__global__ __launch_bounds__(RC_BLOCK) void k_cast_wide(const QNode4* __restrict__ nodes, const float* __restrict__ qframe,
                                                        const float4* __restrict__ tris,
                                                        uint32_t nleaves, uint32_t ntri_total, ncx_pose pose,
                                                        const float* __restrict__ ox, const float* __restrict__ oy,
@@ -324,7 +291,7 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_pt(const QNode* __restrict__ 
   uint32_t best_f = 0xFFFFFFFFu, node = 0, top = 0;
   int sp = 0;
   bool top_valid = false;
-  uint32_t pl0 = NONE_LEAF, pl1 = NONE_LEAF;  // leaves hit but not yet tested
+  uint32_t pl0 = NONE_LEAF, pl1 = NONE_LEAF, pl2 = NONE_LEAF, pl3 = NONE_LEAF;  // leaves hit, not yet tested
   bool fin = false;                          // the ray's traversal has ended
   // The quantisation frame exists only for meshes with internal nodes (more than one triangle).
   f3 qo = f3{0.f, 0.f, 0.f}, qs = qo;
@@ -384,6 +351,8 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_pt(const QNode* __restrict__ 
           top_valid = false;
           pl0 = NONE_LEAF;
           pl1 = NONE_LEAF;
+          pl2 = NONE_LEAF;
+          pl3 = NONE_LEAF;
           fin = false;
           has_ray = true;
         } else if (pool_empty) {
@@ -438,8 +407,12 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_pt(const QNode* __restrict__ 
       if (waiting) {
         if (pl0 != NONE_LEAF) test_leaf(pl0);
         if (pl1 != NONE_LEAF) test_leaf(pl1);
+        if (pl2 != NONE_LEAF) test_leaf(pl2);
+        if (pl3 != NONE_LEAF) test_leaf(pl3);
         pl0 = NONE_LEAF;
         pl1 = NONE_LEAF;
+        pl2 = NONE_LEAF;
+        pl3 = NONE_LEAF;
         if (fin) finish();
       }
       continue;
@@ -450,49 +423,79 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_pt(const QNode* __restrict__ 
       fin = true;
       continue;
     }
-    // One traversal step.  The order in which leaves are tested does not change the result: the
-    // kept hit is the least (toi, feature) over every tested triangle, and a box is skipped only
-    // when its entry parameter exceeds the current best toi.
+    // One traversal step over a 4-wide node.  The order in which leaves are tested does not
+    // change the result: the kept hit is the least (toi, feature) over every tested triangle, and
+    // a box is skipped only when its entry parameter exceeds the current best toi.
     const uint4* q = (const uint4*)(nodes + node);
-    const uint4 qa = q[0], qb = q[1];
-    const uint32_t cl = qb.z, cr = qb.w;
-    const float lx0 = qdec(qa.x & 0xFFFFu, qo.x, qs.x), ly0 = qdec(qa.x >> 16, qo.y, qs.y);
-    const float lz0 = qdec(qa.y & 0xFFFFu, qo.z, qs.z), lx1 = qdec(qa.y >> 16, qo.x, qs.x);
-    const float ly1 = qdec(qa.z & 0xFFFFu, qo.y, qs.y), lz1 = qdec(qa.z >> 16, qo.z, qs.z);
-    const float rx0 = qdec(qa.w & 0xFFFFu, qo.x, qs.x), ry0 = qdec(qa.w >> 16, qo.y, qs.y);
-    const float rz0 = qdec(qb.x & 0xFFFFu, qo.z, qs.z), rx1 = qdec(qb.x >> 16, qo.x, qs.x);
-    const float ry1 = qdec(qb.y & 0xFFFFu, qo.y, qs.y), rz1 = qdec(qb.y >> 16, qo.z, qs.z);
-    float tl, tr;
-    if (__ballot(zx | zy | zz) == 0) {  // no lane's ray has a zero direction component (uniform)
-      tl = slab_nz(lx0, ly0, lz0, lx1, ly1, lz1, o, inv, max_toi);
-      tr = slab_nz(rx0, ry0, rz0, rx1, ry1, rz1, o, inv, max_toi);
-    } else {
-      tl = slab(lx0, ly0, lz0, lx1, ly1, lz1, o, inv, zx, zy, zz, max_toi);
-      tr = slab(rx0, ry0, rz0, rx1, ry1, rz1, o, inv, zx, zy, zz, max_toi);
+    const uint4 qa = q[0], qb = q[1], qc = q[2], qd = q[3];
+    // lo[0..5] = qa.x qa.y qa.z qa.w qb.x qb.y, hi[0..5] = qb.z qb.w qc.x qc.y qc.z qc.w
+    const uint32_t lo[6] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y};
+    const uint32_t hi[6] = {qb.z, qb.w, qc.x, qc.y, qc.z, qc.w};
+    const uint32_t ch[4] = {qd.x, qd.y, qd.z, qd.w};
+    const bool nz = __ballot(zx | zy | zz) == 0;  // no lane's ray has a zero direction component
+    float tk[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const uint32_t* w = c < 2 ? lo : hi;
+      const int sh = (c & 1) * 16;
+      const float x0 = qdec((w[0] >> sh) & 0xFFFFu, qo.x, qs.x), y0 = qdec((w[1] >> sh) & 0xFFFFu, qo.y, qs.y);
+      const float z0 = qdec((w[2] >> sh) & 0xFFFFu, qo.z, qs.z), x1 = qdec((w[3] >> sh) & 0xFFFFu, qo.x, qs.x);
+      const float y1 = qdec((w[4] >> sh) & 0xFFFFu, qo.y, qs.y), z1 = qdec((w[5] >> sh) & 0xFFFFu, qo.z, qs.z);
+      tk[c] = nz ? slab_nz(x0, y0, z0, x1, y1, z1, o, inv, max_toi)
+                 : slab(x0, y0, z0, x1, y1, z1, o, inv, zx, zy, zz, max_toi);
     }
-    bool hl = tl <= best_t, hr = tr <= best_t;
-    if (hl && (cl & NCX_LEAF_BIT)) { pl0 = cl & ~NCX_LEAF_BIT; hl = false; }
-    if (hr && (cr & NCX_LEAF_BIT)) {
-      if (pl0 == NONE_LEAF) pl0 = cr & ~NCX_LEAF_BIT;
-      else pl1 = cr & ~NCX_LEAF_BIT;
-      hr = false;
+    // leaves hit wait for the leaf phase; internal children hit, nearest first
+    const float INF = __builtin_inff();
+    float k0 = INF, k1 = INF, k2 = INF, k3 = INF;
+    uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const uint32_t cr = ch[c];
+      const bool hit = cr != RC_EMPTY && tk[c] <= best_t;
+      if (hit && (cr & NCX_LEAF_BIT)) {
+        const uint32_t lf = cr & ~NCX_LEAF_BIT;
+        if (pl0 == NONE_LEAF) pl0 = lf;
+        else if (pl1 == NONE_LEAF) pl1 = lf;
+        else if (pl2 == NONE_LEAF) pl2 = lf;
+        else pl3 = lf;
+      } else if (hit) {
+        if (c == 0) { k0 = tk[c]; r0 = cr; }
+        else if (c == 1) { k1 = tk[c]; r1 = cr; }
+        else if (c == 2) { k2 = tk[c]; r2 = cr; }
+        else { k3 = tk[c]; r3 = cr; }
+      }
     }
-    if (hl && hr) {
-      uint32_t nearc = tl <= tr ? cl : cr, farc = tl <= tr ? cr : cl;
-      // The stack's top entry stays in a register; pushing over it moves it to memory.
+    // sorting network on (entry parameter, child): (0,1) (2,3) (0,2) (1,3) (1,2)
+    auto cswap = [](float& ka, uint32_t& ra, float& kb, uint32_t& rb) {
+      const bool sw = kb < ka;
+      const float kt = sw ? kb : ka;
+      kb = sw ? ka : kb;
+      ka = kt;
+      const uint32_t rt = sw ? rb : ra;
+      rb = sw ? ra : rb;
+      ra = rt;
+    };
+    cswap(k0, r0, k1, r1);
+    cswap(k2, r2, k3, r3);
+    cswap(k0, r0, k2, r2);
+    cswap(k1, r1, k3, r3);
+    cswap(k1, r1, k2, r2);
+    // The stack's top entry stays in a register; pushing over it moves it to memory.
+    auto push = [&](uint32_t x) {
       if (top_valid) {
         if (sp < RC_LDS_STACK) lstack[sp * RC_BLOCK + tid] = top;
         else if (sp < RC_LDS_STACK + (int)gdepth) gstack[sp - RC_LDS_STACK] = top;
         else { atomicOr(err, 1u); fin = true; }
         sp++;
       }
-      top = farc;
+      top = x;
       top_valid = true;
-      node = nearc;
-    } else if (hl) {
-      node = cl;
-    } else if (hr) {
-      node = cr;
+    };
+    if (k0 != INF) {
+      if (k3 != INF) push(r3);
+      if (k2 != INF) push(r2);
+      if (k1 != INF) push(r1);
+      node = r0;
     } else if (top_valid) {
       node = top;
       top_valid = false;
@@ -506,36 +509,106 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_pt(const QNode* __restrict__ 
   }
 }
 
-// Lays the cast tree out in depth-first preorder (left child first), so a node's left child is
-// the next 32 bytes and descending often stays in the cache line just loaded.  One-time host
-// pass at mesh creation; leaf references (triangle positions) are unchanged.
-static int preorder_nodes(ncx_mesh* m) {
+// Quantisation on the host: qdec as the kernel decodes (fmaf is correctly rounded on both sides),
+// and the lowest q whose decode is <= x (a minimum) or the highest-needed q whose decode is >= x
+// (a maximum).
+static inline float qdec_h(uint32_t q, float o, float st) { return std::fmaf((float)q, st, o); }
+static inline uint16_t qenc_h(float x, float o, float st, float inv, bool is_max) {
+  if (!(st > 0.f)) return 0;
+  float t = (x - o) * inv;
+  int q = is_max ? (int)std::ceil(t) : (int)std::floor(t);
+  q = q < 0 ? 0 : (q > 65535 ? 65535 : q);
+  if (is_max) {
+    while (q < 65535 && qdec_h((uint32_t)q, o, st) < x) ++q;
+  } else {
+    while (q > 0 && qdec_h((uint32_t)q, o, st) > x) --q;
+  }
+  return (uint16_t)q;
+}
+
+// The cast tree: each node takes a binary node's two children and, while it has fewer than four,
+// replaces the internal child of largest box surface by that child's two children.  Nodes in
+// depth-first preorder (a node's first internal child follows it, often in the cache line just
+// loaded).  One-time host pass at mesh creation over the exact binary boxes.
+static int build_wide(ncx_mesh* m) {
   const size_t ni = m->ntri - 1;
-  std::vector<QNode> src(ni), dst(ni);
-  NCX_HIP_CHECK(hipMemcpy(src.data(), m->qnodes.p, ni * sizeof(QNode), hipMemcpyDeviceToHost));
-  std::vector<uint32_t> newid(ni, 0xFFFFFFFFu), stack, depth(ni, 0);
-  stack.reserve(128);
-  stack.push_back(0);
-  uint32_t next = 0, max_depth = 0;
-  while (!stack.empty()) {
-    uint32_t v = stack.back();
-    stack.pop_back();
-    NCX_REQUIRE(v < ni && newid[v] == 0xFFFFFFFFu, "ncx_trimesh_create: malformed tree");
-    newid[v] = next++;
-    max_depth = std::max(max_depth, depth[v]);
-    const QNode& q = src[v];
-    if (!(q.child_r & NCX_LEAF_BIT)) { depth[q.child_r] = depth[v] + 1; stack.push_back(q.child_r); }
-    if (!(q.child_l & NCX_LEAF_BIT)) { depth[q.child_l] = depth[v] + 1; stack.push_back(q.child_l); }
+  std::vector<LNode> L(ni);
+  float fr[6];
+  NCX_HIP_CHECK(hipMemcpy(L.data(), m->bvh.nodes.p, ni * sizeof(LNode), hipMemcpyDeviceToHost));
+  NCX_HIP_CHECK(hipMemcpy(fr, m->qframe.p, sizeof(fr), hipMemcpyDeviceToHost));
+  float inv[3];
+  for (int d = 0; d < 3; ++d) inv[d] = fr[3 + d] > 0.f ? 1.f / fr[3 + d] : 0.f;
+  struct Ch {
+    float b[6];
+    uint32_t ref;
+  };
+  auto area = [](const float* b) {
+    const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return dx * dy + dy * dz + dz * dx;
+  };
+  std::vector<QNode4> out;
+  out.reserve(ni / 2 + 4);
+  uint32_t max_depth = 0;
+  bool ok = true;
+  // explicit stack of (binary node, out index, depth); a node's slot is reserved when it is
+  // reached, its children's slots as they are popped (preorder: first child first)
+  struct Item {
+    uint32_t v, slot_of_parent, cidx, depth;
+  };
+  std::vector<Item> st;
+  st.push_back({0u, 0xFFFFFFFFu, 0u, 0u});
+  while (!st.empty()) {
+    const Item it = st.back();
+    st.pop_back();
+    if (it.v >= ni) {
+      ncx_set_error("ncx_trimesh_create: malformed tree");
+      return NCX_E_INVALID;
+    }
+    const uint32_t idx = (uint32_t)out.size();
+    out.push_back(QNode4{});
+    if (it.slot_of_parent != 0xFFFFFFFFu) out[it.slot_of_parent].child[it.cidx] = idx;
+    max_depth = std::max(max_depth, it.depth);
+    const LNode& a = L[it.v];
+    Ch c[4];
+    c[0] = Ch{{a.lminx, a.lminy, a.lminz, a.lmaxx, a.lmaxy, a.lmaxz}, a.child_l};
+    c[1] = Ch{{a.rminx, a.rminy, a.rminz, a.rmaxx, a.rmaxy, a.rmaxz}, a.child_r};
+    int n = 2;
+    while (n < 4) {
+      int best = -1;
+      float ba = -1.f;
+      for (int i = 0; i < n; ++i)
+        if (!(c[i].ref & NCX_LEAF_BIT) && area(c[i].b) > ba) { ba = area(c[i].b); best = i; }
+      if (best < 0) break;
+      const LNode& u = L[c[best].ref];
+      c[best] = Ch{{u.lminx, u.lminy, u.lminz, u.lmaxx, u.lmaxy, u.lmaxz}, u.child_l};
+      c[n++] = Ch{{u.rminx, u.rminy, u.rminz, u.rmaxx, u.rmaxy, u.rmaxz}, u.child_r};
+    }
+    QNode4& q = out[idx];
+    for (int k = 0; k < 6; ++k) { q.lo[k] = 0; q.hi[k] = 0; }
+    for (int i = 0; i < 4; ++i) {
+      if (i >= n) { q.child[i] = RC_EMPTY; continue; }
+      for (int k = 0; k < 6; ++k) {
+        const int d = k % 3;
+        const bool is_max = k >= 3;
+        const uint16_t v = qenc_h(c[i].b[k], fr[d], fr[3 + d], inv[d], is_max);
+        const float dv = qdec_h(v, fr[d], fr[3 + d]);
+        ok = ok && (is_max ? dv >= c[i].b[k] : dv <= c[i].b[k]);
+        uint32_t& w = i < 2 ? q.lo[k] : q.hi[k];
+        w |= (uint32_t)v << ((i & 1) * 16);
+      }
+      q.child[i] = c[i].ref;  // internal children are patched when they are emitted
+    }
+    // push internal children in reverse so the first is emitted first
+    for (int i = n - 1; i >= 0; --i)
+      if (!(c[i].ref & NCX_LEAF_BIT)) st.push_back({c[i].ref, idx, (uint32_t)i, it.depth + 1});
   }
-  m->levels = max_depth + 1;
-  NCX_REQUIRE(next == ni, "ncx_trimesh_create: tree does not reach every node");
-  for (size_t v = 0; v < ni; ++v) {
-    QNode q = src[v];
-    if (!(q.child_l & NCX_LEAF_BIT)) q.child_l = newid[q.child_l];
-    if (!(q.child_r & NCX_LEAF_BIT)) q.child_r = newid[q.child_r];
-    dst[newid[v]] = q;
-  }
-  NCX_HIP_CHECK(hipMemcpy(m->qnodes.p, dst.data(), ni * sizeof(QNode), hipMemcpyHostToDevice));
+  NCX_REQUIRE(ok, "ncx_trimesh_create: a quantised 4-wide box does not contain its exact box");
+  m->levels4 = max_depth + 1;
+  NCX_REQUIRE(3 * m->levels4 + 1 <= RC_LDS_STACK + RC_GLOBAL_STACK,
+              "ncx_trimesh_create: tree too deep for the cast's traversal stack");
+  int rc;
+  if ((rc = m->qnodes4.reserve(out.size() * sizeof(QNode4)))) return rc;
+  NCX_HIP_CHECK(hipMemcpy(m->qnodes4.p, out.data(), out.size() * sizeof(QNode4), hipMemcpyHostToDevice));
   return NCX_OK;
 }
 
@@ -554,7 +627,7 @@ extern "C" int ncx_trimesh_create(ncx_ctx* ctx, const float* points_xyz, size_t 
   auto fail = [&](int code) {
     m->bvh.release();
     m->tris.release(); m->boxes.release(); m->pts.release(); m->idx.release(); m->err.release(); m->work.release();
-    m->qnodes.release(); m->qframe.release();
+    m->qframe.release(); m->qnodes4.release();
     delete m;
     return code;
   };
@@ -578,23 +651,13 @@ extern "C" int ncx_trimesh_create(ncx_ctx* ctx, const float* points_xyz, size_t 
   hipLaunchKernelGGL(k_tri_reorder, dim3(ceil_div(ntriangles, 256)), dim3(256), 0, ctx->stream, m->pts.as<float>(),
                      m->idx.as<uint32_t>(), m->bvh.order.as<uint32_t>(), ntriangles, m->tris.as<float4>());
   if ((rc = m->qframe.reserve(64)) || (rc = fill_u32(ctx, m->qframe.as<uint32_t>(), 16, 0u))) return fail(rc);
-  if (ntriangles > 1) {
-    if ((rc = m->qnodes.reserve((ntriangles - 1) * sizeof(QNode)))) return fail(rc);
+  if (ntriangles > 1)
     hipLaunchKernelGGL(k_qframe, dim3(1), dim3(64), 0, ctx->stream, m->bvh.nodes.as<LNode>(), m->qframe.as<float>());
-    hipLaunchKernelGGL(k_quantize, dim3(ceil_div(ntriangles - 1, 256)), dim3(256), 0, ctx->stream,
-                       m->bvh.nodes.as<LNode>(), ntriangles - 1, m->qframe.as<float>(), m->qnodes.as<QNode>(),
-                       m->err.as<uint32_t>());
-  }
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
     ncx_set_error("ncx_trimesh_create: build failed");
     return fail(NCX_E_HIP);
   }
-  uint32_t qerr = 0;
-  if (hipMemcpy(&qerr, m->err.p, 4, hipMemcpyDeviceToHost) != hipSuccess || qerr != 0) {
-    ncx_set_error("ncx_trimesh_create: a quantised node box does not contain its exact box");
-    return fail(NCX_E_HIP);
-  }
-  if (ntriangles > 2 && (rc = preorder_nodes(m))) return fail(rc);
+  if (ntriangles > 1 && (rc = build_wide(m))) return fail(rc);
   *out = m;
   return NCX_OK;
 }
@@ -603,7 +666,7 @@ extern "C" int ncx_trimesh_destroy(ncx_mesh* m) {
   if (!m) return NCX_OK;
   m->bvh.release();
   m->tris.release(); m->boxes.release(); m->pts.release(); m->idx.release(); m->overflow.release(); m->err.release(); m->work.release();
-  m->qnodes.release(); m->qframe.release();
+  m->qframe.release(); m->qnodes4.release();
   m->release_composite();
   if (m->ev0) (void)hipEventDestroy(m->ev0);
   if (m->ev1) (void)hipEventDestroy(m->ev1);
@@ -629,13 +692,13 @@ static int cast_impl(ncx_mesh* m, const ncx_pose* pose, const float* const r[6],
   else {
     // persistent grid: about as many blocks as are resident (LDS stacks allow 13 per CU)
     unsigned pblocks = std::min<unsigned>(RC_GRID_BLOCKS, (unsigned)ctx->num_cus * 13);
-    NCX_REQUIRE(m->qframe.p && (m->ntri == 1 || m->qnodes.p), "ncx_trimesh_cast_rays: mesh nodes missing");
+    NCX_REQUIRE(m->qframe.p && (m->ntri == 1 || m->qnodes4.p), "ncx_trimesh_cast_rays: mesh nodes missing");
     if ((rc = fill_u32(ctx, m->work.as<uint32_t>(), RC_PARTS * RC_PART_STRIDE, 0u))) return rc;
     if (m->timing) NCX_HIP_CHECK(hipEventRecord(m->ev0, ctx->stream));
-    hipLaunchKernelGGL(k_cast_pt, dim3(pblocks), dim3(RC_BLOCK), 0, ctx->stream, m->qnodes.as<QNode>(),
-                       m->qframe.as<float>(), m->tris.as<float4>(), (uint32_t)m->ntri, (uint32_t)m->ntri, *pose, r[0], r[1], r[2], r[3], r[4],
-                       r[5], (uint32_t)n, max_toi, toi, nrm, feat, m->overflow.as<uint32_t>(), (uint32_t)RC_GLOBAL_STACK,
-                       m->err.as<uint32_t>(), m->work.as<uint32_t>());
+    hipLaunchKernelGGL(k_cast_wide, dim3(pblocks), dim3(RC_BLOCK), 0, ctx->stream, m->qnodes4.as<QNode4>(),
+                       m->qframe.as<float>(), m->tris.as<float4>(), (uint32_t)m->ntri, (uint32_t)m->ntri, *pose, r[0],
+                       r[1], r[2], r[3], r[4], r[5], (uint32_t)n, max_toi, toi, nrm, feat, m->overflow.as<uint32_t>(),
+                       (uint32_t)RC_GLOBAL_STACK, m->err.as<uint32_t>(), m->work.as<uint32_t>());
     NCX_HIP_CHECK(hipGetLastError());
     if (m->timing) NCX_HIP_CHECK(hipEventRecord(m->ev1, ctx->stream));
   }
@@ -644,7 +707,7 @@ static int cast_impl(ncx_mesh* m, const ncx_pose* pose, const float* const r[6],
   return ncx_trimesh_check(m);
 }
 
-// Measurement: events around each cast's traversal kernel (k_cast_pt), read back with
+// Measurement: events around each cast's traversal kernel (k_cast_wide), read back with
 // ncx_trimesh_kernel_ms after the cast has completed.
 extern "C" int ncx_trimesh_set_timing(ncx_mesh* m, int enable) {
   NCX_REQUIRE(m, "ncx_trimesh_set_timing: null argument");
@@ -670,7 +733,7 @@ extern "C" int ncx_trimesh_check(ncx_mesh* m) {
   NCX_HIP_CHECK(hipMemcpyAsync(&err, m->err.p, 4, hipMemcpyDeviceToHost, m->ctx->stream));
   NCX_HIP_CHECK(hipStreamSynchronize(m->ctx->stream));
   if (err) {
-    ncx_set_error("ncx_trimesh_cast_rays: traversal stack exhausted (tree deeper than 128 levels)");
+    ncx_set_error("ncx_trimesh_cast_rays: traversal stack exhausted");
     return NCX_E_CAPACITY;
   }
   return NCX_OK;
@@ -737,10 +800,11 @@ extern "C" int ncx_trimesh_cast_rays_noalloc(ncx_mesh* m, const ncx_pose* pose, 
     sp = g_scratch_ptr[ctx->device];
   }
   NCX_REQUIRE(sp, "ncx_trimesh_cast_rays_noalloc: scratch not set up on this device (ncx_init)");
-  NCX_REQUIRE(m->qframe.p && (m->ntri == 1 || m->qnodes.p), "ncx_trimesh_cast_rays_noalloc: mesh nodes missing");
-  // A traversal keeps at most `levels` pending subtrees: one in a register, RC_LDS_STACK in LDS,
-  // the rest in its spill stack.
-  const uint32_t gdepth = m->levels + 1 > RC_LDS_STACK ? m->levels + 1 - RC_LDS_STACK : 1u;
+  NCX_REQUIRE(m->qframe.p && (m->ntri == 1 || m->qnodes4.p), "ncx_trimesh_cast_rays_noalloc: mesh nodes missing");
+  // A traversal keeps at most 3 pending subtrees per level (the children it does not descend
+  // into): one in a register, RC_LDS_STACK in LDS, the rest in its spill stack.
+  const uint32_t need = 3 * m->levels4 + 1;
+  const uint32_t gdepth = need > RC_LDS_STACK ? need - RC_LDS_STACK : 1u;
   const size_t counters = (size_t)RC_PARTS * RC_PART_STRIDE;
   const size_t per_block = (size_t)RC_BLOCK * gdepth;
   unsigned pblocks = std::min<unsigned>(RC_GRID_BLOCKS, (unsigned)ctx->num_cus * 13);
@@ -748,14 +812,14 @@ extern "C" int ncx_trimesh_cast_rays_noalloc(ncx_mesh* m, const ncx_pose* pose, 
   const size_t fit = (RC_SCRATCH_WORDS - counters) / per_block;
   if (fit < pblocks) pblocks = (unsigned)fit;
   if (pblocks < std::min<unsigned>((unsigned)ctx->num_cus, ceil_div(n, RC_BLOCK))) {
-    ncx_set_error("ncx_trimesh_cast_rays_noalloc: the mesh's " + std::to_string(m->levels) +
+    ncx_set_error("ncx_trimesh_cast_rays_noalloc: the mesh's " + std::to_string(m->levels4) +
                   "-level tree needs " + std::to_string(per_block * 4) + " bytes of spill stack per block; " +
                   std::to_string(RC_SCRATCH_BYTES) + " bytes of scratch do not hold one block per CU");
     return NCX_E_CAPACITY;
   }
   int rc;
   if ((rc = fill_u32(ctx, sp, counters, 0u))) return rc;
-  hipLaunchKernelGGL(k_cast_pt, dim3(pblocks), dim3(RC_BLOCK), 0, ctx->stream, m->qnodes.as<QNode>(),
+  hipLaunchKernelGGL(k_cast_wide, dim3(pblocks), dim3(RC_BLOCK), 0, ctx->stream, m->qnodes4.as<QNode4>(),
                      m->qframe.as<float>(), m->tris.as<float4>(), (uint32_t)m->ntri, (uint32_t)m->ntri, *pose, r[0],
                      r[1], r[2], r[3], r[4], r[5], (uint32_t)n, max_toi, toi, nrm, feat, sp + counters, gdepth,
                      m->err.as<uint32_t>(), sp);
