@@ -485,6 +485,20 @@ def bench_queries(args, ncx, torch, ctx):
     ib = torch.as_tensor(w["ib"].astype(np.int32), device=dev)
     t = _timed(torch, lambda: query.time_of_impact(ss, poses, velt, ia, ib, 10.0, 0.0), 5)
     out["time_of_impact"] = {"pairs": len(w["ia"]), "ms": t * 1e3, "pairs_per_s": len(w["ia"]) / t}
+    if not args.no_cpu:
+        from oracle import pyoracle as orc
+        from ncollide_amd.math import to_oracle_poses
+        rec = w["records"]
+        oss = orc.ShapeSet(rec["type"], np.stack([rec["p0"], rec["p1"], rec["p2"]], 1), rec["hull_offset"],
+                           rec["hull_count"], w["pool"], dtype=np.float32)
+        k = min(65536, len(w["ia"]))
+        nth = min(16, os.cpu_count() or 1)
+        tc = time.perf_counter()
+        orc.time_of_impact(oss, to_oracle_poses(w["poses"]), vel, w["ia"][:k], w["ib"][:k], np.float32(10.0),
+                           np.float32(0.0), nthreads=nth)
+        out["time_of_impact"]["cpu_baseline"] = {
+            "value": k / (time.perf_counter() - tc), "unit": "pairs/s", "cores": nth, "kind": "port",
+            "sample": f"first {k} pairs, query::time_of_impact (GJK ray cast on the Minkowski difference)"}
     # scene shapes: ray casts, world first interference
     sw = workloads.full_pipeline(seed=5, n=args.scene_shapes)
     sss = ncx.ShapeSet(records=sw["records"], pool=sw["pool"], ctx=ctx)
@@ -497,6 +511,20 @@ def bench_queries(args, ncx, torch, ctx):
     ot, dt_, it = (torch.as_tensor(a, device=dev) for a in (o, d, idx))
     t = _timed(torch, lambda: query.cast_rays(sss, sp, it, ot, dt_, 50.0, True), 5)
     out["shape_ray_casts"] = {"rays": nr, "ms": t * 1e3, "rays_per_s": nr / t}
+    if not args.no_cpu:
+        from oracle import pyoracle as orc
+        from ncollide_amd.math import to_oracle_poses
+        rec = sw["records"]
+        soss = orc.ShapeSet(rec["type"], np.stack([rec["p0"], rec["p1"], rec["p2"]], 1), rec["hull_offset"],
+                            rec["hull_count"], sw["pool"], dtype=np.float32)
+        k = 262144
+        nth = min(16, os.cpu_count() or 1)
+        tc = time.perf_counter()
+        orc.shape_rays(soss, to_oracle_poses(sw["poses"]), idx[:k].astype(np.uint32), o[:k], d[:k], np.float32(50.0),
+                       True, nthreads=nth)
+        out["shape_ray_casts"]["cpu_baseline"] = {
+            "value": k / (time.perf_counter() - tc), "unit": "rays/s", "cores": nth, "kind": "port",
+            "sample": f"first {k} rays, RayCast::toi_and_normal_with_ray of each ray's shape"}
     world = World(sss, margin=sw["margin"], prediction=0.0, max_pairs=args.scene_max_pairs)
     world.set_contacts(False)
     world.step(sp, use_graph=False)
