@@ -24,6 +24,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+FP32_PEAK = 157.3e12  # dense FP32 vector peak of an MI355X, FLOP/s
 HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
 
 
@@ -233,8 +234,13 @@ def cpu_baseline_hulls(w, nsample=16384):
     threads = min(os.cpu_count() or 1, 64)
     t = time.perf_counter()
     c = orc.contact(oss, op, ia, ib, 0.0, nthreads=threads)
-    orc.distance(oss, op, ia, ib, nthreads=threads)
+    _, dit = orc.distance(oss, op, ia, ib, nthreads=threads)
     dt = time.perf_counter() - t
+    # FP32 work of the support scans: 5 flops (3 mul, 2 add) per hull vertex per support call,
+    # support calls = the initial CSO point of each query + one per GJK and EPA iteration
+    nv = rec["hull_count"].astype(np.float64)
+    calls = 2.0 + c["gjk_iters"] + c["epa_iters"] + dit
+    flops_per_pair = float(np.mean(5.0 * (nv[ia] + nv[ib]) * calls))
     n1 = min(2048, len(ia))
     t = time.perf_counter()
     orc.contact(oss, op, ia[:n1], ib[:n1], 0.0, nthreads=1)
@@ -243,7 +249,8 @@ def cpu_baseline_hulls(w, nsample=16384):
     return dict(value=len(ia) / dt, unit="pairs/s (contact + distance)", cores=threads, kind="port",
                 sample=f"first {len(ia)} pairs of the same workload, query::contact + query::distance "
                        f"(GJK/EPA) per pair, {threads} threads",
-                single_thread_pairs_per_s=st, epa_iters_per_pair=float(c["epa_iters"].mean()))
+                single_thread_pairs_per_s=st, epa_iters_per_pair=float(c["epa_iters"].mean()),
+                flops_per_pair=flops_per_pair)
 
 
 def bench_learned(args, ncx, torch, ctx):
@@ -694,7 +701,18 @@ def main():
             h = bench_hulls(args, ncx, torch, ctx)
             sec["hull_pairs"] = {k: v for k, v in h.items() if k not in ("w", "gen_s")}
             if not args.no_cpu:
-                sec["hull_pairs"]["cpu_baseline"] = cpu_baseline_hulls(h["w"])
+                cb = cpu_baseline_hulls(h["w"])
+                fpp = cb.pop("flops_per_pair")
+                sec["hull_pairs"]["cpu_baseline"] = cb
+                # the FP32 leg of the roofline: support-scan flops per pair (oracle's iteration
+                # counts) over the measured time; roofline pairs/s = 1 / max(bytes/BW, flops/FP32 peak)
+                hp = sec["hull_pairs"]
+                t_pair = (hp["contact_ms"] + hp["distance_ms"]) / 1e3 / hp["pairs"]
+                roof = 1.0 / max(hp["roofline"]["bytes_per_pair"] / HBM_PEAK, fpp / FP32_PEAK)
+                hp["roofline"]["valu"] = {"flops_per_pair": fpp, "achieved_tflops": fpp / t_pair / 1e12,
+                                          "peak_tflops": FP32_PEAK / 1e12, "frac": fpp / t_pair / FP32_PEAK}
+                hp["roofline"]["roofline_pairs_per_s"] = roof
+                hp["roofline"]["frac_of_roofline_pairs_per_s"] = (1.0 / t_pair) / roof
         mlp = None
         if want("learned"):
             log("learned warm start ...")
