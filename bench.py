@@ -12,9 +12,14 @@ plain GJK over 64 coherent steps, the 1,048,576-shape scene step, and the
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
 import argparse
+import csv
 import json
 import os
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import time
 
 import ctypes as C
@@ -40,6 +45,8 @@ def parse():
     p.add_argument("--copy-calibration", action="store_true",
                    help="copy 256 MiB device to device once before the timed casts (the reference dispatch "
                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
+    p.add_argument("--no-traffic", action="store_true",
+                   help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes over the ray cast")
     p.add_argument("--rays", type=int, default=1 << 20)
     p.add_argument("--hull-pairs", type=int, default=1 << 18)
     p.add_argument("--train-samples", type=int, default=2_000_000)
@@ -644,12 +651,70 @@ def cpu_model():
     return None
 
 
+def _pmc_pass(exe, counter, args):
+    """One rocprofv3 --pmc pass over a short ray-cast run (a child process): per-dispatch values of
+    `counter` (KiB) for k_cast_wide and for the 256 MiB calibration copy."""
+    d = tempfile.mkdtemp(prefix="ncx_pmc_", dir="/tmp")
+    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable, "-u",
+           os.path.abspath(__file__), "--quick", "--no-cpu", "--no-traffic", "--copy-calibration", "--steps", "3",
+           "--warmup", "1", "--rays", str(args.rays)]
+    p = subprocess.Popen(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.DEVNULL,
+                         stderr=subprocess.DEVNULL, start_new_session=True)
+    try:
+        p.wait(timeout=240)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.wait()
+        shutil.rmtree(d, ignore_errors=True)
+        return None
+    cast, copy = [], []
+    for root, _, files in os.walk(d):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                with open(os.path.join(root, f)) as fh:
+                    for r in csv.DictReader(fh):
+                        if r.get("Counter_Name") != counter:
+                            continue
+                        if "k_cast_wide" in r["Kernel_Name"]:
+                            cast.append(float(r["Counter_Value"]))
+                        elif "copyBuffer" in r["Kernel_Name"]:
+                            copy.append(float(r["Counter_Value"]))
+    shutil.rmtree(d, ignore_errors=True)
+    if not cast or not copy:
+        return None
+    return {"cast_kib": sum(cast) / len(cast), "dispatches": len(cast), "copy_kib": max(copy)}
+
+
+def measure_traffic(args):
+    """HBM bytes per k_cast_wide launch, read in this run: rocprofv3 FETCH_SIZE, then WRITE_SIZE, in two
+    child runs of the cast, each scaled by what the same counter reads for a device-to-device copy of
+    256 MiB in that run.  Runs before this process touches the GPU; None when rocprofv3 is absent or a
+    pass fails."""
+    exe = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3") else None)
+    if exe is None:
+        return None
+    f = _pmc_pass(exe, "FETCH_SIZE", args)
+    w = _pmc_pass(exe, "WRITE_SIZE", args) if f else None
+    if not f or not w:
+        return None
+    copy_kib = 256.0 * 1024.0
+    ff, fw = copy_kib / f["copy_kib"], copy_kib / w["copy_kib"]
+    fetch, write = f["cast_kib"] * ff * 1024.0, w["cast_kib"] * fw * 1024.0
+    return {"bytes_per_launch": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
+            "fetch_kib_raw": f["cast_kib"], "write_kib_raw": w["cast_kib"], "fetch_factor": ff, "write_factor": fw,
+            "dispatches": f["dispatches"], "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes in this run"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
                                    "design note only)", "n_gpus": args.gpus}))
         sys.exit(2)
+    traffic = None
+    if not args.no_traffic:
+        log("ray-cast HBM traffic (rocprofv3 FETCH_SIZE, WRITE_SIZE passes) ...")
+        traffic = measure_traffic(args)
     import torch
     import ncollide_amd as ncx
     ctx = ncx.default_context()
@@ -679,12 +744,20 @@ def main():
                                "max_toi 100, first-hit toi + normal + triangle id",
                    "rays": r["n"], "triangles": 131072, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": r["achieved"] / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": r["achieved"] / HBM_PEAK, "traffic": _traffic("raycast"),
+                     "frac": r["achieved"] / HBM_PEAK,
+                     "traffic": traffic["bytes_per_launch"] if traffic else _traffic("raycast"),
                      "kernel": "k_cast_wide", "kernel_ms": r["kernel_ms"],
                      "alg_bytes_per_launch": r["alg_bytes"], "nodes_per_ray": r["nodes_per_ray"],
                      "tris_per_ray": r["tris_per_ray"]},
         "hit_fraction": r["hits"],
     }
+    if traffic:
+        # counter bytes over the kernel time measured in this run
+        traffic["hbm_gbs"] = traffic["bytes_per_launch"] / (r["kernel_ms"] / 1e3) / 1e9
+        traffic["hbm_frac"] = traffic["hbm_gbs"] * 1e9 / HBM_PEAK
+        result["roofline"]["traffic_measured"] = traffic
+    else:
+        result["roofline"]["traffic_source"] = "profiles/raycast_traffic.json (committed rocprofv3 passes)"
     if not args.no_cpu:
         log("cpu baseline ...")
         cb = cpu_baseline_rays(r["pts"], r["tri"], r["o"], r["d"])

@@ -265,6 +265,14 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast(const LNode* __restrict__ nod
 #define NONE_LEAF 0xFFFFFFFFu
 #define RC_LEAF_PHASE 32u
 static_assert(RC_LEAF_PHASE >= 1 && RC_LEAF_PHASE <= 64, "leaf phase threshold out of range");
+// Finished rays are staged in LDS by chunk and written out a chunk at a time.
+#define RC_WAVES (RC_BLOCK / 64)
+#define RC_CAST_STACK 12
+#define RC_SLOTS 3u
+#define RC_STG_BASE (5u * RC_CHUNK + RC_CHUNK / 4u)
+#define RC_STG_WORDS (RC_STG_BASE + 1u)
+static_assert(RC_CHUNK == 64u, "one staged entry per lane of a chunk");
+static_assert(RC_SLOTS >= 2u && RC_SLOTS <= 4u, "slot state is 16 bits a slot in one 64-bit word");
 __global__ __launch_bounds__(RC_BLOCK) void k_cast_wide(const QNode4* __restrict__ nodes, const float* __restrict__ qframe,
                                                        const float4* __restrict__ tris,
                                                        uint32_t nleaves, uint32_t ntri_total, ncx_pose pose,
@@ -275,9 +283,20 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_wide(const QNode4* __restrict
                                                        float* __restrict__ nrm_out, uint32_t* __restrict__ feat_out,
                                                        uint32_t* __restrict__ ovf, uint32_t gdepth,
                                                        uint32_t* __restrict__ err, uint32_t* __restrict__ work) {
-  __shared__ uint32_t lstack[RC_LDS_STACK * RC_BLOCK];
+  __shared__ uint32_t lstack[RC_CAST_STACK * RC_BLOCK];
+  // Result staging: RC_SLOTS chunk slots per wave.  A lane that finishes a ray of a staged chunk
+  // puts (toi, normal, feature) at the ray's place in its slot; once every ray of the chunk is
+  // in, the wave writes the slot out with whole-line stores (a scattered 4-byte store per
+  // finished ray left most output lines partly written in L2, and their write-backs came to
+  // ~5x the output bytes).  Chunk k (k = 0, 1, ... in the order the wave takes them) goes into
+  // slot k % RC_SLOTS; what chunk k - RC_SLOTS left there unfinished is written out as it
+  // stands, and that chunk's rays still running write their results directly.
+  // A slot is RC_STG_WORDS words: toi [0, 64), normals [64, 256), features [256, 320), one byte
+  // per entry set once it is staged, then the chunk's first ray.
+  __shared__ uint32_t stg[RC_WAVES * RC_SLOTS * RC_STG_WORDS];
   const int tid = threadIdx.x;
   const int lane = __lane_id();
+  const uint32_t wv = (uint32_t)tid / 64u;
   const size_t gtid = (size_t)blockIdx.x * RC_BLOCK + tid;
   uint32_t* gstack = ovf + gtid * gdepth;
   iso m;
@@ -309,7 +328,78 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_wide(const QNode4* __restrict
   bool pool_empty = false;
   const uint32_t part0 = wave % RC_PARTS;
   uint32_t part_k = 0;
+  // Slot state (the same in every lane), 16 bits a slot: the chunk's length (bits 0-7, 0 = free)
+  // and the count of its rays not yet staged (bits 8-15).  Chunk k is still staged while at most
+  // k + RC_SLOTS chunks have been taken.
+  uint64_t st = 0;
+  uint32_t chunks = 0, cb = 0;  // chunks taken, first ray of the current chunk
+  uint32_t myq = 0, me = 0;     // the lane's ray: its chunk number, its entry | slot << 6
+  bool staged = false;          // the lane staged a result since the last count
+  // Writes slot s out: every entry, or (partial) only those already staged.
+  auto flush = [&](uint32_t s, bool partial) {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint32_t* sl = stg + (wv * RC_SLOTS + s) * RC_STG_WORDS;
+    const uint32_t base = sl[RC_STG_BASE], len = (uint32_t)(st >> (16u * s)) & 0xFFu;
+    const uint8_t* ok = (const uint8_t*)(sl + 5 * RC_CHUNK);
+    if ((uint32_t)lane < len && (!partial || ok[lane])) {
+      toi_out[base + lane] = __uint_as_float(sl[lane]);
+      feat_out[base + lane] = sl[4 * RC_CHUNK + lane];
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < 3; ++c) {
+      const uint32_t x = c * RC_CHUNK + (uint32_t)lane;
+      if (x < 3u * len && (!partial || ok[x / 3u])) nrm_out[3u * (size_t)base + x] = __uint_as_float(sl[RC_CHUNK + x]);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  };
   for (;;) {
+    // a ray whose traversal and leaf tests are over is finished here, in one place
+    auto finish = [&]() {
+      float t = -1.f;
+      f3 wn = f3{0.f, 0.f, 0.f};
+      if (best_f != 0xFFFFFFFFu) {
+        wn = xform_vec(m, best_n);  // res.normal = m * res.normal (ray_trimesh.rs:47)
+        t = best_t;
+      }
+      if (chunks <= myq + RC_SLOTS) {
+        uint32_t* sl = stg + (wv * RC_SLOTS + (me >> 6)) * RC_STG_WORDS;
+        const uint32_t e = me & 63u;
+        sl[e] = __float_as_uint(t);
+        sl[RC_CHUNK + 3u * e] = __float_as_uint(wn.x);
+        sl[RC_CHUNK + 3u * e + 1] = __float_as_uint(wn.y);
+        sl[RC_CHUNK + 3u * e + 2] = __float_as_uint(wn.z);
+        sl[4 * RC_CHUNK + e] = best_f;
+        ((uint8_t*)(sl + 5 * RC_CHUNK))[e] = 1;
+        staged = true;
+      } else {
+        toi_out[r] = t;
+        nrm_out[3 * (size_t)r] = wn.x;
+        nrm_out[3 * (size_t)r + 1] = wn.y;
+        nrm_out[3 * (size_t)r + 2] = wn.z;
+        feat_out[r] = best_f;
+      }
+      has_ray = false;
+    };
+    if (has_ray && fin && pl0 == NONE_LEAF) finish();
+    // count the results staged; write out the chunks now complete
+    {
+      if (__ballot(staged)) {
+#pragma unroll
+        for (uint32_t s = 0; s < RC_SLOTS; ++s)
+          st -= (uint64_t)__popcll(__ballot(staged && (me >> 6) == s)) << (16u * s + 8u);
+        staged = false;
+#pragma unroll
+        for (uint32_t s = 0; s < RC_SLOTS; ++s) {
+          const uint32_t v = (uint32_t)(st >> (16u * s)) & 0xFFFFu;
+          if (v != 0 && (v >> 8) == 0) {
+            flush(s, false);
+            st &= ~(0xFFFFull << (16u * s));
+          }
+        }
+      }
+    }
     // refill lanes without a ray
     const bool want = !has_ray && !exhausted;
     const uint64_t need = __ballot(want);
@@ -322,10 +412,21 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_wide(const QNode4* __restrict
           const uint32_t hi = (uint32_t)(((uint64_t)nrays * (p + 1)) / RC_PARTS);
           uint32_t base = 0;
           if (lane == leader) base = atomicAdd(work + p * RC_PART_STRIDE, RC_CHUNK);
-          base = __shfl(base, leader, 64);
+          base = __builtin_amdgcn_readlane(base, leader);  // wave-uniform: chunk and slot state stay scalar
           if (base < hi - lo) {
             next = lo + base;
             end = hi - next > RC_CHUNK ? next + RC_CHUNK : hi;
+            // the chunk goes into the other slot; what is still there is written out as it
+            // stands, and that chunk's remaining rays write their results directly
+            const uint32_t ns = chunks % RC_SLOTS;
+            if ((st >> (16u * ns)) & 0xFFFFu) flush(ns, true);
+            const uint64_t len = end - next;
+            st = (st & ~(0xFFFFull << (16u * ns))) | ((len | len << 8) << (16u * ns));
+            uint32_t* sl = stg + (wv * RC_SLOTS + ns) * RC_STG_WORDS;
+            ((uint8_t*)(sl + 5 * RC_CHUNK))[lane] = 0;
+            if (lane == 0) sl[RC_STG_BASE] = next;
+            cb = next;
+            ++chunks;
             break;
           }
           ++part_k;
@@ -338,6 +439,8 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_wide(const QNode4* __restrict
       if (want) {
         if (rank < avail) {
           r = next + rank;
+          myq = chunks - 1u;
+          me = (r - cb) | (myq % RC_SLOTS) << 6;
           // Ray::inverse_transform_by (ray.rs:36-41)
           o = inv_xform_pt(m, f3{ox[r], oy[r], oz[r]});
           d = inv_xform_vec(m, f3{dx[r], dy[r], dz[r]});
@@ -377,22 +480,6 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_wide(const QNode4* __restrict
         }
       }
     };
-    auto finish = [&]() {
-      if (best_f != 0xFFFFFFFFu) {
-        f3 wn = xform_vec(m, best_n);  // res.normal = m * res.normal (ray_trimesh.rs:47)
-        toi_out[r] = best_t;
-        nrm_out[3 * r] = wn.x;
-        nrm_out[3 * r + 1] = wn.y;
-        nrm_out[3 * r + 2] = wn.z;
-      } else {
-        toi_out[r] = -1.f;
-        nrm_out[3 * r] = 0.f;
-        nrm_out[3 * r + 1] = 0.f;
-        nrm_out[3 * r + 2] = 0.f;
-      }
-      feat_out[r] = best_f;
-      has_ray = false;
-    };
     // Leaf phase: lanes that hold untested leaves (or whose traversal has ended) wait; once at
     // least RC_LEAF_PHASE / 64 of the lanes with a ray wait, the wavefront tests every held leaf
     // at once, so the triangle loads and tests run for many lanes together rather than in most
@@ -413,7 +500,6 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_wide(const QNode4* __restrict
         pl1 = NONE_LEAF;
         pl2 = NONE_LEAF;
         pl3 = NONE_LEAF;
-        if (fin) finish();
       }
       continue;
     }
@@ -483,8 +569,8 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_wide(const QNode4* __restrict
     // The stack's top entry stays in a register; pushing over it moves it to memory.
     auto push = [&](uint32_t x) {
       if (top_valid) {
-        if (sp < RC_LDS_STACK) lstack[sp * RC_BLOCK + tid] = top;
-        else if (sp < RC_LDS_STACK + (int)gdepth) gstack[sp - RC_LDS_STACK] = top;
+        if (sp < RC_CAST_STACK) lstack[sp * RC_BLOCK + tid] = top;
+        else if (sp < RC_CAST_STACK + (int)gdepth) gstack[sp - RC_CAST_STACK] = top;
         else { atomicOr(err, 1u); fin = true; }
         sp++;
       }
@@ -503,10 +589,19 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_wide(const QNode4* __restrict
       fin = true;
     } else {
       sp--;
-      node = sp < RC_LDS_STACK ? lstack[sp * RC_BLOCK + tid] : gstack[sp - RC_LDS_STACK];
+      node = sp < RC_CAST_STACK ? lstack[sp * RC_BLOCK + tid] : gstack[sp - RC_CAST_STACK];
     }
-    if (fin && pl0 == NONE_LEAF) finish();
   }
+}
+
+// Blocks of k_cast_wide resident on one CU (registers and LDS), for the persistent grid.
+static unsigned cast_blocks_per_cu() {
+  static const unsigned n = [] {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_cast_wide, RC_BLOCK, 0) != hipSuccess || b < 1) b = 1;
+    return (unsigned)b;
+  }();
+  return n;
 }
 
 // Quantisation on the host: qdec as the kernel decodes (fmaf is correctly rounded on both sides),
@@ -690,15 +785,18 @@ static int cast_impl(ncx_mesh* m, const ncx_pose* pose, const float* const r[6],
                        m->tris.as<float4>(), (uint32_t)m->ntri, (uint32_t)m->ntri, *pose, r[0], r[1], r[2], r[3], r[4],
                        r[5], n, max_toi, toi, nrm, feat, sn, st, m->overflow.as<uint32_t>(), m->err.as<uint32_t>());
   else {
-    // persistent grid: about as many blocks as are resident (LDS stacks allow 13 per CU)
-    unsigned pblocks = std::min<unsigned>(RC_GRID_BLOCKS, (unsigned)ctx->num_cus * 13);
+    // persistent grid: as many blocks as are resident.  The spill stacks take the stack
+    // entries the LDS does not hold, within the reserved buffer.
+    const uint32_t gdepth = RC_LDS_STACK + RC_GLOBAL_STACK - RC_CAST_STACK;
+    unsigned pblocks = std::min<unsigned>(RC_GRID_BLOCKS, (unsigned)ctx->num_cus * cast_blocks_per_cu());
+    pblocks = std::min<unsigned>(pblocks, (unsigned)(((size_t)RC_GRID_BLOCKS * RC_GLOBAL_STACK) / gdepth));
     NCX_REQUIRE(m->qframe.p && (m->ntri == 1 || m->qnodes4.p), "ncx_trimesh_cast_rays: mesh nodes missing");
     if ((rc = fill_u32(ctx, m->work.as<uint32_t>(), RC_PARTS * RC_PART_STRIDE, 0u))) return rc;
     if (m->timing) NCX_HIP_CHECK(hipEventRecord(m->ev0, ctx->stream));
     hipLaunchKernelGGL(k_cast_wide, dim3(pblocks), dim3(RC_BLOCK), 0, ctx->stream, m->qnodes4.as<QNode4>(),
                        m->qframe.as<float>(), m->tris.as<float4>(), (uint32_t)m->ntri, (uint32_t)m->ntri, *pose, r[0],
                        r[1], r[2], r[3], r[4], r[5], (uint32_t)n, max_toi, toi, nrm, feat, m->overflow.as<uint32_t>(),
-                       (uint32_t)RC_GLOBAL_STACK, m->err.as<uint32_t>(), m->work.as<uint32_t>());
+                       gdepth, m->err.as<uint32_t>(), m->work.as<uint32_t>());
     NCX_HIP_CHECK(hipGetLastError());
     if (m->timing) NCX_HIP_CHECK(hipEventRecord(m->ev1, ctx->stream));
   }
@@ -802,12 +900,12 @@ extern "C" int ncx_trimesh_cast_rays_noalloc(ncx_mesh* m, const ncx_pose* pose, 
   NCX_REQUIRE(sp, "ncx_trimesh_cast_rays_noalloc: scratch not set up on this device (ncx_init)");
   NCX_REQUIRE(m->qframe.p && (m->ntri == 1 || m->qnodes4.p), "ncx_trimesh_cast_rays_noalloc: mesh nodes missing");
   // A traversal keeps at most 3 pending subtrees per level (the children it does not descend
-  // into): one in a register, RC_LDS_STACK in LDS, the rest in its spill stack.
+  // into): one in a register, RC_CAST_STACK in LDS, the rest in its spill stack.
   const uint32_t need = 3 * m->levels4 + 1;
-  const uint32_t gdepth = need > RC_LDS_STACK ? need - RC_LDS_STACK : 1u;
+  const uint32_t gdepth = need > RC_CAST_STACK ? need - RC_CAST_STACK : 1u;
   const size_t counters = (size_t)RC_PARTS * RC_PART_STRIDE;
   const size_t per_block = (size_t)RC_BLOCK * gdepth;
-  unsigned pblocks = std::min<unsigned>(RC_GRID_BLOCKS, (unsigned)ctx->num_cus * 13);
+  unsigned pblocks = std::min<unsigned>(RC_GRID_BLOCKS, (unsigned)ctx->num_cus * cast_blocks_per_cu());
   pblocks = std::min<unsigned>(pblocks, ceil_div(n, RC_BLOCK));
   const size_t fit = (RC_SCRATCH_WORDS - counters) / per_block;
   if (fit < pblocks) pblocks = (unsigned)fit;
