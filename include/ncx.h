@@ -370,6 +370,10 @@ int ncx_world_set_contacts(ncx_world* world, int enable);
 /* No reference counterpart (measurement): internal nodes visited by one pair-count traversal of
  * the current tree, the per-pass work of the step's pair search.  Synchronous. */
 int ncx_world_query_stats(ncx_world* world, unsigned long long* nodes_visited);
+/* No reference counterpart (measurement): of the last step's pair search, out[0] = leaves with
+ * more than 512 later partners, out[1] = of those, the leaves whose partners past the 512th went
+ * to the spill chunks (the others were searched a second time).  Waits for the world's stream. */
+int ncx_world_spill_stats(ncx_world* world, uint32_t out[2]);
 /* Waits for the last step: *npairs = pair count; NCX_E_CAPACITY if it exceeded max_pairs.
  * Every contact is complete on return: pairs whose EPA polytope outgrew the step's LDS arena
  * tiers are continued here with device-memory arenas, so the step's poses must stay unchanged

@@ -85,6 +85,7 @@ SIGNATURES = {
     "ncx_world_step": (C.c_int, [P, P, C.c_int]),
     "ncx_world_set_contacts": (C.c_int, [P, C.c_int]),
     "ncx_world_query_stats": (C.c_int, [P, P]),
+    "ncx_world_spill_stats": (C.c_int, [P, P]),
     "ncx_world_sync": (C.c_int, [P, P, P]),
     "ncx_world_outputs": (C.c_int, [P, P, P, P, P, P, P, P, P]),
     "ncx_world_stored_aabbs": (C.c_int, [P, P]),

@@ -1025,10 +1025,18 @@ __device__ __forceinline__ float grp16_max(float x) {
   return x;
 }
 
+// STAGE: the boxes go through the broad phase's stored-box rule instead of to b0..b5 (the scene
+// step's staging, fused): `st` holds the stored boxes (six SoA arrays of n), each is replaced by
+// the new box loosened by `stage_margin` unless it contains the new box
+// (dbvt_broad_phase.rs:325-347), or by the new box itself on the first step; the stored box is
+// mirrored as two float4 in `sb`.
+template <bool STAGE>
 __global__ __launch_bounds__(256) void k_shape_aabbs(const ncx_shape* __restrict__ shapes, Pool P,
                                                       const ncx_pose* __restrict__ poses, size_t n, float margin,
                                                       float* b0, float* b1, float* b2, float* b3, float* b4,
-                                                      float* b5) {
+                                                      float* b5, float* __restrict__ st = nullptr,
+                                                      float4* __restrict__ sb = nullptr, float stage_margin = 0.f,
+                                                      int first = 0) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = i < n;  // no early exit: every lane takes part in the hull loop
   const int lane = __lane_id();
@@ -1126,13 +1134,45 @@ __global__ __launch_bounds__(256) void k_shape_aabbs(const ncx_shape* __restrict
     mn = mn + f3{-margin, -margin, -margin};
     mx = mx + f3{margin, margin, margin};
   }
-  b0[i] = mn.x; b1[i] = mn.y; b2[i] = mn.z;
-  b3[i] = mx.x; b4[i] = mx.y; b5[i] = mx.z;
+  if constexpr (STAGE) {
+    float b[6] = {mn.x, mn.y, mn.z, mx.x, mx.y, mx.z};
+    if (!first) {
+      // AABB::contains (aabb.rs:161-163)
+      const bool contained = st[0 * n + i] <= b[0] && st[1 * n + i] <= b[1] && st[2 * n + i] <= b[2] &&
+                             st[3 * n + i] >= b[3] && st[4 * n + i] >= b[4] && st[5 * n + i] >= b[5];
+      if (contained) return;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        b[d] = b[d] + (-stage_margin);
+        b[3 + d] = b[3 + d] + stage_margin;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 6; ++d) st[d * n + i] = b[d];
+    sb[2 * i] = make_float4(b[0], b[1], b[2], b[3]);
+    sb[2 * i + 1] = make_float4(b[4], b[5], 0.f, 0.f);
+  } else {
+    b0[i] = mn.x; b1[i] = mn.y; b2[i] = mn.z;
+    b3[i] = mx.x; b4[i] = mx.y; b5[i] = mx.z;
+  }
+}
+
+// The scene step's shape boxes (loosened by `margin`) through the stored-box rule, into the
+// stored boxes `st` and their float4 mirror `sb` (see k_shape_aabbs<true>).
+int shapes_aabbs_stage_on(ncx_shapes* s, ncx_ctx* ctx, const ncx_pose* poses, float margin, float* st, float4* sb,
+                          float stage_margin, bool first) {
+  if (s->n == 0) return NCX_OK;
+  hipLaunchKernelGGL(k_shape_aabbs<true>, dim3(ceil_div(s->n, 256)), dim3(256), 0, ctx->stream,
+                     s->shapes.as<ncx_shape>(), Pool{s->hx.as<float>(), s->hy.as<float>(), s->hz.as<float>(), s->tri}, poses,
+                     s->n, margin, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, st, sb, stage_margin,
+                     first ? 1 : 0);
+  NCX_HIP_CHECK(hipGetLastError());
+  return NCX_OK;
 }
 
 int shapes_aabbs_on(ncx_shapes* s, ncx_ctx* ctx, const ncx_pose* poses, float margin, float* const out[6]) {
   if (s->n == 0) return NCX_OK;
-  hipLaunchKernelGGL(k_shape_aabbs, dim3(ceil_div(s->n, 256)), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(k_shape_aabbs<false>, dim3(ceil_div(s->n, 256)), dim3(256), 0, ctx->stream,
                      s->shapes.as<ncx_shape>(), Pool{s->hx.as<float>(), s->hy.as<float>(), s->hz.as<float>(), s->tri}, poses,
                      s->n, margin, out[0], out[1], out[2], out[3], out[4], out[5]);
   NCX_HIP_CHECK(hipGetLastError());
@@ -1144,7 +1184,7 @@ extern "C" int ncx_shapes_aabbs_dev(ncx_shapes* s, const ncx_pose* poses, float 
   NCX_REQUIRE(margin >= 0.f, "ncx_shapes_aabbs_dev: the loosening margin must be positive");
   for (int d = 0; d < 6; ++d) NCX_REQUIRE(out[d], "ncx_shapes_aabbs_dev: null output");
   if (s->n == 0) return NCX_OK;
-  hipLaunchKernelGGL(k_shape_aabbs, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->ctx->stream,
+  hipLaunchKernelGGL(k_shape_aabbs<false>, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->ctx->stream,
                      s->shapes.as<ncx_shape>(), Pool{s->hx.as<float>(), s->hy.as<float>(), s->hz.as<float>(), s->tri}, poses,
                      s->n, margin, out[0], out[1], out[2], out[3], out[4], out[5]);
   NCX_HIP_CHECK(hipGetLastError());

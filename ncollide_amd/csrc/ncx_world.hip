@@ -22,6 +22,8 @@ int mlp_predict_device_count(ncx_mlp* m, ncx_ctx* ctx, const ncx_shape* shapes, 
                              const float* desc, const uint32_t* ia, const uint32_t* ib, const uint32_t* n_dev,
                              size_t cap, float* dirs);
 int shapes_aabbs_on(ncx_shapes* s, ncx_ctx* ctx, const ncx_pose* poses, float margin, float* const out[6]);
+int shapes_aabbs_stage_on(ncx_shapes* s, ncx_ctx* ctx, const ncx_pose* poses, float margin, float* st, float4* sb,
+                          float stage_margin, bool first);
 int shape_descriptors_on(ncx_shapes* s, ncx_ctx* ctx, float* desc);
 
 struct ncx_world {
@@ -33,14 +35,22 @@ struct ncx_world {
   int rebuild_every = 2;
   int since_build = 0;
   bool first = true;
-  DevBuf box_new, stored;
+  DevBuf stored;
   DevBuf sbox;  // the stored boxes again, two float4 per proxy (min xyz, max x | max yz), for gathers
   LBVH bvh;
   uint32_t spill = 1;
   DevBuf gstack;  // search-stack spill area, `spill` entries per query thread
   DevBuf qwork;   // the query passes' work counters (WQ_PARTS per pass)
   DevBuf lbuf;    // the count pass's first WQ_LOCAL partners per leaf (sorted leaf positions)
-  DevBuf cnt, offs, counters;  // counters: [0] pair total, [1] pair overflow
+  // the count pass's partners past the first WQ_LOCAL: per leaf a directory (ovdir) of chunks of
+  // WQ_OCH entries taken from a pool; a leaf that finds no directory, or outgrows one, is
+  // listed WQ_NODIR and traversed again by the write pass
+  DevBuf ovdir, odirs, ochunks;
+  DevBuf relist;  // the leaves marked WQ_NODIR, for the write pass
+  uint32_t ndir = 0, nch = 0;
+  // counters: [0] pair total, [1] pair overflow, [2] directories, [3] chunks taken, [4] leaves
+  // listed for the write pass
+  DevBuf cnt, offs, counters;
   DevBuf pa, pb;
   DevBuf has, w1, w2, nrm, depth_out, iters;
   ContactTiers tiers;  // this scene's contact overflow lists
@@ -136,32 +146,9 @@ __global__ void k_tw_lookup(const uint32_t* __restrict__ pa, const uint32_t* __r
   }
 }
 
-// The stored boxes are kept twice: SoA (the interface's layout, read coalesced here) and as two
-// float4 per proxy, which the refit and the pair query gather by sorted leaf: one cache line
-// per box instead of six.
-__global__ void k_world_stage(const float* __restrict__ nb, float* __restrict__ st, float4* __restrict__ sb, size_t n,
-                              float margin, int first) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float b[6];
-#pragma unroll
-  for (int d = 0; d < 6; ++d) b[d] = nb[d * n + i];
-  if (!first) {
-    // AABB::contains (aabb.rs:161-163)
-    const bool contained = st[0 * n + i] <= b[0] && st[1 * n + i] <= b[1] && st[2 * n + i] <= b[2] &&
-                           st[3 * n + i] >= b[3] && st[4 * n + i] >= b[4] && st[5 * n + i] >= b[5];
-    if (contained) return;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      b[d] = b[d] + (-margin);
-      b[3 + d] = b[3 + d] + margin;
-    }
-  }
-#pragma unroll
-  for (int d = 0; d < 6; ++d) st[d * n + i] = b[d];
-  sb[2 * i] = make_float4(b[0], b[1], b[2], b[3]);
-  sb[2 * i + 1] = make_float4(b[4], b[5], 0.f, 0.f);
-}
+// The stored boxes are kept twice: SoA (the interface's layout) and as two float4 per proxy,
+// which the refit and the pair query gather by sorted leaf: one cache line per box instead of
+// six.  The shape-box kernel writes both (k_shape_aabbs<true>).
 
 __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float mny, float mnz, float mxx, float mxy,
                                              float mxz) {
@@ -171,10 +158,13 @@ __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float 
 // Self-query of sorted leaf k against leaves at later positions (each pair is found once, by its
 // earlier leaf).  Depth-first with the stack's top in a register, the next WQ_LSTACK entries in
 // LDS and the rest in a per-thread spill area of `spill` entries (the tree depth bounds it).
-//   MODE_COUNT: count the pairs and keep the first WQ_LOCAL partners (leaf positions: no
-//               dependent gather inside the traversal) in lbuf;
-//   MODE_WRITE: for leaves with more than WQ_LOCAL pairs, the partners past the first WQ_LOCAL
-//               (leaf positions) at their slots offs[k] + index.
+//   MODE_COUNT: count the pairs and keep every partner (leaf positions: no dependent gather
+//               inside the traversal): the first WQ_LOCAL in lbuf, the rest in chunks of WQ_OCH
+//               entries, listed in the leaf's directory of WQ_DIRN chunks.  A leaf that finds the
+//               directories used up, or has more than WQ_LOCAL + WQ_DIRN * WQ_OCH partners, is
+//               marked WQ_NODIR and listed in relist (the list's length in counters[4]);
+//   MODE_WRITE: for the leaves listed in relist, the partners past the first WQ_LOCAL (leaf
+//               positions) at their slots offs[k] + index.
 // Persistent: a lane that finishes a leaf takes the next one at once from its wavefront's chunk of
 // WQ_CHUNK leaves (ballot + rank), and a wavefront whose chunk is used up takes the next chunk of
 // its part of the leaves with one atomicAdd on that part's counter (WQ_PARTS counters, each on
@@ -184,6 +174,9 @@ __device__ __forceinline__ bool wbox_overlap(const float q[6], float mnx, float 
 // eighth of the leaves and the tree nodes above them.
 #define WQ_LSTACK 16
 #define WQ_LOCAL 512
+#define WQ_OCH 256u
+#define WQ_DIRN 64u
+#define WQ_NODIR 0xFFFFFFFFu
 #define WQ_CHUNK 64u
 #define WQ_PARTS 8u
 #define WQ_PART_STRIDE 64u
@@ -201,7 +194,10 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
                                                            uint32_t* __restrict__ gstack, uint32_t spill,
                                                            uint32_t* __restrict__ lbuf,
                                                            unsigned long long* __restrict__ visits,
-                                                           uint32_t* __restrict__ work) {
+                                                           uint32_t* __restrict__ work, uint32_t* __restrict__ ovdir,
+                                                           uint32_t* __restrict__ odirs, uint32_t ndir,
+                                                           uint32_t* __restrict__ ochunks, uint32_t nch,
+                                                           uint32_t* __restrict__ relist) {
   __shared__ uint32_t lstack[WQ_LSTACK * WQ_BLOCK];
   const int tid = threadIdx.x;
   const int lane = __lane_id();
@@ -212,12 +208,17 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
   uint32_t next = 0, end = 0;  // the wavefront's chunk [next, end) (the same in every lane)
   bool pool_empty = false, exhausted = false, has = false;
   uint32_t k = 0, prim = 0, c = 0;
+  if (MODE == MODE_WRITE) {  // the write pass walks the listed leaves (usually none)
+    n = counters[4];
+    if (n == 0) return;
+  }
   size_t w = 0;
   float q[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int sp = 0;
   uint32_t node = 0, top = 0;
   bool top_valid = false;
   unsigned long long nvis = 0;
+  uint32_t dir = WQ_NODIR, chunk = 0;  // count pass: the leaf's directory and current chunk
   auto emit = [&](uint32_t leafpos) {
     if (MODE == MODE_WRITE) {
       // a pair past the buffered ones: its leaf position goes to pb[w], which the copy kernel,
@@ -229,7 +230,21 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
       w++;
       c++;
     } else {
-      if (c < WQ_LOCAL) lbuf[(size_t)k * WQ_LOCAL + c] = leafpos;  // the emit kernel maps it to a shape
+      if (c < WQ_LOCAL) {
+        lbuf[(size_t)k * WQ_LOCAL + c] = leafpos;  // the emit kernel maps it to a shape
+      } else {
+        const uint32_t j = c - WQ_LOCAL;
+        if (j == 0) {  // the first partner past the buffer: a directory
+          dir = atomicAdd(&counters[2], 1u);
+          if (dir >= ndir) dir = WQ_NODIR;
+        }
+        if (dir != WQ_NODIR && j % WQ_OCH == 0) {  // a new chunk
+          if (j / WQ_OCH < WQ_DIRN) chunk = atomicAdd(&counters[3], 1u);
+          if (j / WQ_OCH >= WQ_DIRN || chunk >= nch) dir = WQ_NODIR;
+          else odirs[(size_t)dir * WQ_DIRN + j / WQ_OCH] = chunk;
+        }
+        if (dir != WQ_NODIR) ochunks[(size_t)chunk * WQ_OCH + j % WQ_OCH] = leafpos;
+      }
       c++;
     }
   };
@@ -261,18 +276,17 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
       if (want) {
         if (rank < avail) {
-          k = next + rank;
-          if (MODE == MODE_COUNT || cnt[k] > WQ_LOCAL) {
-            prim = order[k];
-            const float4 b0 = sbox[2 * (size_t)prim], b1 = sbox[2 * (size_t)prim + 1];
-            q[0] = b0.x; q[1] = b0.y; q[2] = b0.z; q[3] = b0.w; q[4] = b1.x; q[5] = b1.y;
-            c = 0;
-            w = MODE == MODE_WRITE ? offs[k] : 0;
-            sp = 0;
-            node = 0;
-            top_valid = false;
-            has = true;
-          }
+          k = MODE == MODE_WRITE ? relist[next + rank] : next + rank;
+          prim = order[k];
+          const float4 b0 = sbox[2 * (size_t)prim], b1 = sbox[2 * (size_t)prim + 1];
+          q[0] = b0.x; q[1] = b0.y; q[2] = b0.z; q[3] = b0.w; q[4] = b1.x; q[5] = b1.y;
+          c = 0;
+          dir = WQ_NODIR;
+          w = MODE == MODE_WRITE ? offs[k] : 0;
+          sp = 0;
+          node = 0;
+          top_valid = false;
+          has = true;
         } else if (pool_empty) {
           exhausted = true;
         }
@@ -287,9 +301,9 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
     if (!has) continue;
     nvis++;
     const float4* nq = (const float4*)(nodes + node);
-    float4 q0 = nq[0], q1 = nq[1], q2 = nq[2], q3 = nq[3];
-    uint32_t cl = __float_as_uint(q3.x), cr = __float_as_uint(q3.y);
-    uint32_t ll = __float_as_uint(q3.z), lr = __float_as_uint(q3.w);
+    const float4 q0 = nq[0], q1 = nq[1], q2 = nq[2], q3 = nq[3];
+    const uint32_t cl = __float_as_uint(q3.x), cr = __float_as_uint(q3.y);
+    const uint32_t ll = __float_as_uint(q3.z), lr = __float_as_uint(q3.w);
     bool hl = ll > k && wbox_overlap(q, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
     bool hr = lr > k && wbox_overlap(q, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
     if (hl && (cl & NCX_LEAF_BIT)) { emit(cl & ~NCX_LEAF_BIT); hl = false; }
@@ -318,7 +332,13 @@ __global__ __launch_bounds__(WQ_BLOCK) void k_world_query(const LNode* __restric
       node = sp < WQ_LSTACK ? lstack[sp * WQ_BLOCK + tid] : gs[sp - WQ_LSTACK];
     }
     if (done) {
-      if (MODE == MODE_COUNT) cnt[k] = c;
+      if (MODE == MODE_COUNT) {
+        cnt[k] = c;
+        if (c > WQ_LOCAL) {
+          ovdir[k] = dir;
+          if (dir == WQ_NODIR) relist[atomicAdd(&counters[4], 1u)] = k;
+        }
+      }
       has = false;
     }
   }
@@ -334,8 +354,8 @@ static unsigned world_query_blocks(const ncx_ctx* c, size_t n) {
 
 // Every pair, in the order the traversal found it, so the list is the same as one full write
 // traversal per leaf would give: a leaf's first WQ_LOCAL partners from the count pass's buffer,
-// the rest from the leaf positions the write pass left in pb; each is mapped to its shape and the
-// pair ordered (a < b).  A block takes EMIT_LEAVES consecutive leaves; its
+// the rest from its chunks (or, for a leaf marked WQ_NODIR, from the leaf positions the write pass
+// left in pb); each is mapped to its shape and the pair ordered (a < b).  A block takes EMIT_LEAVES consecutive leaves; its
 // threads walk the concatenation of those leaves' pairs (a block-wide prefix sum over the
 // counts, then a binary search for each pair's leaf), so consecutive threads write consecutive
 // pair slots.
@@ -349,12 +369,15 @@ __global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __
                                                               const uint32_t* __restrict__ offs,
                                                               const uint32_t* __restrict__ lbuf, size_t cap,
                                                               uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
-                                                              uint32_t* __restrict__ counters) {
+                                                              uint32_t* __restrict__ counters,
+                                                              const uint32_t* __restrict__ ovdir,
+                                                              const uint32_t* __restrict__ odirs,
+                                                              const uint32_t* __restrict__ ochunks) {
   __shared__ uint32_t s_pre[EMIT_LEAVES + 1];
   const uint32_t t = threadIdx.x;
   const uint32_t k0 = blockIdx.x * EMIT_LEAVES;
   // every pair of the block's leaves: the first WQ_LOCAL of a leaf from the count pass's buffer,
-  // the rest from the leaf positions the write pass left in pb
+  // the rest from its chunks or the leaf positions the write pass left in pb
   if (t < EMIT_LEAVES) {  // the first wavefront, all lanes active: inclusive prefix sum of the counts
     const uint32_t k = k0 + t;
     uint32_t x = k < n ? cnt[k] : 0u;
@@ -391,7 +414,14 @@ __global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __
       w0[u] = offs[kl[u]];
       prim[u] = order[kl[u]];
       const size_t w = (size_t)w0[u] + ii[u];
-      const uint32_t lp = ii[u] < WQ_LOCAL ? lbuf[(size_t)kl[u] * WQ_LOCAL + ii[u]] : (w < cap ? pb[w] : 0u);
+      uint32_t lp;
+      if (ii[u] < WQ_LOCAL) {
+        lp = lbuf[(size_t)kl[u] * WQ_LOCAL + ii[u]];
+      } else {
+        const uint32_t dd = ovdir[kl[u]], j = ii[u] - WQ_LOCAL;
+        if (dd != WQ_NODIR) lp = ochunks[(size_t)odirs[(size_t)dd * WQ_DIRN + j / WQ_OCH] * WQ_OCH + j % WQ_OCH];
+        else lp = w < cap ? pb[w] : 0u;
+      }
       h2[u] = order[lp];
     }
 #pragma unroll
@@ -429,12 +459,10 @@ static int world_enqueue(ncx_world* w, const ncx_pose* poses, bool refit) {
   ncx_ctx* c = &w->wctx;
   size_t n = w->n;
   int rc;
-  float* nb[6];
-  for (int d = 0; d < 6; ++d) nb[d] = w->box_new.as<float>() + d * n;
-  if ((rc = shapes_aabbs_on(w->shapes, c, poses, w->prediction, nb))) return rc;
-  hipLaunchKernelGGL(k_world_stage, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, w->box_new.as<float>(),
-                     w->stored.as<float>(), w->sbox.as<float4>(), n, w->margin, w->first ? 1 : 0);
-  NCX_HIP_CHECK(hipGetLastError());
+  // shape boxes and the stored-box rule in one kernel
+  if ((rc = shapes_aabbs_stage_on(w->shapes, c, poses, w->prediction, w->stored.as<float>(), w->sbox.as<float4>(),
+                                  w->margin, w->first)))
+    return rc;
   if (refit) {
     const float* box[6];
     for (int d = 0; d < 6; ++d) box[d] = w->stored.as<float>() + d * n;
@@ -457,7 +485,7 @@ static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses, bool ea
                        w->tw_keys.as<unsigned long long>(), w->tw_dirs.as<float4>(), w->tw_mask);
     NCX_HIP_CHECK(hipGetLastError());
   }
-  if ((rc = fill_u32(c, ctr, 2, 0u))) return rc;
+  if ((rc = fill_u32(c, ctr, 5, 0u))) return rc;
   if (n >= 2) {
     unsigned nb = world_query_blocks(c, n);
     const LNode* nodes = w->bvh.nodes.as<LNode>();
@@ -468,19 +496,23 @@ static int world_pairs_and_contacts(ncx_world* w, const ncx_pose* poses, bool ea
     hipLaunchKernelGGL((k_world_query<MODE_COUNT>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
                        (uint32_t)n, w->cnt.as<uint32_t>(), (const uint32_t*)nullptr, w->cap, (uint32_t*)nullptr,
                        (uint32_t*)nullptr, ctr, w->gstack.as<uint32_t>(), w->spill, w->lbuf.as<uint32_t>(),
-                       (unsigned long long*)nullptr, qw);
+                       (unsigned long long*)nullptr, qw, w->ovdir.as<uint32_t>(), w->odirs.as<uint32_t>(), w->ndir,
+                       w->ochunks.as<uint32_t>(), w->nch, w->relist.as<uint32_t>());
     NCX_HIP_CHECK(hipGetLastError());
     if ((rc = scan_exclusive_u32(c, w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), n, w->bvh.scan_tmp, ctr)))
       return rc;
-    // the write pass first: it leaves leaf positions in pb for the copy kernel to map
+    // the write pass (leaves marked WQ_NODIR only) first: it leaves leaf positions in pb for the
+    // copy kernel to map
     hipLaunchKernelGGL((k_world_query<MODE_WRITE>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, nodes, order, st,
                        (uint32_t)n, w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), w->cap, w->pa.as<uint32_t>(),
                        w->pb.as<uint32_t>(), ctr, w->gstack.as<uint32_t>(), w->spill, w->lbuf.as<uint32_t>(),
-                       (unsigned long long*)nullptr, qw + WQ_PARTS * WQ_PART_STRIDE);
+                       (unsigned long long*)nullptr, qw + WQ_PARTS * WQ_PART_STRIDE, w->ovdir.as<uint32_t>(),
+                       w->odirs.as<uint32_t>(), w->ndir, w->ochunks.as<uint32_t>(), w->nch, w->relist.as<uint32_t>());
     NCX_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_world_emit, dim3(ceil_div(n, EMIT_LEAVES)), dim3(WQ_EMIT_BLOCK), 0, c->stream, order, (uint32_t)n,
                        w->cnt.as<uint32_t>(), w->offs.as<uint32_t>(), w->lbuf.as<uint32_t>(), w->cap,
-                       w->pa.as<uint32_t>(), w->pb.as<uint32_t>(), ctr);
+                       w->pa.as<uint32_t>(), w->pb.as<uint32_t>(), ctr, w->ovdir.as<uint32_t>(),
+                       w->odirs.as<uint32_t>(), w->ochunks.as<uint32_t>());
     NCX_HIP_CHECK(hipGetLastError());
   }
   if (!w->contacts) return NCX_OK;
@@ -528,6 +560,10 @@ extern "C" int ncx_world_create(ncx_ctx* ctx, ncx_shapes* shapes, float margin, 
   w->margin = margin;
   w->prediction = prediction;
   size_t n = w->n, cap = max_pairs;
+  // Directories for one leaf in 64 (at least 64, at most 65,536); the chunk pool holds every pair
+  // the pair list can hold plus one partly filled chunk per directory.
+  w->ndir = (uint32_t)std::min<size_t>(std::max<size_t>(n / 64, 64), 65536);
+  w->nch = (uint32_t)std::min<size_t>(cap / WQ_OCH + w->ndir, 0xFFFFFFFFull / WQ_OCH);
   int rc;
   if (hipStreamCreateWithFlags(&w->wctx.stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&w->ev_in, hipEventDisableTiming) != hipSuccess ||
@@ -536,11 +572,14 @@ extern "C" int ncx_world_create(ncx_ctx* ctx, ncx_shapes* shapes, float margin, 
     ncx_set_error("ncx_world_create: stream/event creation failed");
     return NCX_E_HIP;
   }
-  if ((rc = w->box_new.reserve(n * 24 + 64)) || (rc = w->stored.reserve(n * 24 + 64)) ||
+  if ((rc = w->stored.reserve(n * 24 + 64)) ||
       (rc = w->sbox.reserve(n * 32 + 64)) ||
       (rc = w->cnt.reserve(n * 4 + 64)) || (rc = w->offs.reserve(n * 4 + 64)) || (rc = w->counters.reserve(64)) ||
-      (rc = w->lbuf.reserve(n * (size_t)WQ_LOCAL * 4 + 64)) ||
+      (rc = w->lbuf.reserve(n * (size_t)WQ_LOCAL * 4 + 64)) || (rc = w->ovdir.reserve(n * 4 + 64)) ||
+      (rc = w->relist.reserve(n * 4 + 64)) ||
       (rc = w->qwork.reserve(2 * WQ_PARTS * WQ_PART_STRIDE * 4)) ||
+      (rc = w->odirs.reserve((size_t)w->ndir * WQ_DIRN * 4 + 64)) ||
+      (rc = w->ochunks.reserve((size_t)w->nch * WQ_OCH * 4 + 64)) ||
       (rc = w->pa.reserve(cap * 4)) || (rc = w->pb.reserve(cap * 4)) || (rc = w->has.reserve(cap)) ||
       (rc = w->w1.reserve(cap * 12)) || (rc = w->w2.reserve(cap * 12)) || (rc = w->nrm.reserve(cap * 12)) ||
       (rc = w->depth_out.reserve(cap * 4)) || (rc = w->iters.reserve(cap * 4)) || (rc = contact_tiers_reserve(w->tiers, std::min<size_t>(cap, (size_t)1 << 22)))) {
@@ -596,17 +635,35 @@ extern "C" int ncx_world_query_stats(ncx_world* w, unsigned long long* nodes_vis
   int rc;
   if ((rc = v.reserve(8))) return rc;
   if ((rc = fill_u32(c, v.as<uint32_t>(), 2, 0u))) return rc;
+  if ((rc = fill_u32(c, w->counters.as<uint32_t>() + 2, 3, 0u))) return rc;
   unsigned nb = world_query_blocks(c, w->n);
   if ((rc = fill_u32(c, w->qwork.as<uint32_t>(), WQ_PARTS * WQ_PART_STRIDE, 0u))) return rc;
   hipLaunchKernelGGL((k_world_query<MODE_COUNT>), dim3(nb), dim3(WQ_BLOCK), 0, c->stream, w->bvh.nodes.as<LNode>(),
                      w->bvh.order.as<uint32_t>(), w->sbox.as<float4>(), (uint32_t)w->n, w->cnt.as<uint32_t>(),
                      (const uint32_t*)nullptr, w->cap, (uint32_t*)nullptr, (uint32_t*)nullptr,
                      w->counters.as<uint32_t>(), w->gstack.as<uint32_t>(), w->spill, w->lbuf.as<uint32_t>(),
-                     v.as<unsigned long long>(), w->qwork.as<uint32_t>());
+                     v.as<unsigned long long>(), w->qwork.as<uint32_t>(), w->ovdir.as<uint32_t>(),
+                     w->odirs.as<uint32_t>(), w->ndir, w->ochunks.as<uint32_t>(), w->nch, w->relist.as<uint32_t>());
   NCX_HIP_CHECK(hipGetLastError());
   NCX_HIP_CHECK(hipMemcpyAsync(nodes_visited, v.p, 8, hipMemcpyDeviceToHost, c->stream));
   NCX_HIP_CHECK(hipStreamSynchronize(c->stream));
   v.release();
+  return NCX_OK;
+}
+
+extern "C" int ncx_world_spill_stats(ncx_world* w, uint32_t out[2]) {
+  NCX_REQUIRE(w && out, "ncx_world_spill_stats: null argument");
+  out[0] = out[1] = 0;
+  if (w->first || w->n < 2) return NCX_OK;
+  ncx_ctx* c = &w->wctx;
+  std::vector<uint32_t> cnt(w->n), dir(w->n), ctr(4);
+  NCX_HIP_CHECK(hipMemcpyAsync(cnt.data(), w->cnt.p, w->n * 4, hipMemcpyDeviceToHost, c->stream));
+  NCX_HIP_CHECK(hipMemcpyAsync(dir.data(), w->ovdir.p, w->n * 4, hipMemcpyDeviceToHost, c->stream));
+  NCX_HIP_CHECK(hipMemcpyAsync(ctr.data(), w->counters.p, 16, hipMemcpyDeviceToHost, c->stream));
+  NCX_HIP_CHECK(hipStreamSynchronize(c->stream));
+  out[0] = ctr[2];
+  for (size_t k = 0; k < w->n; ++k)
+    if (cnt[k] > WQ_LOCAL && dir[k] != WQ_NODIR) out[1]++;
   return NCX_OK;
 }
 
@@ -622,7 +679,8 @@ extern "C" int ncx_world_destroy(ncx_world* w) {
   }
   if (w->ev_in) (void)hipEventDestroy(w->ev_in);
   if (w->ev_out) (void)hipEventDestroy(w->ev_out);
-  DevBuf* bufs[] = {&w->box_new, &w->stored, &w->sbox, &w->gstack, &w->qwork, &w->lbuf, &w->cnt, &w->offs, &w->counters, &w->pa, &w->pb,
+  DevBuf* bufs[] = {&w->stored, &w->sbox, &w->gstack, &w->qwork, &w->lbuf, &w->ovdir, &w->odirs,
+                    &w->ochunks, &w->relist, &w->cnt, &w->offs, &w->counters, &w->pa, &w->pb,
                     &w->has, &w->w1, &w->w2, &w->nrm, &w->depth_out, &w->iters, &w->desc, &w->dirs,
                     &w->qcnt, &w->qscan, &w->qtot, &w->tw_keys, &w->tw_dirs, &w->tw_init, &w->tw_fdir};
   for (DevBuf* b : bufs) b->release();

@@ -36,6 +36,13 @@ class World:
         _lib.check(self.ctx.lib.ncx_world_query_stats(self.h, C.byref(v)))
         return v.value
 
+    def spill_stats(self):
+        """Of the last step's pair search: (leaves with more than 512 later partners, of those the
+        leaves whose further partners went to the spill chunks rather than a second search)."""
+        out = (C.c_uint32 * 2)()
+        _lib.check(self.ctx.lib.ncx_world_spill_stats(self.h, out))
+        return out[0], out[1]
+
     def step(self, poses, use_graph=True):
         """poses: (n, 8) float32 CUDA tensor (keep the same tensor across steps to replay the graph)."""
         _lib.check(self.ctx.lib.ncx_world_step(self.h, _lib.ptr(poses), 1 if use_graph else 0))

@@ -198,8 +198,45 @@ def test_world_dense_cluster_pairs_match_brute_force(ctx, orc, use_graph):
             assert np.array_equal(got, ref_bp.pairs())
         deg = np.bincount(np.concatenate([(got >> np.uint64(32)).astype(np.int64),
                                           (got & np.uint64(0xFFFFFFFF)).astype(np.int64)]), minlength=3000)
+        over, spilled = world.spill_stats()
         print(f"step {step}: {len(got)} pairs, partners per shape max {deg.max()}, "
-              f"{np.sum(deg > 512)} shapes above 512, {np.sum((deg >= 500) & (deg <= 530))} in 500..530")
+              f"{np.sum(deg > 512)} shapes above 512, {np.sum((deg >= 500) & (deg <= 530))} in 500..530; "
+              f"leaves past 512 later partners {over}, {spilled} of them in spill chunks")
         assert deg.max() > 1500 and np.sum((deg >= 500) & (deg <= 530)) > 0
+        # both ways past the 512 buffered partners run: the spill chunks (64 directories at this
+        # size) and, for the leaves beyond them, the second search
+        assert 0 < spilled < over
         p_np[:, :3] += vel
         poses.copy_(torch.as_tensor(p_np, device=ctx.device))
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_world_leaf_past_the_spill_directory(ctx, orc, use_graph):
+    """One cuboid overlapping 20,000 small balls, centred at the scene's lowest corner so that its
+    leaf comes first in Morton order: its 20,000 later partners are more than the count pass
+    keeps (512 buffered + 64 chunks of 256), so that leaf is searched a second time.  The pair set
+    must equal brute force over the world's stored boxes."""
+    rng = np.random.default_rng(17)
+    n = 20001
+    rec = np.zeros(n, workloads.SHAPE_DTYPE)
+    rec["type"][0] = ncx.CUBOID
+    rec["p0"][0] = rec["p1"][0] = rec["p2"][0] = 12.0
+    rec["type"][1:] = ncx.BALL
+    rec["p0"][1:] = rng.uniform(0.05, 0.15, n - 1)
+    t = np.zeros((n, 3), np.float32)
+    t[1:] = rng.uniform(1.0, 11.0, (n - 1, 3))
+    q = workloads.uniform_quats(rng, n)
+    q[0] = (0.0, 0.0, 0.0, 1.0)
+    poses = torch.as_tensor(workloads.poses8(t, q), device=ctx.device)
+    ss = ncx.ShapeSet(records=rec, pool=np.zeros((1, 3), np.float32), ctx=ctx)
+    world = World(ss, margin=0.01, prediction=0.0, max_pairs=1 << 20, rebuild_every=2)
+    world.set_contacts(False)
+    for step in range(2):
+        world.step(poses, use_graph=use_graph)
+        got = np.sort(keys_of(world.results()))
+        stored = np.ascontiguousarray(world.stored_aabbs().cpu().numpy().T)
+        want = np.sort(orc.pairs_brute(stored))
+        assert np.array_equal(got, want), f"step {step}: {len(got)} pairs against {len(want)} by brute force"
+        assert np.sum((got >> np.uint64(32)) == 0) == n - 1  # the cuboid touches every ball
+        over, spilled = world.spill_stats()
+        assert over >= 1 and spilled < over
