@@ -22,8 +22,13 @@
 // and culling stays conservative (only more boxes pass).  Per coordinate the four children side
 // by side: lo[c] holds children 0, 1 of coordinate c in its low and high halves, hi[c] children
 // 2, 3; coordinates min x, y, z, max x, y, z.
-// child: internal 4-wide node index, (leaf position | NCX_LEAF_BIT), or RC_EMPTY.
+// child: internal 4-wide node index, a leaf (NCX_LEAF_BIT | first << 2 | count - 1: the
+// triangles at sorted positions [first, first + count)), or RC_EMPTY.
 #define RC_EMPTY 0xFFFFFFFFu
+// Triangles a cast-tree leaf may hold: a subtree of the binary tree with at most this many
+// triangles becomes one leaf (its triangles are consecutive in sorted order).
+#define RC_LEAF_TRIS 2u
+static_assert(RC_LEAF_TRIS >= 1 && RC_LEAF_TRIS <= 4, "two bits of a leaf reference hold its count");
 struct alignas(16) QNode4 {
   uint32_t lo[6];
   uint32_t hi[6];
@@ -465,18 +470,24 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_wide(const QNode4* __restrict
       const uint32_t cnt = (uint32_t)__popcll(need);
       next += cnt < avail ? cnt : avail;
     }
-    auto test_leaf = [&](uint32_t k) {
-      float4 A = tris[3 * k], B = tris[3 * k + 1], C = tris[3 * k + 2];
-      float t;
-      f3 nn;
-      bool back;
-      if (ray_tri(f3{A.x, A.y, A.z}, f3{B.x, B.y, B.z}, f3{C.x, C.y, C.z}, o, d, &t, &nn, &back) && t <= max_toi) {
-        uint32_t f = __float_as_uint(A.w);
-        uint32_t feat = back ? f + ntri_total : f;
-        if (t < best_t || (t == best_t && feat < best_f)) {
-          best_t = t;
-          best_f = feat;
-          best_n = nn;
+    auto test_leaf = [&](uint32_t ref) {  // the leaf's triangles [ref >> 2, (ref >> 2) + (ref & 3) + 1)
+      const uint32_t k0 = ref >> 2, nk = (ref & 3u) + 1u;
+#pragma unroll
+      for (uint32_t j = 0; j < RC_LEAF_TRIS; ++j) {
+        if (j >= nk) break;
+        const uint32_t k = k0 + j;
+        float4 A = tris[3 * k], B = tris[3 * k + 1], C = tris[3 * k + 2];
+        float t;
+        f3 nn;
+        bool back;
+        if (ray_tri(f3{A.x, A.y, A.z}, f3{B.x, B.y, B.z}, f3{C.x, C.y, C.z}, o, d, &t, &nn, &back) && t <= max_toi) {
+          uint32_t f = __float_as_uint(A.w);
+          uint32_t feat = back ? f + ntri_total : f;
+          if (t < best_t || (t == best_t && feat < best_f)) {
+            best_t = t;
+            best_f = feat;
+            best_n = nn;
+          }
         }
       }
     };
@@ -505,7 +516,7 @@ __global__ __launch_bounds__(RC_BLOCK) void k_cast_wide(const QNode4* __restrict
     }
     if (waiting) continue;
     if (nleaves == 1) {
-      pl0 = 0;
+      pl0 = 0;  // the one triangle: first 0, count 1
       fin = true;
       continue;
     }
@@ -627,9 +638,16 @@ static inline uint16_t qenc_h(float x, float o, float st, float inv, bool is_max
 // loaded).  One-time host pass at mesh creation over the exact binary boxes.
 static int build_wide(ncx_mesh* m) {
   const size_t ni = m->ntri - 1;
+  NCX_REQUIRE(m->ntri < (1ull << 29), "ncx_trimesh_create: more than 2^29 triangles for the cast tree's leaves");
   std::vector<LNode> L(ni);
+  std::vector<uint32_t> first(ni);
   float fr[6];
   NCX_HIP_CHECK(hipMemcpy(L.data(), m->bvh.nodes.p, ni * sizeof(LNode), hipMemcpyDeviceToHost));
+  NCX_HIP_CHECK(hipMemcpy(first.data(), m->bvh.first.p, ni * 4, hipMemcpyDeviceToHost));
+  // a binary child's triangle count (a leaf holds one)
+  auto count_of = [&](uint32_t ref) -> uint32_t {
+    return (ref & NCX_LEAF_BIT) ? 1u : L[ref].last_r - first[ref] + 1u;
+  };
   NCX_HIP_CHECK(hipMemcpy(fr, m->qframe.p, sizeof(fr), hipMemcpyDeviceToHost));
   float inv[3];
   for (int d = 0; d < 3; ++d) inv[d] = fr[3 + d] > 0.f ? 1.f / fr[3 + d] : 0.f;
@@ -672,7 +690,7 @@ static int build_wide(ncx_mesh* m) {
       int best = -1;
       float ba = -1.f;
       for (int i = 0; i < n; ++i)
-        if (!(c[i].ref & NCX_LEAF_BIT) && area(c[i].b) > ba) { ba = area(c[i].b); best = i; }
+        if (count_of(c[i].ref) > RC_LEAF_TRIS && area(c[i].b) > ba) { ba = area(c[i].b); best = i; }
       if (best < 0) break;
       const LNode& u = L[c[best].ref];
       c[best] = Ch{{u.lminx, u.lminy, u.lminz, u.lmaxx, u.lmaxy, u.lmaxz}, u.child_l};
@@ -691,11 +709,19 @@ static int build_wide(ncx_mesh* m) {
         uint32_t& w = i < 2 ? q.lo[k] : q.hi[k];
         w |= (uint32_t)v << ((i & 1) * 16);
       }
-      q.child[i] = c[i].ref;  // internal children are patched when they are emitted
+      // leaves: a binary leaf, or a subtree of at most RC_LEAF_TRIS triangles; internal children
+      // are patched when they are emitted
+      const uint32_t cnt = count_of(c[i].ref);
+      if (cnt <= RC_LEAF_TRIS) {
+        const uint32_t f0 = (c[i].ref & NCX_LEAF_BIT) ? (c[i].ref & ~NCX_LEAF_BIT) : first[c[i].ref];
+        q.child[i] = NCX_LEAF_BIT | f0 << 2 | (cnt - 1u);
+      } else {
+        q.child[i] = c[i].ref;
+      }
     }
     // push internal children in reverse so the first is emitted first
     for (int i = n - 1; i >= 0; --i)
-      if (!(c[i].ref & NCX_LEAF_BIT)) st.push_back({c[i].ref, idx, (uint32_t)i, it.depth + 1});
+      if (count_of(c[i].ref) > RC_LEAF_TRIS) st.push_back({c[i].ref, idx, (uint32_t)i, it.depth + 1});
   }
   NCX_REQUIRE(ok, "ncx_trimesh_create: a quantised 4-wide box does not contain its exact box");
   m->levels4 = max_depth + 1;
