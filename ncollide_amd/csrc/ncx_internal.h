@@ -267,7 +267,6 @@ struct ncx_mesh {
   size_t npoints = 0, ntri = 0;
   LBVH bvh;
   DevBuf tris;      // float4[3 * ntri] in leaf order: a (w = face id bits), b, c
-  DevBuf boxes;     // 6 SoA arrays of triangle boxes (build input)
   DevBuf pts, idx;  // uploaded mesh
   DevBuf overflow;  // spill area for traversal stacks deeper than the LDS part
   DevBuf err;       // error flag (stack exhausted)
