@@ -727,10 +727,11 @@ static int build_wide(ncx_mesh* m) {
 // every node's box sits in its parent's child slot, grown by expand * (1 + |coordinate|) at the
 // leaves.  Against the Morton-code LBVH it visits fewer nodes per ray on the heightfield workload
 // (the cast does not depend on the tree: every result is the least (toi, feature)).  To keep the
-// depth within the traversal stacks (at most 64 levels), a range whose depth plus log2(size)
-// reaches SAH_MEDIAN_DEPTH is split at its median instead.
+// depth within the traversal stacks (the cast tree's 3 entries per level, 128 in all; the
+// composite queries' 64 levels), a range whose depth plus log2(size) reaches SAH_MEDIAN_DEPTH is
+// split at its median instead, so no leaf is deeper than SAH_MEDIAN_DEPTH + 1.
 #define SAH_BINS 32
-#define SAH_MEDIAN_DEPTH 56
+#define SAH_MEDIAN_DEPTH 40
 static void sah_mesh_tree(const float* pts, const uint32_t* idx, size_t nt, float expand, std::vector<LNode>& nodes,
                           std::vector<uint32_t>& order, std::vector<uint32_t>& first) {
   std::vector<float> bl(nt * 3), bh(nt * 3), cen(nt * 3);

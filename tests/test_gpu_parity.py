@@ -126,6 +126,38 @@ def test_raycast_edge_cases(ctx, orc):
     assert f[0] == 0 and f[1] == 1 and f[2] == 0xFFFFFFFF and f[3] == 0xFFFFFFFF and f[5] == 0xFFFFFFFF
 
 
+@pytest.mark.parametrize("layout", ["geometric", "coincident"])
+def test_raycast_unbalanced_meshes(ctx, orc, layout):
+    """Meshes that a surface-area split would cut one triangle at a time (3,000 small triangles
+    at geometrically growing distances along x), or cannot split at all (2,000 copies of one
+    triangle): the tree stays within the traversal stacks and every hit equals brute force."""
+    rng = np.random.default_rng(11)
+    if layout == "geometric":
+        n = 3000
+        x = (1.02 ** np.arange(n)).astype(np.float32)
+        base = np.stack([x, np.zeros(n), np.zeros(n)], 1)
+        pts = np.concatenate([base, base + [0, 1, 0], base + [0, 0, 1]]).astype(np.float32)
+        tri = np.stack([np.arange(n), np.arange(n) + n, np.arange(n) + 2 * n], 1).astype(np.uint32)
+        # rays along +x from between the triangles, through their (y, z) extent
+        o = np.stack([x[rng.integers(0, n, 2000)] * rng.uniform(0.5, 1.0, 2000), rng.uniform(0.05, 0.45, 2000),
+                      rng.uniform(0.05, 0.45, 2000)], 1).astype(np.float32)
+        d = np.tile(np.array([[1, 0, 0]], np.float32), (2000, 1))
+    else:
+        n = 2000
+        pts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+        tri = np.tile(np.array([[0, 1, 2]], np.uint32), (n, 1))
+        o = np.concatenate([rng.uniform(-0.2, 1.2, (1000, 2)), np.ones((1000, 1))], 1).astype(np.float32)
+        d = np.tile(np.array([[0, 0, -1]], np.float32), (1000, 1))
+    mesh = ncx.TriMesh(pts, tri, ctx=ctx)
+    toi, nrm, feat = mesh.toi_and_normal_with_ray(o, d, 1e30)
+    om = orc.TriMesh(pts, tri)
+    bt, bn, bf, _ = om.brute(o, d, 1e30, nthreads=8)
+    assert np.array_equal(feat.cpu().numpy().view(np.uint32), bf)
+    assert np.array_equal(toi.cpu().numpy(), bt)
+    assert np.array_equal(nrm.cpu().numpy(), bn)
+    assert (bf != 0xFFFFFFFF).mean() > 0.2
+
+
 def test_raycast_posed_mesh(ctx, orc):
     pts, tri = workloads.heightfield_mesh(seed=7, cells=64)
     rng = np.random.default_rng(5)
