@@ -746,6 +746,13 @@ static void sah_mesh_tree(const float* pts, const uint32_t* idx, size_t nt, floa
       cen[3 * t + d] = 0.5f * lo + 0.5f * hi;
     }
   }
+  // a centroid's bin (NaN and out-of-range values go to the end bins, so the index is in range)
+  auto bin_of = [](float c, float c0, float scale) {
+    const float f = (c - c0) * scale;
+    return f >= 0.f ? (f < (float)SAH_BINS ? (int)f : SAH_BINS - 1) : 0;
+  };
+  // the median split's order: NaN as +infinity, ties by index (a strict weak order)
+  auto key = [](float c) { return c == c ? c : INFINITY; };
   std::vector<uint32_t> perm(nt);
   for (size_t t = 0; t < nt; ++t) perm[t] = (uint32_t)t;
   nodes.assign(nt > 1 ? nt - 1 : 0, LNode{});
@@ -801,7 +808,7 @@ static void sah_mesh_tree(const float* pts, const uint32_t* idx, size_t nt, floa
           for (int a = 0; a < 3; ++a) { lo[k][a] = INFINITY; hi[k][a] = -INFINITY; }
         for (uint32_t i = tk.b; i < tk.e; ++i) {
           const uint32_t t = perm[i];
-          const int k = std::min(SAH_BINS - 1, (int)((cen[3 * t + d] - cmin[d]) * scale));
+          const int k = bin_of(cen[3 * t + d], cmin[d], scale);
           cnt[k]++;
           for (int a = 0; a < 3; ++a) {
             lo[k][a] = std::fmin(lo[k][a], bl[3 * t + a]);
@@ -832,16 +839,17 @@ static void sah_mesh_tree(const float* pts, const uint32_t* idx, size_t nt, floa
       const float scale = SAH_BINS / (cmax[best_axis] - cmin[best_axis]);
       const float c0 = cmin[best_axis];
       const int ax = best_axis, bb = best_bin;
-      mid = (uint32_t)(std::partition(perm.begin() + tk.b, perm.begin() + tk.e, [&](uint32_t t) {
-                         return std::min(SAH_BINS - 1, (int)((cen[3 * t + ax] - c0) * scale)) <= bb;
-                       }) - perm.begin());
+      mid = (uint32_t)(std::partition(perm.begin() + tk.b, perm.begin() + tk.e,
+                                      [&](uint32_t t) { return bin_of(cen[3 * t + ax], c0, scale) <= bb; }) -
+                       perm.begin());
     } else {  // no usable split (equal centroids) or depth bound: the median on the longest axis
       int ax = 0;
       for (int d = 1; d < 3; ++d)
         if (cmax[d] - cmin[d] > cmax[ax] - cmin[ax]) ax = d;
       mid = tk.b + n / 2;
       std::nth_element(perm.begin() + tk.b, perm.begin() + mid, perm.begin() + tk.e, [&](uint32_t x, uint32_t y) {
-        return cen[3 * x + ax] < cen[3 * y + ax] || (cen[3 * x + ax] == cen[3 * y + ax] && x < y);
+        const float kx = key(cen[3 * x + ax]), ky = key(cen[3 * y + ax]);
+        return kx < ky || (kx == ky && x < y);
       });
     }
     const int64_t ln = mid - tk.b > 1 ? (int64_t)next_node++ : -1;
