@@ -196,7 +196,7 @@ struct MDArgs {
   uint32_t* err;
 };
 
-#define MD_G 8
+#define MD_G 2
 #define MD_BLOCK 128
 
 // Distance of each query (one group of MD_G lanes per query): nearest-first search of the tree,
