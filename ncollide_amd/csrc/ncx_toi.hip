@@ -13,7 +13,7 @@
 // arithmetic restates the reference in f32 with its operation order.
 #include "ncx_gjk.h"
 
-#define TOI_G 8
+#define TOI_G 4
 #define TOI_BS 256
 
 struct TOIOut {
