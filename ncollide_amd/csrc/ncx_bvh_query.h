@@ -154,9 +154,14 @@ __global__ __launch_bounds__(BQ_BLOCK) void k_bq_collect(BQArgs a) {
 
 // First hit of each ray among the proxies' shapes (solid = true, toi <= max_toi): the least
 // (toi, handle) over the leaves whose box the ray enters no later than the best hit so far.
+// G lanes per ray: they run the same search, and split the support scans of hull shapes at the
+// leaves (shape_ray_cast<G>).
+#define BQ_RAY_G 4
+template <int G>
 static __global__ __launch_bounds__(BQ_BLOCK) void k_bq_first_ray(BQArgs a) {
-  const uint32_t qi = blockIdx.x * BQ_BLOCK + threadIdx.x;
-  if (qi >= a.nq) return;
+  const uint32_t qi = (blockIdx.x * BQ_BLOCK + threadIdx.x) / G;
+  const int gl = threadIdx.x % G;
+  if (qi >= a.nq) return;  // whole groups: BQ_BLOCK is a multiple of G
   float qv[6];
   for (int d = 0; d < 6; ++d) qv[d] = a.q[d][qi];
   const f3 o = f3{qv[0], qv[1], qv[2]}, dir = f3{qv[3], qv[4], qv[5]};
@@ -168,7 +173,7 @@ static __global__ __launch_bounds__(BQ_BLOCK) void k_bq_first_ray(BQArgs a) {
     float t;
     f3 n;
     uint32_t feat;
-    if (shape_ray_cast<1>(load_shape(a.shapes, h), load_pose(a.poses, h), o, dir, a.max_toi, true, a.P, 0, t, n,
+    if (shape_ray_cast<G>(load_shape(a.shapes, h), load_pose(a.poses, h), o, dir, a.max_toi, true, a.P, gl, t, n,
                           feat)) {
       if (t < best_t || (t == best_t && h < best_h)) {
         best_t = t;
@@ -219,12 +224,14 @@ static __global__ __launch_bounds__(BQ_BLOCK) void k_bq_first_ray(BQArgs a) {
       }
     }
   }
+  if (gl != 0) return;
   a.handles[qi] = best_h;
   a.toi[qi] = best_h == BQ_NONE ? -1.f : best_t;
   a.nrm[3 * (size_t)qi] = best_n.x;
   a.nrm[3 * (size_t)qi + 1] = best_n.y;
   a.nrm[3 * (size_t)qi + 2] = best_n.z;
 }
+static_assert(BQ_BLOCK % BQ_RAY_G == 0, "a block holds whole groups");
 
 // Count pass, scan, write pass.  *total = hits over all queries; NCX_E_CAPACITY when above cap
 // (nothing written).  offs (nq + 1 entries) receives each query's first slot and the total.

@@ -927,7 +927,8 @@ extern "C" int ncx_world_first_interference_with_rays(ncx_world* w, const ncx_po
   int rc;
   if ((rc = w->qtot.reserve(16)) || (rc = fill_u32(w->ctx, w->qtot.as<uint32_t>() + 1, 1, 0u))) return rc;
   a.err = w->qtot.as<uint32_t>() + 1;
-  hipLaunchKernelGGL(k_bq_first_ray, dim3(ceil_div(n, BQ_BLOCK)), dim3(BQ_BLOCK), 0, w->ctx->stream, a);
+  hipLaunchKernelGGL(k_bq_first_ray<BQ_RAY_G>, dim3(ceil_div(n * BQ_RAY_G, BQ_BLOCK)), dim3(BQ_BLOCK), 0,
+                     w->ctx->stream, a);
   NCX_HIP_CHECK(hipGetLastError());
   uint32_t err = 0;
   NCX_HIP_CHECK(hipMemcpyAsync(&err, w->qtot.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, w->ctx->stream));
