@@ -195,8 +195,8 @@ int ncx_shapes_cast_rays(ncx_shapes* s, const ncx_pose* poses_dev, const uint32_
                          float* normal_dev, uint32_t* feature_dev);
 
 /* ---------------------------------------------------------------- TriMesh ray casting
- * TriMesh::new (src/shape/trimesh.rs:100-197) with a GPU-built LBVH in place of
- * the median-split BVT; host arrays. */
+ * TriMesh::new (src/shape/trimesh.rs:100-197) with a binned-SAH tree (built on the host, then
+ * a 4-wide cast copy on the device) in place of the median-split BVT; host arrays. */
 int ncx_trimesh_create(ncx_ctx* ctx, const float* points_xyz, size_t npoints, const uint32_t* indices,
                        size_t ntriangles, ncx_mesh** mesh);
 int ncx_trimesh_destroy(ncx_mesh* mesh);
