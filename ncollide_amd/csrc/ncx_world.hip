@@ -361,7 +361,7 @@ static unsigned world_query_blocks(const ncx_ctx* c, size_t n) {
 // pair slots.
 #define WQ_EMIT_BLOCK 256
 #define EMIT_LEAVES 64  // leaves per block: 256 leaves per block left dense clusters to few blocks
-#define EMIT_UNROLL 4
+#define EMIT_UNROLL 2
 static_assert(EMIT_LEAVES == 64, "the emit kernel's prefix sum over a block's leaf counts is one wavefront's scan");
 static_assert(WQ_EMIT_BLOCK >= EMIT_LEAVES, "the first wavefront holds one leaf count per lane");
 __global__ __launch_bounds__(WQ_EMIT_BLOCK) void k_world_emit(const uint32_t* __restrict__ order, uint32_t n,
