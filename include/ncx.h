@@ -138,9 +138,10 @@ int ncx_shapes_create(ncx_ctx* ctx, const ncx_shape* shapes, size_t n, const flo
 int ncx_shapes_destroy(ncx_shapes* s);
 int ncx_shapes_count(ncx_shapes* s, size_t* n);
 /* No reference counterpart: the autotuner's choice per vertex-count bucket (pairs by their larger
- * hull vertex count: none, <= 32, <= 96, more) for query 0 (contact) or 1 (distance): 0, 1, 2 =
- * 8, 16, 32 lanes per pair with the 44-vertex first-tier EPA arena, 3, 4 = 8, 16 lanes with the
- * 24-vertex arena (3, 4 contact only), -1 before the first batch large enough to tune on;
+ * hull vertex count: none, <= 32, <= 96, more) for query 0 (contact) or 1 (distance): contact
+ * 0, 1, 2 = 8, 16, 32 lanes per pair with the 44-vertex first-tier EPA arena, 3, 4 = 8, 16 lanes
+ * with the 24-vertex arena; distance (no arena) 0, 1, 2, 3, 4 = 8, 16, 32, 4, 2 lanes per pair;
+ * -1 before the first batch large enough to tune on;
  * sample_ms (nullable, 20 floats) the sample time of each variant per bucket (0 for a variant not
  * tried). */
 int ncx_shapes_tuning(ncx_shapes* s, int query, int32_t* choice, float* sample_ms);

@@ -560,9 +560,11 @@ static int rerun_overflows(ContactTiers& T, ncx_ctx* ctx, KArgs a, uint32_t n, c
 // 24-vertex, 80-face arena (variants 3-4); every variant holds 8 pairs per block.  The wide
 // arena's variants differ in occupancy and lanes per pair only; the small arena's hold more pairs
 // per CU and send more pairs to the second tier, which the tuner times with them.  Distance
-// queries need no EPA arena and use 128-thread variants without LDS (variants 0-2).
-static const int T1_NVARIANTS[2] = {NCX_T1_VARIANTS, 3};
-static const int T1_VARIANT_G[NCX_T1_VARIANTS] = {8, 16, 32, 8, 16};
+// queries need no EPA arena and use 128-thread variants without LDS with 8, 16, 32, 4 or 2 lanes.
+static const int T1_NVARIANTS[2] = {NCX_T1_VARIANTS, NCX_T1_VARIANTS};
+// lanes per pair: contact variants 3, 4 are the small-arena ones; distance variants 3, 4 (no
+// arena) take 4 and 2 lanes, for pairs whose supports are closed forms or small hulls
+static const int T1_VARIANT_G[2][NCX_T1_VARIANTS] = {{8, 16, 32, 8, 16}, {8, 16, 32, 4, 2}};
 static const int T1_VARIANT_BS[2][NCX_T1_VARIANTS] = {{64, 128, 256, 64, 128}, {128, 128, 128, 128, 128}};
 // twice the resident blocks (LDS: 4 wide-arena or 7 small-arena blocks per CU; VGPRs: 8 waves).
 // A middle arena (32 vertices, 112 faces, 8 lanes) was never the tuner's choice on the scene's
@@ -582,6 +584,8 @@ static int launch_t1(int variant, bool distance, hipStream_t stream, const KArgs
     switch (variant) {
       case 0: hipLaunchKernelGGL((k_pairs<8, 128, 0u, 0u, 0u, 0u, 1>), dim3(blocks), dim3(128), 0, stream, a); break;
       case 2: hipLaunchKernelGGL((k_pairs<32, 128, 0u, 0u, 0u, 0u, 1>), dim3(blocks), dim3(128), 0, stream, a); break;
+      case 3: hipLaunchKernelGGL((k_pairs<4, 128, 0u, 0u, 0u, 0u, 1>), dim3(blocks), dim3(128), 0, stream, a); break;
+      case 4: hipLaunchKernelGGL((k_pairs<2, 128, 0u, 0u, 0u, 0u, 1>), dim3(blocks), dim3(128), 0, stream, a); break;
       default: hipLaunchKernelGGL((k_pairs<16, 128, 0u, 0u, 0u, 0u, 1>), dim3(blocks), dim3(128), 0, stream, a); break;
     }
   } else {
@@ -842,7 +846,7 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
     ab.work = ncx_work_set(ovf, NCX_WORK_BKT);
     unsigned blocks = (unsigned)ctx->num_cus * T1_BLOCKS_PER_CU[mode][v];
     if (!n_dev)
-      blocks = std::max(1u, std::min<unsigned>(blocks, ceil_div(npairs * T1_VARIANT_G[v], T1_VARIANT_BS[mode][v])));
+      blocks = std::max(1u, std::min<unsigned>(blocks, ceil_div(npairs * T1_VARIANT_G[mode][v], T1_VARIANT_BS[mode][v])));
     if ((rc = launch_t1(v, mode == 1, ctx->stream, ab, blocks))) return rc;
   } else {
     // one launch per run of consecutive buckets sharing a variant, over the partitioned list
@@ -859,7 +863,7 @@ static int launch_tiers(ncx_shapes* S, ContactTiers& T, ncx_ctx* ctx, KArgs a, s
       ab.work = ncx_work_set(ovf, NCX_WORK_BKT + b0);
       unsigned blocks = (unsigned)ctx->num_cus * T1_BLOCKS_PER_CU[mode][v];
       if (!n_dev)
-        blocks = std::max(1u, std::min<unsigned>(blocks, ceil_div(npairs * T1_VARIANT_G[v], T1_VARIANT_BS[mode][v])));
+        blocks = std::max(1u, std::min<unsigned>(blocks, ceil_div(npairs * T1_VARIANT_G[mode][v], T1_VARIANT_BS[mode][v])));
       if ((rc = launch_t1(v, mode == 1, ctx->stream, ab, blocks))) return rc;
       b0 = b1 + 1;
     }

@@ -111,16 +111,19 @@ class ShapeSet:
 
     # first-tier variants: (lanes per pair, EPA arena vertices)
     VARIANTS = ((8, 44), (16, 44), (32, 44), (8, 24), (16, 24))
+    # distance queries have no EPA arena: lanes per pair of each variant
+    DISTANCE_VARIANTS = ((8, 0), (16, 0), (32, 0), (4, 0), (2, 0))
 
     def tuning(self, query="contact"):
         """The autotuner's choice per vertex-count bucket (no hull, <= 32, <= 96, more vertices):
-        the variant (lanes per pair, first-tier EPA arena vertices) or None before tuning, and each
-        variant's sample time (ms; 0 for variants not tried)."""
-        nv = len(self.VARIANTS)
+        the variant (lanes per pair, first-tier EPA arena vertices, 0 for distance) or None before
+        tuning, and each variant's sample time (ms; 0 for variants not tried)."""
+        table = self.VARIANTS if query == "contact" else self.DISTANCE_VARIANTS
+        nv = len(table)
         choice = (C.c_int32 * 4)()
         ms = (C.c_float * (4 * nv))()
         _lib.check(self.ctx.lib.ncx_shapes_tuning(self.h, 0 if query == "contact" else 1, choice, ms))
-        return ([self.VARIANTS[c] if c >= 0 else None for c in choice], [list(ms[nv * b:nv * b + nv]) for b in range(4)])
+        return ([table[c] if c >= 0 else None for c in choice], [list(ms[nv * b:nv * b + nv]) for b in range(4)])
 
     def set_tuning(self, variants, query="contact"):
         """Pins the first-tier variant of each bucket: a list of 4 indices into VARIANTS (None =

@@ -336,6 +336,27 @@ def test_contact_every_first_tier_variant_matches_oracle(ctx, orc, variant):
           f"{int(np.sum(r['epa_iters'] > 20))} EPA runs past 20 iterations")
 
 
+@pytest.mark.parametrize("variant", range(5))
+def test_distance_every_first_tier_variant_matches_oracle(ctx, orc, variant):
+    """Each distance kernel variant (8, 16, 32, 4, 2 lanes per pair), pinned on every bucket:
+    the same bit-exact distances and GJK iteration counts."""
+    w = workloads.full_pipeline(seed=6, n=6000, box=30.0, nclusters=3)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    ss.set_tuning([variant] * 4, query="distance")
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    bb = ss.aabbs(poses, margin=0.3).cpu().numpy().T
+    keys = orc.pairs_brute(bb)
+    ia = (keys >> np.uint64(32)).astype(np.uint32)
+    ib = (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    d, it = ncx.query.distance(ss, poses, ia, ib)
+    rd, rit = orc.distance(oracle_shapes(orc, w["records"], w["pool"]), to_oracle_poses(w["poses"]), ia, ib,
+                           nthreads=8)
+    assert np.array_equal(d.cpu().numpy().view(np.uint32), np.asarray(rd, np.float32).view(np.uint32))
+    assert np.array_equal(it.cpu().numpy(), np.asarray(rit))
+    assert ss.tuning("distance")[0] == [ss.DISTANCE_VARIANTS[variant]] * 4
+    assert (np.asarray(rd) > 0).mean() > 0.3
+
+
 def _curved_pairs(seed, n):
     """n isolated Ball-Capsule / Capsule-Capsule pairs at random overlaps (shapes 2i, 2i + 1)."""
     rng = np.random.default_rng(seed)
