@@ -1174,15 +1174,6 @@ int shapes_aabbs_stage_on(ncx_shapes* s, ncx_ctx* ctx, const ncx_pose* poses, fl
   return NCX_OK;
 }
 
-int shapes_aabbs_on(ncx_shapes* s, ncx_ctx* ctx, const ncx_pose* poses, float margin, float* const out[6]) {
-  if (s->n == 0) return NCX_OK;
-  hipLaunchKernelGGL(k_shape_aabbs<false>, dim3(ceil_div(s->n, 256)), dim3(256), 0, ctx->stream,
-                     s->shapes.as<ncx_shape>(), Pool{s->hx.as<float>(), s->hy.as<float>(), s->hz.as<float>(), s->tri}, poses,
-                     s->n, margin, out[0], out[1], out[2], out[3], out[4], out[5]);
-  NCX_HIP_CHECK(hipGetLastError());
-  return NCX_OK;
-}
-
 extern "C" int ncx_shapes_aabbs_dev(ncx_shapes* s, const ncx_pose* poses, float margin, float* const out[6]) {
   NCX_REQUIRE(s && poses && out, "ncx_shapes_aabbs_dev: null argument");
   NCX_REQUIRE(margin >= 0.f, "ncx_shapes_aabbs_dev: the loosening margin must be positive");

@@ -21,7 +21,6 @@ struct ncx_mlp;
 int mlp_predict_device_count(ncx_mlp* m, ncx_ctx* ctx, const ncx_shape* shapes, const ncx_pose* poses,
                              const float* desc, const uint32_t* ia, const uint32_t* ib, const uint32_t* n_dev,
                              size_t cap, float* dirs);
-int shapes_aabbs_on(ncx_shapes* s, ncx_ctx* ctx, const ncx_pose* poses, float margin, float* const out[6]);
 int shapes_aabbs_stage_on(ncx_shapes* s, ncx_ctx* ctx, const ncx_pose* poses, float margin, float* st, float4* sb,
                           float stage_margin, bool first);
 int shape_descriptors_on(ncx_shapes* s, ncx_ctx* ctx, float* desc);
