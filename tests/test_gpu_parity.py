@@ -158,6 +158,26 @@ def test_raycast_unbalanced_meshes(ctx, orc, layout):
     assert (bf != 0xFFFFFFFF).mean() > 0.2
 
 
+@pytest.mark.parametrize("ntri", [2, 3, 5, 9])
+def test_raycast_small_meshes(ctx, orc, ntri):
+    """Meshes of a few triangles, where the cast tree's leaves of up to two triangles meet its
+    root: every hit equals brute force."""
+    rng = np.random.default_rng(ntri)
+    pts = rng.uniform(-1, 1, (3 * ntri, 3)).astype(np.float32)
+    tri = np.arange(3 * ntri, dtype=np.uint32).reshape(ntri, 3)
+    o = rng.uniform(-2, 2, (3000, 3)).astype(np.float32)
+    d = workloads.unit_vectors(rng, 3000).astype(np.float32)
+    d = np.where((o @ np.ones(3))[:, None] > 0, -np.abs(d), np.abs(d)).astype(np.float32)  # mostly inward
+    mesh = ncx.TriMesh(pts, tri, ctx=ctx)
+    toi, nrm, feat = mesh.toi_and_normal_with_ray(o, d, 10.0)
+    om = orc.TriMesh(pts, tri)
+    bt, bn, bf, _ = om.brute(o, d, 10.0, nthreads=8)
+    assert np.array_equal(feat.cpu().numpy().view(np.uint32), bf)
+    assert np.array_equal(toi.cpu().numpy(), bt)
+    assert np.array_equal(nrm.cpu().numpy(), bn)
+    assert (bf != 0xFFFFFFFF).sum() > 50
+
+
 def test_raycast_posed_mesh(ctx, orc):
     pts, tri = workloads.heightfield_mesh(seed=7, cells=64)
     rng = np.random.default_rng(5)

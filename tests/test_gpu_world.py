@@ -240,3 +240,33 @@ def test_world_leaf_past_the_spill_directory(ctx, orc, use_graph):
         assert np.sum((got >> np.uint64(32)) == 0) == n - 1  # the cuboid touches every ball
         over, spilled = world.spill_stats()
         assert over >= 1 and spilled < over
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_world_tiny_scenes(ctx, orc, n):
+    """Scenes of one to three overlapping balls: the pair list and contacts of each step equal
+    brute force over the stored boxes and the oracle's contacts."""
+    rec = np.zeros(n, workloads.SHAPE_DTYPE)
+    rec["type"][:] = ncx.BALL
+    rec["p0"][:] = 0.5
+    t = np.zeros((n, 3), np.float32)
+    t[:, 0] = np.arange(n) * 0.6
+    q = np.tile(np.array([[0, 0, 0, 1]], np.float32), (n, 1))
+    p_np = workloads.poses8(t, q)
+    poses = torch.as_tensor(p_np, device=ctx.device)
+    ss = ncx.ShapeSet(records=rec, pool=np.zeros((1, 3), np.float32), ctx=ctx)
+    world = World(ss, margin=0.05, prediction=0.0, max_pairs=64, rebuild_every=2)
+    oss = oracle_shapes(orc, rec, np.zeros((1, 3), np.float32))
+    for step in range(3):
+        world.step(poses, use_graph=step > 0)
+        r = world.results()
+        got = np.sort(keys_of(r))
+        stored = np.ascontiguousarray(world.stored_aabbs().cpu().numpy().T)
+        assert np.array_equal(got, np.sort(orc.pairs_brute(stored)))
+        assert len(got) == n - 1  # neighbours 0.6 apart with radius 0.5 touch; 1.2 apart do not
+        if len(got):
+            ia = r["ia"].cpu().numpy().astype(np.uint32)
+            ib = r["ib"].cpu().numpy().astype(np.uint32)
+            ref = orc.contact(oss, to_oracle_poses(p_np), ia, ib, 0.0)
+            assert np.array_equal(r["has_contact"].cpu().numpy().astype(bool), ref["has"].astype(bool))
+            assert np.array_equal(r["depth"].cpu().numpy(), np.asarray(ref["depth"], np.float32))
