@@ -574,8 +574,18 @@ __device__ __forceinline__ CSO cso_from_r(const RCtx& c, f3 dir, const Pool& P, 
   return r;
 }
 
+__device__ __forceinline__ bool same_bits(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
+__device__ __forceinline__ bool same_bits3(f3 a, f3 b) {
+  return same_bits(a.x, b.x) && same_bits(a.y, b.y) && same_bits(a.z, b.z);
+}
+
 // gjk::minkowski_ray_cast (src/query/algorithms/gjk.rs:245-365) on m1*g1 - m2*g2, ray (o, d).
 // Returns false for None.  The improved_fixed_point_support feature is off (the crate's default).
+// An iteration that leaves every value the loop reads bit-identical (the simplex's points and
+// dimension, the ray origin and toi, the projection, max_bound, last_chance and the last normal)
+// is repeated identically until the 10000-iteration cap, so it returns None at once: a shift too
+// small to change the origin (its f32 sums absorb it) with a support point the simplex rejects
+// as affinely dependent is such a fixed point.
 template <int G>
 __device__ __forceinline__ bool minkowski_ray_cast(const RCtx& c, f3 ro, f3 rd, float max_toi, Simplex& s,
                                                    const Pool& P, int gl, float& toi_out, f3& normal_out) {
@@ -599,6 +609,10 @@ __device__ __forceinline__ bool minkowski_ray_cast(const RCtx& c, f3 ro, f3 rd, 
   bool last_chance = false;
   for (;;) {
     const float old_max_bound = max_bound;
+    const f3 proj_in = proj;
+    const bool lc_in = last_chance;
+    const int dim_in = s.dim;
+    bool same = true;  // nothing changed so far in this iteration
     {
       f3 nd;
       float dist;
@@ -623,17 +637,24 @@ __device__ __forceinline__ bool minkowski_ray_cast(const RCtx& c, f3 ro, f3 rd, 
     }
     if (has_t) {
       if (denom < 0.f && t > 0.f) {
+        same = same_bits3(ldir, dir);
         ldir = dir;
+        const float ltoi_in = ltoi;
         ltoi += t;
         if (ltoi / ray_length > max_toi) return false;
         const f3 shift = cd * t;
+        const f3 co_in = co;
         co = co + shift;
         max_bound = NCX_FLT_MAX;
         const f3 ns = -shift;  // VoronoiSimplex::modify_pnts over vertices 0..=dim (voronoi_simplex3.rs:347-351)
-        s.v0.p = s.v0.p + ns;
-        s.v1.p = sel3(s.dim >= 1, s.v1.p + ns, s.v1.p);
-        s.v2.p = sel3(s.dim >= 2, s.v2.p + ns, s.v2.p);
-        s.v3.p = sel3(s.dim >= 3, s.v3.p + ns, s.v3.p);
+        const f3 v0 = s.v0.p + ns, v1 = s.v1.p + ns, v2 = s.v2.p + ns, v3 = s.v3.p + ns;
+        same = same && same_bits(ltoi, ltoi_in) && same_bits3(co, co_in) && same_bits3(v0, s.v0.p) &&
+               (s.dim < 1 || same_bits3(v1, s.v1.p)) && (s.dim < 2 || same_bits3(v2, s.v2.p)) &&
+               (s.dim < 3 || same_bits3(v3, s.v3.p));
+        s.v0.p = v0;
+        s.v1.p = sel3(s.dim >= 1, v1, s.v1.p);
+        s.v2.p = sel3(s.dim >= 2, v2, s.v2.p);
+        s.v3.p = sel3(s.dim >= 3, v3, s.v3.p);
         last_chance = false;
       }
     } else if (denom > eps_tol) {
@@ -642,10 +663,11 @@ __device__ __forceinline__ bool minkowski_ray_cast(const RCtx& c, f3 ro, f3 rd, 
     if (last_chance) return false;
     const float min_bound = -dot(dir, sp.p - co);
     if (max_bound - min_bound <= eps_rel * max_bound) return false;
+    bool added;
     {
       CSO tp = sp;
       tp.p = sp.p + (-co);
-      (void)sx_add(s, tp);
+      added = sx_add(s, tp);
     }
     proj = sx_project_reduce(s);
     if (s.dim == 3) {
@@ -654,6 +676,11 @@ __device__ __forceinline__ bool minkowski_ray_cast(const RCtx& c, f3 ro, f3 rd, 
       normal_out = ldir;
       return true;
     }
+    // a rejected point leaves the points in place, and a reduction that keeps the dimension of
+    // a point, segment or triangle keeps them in order
+    if (same && !added && s.dim == dim_in && last_chance == lc_in && same_bits(max_bound, old_max_bound) &&
+        same_bits3(proj, proj_in))
+      return false;
     niter += 1;
     if (niter == 10000) return false;
   }
@@ -767,9 +794,10 @@ __device__ __forceinline__ bool ray_box_normal(f3 h, f3 ro, f3 rd, float max_toi
 // (ray_cuboid.rs:17-26 -> ray_aabb.rs:45-61), Capsule and ConvexHull (ray_support_map.rs:117-163:
 // local-frame ray, normal rotated back).  feature: face id, 0 for a ball, NCX_FEAT_UNKNOWN for
 // support-map shapes.
-template <int G>
-__device__ __forceinline__ bool shape_ray_cast(const ShapeD& g, const iso& m, f3 ro, f3 rd, float max_toi, bool solid,
-                                               const Pool& P, int gl, float& toi, f3& normal, uint32_t& feature) {
+// The closed forms (Ball, Cuboid) of shape_ray_cast below; no cross-lane work, so one lane can
+// cast a ray alone.  A shape of another type gives no hit.
+__device__ __forceinline__ bool shape_ray_cast_closed(const ShapeD& g, const iso& m, f3 ro, f3 rd, float max_toi,
+                                                      bool solid, float& toi, f3& normal, uint32_t& feature) {
   if (g.type == NCX_BALL) {
     bool inside;
     float t;
@@ -781,8 +809,8 @@ __device__ __forceinline__ bool shape_ray_cast(const ShapeD& g, const iso& m, f3
     feature = 0;
     return true;
   }
-  const f3 lo = inv_xform_pt(m, ro), ld = inv_xform_vec(m, rd);
   if (g.type == NCX_CUBOID) {
+    const f3 lo = inv_xform_pt(m, ro), ld = inv_xform_vec(m, rd);
     float t;
     f3 n;
     int side;
@@ -792,7 +820,16 @@ __device__ __forceinline__ bool shape_ray_cast(const ShapeD& g, const iso& m, f3
     feature = side < 0 ? (uint32_t)(-side - 1 + 3) : (uint32_t)(side - 1);
     return true;
   }
+  return false;
+}
+
+template <int G>
+__device__ __forceinline__ bool shape_ray_cast(const ShapeD& g, const iso& m, f3 ro, f3 rd, float max_toi, bool solid,
+                                               const Pool& P, int gl, float& toi, f3& normal, uint32_t& feature) {
+  if (g.type == NCX_BALL || g.type == NCX_CUBOID)
+    return shape_ray_cast_closed(g, m, ro, rd, max_toi, solid, toi, normal, feature);
   if (g.type == NCX_CAPSULE || g.type == NCX_CONVEX_HULL) {
+    const f3 lo = inv_xform_pt(m, ro), ld = inv_xform_vec(m, rd);
     float t;
     f3 n;
     if (!ray_support_map<G>(g, lo, ld, max_toi, solid, P, gl, t, n)) return false;

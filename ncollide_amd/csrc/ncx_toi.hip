@@ -8,8 +8,9 @@
 // RayCast::toi_and_normal_with_ray of the same four shapes (ray_ball.rs, ray_cuboid.rs,
 // ray_aabb.rs, ray_support_map.rs) over a batch of (shape, ray) pairs.
 //
-// One group of TOI_G lanes evaluates one pair (or one ray); the lanes split ConvexHull support
-// scans and run the same control flow.  Groups take queries in a grid-stride loop.  All
+// One group of TOI_G lanes evaluates one pair, or one ray on a support-map shape; the lanes split
+// ConvexHull support scans and run the same control flow.  Groups take queries in a grid-stride
+// loop; the ray kernel sorts each wavefront's rays by shape kind first (k_shape_rays).  All
 // arithmetic restates the reference in f32 with its operation order.
 #include "ncx_gjk.h"
 
@@ -193,35 +194,85 @@ struct SRArgs {
   uint32_t* err;
 };
 
+// Shape ray casts, compacted per wavefront.  A wavefront takes 64 consecutive rays at a time:
+// each ray on a Ball or a Cuboid (closed forms) is cast by its own lane at once; the rays on
+// support-map shapes are listed in LDS, ConvexHulls first and Capsules after, and are cast
+// G lanes per ray, 64 / G at a time, so that each pass holds rays of one kind save at most one
+// pass.  The outputs of every ray are what shape_ray_cast<G> gives it.
+#define SR_WAVES (TOI_BS / 64)
 template <int G>
 __global__ __launch_bounds__(TOI_BS) void k_shape_rays(SRArgs a) {
-  const int gl = threadIdx.x % G;
-  const size_t group = ((size_t)blockIdx.x * TOI_BS + threadIdx.x) / G;
-  const size_t ngroups = (size_t)gridDim.x * TOI_BS / G;
-  for (size_t k = group; k < a.n; k += ngroups) {
-    const uint32_t i = a.idx[k];
-    float t = -1.f;
-    f3 nn = f3{0.f, 0.f, 0.f};
-    uint32_t f = 0xFFFFFFFFu;
-    if (i >= a.nshapes) {
-      if (gl == 0) atomicOr(a.err, 1u);
-    } else {
-      const iso m = load_pose(a.poses, i);
-      float tt;
-      f3 n;
-      uint32_t ff;
-      if (shape_ray_cast<G>(load_shape(a.shapes, i), m, f3{a.ox[k], a.oy[k], a.oz[k]},
-                            f3{a.dx[k], a.dy[k], a.dz[k]}, a.max_toi, a.solid != 0, a.P, gl, tt, n, ff)) {
-        t = tt;
-        nn = n;
-        f = ff;
+  static_assert(TOI_BS % 64 == 0 && 64 % G == 0, "whole wavefronts of whole groups");
+  __shared__ uint8_t list[SR_WAVES][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int gl = lane % G, gi = lane / G;
+  const size_t nw = (size_t)gridDim.x * SR_WAVES;
+  for (size_t base = ((size_t)blockIdx.x * SR_WAVES + wv) * 64; base < a.n; base += nw * 64) {
+    const size_t k = base + (size_t)lane;
+    int cls = 0;  // 0: done by this lane; 1: ConvexHull; 2: Capsule
+    if (k < a.n) {
+      const uint32_t i = a.idx[k];
+      float t = -1.f;
+      f3 nn = f3{0.f, 0.f, 0.f};
+      uint32_t f = 0xFFFFFFFFu;
+      if (i >= a.nshapes) {
+        atomicOr(a.err, 1u);
+      } else {
+        const ShapeD g = load_shape(a.shapes, i);
+        if (g.type == NCX_CONVEX_HULL) cls = 1;
+        else if (g.type == NCX_CAPSULE) cls = 2;
+        else {
+          float tt;
+          f3 n;
+          uint32_t ff;
+          if (shape_ray_cast_closed(g, load_pose(a.poses, i), f3{a.ox[k], a.oy[k], a.oz[k]},
+                                    f3{a.dx[k], a.dy[k], a.dz[k]}, a.max_toi, a.solid != 0, tt, n, ff)) {
+            t = tt;
+            nn = n;
+            f = ff;
+          }
+        }
+      }
+      if (cls == 0) {
+        a.toi[k] = t;
+        store3(a.nrm, k, nn);
+        a.feat[k] = f;
       }
     }
-    if (gl == 0) {
-      a.toi[k] = t;
-      store3(a.nrm, k, nn);
-      a.feat[k] = f;
+    const uint64_t mh = __ballot(cls == 1), mc = __ballot(cls == 2);
+    const uint32_t nh = (uint32_t)__popcll(mh), m = nh + (uint32_t)__popcll(mc);
+    if (cls) {
+      const uint64_t mm = cls == 1 ? mh : mc;
+      const uint32_t rank =
+          __builtin_amdgcn_mbcnt_hi((uint32_t)(mm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mm, 0u));
+      list[wv][(cls == 1 ? 0u : nh) + rank] = (uint8_t)lane;
     }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (uint32_t e0 = 0; e0 < m; e0 += 64 / G) {
+      const uint32_t e = e0 + (uint32_t)gi;
+      if (e < m) {
+        const size_t kk = base + list[wv][e];
+        const uint32_t i = a.idx[kk];  // a shape of the set: listed above
+        float t = -1.f, tt;
+        f3 nn = f3{0.f, 0.f, 0.f}, n;
+        uint32_t f = 0xFFFFFFFFu, ff;
+        if (shape_ray_cast<G>(load_shape(a.shapes, i), load_pose(a.poses, i), f3{a.ox[kk], a.oy[kk], a.oz[kk]},
+                              f3{a.dx[kk], a.dy[kk], a.dz[kk]}, a.max_toi, a.solid != 0, a.P, gl, tt, n, ff)) {
+          t = tt;
+          nn = n;
+          f = ff;
+        }
+        if (gl == 0) {
+          a.toi[kk] = t;
+          store3(a.nrm, kk, nn);
+          a.feat[kk] = f;
+        }
+      }
+    }
+    // the list is read in full before the next window writes it
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
 }
 
@@ -237,8 +288,7 @@ static int query_err_check(ncx_shapes* s, const char* what) {
   return NCX_OK;
 }
 
-static unsigned query_blocks(ncx_ctx* ctx, size_t n) {
-  const size_t per_block = TOI_BS / TOI_G;
+static unsigned query_blocks(ncx_ctx* ctx, size_t n, size_t per_block = TOI_BS / TOI_G) {
   return std::max(1u, std::min<unsigned>(ceil_div(n, per_block), (unsigned)ctx->num_cus * 16));
 }
 
@@ -304,7 +354,7 @@ extern "C" int ncx_shapes_cast_rays(ncx_shapes* s, const ncx_pose* poses_dev, co
   a.nrm = normal_dev;
   a.feat = feature_dev;
   a.err = s->qerr.as<uint32_t>();
-  hipLaunchKernelGGL(k_shape_rays<TOI_G>, dim3(query_blocks(ctx, n)), dim3(TOI_BS), 0, ctx->stream, a);
+  hipLaunchKernelGGL(k_shape_rays<TOI_G>, dim3(query_blocks(ctx, n, TOI_BS)), dim3(TOI_BS), 0, ctx->stream, a);
   NCX_HIP_CHECK(hipGetLastError());
   return query_err_check(s, "ncx_shapes_cast_rays");
 }

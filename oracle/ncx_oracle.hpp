@@ -1918,8 +1918,9 @@ template <class R> inline bool ray_aabb_normal(const AABB<R>& bb, const Ray<R>& 
 }
 
 // ray_intersection_with_support_map_with_params (ray_support_map.rs:15-62), shape at identity.
+// *iters (if given) counts the support evaluations of both casts.
 template <class R> inline bool ray_support_map(const Shape<R>& g, const Ray<R>& ray, R max_toi, bool solid, R* toi,
-                                               V3<R>* normal) {
+                                               V3<R>* normal, int* iters = nullptr) {
   Iso<R> id;
   id.t = V3<R>(0, 0, 0);
   id.q = Quat<R>{R(0), R(0), R(0), R(1)};
@@ -1928,7 +1929,11 @@ template <class R> inline bool ray_support_map(const Shape<R>& g, const Ray<R>& 
   VoronoiSimplex<R> simplex;  // the reset with the -dir support point is overwritten by cast_ray
   R t;
   V3<R> n;
-  if (!minkowski_ray_cast(id, smap(g), id, smap(origin), ray, max_toi, simplex, &t, &n)) return false;
+  int it1 = 0, it2 = 0;
+  if (iters) *iters = 0;
+  const bool h1 = minkowski_ray_cast(id, smap(g), id, smap(origin), ray, max_toi, simplex, &t, &n, &it1);
+  if (iters) *iters = it1;
+  if (!h1) return false;
   if (solid || t != R(0)) { *toi = t; *normal = n; return true; }
   // the ray starts inside the shape: cast back from a point past the far side
   V3<R> ndir = normalize(ray.dir);
@@ -1937,7 +1942,9 @@ template <class R> inline bool ray_support_map(const Shape<R>& g, const Ray<R>& 
   R shift = dot(supp - ray.origin, ndir) + eps;
   Ray<R> nr{ray.origin + ndir * shift, -ray.dir};
   VoronoiSimplex<R> s2;
-  if (!minkowski_ray_cast(id, smap(g), id, smap(origin), nr, shift + eps, s2, &t, &n)) return false;
+  const bool h2 = minkowski_ray_cast(id, smap(g), id, smap(origin), nr, shift + eps, s2, &t, &n, &it2);
+  if (iters) *iters = it1 + it2;
+  if (!h2) return false;
   R tt = shift - t;
   if (!(tt <= max_toi)) return false;
   *toi = tt;
@@ -1949,7 +1956,8 @@ template <class R> inline bool ray_support_map(const Shape<R>& g, const Ray<R>& 
 // -> ray_aabb.rs:45-61), Capsule and ConvexHull (ray_support_map.rs:117-163: local-frame ray,
 // normal back to world).
 template <class R> inline bool shape_ray_cast(const Shape<R>& g, const Iso<R>& m, const Ray<R>& ray, R max_toi,
-                                              bool solid, RayHit<R>* out) {
+                                              bool solid, RayHit<R>* out, int* iters = nullptr) {
+  if (iters) *iters = 0;
   switch (g.type) {
     case BALL: {
       bool inside;
@@ -1980,7 +1988,7 @@ template <class R> inline bool shape_ray_cast(const Shape<R>& g, const Iso<R>& m
       Ray<R> ls{inv_xform_pt(m, ray.origin), inv_xform_vec(m, ray.dir)};
       R t;
       V3<R> n;
-      if (!ray_support_map(g, ls, max_toi, solid, &t, &n)) return false;
+      if (!ray_support_map(g, ls, max_toi, solid, &t, &n, iters)) return false;
       out->toi = t;
       out->normal = xform_vec(m, n);
       out->feature = FEAT_UNKNOWN;

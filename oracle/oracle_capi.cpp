@@ -312,14 +312,17 @@ int toi_batch(int64_t ns, const uint32_t* types, const R* params, const uint32_t
 template <class R>
 int shape_rays(int64_t ns, const uint32_t* types, const R* params, const uint32_t* hoff, const uint32_t* hcnt,
                const R* hpts, int64_t npts_total, const R* poses, const uint32_t* idx, const R* orig, const R* dir,
-               int64_t n, R max_toi, int solid, R* toi, R* nrm, uint32_t* feat, int nthreads) {
+               int64_t n, R max_toi, int solid, R* toi, R* nrm, uint32_t* feat, int32_t* iters, int nthreads) {
   ShapeSet<R> ss;
   ss.load(ns, types, params, hoff, hcnt, hpts, npts_total);
   parallel_for(n, nthreads, [&](int64_t k) {
     Ray<R> ray{V3<R>(orig[3 * k], orig[3 * k + 1], orig[3 * k + 2]), V3<R>(dir[3 * k], dir[3 * k + 1], dir[3 * k + 2])};
     RayHit<R> h;
     uint32_t i = idx[k];
-    if (shape_ray_cast(ss.shapes[i], pose_at(poses, i), ray, max_toi, solid != 0, &h)) {
+    int it = 0;
+    const bool hit = shape_ray_cast(ss.shapes[i], pose_at(poses, i), ray, max_toi, solid != 0, &h, &it);
+    if (iters) iters[k] = it;
+    if (hit) {
       toi[k] = h.toi;
       for (int d = 0; d < 3; ++d) nrm[3 * k + d] = h.normal[d];
       feat[k] = h.feature;
@@ -668,9 +671,9 @@ int64_t orc_pairs_brute(const float* bb6, int64_t n, uint64_t* out, int64_t cap)
   int orc_shape_rays_##T(int64_t ns, const uint32_t* types, const R* params, const uint32_t* hoff,                \
                          const uint32_t* hcnt, const R* hpts, int64_t npts, const R* poses, const uint32_t* idx,   \
                          const R* orig, const R* dir, int64_t n, R max_toi, int solid, R* toi, R* nrm,              \
-                         uint32_t* feat, int nthreads) {                                                           \
+                         uint32_t* feat, int32_t* iters, int nthreads) {                                           \
     return shape_rays<R>(ns, types, params, hoff, hcnt, hpts, npts, poses, idx, orig, dir, n, max_toi, solid, toi, \
-                         nrm, feat, nthreads);                                                                     \
+                         nrm, feat, iters, nthreads);                                                              \
   }
 ORC_TOI(f32, float)
 ORC_TOI(f64, double)

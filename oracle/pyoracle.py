@@ -55,7 +55,7 @@ def _declare(L):
         sig(f"orc_trimesh_brute_{t}", C.c_int, P, P, I64, P, P, P, I64, R, P, P, P, P, C.c_int)
         sig(f"orc_toi_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P, P, P, I64, R, R, P, P, P, P, P, P, P, P,
             C.c_int)
-        sig(f"orc_shape_rays_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P, P, P, I64, R, C.c_int, P, P, P, C.c_int)
+        sig(f"orc_shape_rays_{t}", C.c_int, I64, P, P, P, P, P, I64, P, P, P, P, I64, R, C.c_int, P, P, P, P, C.c_int)
     sig("orc_world_rays_f32", I64, I64, P, P, P, P, P, I64, P, P, P, P, I64, C.c_float, C.c_int, P, P, P, P, I64,
         C.c_int)
     sig("orc_manifold_store_new", P, I64)
@@ -345,8 +345,9 @@ def time_of_impact(ss, poses, vel, ia, ib, max_toi, target_distance, nthreads=1)
                 iters=it)
 
 
-def shape_rays(ss, poses, idx, origins, dirs, max_toi, solid, nthreads=1):
-    """RayCast::toi_and_normal_with_ray of shape idx[k] against ray k; miss: toi -1, feature UINT32_MAX."""
+def shape_rays(ss, poses, idx, origins, dirs, max_toi, solid, nthreads=1, iters=False):
+    """RayCast::toi_and_normal_with_ray of shape idx[k] against ray k; miss: toi -1, feature UINT32_MAX.
+    iters: also return each ray's count of support evaluations (0 for the closed forms)."""
     L = lib()
     R = ss.R
     poses = np.ascontiguousarray(poses, dtype=R).reshape(-1, 7)
@@ -357,9 +358,10 @@ def shape_rays(ss, poses, idx, origins, dirs, max_toi, solid, nthreads=1):
     toi = np.zeros(n, R)
     nrm = np.zeros((n, 3), R)
     feat = np.zeros(n, np.uint32)
+    it = np.zeros(n, np.int32) if iters else None
     getattr(L, f"orc_shape_rays_{ss.tag}")(*ss.args(), _p(poses), _p(idx), _p(o), _p(d), n, max_toi, int(solid),
-                                            _p(toi), _p(nrm), _p(feat), nthreads)
-    return toi, nrm, feat
+                                            _p(toi), _p(nrm), _p(feat), _p(it) if iters else None, nthreads)
+    return (toi, nrm, feat, it) if iters else (toi, nrm, feat)
 
 
 def world_rays(ss, poses, boxes6, origins, dirs, max_toi, mode, nthreads=8):

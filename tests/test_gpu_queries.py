@@ -78,6 +78,28 @@ def test_toi_hull_pairs_match_oracle(ctx, orc):
     assert np.sum(r["status"][h] == 3) > 1000 and np.sum(r["status"][h] == 1) > 1000
 
 
+@pytest.mark.parametrize("nr", [1, 63, 65, 130, 1000])
+def test_shape_ray_casts_ragged_batches(ctx, orc, nr):
+    """Batches that end inside a 64-ray window, over mixed shape kinds, equal the oracle."""
+    w = workloads.full_pipeline(seed=19, n=500, box=10.0, nclusters=2)
+    n = len(w["records"])
+    rng = np.random.default_rng(nr)
+    idx = rng.integers(0, n, nr).astype(np.uint32)
+    c = w["poses"][idx, :3].astype(np.float64)
+    o = c + rng.normal(0, 2.0, (nr, 3))
+    d = c + rng.uniform(-0.3, 0.3, (nr, 3)) - o
+    o, d = o.astype(np.float32), d.astype(np.float32)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    gt, gn, gf = query.cast_rays(ss, w["poses"], idx, o, d, 50.0, True)
+    oss = oracle_shapes(orc, w["records"], w["pool"])
+    rt, rn, rf = orc.shape_rays(oss, to_oracle_poses(w["poses"]), idx, o, d, np.float32(50.0), True, nthreads=8)
+    assert np.array_equal(gf.cpu().numpy().view(np.uint32), rf)
+    assert np.array_equal(_bits(gt.cpu().numpy()), _bits(rt))
+    assert np.array_equal(_bits(gn.cpu().numpy()), _bits(rn))
+    if nr >= 130:
+        assert (rf != 0xFFFFFFFF).sum() > nr // 4
+
+
 @pytest.mark.parametrize("solid", [True, False])
 def test_shape_ray_casts_match_oracle(ctx, orc, solid):
     w = workloads.full_pipeline(seed=16, n=4000, box=20.0, nclusters=3)
@@ -108,3 +130,18 @@ def test_shape_ray_casts_match_oracle(ctx, orc, solid):
           + ", ".join(f"{k}: {np.mean(hit[types == k]):.2f}" for k in range(4)))
     for k in range(4):
         assert np.sum(hit & (types == k)) > 500
+
+
+def test_shape_rays_capsule_iteration_cap_fixture(ctx):
+    """Capsule rays whose GJK ray cast stops at its 10000-iteration cap (None) and converging
+    controls (tests/golden/capsule_ray_fixed_points.npz): the GPU gives the fixture's outputs bit
+    for bit."""
+    import os
+    from ncollide_amd.shape import Capsule
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "capsule_ray_fixed_points.npz"))
+    ss = ncx.ShapeSet(shapes=[Capsule(h, r) for h, r in zip(z["half_height"], z["radius"])], ctx=ctx)
+    gt, gn, gf = query.cast_rays(ss, z["poses"], z["shape"], z["origin"], z["dir"], 50.0, True)
+    assert np.array_equal(gf.cpu().numpy().view(np.uint32), z["feature"])
+    assert np.array_equal(_bits(gt.cpu().numpy()), _bits(z["toi"]))
+    assert np.array_equal(_bits(gn.cpu().numpy()), _bits(z["normal"]))
+    assert np.all(z["feature"][z["capped"]] == 0xFFFFFFFF)

@@ -350,3 +350,60 @@ def test_shape_rays_misses_and_max_toi(orc):
     assert (toi[:4] > 3.5).all() and (toi[:4] < 5).all() and (feat[4:] == 0xFFFFFFFF).all()
     toi2, _, feat2 = orc.shape_rays(ss, poses, idx, o, d, 3.0, True)
     assert (feat2 == 0xFFFFFFFF).all()
+
+
+def _ray_capsule_f64(hh, r, lo, ld):
+    """Least t >= 0 where the ray lo + t ld meets the capsule (segment (0, +-hh, 0) dilated by r), or None."""
+    cands = []
+    for cy in (-hh, hh):
+        dc = lo - np.array([0, cy, 0])
+        A, B, C = ld @ ld, dc @ ld, dc @ dc - r * r
+        disc = B * B - A * C
+        if disc >= 0:
+            cands += [(-B - np.sqrt(disc)) / A, (-B + np.sqrt(disc)) / A]
+    A = ld[0] ** 2 + ld[2] ** 2
+    B = lo[0] * ld[0] + lo[2] * ld[2]
+    C = lo[0] ** 2 + lo[2] ** 2 - r * r
+    disc = B * B - A * C
+    if A > 0 and disc >= 0:
+        for tc in ((-B - np.sqrt(disc)) / A, (-B + np.sqrt(disc)) / A):
+            if abs(lo[1] + ld[1] * tc) <= hh:
+                cands.append(tc)
+    cands = [c for c in cands if c >= 0]
+    return min(cands) if cands else None
+
+
+def test_shape_rays_capsule_iteration_cap_fixture(orc):
+    """tests/golden/capsule_ray_fixed_points.npz (written by make_capsule_ray_fixed_points.py):
+    capsule rays whose GJK ray cast (gjk.rs:245-365) runs to its 10000-iteration cap and returns
+    None, although the ray meets the capsule, and rays that converge.  The oracle reproduces the
+    fixture bit for bit and reports the cap through its support-evaluation count."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "capsule_ray_fixed_points.npz"))
+    n = len(z["half_height"])
+    ss = shapeset(orc, [("capsule", float(h), float(r)) for h, r in zip(z["half_height"], z["radius"])],
+                  dtype=np.float32)
+    from ncollide_amd.math import to_oracle_poses
+    poses = to_oracle_poses(z["poses"])
+    toi, nrm, feat, it = orc.shape_rays(ss, poses, z["shape"], z["origin"], z["dir"], np.float32(50.0), True,
+                                        nthreads=8, iters=True)
+    b = lambda a: np.ascontiguousarray(a, np.float32).view(np.uint32)
+    assert np.array_equal(b(toi), b(z["toi"])) and np.array_equal(b(nrm), b(z["normal"]))
+    assert np.array_equal(feat, z["feature"]) and np.array_equal(it, z["iters"])
+    cap = z["capped"]
+    assert cap.sum() >= 32 and (~cap).sum() >= 32
+    assert np.all(it[cap] >= 10000) and np.all(feat[cap] == 0xFFFFFFFF) and np.all(it[~cap] < 100)
+    # what geometry says: every ray meets its capsule, at the oracle's toi for the controls
+    p64 = poses.astype(np.float64)
+    for k in range(len(toi)):
+        s = int(z["shape"][k])
+        q, t = p64[s, 3:], p64[s, :3]
+        qc = np.array([-q[0], -q[1], -q[2], q[3]])
+        lo = qrot(qc, z["origin"][k].astype(np.float64) - t)
+        ld = qrot(qc, z["dir"][k].astype(np.float64))
+        exact = _ray_capsule_f64(float(z["half_height"][s]), float(z["radius"][s]), lo, ld)
+        assert exact is not None, k
+        if not cap[k]:
+            # f32 GJK stops at a relative gap of sqrt(10 f32::EPSILON) ~ 1.1e-3 (gjk.rs:244, 336)
+            assert abs(toi[k] - exact) <= 2e-3 * (1 + exact), (k, toi[k], exact)
+    assert n == len(np.unique(z["shape"]))
