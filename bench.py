@@ -566,6 +566,27 @@ def bench_queries(args, ncx, torch, ctx):
                                 "contact_queries_per_s": nq / tcn, "distance_ms": tdi * 1e3,
                                 "distance_queries_per_s": nq / tdi,
                                 "contact_fraction": float(g["has_contact"].float().mean().item())}
+    if not args.no_cpu:
+        from oracle import pyoracle as orc
+        from ncollide_amd.math import to_oracle_poses
+        rec = sw["records"]
+        coss = orc.ShapeSet(rec["type"], np.stack([rec["p0"], rec["p1"], rec["p2"]], 1), rec["hull_offset"],
+                            rec["hull_count"], sw["pool"], dtype=np.float32)
+        kc, kd = 16384, 4096  # a distance query takes about seven times the CPU time of a contact
+        nth = min(16, os.cpu_count() or 1)
+        mp7 = np.array([0, 0, 0, 0, 0, 0, 1], np.float32)
+        op = to_oracle_poses(mp)
+        tc = time.perf_counter()
+        orc.mesh_contact(coss, pts, tri, mp7, op, np.arange(kc, dtype=np.uint32), np.float32(0.0), nthreads=nth)
+        tcc = time.perf_counter() - tc
+        tc = time.perf_counter()
+        orc.mesh_distance(coss, pts, tri, mp7, op, np.arange(kd, dtype=np.uint32), nthreads=nth)
+        tcd = time.perf_counter() - tc
+        out["composite_trimesh"]["cpu_baseline"] = {
+            "value": 2.0 / (tcc / kc + tcd / kd), "unit": "queries/s (one contact + one distance per query)",
+            "cores": nth, "kind": "port", "contact_queries_per_s": kc / tcc, "distance_queries_per_s": kd / tcd,
+            "sample": f"first {kc} queries for contact_composite_shape_shape, first {kd} for "
+                      "distance_composite_shape_shape"}
     return out
 
 
