@@ -8,8 +8,8 @@
 // RayCast::toi_and_normal_with_ray of the same four shapes (ray_ball.rs, ray_cuboid.rs,
 // ray_aabb.rs, ray_support_map.rs) over a batch of (shape, ray) pairs.
 //
-// One group of TOI_G lanes evaluates one pair, or one ray on a support-map shape; the lanes split
-// ConvexHull support scans and run the same control flow.  Groups take queries in a grid-stride
+// One group of TOI_G lanes evaluates one pair (SR_G lanes one ray on a support-map shape); the
+// lanes split ConvexHull support scans and run the same control flow.  Groups take queries in a grid-stride
 // loop; the ray kernel sorts each wavefront's rays by shape kind first (k_shape_rays).  All
 // arithmetic restates the reference in f32 with its operation order.
 #include "ncx_gjk.h"
@@ -200,6 +200,7 @@ struct SRArgs {
 // G lanes per ray, 64 / G at a time, so that each pass holds rays of one kind save at most one
 // pass.  The outputs of every ray are what shape_ray_cast<G> gives it.
 #define SR_WAVES (TOI_BS / 64)
+#define SR_G 2  // lanes per support-map ray (1.67 ms for 1,048,576 scene rays; 4 lanes: 1.91 ms, 8: 2.55 ms)
 template <int G>
 __global__ __launch_bounds__(TOI_BS) void k_shape_rays(SRArgs a) {
   static_assert(TOI_BS % 64 == 0 && 64 % G == 0, "whole wavefronts of whole groups");
@@ -354,7 +355,7 @@ extern "C" int ncx_shapes_cast_rays(ncx_shapes* s, const ncx_pose* poses_dev, co
   a.nrm = normal_dev;
   a.feat = feature_dev;
   a.err = s->qerr.as<uint32_t>();
-  hipLaunchKernelGGL(k_shape_rays<TOI_G>, dim3(query_blocks(ctx, n, TOI_BS)), dim3(TOI_BS), 0, ctx->stream, a);
+  hipLaunchKernelGGL(k_shape_rays<SR_G>, dim3(query_blocks(ctx, n, TOI_BS)), dim3(TOI_BS), 0, ctx->stream, a);
   NCX_HIP_CHECK(hipGetLastError());
   return query_err_check(s, "ncx_shapes_cast_rays");
 }
