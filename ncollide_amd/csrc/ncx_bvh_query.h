@@ -40,6 +40,7 @@ struct BQArgs {
   float* toi;
   float* nrm;
   uint32_t* err;          // set when a traversal stack would overflow (a tree deeper than the bound)
+  const uint32_t* perm;   // first-hit rays: group k takes ray perm[k] (null: ray k)
 };
 
 #define BQ_STACK (LBVH_MAX_DEPTH + 2)
@@ -159,9 +160,10 @@ __global__ __launch_bounds__(BQ_BLOCK) void k_bq_collect(BQArgs a) {
 #define BQ_RAY_G 4
 template <int G>
 static __global__ __launch_bounds__(BQ_BLOCK) void k_bq_first_ray(BQArgs a) {
-  const uint32_t qi = (blockIdx.x * BQ_BLOCK + threadIdx.x) / G;
+  const uint32_t gi = (blockIdx.x * BQ_BLOCK + threadIdx.x) / G;
   const int gl = threadIdx.x % G;
-  if (qi >= a.nq) return;  // whole groups: BQ_BLOCK is a multiple of G
+  if (gi >= a.nq) return;  // whole groups: BQ_BLOCK is a multiple of G
+  const uint32_t qi = a.perm ? a.perm[gi] : gi;
   float qv[6];
   for (int d = 0; d < 6; ++d) qv[d] = a.q[d][qi];
   const f3 o = f3{qv[0], qv[1], qv[2]}, dir = f3{qv[3], qv[4], qv[5]};

@@ -68,6 +68,7 @@ struct ncx_world {
   bool contacts = true;        // run the narrow phase in each step
   bool last_had_contacts = false;
   DevBuf qcnt, qscan, qtot;    // interference queries: per-query counts, scan scratch, total
+  DevBuf rkeys, rvals, rtk, rtv, rhist, rscan;  // first-interference rays: sort keys, order, sort scratch
   // the polyhedron manifold generator's last_gjk_dir kept per pair across steps
   bool temporal = false;
   bool synced = true;          // ncx_world_sync has run since the last step
@@ -681,7 +682,8 @@ extern "C" int ncx_world_destroy(ncx_world* w) {
   DevBuf* bufs[] = {&w->stored, &w->sbox, &w->gstack, &w->qwork, &w->lbuf, &w->ovdir, &w->odirs,
                     &w->ochunks, &w->relist, &w->cnt, &w->offs, &w->counters, &w->pa, &w->pb,
                     &w->has, &w->w1, &w->w2, &w->nrm, &w->depth_out, &w->iters, &w->desc, &w->dirs,
-                    &w->qcnt, &w->qscan, &w->qtot, &w->tw_keys, &w->tw_dirs, &w->tw_init, &w->tw_fdir};
+                    &w->qcnt, &w->qscan, &w->qtot, &w->tw_keys, &w->tw_dirs, &w->tw_init, &w->tw_fdir,
+                    &w->rkeys, &w->rvals, &w->rtk, &w->rtv, &w->rhist, &w->rscan};
   for (DevBuf* b : bufs) b->release();
   w->bvh.release();
   contact_tiers_release(w->tiers);
@@ -907,6 +909,48 @@ extern "C" int ncx_world_interferences_with_aabbs(ncx_world* w, const float* con
   return bq_collect_run(w->ctx, a, BQ_BOX, offsets_dev, cap, total, w->qcnt, w->qscan, w->qtot);
 }
 
+// Sort key of a first-interference ray: the octant of its direction (bits 27-29), then the
+// Morton code of its origin on a 512^3 grid over the tree's root box, clamped (bits 0-26).  Taken
+// in key order, a wavefront's rays start close by and head the same way, so their traversals
+// and leaf tests are alike.
+struct RayKeyArgs {
+  const float* q[6];
+  const LNode* nodes;
+  uint32_t n;
+  uint32_t* keys;
+  uint32_t* vals;
+};
+
+__device__ __forceinline__ uint32_t ray_spread10(uint32_t v) {  // bit k of v -> bit 3k
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_ray_keys(RayKeyArgs a) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= a.n) return;
+  const LNode r = a.nodes[0];
+  const float lo[3] = {fminf(r.lminx, r.rminx), fminf(r.lminy, r.rminy), fminf(r.lminz, r.rminz)};
+  const float hi[3] = {fmaxf(r.lmaxx, r.rmaxx), fmaxf(r.lmaxy, r.rmaxy), fmaxf(r.lmaxz, r.rmaxz)};
+  uint32_t key = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float ext = hi[d] - lo[d];
+    const float u = ext > 0.f ? (a.q[d][i] - lo[d]) / ext : 0.f;
+    const uint32_t c = (uint32_t)fminf(fmaxf(u * 512.f, 0.f), 511.f);  // NaN -> 0
+    key |= ray_spread10(c) << (2 - d);
+    if (a.q[3 + d][i] < 0.f) key |= 1u << (27 + d);
+  }
+  a.keys[i] = key;
+  a.vals[i] = i;
+}
+
+// Batches of at least this many first-interference rays are taken in key order.
+#define BQ_SORT_MIN 1024
+
 extern "C" int ncx_world_first_interference_with_rays(ncx_world* w, const ncx_pose* poses_dev,
                                                       const float* const ray_soa_dev[6], size_t n, float max_toi,
                                                       uint32_t* handle_dev, float* toi_dev, float* normal_dev) {
@@ -926,6 +970,20 @@ extern "C" int ncx_world_first_interference_with_rays(ncx_world* w, const ncx_po
   int rc;
   if ((rc = w->qtot.reserve(16)) || (rc = fill_u32(w->ctx, w->qtot.as<uint32_t>() + 1, 1, 0u))) return rc;
   a.err = w->qtot.as<uint32_t>() + 1;
+  a.perm = nullptr;
+  if (n >= BQ_SORT_MIN && w->n > 1) {
+    if ((rc = w->rkeys.reserve(n * 4)) || (rc = w->rvals.reserve(n * 4))) return rc;
+    RayKeyArgs k;
+    for (int d = 0; d < 6; ++d) k.q[d] = ray_soa_dev[d];
+    k.nodes = w->bvh.nodes.as<LNode>();
+    k.n = (uint32_t)n;
+    k.keys = w->rkeys.as<uint32_t>();
+    k.vals = w->rvals.as<uint32_t>();
+    hipLaunchKernelGGL(k_ray_keys, dim3(ceil_div(n, 256)), dim3(256), 0, w->ctx->stream, k);
+    NCX_HIP_CHECK(hipGetLastError());
+    if ((rc = radix_sort_u32(w->ctx, k.keys, k.vals, n, 0, 30, w->rtk, w->rtv, w->rhist, w->rscan))) return rc;
+    a.perm = k.vals;  // a permutation of 0..n-1
+  }
   hipLaunchKernelGGL(k_bq_first_ray<BQ_RAY_G>, dim3(ceil_div(n * BQ_RAY_G, BQ_BLOCK)), dim3(BQ_BLOCK), 0,
                      w->ctx->stream, a);
   NCX_HIP_CHECK(hipGetLastError());
