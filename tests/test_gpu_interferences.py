@@ -214,3 +214,28 @@ def test_first_interference_with_ray_reference_case(ctx):
     b = dc @ d[0]
     toi_ball = -b - np.sqrt(b * b - (dc @ dc - 1.0))
     assert h.cpu().numpy().tolist() == [1] and abs(t.cpu().numpy()[0] - toi_ball) < 1e-4
+
+
+@pytest.mark.parametrize("nr", [1023, 1024, 3000])
+def test_world_first_hits_either_side_of_the_sorted_batch_size(ctx, orc, nr):
+    """Batches of 1,024 rays or more are taken in origin/octant key order, smaller ones in input
+    order: both give the oracle's first hits in each ray's own slot."""
+    w = workloads.full_pipeline(seed=41, n=3000, box=20.0, nclusters=2)
+    ss = ncx.ShapeSet(records=w["records"], pool=w["pool"], ctx=ctx)
+    world = World(ss, margin=0.05, prediction=0.0, max_pairs=1 << 18)
+    poses = torch.as_tensor(w["poses"], device=ctx.device)
+    world.step(poses, use_graph=False)
+    world.sync()
+    stored = world.stored_aabbs().cpu().numpy().T.copy()
+    rng = np.random.default_rng(nr)
+    tgt = w["poses"][rng.integers(0, len(w["poses"]), nr), :3]
+    ro = (tgt + rng.normal(0, 5, (nr, 3))).astype(np.float32)
+    rd = ((tgt - ro) * rng.uniform(0.2, 1.5, (nr, 1))).astype(np.float32)
+    rd[::5] = -rd[::5]  # every octant, and rays that leave the scene
+    gh, gt, gn = world.first_interference_with_rays(poses, ro, rd, 40.0)
+    oss = oracle_shapes(orc, w["records"], w["pool"])
+    rh, rt, rn = orc.world_rays(oss, to_oracle_poses(w["poses"]), stored, ro, rd, np.float32(40.0), 0)
+    assert np.array_equal(gh.cpu().numpy().view(np.uint32), rh)
+    assert np.array_equal(gt.cpu().numpy().view(np.uint32), rt.view(np.uint32))
+    assert np.array_equal(gn.cpu().numpy().view(np.uint32), rn.view(np.uint32))
+    assert np.mean(rh != 0xFFFFFFFF) > 0.3
