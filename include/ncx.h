@@ -395,7 +395,7 @@ int ncx_world_stored_aabbs(ncx_world* world, const float** soa_dev);
  * 64-107): per ray the hit with the least toi (then the least handle: the reference keeps the
  * first of equal-toi hits its heap pops), handle UINT32_MAX and toi -1 on a miss.  Batches of
  * 1,024 rays or more are searched in an order sorted by origin and direction octant, with
- * 24 bytes per ray of sort buffers kept by the world; results go to each ray's own slot.
+ * about 16 bytes per ray of sort buffers kept by the world; results go to each ray's own slot.
  * ncx_world_interferences_with_aabbs: glue::interferences_with_aabb (glue/query.rs) = the stored
  * boxes intersecting each query box. */
 int ncx_world_interferences_with_rays(ncx_world* world, const ncx_pose* poses_dev, const float* const ray_soa_dev[6],
